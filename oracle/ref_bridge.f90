@@ -1,0 +1,349 @@
+! ref_bridge.f90 -- C-callable bridge into the reference tjsq Fortran routines.
+!
+! TEST INFRASTRUCTURE ONLY.  This file is compiled together with the
+! reference's own unmodified sources (read where they lie under the reference
+! checkout) into oracle/_ref/libtjsq_ref.so by oracle/Makefile.  It lets the
+! tests run the reference's hmerge + model_wave (code/tjsq/get_sys.f90:4383,
+! :4127) and lanczos_memo_new (code/tjsq/eigen.f90:2) on the same flat inputs
+! the oracle and the HIP path receive.  The flat layout is the one described
+! in oracle/tjsq_oracle.h.
+!
+! The reference is compiled with its shipped parameters (common.f90:10-12:
+! Nx=24, Ny=6, up_bias=0); the bridge builds the site operators itself so a
+! run may use (N, 2S) quantum numbers (up_bias = 1), which the apply kernels
+! handle through the operators' up_dif alone.
+module ref_bridge_util
+  use pubdata
+  implicit none
+contains
+  subroutine put_block(b, len, hdr, sub, off, data)
+    type(Total_Block), intent(inout) :: b
+    integer, intent(in) :: len, hdr(4), sub(5, *)
+    integer(8), intent(in) :: off(*)
+    real(8), intent(in) :: data(*)
+    integer :: i, r, c
+    integer(8) :: o
+    b%len = len
+    b%up_dif = hdr(2)
+    b%down_dif = hdr(3)
+    b%num = hdr(4)
+    b%dim = 0
+    if (allocated(b%sub)) deallocate(b%sub)
+    allocate(b%sub(b%num))
+    do i = 1, b%num
+      b%sub(i)%num_up = sub(1, i)
+      b%sub(i)%num_down = sub(2, i)
+      b%sub(i)%down_dif = sub(3, i)
+      b%sub(i)%row_dim = sub(4, i)
+      b%sub(i)%sdim = sub(5, i)
+      b%dim = b%dim + sub(5, i)
+      allocate(b%sub(i)%mat(sub(4, i), sub(5, i)))
+      o = off(i)
+      do c = 1, sub(5, i)
+        do r = 1, sub(4, i)
+          b%sub(i)%mat(r, c) = data(o + (c - 1) * sub(4, i) + r)
+        end do
+      end do
+    end do
+  end subroutine put_block
+
+  subroutine put_model(m, len, nops, hdr, sub, off, data)
+    type(Total_Model), intent(inout) :: m
+    integer, intent(in) :: len, nops, hdr(4, *), sub(5, *)
+    integer(8), intent(in) :: off(*)
+    real(8), intent(in) :: data(*)
+    integer :: op, first, site
+    m%len = len
+    first = 1
+    call put_block(m%ham, len, hdr(:, 1), sub(1, first), off(first), data)
+    first = first + hdr(4, 1)
+    do op = 2, nops
+      site = (op - 2) / 4 + 1
+      select case (mod(op - 2, 4))
+      case (0)
+        call put_block(m%sub(site)%elec_up, len, hdr(:, op), sub(1, first), off(first), data)
+      case (1)
+        call put_block(m%sub(site)%elec_down, len, hdr(:, op), sub(1, first), off(first), data)
+      case (2)
+        call put_block(m%sub(site)%spin_sd, len, hdr(:, op), sub(1, first), off(first), data)
+      case (3)
+        call put_block(m%sub(site)%num_sn, len, hdr(:, op), sub(1, first), off(first), data)
+      end select
+      first = first + hdr(4, op)
+    end do
+  end subroutine put_model
+
+  subroutine put_basis(bs, len, nsec, bsec, bsub)
+    type(Total_Basis), intent(inout) :: bs
+    integer, intent(in) :: len, nsec, bsec(4, *), bsub(6, *)
+    integer :: i, j, q
+    bs%len = len
+    bs%num = nsec
+    bs%dim = 0
+    allocate(bs%sub(nsec))
+    q = 0
+    do i = 1, nsec
+      bs%sub(i)%new_num_up = bsec(1, i)
+      bs%sub(i)%new_num_down = bsec(2, i)
+      bs%sub(i)%dim = bsec(3, i)
+      bs%sub(i)%num = bsec(4, i)
+      bs%sub(i)%idx = bsec(4, i)
+      bs%dim = bs%dim + bsec(3, i)
+      allocate(bs%sub(i)%sub(bsec(4, i)))
+      do j = 1, bsec(4, i)
+        q = q + 1
+        bs%sub(i)%sub(j)%bl_num_up = bsub(1, q)
+        bs%sub(i)%sub(j)%bl_num_down = bsub(2, q)
+        bs%sub(i)%sub(j)%st_num_up = bsub(3, q)
+        bs%sub(i)%sub(j)%st_num_down = bsub(4, q)
+        bs%sub(i)%sub(j)%spos = bsub(5, q)
+        bs%sub(i)%sub(j)%sdim = bsub(6, q)
+      end do
+    end do
+  end subroutine put_basis
+
+  subroutine put_wave(w, sys_len, env_len, nw, wsec, x)
+    type(Wavefunction), intent(inout) :: w
+    integer, intent(in) :: sys_len, env_len, nw, wsec(6, *)
+    real(8), intent(in) :: x(*)
+    integer :: i, r, c
+    integer(8) :: o
+    w%sys_len = sys_len
+    w%env_len = env_len
+    w%num = nw
+    w%dim = 0
+    allocate(w%sub(nw))
+    o = 0
+    do i = 1, nw
+      w%sub(i)%sys_num_up = wsec(1, i)
+      w%sub(i)%sys_num_down = wsec(2, i)
+      w%sub(i)%env_num_up = wsec(3, i)
+      w%sub(i)%env_num_down = wsec(4, i)
+      w%sub(i)%sys_dim = wsec(5, i)
+      w%sub(i)%env_dim = wsec(6, i)
+      allocate(w%sub(i)%vec(wsec(5, i), wsec(6, i)))
+      do c = 1, wsec(6, i)
+        do r = 1, wsec(5, i)
+          w%sub(i)%vec(r, c) = x(o + (c - 1) * wsec(5, i) + r)
+        end do
+      end do
+      o = o + int(wsec(5, i), 8) * wsec(6, i)
+      w%dim = w%dim + wsec(5, i) * wsec(6, i)
+    end do
+  end subroutine put_wave
+
+  subroutine get_wave(w, y)
+    type(Wavefunction), intent(in) :: w
+    real(8), intent(inout) :: y(*)
+    integer :: i, r, c
+    integer(8) :: o
+    o = 0
+    do i = 1, w%num
+      do c = 1, w%sub(i)%env_dim
+        do r = 1, w%sub(i)%sys_dim
+          y(o + (c - 1) * w%sub(i)%sys_dim + r) = w%sub(i)%vec(r, c)
+        end do
+      end do
+      o = o + int(w%sub(i)%sys_dim, 8) * w%sub(i)%env_dim
+    end do
+  end subroutine get_wave
+
+  subroutine free_block(b)
+    type(Total_Block), intent(inout) :: b
+    integer :: i
+    if (.not. allocated(b%sub)) return
+    do i = 1, size(b%sub)
+      if (allocated(b%sub(i)%mat)) deallocate(b%sub(i)%mat)
+    end do
+    deallocate(b%sub)
+  end subroutine free_block
+
+  subroutine free_set(s)
+    type(Block_Set), intent(inout) :: s
+    call free_block(s%elec_up)
+    call free_block(s%elec_down)
+    call free_block(s%spin_sd)
+    call free_block(s%spin_sz)
+    call free_block(s%num_sn)
+  end subroutine free_set
+
+  subroutine free_model(m)
+    type(Total_Model), intent(inout) :: m
+    integer :: i
+    call free_block(m%ham)
+    do i = 1, nxx
+      call free_set(m%sub(i))
+      call free_set(m%sub_s(i))
+    end do
+    do i = 1, 2
+      call free_set(m%snc(i))
+      call free_set(m%sdc(i))
+    end do
+  end subroutine free_model
+
+  ! site operators of Get_site_operator_su2_U0 (dmrg_sub.f90:370-473) with the
+  ! particle-number label ub taking the place of the compile-time up_bias
+  subroutine make_site_ops(ub)
+    integer, intent(in) :: ub
+    call free_block(st_elec_up)
+    call free_block(st_elec_down)
+    call free_block(num_elec)
+    call free_block(st_sd)
+    call free_block(st_si)
+    st_elec_up%len = 1; st_elec_up%num = 1; st_elec_up%dim = 1
+    st_elec_up%up_dif = ub; st_elec_up%down_dif = 1
+    allocate(st_elec_up%sub(1))
+    st_elec_up%sub(1)%num_up = 0; st_elec_up%sub(1)%num_down = 0
+    st_elec_up%sub(1)%down_dif = 1; st_elec_up%sub(1)%row_dim = 1; st_elec_up%sub(1)%sdim = 1
+    allocate(st_elec_up%sub(1)%mat(1, 1)); st_elec_up%sub(1)%mat(1, 1) = -dsqrt(2.0d0)
+
+    st_elec_down%len = 1; st_elec_down%num = 1; st_elec_down%dim = 1
+    st_elec_down%up_dif = -ub; st_elec_down%down_dif = 1
+    allocate(st_elec_down%sub(1))
+    st_elec_down%sub(1)%num_up = ub; st_elec_down%sub(1)%num_down = 1
+    st_elec_down%sub(1)%down_dif = -1; st_elec_down%sub(1)%row_dim = 1; st_elec_down%sub(1)%sdim = 1
+    allocate(st_elec_down%sub(1)%mat(1, 1)); st_elec_down%sub(1)%mat(1, 1) = -dsqrt(2.0d0)
+
+    num_elec%len = 1; num_elec%num = 2; num_elec%dim = 2; num_elec%up_dif = 0; num_elec%down_dif = 0
+    allocate(num_elec%sub(2))
+    num_elec%sub(1)%num_up = 0; num_elec%sub(1)%num_down = 0; num_elec%sub(1)%down_dif = 0
+    num_elec%sub(1)%row_dim = 1; num_elec%sub(1)%sdim = 1
+    allocate(num_elec%sub(1)%mat(1, 1)); num_elec%sub(1)%mat(1, 1) = 0.0d0
+    num_elec%sub(2)%num_up = ub; num_elec%sub(2)%num_down = 1; num_elec%sub(2)%down_dif = 0
+    num_elec%sub(2)%row_dim = 1; num_elec%sub(2)%sdim = 1
+    allocate(num_elec%sub(2)%mat(1, 1)); num_elec%sub(2)%mat(1, 1) = dsqrt(2.0d0)
+
+    st_sd%len = 1; st_sd%num = 1; st_sd%dim = 1; st_sd%up_dif = 0; st_sd%down_dif = su
+    allocate(st_sd%sub(1))
+    st_sd%sub(1)%num_up = ub; st_sd%sub(1)%num_down = 1; st_sd%sub(1)%down_dif = 0
+    st_sd%sub(1)%row_dim = 1; st_sd%sub(1)%sdim = 1
+    allocate(st_sd%sub(1)%mat(1, 1)); st_sd%sub(1)%mat(1, 1) = dsqrt(1.50d0)
+
+    st_si%len = 1; st_si%num = 2; st_si%dim = 2; st_si%up_dif = 0; st_si%down_dif = 0
+    allocate(st_si%sub(2))
+    st_si%sub(1)%num_up = 0; st_si%sub(1)%num_down = 0; st_si%sub(1)%down_dif = 0
+    st_si%sub(1)%row_dim = 1; st_si%sub(1)%sdim = 1
+    allocate(st_si%sub(1)%mat(1, 1)); st_si%sub(1)%mat(1, 1) = 1.0d0
+    st_si%sub(2)%num_up = ub; st_si%sub(2)%num_down = 1; st_si%sub(2)%down_dif = 0
+    st_si%sub(2)%row_dim = 1; st_si%sub(2)%sdim = 1
+    allocate(st_si%sub(2)%mat(1, 1)); st_si%sub(2)%mat(1, 1) = dsqrt(2.0d0)
+  end subroutine make_site_ops
+end module ref_bridge_util
+
+! Set couplings (main.f90:35-47 convention, uniform per bond class), lattice
+! tables (main.f90:50-52), site operators and the 6j machinery.
+integer(c_int) function refb_init(ny_, ub, t_nn, t_nnn, d_nn, d_nnn, n_nn, n_nnn, esite_, lanzero_) &
+    bind(C, name="refb_init")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: ny_, ub
+  real(c_double), value :: t_nn, t_nnn, d_nn, d_nnn, n_nn, n_nnn, esite_, lanzero_
+  refb_init = 0
+  if (ny_ /= Ny) then
+    refb_init = -1
+    return
+  end if
+  jz = 0; jd = 0; jt = 0; jn = 0
+  jt(1:4, 1:num_site) = t_nn
+  jt(5:8, 1:num_site) = t_nnn
+  jd(1:4, 1:num_site) = d_nn
+  jd(5:8, 1:num_site) = d_nnn
+  jn(1:4) = n_nn
+  jn(5:8) = n_nnn
+  jz(1:4, 1:num_site) = -4.0d0 * n_nn
+  jz(5:8, 1:num_site) = -4.0d0 * n_nnn
+  esite = esite_
+  lanzero = lanzero_
+  call dmrg_setup0()
+  call dmrg_setup(num_site/2 - 1, num_site/2 - 1)
+  call dmrg_setup0()
+  call Get_Lattice
+  call make_site_ops(ub)
+  iw6j1 = 0; w6j1 = 0.d0; iw6j2 = 0; w6j2 = 0.d0; iw6j3 = 0; w6j3 = 0.d0
+  call factrl
+  cone = 1.0d0
+  czero = 0.0d0
+end function refb_init
+
+! y = H_eff x through the reference's hmerge + model_wave.
+integer(c_int) function refb_model_wave(sys_len, env_len, sys_nops, sys_hdr, sys_sub, sys_off, sys_data, &
+    env_nops, env_hdr, env_sub, env_off, env_data, sys_nsec, sys_bsec, sys_bsub, env_nsec, env_bsec, env_bsub, &
+    nw, wsec, x, y) bind(C, name="refb_model_wave")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: sys_len, env_len, sys_nops, env_nops, sys_nsec, env_nsec, nw
+  integer(c_int), intent(in) :: sys_hdr(4, *), sys_sub(5, *), env_hdr(4, *), env_sub(5, *)
+  integer(c_int64_t), intent(in) :: sys_off(*), env_off(*)
+  real(c_double), intent(in) :: sys_data(*), env_data(*), x(*)
+  integer(c_int), intent(in) :: sys_bsec(4, *), sys_bsub(6, *), env_bsec(4, *), env_bsub(6, *), wsec(6, *)
+  real(c_double), intent(inout) :: y(*)
+  type(Total_Model) :: sys, env
+  type(Total_Basis) :: sys_bs, env_bs
+  type(Wavefunction) :: invec, outvec
+  call put_model(sys, sys_len, sys_nops, sys_hdr, sys_sub, sys_off, sys_data)
+  call put_model(env, env_len, env_nops, env_hdr, env_sub, env_off, env_data)
+  call put_basis(sys_bs, sys_len + 1, sys_nsec, sys_bsec, sys_bsub)
+  call put_basis(env_bs, env_len + 1, env_nsec, env_bsec, env_bsub)
+  call put_wave(invec, sys_len + 1, env_len + 1, nw, wsec, x)
+  call put_wave(outvec, sys_len + 1, env_len + 1, nw, wsec, x)
+  call dmrg_setup(sys_len, env_len)
+  call hmerge(sys, env, sys_bs, env_bs)
+  call model_wave(sys, env, sys_bs, env_bs, invec, outvec)
+  call get_wave(outvec, y)
+  call free_model(sys)
+  call free_model(env)
+  refb_model_wave = 0
+end function refb_model_wave
+
+! Ground state through the reference's lanczos_memo_new with an injected start
+! vector (eigen.f90:2-175); returns E0 and overwrites x with the ground state.
+integer(c_int) function refb_lanczos(sys_len, env_len, sys_nops, sys_hdr, sys_sub, sys_off, sys_data, &
+    env_nops, env_hdr, env_sub, env_off, env_data, sys_nsec, sys_bsec, sys_bsub, env_nsec, env_bsec, env_bsub, &
+    nw, wsec, x, e0) bind(C, name="refb_lanczos")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: sys_len, env_len, sys_nops, env_nops, sys_nsec, env_nsec, nw
+  integer(c_int), intent(in) :: sys_hdr(4, *), sys_sub(5, *), env_hdr(4, *), env_sub(5, *)
+  integer(c_int64_t), intent(in) :: sys_off(*), env_off(*)
+  real(c_double), intent(in) :: sys_data(*), env_data(*)
+  real(c_double), intent(inout) :: x(*)
+  integer(c_int), intent(in) :: sys_bsec(4, *), sys_bsub(6, *), env_bsec(4, *), env_bsub(6, *), wsec(6, *)
+  real(c_double), intent(out) :: e0
+  type(Total_Model) :: sys, env
+  type(Total_Basis) :: sys_bs, env_bs
+  type(Super_Basis) :: super
+  type(Wavefunction) :: eigvec(1)
+  integer :: i
+  call put_model(sys, sys_len, sys_nops, sys_hdr, sys_sub, sys_off, sys_data)
+  call put_model(env, env_len, env_nops, env_hdr, env_sub, env_off, env_data)
+  call put_basis(sys_bs, sys_len + 1, sys_nsec, sys_bsec, sys_bsub)
+  call put_basis(env_bs, env_len + 1, env_nsec, env_bsec, env_bsub)
+  call put_wave(eigvec(1), sys_len + 1, env_len + 1, nw, wsec, x)
+  super%sys_len = sys_len + 1
+  super%env_len = env_len + 1
+  super%num = nw
+  super%dim = eigvec(1)%dim
+  super%idx = nw
+  allocate(super%sub(nw))
+  do i = 1, nw
+    super%sub(i)%sys_num_up = wsec(1, i)
+    super%sub(i)%sys_num_down = wsec(2, i)
+    super%sub(i)%env_num_up = wsec(3, i)
+    super%sub(i)%env_num_down = wsec(4, i)
+    super%sub(i)%sys_dim = wsec(5, i)
+    super%sub(i)%env_dim = wsec(6, i)
+  end do
+  call dmrg_setup(sys_len, env_len)
+  call lanczos_memo_new(sys, env, sys_bs, env_bs, super, eigvec, 1, 1)
+  e0 = Energys(1)
+  call get_wave(eigvec(1), x)
+  call free_model(sys)
+  call free_model(env)
+  refb_lanczos = 0
+end function refb_lanczos

@@ -1,0 +1,90 @@
+/*
+ * tjsq_oracle.h -- CPU restatement of the tjsq SU(2) two-site DMRG hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (grand_sql_tj_amd) never links, imports or calls it.
+ *
+ * Every routine restates one reference routine (DongNingSheng/Grand_SQL_tJ,
+ * code/tjsq/<name>.f90); the file:line of the routine it follows is cited at each
+ * definition in tjsq_oracle.c.  All data cross this interface as flat arrays:
+ *
+ *  operator  (Total_Block, code/tjsq/common.f90:114-122)
+ *     hdr[4]     = {len, up_dif, down_dif, nsub}
+ *     sub[5*n]   = {num_up, num_down, down_dif, row_dim, sdim} per sub-block
+ *                  (num_* are the KET quantum numbers; the bra is
+ *                   (num_up+up_dif, num_down+down_dif))
+ *     off[n]     = offset (in doubles) of the column-major row_dim x sdim
+ *                  matrix inside the data array
+ *  model     (Total_Model, common.f90:139-149): nops operators, op 0 = ham,
+ *            op 1+4*(i-1)+{0,1,2,3} = active site i's
+ *            {elec_up, elec_down, spin_sd, num_sn}
+ *  basis     (Total_Basis, common.f90:65-82)
+ *     sec[4*ns]  = {new_num_up, new_num_down, dim, nsub}
+ *     sub[6*n]   = {bl_num_up, bl_num_down, st_num_up, st_num_down, spos, sdim}
+ *  wave      (Wavefunction / Super_Basis, common.f90:88-110)
+ *     sec[6*nw]  = {sys_num_up, sys_num_down, env_num_up, env_num_down,
+ *                   sys_dim, env_dim}; data = sectors back to back, each
+ *                   column-major sys_dim x env_dim.
+ */
+#ifndef TJSQ_ORACLE_H
+#define TJSQ_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 6j symbol on doubled arguments (W6JS, code/tjsq/block.f90:1564-1615). */
+double oracle_w6js(int j1, int j2, int j3, int l1, int l2, int l3);
+
+/* Fusion basis of block (x) site (Get_basis, block.f90:9-104).
+ * blk[3*nb] = {num_up, num_down, sdim} of the block's sectors (ham order),
+ * site[3*nsite] likewise.  Writes sec/sub (caller sizes them at 3*nb*nsite)
+ * and returns the number of new sectors; *nsub_total receives the sub count. */
+int oracle_get_basis(int blen, int nb, const int* blk, int nsite, const int* site,
+                     int* sec, int* sub, int* nsub_total);
+
+/* Superblock sector list (Get_super_basis, block.f90:110-161). */
+int oracle_get_super_basis(int nsys, const int* sys_sec, int nenv, const int* env_sec,
+                           int total_up, int total_down, int* wsec);
+
+/* Lattice description for the square cylinder (y periodic, x open). */
+typedef struct {
+  int ny;          /* nleg */
+  int nx_full;     /* columns of the full lattice used for INB1/nleg11 */
+  double jt[2];    /* hopping coupling of the nn (dir 1-4) and nnn (5-8) bonds */
+  double jd[2];    /* spin coupling (reference convention jd = -sqrt(3) J) */
+  double jn[2];    /* density coupling (jn = -J/4) */
+  double esite;    /* on-site potential on every site */
+  int up_bias;     /* 0: SU(2) only (shipped, common.f90:12); 1: (N, 2S) */
+} oracle_lattice;
+
+/* y = H_eff x: hmerge followed by model_wave
+ * (get_sys.f90:4383-4593 and 4127-4240, add_op = 1 branch). */
+int oracle_model_wave(const oracle_lattice* lat, int sys_len, int env_len,
+                      int sys_nops, const int* sys_hdr, const int* sys_sub,
+                      const int64_t* sys_off, const double* sys_data,
+                      int env_nops, const int* env_hdr, const int* env_sub,
+                      const int64_t* env_off, const double* env_data,
+                      int sys_nsec, const int* sys_bsec, const int* sys_bsub,
+                      int env_nsec, const int* env_bsec, const int* env_bsub,
+                      int nw, const int* wsec, const double* x, double* y);
+
+/* Lattice tables after dmrg_setup0 / dmrg_setup(ns, ns21)
+ * (dmrg_sub.f90:1-29, 31-281).  Exposed so tests can pin them. */
+int oracle_lattice_tables(const oracle_lattice* lat, int ns, int ns21,
+                          int* nposi, int* nposi_lat, int* nposi_dm,
+                          int* neib /* (nsite+1)*12, row-major [site][dir] */,
+                          int* inb1_col_sys, int* inb1_col_env,
+                          int* nleg11_sys, int* nleg11_env);
+
+/* Coupling structure produced by hmerge (get_sys.f90:4383-4593):
+ * coup_bs[4] = {(1,1),(1,2),(2,1),(2,2)}, coup_bb[2], inb[0..sys+env+1]. */
+int oracle_hmerge_tables(const oracle_lattice* lat, int sys_len, int env_len,
+                         int* coup_bs, int* coup_bb, int* inb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
