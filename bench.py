@@ -1,0 +1,177 @@
+"""Lanczos ground-state solves/s on one MI355X: mixed precision vs plain FP64.
+
+Workload (one "step" = one Lanczos ground-state solve at one bond, the unit of
+lanczos_memo_new): t-J model on a 6 x 16 cylinder (t = 1, t' = -0.2,
+J = 0.4, J' = J (t'/t)^2, N = 84, doping 1/8, S = 0), centre bond
+[47 sites | site | site | 47 sites], m = 8000 SU(2) multiplets per block.
+The block operators are synthetic and symmetric (grand_sql_tj_amd.synth,
+SplitMix64-seeded): sector dimensions log-normal (median 40, sigma 0.8,
+max 600), reduced operators with the reference's selection rules and
+hermiticity, so H_eff is symmetric.  Inputs are resident in HBM before the
+timed region.  The same fixed start vector starts every solve.
+
+The printed JSON line carries solves/s of the mixed path (value), the FP64
+path for comparison, the energy difference per site, p50/p99 latency, the
+roofline of the dominant kernel (stage-2 grouped GEMM, HIP events on the
+launching stream) and the CPU baseline: the reference's own model_wave
+(compiled Fortran + MKL, one core) timed on a bounded sample of applies of
+the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15 / 3.0}
+HBM_PEAK = 8.0e12
+
+
+def workload(m_mult: int, seed: int):
+    from grand_sql_tj_amd import synth
+    from grand_sql_tj_amd.lattice import TJModel
+    model = TJModel(ny=6, nx=16, t=1.0, tp=-0.2, J=0.4, mu=0.0, up_bias=1)
+    sb = synth.superblock(model, 47, 47, m_mult, m_mult, seed=seed, median=40, sigma=0.8, maxdim=600, nsec=140,
+                          smax=12, filling=0.875, total_up=84)
+    return model, sb
+
+
+def timed_solves(plan, x0_t, steps, warmup, precision, tol, max_iter, torch):
+    lat = []
+    res = None
+    for i in range(warmup + steps):
+        psi = x0_t.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = plan.lanczos(psi, tol=tol, max_iter=max_iter, precision=precision, refine_tol=1e-6, max_refine=1)
+        torch.cuda.synchronize()
+        if i >= warmup:
+            lat.append(time.perf_counter() - t0)
+    return np.array(lat), res
+
+
+def cpu_baseline(model, sb, x0, n_iter, budget_s=20.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    if not oracle.ref_available():
+        return None
+    cwd = os.getcwd()
+    os.chdir(os.environ.get("TMPDIR", "/tmp"))
+    try:
+        times = []
+        while sum(times) < budget_s and len(times) < 3:
+            t0 = time.perf_counter()
+            oracle.ref_model_wave(model, sb, x0)
+            times.append(time.perf_counter() - t0)
+    finally:
+        os.chdir(cwd)
+    t_apply = float(np.median(times))
+    return {"value": 1.0 / (t_apply * n_iter), "unit": "solves/s", "cores": 1, "kind": "reference",
+            "sample": f"{len(times)} reference model_wave applies (hmerge + 14 kernels, MKL sequential) of this "
+                      f"workload, {t_apply:.3f} s each, scaled by the {n_iter} applies of one FP64 solve"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--m", type=int, default=8000)
+    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x3"])
+    ap.add_argument("--max-iter", type=int, default=40)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the solve is not sharded across GPUs in this build "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from grand_sql_tj_amd import capi
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
+    print(f"[bench] building workload m={args.m}", file=sys.stderr, flush=True)
+    model, sb = workload(args.m, args.seed)
+    from grand_sql_tj_amd import synth
+    x0 = synth.random_vector(sb, args.seed)
+    plan = capi.Plan(model, sb)
+    st = plan.stats
+    nsite = sb.sys_len + sb.env_len + 2
+    x0_t = torch.from_numpy(x0).cuda()
+    print(f"[bench] D={sb.dim} contributions={st['n_contrib']} F={st['flops_ref']:.3e}", file=sys.stderr, flush=True)
+
+    lat64, r64 = timed_solves(plan, x0_t, args.steps, args.warmup, "fp64", args.tol, args.max_iter, torch)
+    print(f"[bench] fp64 solve {np.median(lat64)*1e3:.1f} ms E0={r64['e0']:.12f} it={r64['iterations']}",
+          file=sys.stderr, flush=True)
+    latmp, rmp = timed_solves(plan, x0_t, args.steps, args.warmup, args.precision, args.tol, args.max_iter, torch)
+    print(f"[bench] {args.precision} solve {np.median(latmp)*1e3:.1f} ms E0={rmp['e0']:.12f} it={rmp['iterations']} "
+          f"refine={rmp['refinements']}", file=sys.stderr, flush=True)
+
+    # the dominant kernel timed on isolated applies of the mixed precision
+    # (inside a solve the FP64 residual apply shares the stage counters)
+    xf = x0_t.float()
+    yf = torch.empty_like(xf)
+    plan.profile(True)
+    for _ in range(10):
+        plan.apply(xf, yf, precision=args.precision)
+    torch.cuda.synchronize()
+    ms_k, n_k = plan.profile_read()
+    plan.profile(False)
+    avg1 = ms_k[0] / max(1, n_k[0]) * 1e-3
+    avg2 = ms_k[1] / max(1, n_k[1]) * 1e-3
+    dom = 2 if avg2 >= avg1 else 1
+    t_dom = avg2 if dom == 2 else avg1
+
+    value = 1.0 / float(np.mean(latmp))
+    v64 = 1.0 / float(np.mean(lat64))
+    de_site = abs(rmp["e0"] - r64["e0"]) / nsite
+    cpu = None if args.no_cpu else cpu_baseline(model, sb, x0, max(1, r64["applies"]))
+    line = {
+        "metric": "Lanczos solves/s vs plain FP64 GPU at dE/site<=1e-8; p50/p99 solve latency",
+        "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * float(np.mean(latmp)), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": {"fp32": "f32", "bf16x3": "bf16x3->f32"}[args.precision] + "+f64",
+        "data": "synthetic",
+        "config": {"workload": "tjsq Lanczos ground-state solve, t-J 6x16 cylinder (t=1, t'=-0.2, J=0.4, N=84, S=0), "
+                               f"centre bond 47|s|s|47, m={args.m} SU(2) multiplets, synthetic symmetric blocks",
+                   "D": sb.dim, "precision": args.precision, "lanczos_tol": args.tol, "max_iter": args.max_iter},
+        "fp64_solves_per_s": v64, "speedup_vs_fp64": value / v64, "dE_per_site": de_site,
+        "e0_fp64": r64["e0"], "e0_mixed": rmp["e0"], "residual_mixed": rmp["residual"],
+        "iterations_fp64": r64["iterations"], "iterations_mixed": rmp["iterations"],
+        "p50_ms": 1e3 * float(np.percentile(latmp, 50)), "p99_ms": 1e3 * float(np.percentile(latmp, 99)),
+        "p50_ms_fp64": 1e3 * float(np.percentile(lat64, 50)), "p99_ms_fp64": 1e3 * float(np.percentile(lat64, 99)),
+        "apply_stage_ms": {"stage1": avg1 * 1e3, "stage2": avg2 * 1e3},
+    }
+    # roofline of the dominant stage: algorithmic flops of that stage per launch
+    hp = capi.HostPlan(model, sb)
+    # stage-1 flops: distinct A.X products; stage-2 flops: the rest of flops_exec
+    seg1 = hp.seg1
+    tile1 = hp.tile1
+    fl1 = 0.0
+    for t in tile1:
+        s = seg1[int(t["seg_begin"])]
+        fl1 += 2.0 * int(t["mt"]) * int(t["nt"]) * int(s["K"])
+    fl2 = st["flops_exec"] - fl1
+    fl_dom = fl2 if dom == 2 else fl1
+    achieved = fl_dom / t_dom
+    peak = PEAK[args.precision]
+    line["roofline"] = {"bound": "mfma", "kernel": "k_tiles_" + ("bf16x3" if args.precision == "bf16x3" else "f32")
+                        + f" stage {dom}", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                        "frac": achieved / peak, "traffic": None,
+                        "algorithmic_flops_per_launch": fl_dom, "avg_launch_ms": t_dom * 1e3}
+    line["cpu_baseline"] = cpu
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -203,8 +203,9 @@ class OpSetBuilder:
 
 
 def spin_hermitian_factor(bra_2s: int, ket_2s: int) -> float:
-    """Factor f with <a||S||b> = f * <b||S||a> for a hermitian rank-1 block
-    operator in the reduced-matrix convention of the reference (bra 2S = a,
-    ket 2S = b): f = (-1)^((a-b)/2) * sqrt((b+1)/(a+1))."""
-    sign = -1.0 if ((bra_2s - ket_2s) // 2) % 2 else 1.0
-    return sign * np.sqrt((ket_2s + 1.0) / (bra_2s + 1.0))
+    """Factor f with <a||S||b> = f * <b||S||a>^T for a hermitian rank-1 block
+    operator in the reduced-matrix convention of the apply kernels (bra 2S = a,
+    ket 2S = b): f = (-1)^((a-b)/2).  With it the spin-spin terms of H_eff
+    (get_sys.f90:2016-2981, one pass without an explicit conjugate) are
+    symmetric; the tests check <x, H y> = <H x, y>."""
+    return -1.0 if ((bra_2s - ket_2s) // 2) % 2 else 1.0

@@ -1,0 +1,327 @@
+"""ctypes binding of libtjsq_hip.so (the C-ABI declared in include/tjsq_capi.h).
+
+The library is built in-tree by ``grand_sql_tj_amd.build.build()``.  Loading
+fails loudly when it is missing: there is no CPU fallback for the apply or the
+solver.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtjsq_hip.so")
+
+TJSQ_OK = 0
+FP64, FP32, BF16X3 = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
+
+
+class Lattice(C.Structure):
+    _fields_ = [("ny", C.c_int), ("nx_tables", C.c_int), ("jt", C.c_double * 2), ("jd", C.c_double * 2),
+                ("jn", C.c_double * 2), ("esite", C.c_double), ("up_bias", C.c_int)]
+
+
+class OpSetC(C.Structure):
+    _fields_ = [("length", C.c_int), ("nops", C.c_int), ("hdr", C.c_void_p), ("sub", C.c_void_p),
+                ("off", C.c_void_p), ("data", C.c_void_p), ("ndata", C.c_int64), ("data_on_device", C.c_int)]
+
+
+class BasisC(C.Structure):
+    _fields_ = [("length", C.c_int), ("nsec", C.c_int), ("sec", C.c_void_p), ("sub", C.c_void_p)]
+
+
+class PlanStats(C.Structure):
+    _fields_ = [("dim", C.c_int64), ("n_contrib", C.c_int64), ("n_chain", C.c_int64), ("n_stage1", C.c_int64),
+                ("n_out_blocks", C.c_int64), ("n_tiles1", C.c_int64), ("n_tiles2", C.c_int64),
+                ("flops_ref", C.c_double), ("flops_exec", C.c_double), ("bytes_fp64", C.c_double),
+                ("op_elems", C.c_int64), ("scratch_elems", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LanczosOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iter", C.c_int), ("precision", C.c_int), ("refine_tol", C.c_double),
+                ("max_refine", C.c_int)]
+
+
+class LanczosResult(C.Structure):
+    _fields_ = [("e0", C.c_double), ("e_ritz", C.c_double), ("residual", C.c_double), ("iterations", C.c_int),
+                ("applies", C.c_int), ("refinements", C.c_int), ("breakdown", C.c_int), ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# task-table element layouts (must match tjsq_plan.hpp)
+SEG_DTYPE = np.dtype([("l_off", "<i8"), ("r_off", "<i8"), ("l_ld", "<i4"), ("r_ld", "<i4"), ("K", "<i4"),
+                      ("l_space", "u1"), ("r_space", "u1"), ("l_t", "u1"), ("r_t", "u1"), ("c", "<f8")])
+TILE_DTYPE = np.dtype([("out_off", "<i8"), ("out_ld", "<i4"), ("m0", "<i4"), ("n0", "<i4"), ("mt", "<i4"),
+                       ("nt", "<i4"), ("seg_begin", "<i4"), ("seg_end", "<i4"), ("out_space", "<i4")])
+MERGE_DTYPE = np.dtype([("dst_off", "<i8"), ("src_off", "<i8"), ("n", "<i8"), ("coef", "<f8")])
+assert SEG_DTYPE.itemsize == 40 and TILE_DTYPE.itemsize == 40 and MERGE_DTYPE.itemsize == 32
+
+EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_tables", "tjsq_active_sites",
+           "tjsq_plan_create", "tjsq_plan_stats_get", "tjsq_plan_destroy", "tjsq_heff_apply", "tjsq_heff_apply_f32",
+           "tjsq_lanczos_solve", "tjsq_hostplan_create", "tjsq_hostplan_sizes", "tjsq_hostplan_tables",
+           "tjsq_hostplan_stats", "tjsq_hostplan_destroy", "tjsq_plan_profile", "tjsq_plan_profile_read",
+           "tjsq_last_error"]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run grand_sql_tj_amd.build.build() (hipcc, gfx950) first")
+        L = C.CDLL(LIB_PATH)
+        L.tjsq_w6js.restype = C.c_double
+        L.tjsq_w6js.argtypes = [C.c_int] * 6
+        L.tjsq_last_error.restype = C.c_char_p
+        L.tjsq_last_error.argtypes = []
+        P, I, I64 = C.c_void_p, C.c_int, C.c_int64
+        sig = {
+            "tjsq_get_basis": [I, I, P, I, P, P, P, P, P],
+            "tjsq_get_super_basis": [P, P, I, I, P, P],
+            "tjsq_hmerge_tables": [P, I, I, P, P, P],
+            "tjsq_active_sites": [P, I],
+            "tjsq_plan_create": [P, I, I, P, P, P, P, I, P, I, P, P],
+            "tjsq_plan_stats_get": [P, P],
+            "tjsq_plan_destroy": [P],
+            "tjsq_heff_apply": [P, P, P, P],
+            "tjsq_heff_apply_f32": [P, I, P, P, P],
+            "tjsq_lanczos_solve": [P, P, P, P, P],
+            "tjsq_hostplan_create": [P, I, I, P, P, P, P, I, P, P],
+            "tjsq_hostplan_sizes": [P, P],
+            "tjsq_hostplan_tables": [P, P, P, P, P, P],
+            "tjsq_hostplan_stats": [P, P],
+            "tjsq_hostplan_destroy": [P],
+            "tjsq_plan_profile": [P, I],
+            "tjsq_plan_profile_read": [P, P, P],
+        }
+        for name in EXPORTS:
+            f = getattr(L, name)
+            if name in sig:
+                f.restype = C.c_int
+                f.argtypes = sig[name]
+        del I64
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str):
+    if rc != TJSQ_OK:
+        msg = lib().tjsq_last_error().decode()
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+
+def lattice_struct(model) -> Lattice:
+    lat = Lattice()
+    lat.ny = model.ny
+    lat.nx_tables = model.table_columns
+    lat.jt[0], lat.jt[1] = model.jt
+    lat.jd[0], lat.jd[1] = model.jd
+    lat.jn[0], lat.jn[1] = model.jn
+    lat.esite = model.esite
+    lat.up_bias = model.up_bias
+    return lat
+
+
+class _Keep:
+    """Holds the numpy arrays a C struct points to."""
+
+    def __init__(self):
+        self.arrays = []
+
+    def ptr(self, a, dtype):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        self.arrays.append(a)
+        return a.ctypes.data
+
+
+def opset_struct(ops, keep: _Keep, device_data=None) -> OpSetC:
+    s = OpSetC()
+    s.length = ops.length
+    s.nops = ops.nops
+    s.hdr = keep.ptr(ops.hdr, np.int32)
+    s.sub = keep.ptr(ops.sub, np.int32)
+    s.off = keep.ptr(ops.off, np.int64)
+    if device_data is not None:
+        s.data = C.c_void_p(device_data)
+        s.data_on_device = 1
+    else:
+        s.data = keep.ptr(ops.data, np.float64)
+        s.data_on_device = 0
+    s.ndata = int(ops.data.shape[0])
+    return s
+
+
+def basis_struct(bs, keep: _Keep) -> BasisC:
+    b = BasisC()
+    b.length = bs.length
+    b.nsec = bs.nsec
+    b.sec = keep.ptr(bs.sec, np.int32)
+    b.sub = keep.ptr(bs.sub, np.int32)
+    return b
+
+
+def w6js(*j) -> float:
+    return lib().tjsq_w6js(*[int(v) for v in j])
+
+
+def get_basis(block_len, blk, site):
+    blk = np.ascontiguousarray(blk, dtype=np.int32)
+    site = np.ascontiguousarray(site, dtype=np.int32)
+    cap = 3 * max(1, blk.shape[0]) * max(1, site.shape[0])
+    sec = np.zeros(4 * cap, np.int32)
+    sub = np.zeros(6 * cap, np.int32)
+    ns, nq = C.c_int(0), C.c_int(0)
+    check(lib().tjsq_get_basis(block_len, blk.shape[0], blk.ctypes.data, site.shape[0], site.ctypes.data,
+                               sec.ctypes.data, sub.ctypes.data, C.byref(ns), C.byref(nq)), "tjsq_get_basis")
+    return sec[:4 * ns.value].reshape(-1, 4), sub[:6 * nq.value].reshape(-1, 6)
+
+
+def get_super_basis(sys_bs, env_bs, total_up, total_down):
+    keep = _Keep()
+    s, e = basis_struct(sys_bs, keep), basis_struct(env_bs, keep)
+    w = np.zeros(6 * max(1, sys_bs.nsec * env_bs.nsec), np.int32)
+    nw = C.c_int(0)
+    check(lib().tjsq_get_super_basis(C.byref(s), C.byref(e), total_up, total_down, w.ctypes.data, C.byref(nw)),
+          "tjsq_get_super_basis")
+    return w[:6 * nw.value].reshape(-1, 6)
+
+
+def hmerge_tables(model, sys_len, env_len):
+    lat = lattice_struct(model)
+    cbs = np.zeros(4, np.int32)
+    cbb = np.zeros(2, np.int32)
+    inb = np.zeros(sys_len + env_len + 3, np.int32)
+    check(lib().tjsq_hmerge_tables(C.byref(lat), sys_len, env_len, cbs.ctypes.data, cbb.ctypes.data,
+                                   inb.ctypes.data), "tjsq_hmerge_tables")
+    return cbs, cbb, inb
+
+
+def active_sites(model, length) -> int:
+    lat = lattice_struct(model)
+    r = lib().tjsq_active_sites(C.byref(lat), length)
+    if r < 0:
+        check(r, "tjsq_active_sites")
+    return r
+
+
+class HostPlan:
+    """Host-only compile of a superblock's H_eff into task tables."""
+
+    def __init__(self, model, sb):
+        keep = _Keep()
+        lat = lattice_struct(model)
+        s, e = opset_struct(sb.sys, keep), opset_struct(sb.env, keep)
+        sbs, ebs = basis_struct(sb.sys_bs, keep), basis_struct(sb.env_bs, keep)
+        w = keep.ptr(sb.wsec, np.int32)
+        h = C.c_void_p()
+        check(lib().tjsq_hostplan_create(C.byref(lat), sb.sys_len, sb.env_len, C.byref(s), C.byref(e), C.byref(sbs),
+                                         C.byref(ebs), int(sb.wsec.shape[0]), C.c_void_p(w), C.byref(h)),
+              "tjsq_hostplan_create")
+        self.h = h
+        sizes = np.zeros(11, np.int64)
+        check(lib().tjsq_hostplan_sizes(h, sizes.ctypes.data), "tjsq_hostplan_sizes")
+        (n1, t1, n2, t2, nm, self.arena_elems, self.scratch_elems, self.dim, self.sys_base, self.env_base,
+         self.merged_base) = [int(v) for v in sizes]
+        self.seg1 = np.zeros(n1, SEG_DTYPE)
+        self.tile1 = np.zeros(t1, TILE_DTYPE)
+        self.seg2 = np.zeros(n2, SEG_DTYPE)
+        self.tile2 = np.zeros(t2, TILE_DTYPE)
+        self.merge = np.zeros(nm, MERGE_DTYPE)
+        check(lib().tjsq_hostplan_tables(h, self.seg1.ctypes.data, self.tile1.ctypes.data, self.seg2.ctypes.data,
+                                         self.tile2.ctypes.data, self.merge.ctypes.data), "tjsq_hostplan_tables")
+        st = PlanStats()
+        check(lib().tjsq_hostplan_stats(h, C.byref(st)), "tjsq_hostplan_stats")
+        self.stats = st.as_dict()
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.tjsq_hostplan_destroy(self.h)
+            self.h = None
+
+
+class Plan:
+    """Device plan of one superblock (tjsq_plan_create).  Vectors are torch
+    CUDA tensors; the stream is torch's current stream."""
+
+    def __init__(self, model, sb, precisions=("fp64", "fp32", "bf16x3"), device_ops=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("tjsq Plan needs a GPU (HIP device)")
+        self.torch = torch
+        keep = _Keep()
+        lat = lattice_struct(model)
+        dsys = device_ops[0].data_ptr() if device_ops is not None else None
+        denv = device_ops[1].data_ptr() if device_ops is not None else None
+        s, e = opset_struct(sb.sys, keep, dsys), opset_struct(sb.env, keep, denv)
+        sbs, ebs = basis_struct(sb.sys_bs, keep), basis_struct(sb.env_bs, keep)
+        w = keep.ptr(sb.wsec, np.int32)
+        mask = 0
+        for p in precisions:
+            mask |= 1 << PRECISIONS[p]
+        h = C.c_void_p()
+        stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        check(lib().tjsq_plan_create(C.byref(lat), sb.sys_len, sb.env_len, C.byref(s), C.byref(e), C.byref(sbs),
+                                     C.byref(ebs), int(sb.wsec.shape[0]), C.c_void_p(w), mask, stream, C.byref(h)),
+              "tjsq_plan_create")
+        self.h = h
+        self.dim = sb.dim
+        st = PlanStats()
+        check(lib().tjsq_plan_stats_get(h, C.byref(st)), "tjsq_plan_stats_get")
+        self.stats = st.as_dict()
+
+    def _stream(self):
+        return C.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+
+    def apply(self, x, y=None, precision="fp64"):
+        """y = H_eff x (model_wave) for a CUDA tensor x of length D."""
+        torch = self.torch
+        p = PRECISIONS[precision]
+        dt = torch.float64 if p == FP64 else torch.float32
+        if x.dtype != dt or not x.is_cuda or x.numel() != self.dim or not x.is_contiguous():
+            raise ValueError(f"x must be a contiguous CUDA {dt} tensor of length {self.dim}")
+        if y is None:
+            y = torch.empty_like(x)
+        if p == FP64:
+            check(lib().tjsq_heff_apply(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()), self._stream()),
+                  "tjsq_heff_apply")
+        else:
+            check(lib().tjsq_heff_apply_f32(self.h, p, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()),
+                                            self._stream()), "tjsq_heff_apply_f32")
+        return y
+
+    def lanczos(self, psi, tol=1e-10, max_iter=40, precision="fp64", refine_tol=1e-6, max_refine=1):
+        """Ground state (lanczos_memo_new, levels = 1).  psi: CUDA float64
+        start vector, overwritten with the normalised ground state."""
+        torch = self.torch
+        if psi.dtype != torch.float64 or not psi.is_cuda or psi.numel() != self.dim:
+            raise ValueError("psi must be a CUDA float64 tensor of length D")
+        o = LanczosOpts()
+        o.tol, o.max_iter, o.precision = tol, max_iter, PRECISIONS[precision]
+        o.refine_tol, o.max_refine = refine_tol, max_refine
+        r = LanczosResult()
+        check(lib().tjsq_lanczos_solve(self.h, C.c_void_p(psi.data_ptr()), C.byref(o), C.byref(r), self._stream()),
+              "tjsq_lanczos_solve")
+        return r.as_dict()
+
+    def profile(self, on=True):
+        check(lib().tjsq_plan_profile(self.h, 1 if on else 0), "tjsq_plan_profile")
+
+    def profile_read(self):
+        ms = np.zeros(2)
+        n = np.zeros(2, np.int64)
+        check(lib().tjsq_plan_profile_read(self.h, ms.ctypes.data, n.ctypes.data), "tjsq_plan_profile_read")
+        return ms, n
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.tjsq_plan_destroy(self.h)
+            self.h = None

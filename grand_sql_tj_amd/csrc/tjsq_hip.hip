@@ -1,0 +1,1076 @@
+// tjsq_hip.hip -- CDNA4 (gfx950) kernels and the C-ABI of libtjsq_hip.so.
+//
+// The H_eff apply runs as two grouped-GEMM launches over the task tables of
+// tjsq_plan.cpp:
+//   stage 1  T_j = op(A_j) X_j            for the distinct A.X products of the
+//                                         two-sided terms (block x block);
+//   stage 2  Y_tile = sum_s c_s op(L_s) op(R_s)  pulled per output tile,
+//                                         no atomics, fixed summation order.
+// Each workgroup is one 64-lane wavefront that owns one 32x32 output tile and
+// accumulates it in MFMA registers (v_mfma_f64_16x16x4f64 in FP64,
+// v_mfma_f32_16x16x4f32 in FP32, v_mfma_f32_16x16x16bf16_1k three times for
+// the BF16x3 split), staging 32x16 / 16x32 operand panels through LDS.
+//
+// The Lanczos solver (lanczos_memo_new, code/tjsq/eigen.f90:2-175) keeps its
+// Krylov basis on the device; dot products are two-pass deterministic
+// reductions in FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tjsq_capi.h"
+#include "tjsq_plan.hpp"
+
+using tjsq::HostPlan;
+using tjsq::Seg;
+using tjsq::Tile;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& m) {
+  g_err = m;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) return fail(TJSQ_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int TS = 32;      // output tile edge
+constexpr int KC = 16;      // K chunk staged per step
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef short s4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// operand access
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Spaces {
+  const T* ops;
+  const T* x;
+  T* t;      // stage-1 scratch
+  T* y;
+};
+
+template <typename T>
+__device__ __forceinline__ const T* space_ptr(const Spaces<T>& S, int sp) {
+  return sp == tjsq::SP_OPS ? S.ops : (sp == tjsq::SP_X ? S.x : (const T*)S.t);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 grouped tile kernel
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_tiles_f64(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
+                                                  Spaces<double> S, int ntiles) {
+  const int tid = blockIdx.x;
+  if (tid >= ntiles) return;
+  const Tile tl = tiles[tid];
+  const int lane = threadIdx.x;
+  __shared__ double Ls[TS][KC + 1];
+  __shared__ double Rs[KC][TS + 1];
+  d4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  const int lr = lane & 15, lq = lane >> 4;
+  for (int si = tl.seg_begin; si < tl.seg_end; ++si) {
+    const Seg s = segs[si];
+    const double* L = space_ptr(S, s.l_space);
+    if (s.r_space == tjsq::SP_IDENT) {
+      for (int bi = 0; bi < 2; ++bi)
+        for (int bj = 0; bj < 2; ++bj)
+          for (int r = 0; r < 4; ++r) {
+            const int row = bi * 16 + lq * 4 + r, col = bj * 16 + lr;
+            if (row < tl.mt && col < tl.nt)
+              acc[bi][bj][r] += s.c * L[s.l_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * s.l_ld];
+          }
+      continue;
+    }
+    const double* R = space_ptr(S, s.r_space);
+    for (int k0 = 0; k0 < s.K; k0 += KC) {
+      for (int q = 0; q < (TS * KC) / 64; ++q) {
+        const int e = lane + 64 * q;
+        int i, k;
+        if (!s.l_t) { i = e & (TS - 1); k = e / TS; } else { k = e & (KC - 1); i = e / KC; }
+        double v = 0.0;
+        if (i < tl.mt && k0 + k < s.K)
+          v = s.l_t ? L[s.l_off + (k0 + k) + (int64_t)(tl.m0 + i) * s.l_ld]
+                    : L[s.l_off + (tl.m0 + i) + (int64_t)(k0 + k) * s.l_ld];
+        Ls[i][k] = v * s.c;
+      }
+      for (int q = 0; q < (TS * KC) / 64; ++q) {
+        const int e = lane + 64 * q;
+        int j, k;
+        if (!s.r_t) { k = e & (KC - 1); j = e / KC; } else { j = e & (TS - 1); k = e / TS; }
+        double v = 0.0;
+        if (j < tl.nt && k0 + k < s.K)
+          v = s.r_t ? R[s.r_off + (tl.n0 + j) + (int64_t)(k0 + k) * s.r_ld]
+                    : R[s.r_off + (k0 + k) + (int64_t)(tl.n0 + j) * s.r_ld];
+        Rs[k][j] = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < KC / 4; ++kk) {
+        const double a0 = Ls[lr][kk * 4 + lq], a1 = Ls[16 + lr][kk * 4 + lq];
+        const double b0 = Rs[kk * 4 + lq][lr], b1 = Rs[kk * 4 + lq][16 + lr];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      }
+      __syncthreads();
+    }
+  }
+  double* out = tl.out_space == tjsq::SP_T ? S.t : S.y;
+  for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj)
+      for (int r = 0; r < 4; ++r) {
+        const int row = bi * 16 + lq * 4 + r, col = bj * 16 + lr;
+        if (row < tl.mt && col < tl.nt) out[tl.out_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * tl.out_ld] = acc[bi][bj][r];
+      }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 grouped tile kernel
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_tiles_f32(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
+                                                  Spaces<float> S, int ntiles) {
+  const int tid = blockIdx.x;
+  if (tid >= ntiles) return;
+  const Tile tl = tiles[tid];
+  const int lane = threadIdx.x;
+  __shared__ float Ls[TS][KC + 1];
+  __shared__ float Rs[KC][TS + 1];
+  f4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  const int lr = lane & 15, lq = lane >> 4;
+  for (int si = tl.seg_begin; si < tl.seg_end; ++si) {
+    const Seg s = segs[si];
+    const float c = (float)s.c;
+    const float* L = space_ptr(S, s.l_space);
+    if (s.r_space == tjsq::SP_IDENT) {
+      for (int bi = 0; bi < 2; ++bi)
+        for (int bj = 0; bj < 2; ++bj)
+          for (int r = 0; r < 4; ++r) {
+            const int row = bi * 16 + lq * 4 + r, col = bj * 16 + lr;
+            if (row < tl.mt && col < tl.nt)
+              acc[bi][bj][r] += c * L[s.l_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * s.l_ld];
+          }
+      continue;
+    }
+    const float* R = space_ptr(S, s.r_space);
+    for (int k0 = 0; k0 < s.K; k0 += KC) {
+      for (int q = 0; q < (TS * KC) / 64; ++q) {
+        const int e = lane + 64 * q;
+        int i, k;
+        if (!s.l_t) { i = e & (TS - 1); k = e / TS; } else { k = e & (KC - 1); i = e / KC; }
+        float v = 0.f;
+        if (i < tl.mt && k0 + k < s.K)
+          v = s.l_t ? L[s.l_off + (k0 + k) + (int64_t)(tl.m0 + i) * s.l_ld]
+                    : L[s.l_off + (tl.m0 + i) + (int64_t)(k0 + k) * s.l_ld];
+        Ls[i][k] = v * c;
+      }
+      for (int q = 0; q < (TS * KC) / 64; ++q) {
+        const int e = lane + 64 * q;
+        int j, k;
+        if (!s.r_t) { k = e & (KC - 1); j = e / KC; } else { j = e & (TS - 1); k = e / TS; }
+        float v = 0.f;
+        if (j < tl.nt && k0 + k < s.K)
+          v = s.r_t ? R[s.r_off + (tl.n0 + j) + (int64_t)(k0 + k) * s.r_ld]
+                    : R[s.r_off + (k0 + k) + (int64_t)(tl.n0 + j) * s.r_ld];
+        Rs[k][j] = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < KC / 4; ++kk) {
+        const float a0 = Ls[lr][kk * 4 + lq], a1 = Ls[16 + lr][kk * 4 + lq];
+        const float b0 = Rs[kk * 4 + lq][lr], b1 = Rs[kk * 4 + lq][16 + lr];
+        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+      }
+      __syncthreads();
+    }
+  }
+  float* out = tl.out_space == tjsq::SP_T ? S.t : S.y;
+  for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj)
+      for (int r = 0; r < 4; ++r) {
+        const int row = bi * 16 + lq * 4 + r, col = bj * 16 + lr;
+        if (row < tl.mt && col < tl.nt) out[tl.out_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * tl.out_ld] = acc[bi][bj][r];
+      }
+}
+
+// ---------------------------------------------------------------------------
+// BF16x3 grouped tile kernel: a*b ~ ah*bh + ah*bl + al*bh with a = ah + al
+// (bf16 hi part by round-to-nearest-even, lo part the rounded remainder),
+// FP32 accumulation in the MFMA registers.  Operands are FP32 in memory and
+// split while staging into LDS.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
+  unsigned int u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
+
+__global__ __launch_bounds__(64) void k_tiles_bf16x3(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
+                                                     Spaces<float> S, int ntiles) {
+  const int tid = blockIdx.x;
+  if (tid >= ntiles) return;
+  const Tile tl = tiles[tid];
+  const int lane = threadIdx.x;
+  // [hi/lo][i][k] and [hi/lo][j][k] (k fastest so a lane reads 4 consecutive k)
+  __shared__ unsigned short Lh[2][TS][KC + 4];
+  __shared__ unsigned short Rh[2][TS][KC + 4];
+  f4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  const int lr = lane & 15, lq = lane >> 4;
+  for (int si = tl.seg_begin; si < tl.seg_end; ++si) {
+    const Seg s = segs[si];
+    const float c = (float)s.c;
+    const float* L = space_ptr(S, s.l_space);
+    if (s.r_space == tjsq::SP_IDENT) {
+      for (int bi = 0; bi < 2; ++bi)
+        for (int bj = 0; bj < 2; ++bj)
+          for (int r = 0; r < 4; ++r) {
+            const int row = bi * 16 + lq * 4 + r, col = bj * 16 + lr;
+            if (row < tl.mt && col < tl.nt)
+              acc[bi][bj][r] += c * L[s.l_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * s.l_ld];
+          }
+      continue;
+    }
+    const float* R = space_ptr(S, s.r_space);
+    for (int k0 = 0; k0 < s.K; k0 += KC) {
+      for (int q = 0; q < (TS * KC) / 64; ++q) {
+        const int e = lane + 64 * q;
+        int i, k;
+        if (!s.l_t) { i = e & (TS - 1); k = e / TS; } else { k = e & (KC - 1); i = e / KC; }
+        float v = 0.f;
+        if (i < tl.mt && k0 + k < s.K)
+          v = s.l_t ? L[s.l_off + (k0 + k) + (int64_t)(tl.m0 + i) * s.l_ld]
+                    : L[s.l_off + (tl.m0 + i) + (int64_t)(k0 + k) * s.l_ld];
+        v *= c;
+        const unsigned short h = bf16_rne(v);
+        Lh[0][i][k] = h;
+        Lh[1][i][k] = bf16_rne(v - bf16_to_f(h));
+      }
+      for (int q = 0; q < (TS * KC) / 64; ++q) {
+        const int e = lane + 64 * q;
+        int j, k;
+        if (!s.r_t) { k = e & (KC - 1); j = e / KC; } else { j = e & (TS - 1); k = e / TS; }
+        float v = 0.f;
+        if (j < tl.nt && k0 + k < s.K)
+          v = s.r_t ? R[s.r_off + (tl.n0 + j) + (int64_t)(k0 + k) * s.r_ld]
+                    : R[s.r_off + (k0 + k) + (int64_t)(tl.n0 + j) * s.r_ld];
+        const unsigned short h = bf16_rne(v);
+        Rh[0][j][k] = h;
+        Rh[1][j][k] = bf16_rne(v - bf16_to_f(h));
+      }
+      __syncthreads();
+      // one 16x16x16 step covers the whole K chunk: lane supplies k = 4*lq..4*lq+3
+      s4 a[2][2], b[2][2];
+      for (int p = 0; p < 2; ++p)
+        for (int h = 0; h < 2; ++h) {
+          for (int t = 0; t < 4; ++t) {
+            a[p][h][t] = (short)Lh[h][p * 16 + lr][lq * 4 + t];
+            b[p][h][t] = (short)Rh[h][p * 16 + lr][lq * 4 + t];
+          }
+        }
+      for (int bi = 0; bi < 2; ++bi)
+        for (int bj = 0; bj < 2; ++bj) {
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][1], b[bj][0], acc[bi][bj], 0, 0, 0);
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][0], b[bj][1], acc[bi][bj], 0, 0, 0);
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][0], b[bj][0], acc[bi][bj], 0, 0, 0);
+        }
+      __syncthreads();
+    }
+  }
+  float* out = tl.out_space == tjsq::SP_T ? S.t : S.y;
+  for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj)
+      for (int r = 0; r < 4; ++r) {
+        const int row = bi * 16 + lq * 4 + r, col = bj * 16 + lr;
+        if (row < tl.mt && col < tl.nt) out[tl.out_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * tl.out_ld] = acc[bi][bj][r];
+      }
+}
+
+// ---------------------------------------------------------------------------
+// merged operators: dst[e] = sum_t coef_t * src_t[e] in recipe order
+// ---------------------------------------------------------------------------
+struct MergeDst { int64_t dst_off, n; int t_begin, t_end; };
+
+__global__ void k_merge(double* arena, const MergeDst* dsts, const tjsq::MergeTerm* terms, int ndst) {
+  const int d = blockIdx.y;
+  if (d >= ndst) return;
+  const MergeDst D = dsts[d];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < D.n; e += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int t = D.t_begin; t < D.t_end; ++t) acc = acc + terms[t].coef * arena[terms[t].src_off + e];
+    arena[D.dst_off + e] = acc;
+  }
+}
+
+__global__ void k_f64_to_f32(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+__global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ b, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (double)a[i];
+}
+
+// ---------------------------------------------------------------------------
+// Lanczos vector kernels (FP64 arithmetic on FP64 or FP32 storage)
+// ---------------------------------------------------------------------------
+constexpr int RED_BLOCKS = 512;
+constexpr int RED_THREADS = 256;
+
+// partial[b * nv + j] = sum over block b's range of V_j . w  (j < nv), and,
+// when with_self, partial[(RED_BLOCKS + b) * nv + j] = V_j . V_j
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_multidot(const T* __restrict__ V, int64_t ldv, int nv,
+                                                          const T* __restrict__ w, int64_t n, double* partial,
+                                                          int with_self) {
+  __shared__ double red[RED_THREADS / 64][64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int j0 = 0; j0 < nv; j0 += 32) {
+    const int nj = min(32, nv - j0);
+    double s[32], ss[32];
+    for (int j = 0; j < 32; ++j) { s[j] = 0.0; ss[j] = 0.0; }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+      const double wi = (double)w[i];
+      for (int j = 0; j < nj; ++j) {
+        const double v = (double)V[(int64_t)(j0 + j) * ldv + i];
+        s[j] += v * wi;
+        if (with_self) ss[j] += v * v;
+      }
+    }
+    for (int pass = 0; pass < (with_self ? 2 : 1); ++pass) {
+      for (int j = 0; j < nj; ++j) {
+        double v = pass ? ss[j] : s[j];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) red[wid][j] = v;
+      }
+      __syncthreads();
+      if (threadIdx.x < nj) {
+        double t = 0.0;
+        for (int q = 0; q < RED_THREADS / 64; ++q) t += red[q][threadIdx.x];
+        partial[((int64_t)pass * RED_BLOCKS + blockIdx.x) * nv + j0 + threadIdx.x] = t;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__global__ void k_finish_dots(const double* partial, int nv, int nblocks, int with_self, double* out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nv) return;
+  double t = 0.0;
+  for (int b = 0; b < nblocks; ++b) t += partial[(int64_t)b * nv + j];
+  out[j] = t;
+  if (with_self) {
+    double u = 0.0;
+    for (int b = 0; b < nblocks; ++b) u += partial[((int64_t)RED_BLOCKS + b) * nv + j];
+    out[nv + j] = u;
+  }
+}
+
+// w = (w - sum_j coef_j V_j) * post_scale ; coefficients from device memory
+template <typename T>
+__global__ void k_combine(T* __restrict__ w, const T* __restrict__ V, int64_t ldv, int nv, const double* coef,
+                          int64_t n, double post_scale, int accumulate_into_zero) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = accumulate_into_zero ? 0.0 : (double)w[i];
+    double sub = 0.0;
+    for (int j = 0; j < nv; ++j) sub += coef[j] * (double)V[(int64_t)j * ldv + i];
+    w[i] = (T)((acc - sub) * post_scale);
+  }
+}
+
+template <typename T>
+__global__ void k_scale(T* __restrict__ w, int64_t n, double s) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    w[i] = (T)((double)w[i] * s);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// plan object
+// ---------------------------------------------------------------------------
+struct tjsq_plan {
+  HostPlan hp;
+  int device = 0;
+  double* arena = nullptr;       // FP64 operators (sys, env, merged)
+  float* arena32 = nullptr;      // FP32 copy
+  double* t64 = nullptr;         // stage-1 scratch
+  float* t32 = nullptr;
+  Tile* tile1 = nullptr;
+  Tile* tile2 = nullptr;
+  Seg* seg1 = nullptr;
+  Seg* seg2 = nullptr;
+  // Lanczos workspace
+  void* kry = nullptr;           // Krylov basis
+  int64_t kry_bytes = 0;
+  double* dwork = nullptr;       // partials + dot outputs + coefficients
+  float* x32 = nullptr;          // FP32 staging for a residual check
+  double* x64a = nullptr;
+  double* x64b = nullptr;
+  // per-launch HIP events of the two apply stages while profiling is on
+  bool profiling = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof;
+};
+
+namespace {
+
+int grid_for(int64_t n, int threads = 256) {
+  int64_t g = (n + threads - 1) / threads;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, 8192));
+}
+
+int upload(tjsq_plan* p, const tjsq_opset& set, int64_t base, hipStream_t st) {
+  if (set.ndata == 0) return TJSQ_OK;
+  HIPCHK(hipMemcpyAsync(p->arena + base, set.data, sizeof(double) * set.ndata,
+                        set.data_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+  return TJSQ_OK;
+}
+
+template <typename T>
+int launch_tiles(const tjsq_plan* p, int stage, const T* ops, const T* x, T* t, T* y, int precision,
+                 hipStream_t st) {
+  const int n = stage == 1 ? (int)p->hp.tile1.size() : (int)p->hp.tile2.size();
+  if (n == 0) return TJSQ_OK;
+  const Tile* tiles = stage == 1 ? p->tile1 : p->tile2;
+  const Seg* segs = stage == 1 ? p->seg1 : p->seg2;
+  Spaces<T> S{ops, x, t, y};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (p->profiling) {
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, st));
+  }
+  if constexpr (sizeof(T) == 8) {
+    hipLaunchKernelGGL(k_tiles_f64, dim3(n), dim3(64), 0, st, tiles, segs, S, n);
+  } else {
+    if (precision == TJSQ_BF16X3)
+      hipLaunchKernelGGL(k_tiles_bf16x3, dim3(n), dim3(64), 0, st, tiles, segs, S, n);
+    else
+      hipLaunchKernelGGL(k_tiles_f32, dim3(n), dim3(64), 0, st, tiles, segs, S, n);
+  }
+  HIPCHK(hipGetLastError());
+  if (p->profiling) {
+    HIPCHK(hipEventRecord(e1, st));
+    const_cast<tjsq_plan*>(p)->prof.push_back({stage, {e0, e1}});
+  }
+  return TJSQ_OK;
+}
+
+// tridiagonal eigenproblem (implicit QL with Wilkinson shifts, Golub & Van
+// Loan 8.3); d: diagonal (out: eigenvalues ascending), e: off-diagonal,
+// z: n x n eigenvectors (column-major) if wantz
+void tridiag_eig(int n, std::vector<double>& d, std::vector<double> e, std::vector<double>* z) {
+  if (z) {
+    z->assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) (*z)[(size_t)i * n + i] = 1.0;
+  }
+  e.resize(n + 1, 0.0);
+  e[n - 1] = 0.0;
+  for (int l = 0; l < n; ++l) {
+    int iter = 0;
+    int m;
+    do {
+      for (m = l; m < n - 1; ++m) {
+        const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (std::fabs(e[m]) <= 1e-16 * dd) break;
+      }
+      if (m != l) {
+        if (++iter > 60) break;
+        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+        double r = std::hypot(g, 1.0);
+        g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
+        double s = 1.0, c = 1.0, p = 0.0;
+        int i;
+        for (i = m - 1; i >= l; --i) {
+          double f = s * e[i], b = c * e[i];
+          r = std::hypot(f, g);
+          e[i + 1] = r;
+          if (r == 0.0) {
+            d[i + 1] -= p;
+            e[m] = 0.0;
+            break;
+          }
+          s = f / r;
+          c = g / r;
+          g = d[i + 1] - p;
+          r = (d[i] - g) * s + 2.0 * c * b;
+          p = s * r;
+          d[i + 1] = g + p;
+          g = c * r - b;
+          if (z)
+            for (int k = 0; k < n; ++k) {
+              double* zc = z->data();
+              f = zc[(size_t)(i + 1) * n + k];
+              zc[(size_t)(i + 1) * n + k] = s * zc[(size_t)i * n + k] + c * f;
+              zc[(size_t)i * n + k] = c * zc[(size_t)i * n + k] - s * f;
+            }
+        }
+        if (r == 0.0 && i >= l) continue;
+        d[l] -= p;
+        e[l] = g;
+        e[m] = 0.0;
+      }
+    } while (m != l);
+  }
+  // sort ascending with vectors
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](int a, int b) { return d[a] < d[b]; });
+  std::vector<double> d2(n);
+  std::vector<double> z2;
+  if (z) z2.resize((size_t)n * n);
+  for (int i = 0; i < n; ++i) {
+    d2[i] = d[idx[i]];
+    if (z)
+      for (int k = 0; k < n; ++k) z2[(size_t)i * n + k] = (*z)[(size_t)idx[i] * n + k];
+  }
+  d = d2;
+  if (z) *z = z2;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* tjsq_last_error(void) { return g_err.c_str(); }
+
+double tjsq_w6js(int j1, int j2, int j3, int l1, int l2, int l3) { return tjsq::w6js(j1, j2, j3, l1, l2, l3); }
+
+int tjsq_get_basis(int block_len, int nb, const int* blk, int nsite, const int* site, int* sec, int* sub, int* nsec,
+                   int* nsub) {
+  (void)block_len;
+  if (nb < 0 || nsite < 0 || (!blk && nb) || (!site && nsite) || !sec || !sub || !nsec || !nsub)
+    return fail(TJSQ_ERR_ARG, "tjsq_get_basis: null argument");
+  std::vector<int> s, q;
+  *nsec = tjsq::get_basis(nb, blk, nsite, site, s, q);
+  *nsub = (int)(q.size() / 6);
+  std::copy(s.begin(), s.end(), sec);
+  std::copy(q.begin(), q.end(), sub);
+  return TJSQ_OK;
+}
+
+int tjsq_get_super_basis(const tjsq_basis* sys_bs, const tjsq_basis* env_bs, int total_up, int total_down, int* wsec,
+                         int* nw) {
+  if (!sys_bs || !env_bs || !wsec || !nw) return fail(TJSQ_ERR_ARG, "tjsq_get_super_basis: null argument");
+  std::vector<int> w;
+  *nw = tjsq::get_super_basis(*sys_bs, *env_bs, total_up, total_down, w);
+  std::copy(w.begin(), w.end(), wsec);
+  return TJSQ_OK;
+}
+
+int tjsq_hmerge_tables(const tjsq_lattice* lat, int sys_len, int env_len, int* coup_bs, int* coup_bb, int* inb) {
+  if (!lat || !coup_bs || !coup_bb || !inb) return fail(TJSQ_ERR_ARG, "tjsq_hmerge_tables: null argument");
+  if (sys_len + env_len + 2 > lat->ny * lat->nx_tables || sys_len < 1 || env_len < 1)
+    return fail(TJSQ_ERR_ARG, "tjsq_hmerge_tables: lattice too small");
+  tjsq::LatticeTables L;
+  tjsq::build_lattice(*lat, L);
+  tjsq::lattice_setup(L, lat->ny, sys_len, env_len);
+  tjsq::Couplings C;
+  tjsq::hmerge_tables(L, sys_len, env_len, C);
+  coup_bs[0] = C.coup_bs[1][1];
+  coup_bs[1] = C.coup_bs[1][2];
+  coup_bs[2] = C.coup_bs[2][1];
+  coup_bs[3] = C.coup_bs[2][2];
+  coup_bb[0] = C.coup_bb[1];
+  coup_bb[1] = C.coup_bb[2];
+  for (int i = 0; i <= sys_len + env_len + 2; ++i) inb[i] = C.inb[i];
+  return TJSQ_OK;
+}
+
+int tjsq_active_sites(const tjsq_lattice* lat, int len) {
+  if (!lat || len < 1 || len > lat->ny * lat->nx_tables - 2) return fail(TJSQ_ERR_ARG, "tjsq_active_sites: bad length");
+  tjsq::LatticeTables L;
+  tjsq::build_lattice(*lat, L);
+  return L.nleg11[len];
+}
+
+int tjsq_plan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys, const tjsq_opset* env,
+                     const tjsq_basis* sys_bs, const tjsq_basis* env_bs, int nw, const int* wsec, int precision_mask,
+                     void* stream, tjsq_plan** plan) {
+  if (!lat || !sys || !env || !sys_bs || !env_bs || !wsec || !plan || nw < 1)
+    return fail(TJSQ_ERR_ARG, "tjsq_plan_create: null argument");
+  *plan = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(TJSQ_ERR_NODEV, "no GPU visible");
+  tjsq_plan* p = new tjsq_plan();
+  std::string msg;
+  int rc = tjsq::build_host_plan(*lat, sys_len, env_len, *sys, *env, *sys_bs, *env_bs, nw, wsec, TS, p->hp, msg);
+  if (rc != TJSQ_OK) {
+    delete p;
+    return fail(rc, "tjsq_plan_create: " + msg);
+  }
+  hipStream_t st = (hipStream_t)stream;
+  HostPlan& hp = p->hp;
+  auto cleanup = [&](int code, const std::string& m) {
+    tjsq_plan_destroy(p);
+    return fail(code, m);
+  };
+  (void)hipGetDevice(&p->device);
+  if (hipMalloc(&p->arena, sizeof(double) * std::max<int64_t>(1, hp.arena_elems)) != hipSuccess)
+    return cleanup(TJSQ_ERR_NOMEM, "operator arena allocation failed");
+  if ((rc = upload(p, *sys, hp.sys_base, st)) || (rc = upload(p, *env, hp.env_base, st))) {
+    tjsq_plan_destroy(p);
+    return rc;
+  }
+  // merged operators
+  {
+    std::vector<MergeDst> dsts;
+    std::vector<tjsq::MergeTerm> terms = hp.merge;
+    // group recipe terms by destination (recipes are emitted in accumulation
+    // order; stable grouping keeps that order per destination)
+    std::stable_sort(terms.begin(), terms.end(),
+                     [](const tjsq::MergeTerm& a, const tjsq::MergeTerm& b) { return a.dst_off < b.dst_off; });
+    // every merged sub-block (also those without terms) must be zeroed
+    const int64_t mstart = hp.merged_base, mend = hp.arena_elems;
+    if (mend > mstart) HIPCHK(hipMemsetAsync(p->arena + mstart, 0, sizeof(double) * (mend - mstart), st));
+    for (size_t t = 0; t < terms.size();) {
+      size_t u = t;
+      while (u < terms.size() && terms[u].dst_off == terms[t].dst_off) ++u;
+      dsts.push_back({terms[t].dst_off, terms[t].n, (int)t, (int)u});
+      t = u;
+    }
+    if (!dsts.empty()) {
+      MergeDst* dd = nullptr;
+      tjsq::MergeTerm* dt = nullptr;
+      if (hipMalloc(&dd, sizeof(MergeDst) * dsts.size()) != hipSuccess ||
+          hipMalloc(&dt, sizeof(tjsq::MergeTerm) * terms.size()) != hipSuccess)
+        return cleanup(TJSQ_ERR_NOMEM, "merge table allocation failed");
+      HIPCHK(hipMemcpyAsync(dd, dsts.data(), sizeof(MergeDst) * dsts.size(), hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(dt, terms.data(), sizeof(tjsq::MergeTerm) * terms.size(), hipMemcpyHostToDevice, st));
+      int64_t maxn = 1;
+      for (auto& d : dsts) maxn = std::max(maxn, d.n);
+      dim3 grid((unsigned)std::min<int64_t>((maxn + 255) / 256, 64), (unsigned)dsts.size());
+      hipLaunchKernelGGL(k_merge, grid, dim3(256), 0, st, p->arena, dd, dt, (int)dsts.size());
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(st));
+      (void)hipFree(dd);
+      (void)hipFree(dt);
+    }
+  }
+  if (precision_mask & ((1 << TJSQ_FP32) | (1 << TJSQ_BF16X3))) {
+    if (hipMalloc(&p->arena32, sizeof(float) * std::max<int64_t>(1, hp.arena_elems)) != hipSuccess)
+      return cleanup(TJSQ_ERR_NOMEM, "FP32 operator arena allocation failed");
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(hp.arena_elems)), dim3(256), 0, st, p->arena, p->arena32,
+                       hp.arena_elems);
+    HIPCHK(hipGetLastError());
+    if (hipMalloc(&p->t32, sizeof(float) * std::max<int64_t>(1, hp.scratch_elems)) != hipSuccess)
+      return cleanup(TJSQ_ERR_NOMEM, "FP32 scratch allocation failed");
+  }
+  if (hipMalloc(&p->t64, sizeof(double) * std::max<int64_t>(1, hp.scratch_elems)) != hipSuccess)
+    return cleanup(TJSQ_ERR_NOMEM, "scratch allocation failed");
+  auto put = [&](auto** dptr, const auto& vec) -> int {
+    using E = typename std::remove_reference<decltype(vec)>::type::value_type;
+    if (hipMalloc((void**)dptr, sizeof(E) * std::max<size_t>(1, vec.size())) != hipSuccess) return TJSQ_ERR_NOMEM;
+    if (!vec.empty() && hipMemcpyAsync(*dptr, vec.data(), sizeof(E) * vec.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+      return TJSQ_ERR_HIP;
+    return TJSQ_OK;
+  };
+  if (put(&p->tile1, hp.tile1) || put(&p->tile2, hp.tile2) || put(&p->seg1, hp.seg1) || put(&p->seg2, hp.seg2))
+    return cleanup(TJSQ_ERR_NOMEM, "task table allocation failed");
+  HIPCHK(hipStreamSynchronize(st));
+  *plan = p;
+  return TJSQ_OK;
+}
+
+struct tjsq_hostplan {
+  HostPlan hp;
+};
+
+int tjsq_hostplan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys,
+                         const tjsq_opset* env, const tjsq_basis* sys_bs, const tjsq_basis* env_bs, int nw,
+                         const int* wsec, tjsq_hostplan** out) {
+  if (!lat || !sys || !env || !sys_bs || !env_bs || !wsec || !out || nw < 1)
+    return fail(TJSQ_ERR_ARG, "tjsq_hostplan_create: null argument");
+  *out = nullptr;
+  tjsq_hostplan* h = new tjsq_hostplan();
+  std::string msg;
+  const int rc = tjsq::build_host_plan(*lat, sys_len, env_len, *sys, *env, *sys_bs, *env_bs, nw, wsec, TS, h->hp, msg);
+  if (rc != TJSQ_OK) {
+    delete h;
+    return fail(rc, "tjsq_hostplan_create: " + msg);
+  }
+  *out = h;
+  return TJSQ_OK;
+}
+
+int tjsq_hostplan_sizes(const tjsq_hostplan* h, int64_t* sizes) {
+  if (!h || !sizes) return fail(TJSQ_ERR_ARG, "tjsq_hostplan_sizes: null argument");
+  const HostPlan& hp = h->hp;
+  const int64_t v[] = {(int64_t)hp.seg1.size(), (int64_t)hp.tile1.size(), (int64_t)hp.seg2.size(),
+                       (int64_t)hp.tile2.size(), (int64_t)hp.merge.size(), hp.arena_elems, hp.scratch_elems, hp.dim,
+                       hp.sys_base, hp.env_base, hp.merged_base};
+  std::copy(v, v + 11, sizes);
+  return TJSQ_OK;
+}
+
+int tjsq_hostplan_tables(const tjsq_hostplan* h, void* seg1, void* tile1, void* seg2, void* tile2, void* merge) {
+  if (!h) return fail(TJSQ_ERR_ARG, "tjsq_hostplan_tables: null argument");
+  const HostPlan& hp = h->hp;
+  if (seg1) std::memcpy(seg1, hp.seg1.data(), sizeof(Seg) * hp.seg1.size());
+  if (tile1) std::memcpy(tile1, hp.tile1.data(), sizeof(Tile) * hp.tile1.size());
+  if (seg2) std::memcpy(seg2, hp.seg2.data(), sizeof(Seg) * hp.seg2.size());
+  if (tile2) std::memcpy(tile2, hp.tile2.data(), sizeof(Tile) * hp.tile2.size());
+  if (merge) std::memcpy(merge, hp.merge.data(), sizeof(tjsq::MergeTerm) * hp.merge.size());
+  return TJSQ_OK;
+}
+
+int tjsq_hostplan_stats(const tjsq_hostplan* h, tjsq_plan_stats* stats) {
+  if (!h || !stats) return fail(TJSQ_ERR_ARG, "tjsq_hostplan_stats: null argument");
+  *stats = h->hp.stats;
+  return TJSQ_OK;
+}
+
+int tjsq_hostplan_destroy(tjsq_hostplan* h) {
+  delete h;
+  return TJSQ_OK;
+}
+
+int tjsq_plan_stats_get(const tjsq_plan* plan, tjsq_plan_stats* stats) {
+  if (!plan || !stats) return fail(TJSQ_ERR_ARG, "tjsq_plan_stats_get: null argument");
+  *stats = plan->hp.stats;
+  return TJSQ_OK;
+}
+
+int tjsq_plan_destroy(tjsq_plan* p) {
+  if (!p) return TJSQ_OK;
+  (void)hipFree(p->arena);
+  (void)hipFree(p->arena32);
+  (void)hipFree(p->t64);
+  (void)hipFree(p->t32);
+  (void)hipFree(p->tile1);
+  (void)hipFree(p->tile2);
+  (void)hipFree(p->seg1);
+  (void)hipFree(p->seg2);
+  (void)hipFree(p->kry);
+  (void)hipFree(p->dwork);
+  (void)hipFree(p->x32);
+  (void)hipFree(p->x64a);
+  (void)hipFree(p->x64b);
+  delete p;
+  return TJSQ_OK;
+}
+
+int tjsq_plan_profile(tjsq_plan* p, int enable) {
+  if (!p) return fail(TJSQ_ERR_ARG, "tjsq_plan_profile: null plan");
+  p->profiling = enable != 0;
+  return TJSQ_OK;
+}
+
+int tjsq_plan_profile_read(tjsq_plan* p, double* ms, int64_t* launches) {
+  if (!p || !ms || !launches) return fail(TJSQ_ERR_ARG, "tjsq_plan_profile_read: null argument");
+  ms[0] = ms[1] = 0.0;
+  launches[0] = launches[1] = 0;
+  for (auto& r : p->prof) {
+    HIPCHK(hipEventSynchronize(r.second.second));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, r.second.first, r.second.second));
+    ms[r.first - 1] += t;
+    launches[r.first - 1] += 1;
+    (void)hipEventDestroy(r.second.first);
+    (void)hipEventDestroy(r.second.second);
+  }
+  p->prof.clear();
+  return TJSQ_OK;
+}
+
+int tjsq_heff_apply(tjsq_plan* p, const double* x, double* y, void* stream) {
+  if (!p || !x || !y) return fail(TJSQ_ERR_ARG, "tjsq_heff_apply: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = launch_tiles<double>(p, 1, p->arena, x, p->t64, y, TJSQ_FP64, st);
+  if (rc) return rc;
+  return launch_tiles<double>(p, 2, p->arena, x, p->t64, y, TJSQ_FP64, st);
+}
+
+int tjsq_heff_apply_f32(tjsq_plan* p, int precision, const float* x, float* y, void* stream) {
+  if (!p || !x || !y) return fail(TJSQ_ERR_ARG, "tjsq_heff_apply_f32: null argument");
+  if (precision != TJSQ_FP32 && precision != TJSQ_BF16X3) return fail(TJSQ_ERR_ARG, "precision must be FP32 or BF16X3");
+  if (!p->arena32) return fail(TJSQ_ERR_ARG, "plan was created without FP32 operands");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = launch_tiles<float>(p, 1, p->arena32, x, p->t32, y, precision, st);
+  if (rc) return rc;
+  return launch_tiles<float>(p, 2, p->arena32, x, p->t32, y, precision, st);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Lanczos (lanczos_memo_new, eigen.f90:2-175, levels = 1)
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+struct Krylov {
+  tjsq_plan* p;
+  hipStream_t st;
+  int64_t n;
+  T* V;              // (max_iter + 1) x n
+  double* partial;   // 2 * RED_BLOCKS * nv
+  double* dots;      // device results (2 * (max_iter + 1))
+  double* coef;      // device coefficients
+  std::vector<double> hbuf;
+
+  T* vec(int j) { return V + (int64_t)j * n; }
+
+  // host <- [V_j . w for j < nv] (+ [V_j . V_j] when self)
+  int dots_with(int nv, const T* w, bool self, std::vector<double>& out, const T* Vbase = nullptr) {
+    const T* base = Vbase ? Vbase : V;
+    hipLaunchKernelGGL(k_multidot<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, base, n, nv, w, n, partial,
+                       self ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_finish_dots, dim3((nv + 63) / 64), dim3(64), 0, st, partial, nv, RED_BLOCKS, self ? 1 : 0,
+                       dots);
+    HIPCHK(hipGetLastError());
+    out.resize(self ? 2 * nv : nv);
+    HIPCHK(hipMemcpyAsync(out.data(), dots, sizeof(double) * out.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return TJSQ_OK;
+  }
+  // w = (w - sum_j c_j V_{first+j}) * s
+  int combine(T* w, int nv, const std::vector<double>& c, double s, bool into_zero = false, int first = 0) {
+    if (nv > 0) HIPCHK(hipMemcpyAsync(coef, c.data(), sizeof(double) * nv, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_combine<T>, dim3(grid_for(n)), dim3(256), 0, st, w, V + (int64_t)first * n, n, nv, coef, n,
+                       s, into_zero ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    return TJSQ_OK;
+  }
+  int scale(T* w, double s) {
+    hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(n)), dim3(256), 0, st, w, n, s);
+    HIPCHK(hipGetLastError());
+    return TJSQ_OK;
+  }
+  int norm(const T* w, double& out) {
+    std::vector<double> d;
+    int rc = dots_with(1, w, false, d, w);
+    if (rc) return rc;
+    out = std::sqrt(d[0]);
+    return TJSQ_OK;
+  }
+};
+
+int apply_prec(tjsq_plan* p, int precision, const void* x, void* y, hipStream_t st) {
+  if (precision == TJSQ_FP64) return tjsq_heff_apply(p, (const double*)x, (double*)y, st);
+  return tjsq_heff_apply_f32(p, precision, (const float*)x, (float*)y, st);
+}
+
+// One Lanczos pass from v0 (already in slot 0, normalised).  Returns the
+// number of vectors, the Ritz value and the tridiagonal data.
+template <typename T>
+int lanczos_pass(Krylov<T>& K, int precision, double tol, int max_iter, int& last_out, double& ritz_out,
+                 std::vector<double>& alpha, std::vector<double>& beta, int& breakdown, int& applies) {
+  alpha.assign(max_iter + 1, 0.0);
+  beta.assign(max_iter + 1, 0.0);
+  breakdown = 0;
+  int last = 1;  // number of basis vectors (1-based as in eigen.f90)
+  int rc = apply_prec(K.p, precision, K.vec(0), K.vec(1), K.st);
+  if (rc) return rc;
+  ++applies;
+  std::vector<double> d;
+  if ((rc = K.dots_with(1, K.vec(1), false, d, K.vec(0)))) return rc;
+  alpha[1] = d[0];
+  double ritz = alpha[1];
+  while (last < max_iter) {
+    // r = H v_last - alpha_last v_last - beta_{last-1} v_{last-1}   (eigen.f90:63-72)
+    T* w = K.vec(last);
+    if (last < 2) {
+      std::vector<double> c = {alpha[last]};
+      if ((rc = K.combine(w, 1, c, 1.0))) return rc;
+    } else {
+      std::vector<double> c = {beta[last - 1], alpha[last]};  // slots last-2, last-1
+      if ((rc = K.combine(w, 2, c, 1.0, false, last - 2))) return rc;
+    }
+    double b;
+    if ((rc = K.norm(w, b))) return rc;
+    beta[last] = b;
+    if (std::fabs(b) <= tol) {  // eigen.f90:74-77
+      breakdown = 1;
+      break;
+    }
+    if ((rc = K.scale(w, 1.0 / b))) return rc;
+    // classical Gram-Schmidt against v_1..v_last (block.f90:2515-2551)
+    std::vector<double> uv;
+    if ((rc = K.dots_with(last, w, true, uv))) return rc;
+    std::vector<double> c(last);
+    for (int j = 0; j < last; ++j) c[j] = uv[j] / uv[last + j];
+    if ((rc = K.combine(w, last, c, 1.0))) return rc;
+    double nb;
+    if ((rc = K.norm(w, nb))) return rc;
+    if ((rc = K.scale(w, 1.0 / nb))) return rc;
+    ++last;
+    if ((rc = apply_prec(K.p, precision, K.vec(last - 1), K.vec(last), K.st))) return rc;
+    ++applies;
+    if ((rc = K.dots_with(1, K.vec(last), false, d, K.vec(last - 1)))) return rc;
+    alpha[last] = d[0];
+    std::vector<double> dia(alpha.begin() + 1, alpha.begin() + last + 1);
+    std::vector<double> off(beta.begin() + 1, beta.begin() + last + 1);
+    tridiag_eig(last, dia, off, nullptr);
+    const bool done = std::fabs(dia[0] - ritz) <= tol || last == max_iter;
+    ritz = dia[0];
+    if (done) break;
+  }
+  last_out = last;
+  ritz_out = ritz;
+  return TJSQ_OK;
+}
+
+// psi = normalise(sum_i z_i v_i) into out (slot-independent buffer)
+template <typename T>
+int assemble(Krylov<T>& K, int last, const std::vector<double>& alpha, const std::vector<double>& beta, T* out,
+             double& e0) {
+  std::vector<double> dia(alpha.begin() + 1, alpha.begin() + last + 1);
+  std::vector<double> off(beta.begin() + 1, beta.begin() + last + 1);
+  std::vector<double> z;
+  tridiag_eig(last, dia, off, &z);
+  e0 = dia[0];
+  std::vector<double> c(last);
+  for (int i = 0; i < last; ++i) c[i] = -z[i];  // combine subtracts
+  int rc = K.combine(out, last, c, 1.0, true);
+  if (rc) return rc;
+  double nrm;
+  if ((rc = K.norm(out, nrm))) return rc;
+  return K.scale(out, 1.0 / nrm);
+}
+
+template <typename T>
+int ensure_krylov(tjsq_plan* p, int max_iter, Krylov<T>& K, hipStream_t st) {
+  const int64_t n = p->hp.dim;
+  const int64_t need = (int64_t)sizeof(T) * n * (max_iter + 2);
+  if (p->kry_bytes < need) {
+    (void)hipFree(p->kry);
+    p->kry = nullptr;
+    p->kry_bytes = 0;
+    if (hipMalloc(&p->kry, need) != hipSuccess) return fail(TJSQ_ERR_NOMEM, "Krylov basis allocation failed");
+    p->kry_bytes = need;
+  }
+  if (!p->dwork) {
+    if (hipMalloc(&p->dwork, sizeof(double) * (2 * RED_BLOCKS * 256 + 1024)) != hipSuccess)
+      return fail(TJSQ_ERR_NOMEM, "workspace allocation failed");
+  }
+  K.p = p;
+  K.st = st;
+  K.n = n;
+  K.V = (T*)p->kry;
+  K.partial = p->dwork;
+  K.dots = p->dwork + 2 * RED_BLOCKS * 256;
+  K.coef = K.dots + 512;
+  return TJSQ_OK;
+}
+
+}  // namespace
+
+extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_opts* opts,
+                                  tjsq_lanczos_result* res, void* stream) {
+  if (!p || !psi || !opts || !res) return fail(TJSQ_ERR_ARG, "tjsq_lanczos_solve: null argument");
+  if (opts->max_iter < 2 || opts->max_iter > 250) return fail(TJSQ_ERR_ARG, "max_iter must be in [2, 250]");
+  if (opts->precision != TJSQ_FP64 && !p->arena32) return fail(TJSQ_ERR_ARG, "plan has no FP32 operands");
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t st = (hipStream_t)stream;
+  std::memset(res, 0, sizeof(*res));
+  const int64_t n = p->hp.dim;
+  int rc;
+  std::vector<double> alpha, beta;
+  int last = 0, brk = 0, applies = 0;
+  double ritz = 0.0, e0 = 0.0;
+  if (!p->x64a) {
+    if (hipMalloc(&p->x64a, sizeof(double) * n) != hipSuccess || hipMalloc(&p->x64b, sizeof(double) * n) != hipSuccess)
+      return fail(TJSQ_ERR_NOMEM, "residual buffers allocation failed");
+  }
+  if (opts->precision == TJSQ_FP64) {
+    Krylov<double> K;
+    if ((rc = ensure_krylov(p, opts->max_iter, K, st))) return rc;
+    HIPCHK(hipMemcpyAsync(K.vec(0), psi, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    double nrm;
+    if ((rc = K.norm(K.vec(0), nrm))) return rc;
+    if ((rc = K.scale(K.vec(0), 1.0 / nrm))) return rc;
+    if ((rc = lanczos_pass(K, TJSQ_FP64, opts->tol, opts->max_iter, last, ritz, alpha, beta, brk, applies))) return rc;
+    if ((rc = assemble(K, last, alpha, beta, psi, e0))) return rc;
+    res->e0 = e0;
+    res->e_ritz = e0;
+    res->iterations = last;
+  } else {
+    Krylov<float> K;
+    if ((rc = ensure_krylov(p, opts->max_iter, K, st))) return rc;
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(n)), dim3(256), 0, st, psi, K.vec(0), n);
+    double nrm;
+    if ((rc = K.norm(K.vec(0), nrm))) return rc;
+    if ((rc = K.scale(K.vec(0), 1.0 / nrm))) return rc;
+    if ((rc = lanczos_pass(K, opts->precision, opts->tol, opts->max_iter, last, ritz, alpha, beta, brk, applies)))
+      return rc;
+    float* out32 = K.vec(opts->max_iter + 1);
+    if ((rc = assemble(K, last, alpha, beta, out32, e0))) return rc;
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(grid_for(n)), dim3(256), 0, st, out32, psi, n);
+    res->e_ritz = e0;
+    res->iterations = last;
+  }
+  // FP64 Rayleigh quotient and residual |H psi - E psi| (and refinement)
+  int refinements = 0;
+  for (;;) {
+    Krylov<double> K2;
+    K2.p = p;
+    K2.st = st;
+    K2.n = n;
+    K2.V = psi;
+    K2.partial = p->dwork;
+    K2.dots = p->dwork + 2 * RED_BLOCKS * 256;
+    K2.coef = K2.dots + 512;
+    if ((rc = tjsq_heff_apply(p, psi, p->x64a, st))) return rc;
+    ++applies;
+    std::vector<double> d;
+    if ((rc = K2.dots_with(1, p->x64a, true, d))) return rc;  // <psi,Hpsi>, <psi,psi>
+    const double rq = d[0] / d[1];
+    std::vector<double> c = {rq};
+    if ((rc = K2.combine(p->x64a, 1, c, 1.0))) return rc;    // H psi - rq psi
+    double r;
+    if ((rc = K2.norm(p->x64a, r))) return rc;
+    res->residual = r / std::sqrt(d[1]);
+    if (opts->precision != TJSQ_FP64) res->e0 = rq;
+    if (opts->precision == TJSQ_FP64 || res->residual <= opts->refine_tol || refinements >= opts->max_refine) break;
+    // refinement restart: FP64 Lanczos from the current psi
+    Krylov<double> K;
+    if ((rc = ensure_krylov(p, opts->max_iter, K, st))) return rc;
+    HIPCHK(hipMemcpyAsync(K.vec(0), psi, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    double nrm;
+    if ((rc = K.norm(K.vec(0), nrm))) return rc;
+    if ((rc = K.scale(K.vec(0), 1.0 / nrm))) return rc;
+    int last2 = 0, brk2 = 0;
+    if ((rc = lanczos_pass(K, TJSQ_FP64, opts->tol, opts->max_iter, last2, ritz, alpha, beta, brk2, applies))) return rc;
+    if ((rc = assemble(K, last2, alpha, beta, psi, e0))) return rc;
+    res->iterations += last2;
+    brk |= brk2;
+    ++refinements;
+    res->e0 = e0;
+    res->e_ritz = e0;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  res->applies = applies;
+  res->refinements = refinements;
+  res->breakdown = brk;
+  res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return TJSQ_OK;
+}

@@ -1,0 +1,217 @@
+/*
+ * tjsq_capi.h -- C-ABI of the MI355X tjsq Lanczos / H_eff library
+ * (libtjsq_hip.so, built from grand_sql_tj_amd/csrc).
+ *
+ * The library replaces the Lanczos step of the reference tjsq SU(2) two-site
+ * DMRG code (DongNingSheng/Grand_SQL_tJ):
+ *
+ *   lanczos_memo_new(sys,env,sys_bs,env_bs,super,eigvec,levels,num_idx)
+ *                                          code/tjsq/eigen.f90:2-10
+ *   hmerge(sys,env,sys_bs,env_bs)          code/tjsq/get_sys.f90:4383-4387
+ *   model_wave(sys,env,sys_bs,env_bs,invec,outvec)
+ *                                          code/tjsq/get_sys.f90:4127-4134
+ *
+ * and the fusion-table routines they rely on
+ *
+ *   Get_basis(block,site,basis)            code/tjsq/block.f90:9-104
+ *   Get_super_basis(sys_bs,env_bs,super,total_up,total_down)
+ *                                          code/tjsq/block.f90:110-161
+ *   W6JS(j1,j2,j3,l1,l2,l3)                code/tjsq/block.f90:1564-1615
+ *
+ * The reference's derived types are not C-interoperable; every table crosses
+ * this boundary flattened (layout below).  There are no hidden globals: the
+ * lattice couplings that the reference keeps in module pubdata
+ * (code/tjsq/common.f90:35-45) arrive in tjsq_lattice.  Every call returns an
+ * int status (TJSQ_OK = 0); the reference stops the program instead
+ * (get_sys.f90:2042-2045 and similar), the Fortran shim turns a nonzero
+ * status into `error stop`.
+ *
+ * Flat layout (all integer tables int32, column-major matrices):
+ *   operator set (Total_Model, common.f90:139-149): nops operators; op 0 is
+ *     the block Hamiltonian, op 1+4*(i-1)+q is active site i's elec_up (q=0),
+ *     elec_down (1), spin_sd (2), num_sn (3).
+ *     hdr[4*op]  = {len, up_dif, down_dif, nsub}
+ *     sub[5*s]   = {num_up, num_down, down_dif, row_dim, sdim}  (ket QNs)
+ *     off[s]     = element offset of sub-block s inside data
+ *   basis (Total_Basis, common.f90:65-82):
+ *     sec[4*k]   = {new_num_up, new_num_down, dim, nsub}
+ *     sub[6*q]   = {bl_num_up, bl_num_down, st_num_up, st_num_down, spos, sdim}
+ *   wavefunction sectors (Super_Basis, common.f90:88-97):
+ *     wsec[6*i]  = {sys_num_up, sys_num_down, env_num_up, env_num_down,
+ *                   sys_dim, env_dim}; a vector is the sectors back to back,
+ *     each column-major sys_dim x env_dim.
+ *
+ * Threading: a plan is used by one host thread and one stream at a time;
+ * distinct plans are independent.
+ */
+#ifndef TJSQ_CAPI_H
+#define TJSQ_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TJSQ_OK 0
+#define TJSQ_ERR_ARG (-1)     /* invalid argument or inconsistent tables */
+#define TJSQ_ERR_QN (-2)      /* quantum-number mismatch between operators */
+#define TJSQ_ERR_HIP (-3)     /* HIP runtime error */
+#define TJSQ_ERR_NOMEM (-4)   /* device or host allocation failed */
+#define TJSQ_ERR_NODEV (-5)   /* no usable GPU */
+#define TJSQ_ERR_CAPACITY (-6)/* request exceeds admitted memory */
+
+#define TJSQ_FP64 0           /* FP64 MFMA contractions */
+#define TJSQ_FP32 1           /* FP32 MFMA contractions, FP32 accumulation */
+#define TJSQ_BF16X3 2         /* BF16 hi/lo split, 3 MFMA products, FP32 acc. */
+
+/* Lattice and couplings in the reference's conventions (main.f90:35-47):
+ * jt = -sqrt(2) t, jd = -sqrt(3) J, jn = -J/4 for nearest-neighbour bonds
+ * (index 0) and next-nearest bonds (index 1); esite = -mu on every site.
+ * Square cylinder, periodic in y (ny = circumference), open in x; the
+ * active-site tables (dmrg_setup0) are built over nx_tables columns. */
+typedef struct tjsq_lattice {
+  int ny;
+  int nx_tables;
+  double jt[2];
+  double jd[2];
+  double jn[2];
+  double esite;
+  int up_bias;   /* 1: sectors carry (N, 2S); 0: 2S only (reference default) */
+} tjsq_lattice;
+
+typedef struct tjsq_opset {
+  int length;            /* block length (Total_Model%len) */
+  int nops;
+  const int* hdr;
+  const int* sub;
+  const int64_t* off;
+  const double* data;    /* host or device pointer, see data_on_device */
+  int64_t ndata;
+  int data_on_device;    /* 1: data is a device pointer (used in place) */
+} tjsq_opset;
+
+typedef struct tjsq_basis {
+  int length;
+  int nsec;
+  const int* sec;
+  const int* sub;
+} tjsq_basis;
+
+typedef struct tjsq_plan tjsq_plan;
+
+typedef struct tjsq_plan_stats {
+  int64_t dim;              /* D, length of the wavefunction */
+  int64_t n_contrib;        /* GEMM-chain / GEMM / axpy contributions */
+  int64_t n_chain;          /* two-sided contributions (A.X.B) */
+  int64_t n_stage1;         /* distinct A.X products of stage 1 */
+  int64_t n_out_blocks;     /* output sub-blocks */
+  int64_t n_tiles1, n_tiles2;
+  double flops_ref;         /* algorithmic flops per apply, reference order */
+  double flops_exec;        /* flops the two stages execute */
+  double bytes_fp64;        /* compulsory bytes per apply at 8 B/element */
+  int64_t op_elems;         /* operator elements referenced */
+  int64_t scratch_elems;    /* stage-1 scratch elements */
+} tjsq_plan_stats;
+
+typedef struct tjsq_lanczos_opts {
+  double tol;          /* |E0(k) - E0(k-1)| stop (lanzero, eigen.f90:98) */
+  int max_iter;        /* loop_in (eigen.f90:12) */
+  int precision;       /* TJSQ_FP64 / TJSQ_FP32 / TJSQ_BF16X3 */
+  double refine_tol;   /* FP64 residual |H psi - E psi| above which a mixed
+                          solve restarts in FP64 from its own result */
+  int max_refine;      /* FP64 restarts allowed after a mixed solve */
+} tjsq_lanczos_opts;
+
+typedef struct tjsq_lanczos_result {
+  double e0;           /* ground-state energy (FP64 Rayleigh quotient for
+                          mixed precision, Ritz value for FP64) */
+  double e_ritz;       /* last Ritz value of the tridiagonal problem */
+  double residual;     /* FP64 |H psi - e0 psi| (0 if not computed) */
+  int iterations;      /* Lanczos vectors built, all passes */
+  int applies;         /* H_eff applies, all precisions */
+  int refinements;     /* FP64 restarts taken */
+  int breakdown;       /* 1 if beta <= tol ended a pass (eigen.f90:74-77) */
+  double seconds;      /* wall time of the solve */
+} tjsq_lanczos_result;
+
+/* ---- host tables, bit-exact with the reference ------------------------ */
+
+/* 6j symbol on doubled arguments (W6JS). */
+double tjsq_w6js(int j1, int j2, int j3, int l1, int l2, int l3);
+
+/* Get_basis: fuse a block with a site.  blk[3*i] = {num_up, num_down, dim}
+ * of the block's sectors in order, site[3*j] likewise.  sec must hold
+ * 3*nb*nsite*4 ints and sub 3*nb*nsite*6 ints. */
+int tjsq_get_basis(int block_len, int nb, const int* blk, int nsite, const int* site,
+                   int* sec, int* sub, int* nsec, int* nsub);
+
+/* Get_super_basis: wavefunction sectors for total (N, 2S).  wsec must hold
+ * 6*sys->nsec*env->nsec ints. */
+int tjsq_get_super_basis(const tjsq_basis* sys_bs, const tjsq_basis* env_bs, int total_up,
+                         int total_down, int* wsec, int* nw);
+
+/* Coupling structure of hmerge (coup_bs[4] = (1,1),(1,2),(2,1),(2,2),
+ * coup_bb[2], inb[0..sys_len+env_len+2]). */
+int tjsq_hmerge_tables(const tjsq_lattice* lat, int sys_len, int env_len, int* coup_bs,
+                       int* coup_bb, int* inb);
+
+/* Active-site count nleg11(len) of dmrg_setup0. */
+int tjsq_active_sites(const tjsq_lattice* lat, int len);
+
+/* ---- plans and applies ------------------------------------------------ */
+
+/* Compile hmerge + model_wave for one superblock into a device plan:
+ * uploads (or adopts) the operator data, builds the merged operators on the
+ * device and the grouped-GEMM task tables.  precision_mask selects which
+ * operand copies to prepare (bit p for TJSQ_FP64/FP32/BF16X3); FP64 is always
+ * prepared.  stream may be NULL (default stream). */
+int tjsq_plan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys,
+                     const tjsq_opset* env, const tjsq_basis* sys_bs, const tjsq_basis* env_bs,
+                     int nw, const int* wsec, int precision_mask, void* stream, tjsq_plan** plan);
+
+int tjsq_plan_stats_get(const tjsq_plan* plan, tjsq_plan_stats* stats);
+int tjsq_plan_destroy(tjsq_plan* plan);
+
+/* y = H_eff x on device FP64 vectors of length D (model_wave). */
+int tjsq_heff_apply(tjsq_plan* plan, const double* x, double* y, void* stream);
+
+/* y = H_eff x on device FP32 vectors, contractions in TJSQ_FP32 or
+ * TJSQ_BF16X3. */
+int tjsq_heff_apply_f32(tjsq_plan* plan, int precision, const float* x, float* y, void* stream);
+
+/* Per-launch timing of the two apply stages with HIP events recorded on the
+ * launching stream.  profile_read returns the summed milliseconds and launch
+ * counts of stage 1 (ms[0]) and stage 2 (ms[1]) since the last read. */
+int tjsq_plan_profile(tjsq_plan* plan, int enable);
+int tjsq_plan_profile_read(tjsq_plan* plan, double* ms, int64_t* launches);
+
+/* Ground state (lanczos_memo_new, levels = 1): psi (device, FP64, length D)
+ * holds the start vector on entry and the normalised ground state on exit. */
+int tjsq_lanczos_solve(tjsq_plan* plan, double* psi, const tjsq_lanczos_opts* opts,
+                       tjsq_lanczos_result* result, void* stream);
+
+/* ---- host-only plan inspection (no GPU needed) --------------------------
+ * The task tables a plan would upload, for tests of the plan compiler.
+ * sizes[11] = {n_seg1, n_tile1, n_seg2, n_tile2, n_merge, arena_elems,
+ * scratch_elems, dim, sys_base, env_base, merged_base}.  Table element
+ * layouts: segment 40 bytes {int64 l_off, r_off; int32 l_ld, r_ld, K;
+ * uint8 l_space, r_space, l_t, r_t; double c}; tile 40 bytes {int64 out_off;
+ * int32 out_ld, m0, n0, mt, nt, seg_begin, seg_end, out_space}; merge term
+ * 32 bytes {int64 dst_off, src_off, n; double coef}. */
+typedef struct tjsq_hostplan tjsq_hostplan;
+int tjsq_hostplan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys,
+                         const tjsq_opset* env, const tjsq_basis* sys_bs, const tjsq_basis* env_bs, int nw,
+                         const int* wsec, tjsq_hostplan** out);
+int tjsq_hostplan_sizes(const tjsq_hostplan* h, int64_t* sizes);
+int tjsq_hostplan_tables(const tjsq_hostplan* h, void* seg1, void* tile1, void* seg2, void* tile2, void* merge);
+int tjsq_hostplan_stats(const tjsq_hostplan* h, tjsq_plan_stats* stats);
+int tjsq_hostplan_destroy(tjsq_hostplan* h);
+
+/* Last error message of the calling thread (static storage). */
+const char* tjsq_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -84,6 +84,11 @@ def ref_lib():
             lib.refb_model_wave.argtypes = common + [F64P, F64P]
             lib.refb_lanczos.restype = C.c_int
             lib.refb_lanczos.argtypes = common + [F64P, C.POINTER(C.c_double)]
+            lib.refb_w6js.restype = C.c_double
+            lib.refb_w6js.argtypes = [C.c_int] * 6
+            lib.refb_get_basis.restype = C.c_int
+            lib.refb_get_basis.argtypes = [C.c_int, C.c_int, I32P, C.c_int, I32P, I32P, I32P, C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int)]
             _libs["ref"] = lib
         return _libs["ref"]
 
@@ -155,3 +160,42 @@ def ref_lanczos(model, sb, x0: np.ndarray, lanzero: float = 1e-10):
         lib.refb_lanczos(sb.sys_len, sb.env_len, *_m(sb.sys), *_m(sb.env), *_bs(sb.sys_bs), *_bs(sb.env_bs),
                          int(sb.wsec.shape[0]), np.ascontiguousarray(sb.wsec), x, C.byref(e0))
     return e0.value, x
+
+
+_ref_ready = False
+
+
+def ref_w6js(*j) -> float:
+    """W6JS of the reference build."""
+    global _ref_ready
+    with _lock:
+        lib = ref_lib()
+        if not _ref_ready:
+            lib.refb_init(6, 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1e-10)
+            _ref_ready = True
+        return lib.refb_w6js(*[int(v) for v in j])
+
+
+def ref_get_basis(block_len, blk, site):
+    """Get_basis of the reference build: (sec, sub) tables."""
+    blk = np.ascontiguousarray(blk, dtype=np.int32)
+    site = np.ascontiguousarray(site, dtype=np.int32)
+    cap = 3 * max(1, blk.shape[0]) * max(1, site.shape[0])
+    sec = np.zeros(4 * cap, np.int32)
+    sub = np.zeros(6 * cap, np.int32)
+    ns, nq = C.c_int(0), C.c_int(0)
+    with _lock:
+        ref_lib().refb_get_basis(block_len, blk.shape[0], blk, site.shape[0], site, sec, sub, C.byref(ns),
+                                 C.byref(nq))
+    return sec[:4 * ns.value].reshape(-1, 4), sub[:6 * nq.value].reshape(-1, 6)
+
+
+def oracle_get_basis(block_len, blk, site):
+    blk = np.ascontiguousarray(blk, dtype=np.int32)
+    site = np.ascontiguousarray(site, dtype=np.int32)
+    cap = 3 * max(1, blk.shape[0]) * max(1, site.shape[0])
+    sec = np.zeros(4 * cap, np.int32)
+    sub = np.zeros(6 * cap, np.int32)
+    nq = C.c_int(0)
+    ns = oracle_lib().oracle_get_basis(block_len, blk.shape[0], blk, site.shape[0], site, sec, sub, C.byref(nq))
+    return sec[:4 * ns].reshape(-1, 4), sub[:6 * nq.value].reshape(-1, 6)

@@ -240,6 +240,7 @@ integer(c_int) function refb_init(ny_, ub, t_nn, t_nnn, d_nn, d_nnn, n_nn, n_nnn
   implicit none
   integer(c_int), value :: ny_, ub
   real(c_double), value :: t_nn, t_nnn, d_nn, d_nnn, n_nn, n_nnn, esite_, lanzero_
+  logical, save :: caches_ready = .false.
   refb_init = 0
   if (ny_ /= Ny) then
     refb_init = -1
@@ -261,8 +262,13 @@ integer(c_int) function refb_init(ny_, ub, t_nn, t_nnn, d_nn, d_nnn, n_nn, n_nnn
   call dmrg_setup0()
   call Get_Lattice
   call make_site_ops(ub)
-  iw6j1 = 0; w6j1 = 0.d0; iw6j2 = 0; w6j2 = 0.d0; iw6j3 = 0; w6j3 = 0.d0
-  call factrl
+  ! the 6j memo tables (common.f90:16-21) hold values of a pure function of
+  ! their keys, so they are cleared once and kept across calls
+  if (.not. caches_ready) then
+    iw6j1 = 0; w6j1 = 0.d0; iw6j2 = 0; w6j2 = 0.d0; iw6j3 = 0; w6j3 = 0.d0
+    call factrl
+    caches_ready = .true.
+  end if
   cone = 1.0d0
   czero = 0.0d0
 end function refb_init
@@ -347,3 +353,56 @@ integer(c_int) function refb_lanczos(sys_len, env_len, sys_nops, sys_hdr, sys_su
   call free_model(env)
   refb_lanczos = 0
 end function refb_lanczos
+
+! W6JS of the reference (block.f90:1564-1615); refb_init must have run FACTRL.
+real(c_double) function refb_w6js(j1, j2, j3, l1, l2, l3) bind(C, name="refb_w6js")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: j1, j2, j3, l1, l2, l3
+  real(8), external :: w6js
+  refb_w6js = w6js(j1, j2, j3, l1, l2, l3)
+end function refb_w6js
+
+! Get_basis of the reference (block.f90:9-104) for a block given by its
+! sector list blk(3, nb) = (num_up, num_down, dim) and the site st_sz-like
+! list site(3, ns).  Returns the counts; sec(4, *) and sub(6, *) receive the
+! tables in the flat layout.
+integer(c_int) function refb_get_basis(blen, nb, blk, ns, site, sec, sub, nsec, nsub) bind(C, name="refb_get_basis")
+  use iso_c_binding
+  use pubdata
+  implicit none
+  integer(c_int), value :: blen, nb, ns
+  integer(c_int), intent(in) :: blk(3, *), site(3, *)
+  integer(c_int), intent(inout) :: sec(4, *), sub(6, *)
+  integer(c_int), intent(out) :: nsec, nsub
+  type(Total_Block) :: block, st
+  type(Total_Basis) :: basis
+  integer :: i, j, q
+  block%len = blen; block%num = nb; block%up_dif = 0; block%down_dif = 0
+  allocate(block%sub(nb))
+  do i = 1, nb
+    block%sub(i)%num_up = blk(1, i); block%sub(i)%num_down = blk(2, i)
+    block%sub(i)%down_dif = 0; block%sub(i)%row_dim = blk(3, i); block%sub(i)%sdim = blk(3, i)
+  end do
+  st%len = 1; st%num = ns; st%up_dif = 0; st%down_dif = 0
+  allocate(st%sub(ns))
+  do i = 1, ns
+    st%sub(i)%num_up = site(1, i); st%sub(i)%num_down = site(2, i)
+    st%sub(i)%down_dif = 0; st%sub(i)%row_dim = site(3, i); st%sub(i)%sdim = site(3, i)
+  end do
+  call Get_basis(block, st, basis)
+  nsec = basis%num
+  q = 0
+  do i = 1, basis%num
+    sec(1, i) = basis%sub(i)%new_num_up; sec(2, i) = basis%sub(i)%new_num_down
+    sec(3, i) = basis%sub(i)%dim; sec(4, i) = basis%sub(i)%num
+    do j = 1, basis%sub(i)%num
+      q = q + 1
+      sub(1, q) = basis%sub(i)%sub(j)%bl_num_up; sub(2, q) = basis%sub(i)%sub(j)%bl_num_down
+      sub(3, q) = basis%sub(i)%sub(j)%st_num_up; sub(4, q) = basis%sub(i)%sub(j)%st_num_down
+      sub(5, q) = basis%sub(i)%sub(j)%spos; sub(6, q) = basis%sub(i)%sub(j)%sdim
+    end do
+  end do
+  nsub = q
+  refb_get_basis = 0
+end function refb_get_basis

@@ -1,0 +1,51 @@
+"""numpy evaluation of the task tables a plan uploads (test helper).
+
+Executes merge recipes, stage 1 and stage 2 exactly as the device kernels
+define them, so the plan compiler can be checked on the CPU against the
+oracle.  This is a checker of the tables, not a product path.
+"""
+import numpy as np
+
+SP_OPS, SP_X, SP_T, SP_IDENT = 0, 1, 2, 3
+
+
+def _mat(buf, off, ld, rows, cols, trans):
+    """op(stored) restricted to rows x cols: stored column-major with ld."""
+    if not trans:
+        idx = off + np.arange(rows)[:, None] + np.arange(cols)[None, :] * ld
+    else:
+        idx = off + np.arange(rows)[:, None] * ld + np.arange(cols)[None, :]
+    return buf[idx]
+
+
+def evaluate(hp, sb, x):
+    arena = np.zeros(hp.arena_elems)
+    arena[hp.sys_base:hp.sys_base + sb.sys.data.size] = sb.sys.data
+    arena[hp.env_base:hp.env_base + sb.env.data.size] = sb.env.data
+    for m in hp.merge:   # recipes accumulate in table order per destination
+        d, s, n = int(m["dst_off"]), int(m["src_off"]), int(m["n"])
+        arena[d:d + n] = arena[d:d + n] + m["coef"] * arena[s:s + n]
+    T = np.zeros(max(1, hp.scratch_elems))
+    y = np.zeros(hp.dim)
+    spaces = {SP_OPS: arena, SP_X: x, SP_T: T}
+    for stage, tiles, segs, out in ((1, hp.tile1, hp.seg1, T), (2, hp.tile2, hp.seg2, y)):
+        for tl in tiles:
+            m0, n0, mt, nt = int(tl["m0"]), int(tl["n0"]), int(tl["mt"]), int(tl["nt"])
+            acc = np.zeros((mt, nt))
+            for s in segs[int(tl["seg_begin"]):int(tl["seg_end"])]:
+                L = spaces[int(s["l_space"])]
+                if int(s["r_space"]) == SP_IDENT:
+                    acc += s["c"] * _mat(L, int(s["l_off"]) + m0 + n0 * int(s["l_ld"]), int(s["l_ld"]), mt, nt, 0)
+                    continue
+                R = spaces[int(s["r_space"])]
+                K = int(s["K"])
+                lt, rt = int(s["l_t"]), int(s["r_t"])
+                lo = int(s["l_off"]) + (m0 * int(s["l_ld"]) if lt else m0)
+                ro = int(s["r_off"]) + (n0 if rt else n0 * int(s["r_ld"]))
+                Lm = _mat(L, lo, int(s["l_ld"]), mt, K, lt)
+                Rm = _mat(R, ro, int(s["r_ld"]), K, nt, rt)
+                acc += (s["c"] * Lm) @ Rm
+            o, ld = int(tl["out_off"]), int(tl["out_ld"])
+            idx = o + (m0 + np.arange(mt))[:, None] + (n0 + np.arange(nt))[None, :] * ld
+            out[idx] = acc
+    return y

@@ -1,0 +1,102 @@
+"""HIP path parity on the GPU: the C-ABI against the oracle and the reference."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import lanczos as olanczos
+from helpers import GEOMS, model, rel, small_superblock
+from grand_sql_tj_amd import capi, synth
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _cuda(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    return t if dtype is None else t.to(dtype)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    capi.lib()
+
+
+@pytest.mark.parametrize("ub", [1, 0])
+@pytest.mark.parametrize("geom", GEOMS)
+def test_apply_fp64_matches_oracle(ub, geom):
+    m = model(up_bias=ub)
+    sb = small_superblock(m, *geom, seed=7 * geom[0] + geom[1] + ub, msys=40, menv=44, median=5)
+    x = synth.random_vector(sb, 9)
+    p = capi.Plan(m, sb)
+    y = p.apply(_cuda(x)).cpu().numpy()
+    assert rel(y, oracle.model_wave(m, sb, x)) < 1e-13
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-6), ("bf16x3", 2e-5)])
+def test_apply_mixed_precision_tolerance(prec, tol):
+    m = model()
+    sb = small_superblock(m, 11, 11, seed=21, msys=120, menv=120, median=12, nsec=10)
+    x = synth.random_vector(sb, 9)
+    p = capi.Plan(m, sb)
+    y = p.apply(_cuda(x, torch.float32), precision=prec).double().cpu().numpy()
+    assert rel(y, oracle.model_wave(m, sb, x)) < tol
+
+
+def test_apply_larger_blocks_fp64():
+    # sub-blocks larger than one 32x32 tile and K larger than one LDS chunk
+    m = model()
+    sb = small_superblock(m, 11, 11, seed=23, msys=400, menv=380, median=40, nsec=8)
+    x = synth.random_vector(sb, 4)
+    p = capi.Plan(m, sb)
+    y = p.apply(_cuda(x)).cpu().numpy()
+    assert rel(y, oracle.model_wave(m, sb, x)) < 1e-13
+
+
+def test_apply_symmetric_and_deterministic():
+    m = model()
+    sb = small_superblock(m, 12, 10, seed=3, msys=80, menv=80, median=8)
+    p = capi.Plan(m, sb)
+    x, z = _cuda(synth.random_vector(sb, 1)), _cuda(synth.random_vector(sb, 2))
+    hx, hz = p.apply(x), p.apply(z)
+    assert abs(float(torch.dot(z, hx) - torch.dot(x, hz))) < 1e-12 * float(hx.norm() * z.norm())
+    assert torch.equal(p.apply(x), hx)   # pull-style apply: bitwise repeatable
+
+
+def test_lanczos_fp64_matches_reference_and_oracle(tmp_path):
+    m = model()
+    sb = small_superblock(m, 11, 11, seed=3 + 1, msys=30, menv=30, median=4)
+    x0 = synth.random_vector(sb, 1)
+    e_o, psi_o, k_o, _ = olanczos.lanczos(lambda v: oracle.model_wave(m, sb, v), x0, 1e-10, 110)
+    p = capi.Plan(m, sb)
+    psi = _cuda(x0)
+    r = p.lanczos(psi, tol=1e-10, max_iter=110, precision="fp64")
+    assert abs(r["e0"] - e_o) <= 1e-10 * abs(e_o)
+    assert abs(abs(float(np.dot(psi.cpu().numpy(), psi_o))) - 1.0) < 1e-8
+    if oracle.ref_available():
+        cwd = os.getcwd()
+        os.chdir(tmp_path)
+        try:
+            e_ref, _ = oracle.ref_lanczos(m, sb, x0, 1e-10)
+        finally:
+            os.chdir(cwd)
+        assert abs(r["e0"] - e_ref) <= 1e-10 * abs(e_ref)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_lanczos_mixed_precision_energy(prec):
+    m = model()
+    sb = small_superblock(m, 11, 11, seed=8, msys=120, menv=120, median=12, nsec=10)
+    x0 = synth.random_vector(sb, 5)
+    p = capi.Plan(m, sb)
+    ref = _cuda(x0)
+    r64 = p.lanczos(ref, tol=1e-10, max_iter=80, precision="fp64")
+    psi = _cuda(x0)
+    r = p.lanczos(psi, tol=1e-10, max_iter=80, precision=prec, refine_tol=1e-6, max_refine=2)
+    nsite = sb.sys_len + sb.env_len + 2
+    assert abs(r["e0"] - r64["e0"]) / nsite <= 1e-8
+    assert r["residual"] <= 1e-5

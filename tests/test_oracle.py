@@ -1,0 +1,177 @@
+"""The oracle pinned: against the reference build, closed forms and invariants."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import lanczos as olanczos
+from helpers import GEOMS, model, rel, small_superblock
+from grand_sql_tj_amd import blocks, synth
+
+needs_ref = pytest.mark.skipif(not oracle.ref_available(), reason="reference build (oracle/_ref) not present")
+
+
+# ---- 6j symbols -------------------------------------------------------------
+
+def test_w6j_closed_form_zero_argument():
+    # {a b 0; d e f} = delta_ab delta_de (-1)^(a+e+f) / sqrt((2a+1)(2e+1))  (doubled args)
+    for a in range(0, 9):
+        for e in range(0, 9):
+            for f in range(abs(a - e), a + e + 1, 2):
+                w = oracle.w6js(a, a, 0, e, e, f)
+                sign = -1.0 if ((a + e + f) // 2) % 2 else 1.0
+                assert w == pytest.approx(sign / math.sqrt((a + 1) * (e + 1)), abs=1e-15)
+
+
+def test_w6j_orthogonality():
+    # sum_x (2x+1)(2f+1) {a b x; c d f}{a b x; c d f'} = delta_ff'
+    for (a, b, c, d) in [(1, 2, 1, 2), (3, 2, 1, 4), (4, 4, 2, 2), (5, 3, 4, 2)]:
+        fs = [f for f in range(0, 12) if (a + d + f) % 2 == 0 and (c + b + f) % 2 == 0]
+        for f in fs:
+            for fp in fs:
+                s = sum((x + 1) * (f + 1) * oracle.w6js(a, b, x, c, d, f) * oracle.w6js(a, b, x, c, d, fp)
+                        for x in range(0, 14))
+                tri = abs(a - d) <= f <= a + d and abs(c - b) <= f <= c + b
+                want = 1.0 if (f == fp and tri) else 0.0
+                assert s == pytest.approx(want, abs=1e-13)
+
+
+@needs_ref
+def test_w6j_bit_exact_with_reference():
+    rng = np.random.default_rng(1)
+    for _ in range(2000):
+        j = rng.integers(0, 9, size=6)
+        assert oracle.w6js(*j) == oracle.ref_w6js(*j)
+
+
+# ---- fusion bases -------------------------------------------------------------
+
+def _random_blocks(rng, ub):
+    secs = []
+    for _ in range(rng.integers(1, 9)):
+        s2 = int(rng.integers(0, 7))
+        n = s2 + 2 * int(rng.integers(0, 3)) if ub else 0
+        secs.append((n, s2, int(rng.integers(1, 5))))
+    return np.array(sorted(set(secs), key=lambda t: (-t[1], -t[0])), dtype=np.int32)
+
+
+@needs_ref
+@pytest.mark.parametrize("ub", [0, 1])
+def test_get_basis_bit_exact_with_reference(ub):
+    rng = np.random.default_rng(7 + ub)
+    site = blocks.site_sectors(ub)
+    for _ in range(40):
+        blk = _random_blocks(rng, ub)
+        rs, rq = oracle.ref_get_basis(5, blk, site)
+        os_, oq = oracle.oracle_get_basis(5, blk, site)
+        pb = blocks.get_basis(5, blk, site)
+        assert np.array_equal(rs, os_) and np.array_equal(rq, oq)
+        assert np.array_equal(rs, pb.sec) and np.array_equal(rq, pb.sub)
+        # invariants: sum of sdim = dim, triangle rule, 2S then N descending
+        for k, (up, down, dim, n) in enumerate(rs):
+            q0 = int(rs[:k, 3].sum())
+            assert rq[q0:q0 + n, 5].sum() == dim
+            for bu, bd, su_, sd, _, _ in rq[q0:q0 + n]:
+                assert abs(bd - sd) <= down <= bd + sd and bu + su_ == up
+        keys = [(-d, -u) for u, d, _, _ in rs]
+        assert keys == sorted(keys)
+
+
+# ---- H_eff apply ------------------------------------------------------------
+
+@needs_ref
+@pytest.mark.parametrize("ub", [1, 0])
+@pytest.mark.parametrize("geom", GEOMS)
+def test_model_wave_matches_reference(ub, geom):
+    m = model(up_bias=ub)
+    sb = small_superblock(m, *geom, seed=100 * geom[0] + geom[1] + ub)
+    x = synth.random_vector(sb, 3)
+    assert rel(oracle.model_wave(m, sb, x), oracle.ref_model_wave(m, sb, x)) < 1e-13
+
+
+@needs_ref
+@pytest.mark.parametrize("terms", [dict(t=1.0, tp=0.0, J=0.0, mu=0.0), dict(t=0.0, tp=0.0, J=0.4, mu=0.0),
+                                   dict(t=0.0, tp=0.0, J=0.0, mu=0.7), dict(t=0.0, tp=-0.3, J=0.0, mu=0.0),
+                                   dict(t=0.0, tp=0.0, J=0.0, mu=0.0)])
+def test_model_wave_each_term_matches_reference(terms):
+    # isolates hopping, exchange (+density), chemical potential, nnn hopping and
+    # the block Hamiltonians alone
+    for ub in (1, 0):
+        m = model(up_bias=ub, **terms)
+        sb = small_superblock(m, 11, 11, seed=31 + ub)
+        x = synth.random_vector(sb, 5)
+        y = oracle.ref_model_wave(m, sb, x)
+        assert rel(oracle.model_wave(m, sb, x), y) < 1e-13
+
+
+@pytest.mark.parametrize("ub", [1, 0])
+def test_heff_symmetric(ub):
+    m = model(up_bias=ub)
+    sb = small_superblock(m, 11, 11, seed=5 + ub)
+    x, y = synth.random_vector(sb, 7), synth.random_vector(sb, 8)
+    a = x @ oracle.model_wave(m, sb, y)
+    b = y @ oracle.model_wave(m, sb, x)
+    assert abs(a - b) < 1e-13 * max(1.0, abs(a))
+
+
+def test_heff_linear():
+    m = model()
+    sb = small_superblock(m, 7, 9, seed=9)
+    x, y = synth.random_vector(sb, 1), synth.random_vector(sb, 2)
+    lhs = oracle.model_wave(m, sb, 2.0 * x - 0.5 * y)
+    rhs = 2.0 * oracle.model_wave(m, sb, x) - 0.5 * oracle.model_wave(m, sb, y)
+    assert rel(lhs, rhs) < 1e-13
+
+
+def test_spin_hermiticity_relation_is_needed():
+    # a mistake planted in the generator (the sqrt factor of another
+    # convention) must break the symmetry the tests rely on
+    orig = synth.spin_hermitian_factor
+    try:
+        synth.spin_hermitian_factor = lambda a, b: orig(a, b) * math.sqrt((b + 1.0) / (a + 1.0))
+        m = model(t=0.0, tp=0.0, mu=0.0)
+        sb = small_superblock(m, 11, 11, seed=5)
+        x, y = synth.random_vector(sb, 7), synth.random_vector(sb, 8)
+        assert abs(x @ oracle.model_wave(m, sb, y) - y @ oracle.model_wave(m, sb, x)) > 1e-6
+    finally:
+        synth.spin_hermitian_factor = orig
+
+
+# ---- Lanczos ------------------------------------------------------------------
+
+@needs_ref
+@pytest.mark.parametrize("ub", [1, 0])
+def test_lanczos_matches_reference(ub, tmp_path):
+    m = model(up_bias=ub)
+    sb = small_superblock(m, 11, 11, seed=3 + ub, msys=30, menv=30, median=4)
+    x0 = synth.random_vector(sb, 1)
+    cwd = os.getcwd()
+    os.chdir(tmp_path)          # the reference appends to Energy_All.dat
+    try:
+        e_ref, psi_ref = oracle.ref_lanczos(m, sb, x0, 1e-10)
+    finally:
+        os.chdir(cwd)
+    e, psi, k, _ = olanczos.lanczos(lambda v: oracle.model_wave(m, sb, v), x0, 1e-10, 110)
+    assert abs(e - e_ref) <= 1e-12 * abs(e_ref)
+    assert abs(abs(psi @ psi_ref) - 1.0) < 1e-10
+
+
+def test_lanczos_against_dense_diagonalisation():
+    m = model()
+    sb = small_superblock(m, 7, 9, seed=11, msys=12, menv=12, median=2, nsec=6)
+    D = sb.dim
+    H = np.stack([oracle.model_wave(m, sb, e) for e in np.eye(D)], axis=1)
+    assert np.allclose(H, H.T, atol=1e-12)
+    e_exact = np.linalg.eigvalsh(H)[0]
+    e, _, _, _ = olanczos.lanczos(lambda v: oracle.model_wave(m, sb, v), synth.random_vector(sb, 4), 1e-12, 110)
+    assert abs(e - e_exact) < 1e-10 * abs(e_exact)
+
+
+def test_lanczos_breakdown_exit():
+    # a start vector inside a 1-dimensional invariant subspace ends the
+    # recurrence at beta = 0 (eigen.f90:74-77)
+    A = np.diag(np.arange(1.0, 6.0))
+    e, psi, k, brk = olanczos.lanczos(lambda v: A @ v, np.eye(5)[2], 1e-10, 110)
+    assert brk and k == 1 and e == pytest.approx(3.0)
