@@ -155,12 +155,8 @@ def main():
     # roofline of the dominant stage: algorithmic flops of that stage per launch
     hp = capi.HostPlan(model, sb)
     # stage-1 flops: distinct A.X products; stage-2 flops: the rest of flops_exec
-    seg1 = hp.seg1
-    tile1 = hp.tile1
-    fl1 = 0.0
-    for t in tile1:
-        s = seg1[int(t["seg_begin"])]
-        fl1 += 2.0 * int(t["mt"]) * int(t["nt"]) * int(s["K"])
+    k1 = hp.seg1["K"][hp.tile1["seg_begin"]].astype(np.float64)
+    fl1 = float(2.0 * np.sum(hp.tile1["mt"].astype(np.float64) * hp.tile1["nt"] * k1))
     fl2 = st["flops_exec"] - fl1
     fl_dom = fl2 if dom == 2 else fl1
     achieved = fl_dom / t_dom

@@ -315,13 +315,13 @@ __global__ __launch_bounds__(64) void k_tiles_bf16x3(const Tile* __restrict__ ti
 struct MergeDst { int64_t dst_off, n; int t_begin, t_end; };
 
 __global__ void k_merge(double* arena, const MergeDst* dsts, const tjsq::MergeTerm* terms, int ndst) {
-  const int d = blockIdx.y;
-  if (d >= ndst) return;
-  const MergeDst D = dsts[d];
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < D.n; e += (int64_t)gridDim.x * blockDim.x) {
-    double acc = 0.0;
-    for (int t = D.t_begin; t < D.t_end; ++t) acc = acc + terms[t].coef * arena[terms[t].src_off + e];
-    arena[D.dst_off + e] = acc;
+  for (int d = blockIdx.y; d < ndst; d += gridDim.y) {
+    const MergeDst D = dsts[d];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < D.n; e += (int64_t)gridDim.x * blockDim.x) {
+      double acc = 0.0;
+      for (int t = D.t_begin; t < D.t_end; ++t) acc = acc + terms[t].coef * arena[terms[t].src_off + e];
+      arena[D.dst_off + e] = acc;
+    }
   }
 }
 
@@ -666,7 +666,7 @@ int tjsq_plan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tj
       HIPCHK(hipMemcpyAsync(dt, terms.data(), sizeof(tjsq::MergeTerm) * terms.size(), hipMemcpyHostToDevice, st));
       int64_t maxn = 1;
       for (auto& d : dsts) maxn = std::max(maxn, d.n);
-      dim3 grid((unsigned)std::min<int64_t>((maxn + 255) / 256, 64), (unsigned)dsts.size());
+      dim3 grid((unsigned)std::min<int64_t>((maxn + 255) / 256, 64), (unsigned)std::min<size_t>(dsts.size(), 32768));
       hipLaunchKernelGGL(k_merge, grid, dim3(256), 0, st, p->arena, dd, dt, (int)dsts.size());
       HIPCHK(hipGetLastError());
       HIPCHK(hipStreamSynchronize(st));
