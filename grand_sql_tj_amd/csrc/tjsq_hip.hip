@@ -7,7 +7,10 @@
 //   stage 2  Y_tile = sum_s c_s op(L_s) op(R_s)  pulled per output tile,
 //                                         no atomics, fixed summation order.
 // Each workgroup is one 64-lane wavefront that owns one 32x32 output tile and
-// accumulates it in MFMA registers (v_mfma_f64_16x16x4f64 in FP64,
+// accumulates it in MFMA registers.  Result layouts (measured on gfx950): lane
+// l, register r of a 16x16 block holds row l/16 + 4r for the FP64 16x16x4
+// instruction and row 4(l/16) + r for the FP32 and BF16 ones, column l%16 in
+// all three; operand A row i / k-group g comes from lane i + 16g (v_mfma_f64_16x16x4f64 in FP64,
 // v_mfma_f32_16x16x4f32 in FP32, v_mfma_f32_16x16x16bf16_1k three times for
 // the BF16x3 split), staging 32x16 / 16x32 operand panels through LDS.
 //
@@ -91,7 +94,7 @@ __global__ __launch_bounds__(64) void k_tiles_f64(const Tile* __restrict__ tiles
       for (int bi = 0; bi < 2; ++bi)
         for (int bj = 0; bj < 2; ++bj)
           for (int r = 0; r < 4; ++r) {
-            const int row = bi * 16 + lq * 4 + r, col = bj * 16 + lr;
+            const int row = bi * 16 + lq + 4 * r, col = bj * 16 + lr;
             if (row < tl.mt && col < tl.nt)
               acc[bi][bj][r] += s.c * L[s.l_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * s.l_ld];
           }
@@ -136,7 +139,7 @@ __global__ __launch_bounds__(64) void k_tiles_f64(const Tile* __restrict__ tiles
   for (int bi = 0; bi < 2; ++bi)
     for (int bj = 0; bj < 2; ++bj)
       for (int r = 0; r < 4; ++r) {
-        const int row = bi * 16 + lq * 4 + r, col = bj * 16 + lr;
+        const int row = bi * 16 + lq + 4 * r, col = bj * 16 + lr;
         if (row < tl.mt && col < tl.nt) out[tl.out_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * tl.out_ld] = acc[bi][bj][r];
       }
 }
