@@ -37,7 +37,7 @@ class PlanStats(C.Structure):
     _fields_ = [("dim", C.c_int64), ("n_contrib", C.c_int64), ("n_chain", C.c_int64), ("n_stage1", C.c_int64),
                 ("n_out_blocks", C.c_int64), ("n_tiles1", C.c_int64), ("n_tiles2", C.c_int64),
                 ("flops_ref", C.c_double), ("flops_exec", C.c_double), ("bytes_fp64", C.c_double),
-                ("op_elems", C.c_int64), ("scratch_elems", C.c_int64)]
+                ("op_elems", C.c_int64), ("scratch_elems", C.c_int64), ("n_slots", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -60,9 +60,10 @@ class LanczosResult(C.Structure):
 SEG_DTYPE = np.dtype([("l_off", "<i8"), ("r_off", "<i8"), ("l_ld", "<i4"), ("r_ld", "<i4"), ("K", "<i4"),
                       ("l_space", "u1"), ("r_space", "u1"), ("l_t", "u1"), ("r_t", "u1"), ("c", "<f8")])
 TILE_DTYPE = np.dtype([("out_off", "<i8"), ("out_ld", "<i4"), ("m0", "<i4"), ("n0", "<i4"), ("mt", "<i4"),
-                       ("nt", "<i4"), ("seg_begin", "<i4"), ("seg_end", "<i4"), ("out_space", "<i4")])
+                       ("nt", "<i4"), ("seg_begin", "<i4"), ("seg_end", "<i4"), ("out_space", "<i4"), ("slot", "<i4"),
+                       ("reserved", "<i4")])
 MERGE_DTYPE = np.dtype([("dst_off", "<i8"), ("src_off", "<i8"), ("n", "<i8"), ("coef", "<f8")])
-assert SEG_DTYPE.itemsize == 40 and TILE_DTYPE.itemsize == 40 and MERGE_DTYPE.itemsize == 32
+assert SEG_DTYPE.itemsize == 40 and TILE_DTYPE.itemsize == 48 and MERGE_DTYPE.itemsize == 32
 
 EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_tables", "tjsq_active_sites",
            "tjsq_plan_create", "tjsq_plan_stats_get", "tjsq_plan_destroy", "tjsq_heff_apply", "tjsq_heff_apply_f32",
@@ -97,7 +98,7 @@ def lib():
             "tjsq_lanczos_solve": [P, P, P, P, P],
             "tjsq_hostplan_create": [P, I, I, P, P, P, P, I, P, P],
             "tjsq_hostplan_sizes": [P, P],
-            "tjsq_hostplan_tables": [P, P, P, P, P, P],
+            "tjsq_hostplan_tables": [P, P, P, P, P, P, P],
             "tjsq_hostplan_stats": [P, P],
             "tjsq_hostplan_destroy": [P],
             "tjsq_plan_profile": [P, I],
@@ -227,17 +228,19 @@ class HostPlan:
                                          C.byref(ebs), int(sb.wsec.shape[0]), C.c_void_p(w), C.byref(h)),
               "tjsq_hostplan_create")
         self.h = h
-        sizes = np.zeros(11, np.int64)
+        sizes = np.zeros(13, np.int64)
         check(lib().tjsq_hostplan_sizes(h, sizes.ctypes.data), "tjsq_hostplan_sizes")
         (n1, t1, n2, t2, nm, self.arena_elems, self.scratch_elems, self.dim, self.sys_base, self.env_base,
-         self.merged_base) = [int(v) for v in sizes]
+         self.merged_base, nr, self.nslots) = [int(v) for v in sizes]
         self.seg1 = np.zeros(n1, SEG_DTYPE)
         self.tile1 = np.zeros(t1, TILE_DTYPE)
         self.seg2 = np.zeros(n2, SEG_DTYPE)
         self.tile2 = np.zeros(t2, TILE_DTYPE)
         self.merge = np.zeros(nm, MERGE_DTYPE)
+        self.rtile = np.zeros(nr, TILE_DTYPE)
         check(lib().tjsq_hostplan_tables(h, self.seg1.ctypes.data, self.tile1.ctypes.data, self.seg2.ctypes.data,
-                                         self.tile2.ctypes.data, self.merge.ctypes.data), "tjsq_hostplan_tables")
+                                         self.tile2.ctypes.data, self.merge.ctypes.data, self.rtile.ctypes.data),
+              "tjsq_hostplan_tables")
         st = PlanStats()
         check(lib().tjsq_hostplan_stats(h, C.byref(st)), "tjsq_hostplan_stats")
         self.stats = st.as_dict()

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -73,156 +74,22 @@ __device__ __forceinline__ const T* space_ptr(const Spaces<T>& S, int sp) {
 }
 
 // ---------------------------------------------------------------------------
-// FP64 grouped tile kernel
+// Grouped tile kernel (one template for FP64, FP32 and BF16x3)
+//
+// One 64-lane wavefront owns one 32x32 tile and walks its segments in K
+// chunks of KC.  Chunk c+1's operand values are fetched into registers
+// while the MFMAs of chunk c run from LDS, so a lone wave keeps one chunk of
+// global loads in flight.  Axpy segments (identity right operand) come first
+// in every segment list and are added straight into the accumulators.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_tiles_f64(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
-                                                  Spaces<double> S, int ntiles) {
-  const int tid = blockIdx.x;
-  if (tid >= ntiles) return;
-  const Tile tl = tiles[tid];
-  const int lane = threadIdx.x;
-  __shared__ double Ls[TS][KC + 1];
-  __shared__ double Rs[KC][TS + 1];
-  d4 acc[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-  const int lr = lane & 15, lq = lane >> 4;
-  for (int si = tl.seg_begin; si < tl.seg_end; ++si) {
-    const Seg s = segs[si];
-    const double* L = space_ptr(S, s.l_space);
-    if (s.r_space == tjsq::SP_IDENT) {
-      for (int bi = 0; bi < 2; ++bi)
-        for (int bj = 0; bj < 2; ++bj)
-          for (int r = 0; r < 4; ++r) {
-            const int row = bi * 16 + lq + 4 * r, col = bj * 16 + lr;
-            if (row < tl.mt && col < tl.nt)
-              acc[bi][bj][r] += s.c * L[s.l_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * s.l_ld];
-          }
-      continue;
-    }
-    const double* R = space_ptr(S, s.r_space);
-    for (int k0 = 0; k0 < s.K; k0 += KC) {
-      for (int q = 0; q < (TS * KC) / 64; ++q) {
-        const int e = lane + 64 * q;
-        int i, k;
-        if (!s.l_t) { i = e & (TS - 1); k = e / TS; } else { k = e & (KC - 1); i = e / KC; }
-        double v = 0.0;
-        if (i < tl.mt && k0 + k < s.K)
-          v = s.l_t ? L[s.l_off + (k0 + k) + (int64_t)(tl.m0 + i) * s.l_ld]
-                    : L[s.l_off + (tl.m0 + i) + (int64_t)(k0 + k) * s.l_ld];
-        Ls[i][k] = v * s.c;
-      }
-      for (int q = 0; q < (TS * KC) / 64; ++q) {
-        const int e = lane + 64 * q;
-        int j, k;
-        if (!s.r_t) { k = e & (KC - 1); j = e / KC; } else { j = e & (TS - 1); k = e / TS; }
-        double v = 0.0;
-        if (j < tl.nt && k0 + k < s.K)
-          v = s.r_t ? R[s.r_off + (tl.n0 + j) + (int64_t)(k0 + k) * s.r_ld]
-                    : R[s.r_off + (k0 + k) + (int64_t)(tl.n0 + j) * s.r_ld];
-        Rs[k][j] = v;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int kk = 0; kk < KC / 4; ++kk) {
-        const double a0 = Ls[lr][kk * 4 + lq], a1 = Ls[16 + lr][kk * 4 + lq];
-        const double b0 = Rs[kk * 4 + lq][lr], b1 = Rs[kk * 4 + lq][16 + lr];
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-      }
-      __syncthreads();
-    }
-  }
-  double* out = tl.out_space == tjsq::SP_T ? S.t : S.y;
-  for (int bi = 0; bi < 2; ++bi)
-    for (int bj = 0; bj < 2; ++bj)
-      for (int r = 0; r < 4; ++r) {
-        const int row = bi * 16 + lq + 4 * r, col = bj * 16 + lr;
-        if (row < tl.mt && col < tl.nt) out[tl.out_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * tl.out_ld] = acc[bi][bj][r];
-      }
-}
+enum { MODE_F64 = 0, MODE_F32 = 1, MODE_BF16X3 = 2 };
 
-// ---------------------------------------------------------------------------
-// FP32 grouped tile kernel
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_tiles_f32(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
-                                                  Spaces<float> S, int ntiles) {
-  const int tid = blockIdx.x;
-  if (tid >= ntiles) return;
-  const Tile tl = tiles[tid];
-  const int lane = threadIdx.x;
-  __shared__ float Ls[TS][KC + 1];
-  __shared__ float Rs[KC][TS + 1];
-  f4 acc[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-  const int lr = lane & 15, lq = lane >> 4;
-  for (int si = tl.seg_begin; si < tl.seg_end; ++si) {
-    const Seg s = segs[si];
-    const float c = (float)s.c;
-    const float* L = space_ptr(S, s.l_space);
-    if (s.r_space == tjsq::SP_IDENT) {
-      for (int bi = 0; bi < 2; ++bi)
-        for (int bj = 0; bj < 2; ++bj)
-          for (int r = 0; r < 4; ++r) {
-            const int row = bi * 16 + lq * 4 + r, col = bj * 16 + lr;
-            if (row < tl.mt && col < tl.nt)
-              acc[bi][bj][r] += c * L[s.l_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * s.l_ld];
-          }
-      continue;
-    }
-    const float* R = space_ptr(S, s.r_space);
-    for (int k0 = 0; k0 < s.K; k0 += KC) {
-      for (int q = 0; q < (TS * KC) / 64; ++q) {
-        const int e = lane + 64 * q;
-        int i, k;
-        if (!s.l_t) { i = e & (TS - 1); k = e / TS; } else { k = e & (KC - 1); i = e / KC; }
-        float v = 0.f;
-        if (i < tl.mt && k0 + k < s.K)
-          v = s.l_t ? L[s.l_off + (k0 + k) + (int64_t)(tl.m0 + i) * s.l_ld]
-                    : L[s.l_off + (tl.m0 + i) + (int64_t)(k0 + k) * s.l_ld];
-        Ls[i][k] = v * c;
-      }
-      for (int q = 0; q < (TS * KC) / 64; ++q) {
-        const int e = lane + 64 * q;
-        int j, k;
-        if (!s.r_t) { k = e & (KC - 1); j = e / KC; } else { j = e & (TS - 1); k = e / TS; }
-        float v = 0.f;
-        if (j < tl.nt && k0 + k < s.K)
-          v = s.r_t ? R[s.r_off + (tl.n0 + j) + (int64_t)(k0 + k) * s.r_ld]
-                    : R[s.r_off + (k0 + k) + (int64_t)(tl.n0 + j) * s.r_ld];
-        Rs[k][j] = v;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int kk = 0; kk < KC / 4; ++kk) {
-        const float a0 = Ls[lr][kk * 4 + lq], a1 = Ls[16 + lr][kk * 4 + lq];
-        const float b0 = Rs[kk * 4 + lq][lr], b1 = Rs[kk * 4 + lq][16 + lr];
-        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
-      }
-      __syncthreads();
-    }
-  }
-  float* out = tl.out_space == tjsq::SP_T ? S.t : S.y;
-  for (int bi = 0; bi < 2; ++bi)
-    for (int bj = 0; bj < 2; ++bj)
-      for (int r = 0; r < 4; ++r) {
-        const int row = bi * 16 + lq * 4 + r, col = bj * 16 + lr;
-        if (row < tl.mt && col < tl.nt) out[tl.out_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * tl.out_ld] = acc[bi][bj][r];
-      }
-}
+template <int MODE>
+struct KT {
+  using T = typename std::conditional<MODE == MODE_F64, double, float>::type;
+  using Acc = typename std::conditional<MODE == MODE_F64, d4, f4>::type;
+};
 
-// ---------------------------------------------------------------------------
-// BF16x3 grouped tile kernel: a*b ~ ah*bh + ah*bl + al*bh with a = ah + al
-// (bf16 hi part by round-to-nearest-even, lo part the rounded remainder),
-// FP32 accumulation in the MFMA registers.  Operands are FP32 in memory and
-// split while staging into LDS.
-// ---------------------------------------------------------------------------
 __device__ __forceinline__ unsigned short bf16_rne(float f) {
   unsigned int u = __float_as_uint(f);
   u += 0x7FFFu + ((u >> 16) & 1u);
@@ -230,86 +97,188 @@ __device__ __forceinline__ unsigned short bf16_rne(float f) {
 }
 __device__ __forceinline__ float bf16_to_f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
 
-__global__ __launch_bounds__(64) void k_tiles_bf16x3(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
-                                                     Spaces<float> S, int ntiles) {
+// row of register r of lane (lq, lr) inside a 16x16 MFMA result block
+template <int MODE>
+__device__ __forceinline__ int acc_row(int lq, int r) {
+  return MODE == MODE_F64 ? lq + 4 * r : lq * 4 + r;
+}
+
+template <int MODE, typename T>
+__device__ __forceinline__ void fetch_chunk(const Spaces<T>& S, const Seg& s, const Tile& tl, int k0, int lane,
+                                            T* rl, T* rr) {
+  const T* L = space_ptr(S, s.l_space);
+  const T* R = space_ptr(S, s.r_space);
+#pragma unroll
+  for (int q = 0; q < (TS * KC) / 64; ++q) {
+    const int e = lane + 64 * q;
+    int i, k;
+    if (!s.l_t) { i = e & (TS - 1); k = e / TS; } else { k = e & (KC - 1); i = e / KC; }
+    T v = T(0);
+    if (i < tl.mt && k0 + k < s.K)
+      v = s.l_t ? L[s.l_off + (k0 + k) + (int64_t)(tl.m0 + i) * s.l_ld]
+                : L[s.l_off + (tl.m0 + i) + (int64_t)(k0 + k) * s.l_ld];
+    rl[q] = v;
+    int j;
+    if (!s.r_t) { k = e & (KC - 1); j = e / KC; } else { j = e & (TS - 1); k = e / TS; }
+    T w = T(0);
+    if (j < tl.nt && k0 + k < s.K)
+      w = s.r_t ? R[s.r_off + (tl.n0 + j) + (int64_t)(k0 + k) * s.r_ld]
+                : R[s.r_off + (k0 + k) + (int64_t)(tl.n0 + j) * s.r_ld];
+    rr[q] = w;
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(64) void k_tiles(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
+                                              Spaces<typename KT<MODE>::T> S, typename KT<MODE>::T* partial,
+                                              int ntiles) {
+  using T = typename KT<MODE>::T;
+  using Acc = typename KT<MODE>::Acc;
   const int tid = blockIdx.x;
   if (tid >= ntiles) return;
   const Tile tl = tiles[tid];
   const int lane = threadIdx.x;
-  // [hi/lo][i][k] and [hi/lo][j][k] (k fastest so a lane reads 4 consecutive k)
-  __shared__ unsigned short Lh[2][TS][KC + 4];
-  __shared__ unsigned short Rh[2][TS][KC + 4];
-  f4 acc[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
   const int lr = lane & 15, lq = lane >> 4;
-  for (int si = tl.seg_begin; si < tl.seg_end; ++si) {
+  constexpr int LDS_T = MODE == MODE_BF16X3 ? 1 : TS * (KC + 1);
+  constexpr int LDS_R = MODE == MODE_BF16X3 ? 1 : KC * (TS + 1);
+  __shared__ T Ls[LDS_T];
+  __shared__ T Rs[LDS_R];
+  constexpr int LDS_H = MODE == MODE_BF16X3 ? 2 * TS * (KC + 4) : 1;
+  __shared__ unsigned short Lh[LDS_H];
+  __shared__ unsigned short Rh[LDS_H];
+  Acc acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int r = 0; r < 4; ++r) acc[a][b][r] = T(0);
+
+  int si = tl.seg_begin;
+  const int se = tl.seg_end;
+  // axpy segments
+  for (; si < se; ++si) {
     const Seg s = segs[si];
-    const float c = (float)s.c;
-    const float* L = space_ptr(S, s.l_space);
-    if (s.r_space == tjsq::SP_IDENT) {
-      for (int bi = 0; bi < 2; ++bi)
-        for (int bj = 0; bj < 2; ++bj)
-          for (int r = 0; r < 4; ++r) {
-            const int row = bi * 16 + lq * 4 + r, col = bj * 16 + lr;
-            if (row < tl.mt && col < tl.nt)
-              acc[bi][bj][r] += c * L[s.l_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * s.l_ld];
-          }
-      continue;
-    }
-    const float* R = space_ptr(S, s.r_space);
-    for (int k0 = 0; k0 < s.K; k0 += KC) {
+    if (s.r_space != tjsq::SP_IDENT) break;
+    const T* L = space_ptr(S, s.l_space);
+    const T c = (T)s.c;
+    for (int bi = 0; bi < 2; ++bi)
+      for (int bj = 0; bj < 2; ++bj)
+        for (int r = 0; r < 4; ++r) {
+          const int row = bi * 16 + acc_row<MODE>(lq, r), col = bj * 16 + lr;
+          if (row < tl.mt && col < tl.nt)
+            acc[bi][bj][r] += c * L[s.l_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * s.l_ld];
+        }
+  }
+  if (si < se) {
+    Seg cur = segs[si];
+    int k0 = 0;
+    T rl[(TS * KC) / 64], rr[(TS * KC) / 64];
+    fetch_chunk<MODE>(S, cur, tl, 0, lane, rl, rr);
+    for (;;) {
+      // stage the fetched chunk in LDS
+      const T c = (T)cur.c;
+#pragma unroll
       for (int q = 0; q < (TS * KC) / 64; ++q) {
         const int e = lane + 64 * q;
         int i, k;
-        if (!s.l_t) { i = e & (TS - 1); k = e / TS; } else { k = e & (KC - 1); i = e / KC; }
-        float v = 0.f;
-        if (i < tl.mt && k0 + k < s.K)
-          v = s.l_t ? L[s.l_off + (k0 + k) + (int64_t)(tl.m0 + i) * s.l_ld]
-                    : L[s.l_off + (tl.m0 + i) + (int64_t)(k0 + k) * s.l_ld];
-        v *= c;
-        const unsigned short h = bf16_rne(v);
-        Lh[0][i][k] = h;
-        Lh[1][i][k] = bf16_rne(v - bf16_to_f(h));
-      }
-      for (int q = 0; q < (TS * KC) / 64; ++q) {
-        const int e = lane + 64 * q;
-        int j, k;
-        if (!s.r_t) { k = e & (KC - 1); j = e / KC; } else { j = e & (TS - 1); k = e / TS; }
-        float v = 0.f;
-        if (j < tl.nt && k0 + k < s.K)
-          v = s.r_t ? R[s.r_off + (tl.n0 + j) + (int64_t)(k0 + k) * s.r_ld]
-                    : R[s.r_off + (k0 + k) + (int64_t)(tl.n0 + j) * s.r_ld];
-        const unsigned short h = bf16_rne(v);
-        Rh[0][j][k] = h;
-        Rh[1][j][k] = bf16_rne(v - bf16_to_f(h));
+        if (!cur.l_t) { i = e & (TS - 1); k = e / TS; } else { k = e & (KC - 1); i = e / KC; }
+        int j, kr;
+        if (!cur.r_t) { kr = e & (KC - 1); j = e / KC; } else { j = e & (TS - 1); kr = e / TS; }
+        if constexpr (MODE == MODE_BF16X3) {
+          const float v = rl[q] * c;
+          const unsigned short h = bf16_rne(v);
+          Lh[i * (KC + 4) + k] = h;
+          Lh[TS * (KC + 4) + i * (KC + 4) + k] = bf16_rne(v - bf16_to_f(h));
+          const float w = rr[q];
+          const unsigned short g = bf16_rne(w);
+          Rh[j * (KC + 4) + kr] = g;
+          Rh[TS * (KC + 4) + j * (KC + 4) + kr] = bf16_rne(w - bf16_to_f(g));
+        } else {
+          Ls[i * (KC + 1) + k] = rl[q] * c;
+          Rs[kr * (TS + 1) + j] = rr[q];
+        }
       }
       __syncthreads();
-      // one 16x16x16 step covers the whole K chunk: lane supplies k = 4*lq..4*lq+3
-      s4 a[2][2], b[2][2];
-      for (int p = 0; p < 2; ++p)
-        for (int h = 0; h < 2; ++h) {
-          for (int t = 0; t < 4; ++t) {
-            a[p][h][t] = (short)Lh[h][p * 16 + lr][lq * 4 + t];
-            b[p][h][t] = (short)Rh[h][p * 16 + lr][lq * 4 + t];
+      // next chunk position; its loads overlap this chunk's MFMAs
+      int nk0 = k0 + KC, nsi = si;
+      Seg nxt = cur;
+      if (nk0 >= cur.K) {
+        nsi = si + 1;
+        nk0 = 0;
+        if (nsi < se) nxt = segs[nsi];
+      }
+      const bool more = nsi < se;
+      if (more) fetch_chunk<MODE>(S, nxt, tl, nk0, lane, rl, rr);
+      if constexpr (MODE == MODE_BF16X3) {
+        s4 a[2][2], b[2][2];
+        for (int p = 0; p < 2; ++p)
+          for (int h = 0; h < 2; ++h)
+            for (int t = 0; t < 4; ++t) {
+              a[p][h][t] = (short)Lh[h * TS * (KC + 4) + (p * 16 + lr) * (KC + 4) + lq * 4 + t];
+              b[p][h][t] = (short)Rh[h * TS * (KC + 4) + (p * 16 + lr) * (KC + 4) + lq * 4 + t];
+            }
+        for (int bi = 0; bi < 2; ++bi)
+          for (int bj = 0; bj < 2; ++bj) {
+            acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][1], b[bj][0], acc[bi][bj], 0, 0, 0);
+            acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][0], b[bj][1], acc[bi][bj], 0, 0, 0);
+            acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][0], b[bj][0], acc[bi][bj], 0, 0, 0);
+          }
+      } else {
+#pragma unroll
+        for (int kk = 0; kk < KC / 4; ++kk) {
+          const T a0 = Ls[lr * (KC + 1) + kk * 4 + lq], a1 = Ls[(16 + lr) * (KC + 1) + kk * 4 + lq];
+          const T b0 = Rs[(kk * 4 + lq) * (TS + 1) + lr], b1 = Rs[(kk * 4 + lq) * (TS + 1) + 16 + lr];
+          if constexpr (MODE == MODE_F64) {
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+          } else {
+            acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
           }
         }
-      for (int bi = 0; bi < 2; ++bi)
-        for (int bj = 0; bj < 2; ++bj) {
-          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][1], b[bj][0], acc[bi][bj], 0, 0, 0);
-          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][0], b[bj][1], acc[bi][bj], 0, 0, 0);
-          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][0], b[bj][0], acc[bi][bj], 0, 0, 0);
-        }
+      }
       __syncthreads();
+      if (!more) break;
+      cur = nxt;
+      si = nsi;
+      k0 = nk0;
     }
   }
-  float* out = tl.out_space == tjsq::SP_T ? S.t : S.y;
+  T* out;
+  int64_t base;
+  int ld;
+  if (tl.slot >= 0) {
+    out = partial;
+    base = (int64_t)tl.slot * TS * TS;
+    ld = TS;
+  } else {
+    out = tl.out_space == tjsq::SP_T ? S.t : S.y;
+    base = tl.out_off + tl.m0 + (int64_t)tl.n0 * tl.out_ld;
+    ld = tl.out_ld;
+  }
   for (int bi = 0; bi < 2; ++bi)
     for (int bj = 0; bj < 2; ++bj)
       for (int r = 0; r < 4; ++r) {
-        const int row = bi * 16 + lq * 4 + r, col = bj * 16 + lr;
-        if (row < tl.mt && col < tl.nt) out[tl.out_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * tl.out_ld] = acc[bi][bj][r];
+        const int row = bi * 16 + acc_row<MODE>(lq, r), col = bj * 16 + lr;
+        if (row < tl.mt && col < tl.nt) out[base + row + (int64_t)col * ld] = acc[bi][bj][r];
       }
+}
+
+// ordered sum of the partial tiles of split work items
+template <typename T>
+__global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ rt, const T* __restrict__ partial, T* y,
+                                                int n) {
+  const int t = blockIdx.x;
+  if (t >= n) return;
+  const Tile tl = rt[t];
+  for (int e = threadIdx.x; e < tl.mt * tl.nt; e += blockDim.x) {
+    const int row = e % tl.mt, col = e / tl.mt;
+    T acc = T(0);
+    for (int q = tl.seg_begin; q < tl.seg_end; ++q) acc += partial[(int64_t)q * TS * TS + row + col * TS];
+    y[tl.out_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * tl.out_ld] = acc;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -427,6 +396,9 @@ struct tjsq_plan {
   Tile* tile2 = nullptr;
   Seg* seg1 = nullptr;
   Seg* seg2 = nullptr;
+  Tile* rtile = nullptr;
+  double* part64 = nullptr;      // partial tiles of split stage-2 work
+  float* part32 = nullptr;
   // Lanczos workspace
   void* kry = nullptr;           // Krylov basis
   int64_t kry_bytes = 0;
@@ -468,14 +440,22 @@ int launch_tiles(const tjsq_plan* p, int stage, const T* ops, const T* x, T* t, 
     HIPCHK(hipEventRecord(e0, st));
   }
   if constexpr (sizeof(T) == 8) {
-    hipLaunchKernelGGL(k_tiles_f64, dim3(n), dim3(64), 0, st, tiles, segs, S, n);
+    hipLaunchKernelGGL(k_tiles<MODE_F64>, dim3(n), dim3(64), 0, st, tiles, segs, S, p->part64, n);
   } else {
     if (precision == TJSQ_BF16X3)
-      hipLaunchKernelGGL(k_tiles_bf16x3, dim3(n), dim3(64), 0, st, tiles, segs, S, n);
+      hipLaunchKernelGGL(k_tiles<MODE_BF16X3>, dim3(n), dim3(64), 0, st, tiles, segs, S, p->part32, n);
     else
-      hipLaunchKernelGGL(k_tiles_f32, dim3(n), dim3(64), 0, st, tiles, segs, S, n);
+      hipLaunchKernelGGL(k_tiles<MODE_F32>, dim3(n), dim3(64), 0, st, tiles, segs, S, p->part32, n);
   }
   HIPCHK(hipGetLastError());
+  if (stage == 2 && !p->hp.rtile.empty()) {
+    const int nr = (int)p->hp.rtile.size();
+    if constexpr (sizeof(T) == 8)
+      hipLaunchKernelGGL(k_reduce<double>, dim3(nr), dim3(256), 0, st, p->rtile, p->part64, y, nr);
+    else
+      hipLaunchKernelGGL(k_reduce<float>, dim3(nr), dim3(256), 0, st, p->rtile, p->part32, y, nr);
+    HIPCHK(hipGetLastError());
+  }
   if (p->profiling) {
     HIPCHK(hipEventRecord(e1, st));
     const_cast<tjsq_plan*>(p)->prof.push_back({stage, {e0, e1}});
@@ -624,7 +604,7 @@ int tjsq_plan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tj
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(TJSQ_ERR_NODEV, "no GPU visible");
   tjsq_plan* p = new tjsq_plan();
   std::string msg;
-  int rc = tjsq::build_host_plan(*lat, sys_len, env_len, *sys, *env, *sys_bs, *env_bs, nw, wsec, TS, p->hp, msg);
+  int rc = tjsq::build_host_plan(*lat, sys_len, env_len, *sys, *env, *sys_bs, *env_bs, nw, wsec, tjsq::Shape(), p->hp, msg);
   if (rc != TJSQ_OK) {
     delete p;
     return fail(rc, "tjsq_plan_create: " + msg);
@@ -695,8 +675,13 @@ int tjsq_plan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tj
       return TJSQ_ERR_HIP;
     return TJSQ_OK;
   };
-  if (put(&p->tile1, hp.tile1) || put(&p->tile2, hp.tile2) || put(&p->seg1, hp.seg1) || put(&p->seg2, hp.seg2))
+  if (put(&p->tile1, hp.tile1) || put(&p->tile2, hp.tile2) || put(&p->seg1, hp.seg1) || put(&p->seg2, hp.seg2) ||
+      put(&p->rtile, hp.rtile))
     return cleanup(TJSQ_ERR_NOMEM, "task table allocation failed");
+  if (hipMalloc(&p->part64, sizeof(double) * std::max<int64_t>(1, hp.nslots * TS * TS)) != hipSuccess)
+    return cleanup(TJSQ_ERR_NOMEM, "partial tile allocation failed");
+  if (p->arena32 && hipMalloc(&p->part32, sizeof(float) * std::max<int64_t>(1, hp.nslots * TS * TS)) != hipSuccess)
+    return cleanup(TJSQ_ERR_NOMEM, "partial tile allocation failed");
   HIPCHK(hipStreamSynchronize(st));
   *plan = p;
   return TJSQ_OK;
@@ -714,7 +699,7 @@ int tjsq_hostplan_create(const tjsq_lattice* lat, int sys_len, int env_len, cons
   *out = nullptr;
   tjsq_hostplan* h = new tjsq_hostplan();
   std::string msg;
-  const int rc = tjsq::build_host_plan(*lat, sys_len, env_len, *sys, *env, *sys_bs, *env_bs, nw, wsec, TS, h->hp, msg);
+  const int rc = tjsq::build_host_plan(*lat, sys_len, env_len, *sys, *env, *sys_bs, *env_bs, nw, wsec, tjsq::Shape(), h->hp, msg);
   if (rc != TJSQ_OK) {
     delete h;
     return fail(rc, "tjsq_hostplan_create: " + msg);
@@ -728,12 +713,13 @@ int tjsq_hostplan_sizes(const tjsq_hostplan* h, int64_t* sizes) {
   const HostPlan& hp = h->hp;
   const int64_t v[] = {(int64_t)hp.seg1.size(), (int64_t)hp.tile1.size(), (int64_t)hp.seg2.size(),
                        (int64_t)hp.tile2.size(), (int64_t)hp.merge.size(), hp.arena_elems, hp.scratch_elems, hp.dim,
-                       hp.sys_base, hp.env_base, hp.merged_base};
-  std::copy(v, v + 11, sizes);
+                       hp.sys_base, hp.env_base, hp.merged_base, (int64_t)hp.rtile.size(), hp.nslots};
+  std::copy(v, v + 13, sizes);
   return TJSQ_OK;
 }
 
-int tjsq_hostplan_tables(const tjsq_hostplan* h, void* seg1, void* tile1, void* seg2, void* tile2, void* merge) {
+int tjsq_hostplan_tables(const tjsq_hostplan* h, void* seg1, void* tile1, void* seg2, void* tile2, void* merge,
+                         void* rtile) {
   if (!h) return fail(TJSQ_ERR_ARG, "tjsq_hostplan_tables: null argument");
   const HostPlan& hp = h->hp;
   if (seg1) std::memcpy(seg1, hp.seg1.data(), sizeof(Seg) * hp.seg1.size());
@@ -741,6 +727,7 @@ int tjsq_hostplan_tables(const tjsq_hostplan* h, void* seg1, void* tile1, void* 
   if (seg2) std::memcpy(seg2, hp.seg2.data(), sizeof(Seg) * hp.seg2.size());
   if (tile2) std::memcpy(tile2, hp.tile2.data(), sizeof(Tile) * hp.tile2.size());
   if (merge) std::memcpy(merge, hp.merge.data(), sizeof(tjsq::MergeTerm) * hp.merge.size());
+  if (rtile) std::memcpy(rtile, hp.rtile.data(), sizeof(Tile) * hp.rtile.size());
   return TJSQ_OK;
 }
 
@@ -771,6 +758,9 @@ int tjsq_plan_destroy(tjsq_plan* p) {
   (void)hipFree(p->tile2);
   (void)hipFree(p->seg1);
   (void)hipFree(p->seg2);
+  (void)hipFree(p->rtile);
+  (void)hipFree(p->part64);
+  (void)hipFree(p->part32);
   (void)hipFree(p->kry);
   (void)hipFree(p->dwork);
   (void)hipFree(p->x32);

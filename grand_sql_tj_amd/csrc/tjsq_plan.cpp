@@ -927,8 +927,9 @@ int read_opset(const tjsq_opset& in, int64_t base, OpSetView& out, std::string& 
 }  // namespace
 
 int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjsq_opset& sys, const tjsq_opset& env,
-                    const tjsq_basis& sys_bs, const tjsq_basis& env_bs, int nw, const int* wsec, int tile, HostPlan& P,
-                    std::string& msg) {
+                    const tjsq_basis& sys_bs, const tjsq_basis& env_bs, int nw, const int* wsec, const Shape& shape,
+                    HostPlan& P, std::string& msg) {
+  const int tile = shape.tile;
   P = HostPlan();
   P.sys_len = sys_len;
   P.env_len = env_len;
@@ -1178,7 +1179,7 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
         for (int m0 = 0; m0 < c.M; m0 += tile)
           for (int n0 = 0; n0 < c.K2; n0 += tile)
             P.tile1.push_back({toff, c.M, m0, n0, std::min(tile, c.M - m0), std::min(tile, c.K2 - n0), sidx, sidx + 1,
-                               SP_T});
+                               SP_T, -1, 0});
         fl_exec += 2.0 * c.M * c.K1 * c.K2;
       } else {
         t_of[q] = it->second;
@@ -1229,12 +1230,64 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
     }
     P.seg2[r] = s;
   }
+  // work items: per output block, axpy segments first, then GEMM segments cut
+  // into K pieces of at most k_split and grouped into items of bounded work
+  std::vector<Seg> pieces;
+  pieces.reserve(P.seg2.size() * 2);
   for (size_t b = 0; b < P.out_blocks.size(); ++b) {
     const OutBlock& ob = P.out_blocks[b];
+    std::vector<std::pair<int, int>> items;  // [begin, end) into pieces
+    const int first = (int)pieces.size();
+    for (int r = bstart[b]; r < bstart[b + 1]; ++r)
+      if (P.seg2[r].r_space == SP_IDENT) pieces.push_back(P.seg2[r]);
+    for (int r = bstart[b]; r < bstart[b + 1]; ++r) {
+      const Seg& s0 = P.seg2[r];
+      if (s0.r_space == SP_IDENT) continue;
+      for (int k_lo = 0; k_lo < s0.K; k_lo += shape.k_split) {
+        Seg s = s0;
+        s.K = std::min(shape.k_split, s0.K - k_lo);
+        s.l_off += s0.l_t ? (int64_t)k_lo : (int64_t)k_lo * s0.l_ld;
+        s.r_off += s0.r_t ? (int64_t)k_lo * s0.r_ld : (int64_t)k_lo;
+        pieces.push_back(s);
+      }
+    }
+    const int last = (int)pieces.size();
+    int ib = first, cost = 0;
+    for (int q = first; q < last; ++q) {
+      const int c = pieces[q].r_space == SP_IDENT ? 1 : (pieces[q].K + shape.kc - 1) / shape.kc;
+      if (q > ib && cost + c > shape.item_chunks) {
+        items.push_back({ib, q});
+        ib = q;
+        cost = 0;
+      }
+      cost += c;
+    }
+    items.push_back({ib, last});  // also the empty item of a block without terms
     for (int m0 = 0; m0 < ob.M; m0 += tile)
-      for (int n0 = 0; n0 < ob.N; n0 += tile)
-        P.tile2.push_back({ob.off, ob.ld, m0, n0, std::min(tile, ob.M - m0), std::min(tile, ob.N - n0), bstart[b],
-                           bstart[b + 1], SP_X});
+      for (int n0 = 0; n0 < ob.N; n0 += tile) {
+        const int mt = std::min(tile, ob.M - m0), nt = std::min(tile, ob.N - n0);
+        if (items.size() == 1) {
+          P.tile2.push_back({ob.off, ob.ld, m0, n0, mt, nt, items[0].first, items[0].second, SP_X, -1, 0});
+          continue;
+        }
+        const int s0 = (int)P.nslots;
+        for (auto& it : items) P.tile2.push_back({ob.off, ob.ld, m0, n0, mt, nt, it.first, it.second, SP_X, (int)P.nslots++, 0});
+        P.rtile.push_back({ob.off, ob.ld, m0, n0, mt, nt, s0, (int)P.nslots, SP_X, -1, 0});
+      }
+  }
+  P.seg2.swap(pieces);
+  // longest work items first: the grid drains with fewer stragglers
+  {
+    std::vector<int64_t> cost(P.tile2.size(), 0);
+    for (size_t t = 0; t < P.tile2.size(); ++t)
+      for (int q = P.tile2[t].seg_begin; q < P.tile2[t].seg_end; ++q)
+        cost[t] += P.seg2[q].r_space == SP_IDENT ? 1 : P.seg2[q].K;
+    std::vector<int> idx(P.tile2.size());
+    for (size_t t = 0; t < idx.size(); ++t) idx[t] = (int)t;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<Tile> sorted(P.tile2.size());
+    for (size_t t = 0; t < idx.size(); ++t) sorted[t] = P.tile2[idx[t]];
+    P.tile2.swap(sorted);
   }
 
   std::sort(opref.begin(), opref.end());
@@ -1251,6 +1304,7 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
   st.n_out_blocks = (int64_t)P.out_blocks.size();
   st.n_tiles1 = (int64_t)P.tile1.size();
   st.n_tiles2 = (int64_t)P.tile2.size();
+  st.n_slots = P.nslots;
   st.flops_ref = fl_ref;
   st.flops_exec = fl_exec;
   st.op_elems = op_elems;

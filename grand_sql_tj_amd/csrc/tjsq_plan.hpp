@@ -101,6 +101,8 @@ struct Tile {
   int64_t out_off;
   int32_t out_ld, m0, n0, mt, nt, seg_begin, seg_end;
   int32_t out_space;
+  int32_t slot;             // >= 0: write a partial tile to partial slot `slot`
+  int32_t reserved;         // keeps the 48-byte layout explicit
 };
 
 struct HostPlan {
@@ -118,15 +120,29 @@ struct HostPlan {
   std::vector<Contrib> contrib;
   std::vector<Seg> seg1, seg2;
   std::vector<Tile> tile1, tile2;
+  // stage-2 work items that share an output tile write partial tiles; rtile
+  // sums slots [seg_begin, seg_end) of each such tile in order
+  std::vector<Tile> rtile;
+  int64_t nslots = 0;
   int64_t scratch_elems = 0;
   tjsq_plan_stats stats{};
   std::vector<int64_t> ops_used; // distinct operator sub-blocks referenced (elements)
 };
 
 // Builds the whole host plan.  Returns TJSQ_OK or an error code with msg set.
+// Work-item shaping of stage 2: segments longer than k_split are cut into K
+// pieces and consecutive pieces of an output tile are grouped into work items
+// of at most `item_chunks` K-chunks of `kc` (an axpy counts one chunk).
+struct Shape {
+  int tile = 32;
+  int kc = 16;
+  int k_split = 256;
+  int item_chunks = 64;
+};
+
 int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjsq_opset& sys,
                     const tjsq_opset& env, const tjsq_basis& sys_bs, const tjsq_basis& env_bs, int nw,
-                    const int* wsec, int tile, HostPlan& P, std::string& msg);
+                    const int* wsec, const Shape& shape, HostPlan& P, std::string& msg);
 
 int get_basis(int nb, const int* blk, int nsite, const int* site, std::vector<int>& sec,
               std::vector<int>& sub);
@@ -136,5 +152,5 @@ int get_super_basis(const tjsq_basis& s, const tjsq_basis& e, int total_up, int 
 }  // namespace tjsq
 
 static_assert(sizeof(tjsq::Seg) == 40, "Seg layout is part of the C-ABI");
-static_assert(sizeof(tjsq::Tile) == 40, "Tile layout is part of the C-ABI");
+static_assert(sizeof(tjsq::Tile) == 48, "Tile layout is part of the C-ABI");
 static_assert(sizeof(tjsq::MergeTerm) == 32, "MergeTerm layout is part of the C-ABI");

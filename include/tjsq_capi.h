@@ -112,6 +112,7 @@ typedef struct tjsq_plan_stats {
   double bytes_fp64;        /* compulsory bytes per apply at 8 B/element */
   int64_t op_elems;         /* operator elements referenced */
   int64_t scratch_elems;    /* stage-1 scratch elements */
+  int64_t n_slots;          /* partial output tiles of split stage-2 work */
 } tjsq_plan_stats;
 
 typedef struct tjsq_lanczos_opts {
@@ -193,18 +194,22 @@ int tjsq_lanczos_solve(tjsq_plan* plan, double* psi, const tjsq_lanczos_opts* op
 
 /* ---- host-only plan inspection (no GPU needed) --------------------------
  * The task tables a plan would upload, for tests of the plan compiler.
- * sizes[11] = {n_seg1, n_tile1, n_seg2, n_tile2, n_merge, arena_elems,
- * scratch_elems, dim, sys_base, env_base, merged_base}.  Table element
- * layouts: segment 40 bytes {int64 l_off, r_off; int32 l_ld, r_ld, K;
- * uint8 l_space, r_space, l_t, r_t; double c}; tile 40 bytes {int64 out_off;
- * int32 out_ld, m0, n0, mt, nt, seg_begin, seg_end, out_space}; merge term
- * 32 bytes {int64 dst_off, src_off, n; double coef}. */
+ * sizes[13] = {n_seg1, n_tile1, n_seg2, n_tile2, n_merge, arena_elems,
+ * scratch_elems, dim, sys_base, env_base, merged_base, n_rtile, n_slots}.
+ * Table element layouts: segment 40 bytes {int64 l_off, r_off; int32 l_ld,
+ * r_ld, K; uint8 l_space, r_space, l_t, r_t; double c}; tile 48 bytes
+ * {int64 out_off; int32 out_ld, m0, n0, mt, nt, seg_begin, seg_end,
+ * out_space, slot, reserved}; merge term 32 bytes {int64 dst_off, src_off, n; double
+ * coef}.  A stage-2 tile with slot >= 0 writes its 32x32 partial tile to
+ * slot `slot`; reduce tiles (rtile, tile layout) sum slots [seg_begin,
+ * seg_end) in order into their output tile. */
 typedef struct tjsq_hostplan tjsq_hostplan;
 int tjsq_hostplan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys,
                          const tjsq_opset* env, const tjsq_basis* sys_bs, const tjsq_basis* env_bs, int nw,
                          const int* wsec, tjsq_hostplan** out);
 int tjsq_hostplan_sizes(const tjsq_hostplan* h, int64_t* sizes);
-int tjsq_hostplan_tables(const tjsq_hostplan* h, void* seg1, void* tile1, void* seg2, void* tile2, void* merge);
+int tjsq_hostplan_tables(const tjsq_hostplan* h, void* seg1, void* tile1, void* seg2, void* tile2, void* merge,
+                         void* rtile);
 int tjsq_hostplan_stats(const tjsq_hostplan* h, tjsq_plan_stats* stats);
 int tjsq_hostplan_destroy(tjsq_hostplan* h);
 

@@ -27,6 +27,7 @@ def evaluate(hp, sb, x):
         arena[d:d + n] = arena[d:d + n] + m["coef"] * arena[s:s + n]
     T = np.zeros(max(1, hp.scratch_elems))
     y = np.zeros(hp.dim)
+    part = np.zeros(max(1, hp.nslots) * 32 * 32)
     spaces = {SP_OPS: arena, SP_X: x, SP_T: T}
     for stage, tiles, segs, out in ((1, hp.tile1, hp.seg1, T), (2, hp.tile2, hp.seg2, y)):
         for tl in tiles:
@@ -45,7 +46,18 @@ def evaluate(hp, sb, x):
                 Lm = _mat(L, lo, int(s["l_ld"]), mt, K, lt)
                 Rm = _mat(R, ro, int(s["r_ld"]), K, nt, rt)
                 acc += (s["c"] * Lm) @ Rm
+            if int(tl["slot"]) >= 0:
+                idx = int(tl["slot"]) * 1024 + np.arange(mt)[:, None] + np.arange(nt)[None, :] * 32
+                part[idx] = acc
+                continue
             o, ld = int(tl["out_off"]), int(tl["out_ld"])
             idx = o + (m0 + np.arange(mt))[:, None] + (n0 + np.arange(nt))[None, :] * ld
             out[idx] = acc
+    for tl in hp.rtile:   # ordered sums of the partial tiles
+        m0, n0, mt, nt = int(tl["m0"]), int(tl["n0"]), int(tl["mt"]), int(tl["nt"])
+        acc = np.zeros((mt, nt))
+        for q in range(int(tl["seg_begin"]), int(tl["seg_end"])):
+            acc += part[q * 1024 + np.arange(mt)[:, None] + np.arange(nt)[None, :] * 32]
+        o, ld = int(tl["out_off"]), int(tl["out_ld"])
+        y[o + (m0 + np.arange(mt))[:, None] + (n0 + np.arange(nt))[None, :] * ld] = acc
     return y
