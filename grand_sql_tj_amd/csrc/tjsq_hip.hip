@@ -51,7 +51,7 @@ int fail(int code, const std::string& m) {
   } while (0)
 
 constexpr int TS = 32;      // output tile edge
-constexpr int KC = 16;      // K chunk staged per step
+constexpr int KC = 32;      // K chunk staged per step
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -208,19 +208,23 @@ __global__ __launch_bounds__(64) void k_tiles(const Tile* __restrict__ tiles, co
       const bool more = nsi < se;
       if (more) fetch_chunk<MODE>(S, nxt, tl, nk0, lane, rl, rr);
       if constexpr (MODE == MODE_BF16X3) {
-        s4 a[2][2], b[2][2];
-        for (int p = 0; p < 2; ++p)
-          for (int h = 0; h < 2; ++h)
-            for (int t = 0; t < 4; ++t) {
-              a[p][h][t] = (short)Lh[h * TS * (KC + 4) + (p * 16 + lr) * (KC + 4) + lq * 4 + t];
-              b[p][h][t] = (short)Rh[h * TS * (KC + 4) + (p * 16 + lr) * (KC + 4) + lq * 4 + t];
+#pragma unroll
+        for (int kb = 0; kb < KC / 16; ++kb) {
+          // one 16x16x16 step: lane supplies k = kb*16 + 4*lq .. +3
+          s4 a[2][2], b[2][2];
+          for (int p = 0; p < 2; ++p)
+            for (int h = 0; h < 2; ++h)
+              for (int t = 0; t < 4; ++t) {
+                a[p][h][t] = (short)Lh[h * TS * (KC + 4) + (p * 16 + lr) * (KC + 4) + kb * 16 + lq * 4 + t];
+                b[p][h][t] = (short)Rh[h * TS * (KC + 4) + (p * 16 + lr) * (KC + 4) + kb * 16 + lq * 4 + t];
+              }
+          for (int bi = 0; bi < 2; ++bi)
+            for (int bj = 0; bj < 2; ++bj) {
+              acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][1], b[bj][0], acc[bi][bj], 0, 0, 0);
+              acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][0], b[bj][1], acc[bi][bj], 0, 0, 0);
+              acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][0], b[bj][0], acc[bi][bj], 0, 0, 0);
             }
-        for (int bi = 0; bi < 2; ++bi)
-          for (int bj = 0; bj < 2; ++bj) {
-            acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][1], b[bj][0], acc[bi][bj], 0, 0, 0);
-            acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][0], b[bj][1], acc[bi][bj], 0, 0, 0);
-            acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][0], b[bj][0], acc[bi][bj], 0, 0, 0);
-          }
+        }
       } else {
 #pragma unroll
         for (int kk = 0; kk < KC / 4; ++kk) {
@@ -312,54 +316,58 @@ __global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ b
 constexpr int RED_BLOCKS = 512;
 constexpr int RED_THREADS = 256;
 
-// partial[b * nv + j] = sum over block b's range of V_j . w  (j < nv), and,
-// when with_self, partial[(RED_BLOCKS + b) * nv + j] = V_j . V_j
+// partial[b * nv + j] = sum over block b's slice of V_j . w  (j < nv), and,
+// when with_self, partial[(RED_BLOCKS + b) * nv + j] = V_j . V_j.  Each block
+// owns one contiguous slice of the vectors; the slice of w stays in cache
+// while the block walks the nv vectors.
+template <typename T>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < RED_THREADS / 64; ++q) t += red[q];
+  __syncthreads();
+  return t;
+}
+
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_multidot(const T* __restrict__ V, int64_t ldv, int nv,
                                                           const T* __restrict__ w, int64_t n, double* partial,
                                                           int with_self) {
-  __shared__ double red[RED_THREADS / 64][64];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int j0 = 0; j0 < nv; j0 += 32) {
-    const int nj = min(32, nv - j0);
-    double s[32], ss[32];
-    for (int j = 0; j < 32; ++j) { s[j] = 0.0; ss[j] = 0.0; }
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-      const double wi = (double)w[i];
-      for (int j = 0; j < nj; ++j) {
-        const double v = (double)V[(int64_t)(j0 + j) * ldv + i];
-        s[j] += v * wi;
-        if (with_self) ss[j] += v * v;
-      }
+  __shared__ double red[RED_THREADS / 64];
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * per, hi = min(n, lo + per);
+  for (int j = 0; j < nv; ++j) {
+    const T* v = V + (int64_t)j * ldv;
+    double s = 0.0, ss = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += RED_THREADS) {
+      const double vi = (double)v[i];
+      s += vi * (double)w[i];
+      ss += vi * vi;
     }
-    for (int pass = 0; pass < (with_self ? 2 : 1); ++pass) {
-      for (int j = 0; j < nj; ++j) {
-        double v = pass ? ss[j] : s[j];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-        if (lane == 0) red[wid][j] = v;
-      }
-      __syncthreads();
-      if (threadIdx.x < nj) {
-        double t = 0.0;
-        for (int q = 0; q < RED_THREADS / 64; ++q) t += red[q][threadIdx.x];
-        partial[((int64_t)pass * RED_BLOCKS + blockIdx.x) * nv + j0 + threadIdx.x] = t;
-      }
-      __syncthreads();
+    const double ts = block_sum<T>(s, red);
+    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * nv + j] = ts;
+    if (with_self) {
+      const double tss = block_sum<T>(ss, red);
+      if (threadIdx.x == 0) partial[((int64_t)RED_BLOCKS + blockIdx.x) * nv + j] = tss;
     }
   }
 }
 
-__global__ void k_finish_dots(const double* partial, int nv, int nblocks, int with_self, double* out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nv) return;
+// out[j] = sum_b partial[b * nv + j] (fixed tree order); block j (< nv) or
+// nv + j for the self products
+__global__ __launch_bounds__(RED_THREADS) void k_finish_dots(const double* partial, int nv, int nblocks, int with_self,
+                                                             double* out) {
+  __shared__ double red[RED_THREADS / 64];
+  const int j = blockIdx.x % nv, pass = blockIdx.x / nv;
+  if (pass == 1 && !with_self) return;
   double t = 0.0;
-  for (int b = 0; b < nblocks; ++b) t += partial[(int64_t)b * nv + j];
-  out[j] = t;
-  if (with_self) {
-    double u = 0.0;
-    for (int b = 0; b < nblocks; ++b) u += partial[((int64_t)RED_BLOCKS + b) * nv + j];
-    out[nv + j] = u;
-  }
+  for (int b = threadIdx.x; b < nblocks; b += RED_THREADS) t += partial[((int64_t)pass * RED_BLOCKS + b) * nv + j];
+  const double u = block_sum<double>(t, red);
+  if (threadIdx.x == 0) out[pass * nv + j] = u;
 }
 
 // w = (w - sum_j coef_j V_j) * post_scale ; coefficients from device memory
@@ -837,8 +845,8 @@ struct Krylov {
     hipLaunchKernelGGL(k_multidot<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, base, n, nv, w, n, partial,
                        self ? 1 : 0);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_finish_dots, dim3((nv + 63) / 64), dim3(64), 0, st, partial, nv, RED_BLOCKS, self ? 1 : 0,
-                       dots);
+    hipLaunchKernelGGL(k_finish_dots, dim3(nv * (self ? 2 : 1)), dim3(RED_THREADS), 0, st, partial, nv, RED_BLOCKS,
+                       self ? 1 : 0, dots);
     HIPCHK(hipGetLastError());
     out.resize(self ? 2 * nv : nv);
     HIPCHK(hipMemcpyAsync(out.data(), dots, sizeof(double) * out.size(), hipMemcpyDeviceToHost, st));

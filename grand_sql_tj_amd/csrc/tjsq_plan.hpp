@@ -135,9 +135,9 @@ struct HostPlan {
 // of at most `item_chunks` K-chunks of `kc` (an axpy counts one chunk).
 struct Shape {
   int tile = 32;
-  int kc = 16;
+  int kc = 32;
   int k_split = 256;
-  int item_chunks = 64;
+  int item_chunks = 32;
 };
 
 int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjsq_opset& sys,
