@@ -228,10 +228,10 @@ class HostPlan:
                                          C.byref(ebs), int(sb.wsec.shape[0]), C.c_void_p(w), C.byref(h)),
               "tjsq_hostplan_create")
         self.h = h
-        sizes = np.zeros(13, np.int64)
+        sizes = np.zeros(15, np.int64)
         check(lib().tjsq_hostplan_sizes(h, sizes.ctypes.data), "tjsq_hostplan_sizes")
         (n1, t1, n2, t2, nm, self.arena_elems, self.scratch_elems, self.dim, self.sys_base, self.env_base,
-         self.merged_base, nr, self.nslots) = [int(v) for v in sizes]
+         self.merged_base, nr, self.nslots, self.n64_1, self.n64_2) = [int(v) for v in sizes]
         self.seg1 = np.zeros(n1, SEG_DTYPE)
         self.tile1 = np.zeros(t1, TILE_DTYPE)
         self.seg2 = np.zeros(n2, SEG_DTYPE)

@@ -50,7 +50,6 @@ int fail(int code, const std::string& m) {
     if (e_ != hipSuccess) return fail(TJSQ_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
   } while (0)
 
-constexpr int TS = 32;      // output tile edge
 constexpr int KC = 32;      // K chunk staged per step
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -103,23 +102,30 @@ __device__ __forceinline__ int acc_row(int lq, int r) {
   return MODE == MODE_F64 ? lq + 4 * r : lq * 4 + r;
 }
 
-template <int MODE, typename T>
-__device__ __forceinline__ void fetch_chunk(const Spaces<T>& S, const Seg& s, const Tile& tl, int k0, int lane,
+template <int TSZ>
+struct TileCfg {
+  static constexpr int NT = TSZ == 64 ? 256 : 64;   // threads: one wave per 32x32 quarter
+  static constexpr int PER = TSZ * KC / NT;          // operand elements per thread per chunk
+};
+
+template <int MODE, int TSZ, typename T>
+__device__ __forceinline__ void fetch_chunk(const Spaces<T>& S, const Seg& s, const Tile& tl, int k0, int tid,
                                             T* rl, T* rr) {
+  constexpr int NT = TileCfg<TSZ>::NT, PER = TileCfg<TSZ>::PER;
   const T* L = space_ptr(S, s.l_space);
   const T* R = space_ptr(S, s.r_space);
 #pragma unroll
-  for (int q = 0; q < (TS * KC) / 64; ++q) {
-    const int e = lane + 64 * q;
+  for (int q = 0; q < PER; ++q) {
+    const int e = tid + NT * q;
     int i, k;
-    if (!s.l_t) { i = e & (TS - 1); k = e / TS; } else { k = e & (KC - 1); i = e / KC; }
+    if (!s.l_t) { i = e % TSZ; k = e / TSZ; } else { k = e % KC; i = e / KC; }
     T v = T(0);
     if (i < tl.mt && k0 + k < s.K)
       v = s.l_t ? L[s.l_off + (k0 + k) + (int64_t)(tl.m0 + i) * s.l_ld]
                 : L[s.l_off + (tl.m0 + i) + (int64_t)(k0 + k) * s.l_ld];
     rl[q] = v;
     int j;
-    if (!s.r_t) { k = e & (KC - 1); j = e / KC; } else { j = e & (TS - 1); k = e / TS; }
+    if (!s.r_t) { k = e % KC; j = e / KC; } else { j = e % TSZ; k = e / TSZ; }
     T w = T(0);
     if (j < tl.nt && k0 + k < s.K)
       w = s.r_t ? R[s.r_off + (tl.n0 + j) + (int64_t)(k0 + k) * s.r_ld]
@@ -128,22 +134,29 @@ __device__ __forceinline__ void fetch_chunk(const Spaces<T>& S, const Seg& s, co
   }
 }
 
-template <int MODE>
-__global__ __launch_bounds__(64) void k_tiles(const Tile* __restrict__ tiles, const Seg* __restrict__ segs,
-                                              Spaces<typename KT<MODE>::T> S, typename KT<MODE>::T* partial,
-                                              int ntiles) {
+// TSZ x TSZ output tile per workgroup; each wave computes a 32x32 quarter
+// (TSZ = 64: 4 waves share the LDS-staged operand panels).
+template <int MODE, int TSZ>
+__global__ __launch_bounds__(TileCfg<TSZ>::NT) void k_tiles(const Tile* __restrict__ tiles,
+                                                            const Seg* __restrict__ segs,
+                                                            Spaces<typename KT<MODE>::T> S,
+                                                            typename KT<MODE>::T* partial, int ntiles) {
   using T = typename KT<MODE>::T;
   using Acc = typename KT<MODE>::Acc;
-  const int tid = blockIdx.x;
-  if (tid >= ntiles) return;
-  const Tile tl = tiles[tid];
-  const int lane = threadIdx.x;
+  constexpr int PER = TileCfg<TSZ>::PER;
+  const int bid = blockIdx.x;
+  if (bid >= ntiles) return;
+  const Tile tl = tiles[bid];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (TSZ == 64) ? (wave >> 1) * 32 : 0, wn = (TSZ == 64) ? (wave & 1) * 32 : 0;
   const int lr = lane & 15, lq = lane >> 4;
-  constexpr int LDS_T = MODE == MODE_BF16X3 ? 1 : TS * (KC + 1);
-  constexpr int LDS_R = MODE == MODE_BF16X3 ? 1 : KC * (TS + 1);
+  constexpr int LDS_T = MODE == MODE_BF16X3 ? 1 : TSZ * (KC + 1);
+  constexpr int LDS_R = MODE == MODE_BF16X3 ? 1 : KC * (TSZ + 1);
   __shared__ T Ls[LDS_T];
   __shared__ T Rs[LDS_R];
-  constexpr int LDS_H = MODE == MODE_BF16X3 ? 2 * TS * (KC + 4) : 1;
+  constexpr int HS = KC + 4;                        // bf16 row stride (k fastest)
+  constexpr int LDS_H = MODE == MODE_BF16X3 ? 2 * TSZ * HS : 1;
   __shared__ unsigned short Lh[LDS_H];
   __shared__ unsigned short Rh[LDS_H];
   Acc acc[2][2];
@@ -153,8 +166,7 @@ __global__ __launch_bounds__(64) void k_tiles(const Tile* __restrict__ tiles, co
 
   int si = tl.seg_begin;
   const int se = tl.seg_end;
-  // axpy segments
-  for (; si < se; ++si) {
+  for (; si < se; ++si) {  // axpy segments lead every list
     const Seg s = segs[si];
     if (s.r_space != tjsq::SP_IDENT) break;
     const T* L = space_ptr(S, s.l_space);
@@ -162,7 +174,7 @@ __global__ __launch_bounds__(64) void k_tiles(const Tile* __restrict__ tiles, co
     for (int bi = 0; bi < 2; ++bi)
       for (int bj = 0; bj < 2; ++bj)
         for (int r = 0; r < 4; ++r) {
-          const int row = bi * 16 + acc_row<MODE>(lq, r), col = bj * 16 + lr;
+          const int row = wm + bi * 16 + acc_row<MODE>(lq, r), col = wn + bj * 16 + lr;
           if (row < tl.mt && col < tl.nt)
             acc[bi][bj][r] += c * L[s.l_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * s.l_ld];
         }
@@ -170,34 +182,32 @@ __global__ __launch_bounds__(64) void k_tiles(const Tile* __restrict__ tiles, co
   if (si < se) {
     Seg cur = segs[si];
     int k0 = 0;
-    T rl[(TS * KC) / 64], rr[(TS * KC) / 64];
-    fetch_chunk<MODE>(S, cur, tl, 0, lane, rl, rr);
+    T rl[PER], rr[PER];
+    fetch_chunk<MODE, TSZ>(S, cur, tl, 0, tid, rl, rr);
     for (;;) {
-      // stage the fetched chunk in LDS
       const T c = (T)cur.c;
 #pragma unroll
-      for (int q = 0; q < (TS * KC) / 64; ++q) {
-        const int e = lane + 64 * q;
+      for (int q = 0; q < PER; ++q) {
+        const int e = tid + TileCfg<TSZ>::NT * q;
         int i, k;
-        if (!cur.l_t) { i = e & (TS - 1); k = e / TS; } else { k = e & (KC - 1); i = e / KC; }
+        if (!cur.l_t) { i = e % TSZ; k = e / TSZ; } else { k = e % KC; i = e / KC; }
         int j, kr;
-        if (!cur.r_t) { kr = e & (KC - 1); j = e / KC; } else { j = e & (TS - 1); kr = e / TS; }
+        if (!cur.r_t) { kr = e % KC; j = e / KC; } else { j = e % TSZ; kr = e / TSZ; }
         if constexpr (MODE == MODE_BF16X3) {
           const float v = rl[q] * c;
           const unsigned short h = bf16_rne(v);
-          Lh[i * (KC + 4) + k] = h;
-          Lh[TS * (KC + 4) + i * (KC + 4) + k] = bf16_rne(v - bf16_to_f(h));
+          Lh[i * HS + k] = h;
+          Lh[TSZ * HS + i * HS + k] = bf16_rne(v - bf16_to_f(h));
           const float w = rr[q];
           const unsigned short g = bf16_rne(w);
-          Rh[j * (KC + 4) + kr] = g;
-          Rh[TS * (KC + 4) + j * (KC + 4) + kr] = bf16_rne(w - bf16_to_f(g));
+          Rh[j * HS + kr] = g;
+          Rh[TSZ * HS + j * HS + kr] = bf16_rne(w - bf16_to_f(g));
         } else {
           Ls[i * (KC + 1) + k] = rl[q] * c;
-          Rs[kr * (TS + 1) + j] = rr[q];
+          Rs[kr * (TSZ + 1) + j] = rr[q];
         }
       }
       __syncthreads();
-      // next chunk position; its loads overlap this chunk's MFMAs
       int nk0 = k0 + KC, nsi = si;
       Seg nxt = cur;
       if (nk0 >= cur.K) {
@@ -206,17 +216,16 @@ __global__ __launch_bounds__(64) void k_tiles(const Tile* __restrict__ tiles, co
         if (nsi < se) nxt = segs[nsi];
       }
       const bool more = nsi < se;
-      if (more) fetch_chunk<MODE>(S, nxt, tl, nk0, lane, rl, rr);
+      if (more) fetch_chunk<MODE, TSZ>(S, nxt, tl, nk0, tid, rl, rr);
       if constexpr (MODE == MODE_BF16X3) {
 #pragma unroll
         for (int kb = 0; kb < KC / 16; ++kb) {
-          // one 16x16x16 step: lane supplies k = kb*16 + 4*lq .. +3
           s4 a[2][2], b[2][2];
           for (int p = 0; p < 2; ++p)
             for (int h = 0; h < 2; ++h)
               for (int t = 0; t < 4; ++t) {
-                a[p][h][t] = (short)Lh[h * TS * (KC + 4) + (p * 16 + lr) * (KC + 4) + kb * 16 + lq * 4 + t];
-                b[p][h][t] = (short)Rh[h * TS * (KC + 4) + (p * 16 + lr) * (KC + 4) + kb * 16 + lq * 4 + t];
+                a[p][h][t] = (short)Lh[h * TSZ * HS + (wm + p * 16 + lr) * HS + kb * 16 + lq * 4 + t];
+                b[p][h][t] = (short)Rh[h * TSZ * HS + (wn + p * 16 + lr) * HS + kb * 16 + lq * 4 + t];
               }
           for (int bi = 0; bi < 2; ++bi)
             for (int bj = 0; bj < 2; ++bj) {
@@ -228,8 +237,8 @@ __global__ __launch_bounds__(64) void k_tiles(const Tile* __restrict__ tiles, co
       } else {
 #pragma unroll
         for (int kk = 0; kk < KC / 4; ++kk) {
-          const T a0 = Ls[lr * (KC + 1) + kk * 4 + lq], a1 = Ls[(16 + lr) * (KC + 1) + kk * 4 + lq];
-          const T b0 = Rs[(kk * 4 + lq) * (TS + 1) + lr], b1 = Rs[(kk * 4 + lq) * (TS + 1) + 16 + lr];
+          const T a0 = Ls[(wm + lr) * (KC + 1) + kk * 4 + lq], a1 = Ls[(wm + 16 + lr) * (KC + 1) + kk * 4 + lq];
+          const T b0 = Rs[(kk * 4 + lq) * (TSZ + 1) + wn + lr], b1 = Rs[(kk * 4 + lq) * (TSZ + 1) + wn + 16 + lr];
           if constexpr (MODE == MODE_F64) {
             acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
             acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
@@ -253,10 +262,10 @@ __global__ __launch_bounds__(64) void k_tiles(const Tile* __restrict__ tiles, co
   T* out;
   int64_t base;
   int ld;
-  if (tl.slot >= 0) {
+  if (tl.slot >= 0) {            // partial tile: slot counts units of 32x32 elements
     out = partial;
-    base = (int64_t)tl.slot * TS * TS;
-    ld = TS;
+    base = (int64_t)tl.slot * 1024;
+    ld = TSZ;
   } else {
     out = tl.out_space == tjsq::SP_T ? S.t : S.y;
     base = tl.out_off + tl.m0 + (int64_t)tl.n0 * tl.out_ld;
@@ -265,22 +274,25 @@ __global__ __launch_bounds__(64) void k_tiles(const Tile* __restrict__ tiles, co
   for (int bi = 0; bi < 2; ++bi)
     for (int bj = 0; bj < 2; ++bj)
       for (int r = 0; r < 4; ++r) {
-        const int row = bi * 16 + acc_row<MODE>(lq, r), col = bj * 16 + lr;
+        const int row = wm + bi * 16 + acc_row<MODE>(lq, r), col = wn + bj * 16 + lr;
         if (row < tl.mt && col < tl.nt) out[base + row + (int64_t)col * ld] = acc[bi][bj][r];
       }
 }
 
-// ordered sum of the partial tiles of split work items
+// ordered sum of the partial tiles of split work items; a reduce tile lists
+// its partial tiles as slot ranges [seg_begin, seg_end) of `slot` units each
+// (slot = 1 for 32x32 partials, 4 for 64x64) with leading dimension out_space
 template <typename T>
 __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ rt, const T* __restrict__ partial, T* y,
                                                 int n) {
   const int t = blockIdx.x;
   if (t >= n) return;
   const Tile tl = rt[t];
+  const int units = tl.slot, pld = tl.out_space;
   for (int e = threadIdx.x; e < tl.mt * tl.nt; e += blockDim.x) {
     const int row = e % tl.mt, col = e / tl.mt;
     T acc = T(0);
-    for (int q = tl.seg_begin; q < tl.seg_end; ++q) acc += partial[(int64_t)q * TS * TS + row + col * TS];
+    for (int q = tl.seg_begin; q < tl.seg_end; q += units) acc += partial[(int64_t)q * 1024 + row + col * pld];
     y[tl.out_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * tl.out_ld] = acc;
   }
 }
@@ -447,13 +459,23 @@ int launch_tiles(const tjsq_plan* p, int stage, const T* ops, const T* x, T* t, 
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, st));
   }
+  const int n64 = stage == 1 ? p->hp.n64_1 : p->hp.n64_2;
+  // 64x64 work items first in the table, then 32x32 ones
   if constexpr (sizeof(T) == 8) {
-    hipLaunchKernelGGL(k_tiles<MODE_F64>, dim3(n), dim3(64), 0, st, tiles, segs, S, p->part64, n);
+    if (n64) hipLaunchKernelGGL((k_tiles<MODE_F64, 64>), dim3(n64), dim3(256), 0, st, tiles, segs, S, p->part64, n64);
+    if (n > n64)
+      hipLaunchKernelGGL((k_tiles<MODE_F64, 32>), dim3(n - n64), dim3(64), 0, st, tiles + n64, segs, S, p->part64,
+                         n - n64);
+  } else if (precision == TJSQ_BF16X3) {
+    if (n64) hipLaunchKernelGGL((k_tiles<MODE_BF16X3, 64>), dim3(n64), dim3(256), 0, st, tiles, segs, S, p->part32, n64);
+    if (n > n64)
+      hipLaunchKernelGGL((k_tiles<MODE_BF16X3, 32>), dim3(n - n64), dim3(64), 0, st, tiles + n64, segs, S, p->part32,
+                         n - n64);
   } else {
-    if (precision == TJSQ_BF16X3)
-      hipLaunchKernelGGL(k_tiles<MODE_BF16X3>, dim3(n), dim3(64), 0, st, tiles, segs, S, p->part32, n);
-    else
-      hipLaunchKernelGGL(k_tiles<MODE_F32>, dim3(n), dim3(64), 0, st, tiles, segs, S, p->part32, n);
+    if (n64) hipLaunchKernelGGL((k_tiles<MODE_F32, 64>), dim3(n64), dim3(256), 0, st, tiles, segs, S, p->part32, n64);
+    if (n > n64)
+      hipLaunchKernelGGL((k_tiles<MODE_F32, 32>), dim3(n - n64), dim3(64), 0, st, tiles + n64, segs, S, p->part32,
+                         n - n64);
   }
   HIPCHK(hipGetLastError());
   if (stage == 2 && !p->hp.rtile.empty()) {
@@ -686,9 +708,9 @@ int tjsq_plan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tj
   if (put(&p->tile1, hp.tile1) || put(&p->tile2, hp.tile2) || put(&p->seg1, hp.seg1) || put(&p->seg2, hp.seg2) ||
       put(&p->rtile, hp.rtile))
     return cleanup(TJSQ_ERR_NOMEM, "task table allocation failed");
-  if (hipMalloc(&p->part64, sizeof(double) * std::max<int64_t>(1, hp.nslots * TS * TS)) != hipSuccess)
+  if (hipMalloc(&p->part64, sizeof(double) * std::max<int64_t>(1, hp.nslots * 1024)) != hipSuccess)
     return cleanup(TJSQ_ERR_NOMEM, "partial tile allocation failed");
-  if (p->arena32 && hipMalloc(&p->part32, sizeof(float) * std::max<int64_t>(1, hp.nslots * TS * TS)) != hipSuccess)
+  if (p->arena32 && hipMalloc(&p->part32, sizeof(float) * std::max<int64_t>(1, hp.nslots * 1024)) != hipSuccess)
     return cleanup(TJSQ_ERR_NOMEM, "partial tile allocation failed");
   HIPCHK(hipStreamSynchronize(st));
   *plan = p;
@@ -721,8 +743,8 @@ int tjsq_hostplan_sizes(const tjsq_hostplan* h, int64_t* sizes) {
   const HostPlan& hp = h->hp;
   const int64_t v[] = {(int64_t)hp.seg1.size(), (int64_t)hp.tile1.size(), (int64_t)hp.seg2.size(),
                        (int64_t)hp.tile2.size(), (int64_t)hp.merge.size(), hp.arena_elems, hp.scratch_elems, hp.dim,
-                       hp.sys_base, hp.env_base, hp.merged_base, (int64_t)hp.rtile.size(), hp.nslots};
-  std::copy(v, v + 13, sizes);
+                       hp.sys_base, hp.env_base, hp.merged_base, (int64_t)hp.rtile.size(), hp.nslots, hp.n64_1, hp.n64_2};
+  std::copy(v, v + 15, sizes);
   return TJSQ_OK;
 }
 

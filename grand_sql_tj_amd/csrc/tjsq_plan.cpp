@@ -929,7 +929,8 @@ int read_opset(const tjsq_opset& in, int64_t base, OpSetView& out, std::string& 
 int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjsq_opset& sys, const tjsq_opset& env,
                     const tjsq_basis& sys_bs, const tjsq_basis& env_bs, int nw, const int* wsec, const Shape& shape,
                     HostPlan& P, std::string& msg) {
-  const int tile = shape.tile;
+  auto edge = [&](int M, int N) { return (M > shape.small || N > shape.small) ? shape.big : shape.small; };
+  std::vector<Tile> t1big, t1small;
   P = HostPlan();
   P.sys_len = sys_len;
   P.env_len = env_len;
@@ -1176,10 +1177,12 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
         s.K = c.K1; s.c = 1.0;
         const int sidx = (int)P.seg1.size();
         P.seg1.push_back(s);
+        const int tile = edge(c.M, c.K2);
+        auto& dst = tile == shape.big ? t1big : t1small;
         for (int m0 = 0; m0 < c.M; m0 += tile)
           for (int n0 = 0; n0 < c.K2; n0 += tile)
-            P.tile1.push_back({toff, c.M, m0, n0, std::min(tile, c.M - m0), std::min(tile, c.K2 - n0), sidx, sidx + 1,
-                               SP_T, -1, 0});
+            dst.push_back({toff, c.M, m0, n0, std::min(tile, c.M - m0), std::min(tile, c.K2 - n0), sidx, sidx + 1,
+                           SP_T, -1, 0});
         fl_exec += 2.0 * c.M * c.K1 * c.K2;
       } else {
         t_of[q] = it->second;
@@ -1194,6 +1197,10 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
       opref.emplace_back(c.b_off, (int64_t)c.K2 * c.N);
     }
   }
+
+  P.n64_1 = (int)t1big.size();
+  P.tile1 = t1big;
+  P.tile1.insert(P.tile1.end(), t1small.begin(), t1small.end());
 
   // ---- stage 2: pull segments per output block ----------------------------
   std::vector<int> order(P.contrib.size());
@@ -1233,6 +1240,7 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
   // work items: per output block, axpy segments first, then GEMM segments cut
   // into K pieces of at most k_split and grouped into items of bounded work
   std::vector<Seg> pieces;
+  std::vector<Tile> t2big, t2small;
   pieces.reserve(P.seg2.size() * 2);
   for (size_t b = 0; b < P.out_blocks.size(); ++b) {
     const OutBlock& ob = P.out_blocks[b];
@@ -1263,32 +1271,43 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
       cost += c;
     }
     items.push_back({ib, last});  // also the empty item of a block without terms
+    const int tile = edge(ob.M, ob.N);
+    const int units = (tile / 32) * (tile / 32);
+    auto& dst = tile == shape.big ? t2big : t2small;
     for (int m0 = 0; m0 < ob.M; m0 += tile)
       for (int n0 = 0; n0 < ob.N; n0 += tile) {
         const int mt = std::min(tile, ob.M - m0), nt = std::min(tile, ob.N - n0);
         if (items.size() == 1) {
-          P.tile2.push_back({ob.off, ob.ld, m0, n0, mt, nt, items[0].first, items[0].second, SP_X, -1, 0});
+          dst.push_back({ob.off, ob.ld, m0, n0, mt, nt, items[0].first, items[0].second, SP_X, -1, 0});
           continue;
         }
         const int s0 = (int)P.nslots;
-        for (auto& it : items) P.tile2.push_back({ob.off, ob.ld, m0, n0, mt, nt, it.first, it.second, SP_X, (int)P.nslots++, 0});
-        P.rtile.push_back({ob.off, ob.ld, m0, n0, mt, nt, s0, (int)P.nslots, SP_X, -1, 0});
+        for (auto& it : items) {
+          dst.push_back({ob.off, ob.ld, m0, n0, mt, nt, it.first, it.second, SP_X, (int)P.nslots, 0});
+          P.nslots += units;
+        }
+        // reduce tile: slot = units per partial, out_space = partial leading dimension
+        P.rtile.push_back({ob.off, ob.ld, m0, n0, mt, nt, s0, (int)P.nslots, tile, units, 0});
       }
   }
   P.seg2.swap(pieces);
-  // longest work items first: the grid drains with fewer stragglers
-  {
-    std::vector<int64_t> cost(P.tile2.size(), 0);
-    for (size_t t = 0; t < P.tile2.size(); ++t)
-      for (int q = P.tile2[t].seg_begin; q < P.tile2[t].seg_end; ++q)
-        cost[t] += P.seg2[q].r_space == SP_IDENT ? 1 : P.seg2[q].K;
-    std::vector<int> idx(P.tile2.size());
+  // longest work items first within each tile size: fewer stragglers
+  auto by_cost = [&](std::vector<Tile>& v) {
+    std::vector<int64_t> cost(v.size(), 0);
+    for (size_t t = 0; t < v.size(); ++t)
+      for (int q = v[t].seg_begin; q < v[t].seg_end; ++q) cost[t] += P.seg2[q].r_space == SP_IDENT ? 1 : P.seg2[q].K;
+    std::vector<int> idx(v.size());
     for (size_t t = 0; t < idx.size(); ++t) idx[t] = (int)t;
     std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    std::vector<Tile> sorted(P.tile2.size());
-    for (size_t t = 0; t < idx.size(); ++t) sorted[t] = P.tile2[idx[t]];
-    P.tile2.swap(sorted);
-  }
+    std::vector<Tile> sorted(v.size());
+    for (size_t t = 0; t < idx.size(); ++t) sorted[t] = v[idx[t]];
+    v.swap(sorted);
+  };
+  by_cost(t2big);
+  by_cost(t2small);
+  P.n64_2 = (int)t2big.size();
+  P.tile2 = t2big;
+  P.tile2.insert(P.tile2.end(), t2small.begin(), t2small.end());
 
   std::sort(opref.begin(), opref.end());
   opref.erase(std::unique(opref.begin(), opref.end()), opref.end());

@@ -123,7 +123,8 @@ struct HostPlan {
   // stage-2 work items that share an output tile write partial tiles; rtile
   // sums slots [seg_begin, seg_end) of each such tile in order
   std::vector<Tile> rtile;
-  int64_t nslots = 0;
+  int64_t nslots = 0;           // partial storage in units of 32x32 elements
+  int n64_1 = 0, n64_2 = 0;     // leading 64x64 work items of tile1 / tile2
   int64_t scratch_elems = 0;
   tjsq_plan_stats stats{};
   std::vector<int64_t> ops_used; // distinct operator sub-blocks referenced (elements)
@@ -134,7 +135,8 @@ struct HostPlan {
 // pieces and consecutive pieces of an output tile are grouped into work items
 // of at most `item_chunks` K-chunks of `kc` (an axpy counts one chunk).
 struct Shape {
-  int tile = 32;
+  int big = 64;             // tile edge for blocks with a side above `small`
+  int small = 32;
   int kc = 32;
   int k_split = 256;
   int item_chunks = 32;

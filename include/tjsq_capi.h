@@ -194,15 +194,18 @@ int tjsq_lanczos_solve(tjsq_plan* plan, double* psi, const tjsq_lanczos_opts* op
 
 /* ---- host-only plan inspection (no GPU needed) --------------------------
  * The task tables a plan would upload, for tests of the plan compiler.
- * sizes[13] = {n_seg1, n_tile1, n_seg2, n_tile2, n_merge, arena_elems,
- * scratch_elems, dim, sys_base, env_base, merged_base, n_rtile, n_slots}.
+ * sizes[15] = {n_seg1, n_tile1, n_seg2, n_tile2, n_merge, arena_elems,
+ * scratch_elems, dim, sys_base, env_base, merged_base, n_rtile, n_slots,
+ * n64_1, n64_2}; the first n64_s work items of stage s are 64x64 tiles, the
+ * rest 32x32.
  * Table element layouts: segment 40 bytes {int64 l_off, r_off; int32 l_ld,
  * r_ld, K; uint8 l_space, r_space, l_t, r_t; double c}; tile 48 bytes
  * {int64 out_off; int32 out_ld, m0, n0, mt, nt, seg_begin, seg_end,
  * out_space, slot, reserved}; merge term 32 bytes {int64 dst_off, src_off, n; double
- * coef}.  A stage-2 tile with slot >= 0 writes its 32x32 partial tile to
- * slot `slot`; reduce tiles (rtile, tile layout) sum slots [seg_begin,
- * seg_end) in order into their output tile. */
+ * coef}.  A stage-2 tile with slot >= 0 writes its partial tile (leading
+ * dimension = its edge) at element 1024 * slot; reduce tiles (rtile, tile
+ * layout) sum the partials at units [seg_begin, seg_end) in steps of `slot`
+ * units, leading dimension out_space, in order into their output tile. */
 typedef struct tjsq_hostplan tjsq_hostplan;
 int tjsq_hostplan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys,
                          const tjsq_opset* env, const tjsq_basis* sys_bs, const tjsq_basis* env_bs, int nw,

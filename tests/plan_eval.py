@@ -27,10 +27,12 @@ def evaluate(hp, sb, x):
         arena[d:d + n] = arena[d:d + n] + m["coef"] * arena[s:s + n]
     T = np.zeros(max(1, hp.scratch_elems))
     y = np.zeros(hp.dim)
-    part = np.zeros(max(1, hp.nslots) * 32 * 32)
+    part = np.zeros(max(1, hp.nslots) * 1024)
+    n64 = {1: hp.n64_1, 2: hp.n64_2}
     spaces = {SP_OPS: arena, SP_X: x, SP_T: T}
     for stage, tiles, segs, out in ((1, hp.tile1, hp.seg1, T), (2, hp.tile2, hp.seg2, y)):
-        for tl in tiles:
+        for ti, tl in enumerate(tiles):
+            edge = 64 if ti < n64[stage] else 32
             m0, n0, mt, nt = int(tl["m0"]), int(tl["n0"]), int(tl["mt"]), int(tl["nt"])
             acc = np.zeros((mt, nt))
             for s in segs[int(tl["seg_begin"]):int(tl["seg_end"])]:
@@ -47,7 +49,7 @@ def evaluate(hp, sb, x):
                 Rm = _mat(R, ro, int(s["r_ld"]), K, nt, rt)
                 acc += (s["c"] * Lm) @ Rm
             if int(tl["slot"]) >= 0:
-                idx = int(tl["slot"]) * 1024 + np.arange(mt)[:, None] + np.arange(nt)[None, :] * 32
+                idx = int(tl["slot"]) * 1024 + np.arange(mt)[:, None] + np.arange(nt)[None, :] * edge
                 part[idx] = acc
                 continue
             o, ld = int(tl["out_off"]), int(tl["out_ld"])
@@ -56,8 +58,9 @@ def evaluate(hp, sb, x):
     for tl in hp.rtile:   # ordered sums of the partial tiles
         m0, n0, mt, nt = int(tl["m0"]), int(tl["n0"]), int(tl["mt"]), int(tl["nt"])
         acc = np.zeros((mt, nt))
-        for q in range(int(tl["seg_begin"]), int(tl["seg_end"])):
-            acc += part[q * 1024 + np.arange(mt)[:, None] + np.arange(nt)[None, :] * 32]
+        units, pld = int(tl["slot"]), int(tl["out_space"])
+        for q in range(int(tl["seg_begin"]), int(tl["seg_end"]), units):
+            acc += part[q * 1024 + np.arange(mt)[:, None] + np.arange(nt)[None, :] * pld]
         o, ld = int(tl["out_off"]), int(tl["out_ld"])
         y[o + (m0 + np.arange(mt))[:, None] + (n0 + np.arange(nt))[None, :] * ld] = acc
     return y
