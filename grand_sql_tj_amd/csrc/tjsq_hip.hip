@@ -906,7 +906,8 @@ int apply_prec(tjsq_plan* p, int precision, const void* x, void* y, hipStream_t 
 // number of vectors, the Ritz value and the tridiagonal data.
 template <typename T>
 int lanczos_pass(Krylov<T>& K, int precision, double tol, int max_iter, int& last_out, double& ritz_out,
-                 std::vector<double>& alpha, std::vector<double>& beta, int& breakdown, int& applies) {
+                 std::vector<double>& alpha, std::vector<double>& beta, int& breakdown, int& applies,
+                 double rtol = 0.0, double* theta2 = nullptr) {
   alpha.assign(max_iter + 1, 0.0);
   beta.assign(max_iter + 1, 0.0);
   breakdown = 0;
@@ -953,8 +954,9 @@ int lanczos_pass(Krylov<T>& K, int precision, double tol, int max_iter, int& las
     std::vector<double> dia(alpha.begin() + 1, alpha.begin() + last + 1);
     std::vector<double> off(beta.begin() + 1, beta.begin() + last + 1);
     tridiag_eig(last, dia, off, nullptr);
-    const bool done = std::fabs(dia[0] - ritz) <= tol || last == max_iter;
+    const bool done = std::fabs(dia[0] - ritz) <= std::max(tol, rtol * std::fabs(dia[0])) || last == max_iter;
     ritz = dia[0];
+    if (theta2) *theta2 = last > 1 ? dia[1] : 0.0;
     if (done) break;
   }
   last_out = last;
@@ -1019,7 +1021,7 @@ extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_
   int rc;
   std::vector<double> alpha, beta;
   int last = 0, brk = 0, applies = 0;
-  double ritz = 0.0, e0 = 0.0;
+  double ritz = 0.0, e0 = 0.0, theta2 = 0.0;
   if (!p->x64a) {
     if (hipMalloc(&p->x64a, sizeof(double) * n) != hipSuccess || hipMalloc(&p->x64b, sizeof(double) * n) != hipSuccess)
       return fail(TJSQ_ERR_NOMEM, "residual buffers allocation failed");
@@ -1031,7 +1033,9 @@ extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_
     double nrm;
     if ((rc = K.norm(K.vec(0), nrm))) return rc;
     if ((rc = K.scale(K.vec(0), 1.0 / nrm))) return rc;
-    if ((rc = lanczos_pass(K, TJSQ_FP64, opts->tol, opts->max_iter, last, ritz, alpha, beta, brk, applies))) return rc;
+    if ((rc = lanczos_pass(K, TJSQ_FP64, opts->tol, opts->max_iter, last, ritz, alpha, beta, brk, applies, 0.0,
+                           &theta2)))
+      return rc;
     if ((rc = assemble(K, last, alpha, beta, psi, e0))) return rc;
     res->e0 = e0;
     res->e_ritz = e0;
@@ -1043,7 +1047,9 @@ extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_
     double nrm;
     if ((rc = K.norm(K.vec(0), nrm))) return rc;
     if ((rc = K.scale(K.vec(0), 1.0 / nrm))) return rc;
-    if ((rc = lanczos_pass(K, opts->precision, opts->tol, opts->max_iter, last, ritz, alpha, beta, brk, applies)))
+    const double rtol = opts->mixed_rtol > 0 ? opts->mixed_rtol : 1e-7;
+    if ((rc = lanczos_pass(K, opts->precision, opts->tol, opts->max_iter, last, ritz, alpha, beta, brk, applies, rtol,
+                           &theta2)))
       return rc;
     float* out32 = K.vec(opts->max_iter + 1);
     if ((rc = assemble(K, last, alpha, beta, out32, e0))) return rc;
@@ -1073,7 +1079,11 @@ extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_
     if ((rc = K2.norm(p->x64a, r))) return rc;
     res->residual = r / std::sqrt(d[1]);
     if (opts->precision != TJSQ_FP64) res->e0 = rq;
-    if (opts->precision == TJSQ_FP64 || res->residual <= opts->refine_tol || refinements >= opts->max_refine) break;
+    res->gap = theta2 > rq ? theta2 - rq : 0.0;
+    res->energy_err_est = res->gap > 0 ? res->residual * res->residual / res->gap : INFINITY;
+    const bool good = opts->energy_tol > 0 ? res->energy_err_est <= opts->energy_tol
+                                           : res->residual <= opts->refine_tol;
+    if (opts->precision == TJSQ_FP64 || good || refinements >= opts->max_refine) break;
     // refinement restart: FP64 Lanczos from the current psi
     Krylov<double> K;
     if ((rc = ensure_krylov(p, opts->max_iter, K, st))) return rc;
@@ -1082,9 +1092,12 @@ extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_
     if ((rc = K.norm(K.vec(0), nrm))) return rc;
     if ((rc = K.scale(K.vec(0), 1.0 / nrm))) return rc;
     int last2 = 0, brk2 = 0;
-    if ((rc = lanczos_pass(K, TJSQ_FP64, opts->tol, opts->max_iter, last2, ritz, alpha, beta, brk2, applies))) return rc;
+    if ((rc = lanczos_pass(K, TJSQ_FP64, opts->tol, opts->max_iter, last2, ritz, alpha, beta, brk2, applies, 0.0,
+                           &theta2)))
+      return rc;
     if ((rc = assemble(K, last2, alpha, beta, psi, e0))) return rc;
     res->iterations += last2;
+    res->refine_iterations += last2;
     brk |= brk2;
     ++refinements;
     res->e0 = e0;

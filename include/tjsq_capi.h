@@ -122,6 +122,12 @@ typedef struct tjsq_lanczos_opts {
   double refine_tol;   /* FP64 residual |H psi - E psi| above which a mixed
                           solve restarts in FP64 from its own result */
   int max_refine;      /* FP64 restarts allowed after a mixed solve */
+  double energy_tol;   /* > 0: a mixed solve restarts in FP64 only when the
+                          estimated energy error |r|^2 / (theta_2 - E) of its
+                          FP64 Rayleigh quotient exceeds energy_tol (theta_2:
+                          second Ritz value); refine_tol is then unused */
+  double mixed_rtol;   /* relative Ritz-value change that ends a mixed pass
+                          (its rounding floor; 0 selects 1e-7) */
 } tjsq_lanczos_opts;
 
 typedef struct tjsq_lanczos_result {
@@ -133,6 +139,9 @@ typedef struct tjsq_lanczos_result {
   int applies;         /* H_eff applies, all precisions */
   int refinements;     /* FP64 restarts taken */
   int breakdown;       /* 1 if beta <= tol ended a pass (eigen.f90:74-77) */
+  int refine_iterations; /* Lanczos vectors of the FP64 restarts */
+  double gap;          /* theta_2 - E of the last pass (0 if unknown) */
+  double energy_err_est; /* |r|^2 / gap of the returned state */
   double seconds;      /* wall time of the solve */
 } tjsq_lanczos_result;
 

@@ -100,3 +100,9 @@ def test_lanczos_mixed_precision_energy(prec):
     nsite = sb.sys_len + sb.env_len + 2
     assert abs(r["e0"] - r64["e0"]) / nsite <= 1e-8
     assert r["residual"] <= 1e-5
+    # energy-targeted refinement: restart in FP64 only if |r|^2/gap > target
+    psi2 = _cuda(x0)
+    target = 1e-9 * nsite
+    r2 = p.lanczos(psi2, tol=1e-10, max_iter=80, precision=prec, max_refine=2, energy_tol=target)
+    assert abs(r2["e0"] - r64["e0"]) / nsite <= 1e-8
+    assert r2["energy_err_est"] <= target or r2["refinements"] == 2
