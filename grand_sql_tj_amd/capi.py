@@ -45,7 +45,7 @@ class PlanStats(C.Structure):
 
 class LanczosOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_int), ("precision", C.c_int), ("refine_tol", C.c_double),
-                ("max_refine", C.c_int), ("energy_tol", C.c_double), ("mixed_rtol", C.c_double)]
+                ("max_refine", C.c_int), ("energy_tol", C.c_double), ("rtol", C.c_double)]
 
 
 class LanczosResult(C.Structure):
@@ -303,7 +303,7 @@ class Plan:
         return y
 
     def lanczos(self, psi, tol=1e-10, max_iter=40, precision="fp64", refine_tol=1e-6, max_refine=1, energy_tol=0.0,
-                mixed_rtol=0.0):
+                rtol=0.0):
         """Ground state (lanczos_memo_new, levels = 1).  psi: CUDA float64
         start vector, overwritten with the normalised ground state."""
         torch = self.torch
@@ -312,7 +312,7 @@ class Plan:
         o = LanczosOpts()
         o.tol, o.max_iter, o.precision = tol, max_iter, PRECISIONS[precision]
         o.refine_tol, o.max_refine = refine_tol, max_refine
-        o.energy_tol, o.mixed_rtol = energy_tol, mixed_rtol
+        o.energy_tol, o.rtol = energy_tol, rtol
         r = LanczosResult()
         check(lib().tjsq_lanczos_solve(self.h, C.c_void_p(psi.data_ptr()), C.byref(o), C.byref(r), self._stream()),
               "tjsq_lanczos_solve")

@@ -1033,8 +1033,8 @@ extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_
     double nrm;
     if ((rc = K.norm(K.vec(0), nrm))) return rc;
     if ((rc = K.scale(K.vec(0), 1.0 / nrm))) return rc;
-    if ((rc = lanczos_pass(K, TJSQ_FP64, opts->tol, opts->max_iter, last, ritz, alpha, beta, brk, applies, 0.0,
-                           &theta2)))
+    if ((rc = lanczos_pass(K, TJSQ_FP64, opts->tol, opts->max_iter, last, ritz, alpha, beta, brk, applies,
+                           opts->rtol > 0 ? opts->rtol : 0.0, &theta2)))
       return rc;
     if ((rc = assemble(K, last, alpha, beta, psi, e0))) return rc;
     res->e0 = e0;
@@ -1047,7 +1047,7 @@ extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_
     double nrm;
     if ((rc = K.norm(K.vec(0), nrm))) return rc;
     if ((rc = K.scale(K.vec(0), 1.0 / nrm))) return rc;
-    const double rtol = opts->mixed_rtol > 0 ? opts->mixed_rtol : 1e-7;
+    const double rtol = opts->rtol > 0 ? opts->rtol : 1e-7;
     if ((rc = lanczos_pass(K, opts->precision, opts->tol, opts->max_iter, last, ritz, alpha, beta, brk, applies, rtol,
                            &theta2)))
       return rc;

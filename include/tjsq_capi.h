@@ -126,8 +126,10 @@ typedef struct tjsq_lanczos_opts {
                           estimated energy error |r|^2 / (theta_2 - E) of its
                           FP64 Rayleigh quotient exceeds energy_tol (theta_2:
                           second Ritz value); refine_tol is then unused */
-  double mixed_rtol;   /* relative Ritz-value change that ends a mixed pass
-                          (its rounding floor; 0 selects 1e-7) */
+  double rtol;         /* relative Ritz-value change that also ends a pass:
+                          |dE| <= max(tol, rtol |E|); 0 selects 1e-7 for
+                          mixed passes (their rounding floor) and none for
+                          FP64 passes */
 } tjsq_lanczos_opts;
 
 typedef struct tjsq_lanczos_result {
