@@ -87,13 +87,15 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--m", type=int, default=8000)
-    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x3"])
+    ap.add_argument("--precision", default="bf16x3", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--max-iter", type=int, default=40)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--warm-iter", type=int, default=200)
     ap.add_argument("--noise", type=float, default=0.01)
+    ap.add_argument("--apply-only", type=int, default=0,
+                    help="profiling mode: a 512 MiB device copy (counter calibration) then N isolated applies")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the solve is not sharded across GPUs in this build "
@@ -108,6 +110,22 @@ def main():
     print(f"[bench] building workload m={args.m}", file=sys.stderr, flush=True)
     model, sb = workload(args.m, args.seed)
     from grand_sql_tj_amd import synth
+    if args.apply_only:
+        plan = capi.Plan(model, sb)
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")   # 512 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        x = torch.from_numpy(synth.random_vector(sb, args.seed)).cuda()
+        if args.precision != "fp64":
+            x = x.float()
+        y = torch.empty_like(x)
+        for _ in range(args.apply_only):
+            plan.apply(x, y, precision=args.precision)
+        torch.cuda.synchronize()
+        print(json.dumps({"apply_only": args.apply_only, "precision": args.precision, "D": sb.dim,
+                          "stats": plan.stats, "copy_bytes": 512 * 1024 * 1024}), flush=True)
+        return
     plan = capi.Plan(model, sb)
     st = plan.stats
     nsite = sb.sys_len + sb.env_len + 2
