@@ -62,7 +62,7 @@ SEG_DTYPE = np.dtype([("l_off", "<i8"), ("r_off", "<i8"), ("l_ld", "<i4"), ("r_l
                       ("l_space", "u1"), ("r_space", "u1"), ("l_t", "u1"), ("r_t", "u1"), ("c", "<f8")])
 TILE_DTYPE = np.dtype([("out_off", "<i8"), ("out_ld", "<i4"), ("m0", "<i4"), ("n0", "<i4"), ("mt", "<i4"),
                        ("nt", "<i4"), ("seg_begin", "<i4"), ("seg_end", "<i4"), ("out_space", "<i4"), ("slot", "<i4"),
-                       ("reserved", "<i4")])
+                       ("group", "<i4")])
 MERGE_DTYPE = np.dtype([("dst_off", "<i8"), ("src_off", "<i8"), ("n", "<i8"), ("coef", "<f8")])
 assert SEG_DTYPE.itemsize == 40 and TILE_DTYPE.itemsize == 48 and MERGE_DTYPE.itemsize == 32
 

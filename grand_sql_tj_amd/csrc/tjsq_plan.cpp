@@ -968,7 +968,7 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
   em.wsys.resize(nw);
   em.wenv.resize(nw);
   em.wbase.resize(nw);
-  int nblk = 0;
+  int nblk = 0, ngroup = 0;
   for (int i = 0; i < nw; ++i) {
     const WSec& w = P.wsec[i];
     const int xs = em.find_sec(P.sys_bs, w.sys_up, w.sys_down), xe = em.find_sec(P.env_bs, w.env_up, w.env_down);
@@ -983,8 +983,12 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
     const MapSec& S = P.sys_bs.sec[xs];
     const MapSec& E = P.env_bs.sec[xe];
     for (const MapSub& ms : S.sub)
-      for (const MapSub& me : E.sub)
-        P.out_blocks.push_back({w.off + (int64_t)me.spos * w.sys_dim + ms.spos, w.sys_dim, ms.sdim, me.sdim});
+      for (size_t e = 0; e < E.sub.size(); ++e) {
+        const MapSub& me = E.sub[e];
+        P.out_blocks.push_back(
+            {w.off + (int64_t)me.spos * w.sys_dim + ms.spos, w.sys_dim, ms.sdim, me.sdim, ngroup + (int)e});
+      }
+    ngroup += (int)E.sub.size();
     nblk += (int)(S.sub.size() * E.sub.size());
   }
 
@@ -1158,6 +1162,7 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
   // ---- compile: stage 1 (T = op(A) X of two-sided contributions) ----------
   std::map<std::tuple<int64_t, int, int64_t, int, int>, int64_t> tmap;  // (a_off, a_t, x_off, M, K2) -> T offset
   std::vector<int64_t> t_of(P.contrib.size(), -1);
+  std::unordered_map<int64_t, int> agroup;  // stage-1 locality: the A operand
   double fl_ref = 0, fl_exec = 0;
   std::vector<std::pair<int64_t, int64_t>> opref;  // (off, elems) of referenced operator sub-blocks
   for (size_t q = 0; q < P.contrib.size(); ++q) {
@@ -1179,10 +1184,11 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
         P.seg1.push_back(s);
         const int tile = edge(c.M, c.K2);
         auto& dst = tile == shape.big ? t1big : t1small;
+        auto ga = agroup.emplace(c.a_off, (int)agroup.size()).first->second;
         for (int m0 = 0; m0 < c.M; m0 += tile)
           for (int n0 = 0; n0 < c.K2; n0 += tile)
             dst.push_back({toff, c.M, m0, n0, std::min(tile, c.M - m0), std::min(tile, c.K2 - n0), sidx, sidx + 1,
-                           SP_T, -1, 0});
+                           SP_T, -1, ga});
         fl_exec += 2.0 * c.M * c.K1 * c.K2;
       } else {
         t_of[q] = it->second;
@@ -1198,9 +1204,6 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
     }
   }
 
-  P.n64_1 = (int)t1big.size();
-  P.tile1 = t1big;
-  P.tile1.insert(P.tile1.end(), t1small.begin(), t1small.end());
 
   // ---- stage 2: pull segments per output block ----------------------------
   std::vector<int> order(P.contrib.size());
@@ -1278,33 +1281,70 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
       for (int n0 = 0; n0 < ob.N; n0 += tile) {
         const int mt = std::min(tile, ob.M - m0), nt = std::min(tile, ob.N - n0);
         if (items.size() == 1) {
-          dst.push_back({ob.off, ob.ld, m0, n0, mt, nt, items[0].first, items[0].second, SP_X, -1, 0});
+          dst.push_back({ob.off, ob.ld, m0, n0, mt, nt, items[0].first, items[0].second, SP_X, -1, ob.group});
           continue;
         }
         const int s0 = (int)P.nslots;
         for (auto& it : items) {
-          dst.push_back({ob.off, ob.ld, m0, n0, mt, nt, it.first, it.second, SP_X, (int)P.nslots, 0});
+          dst.push_back({ob.off, ob.ld, m0, n0, mt, nt, it.first, it.second, SP_X, (int)P.nslots, ob.group});
           P.nslots += units;
         }
         // reduce tile: slot = units per partial, out_space = partial leading dimension
-        P.rtile.push_back({ob.off, ob.ld, m0, n0, mt, nt, s0, (int)P.nslots, tile, units, 0});
+        P.rtile.push_back({ob.off, ob.ld, m0, n0, mt, nt, s0, (int)P.nslots, tile, units, ob.group});
       }
   }
   P.seg2.swap(pieces);
-  // longest work items first within each tile size: fewer stragglers
-  auto by_cost = [&](std::vector<Tile>& v) {
+  // XCD-aware order: work items of one locality group (operands they share)
+  // go to one XCD, whose L2 then serves the repeated operand reads.  Groups
+  // are spread over the XCDs largest first onto the least loaded one; within
+  // an XCD groups run in that order and items longest first.  Workgroup id g
+  // runs on XCD g % xcds, so bin x's p-th item takes id p * xcds + x; bins are
+  // padded with empty items (mt = nt = 0, no segments).
+  auto xcd_order = [&](std::vector<Tile>& v, const std::vector<Seg>& segs) {
+    const int NX = std::max(1, shape.xcds);
     std::vector<int64_t> cost(v.size(), 0);
-    for (size_t t = 0; t < v.size(); ++t)
-      for (int q = v[t].seg_begin; q < v[t].seg_end; ++q) cost[t] += P.seg2[q].r_space == SP_IDENT ? 1 : P.seg2[q].K;
+    for (size_t t = 0; t < v.size(); ++t) {
+      for (int q = v[t].seg_begin; q < v[t].seg_end; ++q) cost[t] += segs[q].r_space == SP_IDENT ? 1 : segs[q].K;
+      cost[t] = std::max<int64_t>(1, cost[t]) * v[t].mt * v[t].nt;
+    }
+    std::unordered_map<int, int64_t> gcost;
+    for (size_t t = 0; t < v.size(); ++t) gcost[v[t].group] += cost[t];
+    std::vector<std::pair<int64_t, int>> order;
+    for (auto& kv : gcost) order.push_back({kv.second, kv.first});
+    std::sort(order.begin(), order.end(), [](const std::pair<int64_t, int>& a, const std::pair<int64_t, int>& b) {
+      return a.first != b.first ? a.first > b.first : a.second < b.second;
+    });
+    std::unordered_map<int, int> gbin, grank;
+    std::vector<int64_t> load(NX, 0);
+    for (size_t r = 0; r < order.size(); ++r) {
+      const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      gbin[order[r].second] = b;
+      grank[order[r].second] = (int)r;
+      load[b] += order[r].first;
+    }
     std::vector<int> idx(v.size());
     for (size_t t = 0; t < idx.size(); ++t) idx[t] = (int)t;
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    std::vector<Tile> sorted(v.size());
-    for (size_t t = 0; t < idx.size(); ++t) sorted[t] = v[idx[t]];
-    v.swap(sorted);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+      const int ra = grank[v[a].group], rb = grank[v[b].group];
+      return ra != rb ? ra < rb : cost[a] > cost[b];
+    });
+    std::vector<std::vector<int>> bins(NX);
+    for (int t : idx) bins[gbin[v[t].group]].push_back(t);
+    size_t maxlen = 0;
+    for (auto& b : bins) maxlen = std::max(maxlen, b.size());
+    Tile null{0, 1, 0, 0, 0, 0, 0, 0, SP_X, -1, -1};
+    std::vector<Tile> out(maxlen * NX, null);
+    for (int x = 0; x < NX; ++x)
+      for (size_t p = 0; p < bins[x].size(); ++p) out[p * NX + x] = v[bins[x][p]];
+    v.swap(out);
   };
-  by_cost(t2big);
-  by_cost(t2small);
+  xcd_order(t2big, P.seg2);
+  xcd_order(t2small, P.seg2);
+  xcd_order(t1big, P.seg1);
+  xcd_order(t1small, P.seg1);
+  P.n64_1 = (int)t1big.size();
+  P.tile1 = t1big;
+  P.tile1.insert(P.tile1.end(), t1small.begin(), t1small.end());
   P.n64_2 = (int)t2big.size();
   P.tile2 = t2big;
   P.tile2.insert(P.tile2.end(), t2small.begin(), t2small.end());

@@ -88,7 +88,7 @@ struct Contrib {
   double c;
 };
 
-struct OutBlock { int64_t off; int ld, M, N; };
+struct OutBlock { int64_t off; int ld, M, N, group; };  // group: (wave sector, env sub-block)
 
 // device task tables (POD, copied verbatim)
 struct Seg {
@@ -102,7 +102,7 @@ struct Tile {
   int32_t out_ld, m0, n0, mt, nt, seg_begin, seg_end;
   int32_t out_space;
   int32_t slot;             // >= 0: write a partial tile to partial slot `slot`
-  int32_t reserved;         // keeps the 48-byte layout explicit
+  int32_t group;            // locality group (work items sharing operands), informational
 };
 
 struct HostPlan {
@@ -140,6 +140,7 @@ struct Shape {
   int kc = 32;
   int k_split = 256;
   int item_chunks = 32;
+  int xcds = 8;             // items of a locality group go to one XCD (round-robin dispatch)
 };
 
 int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjsq_opset& sys,

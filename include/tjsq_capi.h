@@ -212,11 +212,14 @@ int tjsq_lanczos_solve(tjsq_plan* plan, double* psi, const tjsq_lanczos_opts* op
  * Table element layouts: segment 40 bytes {int64 l_off, r_off; int32 l_ld,
  * r_ld, K; uint8 l_space, r_space, l_t, r_t; double c}; tile 48 bytes
  * {int64 out_off; int32 out_ld, m0, n0, mt, nt, seg_begin, seg_end,
- * out_space, slot, reserved}; merge term 32 bytes {int64 dst_off, src_off, n; double
+ * out_space, slot, group}; merge term 32 bytes {int64 dst_off, src_off, n; double
  * coef}.  A stage-2 tile with slot >= 0 writes its partial tile (leading
  * dimension = its edge) at element 1024 * slot; reduce tiles (rtile, tile
  * layout) sum the partials at units [seg_begin, seg_end) in steps of `slot`
- * units, leading dimension out_space, in order into their output tile. */
+ * units, leading dimension out_space, in order into their output tile.
+ * Work items are ordered for the round-robin XCD dispatch (item g runs on
+ * XCD g % 8) so that items sharing operands (same `group`) share one L2;
+ * padding items have mt = nt = 0 and no segments. */
 typedef struct tjsq_hostplan tjsq_hostplan;
 int tjsq_hostplan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys,
                          const tjsq_opset* env, const tjsq_basis* sys_bs, const tjsq_basis* env_bs, int nw,
