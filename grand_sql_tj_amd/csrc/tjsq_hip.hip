@@ -181,6 +181,9 @@ __global__ __launch_bounds__(TileCfg<TSZ>::NT) void k_tiles(const Tile* __restri
   }
   if (si < se) {
     Seg cur = segs[si];
+    // descriptor of the segment after `cur`, loaded one segment ahead so the
+    // chunk fetches at a segment boundary do not wait on it
+    Seg pre = si + 1 < se ? segs[si + 1] : cur;
     int k0 = 0;
     T rl[PER], rr[PER];
     fetch_chunk<MODE, TSZ>(S, cur, tl, 0, tid, rl, rr);
@@ -213,7 +216,8 @@ __global__ __launch_bounds__(TileCfg<TSZ>::NT) void k_tiles(const Tile* __restri
       if (nk0 >= cur.K) {
         nsi = si + 1;
         nk0 = 0;
-        if (nsi < se) nxt = segs[nsi];
+        nxt = pre;
+        if (nsi + 1 < se) pre = segs[nsi + 1];
       }
       const bool more = nsi < se;
       if (more) fetch_chunk<MODE, TSZ>(S, nxt, tl, nk0, tid, rl, rr);
