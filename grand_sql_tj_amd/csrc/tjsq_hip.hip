@@ -221,9 +221,13 @@ __global__ __launch_bounds__(TileCfg<TSZ>::NT) void k_tiles(const Tile* __restri
       }
       const bool more = nsi < se;
       if (more) fetch_chunk<MODE, TSZ>(S, nxt, tl, nk0, tid, rl, rr);
+      // MFMA steps beyond the valid K of this chunk and 16-row / 16-column
+      // blocks outside the tile hold only zeros: skip them (wave-uniform)
+      const int kvalid = min(KC, cur.K - k0);
+      const bool r0 = wm < tl.mt, r1 = wm + 16 < tl.mt, c0 = wn < tl.nt, c1 = wn + 16 < tl.nt;
       if constexpr (MODE == MODE_BF16X3) {
-#pragma unroll
-        for (int kb = 0; kb < KC / 16; ++kb) {
+        const int nkb = (kvalid + 15) / 16;
+        for (int kb = 0; kb < nkb; ++kb) {
           s4 a[2][2], b[2][2];
           for (int p = 0; p < 2; ++p)
             for (int h = 0; h < 2; ++h)
@@ -231,28 +235,30 @@ __global__ __launch_bounds__(TileCfg<TSZ>::NT) void k_tiles(const Tile* __restri
                 a[p][h][t] = (short)Lh[h * TSZ * HS + (wm + p * 16 + lr) * HS + kb * 16 + lq * 4 + t];
                 b[p][h][t] = (short)Rh[h * TSZ * HS + (wn + p * 16 + lr) * HS + kb * 16 + lq * 4 + t];
               }
+          const bool rr_[2] = {r0, r1}, cc_[2] = {c0, c1};
           for (int bi = 0; bi < 2; ++bi)
             for (int bj = 0; bj < 2; ++bj) {
+              if (!(rr_[bi] && cc_[bj])) continue;
               acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][1], b[bj][0], acc[bi][bj], 0, 0, 0);
               acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][0], b[bj][1], acc[bi][bj], 0, 0, 0);
               acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][0], b[bj][0], acc[bi][bj], 0, 0, 0);
             }
         }
       } else {
-#pragma unroll
-        for (int kk = 0; kk < KC / 4; ++kk) {
+        const int nkk = (kvalid + 3) / 4;
+        for (int kk = 0; kk < nkk; ++kk) {
           const T a0 = Ls[(wm + lr) * (KC + 1) + kk * 4 + lq], a1 = Ls[(wm + 16 + lr) * (KC + 1) + kk * 4 + lq];
           const T b0 = Rs[(kk * 4 + lq) * (TSZ + 1) + wn + lr], b1 = Rs[(kk * 4 + lq) * (TSZ + 1) + wn + 16 + lr];
           if constexpr (MODE == MODE_F64) {
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+            if (r0 && c0) acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            if (r0 && c1) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            if (r1 && c0) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            if (r1 && c1) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
           } else {
-            acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+            if (r0 && c0) acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+            if (r0 && c1) acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+            if (r1 && c0) acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+            if (r1 && c1) acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
           }
         }
       }
