@@ -70,7 +70,7 @@ EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_t
            "tjsq_plan_create", "tjsq_plan_stats_get", "tjsq_plan_destroy", "tjsq_heff_apply", "tjsq_heff_apply_f32",
            "tjsq_lanczos_solve", "tjsq_hostplan_create", "tjsq_hostplan_sizes", "tjsq_hostplan_tables",
            "tjsq_hostplan_stats", "tjsq_hostplan_destroy", "tjsq_plan_profile", "tjsq_plan_profile_read",
-           "tjsq_last_error"]
+           "tjsq_heff_apply_host", "tjsq_lanczos_solve_host", "tjsq_last_error"]
 
 _lib = None
 
@@ -104,6 +104,8 @@ def lib():
             "tjsq_hostplan_destroy": [P],
             "tjsq_plan_profile": [P, I],
             "tjsq_plan_profile_read": [P, P, P],
+            "tjsq_heff_apply_host": [P, P, P],
+            "tjsq_lanczos_solve_host": [P, P, P, P],
         }
         for name in EXPORTS:
             f = getattr(L, name)

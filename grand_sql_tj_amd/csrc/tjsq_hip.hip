@@ -1120,3 +1120,39 @@ extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_
   res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return TJSQ_OK;
 }
+
+// ---------------------------------------------------------------------------
+// host-buffer entry points
+// ---------------------------------------------------------------------------
+extern "C" int tjsq_heff_apply_host(tjsq_plan* p, const double* x, double* y) {
+  if (!p || !x || !y) return fail(TJSQ_ERR_ARG, "tjsq_heff_apply_host: null argument");
+  const int64_t n = p->hp.dim;
+  if (!p->x64a) {
+    if (hipMalloc(&p->x64a, sizeof(double) * n) != hipSuccess || hipMalloc(&p->x64b, sizeof(double) * n) != hipSuccess)
+      return fail(TJSQ_ERR_NOMEM, "staging buffer allocation failed");
+  }
+  HIPCHK(hipMemcpy(p->x64b, x, sizeof(double) * n, hipMemcpyHostToDevice));
+  double* yd = nullptr;
+  if (hipMalloc(&yd, sizeof(double) * n) != hipSuccess) return fail(TJSQ_ERR_NOMEM, "staging buffer allocation failed");
+  int rc = tjsq_heff_apply(p, p->x64b, yd, nullptr);
+  if (rc == TJSQ_OK && hipMemcpy(y, yd, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(TJSQ_ERR_HIP, "copy of y to the host failed");
+  (void)hipFree(yd);
+  return rc;
+}
+
+extern "C" int tjsq_lanczos_solve_host(tjsq_plan* p, double* psi, const tjsq_lanczos_opts* opts,
+                                       tjsq_lanczos_result* res) {
+  if (!p || !psi || !opts || !res) return fail(TJSQ_ERR_ARG, "tjsq_lanczos_solve_host: null argument");
+  const int64_t n = p->hp.dim;
+  double* d = nullptr;
+  if (hipMalloc(&d, sizeof(double) * n) != hipSuccess) return fail(TJSQ_ERR_NOMEM, "psi buffer allocation failed");
+  int rc = TJSQ_OK;
+  if (hipMemcpy(d, psi, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(TJSQ_ERR_HIP, "copy of psi to the device failed");
+  if (rc == TJSQ_OK) rc = tjsq_lanczos_solve(p, d, opts, res, nullptr);
+  if (rc == TJSQ_OK && hipMemcpy(psi, d, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(TJSQ_ERR_HIP, "copy of psi to the host failed");
+  (void)hipFree(d);
+  return rc;
+}

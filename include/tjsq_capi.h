@@ -230,6 +230,13 @@ int tjsq_hostplan_tables(const tjsq_hostplan* h, void* seg1, void* tile1, void* 
 int tjsq_hostplan_stats(const tjsq_hostplan* h, tjsq_plan_stats* stats);
 int tjsq_hostplan_destroy(tjsq_hostplan* h);
 
+/* ---- host-buffer entry points (Fortran callers holding data in host
+ * memory): the same operations with x, y, psi in host memory; the library
+ * stages them through device buffers of the plan (PCIe transfers included). */
+int tjsq_heff_apply_host(tjsq_plan* plan, const double* x, double* y);
+int tjsq_lanczos_solve_host(tjsq_plan* plan, double* psi, const tjsq_lanczos_opts* opts,
+                            tjsq_lanczos_result* result);
+
 /* Last error message of the calling thread (static storage). */
 const char* tjsq_last_error(void);
 

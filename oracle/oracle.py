@@ -199,3 +199,45 @@ def oracle_get_basis(block_len, blk, site):
     nq = C.c_int(0)
     ns = oracle_lib().oracle_get_basis(block_len, blk.shape[0], blk, site.shape[0], site, sec, sub, C.byref(nq))
     return sec[:4 * ns].reshape(-1, 4), sub[:6 * nq.value].reshape(-1, 6)
+
+
+def dropin_path() -> str:
+    return os.path.join(HERE, "_ref", "libtjsq_dropin.so")
+
+
+def dropin_lib():
+    """The product's Fortran drop-in shim (grand_sql_tj_amd/fortran) built
+    inside the reference build, with its test entry points."""
+    with _lock:
+        if "dropin" not in _libs:
+            ref_lib()
+            lib = C.CDLL(dropin_path(), mode=C.RTLD_GLOBAL)
+            margs = [C.c_int, I32P, I32P, I64P, F64P]
+            common = [C.c_int, C.c_int] + margs + margs + [C.c_int, I32P, I32P, C.c_int, I32P, I32P, C.c_int, I32P]
+            lib.dropin_model_wave.restype = C.c_int
+            lib.dropin_model_wave.argtypes = common + [F64P, F64P]
+            lib.dropin_lanczos.restype = C.c_int
+            lib.dropin_lanczos.argtypes = common + [F64P, C.POINTER(C.c_double)]
+            _libs["dropin"] = lib
+        return _libs["dropin"]
+
+
+def dropin_model_wave(model, sb, x):
+    with _lock:
+        _ref_init(model)
+        y = np.zeros(sb.dim)
+        dropin_lib().dropin_model_wave(sb.sys_len, sb.env_len, *_m(sb.sys), *_m(sb.env), *_bs(sb.sys_bs),
+                                       *_bs(sb.env_bs), int(sb.wsec.shape[0]), np.ascontiguousarray(sb.wsec),
+                                       np.ascontiguousarray(x, dtype=np.float64), y)
+    return y
+
+
+def dropin_lanczos(model, sb, x0, lanzero=1e-10):
+    with _lock:
+        _ref_init(model, lanzero)
+        x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+        e0 = C.c_double(0.0)
+        dropin_lib().dropin_lanczos(sb.sys_len, sb.env_len, *_m(sb.sys), *_m(sb.env), *_bs(sb.sys_bs),
+                                    *_bs(sb.env_bs), int(sb.wsec.shape[0]), np.ascontiguousarray(sb.wsec), x,
+                                    C.byref(e0))
+    return e0.value, x
