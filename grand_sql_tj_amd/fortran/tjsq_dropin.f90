@@ -145,7 +145,9 @@ contains
     lat%jd = [jd(1, 1), jd(5, 1)]
     lat%jn = [jn(1), jn(5)]
     lat%esite = esite(1)
-    lat%up_bias = up_bias
+    ! the particle label the site operators carry (dmrg_sub.f90:380-381):
+    ! the apply kernels see up_bias only through the operators' up_dif
+    lat%up_bias = st_elec_up%up_dif
   end function lattice_from_pubdata
 
   subroutine make_plan(sys, env, sys_bs, env_bs, wsec, nw, plan, fs, fe, bs, be)
