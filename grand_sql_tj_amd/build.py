@@ -7,7 +7,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = [os.path.join(HERE, "csrc", "tjsq_plan.cpp"), os.path.join(HERE, "csrc", "tjsq_hip.hip")]
+SOURCES = [os.path.join(HERE, "csrc", "tjsq_plan.cpp"), os.path.join(HERE, "csrc", "tjsq_update.cpp"),
+           os.path.join(HERE, "csrc", "tjsq_hip.hip")]
 OUT = os.path.join(HERE, "libtjsq_hip.so")
 
 HIPCC_FLAGS = [
@@ -20,7 +21,9 @@ HIPCC_FLAGS = [
 
 def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc] + HIPCC_FLAGS + SOURCES + ["-o", OUT + ".tmp"]
+    # rocSOLVER (with rocBLAS) diagonalises the density-matrix sectors
+    libs = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+    cmd = [hipcc] + HIPCC_FLAGS + SOURCES + libs + ["-o", OUT + ".tmp"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

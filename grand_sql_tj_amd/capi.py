@@ -43,6 +43,14 @@ class PlanStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TruncInfo(C.Structure):
+    _fields_ = [("num_state", C.c_int), ("dim", C.c_int), ("trunc_err", C.c_double), ("entropy", C.c_double),
+                ("entropy1", C.c_double), ("entropy2", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class LanczosOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_int), ("precision", C.c_int), ("refine_tol", C.c_double),
                 ("max_refine", C.c_int), ("energy_tol", C.c_double), ("rtol", C.c_double)]
@@ -70,7 +78,11 @@ EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_t
            "tjsq_plan_create", "tjsq_plan_stats_get", "tjsq_plan_destroy", "tjsq_heff_apply", "tjsq_heff_apply_f32",
            "tjsq_lanczos_solve", "tjsq_hostplan_create", "tjsq_hostplan_sizes", "tjsq_hostplan_tables",
            "tjsq_hostplan_stats", "tjsq_hostplan_destroy", "tjsq_plan_profile", "tjsq_plan_profile_read",
-           "tjsq_heff_apply_host", "tjsq_lanczos_solve_host", "tjsq_last_error"]
+           "tjsq_heff_apply_host", "tjsq_lanczos_solve_host",
+           "tjsq_block_update_create", "tjsq_trun_project_create", "tjsq_density_create", "tjsq_program_sizes",
+           "tjsq_program_layout", "tjsq_program_tables", "tjsq_program_stats", "tjsq_program_run",
+           "tjsq_program_profile", "tjsq_program_profile_read", "tjsq_program_destroy", "tjsq_truncation_operator",
+           "tjsq_truncation_select", "tjsq_identity_truncation", "tjsq_last_error"]
 
 _lib = None
 
@@ -106,6 +118,20 @@ def lib():
             "tjsq_plan_profile_read": [P, P, P],
             "tjsq_heff_apply_host": [P, P, P],
             "tjsq_lanczos_solve_host": [P, P, P, P],
+            "tjsq_block_update_create": [P, I, I, I, P, P, P, P],
+            "tjsq_trun_project_create": [P, P, P, P],
+            "tjsq_density_create": [I, I, I, P, I, P],
+            "tjsq_program_sizes": [P, P],
+            "tjsq_program_layout": [P, P, P, P],
+            "tjsq_program_tables": [P, P, P, P, P, P],
+            "tjsq_program_stats": [P, P],
+            "tjsq_program_run": [P, P, P, P, P],
+            "tjsq_program_profile": [P, I],
+            "tjsq_program_profile_read": [P, P, P],
+            "tjsq_program_destroy": [P],
+            "tjsq_truncation_operator": [I, P, P, P, I, I, C.c_double, P, P, P, P, P, P],
+            "tjsq_truncation_select": [I, P, P, I, I, C.c_double, P, P, P, P],
+            "tjsq_identity_truncation": [P, P, P, P, P, P],
         }
         for name in EXPORTS:
             f = getattr(L, name)
@@ -333,3 +359,162 @@ class Plan:
         if getattr(self, "h", None) and _lib is not None:
             _lib.tjsq_plan_destroy(self.h)
             self.h = None
+
+
+# ---------------------------------------------------------------------------
+# density matrix, truncation and block update (tjsq_program)
+# ---------------------------------------------------------------------------
+PROGRAM_SIZE_KEYS = ("nops", "nsub", "ndata", "x_elems", "ops_elems", "n_seg1", "n_tile1", "n_seg2", "n_tile2",
+                     "n_rtile", "n_slots", "scratch_elems", "n64_1", "n64_2")
+
+
+class Program:
+    """A compiled density-matrix, projection or block-update program.
+
+    Built on the host (no GPU); ``run`` executes it on device pointers.  The
+    output is an operator set whose layout ``layout()`` returns.
+    """
+
+    def __init__(self, handle):
+        self._h = handle
+        sizes = np.zeros(14, np.int64)
+        check(lib().tjsq_program_sizes(self._h, sizes.ctypes.data), "tjsq_program_sizes")
+        self.sizes = {k: int(v) for k, v in zip(PROGRAM_SIZE_KEYS, sizes)}
+
+    @classmethod
+    def block_update(cls, model, side, sys_len, env_len, pre, basis, trun):
+        keep = _Keep()
+        lat = lattice_struct(model)
+        ps, bs, ts = opset_struct(pre, keep), basis_struct(basis, keep), opset_struct(trun, keep)
+        h = C.c_void_p()
+        check(lib().tjsq_block_update_create(C.byref(lat), side, sys_len, env_len, C.byref(ps), C.byref(bs),
+                                             C.byref(ts), C.byref(h)), "tjsq_block_update_create")
+        return cls(h)
+
+    @classmethod
+    def projection(cls, ops, ndia, trun):
+        keep = _Keep()
+        os_, ts = opset_struct(ops, keep), opset_struct(trun, keep)
+        nd = np.ascontiguousarray(ndia, np.int32)
+        h = C.c_void_p()
+        check(lib().tjsq_trun_project_create(C.byref(os_), nd.ctypes.data, C.byref(ts), C.byref(h)),
+              "tjsq_trun_project_create")
+        return cls(h)
+
+    @classmethod
+    def density(cls, side, length, wsec, levels=1):
+        w = np.ascontiguousarray(wsec, np.int32).reshape(-1, 6)
+        h = C.c_void_p()
+        check(lib().tjsq_density_create(side, length, w.shape[0], w.ctypes.data, levels, C.byref(h)),
+              "tjsq_density_create")
+        return cls(h)
+
+    def layout(self):
+        """(hdr (nops, 4), sub (nsub, 5), off (nsub,)) of the output operator set."""
+        hdr = np.zeros((self.sizes["nops"], 4), np.int32)
+        sub = np.zeros((self.sizes["nsub"], 5), np.int32)
+        off = np.zeros(self.sizes["nsub"], np.int64)
+        check(lib().tjsq_program_layout(self._h, hdr.ctypes.data, sub.ctypes.data, off.ctypes.data),
+              "tjsq_program_layout")
+        return hdr, sub, off
+
+    def tables(self):
+        z = self.sizes
+        seg1 = np.zeros(z["n_seg1"], SEG_DTYPE)
+        tile1 = np.zeros(z["n_tile1"], TILE_DTYPE)
+        seg2 = np.zeros(z["n_seg2"], SEG_DTYPE)
+        tile2 = np.zeros(z["n_tile2"], TILE_DTYPE)
+        rtile = np.zeros(z["n_rtile"], TILE_DTYPE)
+        check(lib().tjsq_program_tables(self._h, seg1.ctypes.data, tile1.ctypes.data, seg2.ctypes.data,
+                                        tile2.ctypes.data, rtile.ctypes.data), "tjsq_program_tables")
+        return {"seg1": seg1, "tile1": tile1, "seg2": seg2, "tile2": tile2, "rtile": rtile}
+
+    def stats(self):
+        st = PlanStats()
+        check(lib().tjsq_program_stats(self._h, C.byref(st)), "tjsq_program_stats")
+        return st.as_dict()
+
+    def run(self, ops, x, y, stream=None):
+        """ops / x / y: torch CUDA float64 tensors (ops may be None for density programs)."""
+        import torch
+        assert y.dtype == torch.float64 and y.is_cuda and y.numel() >= self.sizes["ndata"]
+        assert x.dtype == torch.float64 and x.numel() >= self.sizes["x_elems"]
+        if ops is not None:
+            assert ops.dtype == torch.float64 and ops.numel() >= self.sizes["ops_elems"]
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(lib().tjsq_program_run(self._h, None if ops is None else ops.data_ptr(), x.data_ptr(), y.data_ptr(),
+                                     C.c_void_p(s)), "tjsq_program_run")
+
+    def profile(self, enable=True):
+        check(lib().tjsq_program_profile(self._h, int(enable)), "tjsq_program_profile")
+
+    def profile_read(self):
+        ms = np.zeros(2, np.float64)
+        n = np.zeros(2, np.int64)
+        check(lib().tjsq_program_profile_read(self._h, ms.ctypes.data, n.ctypes.data), "tjsq_program_profile_read")
+        return ms, n
+
+    def opset(self, length, data):
+        """blocks.OpSet over the output layout with host data ``data``."""
+        from .blocks import OpSet
+        hdr, sub, off = self.layout()
+        return OpSet(length, hdr, sub, off, np.ascontiguousarray(data, np.float64))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().tjsq_program_destroy(self._h)
+            self._h = None
+
+
+def truncation_select(den_sub, eigval, kept_min, kept_max, trun_err):
+    """Host selection of Get_truncation_operator on given eigenvalues."""
+    den_sub = np.ascontiguousarray(den_sub, np.int32).reshape(-1, 5)
+    ev = np.ascontiguousarray(eigval, np.float64)
+    n = den_sub.shape[0]
+    counts = np.zeros(n, np.int32)
+    spec = np.zeros(ev.shape[0], np.float64)
+    order = np.zeros(ev.shape[0], np.int32)
+    info = TruncInfo()
+    check(lib().tjsq_truncation_select(n, den_sub.ctypes.data, ev.ctypes.data, kept_min, kept_max,
+                                       C.c_double(trun_err), counts.ctypes.data, spec.ctypes.data, order.ctypes.data,
+                                       C.byref(info)), "tjsq_truncation_select")
+    return counts, spec, order, info.as_dict()
+
+
+def truncation_operator(den_sub, den_off, den, kept_min, kept_max, trun_err, stream=None):
+    """Get_truncation_operator on a device density matrix ``den`` (torch
+    float64, overwritten by the eigenvectors).  Returns (trun_sub, trun_off,
+    spectrum, info); the truncation operator's data is ``den``."""
+    import torch
+    den_sub = np.ascontiguousarray(den_sub, np.int32).reshape(-1, 5)
+    den_off = np.ascontiguousarray(den_off, np.int64)
+    n = den_sub.shape[0]
+    tsub = np.zeros((n, 5), np.int32)
+    toff = np.zeros(n, np.int64)
+    nt = C.c_int(0)
+    spec = np.zeros(int(den_sub[:, 4].sum()), np.float64)
+    info = TruncInfo()
+    s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    check(lib().tjsq_truncation_operator(n, den_sub.ctypes.data, den_off.ctypes.data, den.data_ptr(), kept_min,
+                                         kept_max, C.c_double(trun_err), tsub.ctypes.data, toff.ctypes.data,
+                                         C.byref(nt), spec.ctypes.data, C.byref(info), C.c_void_p(s)),
+          "tjsq_truncation_operator")
+    return tsub[:nt.value].copy(), toff[:nt.value].copy(), spec, info.as_dict()
+
+
+def identity_truncation(basis, data=None, stream=None):
+    """Identity truncation layout over the basis sectors; fills ``data``
+    (device float64) when given.  Returns (trun_sub, trun_off, ndata)."""
+    keep = _Keep()
+    b = basis_struct(basis, keep)
+    tsub = np.zeros((basis.nsec, 5), np.int32)
+    toff = np.zeros(basis.nsec, np.int64)
+    nd = C.c_int64(0)
+    ptr, s = None, None
+    if data is not None:
+        import torch
+        ptr = data.data_ptr()
+        s = C.c_void_p(stream if stream is not None else torch.cuda.current_stream().cuda_stream)
+    check(lib().tjsq_identity_truncation(C.byref(b), tsub.ctypes.data, toff.ctypes.data, C.byref(nd), ptr, s),
+          "tjsq_identity_truncation")
+    return tsub, toff, nd.value

@@ -18,8 +18,10 @@
 // Krylov basis on the device; dot products are two-pass deterministic
 // reductions in FP64.
 #include <hip/hip_runtime.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <chrono>
 #include <cmath>
@@ -30,6 +32,7 @@
 
 #include "tjsq_capi.h"
 #include "tjsq_plan.hpp"
+#include "tjsq_update.hpp"
 
 using tjsq::HostPlan;
 using tjsq::Seg;
@@ -455,6 +458,31 @@ int upload(tjsq_plan* p, const tjsq_opset& set, int64_t base, hipStream_t st) {
   return TJSQ_OK;
 }
 
+// device copies of the task tables, FP64 stage-1 scratch and partial tiles
+// (plus the FP32 ones when with32)
+int upload_tables(tjsq_plan* p, bool with32, hipStream_t st) {
+  const HostPlan& hp = p->hp;
+  if (hipMalloc(&p->t64, sizeof(double) * std::max<int64_t>(1, hp.scratch_elems)) != hipSuccess)
+    return fail(TJSQ_ERR_NOMEM, "scratch allocation failed");
+  if (with32 && hipMalloc(&p->t32, sizeof(float) * std::max<int64_t>(1, hp.scratch_elems)) != hipSuccess)
+    return fail(TJSQ_ERR_NOMEM, "FP32 scratch allocation failed");
+  auto put = [&](auto** dptr, const auto& vec) -> int {
+    using E = typename std::remove_reference<decltype(vec)>::type::value_type;
+    if (hipMalloc((void**)dptr, sizeof(E) * std::max<size_t>(1, vec.size())) != hipSuccess) return TJSQ_ERR_NOMEM;
+    if (!vec.empty() && hipMemcpyAsync(*dptr, vec.data(), sizeof(E) * vec.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+      return TJSQ_ERR_HIP;
+    return TJSQ_OK;
+  };
+  if (put(&p->tile1, hp.tile1) || put(&p->tile2, hp.tile2) || put(&p->seg1, hp.seg1) || put(&p->seg2, hp.seg2) ||
+      put(&p->rtile, hp.rtile))
+    return fail(TJSQ_ERR_NOMEM, "task table allocation failed");
+  if (hipMalloc(&p->part64, sizeof(double) * std::max<int64_t>(1, hp.nslots * 1024)) != hipSuccess)
+    return fail(TJSQ_ERR_NOMEM, "partial tile allocation failed");
+  if (with32 && hipMalloc(&p->part32, sizeof(float) * std::max<int64_t>(1, hp.nslots * 1024)) != hipSuccess)
+    return fail(TJSQ_ERR_NOMEM, "partial tile allocation failed");
+  return TJSQ_OK;
+}
+
 template <typename T>
 int launch_tiles(const tjsq_plan* p, int stage, const T* ops, const T* x, T* t, T* y, int precision,
                  hipStream_t st) {
@@ -703,25 +731,11 @@ int tjsq_plan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tj
     hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(hp.arena_elems)), dim3(256), 0, st, p->arena, p->arena32,
                        hp.arena_elems);
     HIPCHK(hipGetLastError());
-    if (hipMalloc(&p->t32, sizeof(float) * std::max<int64_t>(1, hp.scratch_elems)) != hipSuccess)
-      return cleanup(TJSQ_ERR_NOMEM, "FP32 scratch allocation failed");
   }
-  if (hipMalloc(&p->t64, sizeof(double) * std::max<int64_t>(1, hp.scratch_elems)) != hipSuccess)
-    return cleanup(TJSQ_ERR_NOMEM, "scratch allocation failed");
-  auto put = [&](auto** dptr, const auto& vec) -> int {
-    using E = typename std::remove_reference<decltype(vec)>::type::value_type;
-    if (hipMalloc((void**)dptr, sizeof(E) * std::max<size_t>(1, vec.size())) != hipSuccess) return TJSQ_ERR_NOMEM;
-    if (!vec.empty() && hipMemcpyAsync(*dptr, vec.data(), sizeof(E) * vec.size(), hipMemcpyHostToDevice, st) != hipSuccess)
-      return TJSQ_ERR_HIP;
-    return TJSQ_OK;
-  };
-  if (put(&p->tile1, hp.tile1) || put(&p->tile2, hp.tile2) || put(&p->seg1, hp.seg1) || put(&p->seg2, hp.seg2) ||
-      put(&p->rtile, hp.rtile))
-    return cleanup(TJSQ_ERR_NOMEM, "task table allocation failed");
-  if (hipMalloc(&p->part64, sizeof(double) * std::max<int64_t>(1, hp.nslots * 1024)) != hipSuccess)
-    return cleanup(TJSQ_ERR_NOMEM, "partial tile allocation failed");
-  if (p->arena32 && hipMalloc(&p->part32, sizeof(float) * std::max<int64_t>(1, hp.nslots * 1024)) != hipSuccess)
-    return cleanup(TJSQ_ERR_NOMEM, "partial tile allocation failed");
+  if ((rc = upload_tables(p, p->arena32 != nullptr, st))) {
+    tjsq_plan_destroy(p);
+    return rc;
+  }
   HIPCHK(hipStreamSynchronize(st));
   *plan = p;
   return TJSQ_OK;
@@ -1156,3 +1170,320 @@ extern "C" int tjsq_lanczos_solve_host(tjsq_plan* p, double* psi, const tjsq_lan
   (void)hipFree(d);
   return rc;
 }
+
+// ---------------------------------------------------------------------------
+// density matrix, truncation and block-update programs (tjsq_update.hpp)
+// ---------------------------------------------------------------------------
+struct tjsq_program {
+  tjsq_plan* dev = nullptr;      // compiled tables (dev->hp) and their device copies
+  tjsq::OutLayout out;
+  int64_t x_elems = 0, ops_elems = 0;
+  int kind = 0;                  // 0: block update / projection, 1: density matrix
+  int levels = 1;
+  bool uploaded = false;
+  double* work = nullptr;        // density: sum of the states
+  double* dwork = nullptr;       // density: dot partials, norm, coefficients
+};
+
+namespace {
+
+int make_program(tjsq::Program& pg, int kind, int levels, tjsq_program** out) {
+  tjsq_program* P = new tjsq_program();
+  P->dev = new tjsq_plan();
+  P->dev->hp = std::move(pg.hp);
+  P->out = std::move(pg.out);
+  P->x_elems = pg.x_elems;
+  P->ops_elems = pg.ops_elems;
+  P->kind = kind;
+  P->levels = levels;
+  *out = P;
+  return TJSQ_OK;
+}
+
+// y *= 1 / s[0]
+__global__ void k_scale_inv(double* __restrict__ y, int64_t n, const double* __restrict__ s) {
+  const double f = 1.0 / s[0];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] *= f;
+}
+
+// identity sub-blocks: data[off + r + c n] = (r == c) for an n x n block
+__global__ void k_identity(double* __restrict__ a, int64_t n) {
+  const int64_t nn = n * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = (i % (n + 1) == 0) ? 1.0 : 0.0;
+}
+
+std::mutex g_solver_mu;
+std::vector<rocblas_handle> g_solver;   // one handle per device, created on first use
+
+int solver_handle(rocblas_handle* h) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if ((int)g_solver.size() <= dev) g_solver.resize(dev + 1, nullptr);
+  if (!g_solver[dev] && rocblas_create_handle(&g_solver[dev]) != rocblas_status_success)
+    return fail(TJSQ_ERR_HIP, "rocblas_create_handle failed");
+  *h = g_solver[dev];
+  return TJSQ_OK;
+}
+
+void fill_info(const tjsq::TruncSelection& sel, tjsq_trunc_info* info) {
+  if (!info) return;
+  info->num_state = sel.num_state;
+  info->dim = sel.dim;
+  info->trunc_err = sel.trunc_err;
+  info->entropy = sel.entropy;
+  info->entropy1 = sel.entropy1;
+  info->entropy2 = sel.entropy2;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tjsq_block_update_create(const tjsq_lattice* lat, int side, int sys_len, int env_len, const tjsq_opset* pre,
+                             const tjsq_basis* basis, const tjsq_opset* trun, tjsq_program** prog) {
+  if (!lat || !pre || !basis || !trun || !prog) return fail(TJSQ_ERR_ARG, "tjsq_block_update_create: null argument");
+  *prog = nullptr;
+  tjsq::Program pg;
+  std::string msg;
+  int rc = tjsq::build_update_program(*lat, side, sys_len, env_len, *pre, *basis, *trun, tjsq::Shape(), pg, msg);
+  if (rc) return fail(rc, "tjsq_block_update_create: " + msg);
+  return make_program(pg, 0, 1, prog);
+}
+
+int tjsq_trun_project_create(const tjsq_opset* ops, const int* ndia, const tjsq_opset* trun, tjsq_program** prog) {
+  if (!ops || !trun || !prog) return fail(TJSQ_ERR_ARG, "tjsq_trun_project_create: null argument");
+  *prog = nullptr;
+  tjsq::Program pg;
+  std::string msg;
+  int rc = tjsq::build_projection_program(*ops, ndia, *trun, tjsq::Shape(), pg, msg);
+  if (rc) return fail(rc, "tjsq_trun_project_create: " + msg);
+  return make_program(pg, 0, 1, prog);
+}
+
+int tjsq_density_create(int side, int len, int nw, const int* wsec, int levels, tjsq_program** prog) {
+  if (!wsec || !prog || levels < 1) return fail(TJSQ_ERR_ARG, "tjsq_density_create: bad argument");
+  *prog = nullptr;
+  tjsq::Program pg;
+  std::string msg;
+  int rc = tjsq::build_density_program(side, len, nw, wsec, tjsq::Shape(), pg, msg);
+  if (rc) return fail(rc, "tjsq_density_create: " + msg);
+  return make_program(pg, 1, levels, prog);
+}
+
+int tjsq_program_sizes(const tjsq_program* P, int64_t* sizes) {
+  if (!P || !sizes) return fail(TJSQ_ERR_ARG, "tjsq_program_sizes: null argument");
+  const HostPlan& hp = P->dev->hp;
+  const int64_t v[14] = {P->out.nops, (int64_t)P->out.off.size(), P->out.ndata, P->x_elems, P->ops_elems,
+                         (int64_t)hp.seg1.size(), (int64_t)hp.tile1.size(), (int64_t)hp.seg2.size(),
+                         (int64_t)hp.tile2.size(), (int64_t)hp.rtile.size(), hp.nslots, hp.scratch_elems,
+                         hp.n64_1, hp.n64_2};
+  std::memcpy(sizes, v, sizeof(v));
+  return TJSQ_OK;
+}
+
+int tjsq_program_layout(const tjsq_program* P, int* hdr, int* sub, int64_t* off) {
+  if (!P) return fail(TJSQ_ERR_ARG, "tjsq_program_layout: null argument");
+  if (hdr) std::memcpy(hdr, P->out.hdr.data(), sizeof(int) * P->out.hdr.size());
+  if (sub) std::memcpy(sub, P->out.sub.data(), sizeof(int) * P->out.sub.size());
+  if (off) std::memcpy(off, P->out.off.data(), sizeof(int64_t) * P->out.off.size());
+  return TJSQ_OK;
+}
+
+int tjsq_program_tables(const tjsq_program* P, void* seg1, void* tile1, void* seg2, void* tile2, void* rtile) {
+  if (!P) return fail(TJSQ_ERR_ARG, "tjsq_program_tables: null argument");
+  const HostPlan& hp = P->dev->hp;
+  if (seg1) std::memcpy(seg1, hp.seg1.data(), sizeof(Seg) * hp.seg1.size());
+  if (tile1) std::memcpy(tile1, hp.tile1.data(), sizeof(Tile) * hp.tile1.size());
+  if (seg2) std::memcpy(seg2, hp.seg2.data(), sizeof(Seg) * hp.seg2.size());
+  if (tile2) std::memcpy(tile2, hp.tile2.data(), sizeof(Tile) * hp.tile2.size());
+  if (rtile) std::memcpy(rtile, hp.rtile.data(), sizeof(Tile) * hp.rtile.size());
+  return TJSQ_OK;
+}
+
+int tjsq_program_stats(const tjsq_program* P, tjsq_plan_stats* stats) {
+  if (!P || !stats) return fail(TJSQ_ERR_ARG, "tjsq_program_stats: null argument");
+  *stats = P->dev->hp.stats;
+  return TJSQ_OK;
+}
+
+int tjsq_program_run(tjsq_program* P, const double* ops, const double* x, double* y, void* stream) {
+  if (!P || !y || (!x && P->x_elems > 0) || (!ops && P->ops_elems > 0 && P->kind == 0))
+    return fail(TJSQ_ERR_ARG, "tjsq_program_run: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  int rc;
+  if (!P->uploaded) {
+    (void)hipGetDevice(&P->dev->device);
+    if ((rc = upload_tables(P->dev, false, st))) return rc;
+    P->uploaded = true;
+  }
+  const double* xin = x;
+  double* nrm = nullptr;
+  if (P->kind == 1) {
+    // Get_density_matrix_*_new (den_mat.f90:84-134): sum the states and
+    // normalise; the normalisation is applied to rho as 1 / |psi|^2
+    const int64_t n = P->x_elems;
+    if (!P->dwork) {
+      if (hipMalloc(&P->dwork, sizeof(double) * (2 * RED_BLOCKS + 16 + P->levels)) != hipSuccess)
+        return fail(TJSQ_ERR_NOMEM, "density work allocation failed");
+      std::vector<double> minus(P->levels, -1.0);
+      HIPCHK(hipMemcpy(P->dwork + 2 * RED_BLOCKS + 16, minus.data(), sizeof(double) * P->levels,
+                       hipMemcpyHostToDevice));
+      if (P->levels > 1 && hipMalloc(&P->work, sizeof(double) * std::max<int64_t>(1, n)) != hipSuccess)
+        return fail(TJSQ_ERR_NOMEM, "density work allocation failed");
+    }
+    if (P->levels > 1) {
+      hipLaunchKernelGGL(k_combine<double>, dim3(grid_for(n)), dim3(256), 0, st, P->work, x, n, P->levels,
+                         P->dwork + 2 * RED_BLOCKS + 16, n, 1.0, 1);
+      xin = P->work;
+    }
+    nrm = P->dwork + 2 * RED_BLOCKS;
+    hipLaunchKernelGGL(k_multidot<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, xin, n, 1, xin, n, P->dwork, 0);
+    hipLaunchKernelGGL(k_finish_dots, dim3(1), dim3(RED_THREADS), 0, st, P->dwork, 1, RED_BLOCKS, 0, nrm);
+    HIPCHK(hipGetLastError());
+  }
+  const double* opsp = ops ? ops : xin;
+  if ((rc = launch_tiles<double>(P->dev, 1, opsp, xin, P->dev->t64, y, TJSQ_FP64, st))) return rc;
+  if ((rc = launch_tiles<double>(P->dev, 2, opsp, xin, P->dev->t64, y, TJSQ_FP64, st))) return rc;
+  if (nrm) {
+    hipLaunchKernelGGL(k_scale_inv, dim3(grid_for(P->out.ndata)), dim3(256), 0, st, y, P->out.ndata, nrm);
+    HIPCHK(hipGetLastError());
+  }
+  return TJSQ_OK;
+}
+
+int tjsq_program_profile(tjsq_program* P, int enable) {
+  if (!P) return fail(TJSQ_ERR_ARG, "tjsq_program_profile: null argument");
+  return tjsq_plan_profile(P->dev, enable);
+}
+
+int tjsq_program_profile_read(tjsq_program* P, double* ms, int64_t* launches) {
+  if (!P) return fail(TJSQ_ERR_ARG, "tjsq_program_profile_read: null argument");
+  return tjsq_plan_profile_read(P->dev, ms, launches);
+}
+
+int tjsq_program_destroy(tjsq_program* P) {
+  if (!P) return TJSQ_OK;
+  tjsq_plan_destroy(P->dev);
+  (void)hipFree(P->work);
+  (void)hipFree(P->dwork);
+  delete P;
+  return TJSQ_OK;
+}
+
+int tjsq_truncation_select(int nsec, const int* den_sub, const double* eigval, int kept_min, int kept_max,
+                           double trun_err, int* counts, double* spectrum, int* order, tjsq_trunc_info* info) {
+  tjsq::TruncSelection sel;
+  std::string msg;
+  int rc = tjsq::select_truncation(nsec, den_sub, eigval, kept_min, kept_max, trun_err, sel, msg);
+  if (rc) return fail(rc, "tjsq_truncation_select: " + msg);
+  if (counts) std::memcpy(counts, sel.counts.data(), sizeof(int) * sel.counts.size());
+  if (spectrum) std::memcpy(spectrum, sel.sorted.data(), sizeof(double) * sel.sorted.size());
+  if (order) std::memcpy(order, sel.order.data(), sizeof(int) * sel.order.size());
+  fill_info(sel, info);
+  return TJSQ_OK;
+}
+
+int tjsq_truncation_operator(int nsec, const int* den_sub, const int64_t* den_off, double* den, int kept_min,
+                             int kept_max, double trun_err, int* trun_sub, int64_t* trun_off, int* trun_num,
+                             double* spectrum, tjsq_trunc_info* info, void* stream) {
+  if (nsec < 1 || !den_sub || !den_off || !den || !trun_sub || !trun_off || !trun_num)
+    return fail(TJSQ_ERR_ARG, "tjsq_truncation_operator: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  int64_t dim = 0;
+  for (int k = 0; k < nsec; ++k) {
+    if (den_sub[5 * k + 3] != den_sub[5 * k + 4] || den_sub[5 * k + 4] < 0)
+      return fail(TJSQ_ERR_ARG, "tjsq_truncation_operator: density-matrix sector is not square");
+    dim += den_sub[5 * k + 4];
+  }
+  double* ev = nullptr;
+  double* E = nullptr;
+  rocblas_int* dinfo = nullptr;
+  auto release = [&]() {
+    (void)hipFree(ev);
+    (void)hipFree(E);
+    (void)hipFree(dinfo);
+  };
+  if (hipMalloc(&ev, sizeof(double) * std::max<int64_t>(1, dim)) != hipSuccess ||
+      hipMalloc(&E, sizeof(double) * std::max<int64_t>(1, dim)) != hipSuccess ||
+      hipMalloc(&dinfo, sizeof(rocblas_int) * nsec) != hipSuccess) {
+    release();
+    return fail(TJSQ_ERR_NOMEM, "tjsq_truncation_operator: work allocation failed");
+  }
+  // DSYEV('V','U') per sector (den_mat.f90:192): ascending eigenvalues,
+  // eigenvectors overwrite the sector in place
+  {
+    std::lock_guard<std::mutex> lk(g_solver_mu);
+    rocblas_handle h = nullptr;
+    int rc = solver_handle(&h);
+    if (rc) { release(); return rc; }
+    if (rocblas_set_stream(h, st) != rocblas_status_success) {
+      release();
+      return fail(TJSQ_ERR_HIP, "rocblas_set_stream failed");
+    }
+    int64_t p = 0;
+    for (int k = 0; k < nsec; ++k) {
+      const int n = den_sub[5 * k + 4];
+      if (n > 0 && rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, den + den_off[k], n, ev + p,
+                                    E + p, dinfo + k) != rocblas_status_success) {
+        release();
+        return fail(TJSQ_ERR_HIP, "rocsolver_dsyevd failed");
+      }
+      p += n;
+    }
+  }
+  std::vector<double> hev(dim);
+  std::vector<rocblas_int> hinfo(nsec, 0);
+  if (hipMemcpyAsync(hev.data(), ev, sizeof(double) * dim, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(hinfo.data(), dinfo, sizeof(rocblas_int) * nsec, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    release();
+    return fail(TJSQ_ERR_HIP, "tjsq_truncation_operator: eigenvalue copy failed");
+  }
+  release();
+  for (int k = 0; k < nsec; ++k)
+    if (hinfo[k] != 0) return fail(TJSQ_ERR_HIP, "tjsq_truncation_operator: eigensolver did not converge");
+  tjsq::TruncSelection sel;
+  std::string msg;
+  int rc = tjsq::select_truncation(nsec, den_sub, hev.data(), kept_min, kept_max, trun_err, sel, msg);
+  if (rc) return fail(rc, "tjsq_truncation_operator: " + msg);
+  // den_mat.f90:324-367: sectors with kept states, in density-matrix order;
+  // the kept columns are the last `count` eigenvectors, contiguous in place
+  int nt = 0;
+  for (int k = 0; k < nsec; ++k) {
+    const int c = sel.counts[k];
+    if (c <= 0) continue;
+    const int n = den_sub[5 * k + 4];
+    int* t = trun_sub + 5 * nt;
+    t[0] = den_sub[5 * k]; t[1] = den_sub[5 * k + 1]; t[2] = 0; t[3] = n; t[4] = c;
+    trun_off[nt] = den_off[k] + (int64_t)(n - c) * n;
+    ++nt;
+  }
+  *trun_num = nt;
+  if (spectrum) std::memcpy(spectrum, sel.sorted.data(), sizeof(double) * sel.sorted.size());
+  fill_info(sel, info);
+  return TJSQ_OK;
+}
+
+int tjsq_identity_truncation(const tjsq_basis* basis, int* trun_sub, int64_t* trun_off, int64_t* ndata, double* data,
+                             void* stream) {
+  if (!basis || !trun_sub || !trun_off || !ndata) return fail(TJSQ_ERR_ARG, "tjsq_identity_truncation: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  int64_t off = 0;
+  for (int k = 0; k < basis->nsec; ++k) {
+    const int n = basis->sec[4 * k + 2];
+    int* t = trun_sub + 5 * k;
+    t[0] = basis->sec[4 * k]; t[1] = basis->sec[4 * k + 1]; t[2] = 0; t[3] = n; t[4] = n;
+    trun_off[k] = off;
+    if (data && n > 0) {
+      hipLaunchKernelGGL(k_identity, dim3(grid_for((int64_t)n * n)), dim3(256), 0, st, data + off, (int64_t)n);
+      HIPCHK(hipGetLastError());
+    }
+    off += (int64_t)n * n;
+  }
+  *ndata = off;
+  return TJSQ_OK;
+}
+
+}  // extern "C"

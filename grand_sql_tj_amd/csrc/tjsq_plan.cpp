@@ -929,8 +929,6 @@ int read_opset(const tjsq_opset& in, int64_t base, OpSetView& out, std::string& 
 int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjsq_opset& sys, const tjsq_opset& env,
                     const tjsq_basis& sys_bs, const tjsq_basis& env_bs, int nw, const int* wsec, const Shape& shape,
                     HostPlan& P, std::string& msg) {
-  auto edge = [&](int M, int N) { return (M > shape.small || N > shape.small) ? shape.big : shape.small; };
-  std::vector<Tile> t1big, t1small;
   P = HostPlan();
   P.sys_len = sys_len;
   P.env_len = env_len;
@@ -1159,6 +1157,15 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
   if (!err.empty()) { msg = err; return TJSQ_ERR_QN; }
   if (em.bad_dims) { msg = "operator sub-block dimensions disagree with the enlarged bases"; return TJSQ_ERR_ARG; }
 
+  return compile_program(P, shape);
+}
+
+// Compiles P.contrib / P.out_blocks into the two-stage task tables.
+int compile_program(HostPlan& P, const Shape& shape) {
+  auto edge = [&](int M, int N) { return (M > shape.small || N > shape.small) ? shape.big : shape.small; };
+  std::vector<Tile> t1big, t1small;
+  P.seg1.clear(); P.seg2.clear(); P.tile1.clear(); P.tile2.clear(); P.rtile.clear();
+  P.nslots = 0; P.scratch_elems = 0;
   // ---- compile: stage 1 (T = op(A) X of two-sided contributions) ----------
   std::map<std::tuple<int64_t, int, int64_t, int, int>, int64_t> tmap;  // (a_off, a_t, x_off, M, K2) -> T offset
   std::vector<int64_t> t_of(P.contrib.size(), -1);
@@ -1169,7 +1176,7 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
     const Contrib& c = P.contrib[q];
     if (c.kind == C_BOTH) {
       fl_ref += 2.0 * c.M * c.K1 * c.K2 + 2.0 * c.M * c.K2 * c.N;
-      auto key = std::make_tuple(c.a_off, (int)c.a_t, c.x_off, c.M, c.K2);
+      auto key = std::make_tuple(c.a_off, (int)c.a_t + 2 * (int)c.x_t, c.x_off, c.M, c.K2);
       auto it = tmap.find(key);
       if (it == tmap.end()) {
         const int64_t toff = P.scratch_elems;
@@ -1178,7 +1185,7 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
         t_of[q] = toff;
         Seg s{};
         s.l_off = c.a_off; s.l_ld = c.a_ld; s.l_t = c.a_t; s.l_space = SP_OPS;
-        s.r_off = c.x_off; s.r_ld = c.x_ld; s.r_t = 0; s.r_space = SP_X;
+        s.r_off = c.x_off; s.r_ld = c.x_ld; s.r_t = c.x_t; s.r_space = SP_X;
         s.K = c.K1; s.c = 1.0;
         const int sidx = (int)P.seg1.size();
         P.seg1.push_back(s);
@@ -1201,6 +1208,10 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
     } else if (c.kind == C_RIGHT) {
       fl_ref += 2.0 * c.M * c.N * c.K2;
       opref.emplace_back(c.b_off, (int64_t)c.K2 * c.N);
+    } else if (c.kind == C_PROD) {
+      fl_ref += 2.0 * c.M * c.N * c.K1;
+      if (c.a_space == SP_OPS) opref.emplace_back(c.a_off, (int64_t)c.M * c.K1);
+      if (c.b_space == SP_OPS) opref.emplace_back(c.b_off, (int64_t)c.K1 * c.N);
     }
   }
 
@@ -1231,6 +1242,11 @@ int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjs
         s.l_off = c.x_off; s.l_ld = c.x_ld; s.l_space = SP_X;
         s.r_off = c.b_off; s.r_ld = c.b_ld; s.r_t = c.b_t; s.r_space = SP_OPS; s.K = c.K2;
         fl_exec += 2.0 * c.M * c.N * c.K2;
+        break;
+      case C_PROD:
+        s.l_off = c.a_off; s.l_ld = c.a_ld; s.l_t = c.a_t; s.l_space = c.a_space;
+        s.r_off = c.b_off; s.r_ld = c.b_ld; s.r_t = c.b_t; s.r_space = c.b_space; s.K = c.K1;
+        fl_exec += 2.0 * c.M * c.N * c.K1;
         break;
       default:
         s.l_off = t_of[order[r]]; s.l_ld = c.M; s.l_space = SP_T;

@@ -75,14 +75,19 @@ struct MergeTerm { int64_t dst_off, src_off, n; double coef; };
 
 // ---- contributions ---------------------------------------------------------
 enum : uint8_t { SP_OPS = 0, SP_X = 1, SP_T = 2, SP_IDENT = 3 };
-enum : uint8_t { C_AXPY = 0, C_LEFT = 1, C_RIGHT = 2, C_BOTH = 3 };
+// C_AXPY:  Y += c X                 C_LEFT:  Y += c op(A) X
+// C_RIGHT: Y += c X op(B)           C_BOTH:  Y += c op(A) op(X) op(B)
+// C_PROD:  Y += c op(A) op(B), A and B read from spaces a_space, b_space
+enum : uint8_t { C_AXPY = 0, C_LEFT = 1, C_RIGHT = 2, C_BOTH = 3, C_PROD = 4 };
 
 struct Contrib {
   uint8_t kind;
   uint8_t a_t, b_t;         // op(A) = A^T if a_t, op(B) = B^T if b_t
+  uint8_t x_t;              // op(X) = X^T (C_BOTH)
+  uint8_t a_space, b_space; // operand spaces of C_PROD (SP_OPS or SP_X)
   int64_t a_off, b_off;     // operator arena offsets
   int a_ld, b_ld;           // leading dimensions of the stored A, B
-  int64_t x_off; int x_ld;  // input sub-block (K1 x K2)
+  int64_t x_off; int x_ld;  // input sub-block, op(X) is K1 x K2
   int out_block;            // output sub-block id
   int M, N, K1, K2;
   double c;
@@ -142,6 +147,11 @@ struct Shape {
   int item_chunks = 32;
   int xcds = 8;             // items of a locality group go to one XCD (round-robin dispatch)
 };
+
+// Compiles P.contrib and P.out_blocks into seg1/tile1/seg2/tile2/rtile and
+// the stats' flop and operand counts.  Every output block is written exactly
+// once (a block without contributions is written with zeros).
+int compile_program(HostPlan& P, const Shape& shape);
 
 int build_host_plan(const tjsq_lattice& lat, int sys_len, int env_len, const tjsq_opset& sys,
                     const tjsq_opset& env, const tjsq_basis& sys_bs, const tjsq_basis& env_bs, int nw,

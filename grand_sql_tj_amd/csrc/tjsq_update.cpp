@@ -1,0 +1,703 @@
+// tjsq_update.cpp -- density matrix, truncation and block update programs
+// (see tjsq_update.hpp for the scheme).
+#include "tjsq_update.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace tjsq {
+
+namespace {
+
+constexpr int SU = 2;      // spin step, common.f90:11
+constexpr int NEIBT = 12;  // common.f90:34
+
+double par(int n) { return (n % 2 == 0) ? 1.0 : -1.0; }
+
+// one placement inside an enlarged sub-block: rows [r0, r0+h), columns
+// [c0, c0+w) receive coef * op(source); src < 0 is the identity (h == w)
+struct Place {
+  int r0, c0, h, w;
+  double coef;
+  int64_t src;
+  int ld;
+  uint8_t t;
+};
+struct SymSub {
+  int num_up, num_down, down_dif, row_dim, sdim;
+  std::vector<Place> pl;
+};
+// an enlarged operator (Total_Block) whose sub-blocks are placement sums
+struct SymOp {
+  bool present = false;
+  int len = 0, up_dif = 0, down_dif = 0;
+  std::vector<SymSub> sub;
+};
+
+// 1x1 site operators of Get_site_operator_su2_U0 (dmrg_sub.f90:370-473) with
+// the particle label ub
+struct Site {
+  int up_dif = 0, down_dif = 0;
+  std::vector<OpSub> sub;   // off unused
+  std::vector<double> val;
+};
+struct SiteSet { Site elec_up, elec_down, num_elec, st_sd; };
+SiteSet site_ops(int ub) {
+  SiteSet s;
+  s.elec_up.up_dif = ub; s.elec_up.down_dif = 1;
+  s.elec_up.sub = {{0, 0, 1, 1, 1, 0}}; s.elec_up.val = {-std::sqrt(2.0)};
+  s.elec_down.up_dif = -ub; s.elec_down.down_dif = 1;
+  s.elec_down.sub = {{ub, 1, -1, 1, 1, 0}}; s.elec_down.val = {-std::sqrt(2.0)};
+  s.num_elec.sub = {{0, 0, 0, 1, 1, 0}, {ub, 1, 0, 1, 1, 0}}; s.num_elec.val = {0.0, std::sqrt(2.0)};
+  s.st_sd.down_dif = SU;
+  s.st_sd.sub = {{ub, 1, 0, 1, 1, 0}}; s.st_sd.val = {std::sqrt(1.5)};
+  return s;
+}
+
+int find_sec(const BasisView& b, int up, int down) {
+  for (size_t x = 0; x < b.sec.size(); ++x)
+    if (b.sec[x].new_up == up && b.sec[x].new_down == down) return (int)x;
+  return -1;
+}
+
+struct Ctx {
+  std::string err;
+  void fail(const std::string& m) { if (err.empty()) err = m; }
+};
+
+void place_block(SymSub& d, int r0, int c0, int h, int w, double c, const OpSub& s, bool transposed, Ctx& cx) {
+  const int rows = transposed ? s.sdim : s.row_dim, cols = transposed ? s.row_dim : s.sdim;
+  if (rows != h || cols != w) { cx.fail("operator sub-block dimensions disagree with the fused basis"); return; }
+  d.pl.push_back({r0, c0, h, w, c, s.off, s.row_dim, (uint8_t)transposed});
+}
+
+// update_block_dia, block.f90:382-432
+SymOp upd_block_dia(const Op& blk, const BasisView& bs, int len, Ctx& cx) {
+  SymOp o;
+  o.present = true; o.len = len;
+  for (const MapSec& k : bs.sec) o.sub.push_back({k.new_up, k.new_down, 0, k.dim, k.dim, {}});
+  for (size_t k = 0; k < bs.sec.size(); ++k)
+    for (const MapSub& m : bs.sec[k].sub)
+      for (const OpSub& s : blk.sub)
+        if (s.num_up == m.bl_up && s.num_down == m.bl_down) {
+          const double c = std::sqrt((1.0 + bs.sec[k].new_down) / (1.0 + s.num_down));
+          place_block(o.sub[k], m.spos, m.spos, m.sdim, m.sdim, c, s, false, cx);
+          break;
+        }
+  return o;
+}
+
+// update_site_dia, block.f90:437-498
+SymOp upd_site_dia(const Site& st, const BasisView& bs, int len) {
+  SymOp o;
+  o.present = true; o.len = len;
+  for (const MapSec& k : bs.sec) o.sub.push_back({k.new_up, k.new_down, 0, k.dim, k.dim, {}});
+  for (size_t k = 0; k < bs.sec.size(); ++k)
+    for (const MapSub& m : bs.sec[k].sub)
+      for (size_t i = 0; i < st.sub.size(); ++i)
+        if (st.sub[i].num_up == m.st_up && st.sub[i].num_down == m.st_down) {
+          const double c = std::sqrt((1.0 + bs.sec[k].new_down) / (1.0 + st.sub[i].num_down));
+          o.sub[k].pl.push_back({m.spos, m.spos, m.sdim, m.sdim, st.val[i] * c, -1, 0, 0});
+          break;
+        }
+  return o;
+}
+
+// sub-block list of the non-diagonal updates (steps <1>-<2> of
+// update_block_ndia_*, update_site_ndia_*): ket sector x, bra x + shift
+void ndia_structure(const BasisView& bs, int up_dif, int dd1, SymOp& o) {
+  for (const MapSec& x : bs.sec)
+    for (int dd = -dd1; dd <= dd1; dd += SU) {
+      const int y = find_sec(bs, x.new_up + up_dif, x.new_down + dd);
+      if (y >= 0) o.sub.push_back({x.new_up, x.new_down, dd, bs.sec[y].dim, x.dim, {}});
+    }
+}
+
+// update_block_ndia_sys (side 0, block.f90:503-715) and update_block_ndia_env
+// (side 1, :720-920; fermion sign from the site's 2S)
+SymOp upd_block_ndia(const Op& blk, const BasisView& bs, int len, char flag, int side, Ctx& cx) {
+  SymOp o;
+  o.present = true; o.len = len; o.up_dif = blk.up_dif; o.down_dif = blk.down_dif;
+  const int ud = blk.up_dif, dd1 = blk.down_dif;
+  ndia_structure(bs, ud, dd1, o);
+  for (SymSub& ns : o.sub) {
+    const int rid = find_sec(bs, ns.num_up, ns.num_down);
+    const int lid = find_sec(bs, ns.num_up + ud, ns.num_down + ns.down_dif);
+    if (rid < 0 || lid < 0) continue;
+    const int j1 = ns.num_down + ns.down_dif, j3 = ns.num_down;
+    for (const MapSub& x : bs.sec[rid].sub)
+      for (int sdd = -dd1; sdd <= dd1; sdd += SU) {
+        int bid = -1;
+        for (size_t y = 0; y < blk.sub.size(); ++y)
+          if (blk.sub[y].num_up == x.bl_up && blk.sub[y].num_down == x.bl_down && blk.sub[y].down_dif == sdd) {
+            bid = (int)y;
+            break;
+          }
+        if (bid < 0) continue;
+        const int lsu = x.bl_up + ud, lsd = x.bl_down + sdd;
+        const MapSub* lm = nullptr;
+        for (const MapSub& m : bs.sec[lid].sub)
+          if (m.bl_up == lsu && m.bl_down == lsd && m.st_down == x.st_down) { lm = &m; break; }
+        if (!lm) continue;
+        double c = w6js(j1, dd1, j3, x.bl_down, x.st_down, lsd);
+        if (side == 0) {
+          c = c * std::sqrt((1.0 + j1) * (1.0 + j3));
+          c = c * par((lsd + x.st_down + j3 + dd1) / 2);
+        } else {
+          c = c * par((lsd + x.st_down + j3 + dd1) / 2) * std::sqrt((1.0 + j1) * (1.0 + j3));
+          if (c != 0.0) c = c * (flag == 'F' && x.st_down % 2 != 0 ? -1.0 : 1.0);
+        }
+        if (c != 0.0) place_block(ns, lm->spos, x.spos, lm->sdim, x.sdim, c, blk.sub[bid], false, cx);
+      }
+  }
+  return o;
+}
+
+// update_site_ndia_sys (side 0, block.f90:925-1112; fermion sign from the
+// block's 2S) and update_site_ndia_env (side 1, :1113-1305)
+SymOp upd_site_ndia(const Site& st, const BasisView& bs, int len, char flag, int side) {
+  SymOp o;
+  o.present = true; o.len = len; o.up_dif = st.up_dif; o.down_dif = st.down_dif;
+  const int ud = st.up_dif, dd1 = st.down_dif;
+  ndia_structure(bs, ud, dd1, o);
+  for (SymSub& ns : o.sub) {
+    const int rid = find_sec(bs, ns.num_up, ns.num_down);
+    const int lid = find_sec(bs, ns.num_up + ud, ns.num_down + ns.down_dif);
+    if (rid < 0 || lid < 0) continue;
+    const int j1 = ns.num_down + ns.down_dif, j3 = ns.num_down;
+    for (const MapSub& x : bs.sec[rid].sub)
+      for (int sdd = -dd1; sdd <= dd1; sdd += SU) {
+        int sid = -1;
+        for (size_t y = 0; y < st.sub.size(); ++y)
+          if (st.sub[y].num_up == x.st_up && st.sub[y].num_down == x.st_down && st.sub[y].down_dif == sdd) {
+            sid = (int)y;
+            break;
+          }
+        if (sid < 0) continue;
+        const int lsu = x.st_up + ud, lsd = x.st_down + sdd;
+        const MapSub* lm = nullptr;
+        for (const MapSub& m : bs.sec[lid].sub)
+          if (m.st_up == lsu && m.st_down == lsd && m.bl_down == x.bl_down) { lm = &m; break; }
+        if (!lm) continue;
+        double c = w6js(j1, dd1, j3, x.st_down, x.bl_down, lsd);
+        if (side == 0) {
+          c = c * par((x.bl_down + x.st_down + j1 + dd1) / 2) * std::sqrt((1.0 + j1) * (1.0 + j3));
+          if (c != 0.0) c = c * (flag == 'F' && x.bl_down % 2 != 0 ? -1.0 : 1.0);
+        } else {
+          c = c * std::sqrt((1.0 + j1) * (1.0 + j3)) * par((x.bl_down + x.st_down + j1 + dd1) / 2);
+        }
+        if (c != 0.0) ns.pl.push_back({lm->spos, x.spos, x.sdim, x.sdim, c * st.val[sid], -1, 0, 0});
+      }
+  }
+  return o;
+}
+
+// Get_hamiltonian_ndia, get_sys.f90:583-760: block (x) site bond term into H,
+// with the h.c. written explicitly for fermions (site down_dif = 1)
+void ham_ndia(const Op& blk, const Site& st, const BasisView& bs, double coef, SymOp& H, char fsign, int side,
+              Ctx& cx) {
+  const int ud = blk.up_dif, dd1 = blk.down_dif;
+  for (SymSub& hs : H.sub) {
+    const int rid = find_sec(bs, hs.num_up, hs.num_down);
+    const int lid = find_sec(bs, hs.num_up, hs.num_down + hs.down_dif);
+    if (rid < 0 || lid < 0) continue;
+    for (const MapSub& x : bs.sec[rid].sub)
+      for (int sdd = -dd1; sdd <= dd1; sdd += SU) {
+        int bid = -1;
+        for (size_t y = 0; y < blk.sub.size(); ++y)
+          if (blk.sub[y].num_up == x.bl_up && blk.sub[y].num_down == x.bl_down && blk.sub[y].down_dif == sdd) {
+            bid = (int)y;
+            break;
+          }
+        if (bid < 0) continue;
+        const int lsu = x.bl_up + ud, lsd = x.bl_down + sdd;
+        for (int nsd = std::abs(x.st_down - st.down_dif); nsd <= x.st_down + st.down_dif; nsd += SU) {
+          const int st_dd = x.st_down - nsd, nsu = x.st_up - st.up_dif;
+          int sid = -1;
+          for (size_t y = 0; y < st.sub.size(); ++y)
+            if (st.sub[y].num_up == nsu && st.sub[y].num_down == nsd && st.sub[y].down_dif == st_dd) {
+              sid = (int)y;
+              break;
+            }
+          const MapSub* lm = nullptr;
+          for (const MapSub& m : bs.sec[lid].sub)
+            if (m.bl_up == lsu && m.bl_down == lsd && m.st_down == nsd) { lm = &m; break; }
+          if (!lm || sid < 0) continue;
+          const int j1 = lsd, j2 = x.bl_down, j3 = dd1, j4 = x.st_down, j5 = nsd, j6 = hs.num_down;
+          double c1 = w6js(j1, j2, j3, j4, j5, j6);
+          c1 = c1 * std::sqrt((1.0 + j6) / (1.0 + j3)) * par((j2 + j5 + j3 + j6) / 2) * st.val[sid];
+          double sign = 1.0;
+          if (side == 0 && fsign == 'F') {
+            if (x.bl_up % 2 != 0) sign = -1.0;
+            if (x.bl_down % 2 != 0) sign = -1.0;
+          }
+          if (c1 == 0.0) continue;
+          c1 = c1 * coef * sign;
+          place_block(hs, lm->spos, x.spos, lm->sdim, x.sdim, c1, blk.sub[bid], false, cx);
+          if (st.down_dif == 1) place_block(hs, x.spos, lm->spos, x.sdim, lm->sdim, c1, blk.sub[bid], true, cx);
+        }
+      }
+  }
+}
+
+// Get_hamiltonian_dia, get_sys.f90:506-577 (FlagConjg false, 'N','N'):
+// H += coef / sqrt(1 + 2S) * A * B per sector.  A and B are embeddings over
+// the same fused-basis grid; the product is formed symbolically when one
+// factor of every overlapping pair is an identity placement.
+void ham_dia(const SymOp& A, const SymOp& B, double coef, SymOp& H, Ctx& cx) {
+  for (SymSub& hs : H.sub) {
+    int a = -1, b = -1;
+    for (size_t x = 0; x < A.sub.size(); ++x)
+      if (A.sub[x].num_up == hs.num_up && A.sub[x].num_down == hs.num_down) { a = (int)x; break; }
+    for (size_t x = 0; x < B.sub.size(); ++x)
+      if (B.sub[x].num_up == hs.num_up && B.sub[x].num_down == hs.num_down) { b = (int)x; break; }
+    if (a < 0 || b < 0) continue;
+    const double c1 = coef / std::sqrt(1.0 + hs.num_down);
+    for (const Place& p : A.sub[a].pl)
+      for (const Place& q : B.sub[b].pl) {
+        if (p.c0 + p.w <= q.r0 || q.r0 + q.h <= p.c0) continue;
+        if (p.c0 != q.r0 || p.w != q.h) { cx.fail("operator product over misaligned placements"); return; }
+        if (p.src >= 0 && q.src >= 0) { cx.fail("operator product of two data blocks is not supported"); return; }
+        const double c = c1 * p.coef * q.coef;
+        if (c == 0.0) continue;
+        if (q.src < 0)
+          hs.pl.push_back({p.r0, q.c0, p.h, q.w, c, p.src, p.ld, p.t});
+        else
+          hs.pl.push_back({p.r0, q.c0, p.h, q.w, c, q.src, q.ld, q.t});
+      }
+  }
+}
+
+// block_add_block_two, block.f90:1782-1814 (forward-only sub-block search)
+void add_two(const SymOp& b1, double coef, SymOp& nb) {
+  size_t x1 = 0;
+  for (SymSub& d : nb.sub)
+    for (size_t x = x1; x < b1.sub.size(); ++x) {
+      const SymSub& s = b1.sub[x];
+      if (s.num_up == d.num_up && s.num_down == d.num_down && s.down_dif == d.down_dif) {
+        for (Place p : s.pl) {
+          p.coef *= coef;
+          d.pl.push_back(p);
+        }
+        x1 = x + 1;
+        break;
+      }
+    }
+}
+
+// a stored operator as a SymOp (every sub-block one placement)
+SymOp as_sym(const Op& o) {
+  SymOp s;
+  s.present = true; s.len = o.len; s.up_dif = o.up_dif; s.down_dif = o.down_dif;
+  for (const OpSub& q : o.sub)
+    s.sub.push_back({q.num_up, q.num_down, q.down_dif, q.row_dim, q.sdim,
+                     {{0, 0, q.row_dim, q.sdim, 1.0, q.off, q.row_dim, 0}}});
+  return s;
+}
+
+int read_op(const tjsq_opset& in, int op, int first, Op& o, std::string& msg) {
+  const int* h = in.hdr + 4 * op;
+  o.len = h[0]; o.up_dif = h[1]; o.down_dif = h[2];
+  o.sub.resize(h[3]);
+  for (int s = 0; s < h[3]; ++s) {
+    const int* q = in.sub + 5 * (first + s);
+    OpSub& d = o.sub[s];
+    d.num_up = q[0]; d.num_down = q[1]; d.down_dif = q[2]; d.row_dim = q[3]; d.sdim = q[4];
+    d.off = in.off[first + s];
+    if (d.row_dim < 0 || d.sdim < 0 || d.off < 0 || d.off + (int64_t)d.row_dim * d.sdim > in.ndata) {
+      msg = "operator sub-block outside the data array";
+      return TJSQ_ERR_ARG;
+    }
+  }
+  return TJSQ_OK;
+}
+
+int read_ops(const tjsq_opset& in, std::vector<Op>& ops, std::string& msg) {
+  if (in.nops < 1 || !in.hdr || !in.sub || !in.off) { msg = "empty operator set"; return TJSQ_ERR_ARG; }
+  ops.resize(in.nops);
+  int first = 0;
+  for (int op = 0; op < in.nops; ++op) {
+    int rc = read_op(in, op, first, ops[op], msg);
+    if (rc) return rc;
+    first += in.hdr[4 * op + 3];
+  }
+  return TJSQ_OK;
+}
+
+void read_basis(const tjsq_basis& b, BasisView& v) {
+  v.sec.resize(b.nsec);
+  const int* q = b.sub;
+  for (int k = 0; k < b.nsec; ++k) {
+    MapSec& s = v.sec[k];
+    s.new_up = b.sec[4 * k]; s.new_down = b.sec[4 * k + 1]; s.dim = b.sec[4 * k + 2];
+    s.sub.resize(b.sec[4 * k + 3]);
+    for (auto& m : s.sub) { m = {q[0], q[1], q[2], q[3], q[4], q[5]}; q += 6; }
+  }
+}
+
+int64_t max_extent(const tjsq_opset& in) {
+  int64_t e = 0;
+  int nsub = 0;
+  for (int op = 0; op < in.nops; ++op) nsub += in.hdr[4 * op + 3];
+  for (int s = 0; s < nsub; ++s) e = std::max(e, in.off[s] + (int64_t)in.sub[5 * s + 3] * in.sub[5 * s + 4]);
+  return e;
+}
+
+// Projection of one (symbolic) operator with the truncation operator T:
+// update_trun_dia (block.f90:1311-1358) or update_trun_ndia (:1360-1503).
+// Appends the output operator to pg.out and its contributions to pg.hp.
+struct Projector {
+  Program& pg;
+  const Op& T;
+  Ctx& cx;
+  int find_t(int up, int down) const {
+    for (size_t x = 0; x < T.sub.size(); ++x)
+      if (T.sub[x].num_up == up && T.sub[x].num_down == down) return (int)x;
+    return -1;
+  }
+  int new_block(int row, int col, int group) {
+    pg.hp.out_blocks.push_back({pg.out.ndata, std::max(1, row), row, col, group});
+    pg.out.off.push_back(pg.out.ndata);
+    pg.out.ndata += (int64_t)row * col;
+    return (int)pg.hp.out_blocks.size() - 1;
+  }
+  // every placement of ori sub-block s projected into output block ob
+  void pieces(const SymSub& s, const OpSub& tl, const OpSub& tr, int ob) {
+    if (s.row_dim != tl.row_dim || s.sdim != tr.row_dim) {
+      cx.fail("truncation operator rows disagree with the operator sub-block");
+      return;
+    }
+    for (const Place& p : s.pl) {
+      if (p.coef == 0.0 || tl.sdim == 0 || tr.sdim == 0 || p.h == 0 || p.w == 0) continue;
+      Contrib q{};
+      q.out_block = ob;
+      q.c = p.coef;
+      q.M = tl.sdim;
+      q.N = tr.sdim;
+      q.a_off = tl.off + p.r0; q.a_ld = tl.row_dim; q.a_t = 1;   // T_lhs[rows]^T
+      q.b_off = tr.off + p.c0; q.b_ld = tr.row_dim; q.b_t = 0;   // T_rhs[cols]
+      if (p.src < 0) {
+        q.kind = C_PROD;
+        q.a_space = SP_OPS; q.b_space = SP_OPS;
+        q.K1 = p.h;
+      } else {
+        q.kind = C_BOTH;
+        q.x_off = p.src; q.x_ld = p.ld; q.x_t = p.t;
+        q.K1 = p.h; q.K2 = p.w;
+      }
+      pg.hp.contrib.push_back(q);
+    }
+  }
+  void project(const SymOp& ori, bool dia) {
+    const int first_sub = (int)pg.out.sub.size() / 5;
+    int nsub = 0;
+    const int gbase = (int)pg.hp.out_blocks.size();
+    if (dia) {
+      for (size_t i = 0; i < T.sub.size(); ++i) {
+        const OpSub& t = T.sub[i];
+        pg.out.sub.insert(pg.out.sub.end(), {t.num_up, t.num_down, 0, t.sdim, t.sdim});
+        const int ob = new_block(t.sdim, t.sdim, gbase + (int)i);
+        ++nsub;
+        for (const SymSub& s : ori.sub)
+          if (s.num_up == t.num_up && s.num_down == t.num_down) {
+            pieces(s, t, t, ob);
+            break;
+          }
+      }
+    } else {
+      const int ud = ori.up_dif, dd1 = ori.down_dif;
+      for (size_t x = 0; x < T.sub.size(); ++x)
+        for (int dd = -dd1; dd <= dd1; dd += SU) {
+          const OpSub& tr = T.sub[x];
+          const int ly = find_t(tr.num_up + ud, tr.num_down + dd);
+          if (ly < 0) continue;
+          const OpSub& tl = T.sub[ly];
+          pg.out.sub.insert(pg.out.sub.end(), {tr.num_up, tr.num_down, dd, tl.sdim, tr.sdim});
+          const int ob = new_block(tl.sdim, tr.sdim, gbase + (int)x);
+          ++nsub;
+          const int rid = find_t(tr.num_up, tr.num_down);
+          if (rid < 0) continue;
+          for (const SymSub& s : ori.sub)
+            if (s.num_up == tr.num_up && s.num_down == tr.num_down && s.down_dif == dd) {
+              pieces(s, tl, T.sub[rid], ob);
+              break;
+            }
+        }
+    }
+    (void)first_sub;
+    pg.out.hdr.insert(pg.out.hdr.end(), {ori.len, ori.up_dif, ori.down_dif, nsub});
+    pg.out.nops++;
+  }
+};
+
+int finish(Program& pg, const Shape& shape, Ctx& cx, std::string& msg) {
+  if (!cx.err.empty()) { msg = cx.err; return TJSQ_ERR_QN; }
+  int rc = compile_program(pg.hp, shape);
+  if (rc) { msg = "program compilation failed"; return rc; }
+  pg.hp.dim = pg.out.ndata;
+  pg.hp.stats.dim = pg.out.ndata;
+  return TJSQ_OK;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------------
+// Get_Hamiltonian_Sys (get_sys.f90:4-246) / Get_Hamiltonian_Env (:247-478)
+// --------------------------------------------------------------------------
+int build_update_program(const tjsq_lattice& lat, int side, int sys_len, int env_len, const tjsq_opset& pre,
+                         const tjsq_basis& basis, const tjsq_opset& trun, const Shape& shape, Program& pg,
+                         std::string& msg) {
+  pg = Program();
+  if (side != 0 && side != 1) { msg = "side must be 0 (system) or 1 (environment)"; return TJSQ_ERR_ARG; }
+  if (lat.ny < 1 || lat.nx_tables < 1 || sys_len < 1 || env_len < 1 ||
+      sys_len + env_len + 2 > lat.ny * lat.nx_tables) {
+    msg = "lattice too small for sys_len + env_len + 2 sites";
+    return TJSQ_ERR_ARG;
+  }
+  std::vector<Op> ops, tops;
+  int rc = read_ops(pre, ops, msg);
+  if (rc) return rc;
+  if ((pre.nops - 1) % 4 != 0) { msg = "operator count must be 1 + 4 * active sites"; return TJSQ_ERR_ARG; }
+  if ((rc = read_ops(trun, tops, msg))) return rc;
+  if (trun.nops != 1) { msg = "the truncation operator set must hold one operator"; return TJSQ_ERR_ARG; }
+  const Op& T = tops[0];
+  BasisView bs;
+  read_basis(basis, bs);
+
+  LatticeTables L;
+  build_lattice(lat, L);
+  lattice_setup(L, lat.ny, sys_len, env_len);
+  const int pre_len = side == 0 ? sys_len : env_len;
+  if (pre.hdr[0] != pre_len) { msg = "block length disagrees with sys_len / env_len"; return TJSQ_ERR_ARG; }
+  if (pre_len + 1 > L.nsite - 2) { msg = "block too long for the lattice tables"; return TJSQ_ERR_ARG; }
+  const int nact = (pre.nops - 1) / 4;
+  const int nleg_pre = L.nleg11[pre_len], nleg1 = L.nleg11[pre_len + 1];
+  if (nact < nleg_pre) { msg = "block model has fewer active sites than nleg11(len)"; return TJSQ_ERR_ARG; }
+  const int mshift = sys_len + 1;  // environment chain offset (dmrg_sub.f90:692, 989)
+  const int new_len = pre_len + 1;
+  const int jnew = L.inb1_at(pre_len + 1, pre_len + 1);
+  const int nslot = std::max({nleg_pre, nleg1, jnew < 10000 ? jnew : 0}) + 1;
+  auto op_of = [&](int i1, int q) -> const Op& { return ops[1 + 4 * (i1 - 1) + q]; };
+  const SiteSet st = site_ops(lat.up_bias);
+  Ctx cx;
+
+  // neighbours of the new site inside the block: (active index i1, bond k)
+  std::vector<std::pair<int, int>> nb;
+  for (int k = 1; k <= NEIBT; ++k) {
+    int II;
+    if (side == 0) {
+      II = L.neib_at(pre_len + 1, k);
+    } else {
+      const int I = L.neib_at(L.nposi_lat[pre_len + 1 + mshift], k);
+      II = L.nposi_dm[I] - mshift;
+    }
+    if (II > pre_len || II <= 0) continue;
+    const int i1 = L.inb1_at(II, pre_len);
+    if (i1 > 0 && i1 <= nleg_pre) nb.push_back({i1, k});
+  }
+
+  SymOp H = upd_block_dia(ops[0], bs, new_len, cx);
+  // next%sub(i)%{elec_up, elec_down, spin_sd, num_sn}
+  std::vector<SymOp> nxt[4];
+  for (auto& v : nxt) v.assign(nslot, SymOp());
+  for (int ij = 1; ij <= 5; ij += 2) {
+    for (auto& [i1, k] : nb) {
+      if (ij == 1 && L.jt[k] != 0.0) ham_ndia(op_of(i1, 0), st.elec_up, bs, L.jt[k], H, 'F', side, cx);
+      if (ij == 3 && L.jd[k] != 0.0) ham_ndia(op_of(i1, 2), st.st_sd, bs, L.jd[k], H, 'B', side, cx);
+    }
+    for (int y = 1; y <= pre_len; ++y) {
+      const int I1 = L.inb1_at(y, pre_len);
+      if (I1 > nleg_pre) continue;
+      if (ij == 1) {
+        nxt[0][I1] = upd_block_ndia(op_of(I1, 0), bs, new_len, 'F', side, cx);
+        nxt[1][I1] = upd_block_ndia(op_of(I1, 1), bs, new_len, 'F', side, cx);
+      } else if (ij == 3) {
+        nxt[2][I1] = upd_block_ndia(op_of(I1, 2), bs, new_len, 'B', side, cx);
+      } else {
+        nxt[3][I1] = upd_block_dia(op_of(I1, 3), bs, new_len, cx);
+      }
+    }
+    SymOp tmp, tmp1;
+    if (ij == 1) {
+      tmp = upd_site_ndia(st.elec_up, bs, new_len, 'F', side);
+      tmp1 = upd_site_ndia(st.elec_down, bs, new_len, 'F', side);
+    } else if (ij == 3) {
+      tmp = upd_site_ndia(st.st_sd, bs, new_len, 'B', side);
+    } else {
+      tmp = upd_site_dia(st.num_elec, bs, new_len);
+      for (auto& [i1, k] : nb) ham_dia(nxt[3][i1], tmp, L.jn[k], H, cx);
+      if (L.esite != 0.0) add_two(tmp, L.esite, H);
+    }
+    // move the embedded operators to their indices at length L + 1
+    const int q0 = ij == 1 ? 0 : (ij == 3 ? 2 : 3);
+    const int qn = ij == 1 ? 2 : 1;
+    for (int I = 1; I <= pre_len; ++I) {
+      const int I1 = L.inb1_at(I, pre_len), J = L.inb1_at(I, pre_len + 1);
+      if (J <= nleg1 && J != I1) {
+        if (I1 >= nslot) { msg = "active-site table refers past the block's sites"; return TJSQ_ERR_ARG; }
+        for (int q = q0; q < q0 + qn; ++q) nxt[q][J] = nxt[q][I1];
+      }
+    }
+    if (jnew <= 0 || jnew >= nslot) { msg = "new site is not active"; return TJSQ_ERR_ARG; }
+    nxt[q0][jnew] = tmp;
+    if (ij == 1) nxt[1][jnew] = tmp1;
+    for (int j = nleg1 + 1; j <= nleg_pre; ++j)
+      for (int q = q0; q < q0 + qn; ++q) nxt[q][j] = SymOp();
+  }
+
+  // projection with T (update_trun_dia / update_trun_ndia), output model order
+  Projector pj{pg, T, cx};
+  pj.project(H, true);
+  for (int ii = 1; ii <= nleg1; ++ii)
+    for (int q = 0; q < 4; ++q) {
+      if (!nxt[q][ii].present) { msg = "active-site operator missing after the enlargement"; return TJSQ_ERR_ARG; }
+      pj.project(nxt[q][ii], q == 3);
+    }
+  pg.x_elems = max_extent(pre);
+  pg.ops_elems = max_extent(trun);
+  return finish(pg, shape, cx, msg);
+}
+
+int build_projection_program(const tjsq_opset& ops_in, const int* ndia, const tjsq_opset& trun, const Shape& shape,
+                             Program& pg, std::string& msg) {
+  pg = Program();
+  std::vector<Op> ops, tops;
+  int rc = read_ops(ops_in, ops, msg);
+  if (rc) return rc;
+  if ((rc = read_ops(trun, tops, msg))) return rc;
+  if (trun.nops != 1) { msg = "the truncation operator set must hold one operator"; return TJSQ_ERR_ARG; }
+  Ctx cx;
+  Projector pj{pg, tops[0], cx};
+  for (size_t i = 0; i < ops.size(); ++i) pj.project(as_sym(ops[i]), !(ndia && ndia[i]));
+  pg.x_elems = max_extent(ops_in);
+  pg.ops_elems = max_extent(trun);
+  return finish(pg, shape, cx, msg);
+}
+
+// Get_density_matrix_sys (den_mat.f90:4-38): rho = psi psi^T per sector;
+// Get_density_matrix_env (:44-78): rho = psi^T psi.
+int build_density_program(int side, int len, int nw, const int* wsec, const Shape& shape, Program& pg,
+                          std::string& msg) {
+  pg = Program();
+  if ((side != 0 && side != 1) || nw < 1 || !wsec) { msg = "bad density-matrix arguments"; return TJSQ_ERR_ARG; }
+  int64_t xoff = 0;
+  for (int i = 0; i < nw; ++i) {
+    const int* w = wsec + 6 * i;
+    const int ds = w[4], de = w[5];
+    if (ds < 0 || de < 0) { msg = "negative sector dimension"; return TJSQ_ERR_ARG; }
+    const int n = side == 0 ? ds : de;
+    pg.out.sub.insert(pg.out.sub.end(), {side == 0 ? w[0] : w[2], side == 0 ? w[1] : w[3], 0, n, n});
+    pg.hp.out_blocks.push_back({pg.out.ndata, std::max(1, n), n, n, i});
+    pg.out.off.push_back(pg.out.ndata);
+    pg.out.ndata += (int64_t)n * n;
+    if (n > 0 && (side == 0 ? de : ds) > 0) {
+      Contrib q{};
+      q.kind = C_PROD;
+      q.out_block = i;
+      q.c = 1.0;
+      q.M = n; q.N = n;
+      q.a_space = SP_X; q.b_space = SP_X;
+      q.a_off = xoff; q.a_ld = std::max(1, ds);
+      q.b_off = xoff; q.b_ld = std::max(1, ds);
+      if (side == 0) { q.a_t = 0; q.b_t = 1; q.K1 = de; }
+      else { q.a_t = 1; q.b_t = 0; q.K1 = ds; }
+      pg.hp.contrib.push_back(q);
+    }
+    xoff += (int64_t)ds * de;
+  }
+  pg.out.nops = 1;
+  pg.out.hdr = {len, 0, 0, nw};
+  pg.x_elems = xoff;
+  pg.ops_elems = 0;
+  Ctx cx;
+  return finish(pg, shape, cx, msg);
+}
+
+// --------------------------------------------------------------------------
+// Get_truncation_operator, host part (den_mat.f90:141-383)
+// --------------------------------------------------------------------------
+void quicksort_double(double* a, int* idx, int lower, int upper) {
+  // a[i - 1] holds element i
+  int i = lower, j = upper;
+  const int index = (int)((lower + upper) / 2.0);
+  const double mn = a[index - 1];
+  while (i <= j) {
+    while (a[i - 1] > mn) ++i;
+    while (a[j - 1] < mn) --j;
+    if (i <= j) {
+      std::swap(a[i - 1], a[j - 1]);
+      std::swap(idx[i - 1], idx[j - 1]);
+      ++i;
+      --j;
+    }
+  }
+  if (lower < j) quicksort_double(a, idx, lower, j);
+  if (i < upper) quicksort_double(a, idx, i, upper);
+}
+
+int select_truncation(int nsec, const int* den_sub, const double* eigval, int kept_min, int kept_max,
+                      double trun_err, TruncSelection& sel, std::string& msg) {
+  sel = TruncSelection();
+  if (nsec < 0 || (nsec > 0 && (!den_sub || !eigval))) { msg = "bad truncation arguments"; return TJSQ_ERR_ARG; }
+  int dim = 0;
+  for (int k = 0; k < nsec; ++k) {
+    if (den_sub[5 * k + 3] != den_sub[5 * k + 4]) { msg = "density-matrix sector is not square"; return TJSQ_ERR_ARG; }
+    dim += den_sub[5 * k + 4];
+  }
+  std::vector<double> ev(eigval, eigval + dim);
+  std::vector<int> idx(dim), eigs(dim), sec_of(dim);
+  for (int k = 0, p = 0; k < nsec; ++k)
+    for (int i = 0; i < den_sub[5 * k + 4]; ++i, ++p) {
+      idx[p] = p + 1;
+      eigs[p] = den_sub[5 * k + 1] + 1;   // multiplicity 2S + 1
+      sec_of[p] = k;
+    }
+  double tot_pm = 0.0;
+  for (int i = 0; i < dim; ++i) tot_pm = tot_pm + ev[i] * eigs[i];
+  double entropy1 = 0.0;
+  for (int i = 0; i < dim; ++i)
+    if (ev[i] >= 1.0E-15) {
+      const double v = ev[i] / eigs[i];
+      entropy1 = entropy1 - v * std::log(v) * eigs[i];
+    }
+  for (int i = 0; i < dim; ++i) ev[i] = ev[i] / tot_pm;
+  double entropy2 = 0.0;
+  for (int i = 0; i < dim; ++i)
+    if (ev[i] >= 1.0E-15) entropy2 = entropy2 - ev[i] * std::log(ev[i]) * eigs[i];
+  for (int i = 0; i < dim; ++i) ev[i] = ev[i] * tot_pm;
+  if (dim > 0) quicksort_double(ev.data(), idx.data(), 1, dim);
+  tot_pm = 0.0;
+  for (int i = 0; i < dim; ++i) tot_pm = tot_pm + ev[i];
+  for (int i = 0; i < dim; ++i) ev[i] = ev[i] / tot_pm;
+  int num = 0;
+  double err = 0.0;
+  if (dim <= kept_min) {
+    num = dim;
+  } else {
+    for (int i = kept_min + 1; i <= dim; ++i) err = err + ev[i - 1];
+    num = kept_min;
+    // the count stops at dim: the reference would read past the spectrum
+    while (err > trun_err && num < dim) {
+      ++num;
+      err = err - ev[num - 1];
+      if (err <= trun_err || num >= kept_max) break;
+    }
+  }
+  double ent = 0.0;
+  for (int i = 0; i < dim; ++i)
+    if (ev[i] >= 1.0E-15) ent = ent - ev[i] * std::log(ev[i]);
+  sel.counts.assign(nsec, 0);
+  for (int k = 0; k < num; ++k) sel.counts[sec_of[idx[k] - 1]]++;
+  sel.num_state = num;
+  sel.dim = dim;
+  sel.trunc_err = err;
+  sel.entropy = ent;
+  sel.entropy1 = entropy1;
+  sel.entropy2 = entropy2;
+  sel.sorted = ev;
+  sel.order = idx;
+  return TJSQ_OK;
+}
+
+}  // namespace tjsq

@@ -1,0 +1,85 @@
+// tjsq_update.hpp -- density matrix, truncation and block update programs.
+//
+// After each Lanczos solve the reference (code/tjsq/dmrg_sub.f90:656-693)
+// forms the reduced density matrix of the enlarged block
+// (Get_density_matrix_sys/env, den_mat.f90:4-134), diagonalises it per
+// sector and keeps the largest eigenvectors (Get_truncation_operator,
+// den_mat.f90:141-383), then enlarges the block by one site and projects
+// every operator with the truncation matrix T (Get_Hamiltonian_Sys/Env,
+// get_sys.f90:4-478, with update_block_*, update_site_*, Get_hamiltonian_dia
+// and _ndia, update_trun_dia/ndia of block.f90:382-1503).
+//
+// Here the enlarged operators are never formed.  Every sub-block of an
+// enlarged operator is a sum of placements: c * (a sub-block of the old
+// block operator, possibly transposed) or c * identity, put at a rectangle
+// of the fused basis.  Projecting a placement with T is one GEMM chain
+//
+//     Y[eff sub-block] += c * T_lhs[rows]^T * src * T_rhs[cols]
+//
+// (or c * T_lhs[rows]^T * T_rhs[cols] for an identity), which the grouped
+// tile engine of tjsq_plan.hpp executes: space SP_OPS holds T, SP_X the old
+// block's operator data and the output is the new block's operator data.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "tjsq_plan.hpp"
+
+namespace tjsq {
+
+// Flat layout of a program's output operator set (tjsq_opset layout).
+struct OutLayout {
+  int nops = 0;
+  std::vector<int> hdr;       // 4 per operator
+  std::vector<int> sub;       // 5 per sub-block
+  std::vector<int64_t> off;   // element offset per sub-block
+  int64_t ndata = 0;
+};
+
+struct Program {
+  HostPlan hp;                // contributions and compiled task tables
+  OutLayout out;
+  int64_t x_elems = 0;        // elements the X operand must hold
+  int64_t ops_elems = 0;      // elements the SP_OPS operand must hold
+  int levels = 1;             // density programs: states summed before use
+};
+
+// Get_Hamiltonian_Sys (side 0) / Get_Hamiltonian_Env (side 1): pre is the
+// block model at length L (nops = 1 + 4 * active sites), basis the fused
+// basis of that block with one site (Get_basis), trun the truncation
+// operator over the basis sectors (one operator).  The output is the model
+// at length L + 1 with nleg11(L + 1) active sites.
+int build_update_program(const tjsq_lattice& lat, int side, int sys_len, int env_len, const tjsq_opset& pre,
+                         const tjsq_basis& basis, const tjsq_opset& trun, const Shape& shape, Program& pg,
+                         std::string& msg);
+
+// update_trun_dia (ndia[i] = 0) / update_trun_ndia (ndia[i] = 1) applied to
+// every operator of `ops` with the same truncation operator.
+int build_projection_program(const tjsq_opset& ops, const int* ndia, const tjsq_opset& trun, const Shape& shape,
+                             Program& pg, std::string& msg);
+
+// Get_density_matrix_sys (side 0) / _env (side 1) of a wavefunction with
+// sectors wsec; `len` is the block length recorded in the output operator.
+int build_density_program(int side, int len, int nw, const int* wsec, const Shape& shape, Program& pg,
+                          std::string& msg);
+
+// Host part of Get_truncation_operator: eigval holds the eigenvalues of every
+// density-matrix sector in ascending order (DSYEV order), sectors back to
+// back.  Returns the kept count per sector and the reference's diagnostics.
+struct TruncSelection {
+  int num_state = 0;
+  int dim = 0;
+  double trunc_err = 0, entropy = 0, entropy1 = 0, entropy2 = 0;
+  std::vector<int> counts;      // kept states per density sector
+  std::vector<double> sorted;   // normalised spectrum, descending (QuickSort_Double order)
+  std::vector<int> order;       // 1-based global state index of each sorted value
+};
+int select_truncation(int nsec, const int* den_sub, const double* eigval, int kept_min, int kept_max,
+                      double trun_err, TruncSelection& sel, std::string& msg);
+
+// QuickSort_Double (block.f90:288-332): descending Hoare partition with the
+// middle element as pivot, on 1-based inclusive ranges of a and idx.
+void quicksort_double(double* a, int* idx, int lower, int upper);
+
+}  // namespace tjsq

@@ -237,6 +237,90 @@ int tjsq_heff_apply_host(tjsq_plan* plan, const double* x, double* y);
 int tjsq_lanczos_solve_host(tjsq_plan* plan, double* psi, const tjsq_lanczos_opts* opts,
                             tjsq_lanczos_result* result);
 
+/* ---- block update: density matrix, truncation, enlargement ------------
+ * After a solve the reference forms the reduced density matrix, keeps its
+ * largest eigenvectors and enlarges/projects the block operators
+ * (code/tjsq/dmrg_sub.f90:656-693).  These calls do the same on the device.
+ *
+ * A program is compiled on the host by a *_create call (no GPU needed) and
+ * run with tjsq_program_run(prog, ops, x, y, stream) on device pointers:
+ *   block update / projection: ops = truncation-operator data, x = the old
+ *     block's operator data, y = the new operator data (layout from
+ *     tjsq_program_layout);
+ *   density matrix: x = psi (levels states of length D back to back), ops
+ *     unused, y = rho.
+ * Every output sub-block is written in full.  sizes[14] = {nops, nsub,
+ * ndata, x_elems, ops_elems, n_seg1, n_tile1, n_seg2, n_tile2, n_rtile,
+ * n_slots, scratch_elems, n64_1, n64_2}; the task tables have the layouts
+ * of tjsq_hostplan_tables, with space codes 0 = ops, 1 = x, 2 = stage-1
+ * scratch, 3 = identity. */
+typedef struct tjsq_program tjsq_program;
+
+/* Get_Hamiltonian_Sys (side 0) / Get_Hamiltonian_Env (side 1)
+ * (code/tjsq/get_sys.f90:4-246, 247-478): enlarge the block model `pre`
+ * (length L = sys_len or env_len, nops = 1 + 4 * active sites) by one site
+ * over the fused basis `basis` (Get_basis of the block with the site) and
+ * project every operator with the truncation operator `trun` (one operator
+ * over the basis sectors).  The result is the model at length L + 1 with
+ * nleg11(L + 1) active sites, in the operator-set layout. */
+int tjsq_block_update_create(const tjsq_lattice* lat, int side, int sys_len, int env_len, const tjsq_opset* pre,
+                             const tjsq_basis* basis, const tjsq_opset* trun, tjsq_program** prog);
+
+/* update_trun_dia (ndia[i] = 0) / update_trun_ndia (ndia[i] = 1) of every
+ * operator of ops with one truncation operator (code/tjsq/block.f90:
+ * 1311-1503); ndia may be NULL (all diagonal). */
+int tjsq_trun_project_create(const tjsq_opset* ops, const int* ndia, const tjsq_opset* trun, tjsq_program** prog);
+
+/* Get_density_matrix_sys (side 0) / _env (side 1), levels >= 1 states summed
+ * and normalised first (Get_density_matrix_*_new, code/tjsq/den_mat.f90:
+ * 4-134).  The output operator has one sub-block per wavefunction sector. */
+int tjsq_density_create(int side, int len, int nw, const int* wsec, int levels, tjsq_program** prog);
+
+int tjsq_program_sizes(const tjsq_program* prog, int64_t* sizes);
+int tjsq_program_layout(const tjsq_program* prog, int* hdr, int* sub, int64_t* off);
+int tjsq_program_tables(const tjsq_program* prog, void* seg1, void* tile1, void* seg2, void* tile2, void* rtile);
+int tjsq_program_stats(const tjsq_program* prog, tjsq_plan_stats* stats);
+int tjsq_program_run(tjsq_program* prog, const double* ops, const double* x, double* y, void* stream);
+int tjsq_program_profile(tjsq_program* prog, int enable);
+int tjsq_program_profile_read(tjsq_program* prog, double* ms, int64_t* launches);
+int tjsq_program_destroy(tjsq_program* prog);
+
+typedef struct tjsq_trunc_info {
+  int num_state;       /* kept states (Num_state) */
+  int dim;             /* states of the density matrix */
+  double trunc_err;    /* discarded weight of the normalised spectrum */
+  double entropy;      /* Entropys: -sum w log w of the normalised spectrum */
+  double entropy1;     /* entropy with multiplet weights (den_mat.f90:211-222) */
+  double entropy2;     /* (den_mat.f90:226-235) */
+} tjsq_trunc_info;
+
+/* Get_truncation_operator (code/tjsq/den_mat.f90:141-383) on a density
+ * matrix in device memory (den: sub table den_sub[5*nsec], offsets den_off).
+ * Each sector is diagonalised in place (ascending eigenvalues, eigenvectors
+ * overwrite the sector); the kept states follow the reference: global
+ * descending sort (QuickSort_Double order), kept_min, growth while the
+ * discarded weight exceeds trun_err, cap kept_max.  The truncation operator
+ * is returned as a layout into den: trun_sub[5*trun_num] and trun_off (the
+ * kept columns of a sector are its last eigenvectors).  spectrum (host, may
+ * be NULL) receives the sorted normalised spectrum. */
+int tjsq_truncation_operator(int nsec, const int* den_sub, const int64_t* den_off, double* den, int kept_min,
+                             int kept_max, double trun_err, int* trun_sub, int64_t* trun_off, int* trun_num,
+                             double* spectrum, tjsq_trunc_info* info, void* stream);
+
+/* Host part of Get_truncation_operator on given per-sector ascending
+ * eigenvalues (no GPU): kept count per sector, sorted spectrum and the
+ * 1-based state index of each sorted value (QuickSort_Double's permutation,
+ * code/tjsq/block.f90:288-332); any output may be NULL. */
+int tjsq_truncation_select(int nsec, const int* den_sub, const double* eigval, int kept_min, int kept_max,
+                           double trun_err, int* counts, double* spectrum, int* order, tjsq_trunc_info* info);
+
+/* Identity truncation of the warm-up when the fused basis fits in kept_max
+ * (block_transfer_basis + unit matrices, code/tjsq/dmrg_sub.f90:659-667 and
+ * code/tjsq/block.f90:1645-1673): layout of one sub-block per basis sector;
+ * data (device, may be NULL) receives the unit matrices. */
+int tjsq_identity_truncation(const tjsq_basis* basis, int* trun_sub, int64_t* trun_off, int64_t* ndata, double* data,
+                             void* stream);
+
 /* Last error message of the calling thread (static storage). */
 const char* tjsq_last_error(void);
 

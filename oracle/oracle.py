@@ -60,6 +60,25 @@ def oracle_lib():
             lib.oracle_get_super_basis.argtypes = [C.c_int, I32P, C.c_int, I32P, C.c_int, C.c_int, I32P]
             lib.oracle_hmerge_tables.restype = C.c_int
             lib.oracle_hmerge_tables.argtypes = [C.POINTER(Lattice), C.c_int, C.c_int, I32P, I32P, I32P]
+            lib.oracle_opset_sizes.restype = C.c_int
+            lib.oracle_opset_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                               C.POINTER(C.c_int64)]
+            lib.oracle_opset_copy.restype = None
+            lib.oracle_opset_copy.argtypes = [C.c_void_p, I32P, I32P, I64P, F64P]
+            lib.oracle_opset_free.restype = None
+            lib.oracle_opset_free.argtypes = [C.c_void_p]
+            lib.oracle_get_hamiltonian.restype = C.c_void_p
+            lib.oracle_get_hamiltonian.argtypes = ([C.POINTER(Lattice), C.c_int, C.c_int, C.c_int] + _model_args()
+                                                   + [C.c_int, C.c_int, I32P, I32P, I32P, I32P, I64P, F64P])
+            lib.oracle_update_trun.restype = C.c_void_p
+            lib.oracle_update_trun.argtypes = _model_args() + [I32P, I32P, I32P, I64P, F64P]
+            lib.oracle_density_matrix.restype = C.c_void_p
+            lib.oracle_density_matrix.argtypes = [C.c_int, C.c_int, C.c_int, I32P, C.c_int, F64P]
+            lib.oracle_quicksort_double.restype = None
+            lib.oracle_quicksort_double.argtypes = [F64P, I32P, C.c_int]
+            lib.oracle_truncation_select.restype = C.c_int
+            lib.oracle_truncation_select.argtypes = [C.c_int, I32P, F64P, C.c_int, C.c_int, C.c_double, I32P, F64P,
+                                                     F64P, I32P]
             _libs["oracle"] = lib
         return _libs["oracle"]
 
@@ -89,6 +108,22 @@ def ref_lib():
             lib.refb_get_basis.restype = C.c_int
             lib.refb_get_basis.argtypes = [C.c_int, C.c_int, I32P, C.c_int, I32P, I32P, I32P, C.POINTER(C.c_int),
                                            C.POINTER(C.c_int)]
+            lib.refb_result_sizes.restype = C.c_int
+            lib.refb_result_sizes.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+            lib.refb_result_copy.restype = C.c_int
+            lib.refb_result_copy.argtypes = [I32P, I32P, I64P, F64P]
+            lib.refb_get_hamiltonian.restype = C.c_int
+            lib.refb_get_hamiltonian.argtypes = ([C.c_int, C.c_int, C.c_int] + margs
+                                                 + [C.c_int, C.c_int, I32P, I32P, I32P, I32P, I64P, F64P])
+            lib.refb_update_trun.restype = C.c_int
+            lib.refb_update_trun.argtypes = margs + [I32P, I32P, I32P, I64P, F64P]
+            lib.refb_density.restype = C.c_int
+            lib.refb_density.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, I32P, C.c_int, F64P]
+            lib.refb_truncation.restype = C.c_int
+            lib.refb_truncation.argtypes = [I32P, I32P, I64P, F64P, C.c_int, C.c_int, C.c_double, C.c_int,
+                                            C.POINTER(C.c_int), C.POINTER(C.c_double)]
+            lib.refb_quicksort.restype = C.c_int
+            lib.refb_quicksort.argtypes = [C.c_int, F64P, I32P]
             _libs["ref"] = lib
         return _libs["ref"]
 
@@ -241,3 +276,173 @@ def dropin_lanczos(model, sb, x0, lanzero=1e-10):
                                     *_bs(sb.env_bs), int(sb.wsec.shape[0]), np.ascontiguousarray(sb.wsec), x,
                                     C.byref(e0))
     return e0.value, x
+
+
+# ---------------------------------------------------------------------------
+# block update: density matrix, truncation, enlargement and projection
+# ---------------------------------------------------------------------------
+def _opset(length, hdr, sub, off, data):
+    from types import SimpleNamespace
+    return SimpleNamespace(length=int(length), nops=int(hdr.shape[0]), hdr=hdr, sub=sub, off=off, data=data)
+
+
+def _take_oracle(handle, length=None):
+    lib = oracle_lib()
+    if not handle:
+        raise RuntimeError("oracle call failed (inconsistent tables)")
+    no, ns, nd = C.c_int(0), C.c_int(0), C.c_int64(0)
+    lib.oracle_opset_sizes(handle, C.byref(no), C.byref(ns), C.byref(nd))
+    hdr = np.zeros((no.value, 4), np.int32)
+    sub = np.zeros((ns.value, 5), np.int32)
+    off = np.zeros(ns.value, np.int64)
+    data = np.zeros(max(1, nd.value), np.float64)
+    lib.oracle_opset_copy(handle, hdr, sub, off, data)
+    lib.oracle_opset_free(handle)
+    return _opset(hdr[0, 0] if length is None else length, hdr, sub, off, data[:nd.value])
+
+
+def _take_ref(length=None):
+    lib = ref_lib()
+    no, ns, nd = C.c_int(0), C.c_int(0), C.c_int64(0)
+    lib.refb_result_sizes(C.byref(no), C.byref(ns), C.byref(nd))
+    hdr = np.zeros((no.value, 4), np.int32)
+    sub = np.zeros((ns.value, 5), np.int32)
+    off = np.zeros(ns.value, np.int64)
+    data = np.zeros(max(1, nd.value), np.float64)
+    lib.refb_result_copy(hdr, sub, off, data)
+    return _opset(hdr[0, 0] if length is None else length, hdr, sub, off, data[:nd.value])
+
+
+def _one(op):
+    """Flat arrays of a one-operator set (truncation operators)."""
+    return [np.ascontiguousarray(op.hdr[0], np.int32), np.ascontiguousarray(op.sub, np.int32),
+            np.ascontiguousarray(op.off, np.int64), np.ascontiguousarray(op.data, np.float64)]
+
+
+def get_hamiltonian(model, side, sys_len, env_len, pre, basis, trun):
+    """Oracle Get_Hamiltonian_Sys (side 0) / _Env (side 1): the new block model."""
+    lib = oracle_lib()
+    lat = lattice_struct(model)
+    h = lib.oracle_get_hamiltonian(C.byref(lat), side, sys_len, env_len, *_m(pre), basis.length, *_bs(basis),
+                                   *_one(trun))
+    return _take_oracle(h)
+
+
+def update_trun(ops, ndia, trun):
+    """Oracle update_trun_dia / update_trun_ndia of every operator."""
+    h = oracle_lib().oracle_update_trun(*_m(ops), np.ascontiguousarray(ndia, np.int32), *_one(trun))
+    return _take_oracle(h)
+
+
+def density_matrix(side, length, wsec, psi, levels=1):
+    """Oracle Get_density_matrix_sys/env_new."""
+    w = np.ascontiguousarray(wsec, np.int32).reshape(-1, 6)
+    h = oracle_lib().oracle_density_matrix(side, length, w.shape[0], w, levels,
+                                          np.ascontiguousarray(psi, np.float64))
+    return _take_oracle(h, length)
+
+
+def quicksort_double(arr):
+    """Oracle QuickSort_Double: (values, 1-based permutation)."""
+    a = np.ascontiguousarray(arr, np.float64).copy()
+    idx = np.arange(1, a.shape[0] + 1, dtype=np.int32)
+    oracle_lib().oracle_quicksort_double(a, idx, a.shape[0])
+    return a, idx
+
+
+def truncation_select(den_sub, eigval, kept_min, kept_max, trun_err):
+    """Oracle selection of Get_truncation_operator on given eigenvalues:
+    (counts, info dict, sorted spectrum, order)."""
+    den_sub = np.ascontiguousarray(den_sub, np.int32).reshape(-1, 5)
+    ev = np.ascontiguousarray(eigval, np.float64)
+    counts = np.zeros(den_sub.shape[0], np.int32)
+    info = np.zeros(5, np.float64)
+    srt = np.zeros(ev.shape[0], np.float64)
+    order = np.zeros(ev.shape[0], np.int32)
+    oracle_lib().oracle_truncation_select(den_sub.shape[0], den_sub, ev, kept_min, kept_max, trun_err, counts, info,
+                                          srt, order)
+    d = {"num_state": int(info[0]), "trunc_err": info[1], "entropy": info[2], "entropy1": info[3],
+         "entropy2": info[4]}
+    return counts, d, srt, order
+
+
+def truncation_operator(den, kept_min, kept_max, trun_err):
+    """Oracle Get_truncation_operator: per-sector symmetric eigenproblems by
+    numpy.linalg.eigh (LAPACK syevd; the reference calls DSYEV on the same
+    matrices, den_mat.f90:192), then the restated selection.  Returns
+    (trun, counts, info, spectrum, eigvals per sector)."""
+    subs = np.asarray(den.sub).reshape(-1, 5)
+    evs, vecs = [], []
+    for s in range(subs.shape[0]):
+        n = int(subs[s, 4])
+        o = int(den.off[s])
+        m = np.asarray(den.data[o:o + n * n]).reshape(n, n).T
+        w, v = np.linalg.eigh(m)
+        evs.append(w)
+        vecs.append(v)
+    ev = np.concatenate(evs) if evs else np.zeros(0)
+    counts, info, spec, _ = truncation_select(subs, ev, kept_min, kept_max, trun_err)
+    tsub, toff, chunks, nd = [], [], [], 0
+    for s in range(subs.shape[0]):
+        c = int(counts[s])
+        if c <= 0:
+            continue
+        n = int(subs[s, 4])
+        tsub.append((subs[s, 0], subs[s, 1], 0, n, c))
+        toff.append(nd)
+        chunks.append(np.asarray(vecs[s][:, n - c:], np.float64).T.reshape(-1))
+        nd += n * c
+    trun = _opset(den.length, np.array([[den.length, 0, 0, len(tsub)]], np.int32),
+                  np.array(tsub, np.int32).reshape(-1, 5), np.array(toff, np.int64),
+                  np.concatenate(chunks) if chunks else np.zeros(0))
+    return trun, counts, info, spec, evs
+
+
+def ref_get_hamiltonian(model, side, sys_len, env_len, pre, basis, trun):
+    """Get_Hamiltonian_Sys / _Env of the reference build."""
+    with _lock:
+        lib = _ref_init(model)
+        lib.refb_get_hamiltonian(side, sys_len, env_len, *_m(pre), basis.length, *_bs(basis), *_one(trun))
+        return _take_ref()
+
+
+def ref_update_trun(ops, ndia, trun):
+    with _lock:
+        lib = ref_lib()
+        lib.refb_update_trun(*_m(ops), np.ascontiguousarray(ndia, np.int32), *_one(trun))
+        return _take_ref()
+
+
+def ref_density_matrix(side, sys_len, env_len, wsec, psi, levels=1):
+    """Get_density_matrix_sys/env_new of the reference build (wave lengths
+    sys_len / env_len are the enlarged block lengths)."""
+    w = np.ascontiguousarray(wsec, np.int32).reshape(-1, 6)
+    with _lock:
+        lib = ref_lib()
+        lib.refb_density(side, sys_len, env_len, w.shape[0], w, levels, np.ascontiguousarray(psi, np.float64))
+        return _take_ref()
+
+
+def ref_truncation(den, kept_min, kept_max, trun_err, workdir, system_block=True):
+    """Get_truncation_operator of the reference build, run with the working
+    directory set to ``workdir`` (it appends to three log files there).
+    Returns (trun, num_state, Entropys)."""
+    ns, ent = C.c_int(0), C.c_double(0.0)
+    with _lock:
+        lib = ref_lib()
+        cwd = os.getcwd()
+        os.chdir(workdir)
+        try:
+            lib.refb_truncation(*_one(den), kept_min, kept_max, trun_err, int(system_block), C.byref(ns),
+                                C.byref(ent))
+        finally:
+            os.chdir(cwd)
+        return _take_ref(), ns.value, ent.value
+
+
+def ref_quicksort(arr):
+    a = np.ascontiguousarray(arr, np.float64).copy()
+    idx = np.arange(1, a.shape[0] + 1, dtype=np.int32)
+    with _lock:
+        ref_lib().refb_quicksort(a.shape[0], a, idx)
+    return a, idx

@@ -15,7 +15,21 @@
 module ref_bridge_util
   use pubdata
   implicit none
+  ! operators returned by the last refb_* call that produces operators
+  type(Total_Block), allocatable, save :: res(:)
 contains
+  subroutine set_result(n)
+    integer, intent(in) :: n
+    integer :: i
+    if (allocated(res)) then
+      do i = 1, size(res)
+        call free_block(res(i))
+      end do
+      deallocate(res)
+    end if
+    allocate(res(n))
+  end subroutine set_result
+
   subroutine put_block(b, len, hdr, sub, off, data)
     type(Total_Block), intent(inout) :: b
     integer, intent(in) :: len, hdr(4), sub(5, *)
@@ -406,3 +420,206 @@ integer(c_int) function refb_get_basis(blen, nb, blk, ns, site, sec, sub, nsec, 
   nsub = q
   refb_get_basis = 0
 end function refb_get_basis
+
+! Sizes of the operators held from the last producing call.
+integer(c_int) function refb_result_sizes(nops, nsub, ndata) bind(C, name="refb_result_sizes")
+  use iso_c_binding
+  use ref_bridge_util
+  implicit none
+  integer(c_int), intent(out) :: nops, nsub
+  integer(c_int64_t), intent(out) :: ndata
+  integer :: o, i
+  nops = 0; nsub = 0; ndata = 0
+  if (allocated(res)) then
+    nops = size(res)
+    do o = 1, size(res)
+      if (.not. allocated(res(o)%sub)) cycle
+      do i = 1, res(o)%num
+        nsub = nsub + 1
+        ndata = ndata + int(res(o)%sub(i)%row_dim, 8) * res(o)%sub(i)%sdim
+      end do
+    end do
+  end if
+  refb_result_sizes = 0
+end function refb_result_sizes
+
+! The held operators in the flat layout.
+integer(c_int) function refb_result_copy(hdr, sub, off, data) bind(C, name="refb_result_copy")
+  use iso_c_binding
+  use ref_bridge_util
+  implicit none
+  integer(c_int), intent(inout) :: hdr(4, *), sub(5, *)
+  integer(c_int64_t), intent(inout) :: off(*)
+  real(c_double), intent(inout) :: data(*)
+  integer :: o, i, r, c, s
+  integer(8) :: nd
+  s = 0; nd = 0
+  do o = 1, size(res)
+    hdr(1, o) = res(o)%len; hdr(2, o) = res(o)%up_dif; hdr(3, o) = res(o)%down_dif; hdr(4, o) = 0
+    if (.not. allocated(res(o)%sub)) cycle
+    hdr(4, o) = res(o)%num
+    do i = 1, res(o)%num
+      s = s + 1
+      sub(1, s) = res(o)%sub(i)%num_up; sub(2, s) = res(o)%sub(i)%num_down
+      sub(3, s) = res(o)%sub(i)%down_dif; sub(4, s) = res(o)%sub(i)%row_dim; sub(5, s) = res(o)%sub(i)%sdim
+      off(s) = nd
+      do c = 1, res(o)%sub(i)%sdim
+        do r = 1, res(o)%sub(i)%row_dim
+          data(nd + (c - 1) * res(o)%sub(i)%row_dim + r) = res(o)%sub(i)%mat(r, c)
+        end do
+      end do
+      nd = nd + int(res(o)%sub(i)%row_dim, 8) * res(o)%sub(i)%sdim
+    end do
+  end do
+  refb_result_copy = 0
+end function refb_result_copy
+
+! Get_Hamiltonian_Sys (side 0) / Get_Hamiltonian_Env (side 1) on a block model
+! of length sys_len (or env_len) at the superblock (sys_len, env_len); mshift
+! as the sweep sets it (dmrg_sub.f90:692, 989).  Result: the new model's
+! Hamiltonian and nleg11(len+1) x {elec_up, elec_down, spin_sd, num_sn}.
+integer(c_int) function refb_get_hamiltonian(side, sys_len, env_len, pre_nops, pre_hdr, pre_sub, pre_off, &
+    pre_data, blen, nsec, bsec, bsub, trun_hdr, trun_sub, trun_off, trun_data) bind(C, name="refb_get_hamiltonian")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: side, sys_len, env_len, pre_nops, blen, nsec
+  integer(c_int), intent(in) :: pre_hdr(4, *), pre_sub(5, *), bsec(4, *), bsub(6, *), trun_hdr(4), trun_sub(5, *)
+  integer(c_int64_t), intent(in) :: pre_off(*), trun_off(*)
+  real(c_double), intent(in) :: pre_data(*), trun_data(*)
+  type(Total_Model) :: pre, nxt
+  type(Total_Basis) :: basis
+  type(Total_Block) :: trun
+  integer :: plen, n1, ii
+  plen = sys_len
+  if (side == 1) plen = env_len
+  call put_model(pre, plen, pre_nops, pre_hdr, pre_sub, pre_off, pre_data)
+  call put_basis(basis, blen, nsec, bsec, bsub)
+  call put_block(trun, blen, trun_hdr, trun_sub, trun_off, trun_data)
+  call dmrg_setup(sys_len, env_len)
+  mshift = sys_len + 1
+  if (side == 0) then
+    call Get_Hamiltonian_Sys(pre, nxt, basis, trun)
+  else
+    call Get_Hamiltonian_Env(pre, nxt, basis, trun)
+  end if
+  n1 = nleg11(plen + 1)
+  call set_result(1 + 4 * n1)
+  call block_transfer(pre%ham, res(1))
+  do ii = 1, n1
+    call block_transfer(pre%sub(ii)%elec_up, res(2 + 4 * (ii - 1)))
+    call block_transfer(pre%sub(ii)%elec_down, res(3 + 4 * (ii - 1)))
+    call block_transfer(pre%sub(ii)%spin_sd, res(4 + 4 * (ii - 1)))
+    call block_transfer(pre%sub(ii)%num_sn, res(5 + 4 * (ii - 1)))
+  end do
+  call free_model(pre)
+  call free_model(nxt)
+  call free_block(trun)
+  refb_get_hamiltonian = 0
+end function refb_get_hamiltonian
+
+! update_trun_dia (ndia(o) = 0) / update_trun_ndia (ndia(o) = 1) of each operator.
+integer(c_int) function refb_update_trun(nops, hdr, sub, off, data, ndia, trun_hdr, trun_sub, trun_off, &
+    trun_data) bind(C, name="refb_update_trun")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: nops
+  integer(c_int), intent(in) :: hdr(4, *), sub(5, *), ndia(*), trun_hdr(4), trun_sub(5, *)
+  integer(c_int64_t), intent(in) :: off(*), trun_off(*)
+  real(c_double), intent(in) :: data(*), trun_data(*)
+  type(Total_Block) :: ori, trun
+  integer :: o, first
+  call put_block(trun, trun_hdr(1), trun_hdr, trun_sub, trun_off, trun_data)
+  call set_result(nops)
+  first = 1
+  do o = 1, nops
+    call put_block(ori, hdr(1, o), hdr(:, o), sub(1, first), off(first), data)
+    if (ndia(o) /= 0) then
+      call update_trun_ndia(ori, res(o), trun)
+    else
+      call update_trun_dia(ori, res(o), trun)
+    end if
+    call free_block(ori)
+    first = first + hdr(4, o)
+  end do
+  call free_block(trun)
+  refb_update_trun = 0
+end function refb_update_trun
+
+! Get_density_matrix_sys_new (side 0) / _env_new (side 1) of `levels` states
+! stored back to back in psi.
+integer(c_int) function refb_density(side, sys_len, env_len, nw, wsec, levels, psi) bind(C, name="refb_density")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: side, sys_len, env_len, nw, levels
+  integer(c_int), intent(in) :: wsec(6, *)
+  real(c_double), intent(in) :: psi(*)
+  type(Wavefunction), allocatable :: w(:)
+  integer :: l, i
+  integer(8) :: d
+  d = 0
+  do i = 1, nw
+    d = d + int(wsec(5, i), 8) * wsec(6, i)
+  end do
+  allocate(w(levels))
+  do l = 1, levels
+    call put_wave(w(l), sys_len, env_len, nw, wsec, psi((l - 1) * d + 1))
+  end do
+  call set_result(1)
+  if (side == 0) then
+    call Get_density_matrix_sys_new(res(1), w, levels)
+  else
+    call Get_density_matrix_env_new(res(1), w, levels)
+  end if
+  do l = 1, levels
+    call deallocate_wave(w(l))
+  end do
+  refb_density = 0
+end function refb_density
+
+! Get_truncation_operator (den_mat.f90:141-383) with kept_min / kept_max and
+! the error bound given; writes the reference's log files into the current
+! directory.  Result: the truncation operator; num_state and Entropys out.
+integer(c_int) function refb_truncation(den_hdr, den_sub, den_off, den_data, kmin, kmax, terr, sysblk, &
+    num_state, entropy) bind(C, name="refb_truncation")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), intent(in) :: den_hdr(4), den_sub(5, *)
+  integer(c_int64_t), intent(in) :: den_off(*)
+  real(c_double), intent(in) :: den_data(*)
+  integer(c_int), value :: kmin, kmax, sysblk
+  real(c_double), value :: terr
+  integer(c_int), intent(out) :: num_state
+  real(c_double), intent(out) :: entropy
+  type(Total_Block) :: den
+  integer :: ns
+  call put_block(den, den_hdr(1), den_hdr, den_sub, den_off, den_data)
+  kept_min = kmin
+  kept_max = kmax
+  system_block = sysblk /= 0
+  call set_result(1)
+  ns = 0
+  call Get_truncation_operator(den, res(1), terr, ns)
+  num_state = ns
+  entropy = Entropys
+  call free_block(den)
+  refb_truncation = 0
+end function refb_truncation
+
+! QuickSort_Double (block.f90:288-332) over arr(1:n), idx permuted alike.
+integer(c_int) function refb_quicksort(n, arr, idx) bind(C, name="refb_quicksort")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: n
+  real(c_double), intent(inout) :: arr(*)
+  integer(c_int), intent(inout) :: idx(*)
+  if (n > 0) call QuickSort_Double(arr, idx, 1, n, n)
+  refb_quicksort = 0
+end function refb_quicksort

@@ -1302,3 +1302,586 @@ int oracle_get_super_basis(int nsys, const int* sys_sec, int nenv, const int* en
     }
   return n;
 }
+
+/* ================================================================== */
+/* Block update: density matrix, truncation, enlargement, projection   */
+/* (den_mat.f90, get_sys.f90:4-762, block.f90:382-1503).  Dense        */
+/* restatement: enlarged operators are formed explicitly, as in the    */
+/* reference, then projected.                                          */
+/* ================================================================== */
+
+struct oracle_opset { int nops; OBlock* op; };
+
+static OBlock ob_alloc(int len, int up_dif, int down_dif, int num) {
+  OBlock b; b.len = len; b.up_dif = up_dif; b.down_dif = down_dif; b.num = num; b.owns = 1;
+  b.sub = (OSub*)xcalloc((size_t)(num > 0 ? num : 1), sizeof(OSub));
+  return b;
+}
+static void ob_set(OBlock* b, int i, int up, int down, int dd, int row, int col) {
+  b->sub[i].num_up = up; b->sub[i].num_down = down; b->sub[i].down_dif = dd;
+  b->sub[i].row_dim = row; b->sub[i].sdim = col;
+  b->sub[i].mat = (double*)xcalloc((size_t)row * col + 1, sizeof(double));
+}
+/* block_transfer, block.f90:1710-1739 */
+static OBlock ob_copy(const OBlock* in) {
+  OBlock b = ob_alloc(in->len, in->up_dif, in->down_dif, in->num);
+  for (int i = 0; i < in->num; ++i) {
+    const OSub* s = &in->sub[i];
+    ob_set(&b, i, s->num_up, s->num_down, s->down_dif, s->row_dim, s->sdim);
+    memcpy(b.sub[i].mat, s->mat, sizeof(double) * (size_t)s->row_dim * s->sdim);
+  }
+  return b;
+}
+/* mat(r0:r0+h, c0:c0+w) += coef * src(h x w, leading dim lds), or src^T */
+static void add_rect(OSub* d, int r0, int c0, int h, int w, double coef, const double* src, int lds, int trans) {
+  for (int c = 0; c < w; ++c)
+    for (int r = 0; r < h; ++r) {
+      double v = trans ? src[c + (size_t)r * lds] : src[r + (size_t)c * lds];
+      double* t = &d->mat[(size_t)(r0 + r) + (size_t)(c0 + c) * d->row_dim];
+      *t = *t + coef * v;
+    }
+}
+
+/* update_block_dia, block.f90:382-432 */
+static OBlock o_update_block_dia(const OBlock* block, const OBasis* bs, int blen) {
+  OBlock nb = ob_alloc(blen, 0, 0, bs->num);
+  for (int i = 0; i < bs->num; ++i) ob_set(&nb, i, bs->sec[i].new_up, bs->sec[i].new_down, 0, bs->sec[i].dim, bs->sec[i].dim);
+  for (int k = 0; k < nb.num; ++k)
+    for (int x = 0; x < bs->sec[k].num; ++x) {
+      const OMapSub* m = &bs->sec[k].sub[x];
+      for (int i = 0; i < block->num; ++i)
+        if (block->sub[i].num_up == m->bl_up && block->sub[i].num_down == m->bl_down) {
+          double coef1 = sqrt((1.0 + nb.sub[k].num_down) / (1.0 + block->sub[i].num_down));
+          add_rect(&nb.sub[k], m->spos, m->spos, m->sdim, m->sdim, coef1, block->sub[i].mat, block->sub[i].row_dim, 0);
+          break;
+        }
+    }
+  return nb;
+}
+
+/* update_site_dia, block.f90:437-498 */
+static OBlock o_update_site_dia(const OBlock* site, const OBasis* bs, int blen) {
+  OBlock nb = ob_alloc(blen, 0, 0, bs->num);
+  for (int i = 0; i < bs->num; ++i) ob_set(&nb, i, bs->sec[i].new_up, bs->sec[i].new_down, 0, bs->sec[i].dim, bs->sec[i].dim);
+  for (int k = 0; k < nb.num; ++k)
+    for (int x = 0; x < bs->sec[k].num; ++x) {
+      const OMapSub* m = &bs->sec[k].sub[x];
+      for (int i = 0; i < site->num; ++i)
+        if (site->sub[i].num_up == m->st_up && site->sub[i].num_down == m->st_down) {
+          double coef1 = sqrt((1.0 + nb.sub[k].num_down) / (1.0 + site->sub[i].num_down));
+          for (int y = 0; y < m->sdim; ++y) {
+            double* t = &nb.sub[k].mat[(size_t)(m->spos + y) * (1 + (size_t)nb.sub[k].row_dim)];
+            *t = *t + 1.0 * site->sub[i].mat[0] * coef1;
+          }
+          break;
+        }
+    }
+  return nb;
+}
+
+/* steps <1>-<2> shared by the non-diagonal updates */
+static OBlock ndia_structure(const OBasis* bs, int blen, int up_dif, int dd1) {
+  int num = 0;
+  for (int x = 0; x < bs->num; ++x)
+    for (int dd = -dd1; dd <= dd1; dd += SU)
+      if (find_sec(bs, bs->sec[x].new_up + up_dif, bs->sec[x].new_down + dd) >= 0) ++num;
+  OBlock nb = ob_alloc(blen, up_dif, dd1, num);
+  int id = 0;
+  for (int x = 0; x < bs->num; ++x)
+    for (int dd = -dd1; dd <= dd1; dd += SU) {
+      int y = find_sec(bs, bs->sec[x].new_up + up_dif, bs->sec[x].new_down + dd);
+      if (y >= 0) ob_set(&nb, id++, bs->sec[x].new_up, bs->sec[x].new_down, dd, bs->sec[y].dim, bs->sec[x].dim);
+    }
+  return nb;
+}
+
+/* update_block_ndia_sys (side 0, block.f90:503-715) / _env (side 1, :720-920) */
+static OBlock o_update_block_ndia(const OBlock* block, const OBasis* bs, int blen, char flag, int side) {
+  int up_dif = block->up_dif, dd1 = block->down_dif;
+  OBlock nb = ndia_structure(bs, blen, up_dif, dd1);
+  for (int i = 0; i < nb.num; ++i) {
+    int rhs_up = nb.sub[i].num_up, rhs_down = nb.sub[i].num_down;
+    int rid = find_sec(bs, rhs_up, rhs_down);
+    int lhs_up = rhs_up + up_dif, lhs_down = rhs_down + nb.sub[i].down_dif;
+    int lid = find_sec(bs, lhs_up, lhs_down);
+    if (rid < 0 || lid < 0) continue;
+    for (int x = 0; x < bs->sec[rid].num; ++x) {
+      const OMapSub* r = &bs->sec[rid].sub[x];
+      for (int sdd = -dd1; sdd <= dd1; sdd += SU) {
+        int bid = -1, lsu = 0, lsd = 0;
+        for (int y = 0; y < block->num; ++y)
+          if (block->sub[y].num_up == r->bl_up && block->sub[y].num_down == r->bl_down && block->sub[y].down_dif == sdd) {
+            bid = y; lsu = r->bl_up + up_dif; lsd = r->bl_down + sdd; break;
+          }
+        if (bid < 0) continue;
+        const OMapSub* l = NULL;
+        for (int y = 0; y < bs->sec[lid].num; ++y) {
+          const OMapSub* q = &bs->sec[lid].sub[y];
+          if (q->bl_up == lsu && q->bl_down == lsd && q->st_down == r->st_down) { l = q; break; }
+        }
+        if (!l) continue;
+        int j1 = lhs_down, j2 = dd1, j3 = rhs_down, j4 = r->bl_down, j5 = r->st_down, j6 = lsd;
+        double coef = oracle_w6js(j1, j2, j3, j4, j5, j6);
+        if (side == 0) {
+          coef = coef * sqrt((1.0 + j1) * (1.0 + j3));
+          coef = coef * neg1pow((j6 + j5 + j3 + j2) / 2);
+        } else {
+          double signs = 1.0;
+          if (flag == 'F') signs = (j5 % 2 == 0) ? 1.0 : -1.0;
+          coef = coef * neg1pow((j6 + j5 + j3 + j2) / 2) * sqrt((1.0 + j1) * (1.0 + j3));
+          if (coef != 0.0) coef = coef * signs;
+        }
+        if (coef != 0.0)
+          add_rect(&nb.sub[i], l->spos, r->spos, l->sdim, r->sdim, coef, block->sub[bid].mat, block->sub[bid].row_dim, 0);
+      }
+    }
+  }
+  return nb;
+}
+
+/* update_site_ndia_sys (side 0, block.f90:925-1112) / _env (side 1, :1113-1305) */
+static OBlock o_update_site_ndia(const OBlock* site, const OBasis* bs, int blen, char flag, int side) {
+  int up_dif = site->up_dif, dd1 = site->down_dif;
+  OBlock nb = ndia_structure(bs, blen, up_dif, dd1);
+  for (int i = 0; i < nb.num; ++i) {
+    int rhs_up = nb.sub[i].num_up, rhs_down = nb.sub[i].num_down;
+    int rid = find_sec(bs, rhs_up, rhs_down);
+    int lhs_up = rhs_up + up_dif, lhs_down = rhs_down + nb.sub[i].down_dif;
+    int lid = find_sec(bs, lhs_up, lhs_down);
+    if (rid < 0 || lid < 0) continue;
+    for (int x = 0; x < bs->sec[rid].num; ++x) {
+      const OMapSub* r = &bs->sec[rid].sub[x];
+      for (int sdd = -dd1; sdd <= dd1; sdd += SU) {
+        int sid = -1, lsu = 0, lsd = 0;
+        for (int y = 0; y < site->num; ++y)
+          if (site->sub[y].num_up == r->st_up && site->sub[y].num_down == r->st_down && site->sub[y].down_dif == sdd) {
+            sid = y; lsu = r->st_up + up_dif; lsd = r->st_down + sdd; break;
+          }
+        if (sid < 0) continue;
+        const OMapSub* l = NULL;
+        for (int y = 0; y < bs->sec[lid].num; ++y) {
+          const OMapSub* q = &bs->sec[lid].sub[y];
+          if (q->st_up == lsu && q->st_down == lsd && q->bl_down == r->bl_down) { l = q; break; }
+        }
+        if (!l) continue;
+        int j1 = lhs_down, j2 = dd1, j3 = rhs_down, j4 = r->st_down, j5 = r->bl_down, j6 = lsd;
+        double coef = oracle_w6js(j1, j2, j3, j4, j5, j6);
+        if (side == 0) {
+          double signs = 1.0;
+          if (flag == 'F') signs = (j5 % 2 == 0) ? 1.0 : -1.0;
+          coef = coef * neg1pow((j5 + j4 + j1 + j2) / 2) * sqrt((1.0 + j1) * (1.0 + j3));
+          if (coef != 0.0) coef = coef * signs;
+        } else {
+          coef = coef * sqrt((1.0 + j1) * (1.0 + j3)) * neg1pow((j5 + j4 + j1 + j2) / 2);
+        }
+        if (coef != 0.0)
+          for (int y = 0; y < r->sdim; ++y) {
+            double* t = &nb.sub[i].mat[(size_t)(l->spos + y) + (size_t)(r->spos + y) * nb.sub[i].row_dim];
+            *t = *t + coef * 1.0 * site->sub[sid].mat[0];
+          }
+      }
+    }
+  }
+  return nb;
+}
+
+/* Get_hamiltonian_ndia, get_sys.f90:583-760 */
+static void o_ham_ndia(const OBlock* block, const OBlock* site, const OBasis* bs, double coef, OBlock* ham,
+                       char fsign, char flags) {
+  int up_dif = block->up_dif, dd1 = block->down_dif;
+  for (int i = 0; i < ham->num; ++i) {
+    int rhs_up = ham->sub[i].num_up, rhs_down = ham->sub[i].num_down;
+    int rid = find_sec(bs, rhs_up, rhs_down);
+    int lid = find_sec(bs, rhs_up, rhs_down + ham->sub[i].down_dif);
+    if (rid < 0 || lid < 0) continue;
+    for (int x = 0; x < bs->sec[rid].num; ++x) {
+      const OMapSub* r = &bs->sec[rid].sub[x];
+      for (int sdd = -dd1; sdd <= dd1; sdd += SU) {
+        int bid = -1, lsu = 0, lsd = 0;
+        for (int y = 0; y < block->num; ++y)
+          if (block->sub[y].num_up == r->bl_up && block->sub[y].num_down == r->bl_down && block->sub[y].down_dif == sdd) {
+            bid = y; lsu = r->bl_up + up_dif; lsd = r->bl_down + sdd; break;
+          }
+        if (bid < 0) continue;
+        int lo = abs(r->st_down - site->down_dif), hi = r->st_down + site->down_dif;
+        for (int nsd = lo; nsd <= hi; nsd += SU) {
+          int st_dd = r->st_down - nsd, nsu = r->st_up - site->up_dif, sid = -1;
+          for (int y = 0; y < site->num; ++y)
+            if (site->sub[y].num_up == nsu && site->sub[y].num_down == nsd && site->sub[y].down_dif == st_dd) { sid = y; break; }
+          const OMapSub* l = NULL;
+          for (int y = 0; y < bs->sec[lid].num; ++y) {
+            const OMapSub* q = &bs->sec[lid].sub[y];
+            if (q->bl_up == lsu && q->bl_down == lsd && q->st_down == nsd) { l = q; break; }
+          }
+          if (!l || sid < 0) continue;
+          int j1 = lsd, j2 = r->bl_down, j3 = dd1, j4 = r->st_down, j5 = nsd, j6 = rhs_down;
+          double coef1 = oracle_w6js(j1, j2, j3, j4, j5, j6);
+          coef1 = coef1 * sqrt((1.0 + j6) / (1.0 + j3)) * neg1pow((j2 + j5 + j3 + j6) / 2) * site->sub[sid].mat[0];
+          double sign = 1.0;
+          if (flags == 'S' && fsign == 'F') {
+            if (r->bl_up % 2 == 1) sign = -1.0;
+            if (r->bl_down % 2 == 1) sign = -1.0;
+          }
+          if (coef1 != 0.0) {
+            coef1 = coef1 * coef * sign;
+            add_rect(&ham->sub[i], l->spos, r->spos, l->sdim, r->sdim, coef1, block->sub[bid].mat, block->sub[bid].row_dim, 0);
+            if (site->down_dif == 1)
+              add_rect(&ham->sub[i], r->spos, l->spos, r->sdim, l->sdim, coef1, block->sub[bid].mat, block->sub[bid].row_dim, 1);
+          }
+        }
+      }
+    }
+  }
+}
+
+/* Get_hamiltonian_dia, get_sys.f90:506-577 ('N','N', FlagConjg false) */
+static void o_ham_dia(const OBlock* o1, const OBlock* o2, double coef, OBlock* ham) {
+  for (int i = 0; i < ham->num; ++i) {
+    int up = ham->sub[i].num_up, down = ham->sub[i].num_down, sdim = ham->sub[i].sdim;
+    int a = find_op(o1, up, down, 0, 0), b = find_op(o2, up, down, 0, 0);
+    if (a < 0 || b < 0) continue;
+    double* m = (double*)xcalloc((size_t)sdim * sdim + 1, sizeof(double));
+    double coef1 = coef / sqrt(1.0 + down);
+    gemm('N', 'N', sdim, sdim, sdim, coef1, o1->sub[a].mat, sdim, o2->sub[b].mat, sdim, 0.0, m, sdim);
+    for (size_t q = 0; q < (size_t)sdim * sdim; ++q) ham->sub[i].mat[q] = ham->sub[i].mat[q] + m[q];
+    free(m);
+  }
+}
+
+/* block_add_block_two, block.f90:1782-1814 */
+static void o_add_two(const OBlock* b1, double coef1, OBlock* nb) { add_block_two(b1, coef1, nb); }
+
+/* update_trun_dia, block.f90:1311-1358 */
+static OBlock o_update_trun_dia(const OBlock* ori, const OBlock* trun) {
+  OBlock eff = ob_alloc(ori->len, ori->up_dif, ori->down_dif, trun->num);
+  for (int i = 0; i < trun->num; ++i)
+    ob_set(&eff, i, trun->sub[i].num_up, trun->sub[i].num_down, 0, trun->sub[i].sdim, trun->sub[i].sdim);
+  for (int i = 0; i < eff.num; ++i) {
+    int ed = eff.sub[i].sdim;
+    for (int x = 0; x < ori->num; ++x)
+      if (ori->sub[x].num_up == eff.sub[i].num_up && ori->sub[x].num_down == eff.sub[i].num_down) {
+        int od = ori->sub[x].row_dim;
+        double* mid = (double*)xcalloc((size_t)od * ed + 1, sizeof(double));
+        gemm('N', 'N', od, ed, od, 1.0, ori->sub[x].mat, od, trun->sub[i].mat, od, 0.0, mid, od);
+        gemm('T', 'N', ed, ed, od, 1.0, trun->sub[i].mat, od, mid, od, 0.0, eff.sub[i].mat, ed);
+        free(mid);
+        break;
+      }
+  }
+  return eff;
+}
+
+static int find_trun(const OBlock* t, int up, int down) {
+  for (int y = 0; y < t->num; ++y)
+    if (t->sub[y].num_up == up && t->sub[y].num_down == down) return y;
+  return -1;
+}
+
+/* update_trun_ndia, block.f90:1360-1503 */
+static OBlock o_update_trun_ndia(const OBlock* ori, const OBlock* trun) {
+  int up_dif = ori->up_dif, dd1 = ori->down_dif, num = 0;
+  for (int x = 0; x < trun->num; ++x)
+    for (int dd = -dd1; dd <= dd1; dd += SU)
+      if (find_trun(trun, trun->sub[x].num_up + up_dif, trun->sub[x].num_down + dd) >= 0) ++num;
+  OBlock eff = ob_alloc(ori->len, up_dif, dd1, num);
+  int id = 0;
+  for (int x = 0; x < trun->num; ++x)
+    for (int dd = -dd1; dd <= dd1; dd += SU) {
+      int y = find_trun(trun, trun->sub[x].num_up + up_dif, trun->sub[x].num_down + dd);
+      if (y >= 0) ob_set(&eff, id++, trun->sub[x].num_up, trun->sub[x].num_down, dd, trun->sub[y].sdim, trun->sub[x].sdim);
+    }
+  for (int i = 0; i < eff.num; ++i) {
+    int ru = eff.sub[i].num_up, rd = eff.sub[i].num_down, dd = eff.sub[i].down_dif;
+    int rid = find_trun(trun, ru, rd), lid = find_trun(trun, ru + up_dif, rd + dd), oid = -1;
+    for (int x = 0; x < ori->num; ++x)
+      if (ori->sub[x].num_up == ru && ori->sub[x].num_down == rd && ori->sub[x].down_dif == dd) { oid = x; break; }
+    if (rid < 0 || lid < 0 || oid < 0) continue;
+    int orow = ori->sub[oid].row_dim, ocol = ori->sub[oid].sdim;
+    int rdim = trun->sub[rid].sdim, ldim = trun->sub[lid].sdim;
+    double* mid = (double*)xcalloc((size_t)orow * rdim + 1, sizeof(double));
+    gemm('N', 'N', orow, rdim, ocol, 1.0, ori->sub[oid].mat, orow, trun->sub[rid].mat, ocol, 0.0, mid, orow);
+    gemm('T', 'N', ldim, rdim, orow, 1.0, trun->sub[lid].mat, orow, mid, orow, 0.0, eff.sub[i].mat, ldim);
+    free(mid);
+  }
+  return eff;
+}
+
+static oracle_opset* opset_new(int nops) {
+  oracle_opset* r = (oracle_opset*)xcalloc(1, sizeof(oracle_opset));
+  r->nops = nops;
+  r->op = (OBlock*)xcalloc((size_t)nops, sizeof(OBlock));
+  return r;
+}
+
+int oracle_opset_sizes(const oracle_opset* r, int* nops, int* nsub, int64_t* ndata) {
+  int ns = 0; int64_t nd = 0;
+  for (int o = 0; o < r->nops; ++o)
+    for (int i = 0; i < r->op[o].num; ++i) { ++ns; nd += (int64_t)r->op[o].sub[i].row_dim * r->op[o].sub[i].sdim; }
+  *nops = r->nops; *nsub = ns; *ndata = nd;
+  return 0;
+}
+
+void oracle_opset_copy(const oracle_opset* r, int* hdr, int* sub, int64_t* off, double* data) {
+  int s = 0; int64_t nd = 0;
+  for (int o = 0; o < r->nops; ++o) {
+    const OBlock* b = &r->op[o];
+    hdr[4 * o] = b->len; hdr[4 * o + 1] = b->up_dif; hdr[4 * o + 2] = b->down_dif; hdr[4 * o + 3] = b->num;
+    for (int i = 0; i < b->num; ++i, ++s) {
+      const OSub* q = &b->sub[i];
+      sub[5 * s] = q->num_up; sub[5 * s + 1] = q->num_down; sub[5 * s + 2] = q->down_dif;
+      sub[5 * s + 3] = q->row_dim; sub[5 * s + 4] = q->sdim;
+      off[s] = nd;
+      size_t n = (size_t)q->row_dim * q->sdim;
+      memcpy(data + nd, q->mat, sizeof(double) * n);
+      nd += (int64_t)n;
+    }
+  }
+}
+
+void oracle_opset_free(oracle_opset* r) {
+  if (!r) return;
+  for (int o = 0; o < r->nops; ++o) free_block(&r->op[o]);
+  free(r->op);
+  free(r);
+}
+
+/* Get_Hamiltonian_Sys (side 0, get_sys.f90:4-246) / Get_Hamiltonian_Env
+ * (side 1, :247-478).  mshift = sys_len + 1 places the environment block's
+ * sites on the chain (dmrg_sub.f90:692, 989). */
+oracle_opset* oracle_get_hamiltonian(const oracle_lattice* lat, int side, int sys_len, int env_len, int pre_nops,
+                                     const int* pre_hdr, const int* pre_sub, const int64_t* pre_off,
+                                     const double* pre_data, int blen, int nsec, const int* bsec, const int* bsub,
+                                     const int* trun_hdr, const int* trun_sub, const int64_t* trun_off,
+                                     const double* trun_data) {
+  Lat L; lat_init(&L, lat); lat_setup(&L, lat->ny, sys_len, env_len);
+  SiteOps so; make_site_ops(&so, lat->up_bias);
+  OModel pre = view_model(pre_nops, pre_hdr, pre_sub, pre_off, pre_data);
+  OBasis bs = view_basis(nsec, bsec, bsub);
+  OBlock trun = view_op(trun_hdr, trun_sub, trun_off, trun_data);
+  int pre_len = side == 0 ? sys_len : env_len, mshift = sys_len + 1;
+  int nleg_pre = L.nleg11[pre_len], nleg1 = L.nleg11[pre_len + 1];
+  int nslot = 64, jnew = INB1(&L, pre_len + 1, pre_len + 1);
+  if (pre.nact < nleg_pre || nleg_pre >= nslot || nleg1 >= nslot || jnew >= nslot) {
+    free_model(&pre); free(bs.sec); free_block(&trun); lat_free(&L);
+    return NULL;
+  }
+  OSet* nxt = (OSet*)xcalloc((size_t)nslot, sizeof(OSet));
+  OBlock H = o_update_block_dia(&pre.ham, &bs, blen);
+  for (int ij = 1; ij <= 5; ij += 2) {
+    for (int k = 1; k <= NEIBT; ++k) {
+      int II;
+      if (side == 0) II = NEIB(&L, pre_len + 1, k);
+      else II = L.nposi_dm[NEIB(&L, L.nposi_lat[pre_len + 1 + mshift], k)] - mshift;
+      if (!(II <= pre_len && II > 0)) continue;
+      int i1 = INB1(&L, II, pre_len);
+      if (!(i1 > 0 && i1 <= nleg_pre)) continue;
+      double coefx = L.jt[k], coefsd = L.jd[k];
+      char fl = side == 0 ? 'S' : 'E';
+      if (ij == 1 && coefx != 0.0) o_ham_ndia(&pre.sub[i1].elec_up, &so.elec_up, &bs, coefx, &H, 'F', fl);
+      if (ij == 3 && coefsd != 0.0) o_ham_ndia(&pre.sub[i1].spin_sd, &so.st_sd, &bs, coefsd, &H, 'B', fl);
+    }
+    for (int y = 1; y <= pre_len; ++y) {
+      int I1 = INB1(&L, y, pre_len);
+      if (I1 > nleg_pre) continue;
+      if (ij == 1) {
+        nxt[I1].elec_up = o_update_block_ndia(&pre.sub[I1].elec_up, &bs, blen, 'F', side);
+        nxt[I1].elec_down = o_update_block_ndia(&pre.sub[I1].elec_down, &bs, blen, 'F', side);
+      } else if (ij == 3) {
+        nxt[I1].spin_sd = o_update_block_ndia(&pre.sub[I1].spin_sd, &bs, blen, 'B', side);
+      } else {
+        nxt[I1].num_sn = o_update_block_dia(&pre.sub[I1].num_sn, &bs, blen);
+      }
+    }
+    OBlock tmp, tmp1; memset(&tmp1, 0, sizeof(tmp1));
+    if (ij == 1) {
+      tmp = o_update_site_ndia(&so.elec_up, &bs, blen, 'F', side);
+      tmp1 = o_update_site_ndia(&so.elec_down, &bs, blen, 'F', side);
+    } else if (ij == 3) {
+      tmp = o_update_site_ndia(&so.st_sd, &bs, blen, 'B', side);
+    } else {
+      tmp = o_update_site_dia(&so.num_elec, &bs, blen);
+      for (int k = 1; k <= NEIBT; ++k) {
+        int II;
+        if (side == 0) II = NEIB(&L, pre_len + 1, k);
+        else II = L.nposi_dm[NEIB(&L, L.nposi_lat[pre_len + 1 + mshift], k)] - mshift;
+        if (!(II <= pre_len && II > 0)) continue;
+        int i1 = INB1(&L, II, pre_len);
+        if (!(i1 > 0 && i1 <= nleg_pre)) continue;
+        o_ham_dia(&nxt[i1].num_sn, &tmp, L.jn[k], &H);
+      }
+      if (L.esite != 0.0) o_add_two(&tmp, L.esite, &H);
+    }
+    /* reorder (block_transfer copies) */
+    for (int I = 1; I <= pre_len; ++I) {
+      int I1 = INB1(&L, I, pre_len), J = INB1(&L, I, pre_len + 1);
+      if (J <= nleg1 && J != I1) {
+        if (ij == 1) {
+          free_block(&nxt[J].elec_up); nxt[J].elec_up = ob_copy(&nxt[I1].elec_up);
+          free_block(&nxt[J].elec_down); nxt[J].elec_down = ob_copy(&nxt[I1].elec_down);
+        } else if (ij == 3) {
+          free_block(&nxt[J].spin_sd); nxt[J].spin_sd = ob_copy(&nxt[I1].spin_sd);
+        } else {
+          free_block(&nxt[J].num_sn); nxt[J].num_sn = ob_copy(&nxt[I1].num_sn);
+        }
+      }
+    }
+    int j = INB1(&L, pre_len + 1, pre_len + 1);
+    if (ij == 1) {
+      free_block(&nxt[j].elec_up); nxt[j].elec_up = ob_copy(&tmp);
+      free_block(&nxt[j].elec_down); nxt[j].elec_down = ob_copy(&tmp1);
+    } else if (ij == 3) {
+      free_block(&nxt[j].spin_sd); nxt[j].spin_sd = ob_copy(&tmp);
+    } else {
+      free_block(&nxt[j].num_sn); nxt[j].num_sn = ob_copy(&tmp);
+    }
+    for (int jj = nleg1 + 1; jj <= nleg_pre; ++jj) {
+      if (ij == 1) { free_block(&nxt[jj].elec_up); free_block(&nxt[jj].elec_down); }
+      else if (ij == 3) free_block(&nxt[jj].spin_sd);
+      else free_block(&nxt[jj].num_sn);
+    }
+    free_block(&tmp); free_block(&tmp1);
+  }
+  oracle_opset* out = opset_new(1 + 4 * nleg1);
+  out->op[0] = o_update_trun_dia(&H, &trun);
+  for (int ii = 1; ii <= nleg1; ++ii) {
+    out->op[1 + 4 * (ii - 1)] = o_update_trun_ndia(&nxt[ii].elec_up, &trun);
+    out->op[2 + 4 * (ii - 1)] = o_update_trun_ndia(&nxt[ii].elec_down, &trun);
+    out->op[3 + 4 * (ii - 1)] = o_update_trun_ndia(&nxt[ii].spin_sd, &trun);
+    out->op[4 + 4 * (ii - 1)] = o_update_trun_dia(&nxt[ii].num_sn, &trun);
+  }
+  free_block(&H);
+  for (int i = 0; i < nslot; ++i) free_set(&nxt[i]);
+  free(nxt);
+  free_model(&pre);
+  free(bs.sec);
+  free_block(&trun);
+  lat_free(&L);
+  return out;
+}
+
+/* update_trun_dia / update_trun_ndia of every operator (ndia[i] selects) */
+oracle_opset* oracle_update_trun(int nops, const int* hdr, const int* sub, const int64_t* off, const double* data,
+                                 const int* ndia, const int* trun_hdr, const int* trun_sub, const int64_t* trun_off,
+                                 const double* trun_data) {
+  OBlock trun = view_op(trun_hdr, trun_sub, trun_off, trun_data);
+  oracle_opset* out = opset_new(nops);
+  int first = 0;
+  for (int o = 0; o < nops; ++o) {
+    OBlock ori = view_op(hdr + 4 * o, sub + 5 * first, off + first, data);
+    out->op[o] = (ndia && ndia[o]) ? o_update_trun_ndia(&ori, &trun) : o_update_trun_dia(&ori, &trun);
+    first += hdr[4 * o + 3];
+    free_block(&ori);
+  }
+  free_block(&trun);
+  return out;
+}
+
+/* Get_density_matrix_sys/env_new (den_mat.f90:84-134 with :4-78): states
+ * summed (wave_transfer + add), normalised (wave_normalize, block.f90:2418),
+ * then rho = psi psi^T (side 0) or psi^T psi (side 1) per sector. */
+oracle_opset* oracle_density_matrix(int side, int len, int nw, const int* wsec, int levels, const double* psi) {
+  size_t D = 0;
+  for (int i = 0; i < nw; ++i) D += (size_t)wsec[6 * i + 4] * wsec[6 * i + 5];
+  double* v = (double*)xcalloc(D + 1, sizeof(double));
+  memcpy(v, psi, sizeof(double) * D);
+  for (int l = 1; l < levels; ++l)
+    for (size_t q = 0; q < D; ++q) v[q] = v[q] + psi[(size_t)l * D + q];
+  double nrm = 0.0;
+  for (size_t q = 0; q < D; ++q) nrm += v[q] * v[q];
+  nrm = sqrt(nrm);
+  for (size_t q = 0; q < D; ++q) v[q] = v[q] / nrm;
+  oracle_opset* out = opset_new(1);
+  OBlock* b = &out->op[0];
+  *b = ob_alloc(len, 0, 0, nw);
+  size_t o = 0;
+  for (int i = 0; i < nw; ++i) {
+    const int* w = wsec + 6 * i;
+    int ds = w[4], de = w[5];
+    if (side == 0) {
+      ob_set(b, i, w[0], w[1], 0, ds, ds);
+      gemm('N', 'T', ds, ds, de, 1.0, v + o, ds, v + o, ds, 0.0, b->sub[i].mat, ds);
+    } else {
+      ob_set(b, i, w[2], w[3], 0, de, de);
+      gemm('T', 'N', de, de, ds, 1.0, v + o, ds, v + o, ds, 0.0, b->sub[i].mat, de);
+    }
+    o += (size_t)ds * de;
+  }
+  free(v);
+  return out;
+}
+
+/* QuickSort_Double, block.f90:288-332 (arr/idx 1-based inclusive ranges) */
+static void qsort_double(double* arr, int* idx, int lower, int upper) {
+  int i = lower, j = upper;
+  int index = (int)((lower + upper) / 2.0);
+  double mn = arr[index];
+  while (i <= j) {
+    while (arr[i] > mn) i = i + 1;
+    while (arr[j] < mn) j = j - 1;
+    if (i <= j) {
+      double t = arr[i]; arr[i] = arr[j]; arr[j] = t;
+      int ti = idx[i]; idx[i] = idx[j]; idx[j] = ti;
+      i = i + 1; j = j - 1;
+    }
+  }
+  if (lower < j) qsort_double(arr, idx, lower, j);
+  if (i < upper) qsort_double(arr, idx, i, upper);
+}
+
+void oracle_quicksort_double(double* arr, int* idx, int n) {
+  if (n > 0) qsort_double(arr - 1, idx - 1, 1, n);
+}
+
+/* Get_truncation_operator's selection (den_mat.f90:196-328) on given
+ * ascending per-sector eigenvalues.  info = {num_state, trunc_err, Entropys,
+ * entropy1, entropy2}; counts[k] = kept states of sector k; sorted/order
+ * (may be NULL) receive the normalised descending spectrum and its
+ * 1-based state indices. */
+int oracle_truncation_select(int nsec, const int* den_sub, const double* eigval, int kept_min, int kept_max,
+                             double trun_err, int* counts, double* info, double* sorted, int* order) {
+  int dim = 0;
+  for (int k = 0; k < nsec; ++k) dim += den_sub[5 * k + 4];
+  double* ev = (double*)xcalloc((size_t)dim + 1, sizeof(double));
+  int* idx = (int*)xcalloc((size_t)dim + 1, sizeof(int));
+  int* eigs = (int*)xcalloc((size_t)dim + 1, sizeof(int));
+  int* sec = (int*)xcalloc((size_t)dim + 1, sizeof(int));
+  int p = 0;
+  for (int k = 0; k < nsec; ++k)
+    for (int i = 0; i < den_sub[5 * k + 4]; ++i, ++p) {
+      ev[p] = eigval[p]; idx[p] = p + 1; eigs[p] = den_sub[5 * k + 1] + 1; sec[p] = k;
+    }
+  double tot_pm = 0.0, e1 = 0.0, e2 = 0.0;
+  for (int i = 0; i < dim; ++i) tot_pm = tot_pm + ev[i] * eigs[i];
+  for (int i = 0; i < dim; ++i)
+    if (ev[i] >= 1.0E-15) { double v = ev[i] / eigs[i]; e1 = e1 - v * log(v) * eigs[i]; }
+  for (int i = 0; i < dim; ++i) ev[i] = ev[i] / tot_pm;
+  for (int i = 0; i < dim; ++i)
+    if (ev[i] >= 1.0E-15) e2 = e2 - ev[i] * log(ev[i]) * eigs[i];
+  for (int i = 0; i < dim; ++i) ev[i] = ev[i] * tot_pm;
+  oracle_quicksort_double(ev, idx, dim);
+  tot_pm = 0.0;
+  for (int i = 0; i < dim; ++i) tot_pm = tot_pm + ev[i];
+  for (int i = 0; i < dim; ++i) ev[i] = ev[i] / tot_pm;
+  int num; double err = 0.0;
+  if (dim <= kept_min) {
+    num = dim;
+  } else {
+    for (int i = kept_min + 1; i <= dim; ++i) err = err + ev[i - 1];
+    num = kept_min;
+    while (err > trun_err && num < dim) {
+      num = num + 1;
+      err = err - ev[num - 1];
+      if (err <= trun_err || num >= kept_max) break;
+    }
+  }
+  double ent = 0.0;
+  for (int i = 0; i < dim; ++i)
+    if (ev[i] >= 1.0E-15) ent = ent - ev[i] * log(ev[i]);
+  for (int k = 0; k < nsec; ++k) counts[k] = 0;
+  for (int k = 0; k < num; ++k) counts[sec[idx[k] - 1]]++;
+  info[0] = num; info[1] = err; info[2] = ent; info[3] = e1; info[4] = e2;
+  if (sorted) memcpy(sorted, ev, sizeof(double) * (size_t)dim);
+  if (order) memcpy(order, idx, sizeof(int) * (size_t)dim);
+  free(ev); free(idx); free(eigs); free(sec);
+  return num;
+}

@@ -84,6 +84,41 @@ int oracle_lattice_tables(const oracle_lattice* lat, int ns, int ns21,
 int oracle_hmerge_tables(const oracle_lattice* lat, int sys_len, int env_len,
                          int* coup_bs, int* coup_bb, int* inb);
 
+/* ---- block update (den_mat.f90, get_sys.f90:4-762, block.f90:382-1503) ---- */
+
+/* Result operator set (flat layout as above), owned by the oracle. */
+typedef struct oracle_opset oracle_opset;
+int oracle_opset_sizes(const oracle_opset* r, int* nops, int* nsub, int64_t* ndata);
+void oracle_opset_copy(const oracle_opset* r, int* hdr, int* sub, int64_t* off, double* data);
+void oracle_opset_free(oracle_opset* r);
+
+/* Get_Hamiltonian_Sys (side 0, get_sys.f90:4-246) / Get_Hamiltonian_Env
+ * (side 1, :247-478): enlarge block model `pre` over the fused basis (length
+ * blen) and project with the truncation operator (one operator, trun_hdr[4]).
+ * Returns NULL on inconsistent tables. */
+oracle_opset* oracle_get_hamiltonian(const oracle_lattice* lat, int side, int sys_len, int env_len, int pre_nops,
+                                     const int* pre_hdr, const int* pre_sub, const int64_t* pre_off,
+                                     const double* pre_data, int blen, int nsec, const int* bsec, const int* bsub,
+                                     const int* trun_hdr, const int* trun_sub, const int64_t* trun_off,
+                                     const double* trun_data);
+
+/* update_trun_dia / update_trun_ndia (block.f90:1311-1503) of each operator. */
+oracle_opset* oracle_update_trun(int nops, const int* hdr, const int* sub, const int64_t* off, const double* data,
+                                 const int* ndia, const int* trun_hdr, const int* trun_sub, const int64_t* trun_off,
+                                 const double* trun_data);
+
+/* Get_density_matrix_sys/env_new (den_mat.f90:4-134). */
+oracle_opset* oracle_density_matrix(int side, int len, int nw, const int* wsec, int levels, const double* psi);
+
+/* QuickSort_Double (block.f90:288-332) on arr[0..n-1], idx permuted alike. */
+void oracle_quicksort_double(double* arr, int* idx, int n);
+
+/* Selection part of Get_truncation_operator (den_mat.f90:196-328) on given
+ * ascending per-sector eigenvalues; info[5] = {num_state, trunc_err,
+ * Entropys, entropy1, entropy2}. */
+int oracle_truncation_select(int nsec, const int* den_sub, const double* eigval, int kept_min, int kept_max,
+                             double trun_err, int* counts, double* info, double* sorted, int* order);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,152 @@
+"""Block update on the GPU through the C-ABI: density matrix
+(Get_density_matrix_*, den_mat.f90:4-134), truncation operator
+(Get_truncation_operator, den_mat.f90:141-383) and enlargement + projection
+(Get_Hamiltonian_Sys/Env, get_sys.f90:4-478), each against the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import model, small_superblock, random_trun, opset, op_blocks
+from grand_sql_tj_amd import capi, synth
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _cuda(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64)).to("cuda")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    capi.lib()
+
+
+def _big(ub=1, geom=(11, 11), seed=21, mu=0.3):
+    m = model(up_bias=ub, mu=mu)
+    sb = small_superblock(m, *geom, seed=seed, msys=160, menv=150, median=14, nsec=10)
+    return m, sb
+
+
+def _run(pg, ops, x):
+    y = torch.full((max(1, pg.sizes["ndata"]),), float("nan"), dtype=torch.float64, device="cuda")
+    pg.run(None if ops is None else _cuda(ops), _cuda(x), y)
+    torch.cuda.synchronize()
+    return y.cpu().numpy()[:pg.sizes["ndata"]]
+
+
+@pytest.mark.parametrize("levels", [1, 2])
+@pytest.mark.parametrize("side", [0, 1])
+def test_density_matrix_matches_oracle(side, levels):
+    m, sb = _big()
+    psi = np.concatenate([synth.random_vector(sb, 3 + l) * (1.0 + l) for l in range(levels)])
+    length = sb.sys_len + 1 if side == 0 else sb.env_len + 1
+    pg = capi.Program.density(side, length, sb.wsec, levels)
+    y = _run(pg, None, psi)
+    o = oracle.density_matrix(side, length, sb.wsec, psi, levels)
+    hdr, sub, off = pg.layout()
+    assert (hdr == o.hdr).all() and (sub == o.sub).all()
+    assert np.abs(y - o.data).max() < 1e-14
+
+
+@pytest.mark.parametrize("side", [0, 1])
+@pytest.mark.parametrize("ub", [1, 0])
+def test_block_update_matches_oracle(ub, side):
+    m, sb = _big(ub=ub, geom=(12, 10) if side else (11, 11))
+    pre, bs = (sb.sys, sb.sys_bs) if side == 0 else (sb.env, sb.env_bs)
+    tr = random_trun(bs, 17, 0.6)
+    pg = capi.Program.block_update(m, side, sb.sys_len, sb.env_len, pre, bs, tr)
+    y = _run(pg, tr.data, pre.data)
+    o = oracle.get_hamiltonian(m, side, sb.sys_len, sb.env_len, pre, bs, tr)
+    assert not np.isnan(y).any()
+    assert np.abs(y - o.data).max() / np.abs(o.data).max() < 1e-13
+
+
+def test_projection_matches_oracle():
+    m, sb = _big()
+    pre = sb.sys
+    secs = pre.sectors()
+    from types import SimpleNamespace
+    bs = SimpleNamespace(length=pre.length, nsec=secs.shape[0], sec=np.c_[secs, np.ones(len(secs), np.int32)])
+    tr = random_trun(bs, 9, 0.5)
+    ndia = np.array([0] + [1, 1, 1, 0] * pre.nact, np.int32)
+    pg = capi.Program.projection(pre, ndia, tr)
+    y = _run(pg, tr.data, pre.data)
+    o = oracle.update_trun(pre, ndia, tr)
+    assert np.abs(y - o.data).max() / np.abs(o.data).max() < 1e-13
+
+
+def _density(sb, side, seed):
+    psi = synth.random_vector(sb, seed)
+    length = sb.sys_len + 1 if side == 0 else sb.env_len + 1
+    return oracle.density_matrix(side, length, sb.wsec, psi, 1)
+
+
+def test_truncation_operator_matches_oracle():
+    m, sb = _big()
+    den = _density(sb, 0, 5)
+    kmin, kmax, terr = 20, 120, 1e-6
+    otr, ocounts, oinfo, ospec, _ = oracle.truncation_operator(den, kmin, kmax, terr)
+    d = _cuda(den.data)
+    tsub, toff, spec, info = capi.truncation_operator(den.sub, den.off, d, kmin, kmax, terr)
+    torch.cuda.synchronize()
+    evecs = d.cpu().numpy()
+    assert np.abs(spec - ospec).max() < 1e-13
+    assert info["num_state"] == oinfo["num_state"]
+    assert (tsub == np.asarray(otr.sub)).all()
+    assert abs(info["entropy"] - oinfo["entropy"]) < 1e-12
+    for s in range(tsub.shape[0]):
+        n, c = int(tsub[s, 3]), int(tsub[s, 4])
+        t = evecs[toff[s]:toff[s] + n * c].reshape(c, n).T
+        to = np.asarray(otr.data[otr.off[s]:otr.off[s] + n * c]).reshape(c, n).T
+        assert np.abs(t.T @ t - np.eye(c)).max() < 1e-12                 # orthonormal columns
+        assert np.abs(t @ t.T - to @ to.T).max() < 1e-8                   # same kept subspace
+
+
+def test_identity_truncation():
+    m, sb = _big()
+    tsub, toff, nd = capi.identity_truncation(sb.sys_bs)
+    d = torch.full((nd,), float("nan"), dtype=torch.float64, device="cuda")
+    capi.identity_truncation(sb.sys_bs, d)
+    torch.cuda.synchronize()
+    h = d.cpu().numpy()
+    for s in range(tsub.shape[0]):
+        n = int(tsub[s, 3])
+        assert (h[toff[s]:toff[s] + n * n].reshape(n, n) == np.eye(n)).all()
+
+
+@pytest.mark.parametrize("side", [0, 1])
+def test_update_step_chain_gauge_invariant(side):
+    """rho -> truncation -> enlargement + projection on the device, against
+    the oracle chain (numpy eigenvectors).  The eigenvector gauge differs, so
+    the comparison uses sector spectra of the new Hamiltonian and Frobenius
+    norms of every output sub-block."""
+    m, sb = _big(geom=(11, 11))
+    den = _density(sb, side, 8)
+    pre, bs = (sb.sys, sb.sys_bs) if side == 0 else (sb.env, sb.env_bs)
+    kmin, kmax, terr = 30, 150, 1e-7
+    # device chain
+    d = _cuda(den.data)
+    tsub, toff, _, info = capi.truncation_operator(den.sub, den.off, d, kmin, kmax, terr)
+    trun = opset(bs.length, [[bs.length, 0, 0, tsub.shape[0]]], tsub, toff, np.zeros(1))
+    pg = capi.Program.block_update(m, side, sb.sys_len, sb.env_len, pre, bs, trun)
+    y = torch.full((pg.sizes["ndata"],), float("nan"), dtype=torch.float64, device="cuda")
+    pg.run(d, _cuda(pre.data), y)
+    torch.cuda.synchronize()
+    hdr, sub, off = pg.layout()
+    dev = opset(bs.length, hdr, sub, off, y.cpu().numpy())
+    # oracle chain
+    otr, _, oinfo, _, _ = oracle.truncation_operator(den, kmin, kmax, terr)
+    ref = oracle.get_hamiltonian(m, side, sb.sys_len, sb.env_len, pre, bs, otr)
+    assert info["num_state"] == oinfo["num_state"]
+    assert (hdr == ref.hdr).all() and (sub == ref.sub).all()
+    scale = np.abs(ref.data).max()
+    for op in range(hdr.shape[0]):
+        a, b = op_blocks(dev, op), op_blocks(ref, op)
+        for k in b:
+            assert abs(np.linalg.norm(a[k]) - np.linalg.norm(b[k])) < 1e-9 * scale
+            if op == 0:
+                assert np.abs(np.linalg.eigvalsh(a[k]) - np.linalg.eigvalsh(b[k])).max() < 1e-9 * scale
