@@ -131,7 +131,9 @@ def test_update_step_chain_gauge_invariant(side):
     # device chain
     d = _cuda(den.data)
     tsub, toff, _, info = capi.truncation_operator(den.sub, den.off, d, kmin, kmax, terr)
-    trun = opset(bs.length, [[bs.length, 0, 0, tsub.shape[0]]], tsub, toff, np.zeros(1))
+    # the truncation operator's data is the eigenvector buffer d (device);
+    # the host layout only needs its extent
+    trun = opset(bs.length, [[bs.length, 0, 0, tsub.shape[0]]], tsub, toff, np.zeros(den.data.size))
     pg = capi.Program.block_update(m, side, sb.sys_len, sb.env_len, pre, bs, trun)
     y = torch.full((pg.sizes["ndata"],), float("nan"), dtype=torch.float64, device="cuda")
     pg.run(d, _cuda(pre.data), y)
