@@ -60,6 +60,57 @@ def timed_solves(plan, x0_t, steps, warmup, precision, tol, max_iter, torch, ene
     return np.array(lat), res
 
 
+def block_update_step(model, sb, psi, m_keep, torch, reps=5):
+    """The step after each solve on the device: Get_density_matrix_sys ->
+    Get_truncation_operator (kept_min = m/2, kept_max = m, trun_err 1e-8) ->
+    Get_Hamiltonian_Sys of the system block at this bond.  Times each part
+    with events on the launching stream (the current torch stream)."""
+    from grand_sql_tj_amd import capi
+    st = torch.cuda.current_stream()
+
+    def timed(fn, n):
+        out = []
+        for _ in range(n):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(st)
+            fn()
+            b.record(st)
+            torch.cuda.synchronize()
+            out.append(a.elapsed_time(b))
+        return float(np.median(out))
+
+    pgd = capi.Program.density(0, sb.sys_len + 1, sb.wsec, 1)
+    _, dsub, doff = pgd.layout()
+    rho = torch.empty(pgd.sizes["ndata"], dtype=torch.float64, device="cuda")
+    pgd.run(None, psi, rho)
+    ms_density = timed(lambda: pgd.run(None, psi, rho), reps)
+    rho0 = rho.clone()
+    res = {}
+
+    def trunc():
+        rho.copy_(rho0)
+        res["t"] = capi.truncation_operator(dsub, doff, rho, m_keep // 2, m_keep, 1e-8)
+    # the copy back is a small part of the timed region
+    ms_trunc = timed(trunc, 3)
+    tsub, toff, _, info = res["t"]
+    from grand_sql_tj_amd.blocks import OpSet
+    trun = OpSet(sb.sys_len + 1, np.array([[sb.sys_len + 1, 0, 0, tsub.shape[0]]], np.int32), tsub, toff,
+                 np.zeros(rho.numel()))
+    t0 = time.perf_counter()
+    pgu = capi.Program.block_update(model, 0, sb.sys_len, sb.env_len, sb.sys, sb.sys_bs, trun)
+    build_s = time.perf_counter() - t0
+    pre = torch.from_numpy(sb.sys.data).cuda()
+    out = torch.empty(pgu.sizes["ndata"], dtype=torch.float64, device="cuda")
+    pgu.run(rho, pre, out)
+    ms_update = timed(lambda: pgu.run(rho, pre, out), reps)
+    fl = pgu.stats()["flops_exec"]
+    return {"density_ms": ms_density, "truncation_ms": ms_trunc, "update_ms": ms_update,
+            "update_build_s": build_s, "kept": info["num_state"], "rho_dim": info["dim"],
+            "trunc_err": info["trunc_err"], "update_flops": fl,
+            "update_tflops": fl / (ms_update * 1e-3) / 1e12, "update_frac_fp64_peak": fl / (ms_update * 1e-3) / PEAK["fp64"],
+            "density_flops": pgd.stats()["flops_exec"], "new_ops": pgu.sizes["nops"], "new_elems": pgu.sizes["ndata"]}
+
+
 def cpu_baseline(model, sb, x0, n_iter, budget_s=20.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -92,6 +143,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-update", action="store_true", help="skip the density/truncation/update step")
     ap.add_argument("--warm-iter", type=int, default=200)
     ap.add_argument("--noise", type=float, default=0.01)
     ap.add_argument("--apply-only", type=int, default=0,
@@ -212,6 +264,9 @@ def main():
                         "frac": achieved / peak, "traffic": None,
                         "algorithmic_flops_per_launch": fl_dom, "avg_launch_ms": t_dom * 1e3}
     line["cpu_baseline"] = cpu
+    if not args.no_update:
+        line["block_update"] = block_update_step(model, sb, g, args.m, torch)
+        print(f"[bench] block update {line['block_update']}", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
 

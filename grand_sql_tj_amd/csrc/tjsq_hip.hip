@@ -1214,17 +1214,70 @@ __global__ void k_identity(double* __restrict__ a, int64_t n) {
     a[i] = (i % (n + 1) == 0) ? 1.0 : 0.0;
 }
 
+// Per-device eigensolver state: NSOLVE streams (default priority), each with
+// its own rocBLAS handle, so batched eigensolves of different size classes
+// overlap.  Created on first use and kept until the process ends.
+constexpr int NSOLVE = 4;
+struct SolverState {
+  rocblas_handle h[NSOLVE] = {};
+  hipStream_t s[NSOLVE] = {};
+  hipEvent_t ev[NSOLVE + 1] = {};
+};
 std::mutex g_solver_mu;
-std::vector<rocblas_handle> g_solver;   // one handle per device, created on first use
+std::vector<SolverState*> g_solver;
 
-int solver_handle(rocblas_handle* h) {
+int solver_state(SolverState** out) {
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   if ((int)g_solver.size() <= dev) g_solver.resize(dev + 1, nullptr);
-  if (!g_solver[dev] && rocblas_create_handle(&g_solver[dev]) != rocblas_status_success)
-    return fail(TJSQ_ERR_HIP, "rocblas_create_handle failed");
-  *h = g_solver[dev];
+  if (!g_solver[dev]) {
+    SolverState* S = new SolverState();
+    for (int i = 0; i < NSOLVE; ++i) {
+      HIPCHK(hipStreamCreate(&S->s[i]));
+      if (rocblas_create_handle(&S->h[i]) != rocblas_status_success)
+        return fail(TJSQ_ERR_HIP, "rocblas_create_handle failed");
+      if (rocblas_set_stream(S->h[i], S->s[i]) != rocblas_status_success)
+        return fail(TJSQ_ERR_HIP, "rocblas_set_stream failed");
+    }
+    for (int i = 0; i <= NSOLVE; ++i) HIPCHK(hipEventCreateWithFlags(&S->ev[i], hipEventDisableTiming));
+    g_solver[dev] = S;
+  }
+  *out = g_solver[dev];
   return TJSQ_OK;
+}
+
+// padded size class of an n x n sector for the batched eigensolver
+int eig_class(int n) { return n <= 32 ? 32 : 64 * ((n + 63) / 64); }
+
+// one sector inside its class buffer: rho (n x n, ld n) at src, padded
+// N x N matrix at dst
+struct PadJob { int64_t src, dst; int n, N; };
+
+// dst = [[rho, 0], [0, diag(-(2 + 2j))]]: the padding block decouples exactly
+// in the Householder tridiagonalisation (its reflectors are identities and the
+// coupling column is zero), and its eigenvalues lie below any density-matrix
+// spectrum, so the sector's eigenpairs are the last n ascending ones with
+// eigenvectors zero on the padding rows.
+__global__ void k_pad_in(const double* __restrict__ rho, double* __restrict__ pad, const PadJob* __restrict__ jobs) {
+  const PadJob J = jobs[blockIdx.y];
+  const int64_t NN = (int64_t)J.N * J.N;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < NN; e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % J.N), c = (int)(e / J.N);
+    double v = 0.0;
+    if (r < J.n && c < J.n) v = rho[J.src + r + (int64_t)c * J.n];
+    else if (r == c) v = -2.0 * (1 + (r - J.n));
+    pad[J.dst + e] = v;
+  }
+}
+
+// the sector's eigenvectors (rows 0..n-1 of the last n columns) back over rho
+__global__ void k_pad_out(double* __restrict__ rho, const double* __restrict__ pad, const PadJob* __restrict__ jobs) {
+  const PadJob J = jobs[blockIdx.y];
+  const int64_t nn = (int64_t)J.n * J.n;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % J.n), c = (int)(e / J.n);
+    rho[J.src + e] = pad[J.dst + r + (int64_t)(J.N - J.n + c) * J.N];
+  }
 }
 
 void fill_info(const tjsq::TruncSelection& sel, tjsq_trunc_info* info) {
@@ -1397,53 +1450,103 @@ int tjsq_truncation_operator(int nsec, const int* den_sub, const int64_t* den_of
       return fail(TJSQ_ERR_ARG, "tjsq_truncation_operator: density-matrix sector is not square");
     dim += den_sub[5 * k + 4];
   }
-  double* ev = nullptr;
+  // sectors grouped into padded size classes, one strided-batched DSYEV-like
+  // call (ascending eigenvalues, eigenvectors in place; den_mat.f90:192) per
+  // class, classes spread over the solver streams
+  std::vector<int> cls(nsec);
+  std::vector<std::pair<int, int>> order;   // (class, sector)
+  for (int k = 0; k < nsec; ++k) {
+    cls[k] = eig_class(std::max(1, den_sub[5 * k + 4]));
+    if (den_sub[5 * k + 4] > 0) order.push_back({cls[k], k});
+  }
+  std::stable_sort(order.begin(), order.end());
+  std::vector<PadJob> jobs;
+  struct Batch { int N, first, count; int64_t pad_off, d_off; };
+  std::vector<Batch> batches;
+  int64_t pad_elems = 0, d_elems = 0;
+  for (size_t q = 0; q < order.size(); ++q) {
+    const int N = order[q].first, k = order[q].second;
+    if (batches.empty() || batches.back().N != N) batches.push_back({N, (int)jobs.size(), 0, pad_elems, d_elems});
+    jobs.push_back({den_off[k], pad_elems, den_sub[5 * k + 4], N});
+    batches.back().count++;
+    pad_elems += (int64_t)N * N;
+    d_elems += N;
+  }
+  double* pad = nullptr;
+  double* D = nullptr;
   double* E = nullptr;
   rocblas_int* dinfo = nullptr;
+  PadJob* djobs = nullptr;
+  const int njobs = (int)jobs.size();
   auto release = [&]() {
-    (void)hipFree(ev);
+    (void)hipFree(pad);
+    (void)hipFree(D);
     (void)hipFree(E);
     (void)hipFree(dinfo);
+    (void)hipFree(djobs);
   };
-  if (hipMalloc(&ev, sizeof(double) * std::max<int64_t>(1, dim)) != hipSuccess ||
-      hipMalloc(&E, sizeof(double) * std::max<int64_t>(1, dim)) != hipSuccess ||
-      hipMalloc(&dinfo, sizeof(rocblas_int) * nsec) != hipSuccess) {
+  if (hipMalloc(&pad, sizeof(double) * std::max<int64_t>(1, pad_elems)) != hipSuccess ||
+      hipMalloc(&D, sizeof(double) * std::max<int64_t>(1, d_elems)) != hipSuccess ||
+      hipMalloc(&E, sizeof(double) * std::max<int64_t>(1, d_elems)) != hipSuccess ||
+      hipMalloc(&dinfo, sizeof(rocblas_int) * std::max(1, njobs)) != hipSuccess ||
+      hipMalloc(&djobs, sizeof(PadJob) * std::max(1, njobs)) != hipSuccess) {
     release();
     return fail(TJSQ_ERR_NOMEM, "tjsq_truncation_operator: work allocation failed");
   }
-  // DSYEV('V','U') per sector (den_mat.f90:192): ascending eigenvalues,
-  // eigenvectors overwrite the sector in place
-  {
+  std::vector<double> hD(d_elems);
+  std::vector<rocblas_int> hinfo(std::max(1, njobs), 0);
+  if (njobs > 0) {
     std::lock_guard<std::mutex> lk(g_solver_mu);
-    rocblas_handle h = nullptr;
-    int rc = solver_handle(&h);
+    SolverState* S = nullptr;
+    int rc = solver_state(&S);
     if (rc) { release(); return rc; }
-    if (rocblas_set_stream(h, st) != rocblas_status_success) {
-      release();
-      return fail(TJSQ_ERR_HIP, "rocblas_set_stream failed");
-    }
-    int64_t p = 0;
-    for (int k = 0; k < nsec; ++k) {
-      const int n = den_sub[5 * k + 4];
-      if (n > 0 && rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, den + den_off[k], n, ev + p,
-                                    E + p, dinfo + k) != rocblas_status_success) {
-        release();
-        return fail(TJSQ_ERR_HIP, "rocsolver_dsyevd failed");
+    bool ok = hipMemcpy(djobs, jobs.data(), sizeof(PadJob) * njobs, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipEventRecord(S->ev[NSOLVE], st) == hipSuccess;
+    for (int i = 0; ok && i < NSOLVE; ++i) ok = hipStreamWaitEvent(S->s[i], S->ev[NSOLVE], 0) == hipSuccess;
+    for (size_t b = 0; ok && b < batches.size(); ++b) {
+      const Batch& B = batches[b];
+      hipStream_t sb = S->s[b % NSOLVE];
+      dim3 grid((unsigned)std::min<int64_t>(((int64_t)B.N * B.N + 255) / 256, 64), (unsigned)B.count);
+      hipLaunchKernelGGL(k_pad_in, grid, dim3(256), 0, sb, den, pad, djobs + B.first);
+      ok = hipGetLastError() == hipSuccess;
+      ok = ok && rocsolver_dsyevd_strided_batched(S->h[b % NSOLVE], rocblas_evect_original, rocblas_fill_upper, B.N,
+                                                  pad + B.pad_off, B.N, (rocblas_stride)B.N * B.N, D + B.d_off,
+                                                  B.N, E + B.d_off, B.N, dinfo + B.first,
+                                                  B.count) == rocblas_status_success;
+      if (ok) {
+        hipLaunchKernelGGL(k_pad_out, dim3(grid.x, B.count), dim3(256), 0, sb, den, pad, djobs + B.first);
+        ok = hipGetLastError() == hipSuccess;
       }
-      p += n;
     }
-  }
-  std::vector<double> hev(dim);
-  std::vector<rocblas_int> hinfo(nsec, 0);
-  if (hipMemcpyAsync(hev.data(), ev, sizeof(double) * dim, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipMemcpyAsync(hinfo.data(), dinfo, sizeof(rocblas_int) * nsec, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipStreamSynchronize(st) != hipSuccess) {
-    release();
-    return fail(TJSQ_ERR_HIP, "tjsq_truncation_operator: eigenvalue copy failed");
+    for (int i = 0; ok && i < NSOLVE; ++i)
+      ok = hipEventRecord(S->ev[i], S->s[i]) == hipSuccess && hipStreamWaitEvent(st, S->ev[i], 0) == hipSuccess;
+    ok = ok && hipMemcpyAsync(hD.data(), D, sizeof(double) * d_elems, hipMemcpyDeviceToHost, st) == hipSuccess;
+    ok = ok && hipMemcpyAsync(hinfo.data(), dinfo, sizeof(rocblas_int) * njobs, hipMemcpyDeviceToHost, st) == hipSuccess;
+    ok = ok && hipStreamSynchronize(st) == hipSuccess;
+    if (!ok) {
+      release();
+      return fail(TJSQ_ERR_HIP, "tjsq_truncation_operator: batched eigensolve failed");
+    }
   }
   release();
-  for (int k = 0; k < nsec; ++k)
-    if (hinfo[k] != 0) return fail(TJSQ_ERR_HIP, "tjsq_truncation_operator: eigensolver did not converge");
+  for (int j = 0; j < njobs; ++j)
+    if (hinfo[j] != 0) return fail(TJSQ_ERR_HIP, "tjsq_truncation_operator: eigensolver did not converge");
+  // eigenvalues per sector in density-matrix order; the padding eigenvalues
+  // must come out as placed and below the sector's spectrum
+  std::vector<int64_t> dpos(nsec, -1);
+  for (size_t b = 0, j = 0; b < batches.size(); ++b)
+    for (int q = 0; q < batches[b].count; ++q, ++j)
+      dpos[order[j].second] = batches[b].d_off + (int64_t)q * batches[b].N;
+  std::vector<double> hev(dim);
+  for (int k = 0, p = 0; k < nsec; ++k) {
+    const int n = den_sub[5 * k + 4], N = cls[k];
+    if (n == 0) continue;
+    const double* d = hD.data() + dpos[k];
+    for (int j = 0; j < N - n; ++j)
+      if (std::fabs(d[j] + 2.0 * (N - n - j)) > 1e-9)
+        return fail(TJSQ_ERR_ARG, "tjsq_truncation_operator: density-matrix sector is not positive semi-definite");
+    for (int i = 0; i < n; ++i) hev[p++] = d[N - n + i];
+  }
   tjsq::TruncSelection sel;
   std::string msg;
   int rc = tjsq::select_truncation(nsec, den_sub, hev.data(), kept_min, kept_max, trun_err, sel, msg);
