@@ -82,7 +82,10 @@ EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_t
            "tjsq_block_update_create", "tjsq_trun_project_create", "tjsq_density_create", "tjsq_program_sizes",
            "tjsq_program_layout", "tjsq_program_tables", "tjsq_program_stats", "tjsq_program_run",
            "tjsq_program_profile", "tjsq_program_profile_read", "tjsq_program_destroy", "tjsq_truncation_operator",
-           "tjsq_truncation_select", "tjsq_identity_truncation", "tjsq_last_error"]
+           "tjsq_truncation_select", "tjsq_identity_truncation", "tjsq_batch_create", "tjsq_batch_offsets",
+           "tjsq_batch_stats", "tjsq_batch_graph", "tjsq_batch_apply", "tjsq_batch_apply_f32", "tjsq_batch_profile",
+           "tjsq_batch_profile_read", "tjsq_batch_destroy", "tjsq_plan_create_ex", "tjsq_hostplan_create_ex",
+           "tjsq_plan_reshape", "tjsq_last_error"]
 
 _lib = None
 
@@ -132,6 +135,18 @@ def lib():
             "tjsq_truncation_operator": [I, P, P, P, I, I, C.c_double, P, P, P, P, P, P],
             "tjsq_truncation_select": [I, P, P, I, I, C.c_double, P, P, P, P],
             "tjsq_identity_truncation": [P, P, P, P, P, P],
+            "tjsq_batch_create": [P, I, P, P],
+            "tjsq_batch_offsets": [P, P],
+            "tjsq_batch_stats": [P, P],
+            "tjsq_batch_graph": [P, I],
+            "tjsq_batch_apply": [P, P, P, P],
+            "tjsq_batch_apply_f32": [P, I, P, P, P],
+            "tjsq_batch_profile": [P, I],
+            "tjsq_batch_profile_read": [P, P, P],
+            "tjsq_batch_destroy": [P],
+            "tjsq_plan_create_ex": [P, I, I, P, P, P, P, I, P, I, P, P, P],
+            "tjsq_hostplan_create_ex": [P, I, I, P, P, P, P, I, P, P, P],
+            "tjsq_plan_reshape": [P, P, P],
         }
         for name in EXPORTS:
             f = getattr(L, name)
@@ -243,19 +258,40 @@ def active_sites(model, length) -> int:
     return r
 
 
+class Shape(C.Structure):
+    """Work-item shape of the grouped GEMM (tjsq_shape)."""
+    _fields_ = [("big", C.c_int), ("small", C.c_int), ("k_split", C.c_int), ("item_chunks", C.c_int),
+                ("xcds", C.c_int)]
+    DEFAULT = {"big": 64, "small": 32, "k_split": 256, "item_chunks": 32, "xcds": 8}
+
+    @classmethod
+    def of(cls, d):
+        if d is None:
+            return None
+        v = dict(cls.DEFAULT)
+        v.update(d)
+        return cls(**v)
+
+
+def _shape_ptr(shape):
+    sh = Shape.of(shape)
+    return (C.byref(sh) if sh is not None else None), sh
+
+
 class HostPlan:
     """Host-only compile of a superblock's H_eff into task tables."""
 
-    def __init__(self, model, sb):
+    def __init__(self, model, sb, shape=None):
         keep = _Keep()
         lat = lattice_struct(model)
         s, e = opset_struct(sb.sys, keep), opset_struct(sb.env, keep)
         sbs, ebs = basis_struct(sb.sys_bs, keep), basis_struct(sb.env_bs, keep)
         w = keep.ptr(sb.wsec, np.int32)
         h = C.c_void_p()
-        check(lib().tjsq_hostplan_create(C.byref(lat), sb.sys_len, sb.env_len, C.byref(s), C.byref(e), C.byref(sbs),
-                                         C.byref(ebs), int(sb.wsec.shape[0]), C.c_void_p(w), C.byref(h)),
-              "tjsq_hostplan_create")
+        shp, _sh = _shape_ptr(shape)
+        check(lib().tjsq_hostplan_create_ex(C.byref(lat), sb.sys_len, sb.env_len, C.byref(s), C.byref(e),
+                                            C.byref(sbs), C.byref(ebs), int(sb.wsec.shape[0]), C.c_void_p(w), shp,
+                                            C.byref(h)), "tjsq_hostplan_create_ex")
         self.h = h
         sizes = np.zeros(15, np.int64)
         check(lib().tjsq_hostplan_sizes(h, sizes.ctypes.data), "tjsq_hostplan_sizes")
@@ -284,7 +320,7 @@ class Plan:
     """Device plan of one superblock (tjsq_plan_create).  Vectors are torch
     CUDA tensors; the stream is torch's current stream."""
 
-    def __init__(self, model, sb, precisions=("fp64", "fp32", "bf16x3"), device_ops=None):
+    def __init__(self, model, sb, precisions=("fp64", "fp32", "bf16x3"), device_ops=None, shape=None):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("tjsq Plan needs a GPU (HIP device)")
@@ -301,9 +337,11 @@ class Plan:
             mask |= 1 << PRECISIONS[p]
         h = C.c_void_p()
         stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-        check(lib().tjsq_plan_create(C.byref(lat), sb.sys_len, sb.env_len, C.byref(s), C.byref(e), C.byref(sbs),
-                                     C.byref(ebs), int(sb.wsec.shape[0]), C.c_void_p(w), mask, stream, C.byref(h)),
-              "tjsq_plan_create")
+        shp, _sh = _shape_ptr(shape)
+        check(lib().tjsq_plan_create_ex(C.byref(lat), sb.sys_len, sb.env_len, C.byref(s), C.byref(e), C.byref(sbs),
+                                        C.byref(ebs), int(sb.wsec.shape[0]), C.c_void_p(w), mask, shp, stream,
+                                        C.byref(h)), "tjsq_plan_create_ex")
+        self.shape = shape
         self.h = h
         self.dim = sb.dim
         st = PlanStats()
@@ -312,6 +350,15 @@ class Plan:
 
     def _stream(self):
         return C.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+
+    def reshape(self, shape):
+        """Recompile the task tables for another work-item shape."""
+        shp, _sh = _shape_ptr(shape)
+        check(lib().tjsq_plan_reshape(self.h, shp, self._stream()), "tjsq_plan_reshape")
+        st = PlanStats()
+        check(lib().tjsq_plan_stats_get(self.h, C.byref(st)), "tjsq_plan_stats_get")
+        self.stats = st.as_dict()
+        self.shape = shape
 
     def apply(self, x, y=None, precision="fp64"):
         """y = H_eff x (model_wave) for a CUDA tensor x of length D."""
@@ -518,3 +565,59 @@ def identity_truncation(basis, data=None, stream=None):
     check(lib().tjsq_identity_truncation(C.byref(b), tsub.ctypes.data, toff.ctypes.data, C.byref(nd), ptr, s),
           "tjsq_identity_truncation")
     return tsub, toff, nd.value
+
+
+class Batch:
+    """Applies of several plans in one grouped launch per stage (tjsq_batch).
+
+    Vectors are the members' vectors back to back at ``offsets``."""
+
+    def __init__(self, plans, graph=False):
+        import torch
+        hs = (C.c_void_p * len(plans))(*[p.h.value for p in plans])
+        self._h = C.c_void_p()
+        s = torch.cuda.current_stream().cuda_stream
+        check(lib().tjsq_batch_create(hs, len(plans), C.c_void_p(s), C.byref(self._h)), "tjsq_batch_create")
+        off = np.zeros(len(plans) + 1, np.int64)
+        check(lib().tjsq_batch_offsets(self._h, off.ctypes.data), "tjsq_batch_offsets")
+        self.offsets = off
+        self.dim = int(off[-1])
+        self.graph(graph)
+
+    def graph(self, enable=True):
+        check(lib().tjsq_batch_graph(self._h, int(enable)), "tjsq_batch_graph")
+
+    @property
+    def stats(self):
+        st = PlanStats()
+        check(lib().tjsq_batch_stats(self._h, C.byref(st)), "tjsq_batch_stats")
+        return st.as_dict()
+
+    def apply(self, x, y=None, precision="fp64", stream=None):
+        import torch
+        if y is None:
+            y = torch.empty_like(x)
+        assert x.numel() == self.dim and y.numel() == self.dim and x.is_cuda and y.is_cuda
+        s = C.c_void_p(stream if stream is not None else torch.cuda.current_stream().cuda_stream)
+        prec = PRECISIONS[precision]
+        if prec == FP64:
+            assert x.dtype == torch.float64 and y.dtype == torch.float64
+            check(lib().tjsq_batch_apply(self._h, x.data_ptr(), y.data_ptr(), s), "tjsq_batch_apply")
+        else:
+            assert x.dtype == torch.float32 and y.dtype == torch.float32
+            check(lib().tjsq_batch_apply_f32(self._h, prec, x.data_ptr(), y.data_ptr(), s), "tjsq_batch_apply_f32")
+        return y
+
+    def profile(self, enable=True):
+        check(lib().tjsq_batch_profile(self._h, int(enable)), "tjsq_batch_profile")
+
+    def profile_read(self):
+        ms = np.zeros(2, np.float64)
+        n = np.zeros(2, np.int64)
+        check(lib().tjsq_batch_profile_read(self._h, ms.ctypes.data, n.ctypes.data), "tjsq_batch_profile_read")
+        return ms, n
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().tjsq_batch_destroy(self._h)
+            self._h = None

@@ -458,6 +458,22 @@ int upload(tjsq_plan* p, const tjsq_opset& set, int64_t base, hipStream_t st) {
   return TJSQ_OK;
 }
 
+// tile shape of the C-ABI -> compiler shape; the kernel has 32x32 and 64x64
+// work items and a K chunk of 32
+bool to_shape(const tjsq_shape* in, tjsq::Shape& out) {
+  out = tjsq::Shape();
+  if (!in) return true;
+  if ((in->big != 32 && in->big != 64) || in->small != 32 || in->k_split < 32 || in->item_chunks < 1 ||
+      in->xcds < 1 || in->xcds > 64)
+    return false;
+  out.big = in->big;
+  out.small = in->small;
+  out.k_split = in->k_split;
+  out.item_chunks = in->item_chunks;
+  out.xcds = in->xcds;
+  return true;
+}
+
 // device copies of the task tables, FP64 stage-1 scratch and partial tiles
 // (plus the FP32 ones when with32)
 int upload_tables(tjsq_plan* p, bool with32, hipStream_t st) {
@@ -665,14 +681,23 @@ int tjsq_active_sites(const tjsq_lattice* lat, int len) {
 int tjsq_plan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys, const tjsq_opset* env,
                      const tjsq_basis* sys_bs, const tjsq_basis* env_bs, int nw, const int* wsec, int precision_mask,
                      void* stream, tjsq_plan** plan) {
+  return tjsq_plan_create_ex(lat, sys_len, env_len, sys, env, sys_bs, env_bs, nw, wsec, precision_mask, nullptr, stream,
+                             plan);
+}
+
+int tjsq_plan_create_ex(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys,
+                        const tjsq_opset* env, const tjsq_basis* sys_bs, const tjsq_basis* env_bs, int nw,
+                        const int* wsec, int precision_mask, const tjsq_shape* shape, void* stream, tjsq_plan** plan) {
   if (!lat || !sys || !env || !sys_bs || !env_bs || !wsec || !plan || nw < 1)
     return fail(TJSQ_ERR_ARG, "tjsq_plan_create: null argument");
   *plan = nullptr;
+  tjsq::Shape sh;
+  if (!to_shape(shape, sh)) return fail(TJSQ_ERR_ARG, "tjsq_plan_create: unsupported tile shape");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(TJSQ_ERR_NODEV, "no GPU visible");
   tjsq_plan* p = new tjsq_plan();
   std::string msg;
-  int rc = tjsq::build_host_plan(*lat, sys_len, env_len, *sys, *env, *sys_bs, *env_bs, nw, wsec, tjsq::Shape(), p->hp, msg);
+  int rc = tjsq::build_host_plan(*lat, sys_len, env_len, *sys, *env, *sys_bs, *env_bs, nw, wsec, sh, p->hp, msg);
   if (rc != TJSQ_OK) {
     delete p;
     return fail(rc, "tjsq_plan_create: " + msg);
@@ -748,12 +773,20 @@ struct tjsq_hostplan {
 int tjsq_hostplan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys,
                          const tjsq_opset* env, const tjsq_basis* sys_bs, const tjsq_basis* env_bs, int nw,
                          const int* wsec, tjsq_hostplan** out) {
+  return tjsq_hostplan_create_ex(lat, sys_len, env_len, sys, env, sys_bs, env_bs, nw, wsec, nullptr, out);
+}
+
+int tjsq_hostplan_create_ex(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys,
+                            const tjsq_opset* env, const tjsq_basis* sys_bs, const tjsq_basis* env_bs, int nw,
+                            const int* wsec, const tjsq_shape* shape, tjsq_hostplan** out) {
   if (!lat || !sys || !env || !sys_bs || !env_bs || !wsec || !out || nw < 1)
     return fail(TJSQ_ERR_ARG, "tjsq_hostplan_create: null argument");
   *out = nullptr;
+  tjsq::Shape sh;
+  if (!to_shape(shape, sh)) return fail(TJSQ_ERR_ARG, "tjsq_hostplan_create: unsupported tile shape");
   tjsq_hostplan* h = new tjsq_hostplan();
   std::string msg;
-  const int rc = tjsq::build_host_plan(*lat, sys_len, env_len, *sys, *env, *sys_bs, *env_bs, nw, wsec, tjsq::Shape(), h->hp, msg);
+  const int rc = tjsq::build_host_plan(*lat, sys_len, env_len, *sys, *env, *sys_bs, *env_bs, nw, wsec, sh, h->hp, msg);
   if (rc != TJSQ_OK) {
     delete h;
     return fail(rc, "tjsq_hostplan_create: " + msg);
@@ -793,6 +826,27 @@ int tjsq_hostplan_stats(const tjsq_hostplan* h, tjsq_plan_stats* stats) {
 
 int tjsq_hostplan_destroy(tjsq_hostplan* h) {
   delete h;
+  return TJSQ_OK;
+}
+
+int tjsq_plan_reshape(tjsq_plan* p, const tjsq_shape* shape, void* stream) {
+  if (!p) return fail(TJSQ_ERR_ARG, "tjsq_plan_reshape: null plan");
+  tjsq::Shape sh;
+  if (!to_shape(shape, sh)) return fail(TJSQ_ERR_ARG, "tjsq_plan_reshape: unsupported tile shape");
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(hipDeviceSynchronize());   // no launch may still read the old tables
+  // the contribution list does not depend on the shape: recompile the tables
+  const tjsq_plan_stats keep = p->hp.stats;
+  int rc = tjsq::compile_program(p->hp, sh);
+  if (rc) return fail(rc, "tjsq_plan_reshape: compilation failed");
+  p->hp.stats.dim = keep.dim;
+  p->hp.stats.bytes_fp64 = keep.bytes_fp64;
+  void* bufs[] = {p->tile1, p->tile2, p->seg1, p->seg2, p->rtile, p->t64, p->t32, p->part64, p->part32};
+  for (void* b : bufs) (void)hipFree(b);
+  p->tile1 = p->tile2 = nullptr; p->seg1 = p->seg2 = nullptr; p->rtile = nullptr;
+  p->t64 = nullptr; p->t32 = nullptr; p->part64 = nullptr; p->part32 = nullptr;
+  if ((rc = upload_tables(p, p->arena32 != nullptr, st))) return rc;
+  HIPCHK(hipStreamSynchronize(st));
   return TJSQ_OK;
 }
 
@@ -1586,6 +1640,223 @@ int tjsq_identity_truncation(const tjsq_basis* basis, int* trun_sub, int64_t* tr
     off += (int64_t)n * n;
   }
   *ndata = off;
+  return TJSQ_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches: many applies of uneven sizes in one grouped launch per stage
+// ---------------------------------------------------------------------------
+// The members' task tables are merged with their offsets rebased into
+// concatenated spaces (operators, x / y, stage-1 scratch, partial tiles), so
+// the tile kernel runs the whole batch unchanged.  Each member's work-item
+// lists are padded to a multiple of 8 by the compiler, so every item keeps its
+// XCD (item g runs on XCD g % 8) after concatenation.
+struct tjsq_batch {
+  tjsq_plan* dev = nullptr;                  // merged tables, concatenated arenas
+  std::vector<int64_t> x_off;                // member offsets in x / y (n + 1 entries)
+  hipStream_t cap = nullptr;                 // capture stream (default priority)
+  struct Graph { hipGraphExec_t exec = nullptr; hipGraph_t graph = nullptr; const void* x = nullptr; void* y = nullptr; };
+  Graph g[3];                                // per precision
+  bool use_graph = false;
+};
+
+namespace {
+
+void graph_free(tjsq_batch::Graph& G) {
+  if (G.exec) (void)hipGraphExecDestroy(G.exec);
+  if (G.graph) (void)hipGraphDestroy(G.graph);
+  G = tjsq_batch::Graph();
+}
+
+template <typename T>
+int batch_launch(tjsq_batch* B, int precision, const T* x, T* y, hipStream_t st) {
+  const T* ops;
+  T* t;
+  if constexpr (sizeof(T) == 8) { ops = B->dev->arena; t = B->dev->t64; }
+  else { ops = B->dev->arena32; t = B->dev->t32; }
+  int rc = launch_tiles<T>(B->dev, 1, ops, x, t, y, precision, st);
+  if (rc == TJSQ_OK) rc = launch_tiles<T>(B->dev, 2, ops, x, t, y, precision, st);
+  return rc;
+}
+
+// runs the batch, through a captured hipGraph when enabled (recaptured when
+// x / y change: the graph holds the pointers)
+template <typename T>
+int batch_run(tjsq_batch* B, int precision, const T* x, T* y, hipStream_t st) {
+  if (!B->use_graph || B->dev->profiling) return batch_launch<T>(B, precision, x, y, st);
+  tjsq_batch::Graph& G = B->g[precision];
+  if (!G.exec || G.x != (const void*)x || G.y != (void*)y) {
+    graph_free(G);
+    if (!B->cap) HIPCHK(hipStreamCreate(&B->cap));
+    HIPCHK(hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal));
+    int rc = batch_launch<T>(B, precision, x, y, B->cap);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(B->cap, &graph);
+    if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    if (e != hipSuccess) return fail(TJSQ_ERR_HIP, std::string("graph capture failed: ") + hipGetErrorString(e));
+    G.graph = graph;
+    HIPCHK(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+    G.x = x;
+    G.y = y;
+  }
+  HIPCHK(hipGraphLaunch(G.exec, st));
+  return TJSQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tjsq_batch_create(tjsq_plan* const* plans, int n, void* stream, tjsq_batch** out) {
+  if (!plans || n < 1 || !out) return fail(TJSQ_ERR_ARG, "tjsq_batch_create: bad argument");
+  *out = nullptr;
+  bool with32 = true;
+  for (int i = 0; i < n; ++i) {
+    if (!plans[i]) return fail(TJSQ_ERR_ARG, "tjsq_batch_create: null plan");
+    with32 = with32 && plans[i]->arena32 != nullptr;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  tjsq_batch* B = new tjsq_batch();
+  B->dev = new tjsq_plan();
+  (void)hipGetDevice(&B->dev->device);
+  HostPlan& M = B->dev->hp;
+  std::vector<int64_t> abase(n), tbase(n), sbase(n);
+  std::vector<int> s1base(n), s2base(n);
+  B->x_off.assign(n + 1, 0);
+  int64_t arena = 0, scratch = 0, slots = 0;
+  int seg1 = 0, seg2 = 0;
+  for (int i = 0; i < n; ++i) {
+    const HostPlan& h = plans[i]->hp;
+    abase[i] = arena; tbase[i] = scratch; sbase[i] = slots; s1base[i] = seg1; s2base[i] = seg2;
+    B->x_off[i + 1] = B->x_off[i] + h.dim;
+    arena += h.arena_elems; scratch += h.scratch_elems; slots += h.nslots;
+    seg1 += (int)h.seg1.size(); seg2 += (int)h.seg2.size();
+  }
+  auto rebase_seg = [&](Seg s, int i) {
+    auto base = [&](uint8_t sp) -> int64_t {
+      return sp == tjsq::SP_OPS ? abase[i] : sp == tjsq::SP_X ? B->x_off[i] : sp == tjsq::SP_T ? tbase[i] : 0;
+    };
+    s.l_off += base(s.l_space);
+    s.r_off += base(s.r_space);
+    return s;
+  };
+  auto rebase_tile = [&](Tile t, int i, int segb) {
+    if (t.mt == 0 && t.nt == 0 && t.seg_begin == t.seg_end) return t;   // XCD padding item
+    t.out_off += t.out_space == tjsq::SP_T ? tbase[i] : B->x_off[i];
+    t.seg_begin += segb;
+    t.seg_end += segb;
+    if (t.slot >= 0) t.slot += (int32_t)sbase[i];
+    return t;
+  };
+  for (int i = 0; i < n; ++i) {
+    const HostPlan& h = plans[i]->hp;
+    for (const Seg& s : h.seg1) M.seg1.push_back(rebase_seg(s, i));
+    for (const Seg& s : h.seg2) M.seg2.push_back(rebase_seg(s, i));
+  }
+  for (int part = 0; part < 2; ++part)      // 64x64 items of every member first, then 32x32
+    for (int i = 0; i < n; ++i) {
+      const HostPlan& h = plans[i]->hp;
+      const size_t lo1 = part ? h.n64_1 : 0, hi1 = part ? h.tile1.size() : h.n64_1;
+      const size_t lo2 = part ? h.n64_2 : 0, hi2 = part ? h.tile2.size() : h.n64_2;
+      for (size_t q = lo1; q < hi1; ++q) M.tile1.push_back(rebase_tile(h.tile1[q], i, s1base[i]));
+      for (size_t q = lo2; q < hi2; ++q) M.tile2.push_back(rebase_tile(h.tile2[q], i, s2base[i]));
+      if (!part) { M.n64_1 += (int)(hi1 - lo1); M.n64_2 += (int)(hi2 - lo2); }
+    }
+  for (int i = 0; i < n; ++i)
+    for (Tile r : plans[i]->hp.rtile) {
+      r.out_off += B->x_off[i];
+      r.seg_begin += (int32_t)sbase[i];
+      r.seg_end += (int32_t)sbase[i];
+      M.rtile.push_back(r);
+    }
+  M.arena_elems = arena;
+  M.scratch_elems = scratch;
+  M.nslots = slots;
+  M.dim = B->x_off[n];
+  for (int i = 0; i < n; ++i) {
+    const tjsq_plan_stats& s = plans[i]->hp.stats;
+    M.stats.dim += s.dim; M.stats.n_contrib += s.n_contrib; M.stats.n_chain += s.n_chain;
+    M.stats.n_stage1 += s.n_stage1; M.stats.n_out_blocks += s.n_out_blocks;
+    M.stats.flops_ref += s.flops_ref; M.stats.flops_exec += s.flops_exec; M.stats.bytes_fp64 += s.bytes_fp64;
+    M.stats.op_elems += s.op_elems; M.stats.scratch_elems += s.scratch_elems; M.stats.n_slots += s.n_slots;
+  }
+  M.stats.n_tiles1 = (int64_t)M.tile1.size();
+  M.stats.n_tiles2 = (int64_t)M.tile2.size();
+  auto cleanup = [&](int code, const std::string& m) {
+    tjsq_batch_destroy(B);
+    return fail(code, m);
+  };
+  if (hipMalloc(&B->dev->arena, sizeof(double) * std::max<int64_t>(1, arena)) != hipSuccess)
+    return cleanup(TJSQ_ERR_NOMEM, "batch arena allocation failed");
+  if (with32 && hipMalloc(&B->dev->arena32, sizeof(float) * std::max<int64_t>(1, arena)) != hipSuccess)
+    return cleanup(TJSQ_ERR_NOMEM, "batch FP32 arena allocation failed");
+  for (int i = 0; i < n; ++i) {
+    const int64_t e = plans[i]->hp.arena_elems;
+    if (e == 0) continue;
+    if (hipMemcpyAsync(B->dev->arena + abase[i], plans[i]->arena, sizeof(double) * e, hipMemcpyDeviceToDevice, st) !=
+            hipSuccess ||
+        (with32 && hipMemcpyAsync(B->dev->arena32 + abase[i], plans[i]->arena32, sizeof(float) * e,
+                                  hipMemcpyDeviceToDevice, st) != hipSuccess))
+      return cleanup(TJSQ_ERR_HIP, "batch arena copy failed");
+  }
+  int rc = upload_tables(B->dev, with32, st);
+  if (rc) {
+    tjsq_batch_destroy(B);
+    return rc;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  *out = B;
+  return TJSQ_OK;
+}
+
+int tjsq_batch_offsets(const tjsq_batch* B, int64_t* off) {
+  if (!B || !off) return fail(TJSQ_ERR_ARG, "tjsq_batch_offsets: null argument");
+  std::memcpy(off, B->x_off.data(), sizeof(int64_t) * B->x_off.size());
+  return TJSQ_OK;
+}
+
+int tjsq_batch_stats(const tjsq_batch* B, tjsq_plan_stats* stats) {
+  if (!B || !stats) return fail(TJSQ_ERR_ARG, "tjsq_batch_stats: null argument");
+  *stats = B->dev->hp.stats;
+  return TJSQ_OK;
+}
+
+int tjsq_batch_graph(tjsq_batch* B, int enable) {
+  if (!B) return fail(TJSQ_ERR_ARG, "tjsq_batch_graph: null batch");
+  B->use_graph = enable != 0;
+  return TJSQ_OK;
+}
+
+int tjsq_batch_apply(tjsq_batch* B, const double* x, double* y, void* stream) {
+  if (!B || !x || !y) return fail(TJSQ_ERR_ARG, "tjsq_batch_apply: null argument");
+  return batch_run<double>(B, TJSQ_FP64, x, y, (hipStream_t)stream);
+}
+
+int tjsq_batch_apply_f32(tjsq_batch* B, int precision, const float* x, float* y, void* stream) {
+  if (!B || !x || !y || (precision != TJSQ_FP32 && precision != TJSQ_BF16X3))
+    return fail(TJSQ_ERR_ARG, "tjsq_batch_apply_f32: bad argument");
+  if (!B->dev->arena32) return fail(TJSQ_ERR_ARG, "tjsq_batch_apply_f32: members were created without FP32 operands");
+  return batch_run<float>(B, precision, x, y, (hipStream_t)stream);
+}
+
+int tjsq_batch_profile(tjsq_batch* B, int enable) {
+  if (!B) return fail(TJSQ_ERR_ARG, "tjsq_batch_profile: null batch");
+  return tjsq_plan_profile(B->dev, enable);
+}
+
+int tjsq_batch_profile_read(tjsq_batch* B, double* ms, int64_t* launches) {
+  if (!B) return fail(TJSQ_ERR_ARG, "tjsq_batch_profile_read: null batch");
+  return tjsq_plan_profile_read(B->dev, ms, launches);
+}
+
+int tjsq_batch_destroy(tjsq_batch* B) {
+  if (!B) return TJSQ_OK;
+  for (auto& G : B->g) graph_free(G);
+  if (B->cap) (void)hipStreamDestroy(B->cap);
+  tjsq_plan_destroy(B->dev);
+  delete B;
   return TJSQ_OK;
 }
 

@@ -1190,7 +1190,7 @@ int compile_program(HostPlan& P, const Shape& shape) {
         const int sidx = (int)P.seg1.size();
         P.seg1.push_back(s);
         const int tile = edge(c.M, c.K2);
-        auto& dst = tile == shape.big ? t1big : t1small;
+        auto& dst = tile == 64 ? t1big : t1small;   // the kernel's 64x64 items lead the table
         auto ga = agroup.emplace(c.a_off, (int)agroup.size()).first->second;
         for (int m0 = 0; m0 < c.M; m0 += tile)
           for (int n0 = 0; n0 < c.K2; n0 += tile)
@@ -1292,7 +1292,7 @@ int compile_program(HostPlan& P, const Shape& shape) {
     items.push_back({ib, last});  // also the empty item of a block without terms
     const int tile = edge(ob.M, ob.N);
     const int units = (tile / 32) * (tile / 32);
-    auto& dst = tile == shape.big ? t2big : t2small;
+    auto& dst = tile == 64 ? t2big : t2small;
     for (int m0 = 0; m0 < ob.M; m0 += tile)
       for (int n0 = 0; n0 < ob.N; n0 += tile) {
         const int mt = std::min(tile, ob.M - m0), nt = std::min(tile, ob.N - n0);

@@ -182,6 +182,29 @@ int tjsq_plan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tj
                      const tjsq_opset* env, const tjsq_basis* sys_bs, const tjsq_basis* env_bs,
                      int nw, const int* wsec, int precision_mask, void* stream, tjsq_plan** plan);
 
+/* Work-item shape of the grouped GEMM (what an autotuner varies).  Blocks
+ * with a side above `small` use big x big work items (big = 64 or 32;
+ * small = 32); stage-2 segments are cut into K pieces of at most k_split and
+ * grouped into work items of at most item_chunks K chunks of 32; work items
+ * are binned over `xcds` XCDs (1 = plain order).  NULL selects {64, 32, 256,
+ * 32, 8}. */
+typedef struct tjsq_shape {
+  int big;
+  int small;
+  int k_split;
+  int item_chunks;
+  int xcds;
+} tjsq_shape;
+
+int tjsq_plan_create_ex(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys,
+                        const tjsq_opset* env, const tjsq_basis* sys_bs, const tjsq_basis* env_bs, int nw,
+                        const int* wsec, int precision_mask, const tjsq_shape* shape, void* stream,
+                        tjsq_plan** plan);
+
+/* Recompile a plan's task tables for another work-item shape, keeping its
+ * operators on the device (no launch of the plan may be in flight). */
+int tjsq_plan_reshape(tjsq_plan* plan, const tjsq_shape* shape, void* stream);
+
 int tjsq_plan_stats_get(const tjsq_plan* plan, tjsq_plan_stats* stats);
 int tjsq_plan_destroy(tjsq_plan* plan);
 
@@ -224,6 +247,9 @@ typedef struct tjsq_hostplan tjsq_hostplan;
 int tjsq_hostplan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys,
                          const tjsq_opset* env, const tjsq_basis* sys_bs, const tjsq_basis* env_bs, int nw,
                          const int* wsec, tjsq_hostplan** out);
+int tjsq_hostplan_create_ex(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys,
+                            const tjsq_opset* env, const tjsq_basis* sys_bs, const tjsq_basis* env_bs, int nw,
+                            const int* wsec, const tjsq_shape* shape, tjsq_hostplan** out);
 int tjsq_hostplan_sizes(const tjsq_hostplan* h, int64_t* sizes);
 int tjsq_hostplan_tables(const tjsq_hostplan* h, void* seg1, void* tile1, void* seg2, void* tile2, void* merge,
                          void* rtile);
@@ -320,6 +346,26 @@ int tjsq_truncation_select(int nsec, const int* den_sub, const double* eigval, i
  * data (device, may be NULL) receives the unit matrices. */
 int tjsq_identity_truncation(const tjsq_basis* basis, int* trun_sub, int64_t* trun_off, int64_t* ndata, double* data,
                              void* stream);
+
+/* ---- batches: many applies in one grouped launch per stage -------------
+ * A batch runs the applies of n plans (any sizes and shapes) as one launch of
+ * each stage: the members' task tables are merged, their operator arenas
+ * copied into one.  x and y hold the members' vectors back to back at the
+ * offsets tjsq_batch_offsets returns (n + 1 entries, the last = total).
+ * Each member's result is bit-identical to its plan's own apply.  With
+ * tjsq_batch_graph(b, 1) the two stages are captured once into a hipGraph
+ * (per precision and x / y pair) and replayed.  Plans may be destroyed after
+ * the batch is created. */
+typedef struct tjsq_batch tjsq_batch;
+int tjsq_batch_create(tjsq_plan* const* plans, int n, void* stream, tjsq_batch** batch);
+int tjsq_batch_offsets(const tjsq_batch* batch, int64_t* off);
+int tjsq_batch_stats(const tjsq_batch* batch, tjsq_plan_stats* stats);
+int tjsq_batch_graph(tjsq_batch* batch, int enable);
+int tjsq_batch_apply(tjsq_batch* batch, const double* x, double* y, void* stream);
+int tjsq_batch_apply_f32(tjsq_batch* batch, int precision, const float* x, float* y, void* stream);
+int tjsq_batch_profile(tjsq_batch* batch, int enable);
+int tjsq_batch_profile_read(tjsq_batch* batch, double* ms, int64_t* launches);
+int tjsq_batch_destroy(tjsq_batch* batch);
 
 /* Last error message of the calling thread (static storage). */
 const char* tjsq_last_error(void);

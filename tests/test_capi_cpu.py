@@ -92,3 +92,29 @@ def test_plan_rejects_lattice_too_small():
     small = model(nx=2)
     with pytest.raises(RuntimeError, match="lattice too small"):
         capi.HostPlan(small, sb)
+
+
+# every work-item shape the autotuner may choose compiles to tables that
+# reproduce the oracle (blocks large enough for 64-tiles, K splits and
+# partial tiles)
+TUNE_SHAPES = [None, {"big": 32}, {"k_split": 32, "item_chunks": 1}, {"k_split": 64, "item_chunks": 4},
+               {"k_split": 1024, "item_chunks": 64}, {"xcds": 1}, {"big": 32, "k_split": 128, "xcds": 4}]
+
+
+@pytest.mark.parametrize("shape", TUNE_SHAPES)
+def test_plan_shapes_reproduce_oracle(shape):
+    m = model()
+    sb = small_superblock(m, 11, 11, seed=23, msys=240, menv=220, median=30, nsec=8)
+    x = synth.random_vector(sb, 4)
+    hp = capi.HostPlan(m, sb, shape=shape)
+    assert rel(plan_eval.evaluate(hp, sb, x), oracle.model_wave(m, sb, x)) < 1e-13
+    if shape and shape.get("big") == 32:
+        assert hp.n64_1 == 0 and hp.n64_2 == 0
+
+
+def test_plan_rejects_unsupported_shape():
+    m = model()
+    sb = small_superblock(m, 7, 9, seed=2)
+    for bad in ({"big": 128}, {"small": 16}, {"k_split": 8}, {"item_chunks": 0}, {"xcds": 0}):
+        with pytest.raises(RuntimeError, match="shape"):
+            capi.HostPlan(m, sb, shape=bad)
