@@ -127,7 +127,7 @@ def batch_member(args):
     env_len = n - 2 - sys_len
     model = TJModel(ny=ny, nx=nx, t=1.0, tp=float(r.choice([0.0, -0.2])), J=float(r.uniform(0.2, 0.6)), mu=0.0,
                     up_bias=1)
-    nsec = int(np.clip(m // 150, 8, 40))
+    nsec = int(r.integers(4, 9))
     sb = synth.superblock(model, sys_len, env_len, m, m, seed=seed + i, median=32, sigma=0.8, maxdim=512,
                           nsec=nsec, smax=10, filling=0.875)
     return model, sb, {"ny": ny, "m": m, "sys_len": sys_len, "env_len": env_len}
@@ -149,10 +149,10 @@ def batch_bench(args):
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
-    prec = args.precision if args.precision != "fp64" else "fp32"
+    prec = args.batch_precision
     plans, xs, tuned = [], [], []
     for k, (model, sb, meta) in enumerate(members):
-        p = capi.Plan(model, sb, precisions=("fp64", prec))
+        p = capi.Plan(model, sb, precisions=(prec,))
         if not args.no_tune:
             tuned.append(autotune.tune(p, model, sb, precision=prec, reps=3)["shape"])
         plans.append(p)
@@ -180,12 +180,13 @@ def batch_bench(args):
             p.apply(x, y, precision=prec)
     seq()
     ms_seq = timed(seq, args.steps)
-    b = capi.Batch(plans)
+    b = capi.Batch(plans, precisions=(prec,))
+    del plans, p            # the batch holds its own operand copies
     x = torch.cat(xs)
     y = torch.empty_like(x)
     b.apply(x, y, precision=prec)
     ms_batch = timed(lambda: b.apply(x, y, precision=prec), args.steps)
-    same = all(torch.equal(y[int(b.offsets[i]):int(b.offsets[i + 1])], ys[i]) for i in range(len(plans)))
+    same = all(torch.equal(y[int(b.offsets[i]):int(b.offsets[i + 1])], ys[i]) for i in range(len(xs)))
     b.graph(True)
     b.apply(x, y, precision=prec)
     ms_graph = timed(lambda: b.apply(x, y, precision=prec), args.steps)
@@ -200,12 +201,12 @@ def batch_bench(args):
     fl = stb["flops_exec"]
     best = min(ms_batch, ms_graph)
     line = {
-        "metric": "batched H_eff applies/s (grouped launch per stage)", "value": len(plans) / (best * 1e-3),
+        "metric": "batched H_eff applies/s (grouped launch per stage)", "value": len(xs) / (best * 1e-3),
         "unit": "applies/s", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": best,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": {"fp32": "f32",
                                                                                    "bf16x3": "bf16x3->f32"}[prec],
         "data": "synthetic",
-        "config": {"workload": f"{len(plans)} independent H_eff applies, bonds of width-4/6 t-J cylinders, "
+        "config": {"workload": f"{len(xs)} independent H_eff applies, bonds of width-4/6 t-J cylinders, "
                                "m ~ U[1000, 6000], sector dims log-normal(median 32, max 512)",
                    "D_total": b.dim, "contributions": stb["n_contrib"], "precision": prec},
         "sequential_ms": ms_seq, "batch_ms": ms_batch, "batch_graph_ms": ms_graph,
@@ -254,6 +255,7 @@ def main():
     ap.add_argument("--batch", type=int, default=0,
                     help="instead of the solve benchmark: N independent applies in one grouped launch")
     ap.add_argument("--no-tune", action="store_true", help="batch mode: keep the default work-item shape")
+    ap.add_argument("--batch-precision", default="fp32", choices=["fp32", "bf16x3"])
     ap.add_argument("--warm-iter", type=int, default=200)
     ap.add_argument("--noise", type=float, default=0.01)
     ap.add_argument("--apply-only", type=int, default=0,

@@ -135,7 +135,7 @@ def lib():
             "tjsq_truncation_operator": [I, P, P, P, I, I, C.c_double, P, P, P, P, P, P],
             "tjsq_truncation_select": [I, P, P, I, I, C.c_double, P, P, P, P],
             "tjsq_identity_truncation": [P, P, P, P, P, P],
-            "tjsq_batch_create": [P, I, P, P],
+            "tjsq_batch_create": [P, I, I, P, P],
             "tjsq_batch_offsets": [P, P],
             "tjsq_batch_stats": [P, P],
             "tjsq_batch_graph": [P, I],
@@ -572,12 +572,15 @@ class Batch:
 
     Vectors are the members' vectors back to back at ``offsets``."""
 
-    def __init__(self, plans, graph=False):
+    def __init__(self, plans, graph=False, precisions=("fp64", "fp32", "bf16x3")):
         import torch
         hs = (C.c_void_p * len(plans))(*[p.h.value for p in plans])
         self._h = C.c_void_p()
         s = torch.cuda.current_stream().cuda_stream
-        check(lib().tjsq_batch_create(hs, len(plans), C.c_void_p(s), C.byref(self._h)), "tjsq_batch_create")
+        mask = 0
+        for pr in precisions:
+            mask |= 1 << PRECISIONS[pr]
+        check(lib().tjsq_batch_create(hs, len(plans), mask, C.c_void_p(s), C.byref(self._h)), "tjsq_batch_create")
         off = np.zeros(len(plans) + 1, np.int64)
         check(lib().tjsq_batch_offsets(self._h, off.ctypes.data), "tjsq_batch_offsets")
         self.offsets = off

@@ -476,10 +476,25 @@ bool to_shape(const tjsq_shape* in, tjsq::Shape& out) {
 
 // device copies of the task tables, FP64 stage-1 scratch and partial tiles
 // (plus the FP32 ones when with32)
-int upload_tables(tjsq_plan* p, bool with32, hipStream_t st) {
+// FP64 stage-1 scratch and partial tiles, allocated on first FP64 use
+int ensure_fp64_scratch(tjsq_plan* p) {
+  if (p->t64) return TJSQ_OK;
   const HostPlan& hp = p->hp;
-  if (hipMalloc(&p->t64, sizeof(double) * std::max<int64_t>(1, hp.scratch_elems)) != hipSuccess)
-    return fail(TJSQ_ERR_NOMEM, "scratch allocation failed");
+  if (hipMalloc(&p->t64, sizeof(double) * std::max<int64_t>(1, hp.scratch_elems)) != hipSuccess ||
+      hipMalloc(&p->part64, sizeof(double) * std::max<int64_t>(1, hp.nslots * 1024)) != hipSuccess) {
+    (void)hipFree(p->t64);
+    p->t64 = nullptr;
+    return fail(TJSQ_ERR_NOMEM, "FP64 scratch allocation failed");
+  }
+  return TJSQ_OK;
+}
+
+int upload_tables(tjsq_plan* p, bool with64, bool with32, hipStream_t st) {
+  const HostPlan& hp = p->hp;
+  if (with64) {
+    int rc = ensure_fp64_scratch(p);
+    if (rc) return rc;
+  }
   if (with32 && hipMalloc(&p->t32, sizeof(float) * std::max<int64_t>(1, hp.scratch_elems)) != hipSuccess)
     return fail(TJSQ_ERR_NOMEM, "FP32 scratch allocation failed");
   auto put = [&](auto** dptr, const auto& vec) -> int {
@@ -492,8 +507,6 @@ int upload_tables(tjsq_plan* p, bool with32, hipStream_t st) {
   if (put(&p->tile1, hp.tile1) || put(&p->tile2, hp.tile2) || put(&p->seg1, hp.seg1) || put(&p->seg2, hp.seg2) ||
       put(&p->rtile, hp.rtile))
     return fail(TJSQ_ERR_NOMEM, "task table allocation failed");
-  if (hipMalloc(&p->part64, sizeof(double) * std::max<int64_t>(1, hp.nslots * 1024)) != hipSuccess)
-    return fail(TJSQ_ERR_NOMEM, "partial tile allocation failed");
   if (with32 && hipMalloc(&p->part32, sizeof(float) * std::max<int64_t>(1, hp.nslots * 1024)) != hipSuccess)
     return fail(TJSQ_ERR_NOMEM, "partial tile allocation failed");
   return TJSQ_OK;
@@ -757,7 +770,7 @@ int tjsq_plan_create_ex(const tjsq_lattice* lat, int sys_len, int env_len, const
                        hp.arena_elems);
     HIPCHK(hipGetLastError());
   }
-  if ((rc = upload_tables(p, p->arena32 != nullptr, st))) {
+  if ((rc = upload_tables(p, (precision_mask & 1) != 0, p->arena32 != nullptr, st))) {
     tjsq_plan_destroy(p);
     return rc;
   }
@@ -845,7 +858,7 @@ int tjsq_plan_reshape(tjsq_plan* p, const tjsq_shape* shape, void* stream) {
   for (void* b : bufs) (void)hipFree(b);
   p->tile1 = p->tile2 = nullptr; p->seg1 = p->seg2 = nullptr; p->rtile = nullptr;
   p->t64 = nullptr; p->t32 = nullptr; p->part64 = nullptr; p->part32 = nullptr;
-  if ((rc = upload_tables(p, p->arena32 != nullptr, st))) return rc;
+  if ((rc = upload_tables(p, false, p->arena32 != nullptr, st))) return rc;
   HIPCHK(hipStreamSynchronize(st));
   return TJSQ_OK;
 }
@@ -904,7 +917,9 @@ int tjsq_plan_profile_read(tjsq_plan* p, double* ms, int64_t* launches) {
 int tjsq_heff_apply(tjsq_plan* p, const double* x, double* y, void* stream) {
   if (!p || !x || !y) return fail(TJSQ_ERR_ARG, "tjsq_heff_apply: null argument");
   hipStream_t st = (hipStream_t)stream;
-  int rc = launch_tiles<double>(p, 1, p->arena, x, p->t64, y, TJSQ_FP64, st);
+  int rc = ensure_fp64_scratch(p);
+  if (rc) return rc;
+  rc = launch_tiles<double>(p, 1, p->arena, x, p->t64, y, TJSQ_FP64, st);
   if (rc) return rc;
   return launch_tiles<double>(p, 2, p->arena, x, p->t64, y, TJSQ_FP64, st);
 }
@@ -1422,7 +1437,7 @@ int tjsq_program_run(tjsq_program* P, const double* ops, const double* x, double
   int rc;
   if (!P->uploaded) {
     (void)hipGetDevice(&P->dev->device);
-    if ((rc = upload_tables(P->dev, false, st))) return rc;
+    if ((rc = upload_tables(P->dev, true, false, st))) return rc;
     P->uploaded = true;
   }
   const double* xin = x;
@@ -1709,13 +1724,15 @@ int batch_run(tjsq_batch* B, int precision, const T* x, T* y, hipStream_t st) {
 
 extern "C" {
 
-int tjsq_batch_create(tjsq_plan* const* plans, int n, void* stream, tjsq_batch** out) {
-  if (!plans || n < 1 || !out) return fail(TJSQ_ERR_ARG, "tjsq_batch_create: bad argument");
+int tjsq_batch_create(tjsq_plan* const* plans, int n, int precision_mask, void* stream, tjsq_batch** out) {
+  if (!plans || n < 1 || !out || (precision_mask & 7) == 0) return fail(TJSQ_ERR_ARG, "tjsq_batch_create: bad argument");
   *out = nullptr;
-  bool with32 = true;
+  const bool with64 = (precision_mask & 1) != 0;
+  const bool with32 = (precision_mask & 6) != 0;
   for (int i = 0; i < n; ++i) {
     if (!plans[i]) return fail(TJSQ_ERR_ARG, "tjsq_batch_create: null plan");
-    with32 = with32 && plans[i]->arena32 != nullptr;
+    if (with32 && !plans[i]->arena32)
+      return fail(TJSQ_ERR_ARG, "tjsq_batch_create: a member plan was created without FP32 operands");
   }
   hipStream_t st = (hipStream_t)stream;
   tjsq_batch* B = new tjsq_batch();
@@ -1788,20 +1805,20 @@ int tjsq_batch_create(tjsq_plan* const* plans, int n, void* stream, tjsq_batch**
     tjsq_batch_destroy(B);
     return fail(code, m);
   };
-  if (hipMalloc(&B->dev->arena, sizeof(double) * std::max<int64_t>(1, arena)) != hipSuccess)
+  if (with64 && hipMalloc(&B->dev->arena, sizeof(double) * std::max<int64_t>(1, arena)) != hipSuccess)
     return cleanup(TJSQ_ERR_NOMEM, "batch arena allocation failed");
   if (with32 && hipMalloc(&B->dev->arena32, sizeof(float) * std::max<int64_t>(1, arena)) != hipSuccess)
     return cleanup(TJSQ_ERR_NOMEM, "batch FP32 arena allocation failed");
   for (int i = 0; i < n; ++i) {
     const int64_t e = plans[i]->hp.arena_elems;
     if (e == 0) continue;
-    if (hipMemcpyAsync(B->dev->arena + abase[i], plans[i]->arena, sizeof(double) * e, hipMemcpyDeviceToDevice, st) !=
-            hipSuccess ||
+    if ((with64 && hipMemcpyAsync(B->dev->arena + abase[i], plans[i]->arena, sizeof(double) * e,
+                                  hipMemcpyDeviceToDevice, st) != hipSuccess) ||
         (with32 && hipMemcpyAsync(B->dev->arena32 + abase[i], plans[i]->arena32, sizeof(float) * e,
                                   hipMemcpyDeviceToDevice, st) != hipSuccess))
       return cleanup(TJSQ_ERR_HIP, "batch arena copy failed");
   }
-  int rc = upload_tables(B->dev, with32, st);
+  int rc = upload_tables(B->dev, with64, with32, st);
   if (rc) {
     tjsq_batch_destroy(B);
     return rc;
@@ -1831,13 +1848,14 @@ int tjsq_batch_graph(tjsq_batch* B, int enable) {
 
 int tjsq_batch_apply(tjsq_batch* B, const double* x, double* y, void* stream) {
   if (!B || !x || !y) return fail(TJSQ_ERR_ARG, "tjsq_batch_apply: null argument");
+  if (!B->dev->arena) return fail(TJSQ_ERR_ARG, "tjsq_batch_apply: the batch was created without FP64");
   return batch_run<double>(B, TJSQ_FP64, x, y, (hipStream_t)stream);
 }
 
 int tjsq_batch_apply_f32(tjsq_batch* B, int precision, const float* x, float* y, void* stream) {
   if (!B || !x || !y || (precision != TJSQ_FP32 && precision != TJSQ_BF16X3))
     return fail(TJSQ_ERR_ARG, "tjsq_batch_apply_f32: bad argument");
-  if (!B->dev->arena32) return fail(TJSQ_ERR_ARG, "tjsq_batch_apply_f32: members were created without FP32 operands");
+  if (!B->dev->arena32) return fail(TJSQ_ERR_ARG, "tjsq_batch_apply_f32: the batch was created without FP32");
   return batch_run<float>(B, precision, x, y, (hipStream_t)stream);
 }
 

@@ -177,7 +177,8 @@ int tjsq_active_sites(const tjsq_lattice* lat, int len);
  * uploads (or adopts) the operator data, builds the merged operators on the
  * device and the grouped-GEMM task tables.  precision_mask selects which
  * operand copies to prepare (bit p for TJSQ_FP64/FP32/BF16X3); FP64 is always
- * prepared.  stream may be NULL (default stream). */
+ * prepared (its stage-1 scratch and partial tiles are allocated on the first
+ * FP64 apply when bit 0 is clear).  stream may be NULL (default stream). */
 int tjsq_plan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys,
                      const tjsq_opset* env, const tjsq_basis* sys_bs, const tjsq_basis* env_bs,
                      int nw, const int* wsec, int precision_mask, void* stream, tjsq_plan** plan);
@@ -354,10 +355,11 @@ int tjsq_identity_truncation(const tjsq_basis* basis, int* trun_sub, int64_t* tr
  * offsets tjsq_batch_offsets returns (n + 1 entries, the last = total).
  * Each member's result is bit-identical to its plan's own apply.  With
  * tjsq_batch_graph(b, 1) the two stages are captured once into a hipGraph
- * (per precision and x / y pair) and replayed.  Plans may be destroyed after
- * the batch is created. */
+ * (per precision and x / y pair) and replayed.  precision_mask (bits as in
+ * tjsq_plan_create) selects the operand copies the batch holds.  Plans may be
+ * destroyed after the batch is created. */
 typedef struct tjsq_batch tjsq_batch;
-int tjsq_batch_create(tjsq_plan* const* plans, int n, void* stream, tjsq_batch** batch);
+int tjsq_batch_create(tjsq_plan* const* plans, int n, int precision_mask, void* stream, tjsq_batch** batch);
 int tjsq_batch_offsets(const tjsq_batch* batch, int64_t* off);
 int tjsq_batch_stats(const tjsq_batch* batch, tjsq_plan_stats* stats);
 int tjsq_batch_graph(tjsq_batch* batch, int enable);
