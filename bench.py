@@ -219,6 +219,111 @@ def batch_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def serve_request(args):
+    """One request of the serving scan: the centre bond of a width-4/6,
+    length 8-16 t-J cylinder, J/t ~ U[0.2, 0.6], t'/t in {0, -0.2}, doping in
+    {1/12, 1/8, 1/6}, m ~ U[500, 6000]."""
+    i, seed = args
+    from grand_sql_tj_amd import synth
+    from grand_sql_tj_amd.lattice import TJModel
+    r = np.random.default_rng(seed + 104729 * i)
+    ny = int(r.choice([4, 6]))
+    nx = int(r.integers(8, 17))
+    n = ny * nx
+    dop = float(r.choice([1 / 12, 1 / 8, 1 / 6]))
+    ne = int(round(n * (1 - dop)))
+    ne -= ne % 2
+    m = int(r.integers(500, 6001))
+    model = TJModel(ny=ny, nx=nx, t=1.0, tp=float(r.choice([0.0, -0.2])), J=float(r.uniform(0.2, 0.6)), mu=0.0,
+                    up_bias=1)
+    sys_len = n // 2 - 1
+    env_len = n - 2 - sys_len
+    sb = synth.superblock(model, sys_len, env_len, m, m, seed=seed + i, median=32, sigma=0.8, maxdim=512,
+                          nsec=int(np.clip(m // 400, 4, 14)), smax=10, filling=ne / n, total_up=ne)
+    psi0 = synth.random_vector(sb, seed + i)
+    return model, sb, psi0, {"ny": ny, "nx": nx, "m": m, "N": ne, "J": model.J, "D": sb.dim}
+
+
+def serve_bench(args):
+    """Poisson arrivals of solve requests into the continuous-admission server."""
+    import multiprocessing as mp
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
+        reqs = []
+        for k, res in enumerate(pool.imap(serve_request, [(i, args.seed) for i in range(args.serve + 1)])):
+            reqs.append(res)
+            print(f"[bench] request {k}: {res[3]}", file=sys.stderr, flush=True)
+    synth_s = time.perf_counter() - t0
+    import torch
+    from grand_sql_tj_amd import capi, serve
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
+    prec = args.batch_precision if args.precision == "fp64" else args.precision
+
+    def make(k, rid):
+        model, sb, psi0, meta = reqs[k]
+        nsite = sb.sys_len + sb.env_len + 2
+        return serve.Request(rid, model, sb, psi0, precision=prec, tol=args.tol, max_iter=args.max_iter,
+                             energy_tol=0.1 * 1e-8 * nsite)
+    # warm-up (library, solver handles, allocator) and the one-at-a-time
+    # service time that sets the arrival rate
+    serve.solve_alone(make(args.serve, -1))
+    cal = []
+    for k in range(min(4, args.serve)):
+        t1 = time.perf_counter()
+        serve.solve_alone(make(k, -1))
+        cal.append(time.perf_counter() - t1)
+    s_alone = float(np.mean(cal))
+    lam = args.load / s_alone
+    rng = np.random.default_rng(args.seed)
+    gaps = rng.exponential(1.0 / lam, size=args.serve)
+    print(f"[bench] serving {args.serve} requests, {args.slots} slots, lambda={lam:.2f}/s "
+          f"(alone {s_alone * 1e3:.0f} ms)", file=sys.stderr, flush=True)
+    results = []
+    with serve.Server(slots=args.slots) as srv:
+        t_begin = time.perf_counter()
+        t_next = t_begin
+        for k in range(args.serve):
+            t_next += gaps[k]
+            while time.perf_counter() < t_next:
+                time.sleep(min(0.002, max(0.0, t_next - time.perf_counter())))
+            rq = make(k, k)
+            rq.t_arrival = time.perf_counter()
+            results.append(srv.submit(rq))
+        for r in results:
+            r.wait(600)
+            print(f"[bench] done {r.id} slot={r.slot} latency={r.latency * 1e3:.0f} ms", file=sys.stderr, flush=True)
+        t_end = max(r.t_done for r in results)
+        busy = srv.busy_time
+        peak = srv.peak_in_use
+        budget = srv.budget
+    lat = np.array([r.latency for r in results])
+    wait = np.array([r.t_admit - r.t_arrival for r in results])
+    # the same requests alone: results must be bit-identical
+    check = list(range(0, args.serve, max(1, args.serve // 4)))
+    same = True
+    for k in check:
+        e0, psi = serve.solve_alone(make(k, k))
+        same = same and e0 == results[k].e0 and bool(np.array_equal(psi, results[k].psi))
+    wall = t_end - t_begin
+    line = {
+        "metric": "serving: solves/s with p50/p99 latency (Poisson arrivals, continuous admission)",
+        "value": args.serve / wall, "unit": "solves/s", "n_gpus": 1, "steps": args.serve, "warmup": 1,
+        "ms_per_step": 1e3 * wall / args.serve, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp32": "f32", "bf16x3": "bf16x3->f32"}[prec] + "+f64", "data": "synthetic",
+        "config": {"workload": "ground-state solves at the centre bond of width-4/6, length 8-16 t-J cylinders; "
+                               "J/t~U[0.2,0.6], t'/t in {0,-0.2}, doping in {1/12,1/8,1/6}, m~U[500,6000]",
+                   "slots": args.slots, "arrival_rate_per_s": lam, "precision": prec},
+        "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+        "mean_queue_wait_ms": 1e3 * float(wait.mean()), "alone_service_ms": 1e3 * s_alone,
+        "busy_fraction": busy / wall, "offered_load_vs_alone_rate": args.load,
+        "peak_admitted_bytes": peak, "hbm_budget_bytes": budget,
+        "bit_identical_alone_and_in_mix": same, "requests_checked_alone": len(check), "synthesis_s": synth_s,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(model, sb, x0, n_iter, budget_s=20.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -256,6 +361,11 @@ def main():
                     help="instead of the solve benchmark: N independent applies in one grouped launch")
     ap.add_argument("--no-tune", action="store_true", help="batch mode: keep the default work-item shape")
     ap.add_argument("--batch-precision", default="fp32", choices=["fp32", "bf16x3"])
+    ap.add_argument("--serve", type=int, default=0,
+                    help="instead of the solve benchmark: N solve requests with Poisson arrivals")
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--load", type=float, default=0.8,
+                    help="arrival rate as a fraction of the one-at-a-time service rate")
     ap.add_argument("--warm-iter", type=int, default=200)
     ap.add_argument("--noise", type=float, default=0.01)
     ap.add_argument("--apply-only", type=int, default=0,
@@ -268,6 +378,9 @@ def main():
 
     if args.batch:
         batch_bench(args)
+        return
+    if args.serve:
+        serve_bench(args)
         return
     import torch
     from grand_sql_tj_amd import capi

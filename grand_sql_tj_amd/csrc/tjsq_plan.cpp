@@ -9,6 +9,7 @@
 #include "tjsq_plan.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -26,14 +27,14 @@ static constexpr int NEIBT = 12; // common.f90:34
 // 6j symbol (W6JS, block.f90:1564-1615; FACTRL, block.f90:1508-1516)
 // --------------------------------------------------------------------------
 static const double* fact_table() {
-  static double f[302];
-  static bool ready = false;
-  if (!ready) {
-    f[0] = 1.0;
-    for (int i = 1; i <= 142; ++i) f[i] = f[i - 1] * (double)(float)i;
-    ready = true;
-  }
-  return f;
+  // initialised once, thread-safe (plans are compiled on several threads)
+  static const std::array<double, 302> f = [] {
+    std::array<double, 302> a{};
+    a[0] = 1.0;
+    for (int i = 1; i <= 142; ++i) a[i] = a[i - 1] * (double)(float)i;
+    return a;
+  }();
+  return f.data();
 }
 
 double w6js(int j1, int j2, int j3, int l1, int l2, int l3) {

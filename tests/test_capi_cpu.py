@@ -118,3 +118,16 @@ def test_plan_rejects_unsupported_shape():
     for bad in ({"big": 128}, {"small": 16}, {"k_split": 8}, {"item_chunks": 0}, {"xcds": 0}):
         with pytest.raises(RuntimeError, match="shape"):
             capi.HostPlan(m, sb, shape=bad)
+
+
+def test_serving_footprint_counts_the_plan():
+    from grand_sql_tj_amd import serve
+    m = model()
+    small = small_superblock(m, 7, 9, seed=2, msys=40, menv=44, median=5)
+    big = small_superblock(m, 7, 9, seed=2, msys=160, menv=150, median=14)
+    hp = capi.HostPlan(m, big)
+    f64 = serve.footprint(m, big, "fp64", 40)
+    f32 = serve.footprint(m, big, "fp32", 40)
+    assert f64 >= 8 * (hp.arena_elems + hp.scratch_elems) + 42 * big.dim * 8
+    assert f32 > f64 - 42 * big.dim * 4          # extra FP32 operands outweigh the smaller basis
+    assert serve.footprint(m, small, "fp64", 40) < f64
