@@ -26,10 +26,16 @@ from . import capi
 
 
 class Request:
-    def __init__(self, rid, model, sb, psi0, precision="bf16x3", tol=1e-10, max_iter=40, energy_tol=0.0):
+    """device_ops: optional (sys, env) torch float64 CUDA tensors holding the
+    block operators already on the device (the environments a sweep keeps
+    resident); the plan then copies them device to device."""
+
+    def __init__(self, rid, model, sb, psi0, precision="bf16x3", tol=1e-10, max_iter=40, energy_tol=0.0,
+                 device_ops=None):
         self.id = rid
         self.model = model
         self.sb = sb
+        self.device_ops = device_ops
         self.psi0 = np.ascontiguousarray(psi0, np.float64)
         self.precision = precision
         self.tol = tol
@@ -44,7 +50,7 @@ class Result:
         self.id = req.id
         self.t_arrival = req.t_arrival
         self.footprint = req.footprint
-        self.t_admit = self.t_start = self.t_solved = self.t_done = None
+        self.t_admit = self.t_start = self.t_plan = self.t_solved = self.t_done = None
         self.e0 = None
         self.psi = None
         self.info = None
@@ -157,7 +163,9 @@ class Server:
             try:
                 with torch.cuda.stream(stream):
                     res.t_start = time.perf_counter()
-                    plan = capi.Plan(req.model, req.sb, precisions=("fp64", req.precision))
+                    plan = capi.Plan(req.model, req.sb, precisions=("fp64", req.precision),
+                                     device_ops=req.device_ops)
+                    res.t_plan = time.perf_counter()
                     psi = torch.from_numpy(req.psi0).to("cuda")
                     info = plan.lanczos(psi, tol=req.tol, max_iter=req.max_iter, precision=req.precision,
                                         refine_tol=1e-6, max_refine=1, energy_tol=req.energy_tol)
@@ -192,7 +200,7 @@ def solve_alone(req: Request):
     """The same solve outside the server (the reference for the equality
     check): returns (e0, psi)."""
     import torch
-    plan = capi.Plan(req.model, req.sb, precisions=("fp64", req.precision))
+    plan = capi.Plan(req.model, req.sb, precisions=("fp64", req.precision), device_ops=req.device_ops)
     psi = torch.from_numpy(req.psi0).to("cuda")
     info = plan.lanczos(psi, tol=req.tol, max_iter=req.max_iter, precision=req.precision, refine_tol=1e-6,
                         max_refine=1, energy_tol=req.energy_tol)
