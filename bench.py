@@ -369,6 +369,7 @@ def main():
     ap.add_argument("--batch", type=int, default=0,
                     help="instead of the solve benchmark: N independent applies in one grouped launch")
     ap.add_argument("--no-tune", action="store_true", help="batch mode: keep the default work-item shape")
+    ap.add_argument("--tune", action="store_true", help="autotune the work-item shape of the solve plan first")
     ap.add_argument("--batch-precision", default="fp32", choices=["fp32", "bf16x3"])
     ap.add_argument("--serve", type=int, default=0,
                     help="instead of the solve benchmark: N solve requests with Poisson arrivals")
@@ -416,6 +417,12 @@ def main():
                           "stats": plan.stats, "copy_bytes": 512 * 1024 * 1024}), flush=True)
         return
     plan = capi.Plan(model, sb)
+    tuned = None
+    if args.tune:
+        from grand_sql_tj_amd import autotune
+        tuned = autotune.tune(plan, model, sb, precision=args.precision, reps=5)
+        for shp, ms in tuned["results"]:
+            print(f"[bench] shape {shp}: {ms:.3f} ms/apply", file=sys.stderr, flush=True)
     st = plan.stats
     nsite = sb.sys_len + sb.env_len + 2
     print(f"[bench] D={sb.dim} contributions={st['n_contrib']} F={st['flops_ref']:.3e}", file=sys.stderr, flush=True)
@@ -501,6 +508,8 @@ def main():
                         "frac": achieved / peak, "traffic": None,
                         "algorithmic_flops_per_launch": fl_dom, "avg_launch_ms": t_dom * 1e3}
     line["cpu_baseline"] = cpu
+    if tuned is not None:
+        line["work_item_shape"] = tuned["shape"]
     if not args.no_update:
         line["block_update"] = block_update_step(model, sb, g, args.m, torch)
         print(f"[bench] block update {line['block_update']}", file=sys.stderr, flush=True)

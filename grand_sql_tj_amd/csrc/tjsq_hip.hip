@@ -53,7 +53,6 @@ int fail(int code, const std::string& m) {
     if (e_ != hipSuccess) return fail(TJSQ_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
   } while (0)
 
-constexpr int KC = 32;      // K chunk staged per step
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -79,7 +78,7 @@ __device__ __forceinline__ const T* space_ptr(const Spaces<T>& S, int sp) {
 // Grouped tile kernel (one template for FP64, FP32 and BF16x3)
 //
 // One 64-lane wavefront owns one 32x32 tile and walks its segments in K
-// chunks of KC.  Chunk c+1's operand values are fetched into registers
+// chunks of KC (TileCfg).  Chunk c+1's operand values are fetched into registers
 // while the MFMAs of chunk c run from LDS, so a lone wave keeps one chunk of
 // global loads in flight.  Axpy segments (identity right operand) come first
 // in every segment list and are added straight into the accumulators.
@@ -105,16 +104,20 @@ __device__ __forceinline__ int acc_row(int lq, int r) {
   return MODE == MODE_F64 ? lq + 4 * r : lq * 4 + r;
 }
 
-template <int TSZ>
+template <int MODE, int TSZ>
 struct TileCfg {
   static constexpr int NT = TSZ == 64 ? 256 : 64;   // threads: one wave per 32x32 quarter
+  // K chunk staged per step.  A chunk of 64 for the 4-wave FP32 / BF16x3
+  // items costs one wave per SIMD of occupancy and measured slower: the
+  // kernel is latency-bound, so the chunk stays at 32 in every mode.
+  static constexpr int KC = 32;
   static constexpr int PER = TSZ * KC / NT;          // operand elements per thread per chunk
 };
 
 template <int MODE, int TSZ, typename T>
 __device__ __forceinline__ void fetch_chunk(const Spaces<T>& S, const Seg& s, const Tile& tl, int k0, int tid,
                                             T* rl, T* rr) {
-  constexpr int NT = TileCfg<TSZ>::NT, PER = TileCfg<TSZ>::PER;
+  constexpr int NT = TileCfg<MODE, TSZ>::NT, PER = TileCfg<MODE, TSZ>::PER, KC = TileCfg<MODE, TSZ>::KC;
   const T* L = space_ptr(S, s.l_space);
   const T* R = space_ptr(S, s.r_space);
 #pragma unroll
@@ -140,13 +143,13 @@ __device__ __forceinline__ void fetch_chunk(const Spaces<T>& S, const Seg& s, co
 // TSZ x TSZ output tile per workgroup; each wave computes a 32x32 quarter
 // (TSZ = 64: 4 waves share the LDS-staged operand panels).
 template <int MODE, int TSZ>
-__global__ __launch_bounds__(TileCfg<TSZ>::NT) void k_tiles(const Tile* __restrict__ tiles,
+__global__ __launch_bounds__((TileCfg<MODE, TSZ>::NT)) void k_tiles(const Tile* __restrict__ tiles,
                                                             const Seg* __restrict__ segs,
                                                             Spaces<typename KT<MODE>::T> S,
                                                             typename KT<MODE>::T* partial, int ntiles) {
   using T = typename KT<MODE>::T;
   using Acc = typename KT<MODE>::Acc;
-  constexpr int PER = TileCfg<TSZ>::PER;
+  constexpr int PER = TileCfg<MODE, TSZ>::PER, KC = TileCfg<MODE, TSZ>::KC;
   const int bid = blockIdx.x;
   if (bid >= ntiles) return;
   const Tile tl = tiles[bid];
@@ -194,7 +197,7 @@ __global__ __launch_bounds__(TileCfg<TSZ>::NT) void k_tiles(const Tile* __restri
       const T c = (T)cur.c;
 #pragma unroll
       for (int q = 0; q < PER; ++q) {
-        const int e = tid + TileCfg<TSZ>::NT * q;
+        const int e = tid + TileCfg<MODE, TSZ>::NT * q;
         int i, k;
         if (!cur.l_t) { i = e % TSZ; k = e / TSZ; } else { k = e % KC; i = e / KC; }
         int j, kr;
