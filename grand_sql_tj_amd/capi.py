@@ -85,7 +85,7 @@ EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_t
            "tjsq_truncation_select", "tjsq_identity_truncation", "tjsq_batch_create", "tjsq_batch_offsets",
            "tjsq_batch_stats", "tjsq_batch_graph", "tjsq_batch_apply", "tjsq_batch_apply_f32", "tjsq_batch_profile",
            "tjsq_batch_profile_read", "tjsq_batch_destroy", "tjsq_plan_create_ex", "tjsq_hostplan_create_ex",
-           "tjsq_plan_reshape", "tjsq_last_error"]
+           "tjsq_plan_reshape", "tjsq_plan_graphs", "tjsq_last_error"]
 
 _lib = None
 
@@ -147,6 +147,7 @@ def lib():
             "tjsq_plan_create_ex": [P, I, I, P, P, P, P, I, P, I, P, P, P],
             "tjsq_hostplan_create_ex": [P, I, I, P, P, P, P, I, P, P, P],
             "tjsq_plan_reshape": [P, P, P],
+            "tjsq_plan_graphs": [P, I],
         }
         for name in EXPORTS:
             f = getattr(L, name)
@@ -350,6 +351,10 @@ class Plan:
 
     def _stream(self):
         return C.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+
+    def graphs(self, enable=True):
+        """Replay each Lanczos iteration from a captured hipGraph."""
+        check(lib().tjsq_plan_graphs(self.h, int(enable)), "tjsq_plan_graphs")
 
     def reshape(self, shape):
         """Recompile the task tables for another work-item shape."""

@@ -416,6 +416,25 @@ __global__ void k_scale(T* __restrict__ w, int64_t n, double s) {
     w[i] = (T)((double)w[i] * s);
 }
 
+// w *= 1 / d[0]: the factor k_scale receives from the host, formed on the device
+template <typename T>
+__global__ void k_scale_inv_dev(T* __restrict__ w, int64_t n, const double* __restrict__ d) {
+  const double s = 1.0 / d[0];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    w[i] = (T)((double)w[i] * s);
+}
+
+// dst[0] = sqrt(src[0]) (a norm from its squared dot)
+__global__ void k_sqrt_to(double* dst, const double* src) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) dst[0] = sqrt(src[0]);
+}
+
+// Gram-Schmidt coefficients <v_j, w> / <v_j, v_j> from the dot outputs
+__global__ void k_gs_coef(double* coef, const double* dots, int nv) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nv) coef[j] = dots[j] / dots[nv + j];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -442,6 +461,11 @@ struct tjsq_plan {
   float* x32 = nullptr;          // FP32 staging for a residual check
   double* x64a = nullptr;
   double* x64b = nullptr;
+  // Lanczos iterations captured as hipGraphs, keyed by (precision, basis
+  // size); valid while the Krylov basis stays at `graph_kry`
+  bool use_graphs = false;
+  std::vector<std::pair<int, hipGraphExec_t>> iter_graphs;
+  void* graph_kry = nullptr;
   // per-launch HIP events of the two apply stages while profiling is on
   bool profiling = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof;
@@ -874,6 +898,8 @@ int tjsq_plan_stats_get(const tjsq_plan* plan, tjsq_plan_stats* stats) {
 
 int tjsq_plan_destroy(tjsq_plan* p) {
   if (!p) return TJSQ_OK;
+  for (auto& g : p->iter_graphs) (void)hipGraphExecDestroy(g.second);
+  p->iter_graphs.clear();
   (void)hipFree(p->arena);
   (void)hipFree(p->arena32);
   (void)hipFree(p->t64);
@@ -891,6 +917,12 @@ int tjsq_plan_destroy(tjsq_plan* p) {
   (void)hipFree(p->x64a);
   (void)hipFree(p->x64b);
   delete p;
+  return TJSQ_OK;
+}
+
+int tjsq_plan_graphs(tjsq_plan* p, int enable) {
+  if (!p) return fail(TJSQ_ERR_ARG, "tjsq_plan_graphs: null plan");
+  p->use_graphs = enable != 0;
   return TJSQ_OK;
 }
 
@@ -953,6 +985,8 @@ struct Krylov {
   double* partial;   // 2 * RED_BLOCKS * nv
   double* dots;      // device results (2 * (max_iter + 1))
   double* coef;      // device coefficients
+  double* ab;        // device alpha_j at 2j-1, beta_j at 2j (the three-term coefficients are adjacent)
+  double* tmp;       // device scratch scalars
   std::vector<double> hbuf;
 
   T* vec(int j) { return V + (int64_t)j * n; }
@@ -991,6 +1025,20 @@ struct Krylov {
     out = std::sqrt(d[0]);
     return TJSQ_OK;
   }
+  // ---- device-scalar forms (no host round trip) ----
+  // out[0] = V_j . w for nv vectors from Vbase (and the self dots after them)
+  void dots_dev(int nv, const T* Vbase, const T* w, bool self, double* out) {
+    hipLaunchKernelGGL(k_multidot<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, Vbase, n, nv, w, n, partial,
+                       self ? 1 : 0);
+    hipLaunchKernelGGL(k_finish_dots, dim3(nv * (self ? 2 : 1)), dim3(RED_THREADS), 0, st, partial, nv, RED_BLOCKS,
+                       self ? 1 : 0, out);
+  }
+  // w /= |w|, the norm also stored at nrm
+  void normalize_dev(T* w, double* nrm) {
+    dots_dev(1, w, w, false, tmp + 8);
+    hipLaunchKernelGGL(k_sqrt_to, dim3(1), dim3(64), 0, st, nrm, tmp + 8);
+    hipLaunchKernelGGL(k_scale_inv_dev<T>, dim3(grid_for(n)), dim3(256), 0, st, w, n, nrm);
+  }
 };
 
 int apply_prec(tjsq_plan* p, int precision, const void* x, void* y, hipStream_t st) {
@@ -1000,6 +1048,71 @@ int apply_prec(tjsq_plan* p, int precision, const void* x, void* y, hipStream_t 
 
 // One Lanczos pass from v0 (already in slot 0, normalised).  Returns the
 // number of vectors, the Ritz value and the tridiagonal data.
+// One Lanczos iteration at basis size `last` with every scalar on the device
+// (eigen.f90:63-90 with Gram_Schmidt_process, block.f90:2515-2551):
+//   w = H v_last - alpha_last v_last - beta_{last-1} v_{last-1}; beta_last = |w|;
+//   w /= beta_last; w -= sum_j (<v_j,w> / <v_j,v_j>) v_j; w /= |w|;
+//   v_{last+1} = H w; alpha_{last+1} = <w, H w>.
+// The operations are those of the host-scalar form (same kernels, the same
+// IEEE divisions and square roots), so results are bitwise the same.  A
+// beta at or below tol ends the pass; the rest of that iteration is unused.
+template <typename T>
+int iteration_launches(Krylov<T>& K, int precision, int last) {
+  T* w = K.vec(last);
+  if (last < 2)
+    hipLaunchKernelGGL(k_combine<T>, dim3(grid_for(K.n)), dim3(256), 0, K.st, w, K.vec(0), K.n, 1, K.ab + 1, K.n,
+                       1.0, 0);
+  else
+    hipLaunchKernelGGL(k_combine<T>, dim3(grid_for(K.n)), dim3(256), 0, K.st, w, K.vec(last - 2), K.n, 2,
+                       K.ab + 2 * last - 2, K.n, 1.0, 0);
+  K.normalize_dev(w, K.ab + 2 * last);
+  K.dots_dev(last, K.V, w, true, K.dots);
+  hipLaunchKernelGGL(k_gs_coef, dim3((last + 63) / 64), dim3(64), 0, K.st, K.coef, K.dots, last);
+  hipLaunchKernelGGL(k_combine<T>, dim3(grid_for(K.n)), dim3(256), 0, K.st, w, K.V, K.n, last, K.coef, K.n, 1.0, 0);
+  K.normalize_dev(w, K.tmp + 9);
+  HIPCHK(hipGetLastError());
+  int rc = apply_prec(K.p, precision, w, K.vec(last + 1), K.st);
+  if (rc) return rc;
+  K.dots_dev(1, w, K.vec(last + 1), false, K.ab + 2 * last + 1);
+  HIPCHK(hipGetLastError());
+  return TJSQ_OK;
+}
+
+// the iteration at basis size `last`, replayed from a hipGraph captured on
+// first use (plans with graphs enabled, on a non-default stream)
+template <typename T>
+int iteration(Krylov<T>& K, int precision, int last) {
+  tjsq_plan* p = K.p;
+  if (!p->use_graphs || p->profiling || K.st == nullptr) return iteration_launches(K, precision, last);
+  if (p->graph_kry != p->kry) {     // the basis moved: graphs hold its pointers
+    for (auto& g : p->iter_graphs) (void)hipGraphExecDestroy(g.second);
+    p->iter_graphs.clear();
+    p->graph_kry = p->kry;
+  }
+  const int key = precision * 1024 + last;
+  for (auto& g : p->iter_graphs)
+    if (g.first == key) {
+      HIPCHK(hipGraphLaunch(g.second, K.st));
+      return TJSQ_OK;
+    }
+  HIPCHK(hipStreamBeginCapture(K.st, hipStreamCaptureModeThreadLocal));
+  int rc = iteration_launches(K, precision, last);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(K.st, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(TJSQ_ERR_HIP, std::string("iteration capture failed: ") + hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(TJSQ_ERR_HIP, std::string("graph instantiation failed: ") + hipGetErrorString(e));
+  p->iter_graphs.push_back({key, exec});
+  HIPCHK(hipGraphLaunch(exec, K.st));
+  return TJSQ_OK;
+}
+
 template <typename T>
 int lanczos_pass(Krylov<T>& K, int precision, double tol, int max_iter, int& last_out, double& ritz_out,
                  std::vector<double>& alpha, std::vector<double>& beta, int& breakdown, int& applies,
@@ -1008,45 +1121,30 @@ int lanczos_pass(Krylov<T>& K, int precision, double tol, int max_iter, int& las
   beta.assign(max_iter + 1, 0.0);
   breakdown = 0;
   int last = 1;  // number of basis vectors (1-based as in eigen.f90)
+  if (precision == TJSQ_FP64) {
+    int rc0 = ensure_fp64_scratch(K.p);   // no allocation may happen inside a capture
+    if (rc0) return rc0;
+  }
   int rc = apply_prec(K.p, precision, K.vec(0), K.vec(1), K.st);
   if (rc) return rc;
   ++applies;
-  std::vector<double> d;
-  if ((rc = K.dots_with(1, K.vec(1), false, d, K.vec(0)))) return rc;
-  alpha[1] = d[0];
+  K.dots_dev(1, K.vec(0), K.vec(1), false, K.ab + 1);
+  HIPCHK(hipMemcpyAsync(&alpha[1], K.ab + 1, sizeof(double), hipMemcpyDeviceToHost, K.st));
+  HIPCHK(hipStreamSynchronize(K.st));
   double ritz = alpha[1];
+  double ba[2];
   while (last < max_iter) {
-    // r = H v_last - alpha_last v_last - beta_{last-1} v_{last-1}   (eigen.f90:63-72)
-    T* w = K.vec(last);
-    if (last < 2) {
-      std::vector<double> c = {alpha[last]};
-      if ((rc = K.combine(w, 1, c, 1.0))) return rc;
-    } else {
-      std::vector<double> c = {beta[last - 1], alpha[last]};  // slots last-2, last-1
-      if ((rc = K.combine(w, 2, c, 1.0, false, last - 2))) return rc;
-    }
-    double b;
-    if ((rc = K.norm(w, b))) return rc;
-    beta[last] = b;
-    if (std::fabs(b) <= tol) {  // eigen.f90:74-77
+    if ((rc = iteration(K, precision, last))) return rc;
+    HIPCHK(hipMemcpyAsync(ba, K.ab + 2 * last, 2 * sizeof(double), hipMemcpyDeviceToHost, K.st));
+    HIPCHK(hipStreamSynchronize(K.st));
+    beta[last] = ba[0];
+    if (std::fabs(ba[0]) <= tol) {  // eigen.f90:74-77
       breakdown = 1;
       break;
     }
-    if ((rc = K.scale(w, 1.0 / b))) return rc;
-    // classical Gram-Schmidt against v_1..v_last (block.f90:2515-2551)
-    std::vector<double> uv;
-    if ((rc = K.dots_with(last, w, true, uv))) return rc;
-    std::vector<double> c(last);
-    for (int j = 0; j < last; ++j) c[j] = uv[j] / uv[last + j];
-    if ((rc = K.combine(w, last, c, 1.0))) return rc;
-    double nb;
-    if ((rc = K.norm(w, nb))) return rc;
-    if ((rc = K.scale(w, 1.0 / nb))) return rc;
     ++last;
-    if ((rc = apply_prec(K.p, precision, K.vec(last - 1), K.vec(last), K.st))) return rc;
     ++applies;
-    if ((rc = K.dots_with(1, K.vec(last), false, d, K.vec(last - 1)))) return rc;
-    alpha[last] = d[0];
+    alpha[last] = ba[1];
     std::vector<double> dia(alpha.begin() + 1, alpha.begin() + last + 1);
     std::vector<double> off(beta.begin() + 1, beta.begin() + last + 1);
     tridiag_eig(last, dia, off, nullptr);
@@ -1090,7 +1188,7 @@ int ensure_krylov(tjsq_plan* p, int max_iter, Krylov<T>& K, hipStream_t st) {
     p->kry_bytes = need;
   }
   if (!p->dwork) {
-    if (hipMalloc(&p->dwork, sizeof(double) * (2 * RED_BLOCKS * 256 + 1024)) != hipSuccess)
+    if (hipMalloc(&p->dwork, sizeof(double) * (2 * RED_BLOCKS * 256 + 2048)) != hipSuccess)
       return fail(TJSQ_ERR_NOMEM, "workspace allocation failed");
   }
   K.p = p;
@@ -1100,6 +1198,8 @@ int ensure_krylov(tjsq_plan* p, int max_iter, Krylov<T>& K, hipStream_t st) {
   K.partial = p->dwork;
   K.dots = p->dwork + 2 * RED_BLOCKS * 256;
   K.coef = K.dots + 512;
+  K.ab = K.dots + 1024;
+  K.tmp = K.ab + 512;
   return TJSQ_OK;
 }
 

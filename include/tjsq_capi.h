@@ -222,6 +222,13 @@ int tjsq_heff_apply_f32(tjsq_plan* plan, int precision, const float* x, float* y
 int tjsq_plan_profile(tjsq_plan* plan, int enable);
 int tjsq_plan_profile_read(tjsq_plan* plan, double* ms, int64_t* launches);
 
+/* Capture each Lanczos iteration (recurrence, normalisation, Gram-Schmidt,
+ * apply, alpha) as a hipGraph on its first use and replay it in later
+ * iterations and solves of this plan (enable = 1).  Needs a non-default
+ * stream; results are bitwise those of the launched form.  One host read of
+ * (beta, alpha) per iteration remains for the tridiagonal problem. */
+int tjsq_plan_graphs(tjsq_plan* plan, int enable);
+
 /* Ground state (lanczos_memo_new, levels = 1): psi (device, FP64, length D)
  * holds the start vector on entry and the normalised ground state on exit. */
 int tjsq_lanczos_solve(tjsq_plan* plan, double* psi, const tjsq_lanczos_opts* opts,

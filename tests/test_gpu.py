@@ -106,3 +106,26 @@ def test_lanczos_mixed_precision_energy(prec):
     r2 = p.lanczos(psi2, tol=1e-10, max_iter=80, precision=prec, max_refine=2, energy_tol=target)
     assert abs(r2["e0"] - r64["e0"]) / nsite <= 1e-8
     assert r2["energy_err_est"] <= target or r2["refinements"] == 2
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_lanczos_graph_replay_is_bitwise_the_launched_solve(prec):
+    """Each iteration captured as a hipGraph (first solve) and replayed
+    (second solve) gives the launched form's result bit for bit."""
+    m = model()
+    sb = small_superblock(m, 11, 11, seed=21, msys=120, menv=120, median=12, nsec=10)
+    x = _cuda(synth.random_vector(sb, 9))
+    p = capi.Plan(m, sb)
+    s = torch.cuda.Stream()
+    outs = []
+    with torch.cuda.stream(s):
+        for graphs in (False, True, True):
+            p.graphs(graphs)
+            psi = x.clone()
+            r = p.lanczos(psi, tol=1e-10, max_iter=30, precision=prec, energy_tol=1e-7 if prec != "fp64" else 0.0)
+            s.synchronize()
+            outs.append((r["e0"], r["iterations"], psi.cpu()))
+    for e0, it, psi in outs[1:]:
+        assert e0 == outs[0][0] and it == outs[0][1]
+        assert torch.equal(psi, outs[0][2])
+    assert outs[0][1] >= 3
