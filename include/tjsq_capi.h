@@ -188,7 +188,7 @@ int tjsq_plan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tj
  * small = 32); stage-2 segments are cut into K pieces of at most k_split and
  * grouped into work items of at most item_chunks K chunks of 32; work items
  * are binned over `xcds` XCDs (1 = plain order).  NULL selects {64, 32, 256,
- * 32, 8}. */
+ * 32, 1}. */
 typedef struct tjsq_shape {
   int big;
   int small;
