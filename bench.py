@@ -45,18 +45,28 @@ def workload(m_mult: int, seed: int):
     return model, sb
 
 
-def timed_solves(plan, x0_t, steps, warmup, precision, tol, max_iter, torch, energy_tol=0.0, rtol=0.0):
+def timed_solves(plan, x0_t, steps, warmup, precision, tol, max_iter, torch, energy_tol=0.0, rtol=0.0, stream=None):
+    """Wall time of each solve (the solve synchronises its stream at the end).
+    On a side stream with the plan's graphs enabled, each Lanczos iteration is
+    replayed from its captured hipGraph (captured during the warm-up)."""
     lat = []
     res = None
-    for i in range(warmup + steps):
-        psi = x0_t.clone()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        res = plan.lanczos(psi, tol=tol, max_iter=max_iter, precision=precision, refine_tol=1e-6, max_refine=1,
-                           energy_tol=energy_tol, rtol=rtol)
-        torch.cuda.synchronize()
-        if i >= warmup:
-            lat.append(time.perf_counter() - t0)
+    ctx = torch.cuda.stream(stream) if stream is not None else None
+    if ctx is not None:
+        ctx.__enter__()
+    try:
+        for i in range(warmup + steps):
+            psi = x0_t.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res = plan.lanczos(psi, tol=tol, max_iter=max_iter, precision=precision, refine_tol=1e-6, max_refine=1,
+                               energy_tol=energy_tol, rtol=rtol)
+            torch.cuda.synchronize()
+            if i >= warmup:
+                lat.append(time.perf_counter() - t0)
+    finally:
+        if ctx is not None:
+            ctx.__exit__(None, None, None)
     return np.array(lat), res
 
 
@@ -370,6 +380,8 @@ def main():
                     help="instead of the solve benchmark: N independent applies in one grouped launch")
     ap.add_argument("--no-tune", action="store_true", help="batch mode: keep the default work-item shape")
     ap.add_argument("--tune", action="store_true", help="autotune the work-item shape of the solve plan first")
+    ap.add_argument("--no-graphs", dest="graphs", action="store_false",
+                    help="launch every Lanczos iteration instead of replaying its captured hipGraph")
     ap.add_argument("--batch-precision", default="fp32", choices=["fp32", "bf16x3"])
     ap.add_argument("--serve", type=int, default=0,
                     help="instead of the solve benchmark: N solve requests with Poisson arrivals")
@@ -437,21 +449,28 @@ def main():
     x0 = x0_t.cpu().numpy()
     print(f"[bench] warm start E={warm['e0']:.12f} ({warm['iterations']} it)", file=sys.stderr, flush=True)
 
-    lat64, r64 = timed_solves(plan, x0_t, args.steps, args.warmup, "fp64", args.tol, args.max_iter, torch)
+    side = None
+    if args.graphs:
+        side = torch.cuda.Stream()
+        plan.graphs(True)
+    lat64, r64 = timed_solves(plan, x0_t, args.steps, args.warmup, "fp64", args.tol, args.max_iter, torch,
+                              stream=side)
     print(f"[bench] fp64 solve {np.median(lat64)*1e3:.1f} ms E0={r64['e0']:.12f} it={r64['iterations']}",
           file=sys.stderr, flush=True)
     # the mixed solve restarts in FP64 only when its estimated energy error
     # exceeds a tenth of the 1e-8 per-site target
     etol = 0.1 * 1e-8 * nsite
     latmp, rmp = timed_solves(plan, x0_t, args.steps, args.warmup, args.precision, args.tol, args.max_iter, torch,
-                              energy_tol=etol)
+                              energy_tol=etol, stream=side)
     print(f"[bench] {args.precision} solve {np.median(latmp)*1e3:.1f} ms E0={rmp['e0']:.12f} it={rmp['iterations']} "
           f"refine={rmp['refinements']}", file=sys.stderr, flush=True)
 
     # FP64 with the mixed pass's stopping rule (|dE| <= 1e-7 |E|): the FP64
     # rate at matched iteration counts, reported beside the reference-tolerance
     # FP64 rate
-    latm, rm = timed_solves(plan, x0_t, args.steps, args.warmup, "fp64", args.tol, args.max_iter, torch, rtol=1e-7)
+    latm, rm = timed_solves(plan, x0_t, args.steps, args.warmup, "fp64", args.tol, args.max_iter, torch, rtol=1e-7,
+                            stream=side)
+    plan.graphs(False)
     print(f"[bench] fp64 (rtol 1e-7) solve {np.median(latm)*1e3:.1f} ms E0={rm['e0']:.12f} it={rm['iterations']}",
           file=sys.stderr, flush=True)
 
@@ -510,6 +529,7 @@ def main():
     line["cpu_baseline"] = cpu
     if tuned is not None:
         line["work_item_shape"] = tuned["shape"]
+    line["lanczos_iteration_graphs"] = bool(args.graphs)
     if not args.no_update:
         line["block_update"] = block_update_step(model, sb, g, args.m, torch)
         print(f"[bench] block update {line['block_update']}", file=sys.stderr, flush=True)
