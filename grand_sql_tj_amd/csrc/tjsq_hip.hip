@@ -308,10 +308,22 @@ __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ rt, con
   if (t >= n) return;
   const Tile tl = rt[t];
   const int units = tl.slot, pld = tl.out_space;
+  const int np = (tl.seg_end - tl.seg_begin) / units;
+  const int64_t step = (int64_t)units * 1024;
   for (int e = threadIdx.x; e < tl.mt * tl.nt; e += blockDim.x) {
     const int row = e % tl.mt, col = e / tl.mt;
+    const T* p = partial + (int64_t)tl.seg_begin * 1024 + row + col * pld;
+    // four independent loads in flight, added in ascending partial order
     T acc = T(0);
-    for (int q = tl.seg_begin; q < tl.seg_end; q += units) acc += partial[(int64_t)q * 1024 + row + col * pld];
+    int q = 0;
+    for (; q + 4 <= np; q += 4) {
+      const T a0 = p[q * step], a1 = p[(q + 1) * step], a2 = p[(q + 2) * step], a3 = p[(q + 3) * step];
+      acc += a0;
+      acc += a1;
+      acc += a2;
+      acc += a3;
+    }
+    for (; q < np; ++q) acc += p[q * step];
     y[tl.out_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * tl.out_ld] = acc;
   }
 }
