@@ -19,6 +19,25 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 _lock = threading.RLock()
 _libs = {}
 
+
+class _quiet:
+    """Sends what the reference prints on its standard output (unit 6) to
+    standard error, so a caller's standard output stays its own."""
+
+    def __enter__(self):
+        import sys
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        lib = _libs.get("ref")
+        if lib is not None:
+            lib.refb_flush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+
 I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 I64P = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 F64P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -124,6 +143,8 @@ def ref_lib():
                                             C.POINTER(C.c_int), C.POINTER(C.c_double)]
             lib.refb_quicksort.restype = C.c_int
             lib.refb_quicksort.argtypes = [C.c_int, F64P, I32P]
+            lib.refb_flush.restype = C.c_int
+            lib.refb_flush.argtypes = []
             _libs["ref"] = lib
         return _libs["ref"]
 
@@ -167,8 +188,9 @@ def hmerge_tables(model, sys_len, env_len):
 
 def _ref_init(model, lanzero=1e-10):
     lib = ref_lib()
-    rc = lib.refb_init(model.ny, model.up_bias, model.jt[0], model.jt[1], model.jd[0], model.jd[1], model.jn[0],
-                       model.jn[1], model.esite, lanzero)
+    with _quiet():
+        rc = lib.refb_init(model.ny, model.up_bias, model.jt[0], model.jt[1], model.jd[0], model.jd[1], model.jn[0],
+                           model.jn[1], model.esite, lanzero)
     if rc != 0:
         raise RuntimeError("reference build is compiled for a different lattice width")
     return lib
@@ -192,8 +214,9 @@ def ref_lanczos(model, sb, x0: np.ndarray, lanzero: float = 1e-10):
         lib = _ref_init(model, lanzero)
         x = np.ascontiguousarray(x0, dtype=np.float64).copy()
         e0 = C.c_double(0.0)
-        lib.refb_lanczos(sb.sys_len, sb.env_len, *_m(sb.sys), *_m(sb.env), *_bs(sb.sys_bs), *_bs(sb.env_bs),
-                         int(sb.wsec.shape[0]), np.ascontiguousarray(sb.wsec), x, C.byref(e0))
+        with _quiet():
+            lib.refb_lanczos(sb.sys_len, sb.env_len, *_m(sb.sys), *_m(sb.env), *_bs(sb.sys_bs), *_bs(sb.env_bs),
+                             int(sb.wsec.shape[0]), np.ascontiguousarray(sb.wsec), x, C.byref(e0))
     return e0.value, x
 
 
@@ -206,7 +229,8 @@ def ref_w6js(*j) -> float:
     with _lock:
         lib = ref_lib()
         if not _ref_ready:
-            lib.refb_init(6, 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1e-10)
+            with _quiet():
+                lib.refb_init(6, 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1e-10)
             _ref_ready = True
         return lib.refb_w6js(*[int(v) for v in j])
 
@@ -272,9 +296,10 @@ def dropin_lanczos(model, sb, x0, lanzero=1e-10):
         _ref_init(model, lanzero)
         x = np.ascontiguousarray(x0, dtype=np.float64).copy()
         e0 = C.c_double(0.0)
-        dropin_lib().dropin_lanczos(sb.sys_len, sb.env_len, *_m(sb.sys), *_m(sb.env), *_bs(sb.sys_bs),
-                                    *_bs(sb.env_bs), int(sb.wsec.shape[0]), np.ascontiguousarray(sb.wsec), x,
-                                    C.byref(e0))
+        with _quiet():
+            dropin_lib().dropin_lanczos(sb.sys_len, sb.env_len, *_m(sb.sys), *_m(sb.env), *_bs(sb.sys_bs),
+                                        *_bs(sb.env_bs), int(sb.wsec.shape[0]), np.ascontiguousarray(sb.wsec), x,
+                                        C.byref(e0))
     return e0.value, x
 
 
@@ -433,8 +458,9 @@ def ref_truncation(den, kept_min, kept_max, trun_err, workdir, system_block=True
         cwd = os.getcwd()
         os.chdir(workdir)
         try:
-            lib.refb_truncation(*_one(den), kept_min, kept_max, trun_err, int(system_block), C.byref(ns),
-                                C.byref(ent))
+            with _quiet():
+                lib.refb_truncation(*_one(den), kept_min, kept_max, trun_err, int(system_block), C.byref(ns),
+                                    C.byref(ent))
         finally:
             os.chdir(cwd)
         return _take_ref(), ns.value, ent.value

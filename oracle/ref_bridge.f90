@@ -623,3 +623,13 @@ integer(c_int) function refb_quicksort(n, arr, idx) bind(C, name="refb_quicksort
   if (n > 0) call QuickSort_Double(arr, idx, 1, n, n)
   refb_quicksort = 0
 end function refb_quicksort
+
+! Flushes the reference's standard output (unit 6) so a caller can route what
+! the reference prints (FACTRL's last factorial, the Lanczos log lines) away
+! from its own standard output.
+integer(c_int) function refb_flush() bind(C, name="refb_flush")
+  use iso_c_binding
+  implicit none
+  flush(6)
+  refb_flush = 0
+end function refb_flush
