@@ -513,6 +513,7 @@ bool to_shape(const tjsq_shape* in, tjsq::Shape& out) {
   out.k_split = in->k_split;
   out.item_chunks = in->item_chunks;
   out.xcds = in->xcds;
+  out.merge_right = in->merge_right != 0;
   return true;
 }
 

@@ -1173,8 +1173,53 @@ int compile_program(HostPlan& P, const Shape& shape) {
   std::unordered_map<int64_t, int> agroup;  // stage-1 locality: the A operand
   double fl_ref = 0, fl_exec = 0;
   std::vector<std::pair<int64_t, int64_t>> opref;  // (off, elems) of referenced operator sub-blocks
+  // Two-sided contributions into one output block that share the right
+  // operand B are factored: sum_j c_j op(A_j) op(X_j) op(B) = T' op(B), with
+  // T' formed in stage 1 as one multi-segment product.  lead[q] >= 0 marks
+  // a group member; only the group's first member emits a stage-2 segment.
+  std::vector<int> lead(P.contrib.size(), -1);
+  if (shape.merge_right) {
+    std::map<std::tuple<int, int64_t, int, int, int, int>, std::vector<int>> groups;
+    for (size_t q = 0; q < P.contrib.size(); ++q) {
+      const Contrib& c = P.contrib[q];
+      if (c.kind == C_BOTH) groups[std::make_tuple(c.out_block, c.b_off, (int)c.b_t, c.b_ld, c.K2, c.N)].push_back((int)q);
+    }
+    for (auto& kv : groups) {
+      const std::vector<int>& g = kv.second;
+      if (g.size() < 2) continue;
+      const int64_t toff = P.scratch_elems;
+      const Contrib& c0 = P.contrib[g[0]];
+      P.scratch_elems += (int64_t)c0.M * c0.K2;
+      const int sb = (int)P.seg1.size();
+      for (int q : g) {
+        const Contrib& c = P.contrib[q];
+        lead[q] = g[0];
+        Seg s{};
+        s.l_off = c.a_off; s.l_ld = c.a_ld; s.l_t = c.a_t; s.l_space = SP_OPS;
+        s.r_off = c.x_off; s.r_ld = c.x_ld; s.r_t = c.x_t; s.r_space = SP_X;
+        s.K = c.K1; s.c = c.c;
+        P.seg1.push_back(s);
+        fl_exec += 2.0 * c.M * c.K1 * c.K2;
+      }
+      const int se = (int)P.seg1.size();
+      t_of[g[0]] = toff;
+      const int tile = edge(c0.M, c0.K2);
+      auto& dst = tile == 64 ? t1big : t1small;
+      const int ga = agroup.emplace(c0.a_off, (int)agroup.size()).first->second;
+      for (int m0 = 0; m0 < c0.M; m0 += tile)
+        for (int n0 = 0; n0 < c0.K2; n0 += tile)
+          dst.push_back({toff, c0.M, m0, n0, std::min(tile, c0.M - m0), std::min(tile, c0.K2 - n0), sb, se, SP_T,
+                         -1, ga});
+    }
+  }
   for (size_t q = 0; q < P.contrib.size(); ++q) {
     const Contrib& c = P.contrib[q];
+    if (c.kind == C_BOTH && lead[q] >= 0) {
+      fl_ref += 2.0 * c.M * c.K1 * c.K2 + 2.0 * c.M * c.K2 * c.N;
+      opref.emplace_back(c.a_off, (int64_t)c.M * c.K1);
+      opref.emplace_back(c.b_off, (int64_t)c.K2 * c.N);
+      continue;
+    }
     if (c.kind == C_BOTH) {
       fl_ref += 2.0 * c.M * c.K1 * c.K2 + 2.0 * c.M * c.K2 * c.N;
       auto key = std::make_tuple(c.a_off, (int)c.a_t + 2 * (int)c.x_t, c.x_off, c.M, c.K2);
@@ -1222,14 +1267,25 @@ int compile_program(HostPlan& P, const Shape& shape) {
   for (size_t q = 0; q < order.size(); ++q) order[q] = (int)q;
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return P.contrib[a].out_block < P.contrib[b].out_block; });
+  // a factored group emits one segment (its first member's, coefficient 1:
+  // the members' coefficients are inside T')
+  auto emits = [&](int q) { return lead[q] < 0 || lead[q] == q; };
   std::vector<int> bstart(P.out_blocks.size() + 1, 0);
-  for (const Contrib& c : P.contrib) bstart[c.out_block + 1]++;
+  for (size_t q = 0; q < P.contrib.size(); ++q)
+    if (emits((int)q)) bstart[P.contrib[q].out_block + 1]++;
   for (size_t b = 0; b < P.out_blocks.size(); ++b) bstart[b + 1] += bstart[b];
-  P.seg2.resize(P.contrib.size());
+  {
+    std::vector<int> kept;
+    kept.reserve(order.size());
+    for (int q : order)
+      if (emits(q)) kept.push_back(q);
+    order.swap(kept);
+  }
+  P.seg2.resize(order.size());
   for (size_t r = 0; r < order.size(); ++r) {
     const Contrib& c = P.contrib[order[r]];
     Seg s{};
-    s.c = c.c;
+    s.c = lead[order[r]] >= 0 ? 1.0 : c.c;
     switch (c.kind) {
       case C_AXPY:
         s.l_off = c.x_off; s.l_ld = c.x_ld; s.l_space = SP_X; s.r_space = SP_IDENT; s.K = 0;

@@ -147,6 +147,7 @@ struct Shape {
   int item_chunks = 32;
   int xcds = 1;             // > 1: items of a locality group go to one XCD (round-robin dispatch);
                             // the plain order measured faster on MI355X, so it is the default
+  int merge_right = 0;      // factor two-sided terms of one output block that share op(B)
 };
 
 // Compiles P.contrib and P.out_blocks into seg1/tile1/seg2/tile2/rtile and

@@ -187,14 +187,17 @@ int tjsq_plan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tj
  * with a side above `small` use big x big work items (big = 64 or 32;
  * small = 32); stage-2 segments are cut into K pieces of at most k_split and
  * grouped into work items of at most item_chunks K chunks of 32; work items
- * are binned over `xcds` XCDs (1 = plain order).  NULL selects {64, 32, 256,
- * 32, 1}. */
+ * are binned over `xcds` XCDs (1 = plain order).  merge_right = 1 factors
+ * the two-sided terms of one output block that share their right operand B:
+ * sum_j c_j A_j X_j B = (sum_j c_j A_j X_j) B, the sum formed in stage 1.
+ * NULL selects {64, 32, 256, 32, 1, 0}. */
 typedef struct tjsq_shape {
   int big;
   int small;
   int k_split;
   int item_chunks;
   int xcds;
+  int merge_right;
 } tjsq_shape;
 
 int tjsq_plan_create_ex(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys,

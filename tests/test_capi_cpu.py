@@ -98,7 +98,8 @@ def test_plan_rejects_lattice_too_small():
 # reproduce the oracle (blocks large enough for 64-tiles, K splits and
 # partial tiles)
 TUNE_SHAPES = [None, {"big": 32}, {"k_split": 32, "item_chunks": 1}, {"k_split": 64, "item_chunks": 4},
-               {"k_split": 1024, "item_chunks": 64}, {"xcds": 1}, {"big": 32, "k_split": 128, "xcds": 4}]
+               {"k_split": 1024, "item_chunks": 64}, {"xcds": 8}, {"big": 32, "k_split": 128, "xcds": 4},
+               {"merge_right": 1}, {"merge_right": 1, "big": 32, "k_split": 64, "item_chunks": 2, "xcds": 8}]
 
 
 @pytest.mark.parametrize("shape", TUNE_SHAPES)

@@ -129,3 +129,15 @@ def test_lanczos_graph_replay_is_bitwise_the_launched_solve(prec):
         assert e0 == outs[0][0] and it == outs[0][1]
         assert torch.equal(psi, outs[0][2])
     assert outs[0][1] >= 3
+
+
+@pytest.mark.parametrize("shape", [None, {"merge_right": 1}, {"merge_right": 1, "big": 32, "xcds": 8}])
+def test_apply_work_item_shapes_match_oracle(shape):
+    m = model()
+    sb = small_superblock(m, 11, 11, seed=23, msys=400, menv=380, median=40, nsec=8)
+    x = synth.random_vector(sb, 4)
+    p = capi.Plan(m, sb, shape=shape)
+    ref = oracle.model_wave(m, sb, x)
+    assert rel(p.apply(_cuda(x)).cpu().numpy(), ref) < 1e-13
+    y = p.apply(_cuda(x, torch.float32), precision="bf16x3").double().cpu().numpy()
+    assert rel(y, ref) < 2e-5
