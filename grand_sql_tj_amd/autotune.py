@@ -17,14 +17,14 @@ import numpy as np
 
 # the default shape first: a candidate replaces it only when faster
 CANDIDATES = [
-    {"big": 64, "k_split": 256, "item_chunks": 32, "xcds": 1},
-    {"big": 64, "k_split": 512, "item_chunks": 32, "xcds": 1},
-    {"big": 64, "k_split": 128, "item_chunks": 16, "xcds": 1},
-    {"big": 64, "k_split": 1024, "item_chunks": 64, "xcds": 1},
-    {"big": 64, "k_split": 256, "item_chunks": 8, "xcds": 1},
-    {"big": 64, "k_split": 256, "item_chunks": 32, "xcds": 8},
-    {"big": 32, "k_split": 256, "item_chunks": 32, "xcds": 1},
-    {"big": 32, "k_split": 512, "item_chunks": 64, "xcds": 1},
+    {"big": 64, "k_split": 256, "item_chunks": 32, "xcds": 1, "merge_right": 1},
+    {"big": 64, "k_split": 512, "item_chunks": 32, "xcds": 1, "merge_right": 1},
+    {"big": 64, "k_split": 128, "item_chunks": 16, "xcds": 1, "merge_right": 1},
+    {"big": 64, "k_split": 1024, "item_chunks": 64, "xcds": 1, "merge_right": 1},
+    {"big": 64, "k_split": 256, "item_chunks": 32, "xcds": 8, "merge_right": 1},
+    {"big": 64, "k_split": 256, "item_chunks": 32, "xcds": 1, "merge_right": 0},
+    {"big": 32, "k_split": 256, "item_chunks": 32, "xcds": 1, "merge_right": 1},
+    {"big": 32, "k_split": 512, "item_chunks": 64, "xcds": 1, "merge_right": 1},
 ]
 
 

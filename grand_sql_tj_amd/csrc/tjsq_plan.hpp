@@ -147,7 +147,8 @@ struct Shape {
   int item_chunks = 32;
   int xcds = 1;             // > 1: items of a locality group go to one XCD (round-robin dispatch);
                             // the plain order measured faster on MI355X, so it is the default
-  int merge_right = 0;      // factor two-sided terms of one output block that share op(B)
+  int merge_right = 1;      // factor two-sided terms of one output block that share op(B)
+                            // (6-8 % faster applies at the m = 8000 workload)
 };
 
 // Compiles P.contrib and P.out_blocks into seg1/tile1/seg2/tile2/rtile and

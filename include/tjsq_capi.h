@@ -190,7 +190,7 @@ int tjsq_plan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tj
  * are binned over `xcds` XCDs (1 = plain order).  merge_right = 1 factors
  * the two-sided terms of one output block that share their right operand B:
  * sum_j c_j A_j X_j B = (sum_j c_j A_j X_j) B, the sum formed in stage 1.
- * NULL selects {64, 32, 256, 32, 1, 0}. */
+ * NULL selects {64, 32, 256, 32, 1, 1}. */
 typedef struct tjsq_shape {
   int big;
   int small;
