@@ -34,6 +34,39 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15 / 3.0}
 HBM_PEAK = 8.0e12
+# kernel kinds of Plan.profile_read, and the kernel symbol of each
+KERNEL_KINDS = ("stage1_64", "stage1_32", "stage2_64", "stage2_32", "reduce")
+KERNEL_SYMBOL = {"stage1_64": "k_tiles<{mode}, 64, 1>", "stage1_32": "k_tiles<{mode}, 32, 1>",
+                 "stage2_64": "k_tiles<{mode}, 64, 2>", "stage2_32": "k_tiles<{mode}, 32, 2>", "reduce": "k_reduce"}
+MODE_ID = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "heff_apply_m8000_traffic.json")
+
+
+def kernel_flops(hp):
+    """Useful flops per launch of each kernel kind: 2 mt nt K summed over the
+    GEMM segments of its work items."""
+    out = {}
+    for stage, tiles, segs, n64 in ((1, hp.tile1, hp.seg1, hp.n64_1), (2, hp.tile2, hp.seg2, hp.n64_2)):
+        kk = np.concatenate([[0.0], np.cumsum(segs["K"].astype(np.float64) * (segs["r_space"] != 3))])
+        per = 2.0 * tiles["mt"].astype(np.float64) * tiles["nt"] * (kk[tiles["seg_end"]] - kk[tiles["seg_begin"]])
+        out[f"stage{stage}_64"] = float(per[:n64].sum())
+        out[f"stage{stage}_32"] = float(per[n64:].sum())
+    out["reduce"] = 0.0
+    return out
+
+
+def measured_traffic(kernel, precision, dim):
+    """HBM bytes per launch of `kernel` at this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE measurement (calibrated by a copy of
+    known size); None when that measurement is for another workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("D") != dim:
+        return None
+    return t.get("bytes_per_launch", {}).get(precision, {}).get(kernel)
 
 
 def workload(m_mult: int, seed: int):
@@ -221,7 +254,7 @@ def batch_bench(args):
                    "D_total": b.dim, "contributions": stb["n_contrib"], "precision": prec},
         "sequential_ms": ms_seq, "batch_ms": ms_batch, "batch_graph_ms": ms_graph,
         "speedup_vs_sequential": ms_seq / best, "bit_identical_to_members": bool(same),
-        "stage_ms": {"stage1": ms_k[0] / max(1, n_k[0]), "stage2": ms_k[1] / max(1, n_k[1])},
+        "kernel_ms": {k: float(ms_k[i] / max(1, n_k[i])) for i, k in enumerate(KERNEL_KINDS)},
         "roofline": {"bound": "mfma", "achieved": fl / (best * 1e-3) / 1e12, "peak": PEAK[prec] / 1e12,
                      "unit": "TFLOP/s", "frac": fl / (best * 1e-3) / PEAK[prec], "traffic": None},
         "autotuned_shapes": tuned, "synthesis_s": synth_s,
@@ -484,10 +517,8 @@ def main():
     torch.cuda.synchronize()
     ms_k, n_k = plan.profile_read()
     plan.profile(False)
-    avg1 = ms_k[0] / max(1, n_k[0]) * 1e-3
-    avg2 = ms_k[1] / max(1, n_k[1]) * 1e-3
-    dom = 2 if avg2 >= avg1 else 1
-    t_dom = avg2 if dom == 2 else avg1
+    kavg = {k: float(ms_k[i] / max(1, n_k[i])) for i, k in enumerate(KERNEL_KINDS)}   # ms per launch
+    dom = max(KERNEL_KINDS[:4], key=lambda k: kavg[k])
 
     value = 1.0 / float(np.mean(latmp))
     v64 = 1.0 / float(np.mean(lat64))
@@ -511,21 +542,22 @@ def main():
         "refinements": rmp["refinements"], "energy_err_est": rmp["energy_err_est"],
         "p50_ms": 1e3 * float(np.percentile(latmp, 50)), "p99_ms": 1e3 * float(np.percentile(latmp, 99)),
         "p50_ms_fp64": 1e3 * float(np.percentile(lat64, 50)), "p99_ms_fp64": 1e3 * float(np.percentile(lat64, 99)),
-        "apply_stage_ms": {"stage1": avg1 * 1e3, "stage2": avg2 * 1e3},
+        "apply_kernel_ms": kavg,
+        "apply_ms": float(sum(ms_k[:5]) / max(1, n_k[2])),
+        "apply_tflops_reference_order": st["flops_ref"] / (float(sum(ms_k[:5]) / max(1, n_k[2])) * 1e-3) / 1e12,
     }
-    # roofline of the dominant stage: algorithmic flops of that stage per launch
-    hp = capi.HostPlan(model, sb)
-    # stage-1 flops: distinct A.X products; stage-2 flops: the rest of flops_exec
-    k1 = hp.seg1["K"][hp.tile1["seg_begin"]].astype(np.float64)
-    fl1 = float(2.0 * np.sum(hp.tile1["mt"].astype(np.float64) * hp.tile1["nt"] * k1))
-    fl2 = st["flops_exec"] - fl1
-    fl_dom = fl2 if dom == 2 else fl1
-    achieved = fl_dom / t_dom
+    # roofline of the dominant kernel: the useful flops of its own task table
+    # (sum of 2 mt nt K over its GEMM segments; no MFMA padding, no
+    # recomputation) over its average launch time (events on the launching
+    # stream, isolated applies of the mixed precision)
+    fl = kernel_flops(capi.HostPlan(model, sb))
+    achieved = fl[dom] / (kavg[dom] * 1e-3)
     peak = PEAK[args.precision]
-    line["roofline"] = {"bound": "mfma", "kernel": "k_tiles_" + ("bf16x3" if args.precision == "bf16x3" else "f32")
-                        + f" stage {dom}", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                        "frac": achieved / peak, "traffic": None,
-                        "algorithmic_flops_per_launch": fl_dom, "avg_launch_ms": t_dom * 1e3}
+    kname = KERNEL_SYMBOL[dom].format(mode=MODE_ID[args.precision])
+    line["roofline"] = {"bound": "mfma", "kernel": kname, "achieved": achieved / 1e12, "peak": peak / 1e12,
+                        "unit": "TFLOP/s", "frac": achieved / peak,
+                        "traffic": measured_traffic(kname, args.precision, sb.dim),
+                        "algorithmic_flops_per_launch": fl[dom], "avg_launch_ms": kavg[dom]}
     line["cpu_baseline"] = cpu
     if tuned is not None:
         line["work_item_shape"] = tuned["shape"]

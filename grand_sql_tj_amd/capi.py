@@ -402,8 +402,10 @@ class Plan:
         check(lib().tjsq_plan_profile(self.h, 1 if on else 0), "tjsq_plan_profile")
 
     def profile_read(self):
-        ms = np.zeros(2)
-        n = np.zeros(2, np.int64)
+        """(ms, launches) per kernel kind since the last read: stage-1 64x64,
+        stage-1 32x32, stage-2 64x64, stage-2 32x32, reduction."""
+        ms = np.zeros(PROFILE_KINDS)
+        n = np.zeros(PROFILE_KINDS, np.int64)
         check(lib().tjsq_plan_profile_read(self.h, ms.ctypes.data, n.ctypes.data), "tjsq_plan_profile_read")
         return ms, n
 
@@ -416,6 +418,10 @@ class Plan:
 # ---------------------------------------------------------------------------
 # density matrix, truncation and block update (tjsq_program)
 # ---------------------------------------------------------------------------
+# kernel kinds of profile_read: stage-1 64x64 items, stage-1 32x32, stage-2
+# 64x64, stage-2 32x32, partial-tile reduction
+PROFILE_KINDS = 5
+
 PROGRAM_SIZE_KEYS = ("nops", "nsub", "ndata", "x_elems", "ops_elems", "n_seg1", "n_tile1", "n_seg2", "n_tile2",
                      "n_rtile", "n_slots", "scratch_elems", "n64_1", "n64_2")
 
@@ -501,8 +507,8 @@ class Program:
         check(lib().tjsq_program_profile(self._h, int(enable)), "tjsq_program_profile")
 
     def profile_read(self):
-        ms = np.zeros(2, np.float64)
-        n = np.zeros(2, np.int64)
+        ms = np.zeros(PROFILE_KINDS, np.float64)
+        n = np.zeros(PROFILE_KINDS, np.int64)
         check(lib().tjsq_program_profile_read(self._h, ms.ctypes.data, n.ctypes.data), "tjsq_program_profile_read")
         return ms, n
 
@@ -620,8 +626,8 @@ class Batch:
         check(lib().tjsq_batch_profile(self._h, int(enable)), "tjsq_batch_profile")
 
     def profile_read(self):
-        ms = np.zeros(2, np.float64)
-        n = np.zeros(2, np.int64)
+        ms = np.zeros(PROFILE_KINDS, np.float64)
+        n = np.zeros(PROFILE_KINDS, np.int64)
         check(lib().tjsq_batch_profile_read(self._h, ms.ctypes.data, n.ctypes.data), "tjsq_batch_profile_read")
         return ms, n
 

@@ -145,7 +145,9 @@ __device__ __forceinline__ void fetch_chunk(const Spaces<T>& S, const Seg& s, co
 // latency-bound, so occupancy is asked for: at least 5 (64-tiles) / 4
 // (32-tiles) waves per SIMD.  This lifts the FP32 and BF16x3 items from 3-4
 // to 4-5 waves (BF16x3 applies 9 % faster on MI355X); FP64 keeps 3 / 2.
-template <int MODE, int TSZ>
+// STAGE (1 or 2) does not change the code: it gives the two stages distinct
+// kernel symbols, so a trace reports them apart.
+template <int MODE, int TSZ, int STAGE>
 __global__ __launch_bounds__((TileCfg<MODE, TSZ>::NT), (TSZ == 64 ? 5 : 4)) void k_tiles(const Tile* __restrict__ tiles,
                                                             const Seg* __restrict__ segs,
                                                             Spaces<typename KT<MODE>::T> S,
@@ -555,6 +557,46 @@ int upload_tables(tjsq_plan* p, bool with64, bool with32, hipStream_t st) {
   return TJSQ_OK;
 }
 
+// profiling kinds: 0 stage-1 64x64 items, 1 stage-1 32x32, 2 stage-2 64x64,
+// 3 stage-2 32x32, 4 the partial-tile reduction
+constexpr int NPROF = 5;
+
+template <int MODE, int TSZ, int STAGE, typename T>
+void launch_items(int n, const Tile* tiles, const Seg* segs, const Spaces<T>& S, T* part, hipStream_t st) {
+  hipLaunchKernelGGL((k_tiles<MODE, TSZ, STAGE>), dim3(n), dim3(TileCfg<MODE, TSZ>::NT), 0, st, tiles, segs, S, part,
+                     n);
+}
+
+template <int MODE, int STAGE, typename T>
+int launch_stage(const tjsq_plan* p, int n, int n64, const Tile* tiles, const Seg* segs, const Spaces<T>& S, T* part,
+                 hipStream_t st) {
+  auto timed = [&](int kind, auto&& fn) -> int {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (p->profiling) {
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      HIPCHK(hipEventRecord(e0, st));
+    }
+    fn();
+    HIPCHK(hipGetLastError());
+    if (p->profiling) {
+      HIPCHK(hipEventRecord(e1, st));
+      const_cast<tjsq_plan*>(p)->prof.push_back({kind, {e0, e1}});
+    }
+    return TJSQ_OK;
+  };
+  // 64x64 work items first in the table, then 32x32 ones
+  const int k0 = STAGE == 1 ? 0 : 2;
+  int rc = TJSQ_OK;
+  if (n64) rc = timed(k0, [&] { launch_items<MODE, 64, STAGE>(n64, tiles, segs, S, part, st); });
+  if (!rc && n > n64) rc = timed(k0 + 1, [&] { launch_items<MODE, 32, STAGE>(n - n64, tiles + n64, segs, S, part, st); });
+  if (!rc && STAGE == 2 && !p->hp.rtile.empty()) {
+    const int nr = (int)p->hp.rtile.size();
+    rc = timed(4, [&] { hipLaunchKernelGGL(k_reduce<T>, dim3(nr), dim3(256), 0, st, p->rtile, part, S.y, nr); });
+  }
+  return rc;
+}
+
 template <typename T>
 int launch_tiles(const tjsq_plan* p, int stage, const T* ops, const T* x, T* t, T* y, int precision,
                  hipStream_t st) {
@@ -562,45 +604,18 @@ int launch_tiles(const tjsq_plan* p, int stage, const T* ops, const T* x, T* t, 
   if (n == 0) return TJSQ_OK;
   const Tile* tiles = stage == 1 ? p->tile1 : p->tile2;
   const Seg* segs = stage == 1 ? p->seg1 : p->seg2;
-  Spaces<T> S{ops, x, t, y};
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (p->profiling) {
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, st));
-  }
   const int n64 = stage == 1 ? p->hp.n64_1 : p->hp.n64_2;
-  // 64x64 work items first in the table, then 32x32 ones
+  Spaces<T> S{ops, x, t, y};
   if constexpr (sizeof(T) == 8) {
-    if (n64) hipLaunchKernelGGL((k_tiles<MODE_F64, 64>), dim3(n64), dim3(256), 0, st, tiles, segs, S, p->part64, n64);
-    if (n > n64)
-      hipLaunchKernelGGL((k_tiles<MODE_F64, 32>), dim3(n - n64), dim3(64), 0, st, tiles + n64, segs, S, p->part64,
-                         n - n64);
+    return stage == 1 ? launch_stage<MODE_F64, 1>(p, n, n64, tiles, segs, S, p->part64, st)
+                      : launch_stage<MODE_F64, 2>(p, n, n64, tiles, segs, S, p->part64, st);
   } else if (precision == TJSQ_BF16X3) {
-    if (n64) hipLaunchKernelGGL((k_tiles<MODE_BF16X3, 64>), dim3(n64), dim3(256), 0, st, tiles, segs, S, p->part32, n64);
-    if (n > n64)
-      hipLaunchKernelGGL((k_tiles<MODE_BF16X3, 32>), dim3(n - n64), dim3(64), 0, st, tiles + n64, segs, S, p->part32,
-                         n - n64);
+    return stage == 1 ? launch_stage<MODE_BF16X3, 1>(p, n, n64, tiles, segs, S, p->part32, st)
+                      : launch_stage<MODE_BF16X3, 2>(p, n, n64, tiles, segs, S, p->part32, st);
   } else {
-    if (n64) hipLaunchKernelGGL((k_tiles<MODE_F32, 64>), dim3(n64), dim3(256), 0, st, tiles, segs, S, p->part32, n64);
-    if (n > n64)
-      hipLaunchKernelGGL((k_tiles<MODE_F32, 32>), dim3(n - n64), dim3(64), 0, st, tiles + n64, segs, S, p->part32,
-                         n - n64);
+    return stage == 1 ? launch_stage<MODE_F32, 1>(p, n, n64, tiles, segs, S, p->part32, st)
+                      : launch_stage<MODE_F32, 2>(p, n, n64, tiles, segs, S, p->part32, st);
   }
-  HIPCHK(hipGetLastError());
-  if (stage == 2 && !p->hp.rtile.empty()) {
-    const int nr = (int)p->hp.rtile.size();
-    if constexpr (sizeof(T) == 8)
-      hipLaunchKernelGGL(k_reduce<double>, dim3(nr), dim3(256), 0, st, p->rtile, p->part64, y, nr);
-    else
-      hipLaunchKernelGGL(k_reduce<float>, dim3(nr), dim3(256), 0, st, p->rtile, p->part32, y, nr);
-    HIPCHK(hipGetLastError());
-  }
-  if (p->profiling) {
-    HIPCHK(hipEventRecord(e1, st));
-    const_cast<tjsq_plan*>(p)->prof.push_back({stage, {e0, e1}});
-  }
-  return TJSQ_OK;
 }
 
 // tridiagonal eigenproblem (implicit QL with Wilkinson shifts, Golub & Van
@@ -950,14 +965,16 @@ int tjsq_plan_profile(tjsq_plan* p, int enable) {
 
 int tjsq_plan_profile_read(tjsq_plan* p, double* ms, int64_t* launches) {
   if (!p || !ms || !launches) return fail(TJSQ_ERR_ARG, "tjsq_plan_profile_read: null argument");
-  ms[0] = ms[1] = 0.0;
-  launches[0] = launches[1] = 0;
+  for (int k = 0; k < NPROF; ++k) {
+    ms[k] = 0.0;
+    launches[k] = 0;
+  }
   for (auto& r : p->prof) {
     HIPCHK(hipEventSynchronize(r.second.second));
     float t = 0.f;
     HIPCHK(hipEventElapsedTime(&t, r.second.first, r.second.second));
-    ms[r.first - 1] += t;
-    launches[r.first - 1] += 1;
+    ms[r.first] += t;
+    launches[r.first] += 1;
     (void)hipEventDestroy(r.second.first);
     (void)hipEventDestroy(r.second.second);
   }

@@ -219,9 +219,12 @@ int tjsq_heff_apply(tjsq_plan* plan, const double* x, double* y, void* stream);
  * TJSQ_BF16X3. */
 int tjsq_heff_apply_f32(tjsq_plan* plan, int precision, const float* x, float* y, void* stream);
 
-/* Per-launch timing of the two apply stages with HIP events recorded on the
+/* Per-launch timing of the apply kernels with HIP events recorded on the
  * launching stream.  profile_read returns the summed milliseconds and launch
- * counts of stage 1 (ms[0]) and stage 2 (ms[1]) since the last read. */
+ * counts since the last read in ms[5] / launches[5]: stage-1 64x64 items,
+ * stage-1 32x32 items, stage-2 64x64 items, stage-2 32x32 items, and the
+ * partial-tile reduction.  The stages are distinct kernel symbols
+ * (k_tiles<mode, edge, stage>), so a rocprofv3 trace reports them apart. */
 int tjsq_plan_profile(tjsq_plan* plan, int enable);
 int tjsq_plan_profile_read(tjsq_plan* plan, double* ms, int64_t* launches);
 
