@@ -262,8 +262,8 @@ def active_sites(model, length) -> int:
 class Shape(C.Structure):
     """Work-item shape of the grouped GEMM (tjsq_shape)."""
     _fields_ = [("big", C.c_int), ("small", C.c_int), ("k_split", C.c_int), ("item_chunks", C.c_int),
-                ("xcds", C.c_int), ("merge_right", C.c_int)]
-    DEFAULT = {"big": 64, "small": 32, "k_split": 256, "item_chunks": 32, "xcds": 1, "merge_right": 1}
+                ("xcds", C.c_int), ("merge_right", C.c_int), ("order", C.c_int)]
+    DEFAULT = {"big": 64, "small": 32, "k_split": 256, "item_chunks": 32, "xcds": 1, "merge_right": 1, "order": 0}
 
     @classmethod
     def of(cls, d):

@@ -55,39 +55,26 @@ int fail(int code, const std::string& m) {
 
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
 typedef float f4 __attribute__((ext_vector_type(4)));
-typedef short s4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 // ---------------------------------------------------------------------------
-// operand access
-// ---------------------------------------------------------------------------
-template <typename T>
-struct Spaces {
-  const T* ops;
-  const T* x;
-  T* t;      // stage-1 scratch
-  T* y;
-};
-
-template <typename T>
-__device__ __forceinline__ const T* space_ptr(const Spaces<T>& S, int sp) {
-  return sp == tjsq::SP_OPS ? S.ops : (sp == tjsq::SP_X ? S.x : (const T*)S.t);
-}
-
-// ---------------------------------------------------------------------------
-// Grouped tile kernel (one template for FP64, FP32 and BF16x3)
-//
-// One 64-lane wavefront owns one 32x32 tile and walks its segments in K
-// chunks of KC (TileCfg).  Chunk c+1's operand values are fetched into registers
-// while the MFMAs of chunk c run from LDS, so a lone wave keeps one chunk of
-// global loads in flight.  Axpy segments (identity right operand) come first
-// in every segment list and are added straight into the accumulators.
+// operand words and spaces
 // ---------------------------------------------------------------------------
 enum { MODE_F64 = 0, MODE_F32 = 1, MODE_BF16X3 = 2 };
 
+// T: vectors, accumulators and outputs.  W: one operand element as the tile
+// kernel reads it; in BF16x3 a 32-bit word with bf16(v) in the high half and
+// bf16(v - bf16(v)) in the low half, split once when the operand is made
+// (operators at plan creation, x per apply, T in stage 1's epilogue), so the
+// tile loop does no conversion work.
 template <int MODE>
 struct KT {
   using T = typename std::conditional<MODE == MODE_F64, double, float>::type;
+  using W = typename std::conditional<MODE == MODE_F64, double,
+                                      typename std::conditional<MODE == MODE_F32, float, uint32_t>::type>::type;
   using Acc = typename std::conditional<MODE == MODE_F64, d4, f4>::type;
 };
 
@@ -97,6 +84,28 @@ __device__ __forceinline__ unsigned short bf16_rne(float f) {
   return (unsigned short)(u >> 16);
 }
 __device__ __forceinline__ float bf16_to_f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
+// v - bf16(v) is exact in FP32, so hi + lo carries v to 2^-16 relative
+__device__ __forceinline__ uint32_t split_word(float v) {
+  const unsigned short h = bf16_rne(v);
+  return ((uint32_t)h << 16) | bf16_rne(v - bf16_to_f(h));
+}
+
+// the spaces a launch reads and writes: operators and x as operand words,
+// x itself (axpy terms add it directly), stage-1 scratch (operand words of
+// T), and the output
+template <int MODE>
+struct Spaces {
+  const typename KT<MODE>::W* ops;
+  const typename KT<MODE>::W* xw;
+  const typename KT<MODE>::T* x;
+  typename KT<MODE>::W* t;
+  typename KT<MODE>::T* y;
+};
+
+template <int MODE>
+__device__ __forceinline__ const typename KT<MODE>::W* space_ptr(const Spaces<MODE>& S, int sp) {
+  return sp == tjsq::SP_OPS ? S.ops : (sp == tjsq::SP_X ? S.xw : (const typename KT<MODE>::W*)S.t);
+}
 
 // row of register r of lane (lq, lr) inside a 16x16 MFMA result block
 template <int MODE>
@@ -104,72 +113,172 @@ __device__ __forceinline__ int acc_row(int lq, int r) {
   return MODE == MODE_F64 ? lq + 4 * r : lq * 4 + r;
 }
 
+// ---------------------------------------------------------------------------
+// Grouped tile kernel (one template for FP64, FP32 and BF16x3)
+//
+// A workgroup owns one TSZ x TSZ output tile (TSZ = 64: four waves, one 32x32
+// quarter each; TSZ = 32: one wave) and walks its segment list in K chunks of
+// KC.  Operand elements come in through raw buffer loads: a lane whose
+// element lies outside the tile or past the segment's K gets an offset above
+// the record count and reads zero, so there are no divergent branches
+// around the loads.  Each lane's offsets are one base plus wave-uniform
+// steps, set up once per segment.  Chunk c+1 is loaded into registers while
+// the MFMAs of chunk c run from LDS.  Both panels sit in LDS as [tile index][k]
+// rows; an MFMA operand is 16 (FP32 / BF16x3) or 32 (FP64) consecutive bytes
+// of a row per lane.  The k order inside an MFMA step is a permutation shared
+// by both operands, so the sum over k is unchanged.
+// ---------------------------------------------------------------------------
 template <int MODE, int TSZ>
 struct TileCfg {
-  static constexpr int NT = TSZ == 64 ? 256 : 64;   // threads: one wave per 32x32 quarter
+  static constexpr int NT = TSZ == 64 ? 256 : 64;    // threads
   // K chunk staged per step.  A chunk of 64 for the 4-wave FP32 / BF16x3
-  // items costs one wave per SIMD of occupancy and measured slower: the
-  // kernel is latency-bound, so the chunk stays at 32 in every mode.
+  // items cost one wave per SIMD of occupancy and measured slower.
   static constexpr int KC = 32;
-  static constexpr int PER = TSZ * KC / NT;          // operand elements per thread per chunk
+  static constexpr int PER = TSZ * KC / NT;          // elements per thread per panel and chunk
+  // LDS row stride in operand words: rows start 16-byte aligned and the
+  // operand reads of 8 consecutive lanes fall in distinct banks
+  static constexpr int HS = MODE == MODE_F64 ? KC + 2 : KC + 4;
+  // waves per SIMD asked of the compiler (the kernel is latency-bound)
+  static constexpr int WAVES = TSZ == 64 ? (MODE == MODE_BF16X3 ? 4 : 5) : 4;
 };
 
-template <int MODE, int TSZ, typename T>
-__device__ __forceinline__ void fetch_chunk(const Spaces<T>& S, const Seg& s, const Tile& tl, int k0, int tid,
-                                            T* rl, T* rr) {
-  constexpr int NT = TileCfg<MODE, TSZ>::NT, PER = TileCfg<MODE, TSZ>::PER, KC = TileCfg<MODE, TSZ>::KC;
-  const T* L = space_ptr(S, s.l_space);
-  const T* R = space_ptr(S, s.r_space);
+constexpr uint32_t OOB = 0x80000000u;   // at or above the record count: reads as zero
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7FFFFFFF, 0x00020000);
+}
+
+template <typename W>
+__device__ __forceinline__ W buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(W) == 8)
+    return __builtin_bit_cast(W, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+  else
+    return __builtin_bit_cast(W, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+
+// One operand panel of a segment as the threads load it: TSZ tile rows (or
+// columns) by KC values of k per chunk.  A tile-major operand (contiguous
+// along the tile) gives each lane a fixed tile index and k = tid / TSZ +
+// q NT / TSZ; a k-major one a fixed k = tid % KC and tile index tid / KC +
+// q NT / KC.
+template <int MODE, int TSZ>
+struct Side {
+  __amdgpu_buffer_rsrc_t r;
+  uint32_t lane;   // byte offset of the lane's first element at k0 = 0
+  uint32_t qs;     // byte step between the lane's elements
+  uint32_t ks;     // byte step per K chunk
+  int fix;         // the lane's tile index (tile-major) or k (k-major)
+  int lim;         // tile extent
+  bool kmaj;
+};
+
+template <int MODE, int TSZ>
+__device__ __forceinline__ Side<MODE, TSZ> make_side(const typename KT<MODE>::W* base, int64_t off, int ld,
+                                                     bool kmaj, int t0, int lim, int tid) {
+  using C = TileCfg<MODE, TSZ>;
+  constexpr uint32_t WB = sizeof(typename KT<MODE>::W);
+  Side<MODE, TSZ> s;
+  s.r = make_rsrc(base + off);
+  s.lim = lim;
+  s.kmaj = kmaj;
+  if (!kmaj) {
+    const int a = tid % TSZ, b = tid / TSZ;
+    s.fix = a;
+    s.lane = (uint32_t)((t0 + a) + b * ld) * WB;
+    s.qs = (uint32_t)((C::NT / TSZ) * ld) * WB;
+    s.ks = (uint32_t)(C::KC * ld) * WB;
+  } else {
+    const int b = tid % C::KC, a = tid / C::KC;
+    s.fix = b;
+    s.lane = (uint32_t)(b + (t0 + a) * ld) * WB;
+    s.qs = (uint32_t)((C::NT / C::KC) * ld) * WB;
+    s.ks = (uint32_t)C::KC * WB;
+  }
+  return s;
+}
+
+// the lane's elements of the chunk starting kv values before the segment's
+// end (kv = K - k0), kofs = chunk index * ks
+template <int MODE, int TSZ>
+__device__ __forceinline__ void load_side(const Side<MODE, TSZ>& s, int kv, uint32_t kofs, int tid,
+                                          typename KT<MODE>::W* v) {
+  using C = TileCfg<MODE, TSZ>;
+  using W = typename KT<MODE>::W;
+  if (!s.kmaj) {
+    // 64-tiles: b0 = tid / 64 is the wave index (scalar)
+    const int b0 = TSZ == 64 ? __builtin_amdgcn_readfirstlane(tid / TSZ) : tid / TSZ;
+    const bool oka = s.fix < s.lim;
 #pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    const int e = tid + NT * q;
-    int i, k;
-    if (!s.l_t) { i = e % TSZ; k = e / TSZ; } else { k = e % KC; i = e / KC; }
-    T v = T(0);
-    if (i < tl.mt && k0 + k < s.K)
-      v = s.l_t ? L[s.l_off + (k0 + k) + (int64_t)(tl.m0 + i) * s.l_ld]
-                : L[s.l_off + (tl.m0 + i) + (int64_t)(k0 + k) * s.l_ld];
-    rl[q] = v;
-    int j;
-    if (!s.r_t) { k = e % KC; j = e / KC; } else { j = e % TSZ; k = e / TSZ; }
-    T w = T(0);
-    if (j < tl.nt && k0 + k < s.K)
-      w = s.r_t ? R[s.r_off + (tl.n0 + j) + (int64_t)(k0 + k) * s.r_ld]
-                : R[s.r_off + (k0 + k) + (int64_t)(tl.n0 + j) * s.r_ld];
-    rr[q] = w;
+    for (int q = 0; q < C::PER; ++q) {
+      const bool ok = oka && b0 + q * (C::NT / TSZ) < kv;
+      v[q] = buf_load<W>(s.r, ok ? s.lane + kofs + (uint32_t)q * s.qs : OOB);
+    }
+  } else {
+    const int a0 = tid / C::KC;
+    const bool okk = s.fix < kv;
+#pragma unroll
+    for (int q = 0; q < C::PER; ++q) {
+      const bool ok = okk && a0 + q * (C::NT / C::KC) < s.lim;
+      v[q] = buf_load<W>(s.r, ok ? s.lane + kofs + (uint32_t)q * s.qs : OOB);
+    }
   }
 }
 
-// TSZ x TSZ output tile per workgroup; each wave computes a 32x32 quarter
-// (TSZ = 64: 4 waves share the LDS-staged operand panels).  The kernel is
-// latency-bound, so occupancy is asked for: at least 5 (64-tiles) / 4
-// (32-tiles) waves per SIMD.  This lifts the FP32 and BF16x3 items from 3-4
-// to 4-5 waves (BF16x3 applies 9 % faster on MI355X); FP64 keeps 3 / 2.
+// the loaded elements into the [tile index][k] LDS panel (FP32 / FP64 left
+// operands scaled by the segment coefficient on the way)
+template <int MODE, int TSZ, bool SCALE>
+__device__ __forceinline__ void store_side(typename KT<MODE>::W* lds, bool kmaj, int tid,
+                                           const typename KT<MODE>::W* v, typename KT<MODE>::T c) {
+  using C = TileCfg<MODE, TSZ>;
+  auto val = [&](int q) {
+    if constexpr (SCALE && MODE != MODE_BF16X3) return v[q] * c;
+    else return v[q];
+  };
+  if (!kmaj) {
+    const int a = tid % TSZ, b0 = tid / TSZ;
+#pragma unroll
+    for (int q = 0; q < C::PER; ++q) lds[a * C::HS + b0 + q * (C::NT / TSZ)] = val(q);
+  } else {
+    const int b = tid % C::KC, a0 = tid / C::KC;
+#pragma unroll
+    for (int q = 0; q < C::PER; ++q) lds[(a0 + q * (C::NT / C::KC)) * C::HS + b] = val(q);
+  }
+}
+
+// eight operand words (consecutive k) -> the hi and lo bf16 operands
+__device__ __forceinline__ void split_pairs(u4 x0, u4 x1, bf8& hi, bf8& lo) {
+  constexpr unsigned HI = 0x07060302u, LO = 0x05040100u;
+  u4 h, l;
+  h.x = __builtin_amdgcn_perm(x0.y, x0.x, HI); l.x = __builtin_amdgcn_perm(x0.y, x0.x, LO);
+  h.y = __builtin_amdgcn_perm(x0.w, x0.z, HI); l.y = __builtin_amdgcn_perm(x0.w, x0.z, LO);
+  h.z = __builtin_amdgcn_perm(x1.y, x1.x, HI); l.z = __builtin_amdgcn_perm(x1.y, x1.x, LO);
+  h.w = __builtin_amdgcn_perm(x1.w, x1.z, HI); l.w = __builtin_amdgcn_perm(x1.w, x1.z, LO);
+  hi = __builtin_bit_cast(bf8, h);
+  lo = __builtin_bit_cast(bf8, l);
+}
+
 // STAGE (1 or 2) does not change the code: it gives the two stages distinct
 // kernel symbols, so a trace reports them apart.
 template <int MODE, int TSZ, int STAGE>
-__global__ __launch_bounds__((TileCfg<MODE, TSZ>::NT), (TSZ == 64 ? 5 : 4)) void k_tiles(const Tile* __restrict__ tiles,
-                                                            const Seg* __restrict__ segs,
-                                                            Spaces<typename KT<MODE>::T> S,
-                                                            typename KT<MODE>::T* partial, int ntiles) {
+__global__ __launch_bounds__((TileCfg<MODE, TSZ>::NT), (TileCfg<MODE, TSZ>::WAVES)) void k_tiles(
+    const Tile* __restrict__ tiles, const Seg* __restrict__ segs, Spaces<MODE> S, typename KT<MODE>::T* partial,
+    int ntiles) {
   using T = typename KT<MODE>::T;
+  using W = typename KT<MODE>::W;
   using Acc = typename KT<MODE>::Acc;
-  constexpr int PER = TileCfg<MODE, TSZ>::PER, KC = TileCfg<MODE, TSZ>::KC;
+  using C = TileCfg<MODE, TSZ>;
+  constexpr int PER = C::PER, KC = C::KC, HS = C::HS;
   const int bid = blockIdx.x;
   if (bid >= ntiles) return;
   const Tile tl = tiles[bid];
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  // the wave index is uniform per wave: read it into a scalar register so
+  // the tile-bound tests on it are scalar branches
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = (TSZ == 64) ? (wave >> 1) * 32 : 0, wn = (TSZ == 64) ? (wave & 1) * 32 : 0;
   const int lr = lane & 15, lq = lane >> 4;
-  constexpr int LDS_T = MODE == MODE_BF16X3 ? 1 : TSZ * (KC + 1);
-  constexpr int LDS_R = MODE == MODE_BF16X3 ? 1 : KC * (TSZ + 1);
-  __shared__ T Ls[LDS_T];
-  __shared__ T Rs[LDS_R];
-  constexpr int HS = KC + 4;                        // bf16 row stride (k fastest)
-  constexpr int LDS_H = MODE == MODE_BF16X3 ? 2 * TSZ * HS : 1;
-  __shared__ unsigned short Lh[LDS_H];
-  __shared__ unsigned short Rh[LDS_H];
+  __shared__ __align__(16) W Ls[TSZ * HS];
+  __shared__ __align__(16) W Rs[TSZ * HS];
   Acc acc[2][2];
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
@@ -177,126 +286,162 @@ __global__ __launch_bounds__((TileCfg<MODE, TSZ>::NT), (TSZ == 64 ? 5 : 4)) void
 
   int si = tl.seg_begin;
   const int se = tl.seg_end;
-  for (; si < se; ++si) {  // axpy segments lead every list
+  for (; si < se; ++si) {  // axpy segments lead every list; they add x itself
     const Seg s = segs[si];
     if (s.r_space != tjsq::SP_IDENT) break;
-    const T* L = space_ptr(S, s.l_space);
     const T c = (T)s.c;
     for (int bi = 0; bi < 2; ++bi)
       for (int bj = 0; bj < 2; ++bj)
         for (int r = 0; r < 4; ++r) {
           const int row = wm + bi * 16 + acc_row<MODE>(lq, r), col = wn + bj * 16 + lr;
           if (row < tl.mt && col < tl.nt)
-            acc[bi][bj][r] += c * L[s.l_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * s.l_ld];
+            acc[bi][bj][r] += c * S.x[s.l_off + (tl.m0 + row) + (int64_t)(tl.n0 + col) * s.l_ld];
         }
   }
   if (si < se) {
+    // BF16x3 sums each segment in its own accumulators and scales the sum by
+    // the coefficient (its operand words cannot be scaled)
+    Acc sacc[2][2];
+    if constexpr (MODE == MODE_BF16X3)
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+          for (int r = 0; r < 4; ++r) sacc[a][b][r] = 0.f;
     Seg cur = segs[si];
     // descriptor of the segment after `cur`, loaded one segment ahead so the
-    // chunk fetches at a segment boundary do not wait on it
-    Seg pre = si + 1 < se ? segs[si + 1] : cur;
+    // chunk loads at a segment boundary do not wait on it
+    Seg pre = segs[min(si + 1, se - 1)];
+    // (segments pass by value: a referenced descriptor would live in
+    // per-lane memory and lose its wave-uniformity)
+    auto side_l = [&](Seg g) {
+      return make_side<MODE, TSZ>(space_ptr(S, g.l_space), g.l_off, g.l_ld, g.l_t != 0, tl.m0, tl.mt, tid);
+    };
+    auto side_r = [&](Seg g) {
+      return make_side<MODE, TSZ>(space_ptr(S, g.r_space), g.r_off, g.r_ld, g.r_t == 0, tl.n0, tl.nt, tid);
+    };
+    Side<MODE, TSZ> Lc = side_l(cur), Rc = side_r(cur);
     int k0 = 0;
-    T rl[PER], rr[PER];
-    fetch_chunk<MODE, TSZ>(S, cur, tl, 0, tid, rl, rr);
+    uint32_t kc = 0;   // chunk index inside the segment
+    W rl[PER], rr[PER];
+    load_side(Lc, cur.K, 0u, tid, rl);
+    load_side(Rc, cur.K, 0u, tid, rr);
     for (;;) {
-      const T c = (T)cur.c;
-#pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        const int e = tid + TileCfg<MODE, TSZ>::NT * q;
-        int i, k;
-        if (!cur.l_t) { i = e % TSZ; k = e / TSZ; } else { k = e % KC; i = e / KC; }
-        int j, kr;
-        if (!cur.r_t) { kr = e % KC; j = e / KC; } else { j = e % TSZ; kr = e / TSZ; }
-        if constexpr (MODE == MODE_BF16X3) {
-          const float v = rl[q] * c;
-          const unsigned short h = bf16_rne(v);
-          Lh[i * HS + k] = h;
-          Lh[TSZ * HS + i * HS + k] = bf16_rne(v - bf16_to_f(h));
-          const float w = rr[q];
-          const unsigned short g = bf16_rne(w);
-          Rh[j * HS + kr] = g;
-          Rh[TSZ * HS + j * HS + kr] = bf16_rne(w - bf16_to_f(g));
-        } else {
-          Ls[i * (KC + 1) + k] = rl[q] * c;
-          Rs[kr * (TSZ + 1) + j] = rr[q];
-        }
-      }
+      store_side<MODE, TSZ, true>(Ls, Lc.kmaj, tid, rl, (T)cur.c);
+      store_side<MODE, TSZ, false>(Rs, Rc.kmaj, tid, rr, T(1));
       __syncthreads();
-      int nk0 = k0 + KC, nsi = si;
-      Seg nxt = cur;
-      if (nk0 >= cur.K) {
-        nsi = si + 1;
-        nk0 = 0;
-        nxt = pre;
-        if (nsi + 1 < se) pre = segs[nsi + 1];
-      }
-      const bool more = nsi < se;
-      if (more) fetch_chunk<MODE, TSZ>(S, nxt, tl, nk0, tid, rl, rr);
-      // MFMA steps beyond the valid K of this chunk and 16-row / 16-column
-      // blocks outside the tile hold only zeros: skip them (wave-uniform)
+      // the chunk in LDS is (cur, k0); the loads below fetch the next one
+      // (the sides move on to the next segment at a segment's last chunk)
       const int kvalid = min(KC, cur.K - k0);
-      const bool r0 = wm < tl.mt, r1 = wm + 16 < tl.mt, c0 = wn < tl.nt, c1 = wn + 16 < tl.nt;
-      if constexpr (MODE == MODE_BF16X3) {
-        const int nkb = (kvalid + 15) / 16;
-        for (int kb = 0; kb < nkb; ++kb) {
-          s4 a[2][2], b[2][2];
-          for (int p = 0; p < 2; ++p)
-            for (int h = 0; h < 2; ++h)
-              for (int t = 0; t < 4; ++t) {
-                a[p][h][t] = (short)Lh[h * TSZ * HS + (wm + p * 16 + lr) * HS + kb * 16 + lq * 4 + t];
-                b[p][h][t] = (short)Rh[h * TSZ * HS + (wn + p * 16 + lr) * HS + kb * 16 + lq * 4 + t];
-              }
-          const bool rr_[2] = {r0, r1}, cc_[2] = {c0, c1};
-          for (int bi = 0; bi < 2; ++bi)
-            for (int bj = 0; bj < 2; ++bj) {
-              if (!(rr_[bi] && cc_[bj])) continue;
-              acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][1], b[bj][0], acc[bi][bj], 0, 0, 0);
-              acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][0], b[bj][1], acc[bi][bj], 0, 0, 0);
-              acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[bi][0], b[bj][0], acc[bi][bj], 0, 0, 0);
-            }
+      const bool last_chunk = k0 + KC >= cur.K;
+      const double cseg = cur.c;
+      if (last_chunk) {
+        ++si;
+        k0 = 0;
+        kc = 0;
+        cur = pre;
+        pre = segs[min(si + 1, se - 1)];
+        if (si < se) {
+          Lc = side_l(cur);
+          Rc = side_r(cur);
         }
       } else {
-        const int nkk = (kvalid + 3) / 4;
-        for (int kk = 0; kk < nkk; ++kk) {
-          const T a0 = Ls[(wm + lr) * (KC + 1) + kk * 4 + lq], a1 = Ls[(wm + 16 + lr) * (KC + 1) + kk * 4 + lq];
-          const T b0 = Rs[(kk * 4 + lq) * (TSZ + 1) + wn + lr], b1 = Rs[(kk * 4 + lq) * (TSZ + 1) + wn + 16 + lr];
-          if constexpr (MODE == MODE_F64) {
-            if (r0 && c0) acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            if (r0 && c1) acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            if (r1 && c0) acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            if (r1 && c1) acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-          } else {
-            if (r0 && c0) acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
-            if (r0 && c1) acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
-            if (r1 && c0) acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
-            if (r1 && c1) acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+        k0 += KC;
+        ++kc;
+      }
+      const bool more = si < se;
+      if (more) {
+        load_side(Lc, cur.K - k0, kc * Lc.ks, tid, rl);
+        load_side(Rc, cur.K - k0, kc * Rc.ks, tid, rr);
+      }
+      // 16-row / 16-column blocks outside the tile hold only zeros: skip them
+      // (wave-uniform); so are MFMA k groups past the chunk's valid K
+      const bool r0 = wm < tl.mt, r1 = wm + 16 < tl.mt, c0 = wn < tl.nt, c1 = wn + 16 < tl.nt;
+      const bool rr_[2] = {r0, r1}, cc_[2] = {c0, c1};
+      if constexpr (MODE == MODE_BF16X3) {
+        bf8 bh[2], bl[2];
+        for (int p = 0; p < 2; ++p) {
+          const W* rb = Rs + (wn + p * 16 + lr) * HS + lq * 8;
+          split_pairs(*(const u4*)rb, *(const u4*)(rb + 4), bh[p], bl[p]);
+        }
+        for (int bi = 0; bi < 2; ++bi) {
+          if (!rr_[bi]) continue;
+          bf8 ah, al;
+          const W* la = Ls + (wm + bi * 16 + lr) * HS + lq * 8;
+          split_pairs(*(const u4*)la, *(const u4*)(la + 4), ah, al);
+          for (int bj = 0; bj < 2; ++bj) {
+            if (!cc_[bj]) continue;
+            sacc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[bj], sacc[bi][bj], 0, 0, 0);
+            sacc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[bj], sacc[bi][bj], 0, 0, 0);
+            sacc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[bj], sacc[bi][bj], 0, 0, 0);
           }
         }
+        if (last_chunk) {
+          const float cf = (float)cseg;
+          for (int bi = 0; bi < 2; ++bi)
+            for (int bj = 0; bj < 2; ++bj)
+              for (int r = 0; r < 4; ++r) {
+                acc[bi][bj][r] += cf * sacc[bi][bj][r];
+                sacc[bi][bj][r] = 0.f;
+              }
+        }
+      } else {
+        // k group g of 16: lane (lr, lq) supplies k = 16 g + 4 lq + s in step s
+        // (FP64 in two halves of two steps, one 16-byte read per row each)
+        constexpr int SH = MODE == MODE_F64 ? 2 : 4;
+        for (int g = 0; g * 16 < kvalid; ++g)
+          for (int h = 0; h < 4; h += SH) {
+            T a[2][SH], b[2][SH];
+            for (int p = 0; p < 2; ++p) {
+              const W* la = Ls + (wm + p * 16 + lr) * HS + g * 16 + lq * 4 + h;
+              const W* rb = Rs + (wn + p * 16 + lr) * HS + g * 16 + lq * 4 + h;
+              if constexpr (MODE == MODE_F64) {
+                const d2 av = *(const d2*)la, bv = *(const d2*)rb;
+                a[p][0] = av.x; a[p][1] = av.y;
+                b[p][0] = bv.x; b[p][1] = bv.y;
+              } else {
+                const f4 av = *(const f4*)la, bv = *(const f4*)rb;
+                for (int q = 0; q < 4; ++q) { a[p][q] = av[q]; b[p][q] = bv[q]; }
+              }
+            }
+            for (int q = 0; q < SH; ++q)
+              for (int bi = 0; bi < 2; ++bi)
+                for (int bj = 0; bj < 2; ++bj) {
+                  if (!(rr_[bi] && cc_[bj])) continue;
+                  if constexpr (MODE == MODE_F64)
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi][q], b[bj][q], acc[bi][bj], 0, 0, 0);
+                  else
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[bi][q], b[bj][q], acc[bi][bj], 0, 0, 0);
+                }
+          }
       }
       __syncthreads();
       if (!more) break;
-      cur = nxt;
-      si = nsi;
-      k0 = nk0;
     }
   }
-  T* out;
-  int64_t base;
-  int ld;
   if (tl.slot >= 0) {            // partial tile: slot counts units of 32x32 elements
-    out = partial;
-    base = (int64_t)tl.slot * 1024;
-    ld = TSZ;
-  } else {
-    out = tl.out_space == tjsq::SP_T ? S.t : S.y;
-    base = tl.out_off + tl.m0 + (int64_t)tl.n0 * tl.out_ld;
-    ld = tl.out_ld;
+    const int64_t base = (int64_t)tl.slot * 1024;
+    for (int bi = 0; bi < 2; ++bi)
+      for (int bj = 0; bj < 2; ++bj)
+        for (int r = 0; r < 4; ++r) {
+          const int row = wm + bi * 16 + acc_row<MODE>(lq, r), col = wn + bj * 16 + lr;
+          if (row < tl.mt && col < tl.nt) partial[base + row + (int64_t)col * TSZ] = acc[bi][bj][r];
+        }
+    return;
   }
+  const int64_t base = tl.out_off + tl.m0 + (int64_t)tl.n0 * tl.out_ld;
+  const int ld = tl.out_ld;
+  const bool to_t = tl.out_space == tjsq::SP_T;
   for (int bi = 0; bi < 2; ++bi)
     for (int bj = 0; bj < 2; ++bj)
       for (int r = 0; r < 4; ++r) {
         const int row = wm + bi * 16 + acc_row<MODE>(lq, r), col = wn + bj * 16 + lr;
-        if (row < tl.mt && col < tl.nt) out[base + row + (int64_t)col * ld] = acc[bi][bj][r];
+        if (!(row < tl.mt && col < tl.nt)) continue;
+        const int64_t e = base + row + (int64_t)col * ld;
+        if (to_t) {               // stage-1 product: stored as operand words
+          if constexpr (MODE == MODE_BF16X3) S.t[e] = split_word(acc[bi][bj][r]);
+          else S.t[e] = acc[bi][bj][r];
+        } else {
+          S.y[e] = acc[bi][bj][r];
+        }
       }
 }
 
@@ -353,6 +498,19 @@ __global__ void k_f64_to_f32(const double* __restrict__ a, float* __restrict__ b
 __global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ b, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     b[i] = (double)a[i];
+}
+// BF16x3 operand words of FP64 operators: the low half is the residual of
+// the FP64 value
+__global__ void k_f64_to_split(const double* __restrict__ a, uint32_t* __restrict__ b, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = a[i];
+    const unsigned short h = bf16_rne((float)v);
+    b[i] = ((uint32_t)h << 16) | bf16_rne((float)(v - (double)bf16_to_f(h)));
+  }
+}
+__global__ void k_f32_to_split(const float* __restrict__ a, uint32_t* __restrict__ b, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = split_word(a[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -461,7 +619,9 @@ struct tjsq_plan {
   HostPlan hp;
   int device = 0;
   double* arena = nullptr;       // FP64 operators (sys, env, merged)
-  float* arena32 = nullptr;      // FP32 copy
+  float* arena32 = nullptr;      // FP32 copy (FP32 contractions)
+  uint32_t* arenab = nullptr;    // BF16x3 operand words
+  uint32_t* xw = nullptr;        // BF16x3 operand words of an apply's x
   double* t64 = nullptr;         // stage-1 scratch
   float* t32 = nullptr;
   Tile* tile1 = nullptr;
@@ -508,7 +668,7 @@ bool to_shape(const tjsq_shape* in, tjsq::Shape& out) {
   out = tjsq::Shape();
   if (!in) return true;
   if ((in->big != 32 && in->big != 64) || in->small != 32 || in->k_split < 32 || in->item_chunks < 1 ||
-      in->xcds < 1 || in->xcds > 64)
+      in->xcds < 1 || in->xcds > 64 || in->order < 0 || in->order > 1)
     return false;
   out.big = in->big;
   out.small = in->small;
@@ -516,6 +676,7 @@ bool to_shape(const tjsq_shape* in, tjsq::Shape& out) {
   out.item_chunks = in->item_chunks;
   out.xcds = in->xcds;
   out.merge_right = in->merge_right != 0;
+  out.order = in->order;
   return true;
 }
 
@@ -561,15 +722,17 @@ int upload_tables(tjsq_plan* p, bool with64, bool with32, hipStream_t st) {
 // 3 stage-2 32x32, 4 the partial-tile reduction
 constexpr int NPROF = 5;
 
-template <int MODE, int TSZ, int STAGE, typename T>
-void launch_items(int n, const Tile* tiles, const Seg* segs, const Spaces<T>& S, T* part, hipStream_t st) {
+template <int MODE, int TSZ, int STAGE>
+void launch_items(int n, const Tile* tiles, const Seg* segs, const Spaces<MODE>& S, typename KT<MODE>::T* part,
+                  hipStream_t st) {
   hipLaunchKernelGGL((k_tiles<MODE, TSZ, STAGE>), dim3(n), dim3(TileCfg<MODE, TSZ>::NT), 0, st, tiles, segs, S, part,
                      n);
 }
 
-template <int MODE, int STAGE, typename T>
-int launch_stage(const tjsq_plan* p, int n, int n64, const Tile* tiles, const Seg* segs, const Spaces<T>& S, T* part,
-                 hipStream_t st) {
+template <int MODE, int STAGE>
+int launch_stage(const tjsq_plan* p, int n, int n64, const Tile* tiles, const Seg* segs, const Spaces<MODE>& S,
+                 typename KT<MODE>::T* part, hipStream_t st) {
+  using T = typename KT<MODE>::T;
   auto timed = [&](int kind, auto&& fn) -> int {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (p->profiling) {
@@ -597,25 +760,37 @@ int launch_stage(const tjsq_plan* p, int n, int n64, const Tile* tiles, const Se
   return rc;
 }
 
-template <typename T>
-int launch_tiles(const tjsq_plan* p, int stage, const T* ops, const T* x, T* t, T* y, int precision,
-                 hipStream_t st) {
-  const int n = stage == 1 ? (int)p->hp.tile1.size() : (int)p->hp.tile2.size();
-  if (n == 0) return TJSQ_OK;
-  const Tile* tiles = stage == 1 ? p->tile1 : p->tile2;
-  const Seg* segs = stage == 1 ? p->seg1 : p->seg2;
-  const int n64 = stage == 1 ? p->hp.n64_1 : p->hp.n64_2;
-  Spaces<T> S{ops, x, t, y};
-  if constexpr (sizeof(T) == 8) {
-    return stage == 1 ? launch_stage<MODE_F64, 1>(p, n, n64, tiles, segs, S, p->part64, st)
-                      : launch_stage<MODE_F64, 2>(p, n, n64, tiles, segs, S, p->part64, st);
-  } else if (precision == TJSQ_BF16X3) {
-    return stage == 1 ? launch_stage<MODE_BF16X3, 1>(p, n, n64, tiles, segs, S, p->part32, st)
-                      : launch_stage<MODE_BF16X3, 2>(p, n, n64, tiles, segs, S, p->part32, st);
-  } else {
-    return stage == 1 ? launch_stage<MODE_F32, 1>(p, n, n64, tiles, segs, S, p->part32, st)
-                      : launch_stage<MODE_F32, 2>(p, n, n64, tiles, segs, S, p->part32, st);
+// both stages of an apply over the spaces S
+template <int MODE>
+int launch_apply(const tjsq_plan* p, const Spaces<MODE>& S, typename KT<MODE>::T* part, hipStream_t st) {
+  for (int stage = 1; stage <= 2; ++stage) {
+    const int n = stage == 1 ? (int)p->hp.tile1.size() : (int)p->hp.tile2.size();
+    if (n == 0) continue;
+    const Tile* tiles = stage == 1 ? p->tile1 : p->tile2;
+    const Seg* segs = stage == 1 ? p->seg1 : p->seg2;
+    const int n64 = stage == 1 ? p->hp.n64_1 : p->hp.n64_2;
+    const int rc = stage == 1 ? launch_stage<MODE, 1>(p, n, n64, tiles, segs, S, part, st)
+                              : launch_stage<MODE, 2>(p, n, n64, tiles, segs, S, part, st);
+    if (rc) return rc;
   }
+  return TJSQ_OK;
+}
+
+// apply of plan p (or a batch's merged plan) in one precision: FP64 on
+// double vectors, FP32 / BF16x3 on float vectors
+int apply_spaces(const tjsq_plan* p, int precision, const void* x, void* y, hipStream_t st) {
+  if (precision == TJSQ_FP64) {
+    const double* xd = (const double*)x;
+    return launch_apply<MODE_F64>(p, Spaces<MODE_F64>{p->arena, xd, xd, p->t64, (double*)y}, p->part64, st);
+  }
+  const float* xf = (const float*)x;
+  if (precision == TJSQ_FP32)
+    return launch_apply<MODE_F32>(p, Spaces<MODE_F32>{p->arena32, xf, xf, p->t32, (float*)y}, p->part32, st);
+  const int64_t n = p->hp.dim;
+  hipLaunchKernelGGL(k_f32_to_split, dim3(grid_for(n)), dim3(256), 0, st, xf, p->xw, n);
+  HIPCHK(hipGetLastError());
+  return launch_apply<MODE_BF16X3>(
+      p, Spaces<MODE_BF16X3>{p->arenab, p->xw, xf, (uint32_t*)p->t32, (float*)y}, p->part32, st);
 }
 
 // tridiagonal eigenproblem (implicit QL with Wilkinson shifts, Golub & Van
@@ -821,14 +996,22 @@ int tjsq_plan_create_ex(const tjsq_lattice* lat, int sys_len, int env_len, const
       (void)hipFree(dt);
     }
   }
-  if (precision_mask & ((1 << TJSQ_FP32) | (1 << TJSQ_BF16X3))) {
+  if (precision_mask & (1 << TJSQ_FP32)) {
     if (hipMalloc(&p->arena32, sizeof(float) * std::max<int64_t>(1, hp.arena_elems)) != hipSuccess)
       return cleanup(TJSQ_ERR_NOMEM, "FP32 operator arena allocation failed");
     hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(hp.arena_elems)), dim3(256), 0, st, p->arena, p->arena32,
                        hp.arena_elems);
     HIPCHK(hipGetLastError());
   }
-  if ((rc = upload_tables(p, (precision_mask & 1) != 0, p->arena32 != nullptr, st))) {
+  if (precision_mask & (1 << TJSQ_BF16X3)) {
+    if (hipMalloc(&p->arenab, sizeof(uint32_t) * std::max<int64_t>(1, hp.arena_elems)) != hipSuccess ||
+        hipMalloc(&p->xw, sizeof(uint32_t) * std::max<int64_t>(1, hp.dim)) != hipSuccess)
+      return cleanup(TJSQ_ERR_NOMEM, "BF16x3 operand allocation failed");
+    hipLaunchKernelGGL(k_f64_to_split, dim3(grid_for(hp.arena_elems)), dim3(256), 0, st, p->arena, p->arenab,
+                       hp.arena_elems);
+    HIPCHK(hipGetLastError());
+  }
+  if ((rc = upload_tables(p, (precision_mask & 1) != 0, p->arena32 || p->arenab, st))) {
     tjsq_plan_destroy(p);
     return rc;
   }
@@ -916,7 +1099,7 @@ int tjsq_plan_reshape(tjsq_plan* p, const tjsq_shape* shape, void* stream) {
   for (void* b : bufs) (void)hipFree(b);
   p->tile1 = p->tile2 = nullptr; p->seg1 = p->seg2 = nullptr; p->rtile = nullptr;
   p->t64 = nullptr; p->t32 = nullptr; p->part64 = nullptr; p->part32 = nullptr;
-  if ((rc = upload_tables(p, false, p->arena32 != nullptr, st))) return rc;
+  if ((rc = upload_tables(p, false, p->arena32 || p->arenab, st))) return rc;
   HIPCHK(hipStreamSynchronize(st));
   return TJSQ_OK;
 }
@@ -933,6 +1116,8 @@ int tjsq_plan_destroy(tjsq_plan* p) {
   p->iter_graphs.clear();
   (void)hipFree(p->arena);
   (void)hipFree(p->arena32);
+  (void)hipFree(p->arenab);
+  (void)hipFree(p->xw);
   (void)hipFree(p->t64);
   (void)hipFree(p->t32);
   (void)hipFree(p->tile1);
@@ -987,19 +1172,15 @@ int tjsq_heff_apply(tjsq_plan* p, const double* x, double* y, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   int rc = ensure_fp64_scratch(p);
   if (rc) return rc;
-  rc = launch_tiles<double>(p, 1, p->arena, x, p->t64, y, TJSQ_FP64, st);
-  if (rc) return rc;
-  return launch_tiles<double>(p, 2, p->arena, x, p->t64, y, TJSQ_FP64, st);
+  return apply_spaces(p, TJSQ_FP64, x, y, st);
 }
 
 int tjsq_heff_apply_f32(tjsq_plan* p, int precision, const float* x, float* y, void* stream) {
   if (!p || !x || !y) return fail(TJSQ_ERR_ARG, "tjsq_heff_apply_f32: null argument");
   if (precision != TJSQ_FP32 && precision != TJSQ_BF16X3) return fail(TJSQ_ERR_ARG, "precision must be FP32 or BF16X3");
-  if (!p->arena32) return fail(TJSQ_ERR_ARG, "plan was created without FP32 operands");
-  hipStream_t st = (hipStream_t)stream;
-  int rc = launch_tiles<float>(p, 1, p->arena32, x, p->t32, y, precision, st);
-  if (rc) return rc;
-  return launch_tiles<float>(p, 2, p->arena32, x, p->t32, y, precision, st);
+  if (precision == TJSQ_FP32 && !p->arena32) return fail(TJSQ_ERR_ARG, "plan was created without FP32 operands");
+  if (precision == TJSQ_BF16X3 && !p->arenab) return fail(TJSQ_ERR_ARG, "plan was created without BF16x3 operands");
+  return apply_spaces(p, precision, x, y, (hipStream_t)stream);
 }
 
 }  // extern "C"
@@ -1242,7 +1423,8 @@ extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_
                                   tjsq_lanczos_result* res, void* stream) {
   if (!p || !psi || !opts || !res) return fail(TJSQ_ERR_ARG, "tjsq_lanczos_solve: null argument");
   if (opts->max_iter < 2 || opts->max_iter > 250) return fail(TJSQ_ERR_ARG, "max_iter must be in [2, 250]");
-  if (opts->precision != TJSQ_FP64 && !p->arena32) return fail(TJSQ_ERR_ARG, "plan has no FP32 operands");
+  if (opts->precision == TJSQ_FP32 && !p->arena32) return fail(TJSQ_ERR_ARG, "plan has no FP32 operands");
+  if (opts->precision == TJSQ_BF16X3 && !p->arenab) return fail(TJSQ_ERR_ARG, "plan has no BF16x3 operands");
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t st = (hipStream_t)stream;
   std::memset(res, 0, sizeof(*res));
@@ -1602,8 +1784,8 @@ int tjsq_program_run(tjsq_program* P, const double* ops, const double* x, double
     HIPCHK(hipGetLastError());
   }
   const double* opsp = ops ? ops : xin;
-  if ((rc = launch_tiles<double>(P->dev, 1, opsp, xin, P->dev->t64, y, TJSQ_FP64, st))) return rc;
-  if ((rc = launch_tiles<double>(P->dev, 2, opsp, xin, P->dev->t64, y, TJSQ_FP64, st))) return rc;
+  if ((rc = launch_apply<MODE_F64>(P->dev, Spaces<MODE_F64>{opsp, xin, xin, P->dev->t64, y}, P->dev->part64, st)))
+    return rc;
   if (nrm) {
     hipLaunchKernelGGL(k_scale_inv, dim3(grid_for(P->out.ndata)), dim3(256), 0, st, y, P->out.ndata, nrm);
     HIPCHK(hipGetLastError());
@@ -1824,13 +2006,7 @@ void graph_free(tjsq_batch::Graph& G) {
 
 template <typename T>
 int batch_launch(tjsq_batch* B, int precision, const T* x, T* y, hipStream_t st) {
-  const T* ops;
-  T* t;
-  if constexpr (sizeof(T) == 8) { ops = B->dev->arena; t = B->dev->t64; }
-  else { ops = B->dev->arena32; t = B->dev->t32; }
-  int rc = launch_tiles<T>(B->dev, 1, ops, x, t, y, precision, st);
-  if (rc == TJSQ_OK) rc = launch_tiles<T>(B->dev, 2, ops, x, t, y, precision, st);
-  return rc;
+  return apply_spaces(B->dev, precision, x, y, st);
 }
 
 // runs the batch, through a captured hipGraph when enabled (recaptured when
@@ -1865,11 +2041,14 @@ int tjsq_batch_create(tjsq_plan* const* plans, int n, int precision_mask, void* 
   if (!plans || n < 1 || !out || (precision_mask & 7) == 0) return fail(TJSQ_ERR_ARG, "tjsq_batch_create: bad argument");
   *out = nullptr;
   const bool with64 = (precision_mask & 1) != 0;
-  const bool with32 = (precision_mask & 6) != 0;
+  const bool withf = (precision_mask & (1 << TJSQ_FP32)) != 0;
+  const bool withb = (precision_mask & (1 << TJSQ_BF16X3)) != 0;
   for (int i = 0; i < n; ++i) {
     if (!plans[i]) return fail(TJSQ_ERR_ARG, "tjsq_batch_create: null plan");
-    if (with32 && !plans[i]->arena32)
+    if (withf && !plans[i]->arena32)
       return fail(TJSQ_ERR_ARG, "tjsq_batch_create: a member plan was created without FP32 operands");
+    if (withb && !plans[i]->arenab)
+      return fail(TJSQ_ERR_ARG, "tjsq_batch_create: a member plan was created without BF16x3 operands");
   }
   hipStream_t st = (hipStream_t)stream;
   tjsq_batch* B = new tjsq_batch();
@@ -1944,18 +2123,23 @@ int tjsq_batch_create(tjsq_plan* const* plans, int n, int precision_mask, void* 
   };
   if (with64 && hipMalloc(&B->dev->arena, sizeof(double) * std::max<int64_t>(1, arena)) != hipSuccess)
     return cleanup(TJSQ_ERR_NOMEM, "batch arena allocation failed");
-  if (with32 && hipMalloc(&B->dev->arena32, sizeof(float) * std::max<int64_t>(1, arena)) != hipSuccess)
+  if (withf && hipMalloc(&B->dev->arena32, sizeof(float) * std::max<int64_t>(1, arena)) != hipSuccess)
     return cleanup(TJSQ_ERR_NOMEM, "batch FP32 arena allocation failed");
+  if (withb && (hipMalloc(&B->dev->arenab, sizeof(uint32_t) * std::max<int64_t>(1, arena)) != hipSuccess ||
+                hipMalloc(&B->dev->xw, sizeof(uint32_t) * std::max<int64_t>(1, M.dim)) != hipSuccess))
+    return cleanup(TJSQ_ERR_NOMEM, "batch BF16x3 operand allocation failed");
   for (int i = 0; i < n; ++i) {
     const int64_t e = plans[i]->hp.arena_elems;
     if (e == 0) continue;
     if ((with64 && hipMemcpyAsync(B->dev->arena + abase[i], plans[i]->arena, sizeof(double) * e,
                                   hipMemcpyDeviceToDevice, st) != hipSuccess) ||
-        (with32 && hipMemcpyAsync(B->dev->arena32 + abase[i], plans[i]->arena32, sizeof(float) * e,
-                                  hipMemcpyDeviceToDevice, st) != hipSuccess))
+        (withf && hipMemcpyAsync(B->dev->arena32 + abase[i], plans[i]->arena32, sizeof(float) * e,
+                                 hipMemcpyDeviceToDevice, st) != hipSuccess) ||
+        (withb && hipMemcpyAsync(B->dev->arenab + abase[i], plans[i]->arenab, sizeof(uint32_t) * e,
+                                 hipMemcpyDeviceToDevice, st) != hipSuccess))
       return cleanup(TJSQ_ERR_HIP, "batch arena copy failed");
   }
-  int rc = upload_tables(B->dev, with64, with32, st);
+  int rc = upload_tables(B->dev, with64, withf || withb, st);
   if (rc) {
     tjsq_batch_destroy(B);
     return rc;
@@ -1992,7 +2176,10 @@ int tjsq_batch_apply(tjsq_batch* B, const double* x, double* y, void* stream) {
 int tjsq_batch_apply_f32(tjsq_batch* B, int precision, const float* x, float* y, void* stream) {
   if (!B || !x || !y || (precision != TJSQ_FP32 && precision != TJSQ_BF16X3))
     return fail(TJSQ_ERR_ARG, "tjsq_batch_apply_f32: bad argument");
-  if (!B->dev->arena32) return fail(TJSQ_ERR_ARG, "tjsq_batch_apply_f32: the batch was created without FP32");
+  if (precision == TJSQ_FP32 && !B->dev->arena32)
+    return fail(TJSQ_ERR_ARG, "tjsq_batch_apply_f32: the batch was created without FP32");
+  if (precision == TJSQ_BF16X3 && !B->dev->arenab)
+    return fail(TJSQ_ERR_ARG, "tjsq_batch_apply_f32: the batch was created without BF16x3");
   return batch_run<float>(B, precision, x, y, (hipStream_t)stream);
 }
 

@@ -1313,12 +1313,28 @@ int compile_program(HostPlan& P, const Shape& shape) {
     }
     P.seg2[r] = s;
   }
+  // order 1: each block's GEMM segments sorted by right operand (axpy
+  // segments stay first), and blocks visited column strip by column strip
+  std::vector<int> border(P.out_blocks.size());
+  for (size_t b = 0; b < border.size(); ++b) border[b] = (int)b;
+  if (shape.order == 1) {
+    for (size_t b = 0; b < P.out_blocks.size(); ++b)
+      std::stable_sort(P.seg2.begin() + bstart[b], P.seg2.begin() + bstart[b + 1], [](const Seg& a, const Seg& c) {
+        const bool ia = a.r_space == SP_IDENT, ic = c.r_space == SP_IDENT;
+        if (ia != ic) return ia;
+        if (ia) return false;
+        return std::make_tuple((int)a.r_space, a.r_off, (int)a.r_t, a.l_off) <
+               std::make_tuple((int)c.r_space, c.r_off, (int)c.r_t, c.l_off);
+      });
+    std::stable_sort(border.begin(), border.end(),
+                     [&](int a, int c) { return P.out_blocks[a].group < P.out_blocks[c].group; });
+  }
   // work items: per output block, axpy segments first, then GEMM segments cut
   // into K pieces of at most k_split and grouped into items of bounded work
   std::vector<Seg> pieces;
   std::vector<Tile> t2big, t2small;
   pieces.reserve(P.seg2.size() * 2);
-  for (size_t b = 0; b < P.out_blocks.size(); ++b) {
+  for (int b : border) {
     const OutBlock& ob = P.out_blocks[b];
     std::vector<std::pair<int, int>> items;  // [begin, end) into pieces
     const int first = (int)pieces.size();
@@ -1373,7 +1389,7 @@ int compile_program(HostPlan& P, const Shape& shape) {
   // an XCD groups run in that order and items longest first.  Workgroup id g
   // runs on XCD g % xcds, so bin x's p-th item takes id p * xcds + x; bins are
   // padded with empty items (mt = nt = 0, no segments).
-  auto xcd_order = [&](std::vector<Tile>& v, const std::vector<Seg>& segs) {
+  auto xcd_order = [&](std::vector<Tile>& v, const std::vector<Seg>& segs, bool keep) {
     const int NX = std::max(1, shape.xcds);
     std::vector<int64_t> cost(v.size(), 0);
     for (size_t t = 0; t < v.size(); ++t) {
@@ -1397,9 +1413,11 @@ int compile_program(HostPlan& P, const Shape& shape) {
     }
     std::vector<int> idx(v.size());
     for (size_t t = 0; t < idx.size(); ++t) idx[t] = (int)t;
+    // keep: a group's items stay in table order (the blocks of a column
+    // strip walk their operands together); else its longest items run first
     std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
       const int ra = grank[v[a].group], rb = grank[v[b].group];
-      return ra != rb ? ra < rb : cost[a] > cost[b];
+      return ra != rb ? ra < rb : (!keep && cost[a] > cost[b]);
     });
     std::vector<std::vector<int>> bins(NX);
     for (int t : idx) bins[gbin[v[t].group]].push_back(t);
@@ -1411,10 +1429,10 @@ int compile_program(HostPlan& P, const Shape& shape) {
       for (size_t p = 0; p < bins[x].size(); ++p) out[p * NX + x] = v[bins[x][p]];
     v.swap(out);
   };
-  xcd_order(t2big, P.seg2);
-  xcd_order(t2small, P.seg2);
-  xcd_order(t1big, P.seg1);
-  xcd_order(t1small, P.seg1);
+  xcd_order(t2big, P.seg2, shape.order == 1);
+  xcd_order(t2small, P.seg2, shape.order == 1);
+  xcd_order(t1big, P.seg1, false);
+  xcd_order(t1small, P.seg1, false);
   P.n64_1 = (int)t1big.size();
   P.tile1 = t1big;
   P.tile1.insert(P.tile1.end(), t1small.begin(), t1small.end());

@@ -149,6 +149,10 @@ struct Shape {
                             // the plain order measured faster on MI355X, so it is the default
   int merge_right = 1;      // factor two-sided terms of one output block that share op(B)
                             // (6-8 % faster applies at the m = 8000 workload)
+  int order = 0;            // stage-2 item order: 0 = output blocks in wave-sector order;
+                            // 1 = column strips (blocks sharing an env sub-block) together,
+                            // each block's segments sorted by their right operand, so the
+                            // strip's items walk the shared env operators in one order
 };
 
 // Compiles P.contrib and P.out_blocks into seg1/tile1/seg2/tile2/rtile and

@@ -190,7 +190,10 @@ int tjsq_plan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tj
  * are binned over `xcds` XCDs (1 = plain order).  merge_right = 1 factors
  * the two-sided terms of one output block that share their right operand B:
  * sum_j c_j A_j X_j B = (sum_j c_j A_j X_j) B, the sum formed in stage 1.
- * NULL selects {64, 32, 256, 32, 1, 1}. */
+ * order = 1 lays stage-2 work items out by column strip (output blocks that
+ * share an env sub-block, hence their env operators) with each block's
+ * segments sorted by right operand; 0 keeps wave-sector order.
+ * NULL selects {64, 32, 256, 32, 1, 1, 0}. */
 typedef struct tjsq_shape {
   int big;
   int small;
@@ -198,6 +201,7 @@ typedef struct tjsq_shape {
   int item_chunks;
   int xcds;
   int merge_right;
+  int order;
 } tjsq_shape;
 
 int tjsq_plan_create_ex(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys,

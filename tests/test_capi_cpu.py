@@ -99,7 +99,9 @@ def test_plan_rejects_lattice_too_small():
 # partial tiles)
 TUNE_SHAPES = [None, {"big": 32}, {"k_split": 32, "item_chunks": 1}, {"k_split": 64, "item_chunks": 4},
                {"k_split": 1024, "item_chunks": 64}, {"xcds": 8}, {"big": 32, "k_split": 128, "xcds": 4},
-               {"merge_right": 1}, {"merge_right": 1, "big": 32, "k_split": 64, "item_chunks": 2, "xcds": 8}]
+               {"merge_right": 1}, {"merge_right": 1, "big": 32, "k_split": 64, "item_chunks": 2, "xcds": 8},
+               {"order": 1}, {"order": 1, "xcds": 8, "k_split": 64, "item_chunks": 2},
+               {"order": 1, "merge_right": 0, "big": 32}]
 
 
 @pytest.mark.parametrize("shape", TUNE_SHAPES)
@@ -116,7 +118,7 @@ def test_plan_shapes_reproduce_oracle(shape):
 def test_plan_rejects_unsupported_shape():
     m = model()
     sb = small_superblock(m, 7, 9, seed=2)
-    for bad in ({"big": 128}, {"small": 16}, {"k_split": 8}, {"item_chunks": 0}, {"xcds": 0}):
+    for bad in ({"big": 128}, {"small": 16}, {"k_split": 8}, {"item_chunks": 0}, {"xcds": 0}, {"order": 2}):
         with pytest.raises(RuntimeError, match="shape"):
             capi.HostPlan(m, sb, shape=bad)
 
