@@ -135,6 +135,7 @@ struct TileCfg {
   // items cost one wave per SIMD of occupancy and measured slower.
   static constexpr int KC = 32;
   static constexpr int PER = TSZ * KC / NT;          // elements per thread per panel and chunk
+  static_assert(KC % 32 == 0 && PER >= 1, "K chunk must be a multiple of 32");
   // LDS row stride in operand words: rows start 16-byte aligned and the
   // operand reads of 8 consecutive lanes fall in distinct banks
   static constexpr int HS = MODE == MODE_F64 ? KC + 2 : KC + 4;
@@ -157,10 +158,17 @@ __device__ __forceinline__ W buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
 }
 
 // One operand panel of a segment as the threads load it: TSZ tile rows (or
-// columns) by KC values of k per chunk.  A tile-major operand (contiguous
-// along the tile) gives each lane a fixed tile index and k = tid / TSZ +
-// q NT / TSZ; a k-major one a fixed k = tid % KC and tile index tid / KC +
-// q NT / KC.
+// columns) by KC values of k per chunk.  A k-major operand (contiguous along
+// k) gives each lane a fixed k = tid % KC and tile index tid / KC + q NT / KC.
+// A tile-major one (contiguous along the tile) gives the lane tile index
+// a = tid % TSZ and k = tid / TSZ + q NT / TSZ for 8-byte words; for 4-byte
+// words a = tid % 16 + 16 (q % G) and k = (tid / 16) % 4 + 4 wave +
+// 4 NW (q / G), G = TSZ / 16, NW = waves, so that one store instruction
+// writes 16 rows x 4 k, which fall in distinct LDS banks (a store of 64 rows
+// at one k would hit only 16 banks with the 16-byte aligned row stride).
+template <int MODE>
+constexpr bool quad_map() { return MODE != MODE_F64; }
+
 template <int MODE, int TSZ>
 struct Side {
   __amdgpu_buffer_rsrc_t r;
@@ -181,7 +189,13 @@ __device__ __forceinline__ Side<MODE, TSZ> make_side(const typename KT<MODE>::W*
   s.r = make_rsrc(base + off);
   s.lim = lim;
   s.kmaj = kmaj;
-  if (!kmaj) {
+  if (!kmaj && quad_map<MODE>()) {
+    const int a = tid % 16, b = (tid / 16) % 4 + 4 * (tid / 64);
+    s.fix = a;
+    s.lane = (uint32_t)((t0 + a) + b * ld) * WB;
+    s.qs = (uint32_t)(4 * (C::NT / 64) * ld) * WB;
+    s.ks = (uint32_t)(C::KC * ld) * WB;
+  } else if (!kmaj) {
     const int a = tid % TSZ, b = tid / TSZ;
     s.fix = a;
     s.lane = (uint32_t)((t0 + a) + b * ld) * WB;
@@ -204,7 +218,16 @@ __device__ __forceinline__ void load_side(const Side<MODE, TSZ>& s, int kv, uint
                                           typename KT<MODE>::W* v) {
   using C = TileCfg<MODE, TSZ>;
   using W = typename KT<MODE>::W;
-  if (!s.kmaj) {
+  if (!s.kmaj && quad_map<MODE>()) {
+    constexpr int G = TSZ / 16;
+    constexpr uint32_t WB = sizeof(W);
+    const int b0 = (tid / 16) % 4 + 4 * (tid / 64);
+#pragma unroll
+    for (int q = 0; q < C::PER; ++q) {
+      const bool ok = s.fix + 16 * (q % G) < s.lim && b0 + 4 * (C::NT / 64) * (q / G) < kv;
+      v[q] = buf_load<W>(s.r, ok ? s.lane + kofs + (uint32_t)(16 * (q % G)) * WB + (uint32_t)(q / G) * s.qs : OOB);
+    }
+  } else if (!s.kmaj) {
     // 64-tiles: b0 = tid / 64 is the wave index (scalar)
     const int b0 = TSZ == 64 ? __builtin_amdgcn_readfirstlane(tid / TSZ) : tid / TSZ;
     const bool oka = s.fix < s.lim;
@@ -234,7 +257,12 @@ __device__ __forceinline__ void store_side(typename KT<MODE>::W* lds, bool kmaj,
     if constexpr (SCALE && MODE != MODE_BF16X3) return v[q] * c;
     else return v[q];
   };
-  if (!kmaj) {
+  if (!kmaj && quad_map<MODE>()) {
+    constexpr int G = TSZ / 16;
+    const int a = tid % 16, b = (tid / 16) % 4 + 4 * (tid / 64);
+#pragma unroll
+    for (int q = 0; q < C::PER; ++q) lds[(a + 16 * (q % G)) * C::HS + b + 4 * (C::NT / 64) * (q / G)] = val(q);
+  } else if (!kmaj) {
     const int a = tid % TSZ, b0 = tid / TSZ;
 #pragma unroll
     for (int q = 0; q < C::PER; ++q) lds[a * C::HS + b0 + q * (C::NT / TSZ)] = val(q);
@@ -357,21 +385,24 @@ __global__ __launch_bounds__((TileCfg<MODE, TSZ>::NT), (TileCfg<MODE, TSZ>::WAVE
       const bool r0 = wm < tl.mt, r1 = wm + 16 < tl.mt, c0 = wn < tl.nt, c1 = wn + 16 < tl.nt;
       const bool rr_[2] = {r0, r1}, cc_[2] = {c0, c1};
       if constexpr (MODE == MODE_BF16X3) {
-        bf8 bh[2], bl[2];
-        for (int p = 0; p < 2; ++p) {
-          const W* rb = Rs + (wn + p * 16 + lr) * HS + lq * 8;
-          split_pairs(*(const u4*)rb, *(const u4*)(rb + 4), bh[p], bl[p]);
-        }
-        for (int bi = 0; bi < 2; ++bi) {
-          if (!rr_[bi]) continue;
-          bf8 ah, al;
-          const W* la = Ls + (wm + bi * 16 + lr) * HS + lq * 8;
-          split_pairs(*(const u4*)la, *(const u4*)(la + 4), ah, al);
-          for (int bj = 0; bj < 2; ++bj) {
-            if (!cc_[bj]) continue;
-            sacc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[bj], sacc[bi][bj], 0, 0, 0);
-            sacc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[bj], sacc[bi][bj], 0, 0, 0);
-            sacc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[bj], sacc[bi][bj], 0, 0, 0);
+        // k step g of 32: lane (lr, lq) supplies k = 32 g + 8 lq + t, t < 8
+        for (int g = 0; g * 32 < kvalid; ++g) {
+          bf8 bh[2], bl[2];
+          for (int p = 0; p < 2; ++p) {
+            const W* rb = Rs + (wn + p * 16 + lr) * HS + g * 32 + lq * 8;
+            split_pairs(*(const u4*)rb, *(const u4*)(rb + 4), bh[p], bl[p]);
+          }
+          for (int bi = 0; bi < 2; ++bi) {
+            if (!rr_[bi]) continue;
+            bf8 ah, al;
+            const W* la = Ls + (wm + bi * 16 + lr) * HS + g * 32 + lq * 8;
+            split_pairs(*(const u4*)la, *(const u4*)(la + 4), ah, al);
+            for (int bj = 0; bj < 2; ++bj) {
+              if (!cc_[bj]) continue;
+              sacc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[bj], sacc[bi][bj], 0, 0, 0);
+              sacc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[bj], sacc[bi][bj], 0, 0, 0);
+              sacc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[bj], sacc[bi][bj], 0, 0, 0);
+            }
           }
         }
         if (last_chunk) {

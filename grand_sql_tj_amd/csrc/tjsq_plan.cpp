@@ -358,6 +358,9 @@ struct Emitter {
       bad_dims = true;
       return;
     }
+    // a vanishing coefficient (a zero 6j symbol or coupling) adds exactly
+    // zero for finite operands: its product is not formed
+    if (c == 0.0) return;
     Contrib q{};
     q.kind = kind;
     if (A) { q.a_off = A->off; q.a_ld = A->ld; q.a_t = A->t; }
