@@ -14,8 +14,8 @@ solve from the previous bond's ground state carried over.
 
 The printed JSON line carries solves/s of the mixed path (value), the FP64
 path for comparison, the energy difference per site, p50/p99 latency, the
-roofline of the dominant kernel (stage-2 grouped GEMM, HIP events on the
-launching stream) and the CPU baseline: the reference's own model_wave
+roofline of the dominant kernel (the grouped-GEMM stage with the longest
+launches, HIP events on the launching stream) and the CPU baseline: the reference's own model_wave
 (compiled Fortran + MKL, one core) timed on a bounded sample of applies of
 the same workload.
 """
