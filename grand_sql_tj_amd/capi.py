@@ -79,7 +79,8 @@ EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_t
            "tjsq_lanczos_solve", "tjsq_hostplan_create", "tjsq_hostplan_sizes", "tjsq_hostplan_tables",
            "tjsq_hostplan_stats", "tjsq_hostplan_destroy", "tjsq_plan_profile", "tjsq_plan_profile_read",
            "tjsq_heff_apply_host", "tjsq_lanczos_solve_host",
-           "tjsq_block_update_create", "tjsq_trun_project_create", "tjsq_density_create", "tjsq_program_sizes",
+           "tjsq_block_update_create", "tjsq_trun_project_create", "tjsq_density_create", "tjsq_wave_transform_create",
+           "tjsq_program_sizes",
            "tjsq_program_layout", "tjsq_program_tables", "tjsq_program_stats", "tjsq_program_run",
            "tjsq_program_profile", "tjsq_program_profile_read", "tjsq_program_destroy", "tjsq_truncation_operator",
            "tjsq_truncation_select", "tjsq_identity_truncation", "tjsq_batch_create", "tjsq_batch_offsets",
@@ -124,6 +125,7 @@ def lib():
             "tjsq_block_update_create": [P, I, I, I, P, P, P, P],
             "tjsq_trun_project_create": [P, P, P, P],
             "tjsq_density_create": [I, I, I, P, I, P],
+            "tjsq_wave_transform_create": [I, P, P, P, P, I, P, I, P, P],
             "tjsq_program_sizes": [P, P],
             "tjsq_program_layout": [P, P, P, P],
             "tjsq_program_tables": [P, P, P, P, P, P],
@@ -457,6 +459,21 @@ class Program:
         h = C.c_void_p()
         check(lib().tjsq_trun_project_create(C.byref(os_), nd.ctypes.data, C.byref(ts), C.byref(h)),
               "tjsq_trun_project_create")
+        return cls(h)
+
+    @classmethod
+    def wave_transform(cls, direction, sys_bs, env_bs, sys_trun, env_trun, wsec_old, wsec_new):
+        """wave_trans_LR (direction 0) / wave_trans_RL (1): run with ops =
+        concat(sys_trun.data, env_trun.data), x = old psi, y = new psi."""
+        keep = _Keep()
+        sb, eb = basis_struct(sys_bs, keep), basis_struct(env_bs, keep)
+        st, et = opset_struct(sys_trun, keep), opset_struct(env_trun, keep)
+        wo = np.ascontiguousarray(wsec_old, np.int32).reshape(-1, 6)
+        wn = np.ascontiguousarray(wsec_new, np.int32).reshape(-1, 6)
+        h = C.c_void_p()
+        check(lib().tjsq_wave_transform_create(direction, C.byref(sb), C.byref(eb), C.byref(st), C.byref(et),
+                                               wo.shape[0], wo.ctypes.data, wn.shape[0], wn.ctypes.data,
+                                               C.byref(h)), "tjsq_wave_transform_create")
         return cls(h)
 
     @classmethod

@@ -139,8 +139,10 @@ struct TileCfg {
   // LDS row stride in operand words: rows start 16-byte aligned and the
   // operand reads of 8 consecutive lanes fall in distinct banks
   static constexpr int HS = MODE == MODE_F64 ? KC + 2 : KC + 4;
-  // waves per SIMD asked of the compiler (the kernel is latency-bound)
-  static constexpr int WAVES = TSZ == 64 ? (MODE == MODE_BF16X3 ? 4 : 5) : 4;
+  // waves per SIMD asked of the compiler (the kernel is latency-bound):
+  // 4 for the FP64 and BF16x3 64-tiles (119 / 105 VGPRs; FP64 measured 10 %
+  // faster than at 3, BF16x3 slower at 5 where 10 VGPRs spill), 5 for FP32
+  static constexpr int WAVES = TSZ == 64 ? (MODE == MODE_BF16X3 || MODE == MODE_F64 ? 4 : 5) : 4;
 };
 
 constexpr uint32_t OOB = 0x80000000u;   // at or above the record count: reads as zero
@@ -1596,11 +1598,11 @@ struct tjsq_program {
   tjsq_plan* dev = nullptr;      // compiled tables (dev->hp) and their device copies
   tjsq::OutLayout out;
   int64_t x_elems = 0, ops_elems = 0;
-  int kind = 0;                  // 0: block update / projection, 1: density matrix
+  int kind = 0;                  // 0: block update / projection, 1: density matrix, 2: wave transform
   int levels = 1;
   bool uploaded = false;
   double* work = nullptr;        // density: sum of the states
-  double* dwork = nullptr;       // density: dot partials, norm, coefficients
+  double* dwork = nullptr;       // density / wave transform: dot partials, norm, coefficients
 };
 
 namespace {
@@ -1743,6 +1745,19 @@ int tjsq_density_create(int side, int len, int nw, const int* wsec, int levels, 
   return make_program(pg, 1, levels, prog);
 }
 
+int tjsq_wave_transform_create(int dir, const tjsq_basis* sys_bs, const tjsq_basis* env_bs, const tjsq_opset* sys_trun,
+                               const tjsq_opset* env_trun, int nw_old, const int* wsec_old, int nw_new,
+                               const int* wsec_new, tjsq_program** prog) {
+  if (!sys_bs || !env_bs || !sys_trun || !env_trun || !prog) return fail(TJSQ_ERR_ARG, "tjsq_wave_transform_create: null argument");
+  *prog = nullptr;
+  tjsq::Program pg;
+  std::string msg;
+  int rc = tjsq::build_wave_transform_program(dir, *sys_bs, *env_bs, *sys_trun, *env_trun, nw_old, wsec_old, nw_new,
+                                              wsec_new, tjsq::Shape(), pg, msg);
+  if (rc) return fail(rc, "tjsq_wave_transform_create: " + msg);
+  return make_program(pg, 2, 1, prog);
+}
+
 int tjsq_program_sizes(const tjsq_program* P, int64_t* sizes) {
   if (!P || !sizes) return fail(TJSQ_ERR_ARG, "tjsq_program_sizes: null argument");
   const HostPlan& hp = P->dev->hp;
@@ -1819,6 +1834,19 @@ int tjsq_program_run(tjsq_program* P, const double* ops, const double* x, double
     return rc;
   if (nrm) {
     hipLaunchKernelGGL(k_scale_inv, dim3(grid_for(P->out.ndata)), dim3(256), 0, st, y, P->out.ndata, nrm);
+    HIPCHK(hipGetLastError());
+  }
+  if (P->kind == 2) {
+    // wave_normalize (block.f90:2418-2432): y /= |y|, the norm an FP64
+    // two-pass reduction
+    const int64_t n = P->out.ndata;
+    if (!P->dwork && hipMalloc(&P->dwork, sizeof(double) * (2 * RED_BLOCKS + 16)) != hipSuccess)
+      return fail(TJSQ_ERR_NOMEM, "wave transform work allocation failed");
+    double* sq = P->dwork + 2 * RED_BLOCKS;
+    hipLaunchKernelGGL(k_multidot<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, y, n, 1, y, n, P->dwork, 0);
+    hipLaunchKernelGGL(k_finish_dots, dim3(1), dim3(RED_THREADS), 0, st, P->dwork, 1, RED_BLOCKS, 0, sq);
+    hipLaunchKernelGGL(k_sqrt_to, dim3(1), dim3(64), 0, st, sq + 1, sq);
+    hipLaunchKernelGGL(k_scale_inv, dim3(grid_for(n)), dim3(256), 0, st, y, n, sq + 1);
     HIPCHK(hipGetLastError());
   }
   return TJSQ_OK;

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace tjsq {
@@ -610,6 +611,233 @@ int build_density_program(int side, int len, int nw, const int* wsec, const Shap
   pg.out.hdr = {len, 0, 0, nw};
   pg.x_elems = xoff;
   pg.ops_elems = 0;
+  Ctx cx;
+  return finish(pg, shape, cx, msg);
+}
+
+// --------------------------------------------------------------------------
+// wave_trans_LR / wave_trans_RL (dmrg_sub.f90:1093-1384)
+// --------------------------------------------------------------------------
+namespace {
+
+// (-1)**n of an integer n, as Fortran's integer power gives it
+double neg_pow(int n) { return (n % 2 == 0) ? 1.0 : -1.0; }
+
+// recoupling coefficient of wave_trans_LR (dmrg_sub.f90:1207-1214):
+// j12 = 2S of the old system sector, j3 = the moving site's 2S, j123 = the
+// new system sector's 2S, j4 = the new environment sector's 2S, j34 = the
+// old environment sector's 2S
+double coef_lr(int j12, int j3, int j123, int j4, int j34) {
+  double c = neg_pow((j12 + j3 + j123) / 2) / std::sqrt((1.0 + j12) * (1.0 + j123));
+  if (c != 0.0) {
+    c = c * std::sqrt((1.0 + j123) * (j34 + 1.0));
+    if (((j12 + j3 + j4) / 2) % 2 != 0) c = -c;
+    if (((j3 + j4 - j34) / 2) % 2 != 0) c = -c;
+  }
+  return c;
+}
+
+// wave_trans_RL (dmrg_sub.f90:1354-1361): j12 = old environment 2S, j3 =
+// site 2S, j4 = new system 2S, j123 = new environment 2S, j34 = old system 2S
+double coef_rl(int j12, int j3, int j4, int j123, int j34) {
+  double c = neg_pow((j34 + j4 + j3) / 2) / std::sqrt((1.0 + j12) * (1.0 + j4));
+  if (c != 0.0) {
+    c = c * std::sqrt((1.0 + j123) * (j34 + 1.0));
+    if (((j4 + j3 + j12) / 2) % 2 != 0) c = -c;
+    if (((j12 + j3 - j123) / 2) % 2 != 0) c = -c;
+  }
+  return c;
+}
+
+int first_sub(const Op& o, int up, int down) {
+  for (size_t k = 0; k < o.sub.size(); ++k)
+    if (o.sub[k].num_up == up && o.sub[k].num_down == down) return (int)k;
+  return -1;
+}
+
+// first sector of b with quantum numbers (up, down) holding a sub-block
+// (bl_up, bl_down) x (st_up, st_down) (the reference's nested search with
+// goto on the first match)
+bool find_fused(const BasisView& b, int up, int down, int bl_up, int bl_down, int st_up, int st_down, int& y,
+                int& z) {
+  for (size_t s = 0; s < b.sec.size(); ++s) {
+    if (b.sec[s].new_up != up || b.sec[s].new_down != down) continue;
+    for (size_t q = 0; q < b.sec[s].sub.size(); ++q) {
+      const MapSub& m = b.sec[s].sub[q];
+      if (m.bl_up == bl_up && m.bl_down == bl_down && m.st_up == st_up && m.st_down == st_down) {
+        y = (int)s;
+        z = (int)q;
+        return true;
+      }
+    }
+  }
+  return false;
+}
+
+int first_sector(const BasisView& b, int up, int down) {
+  for (size_t s = 0; s < b.sec.size(); ++s)
+    if (b.sec[s].new_up == up && b.sec[s].new_down == down) return (int)s;
+  return -1;
+}
+
+}  // namespace
+
+int build_wave_transform_program(int dir, const tjsq_basis& sys_bs, const tjsq_basis& env_bs,
+                                 const tjsq_opset& sys_trun, const tjsq_opset& env_trun, int nw_old,
+                                 const int* wsec_old, int nw_new, const int* wsec_new, const Shape& shape,
+                                 Program& pg, std::string& msg) {
+  pg = Program();
+  if (dir != 0 && dir != 1) { msg = "direction must be 0 (left to right) or 1 (right to left)"; return TJSQ_ERR_ARG; }
+  if (nw_old < 0 || nw_new < 1 || (nw_old > 0 && !wsec_old) || !wsec_new) {
+    msg = "bad wavefunction sector lists";
+    return TJSQ_ERR_ARG;
+  }
+  std::vector<Op> st, et;
+  int rc = read_ops(sys_trun, st, msg);
+  if (rc) return rc;
+  if ((rc = read_ops(env_trun, et, msg))) return rc;
+  if (sys_trun.nops != 1 || env_trun.nops != 1) { msg = "each truncation operator set must hold one operator"; return TJSQ_ERR_ARG; }
+  const Op& S = st[0];
+  const Op& E = et[0];
+  BasisView SB, EB;
+  read_basis(sys_bs, SB);
+  read_basis(env_bs, EB);
+  auto sectors = [&](int n, const int* w, std::vector<WSec>& out, int64_t& total) -> bool {
+    out.resize(n);
+    total = 0;
+    for (int i = 0; i < n; ++i) {
+      const int* q = w + 6 * i;
+      if (q[4] < 0 || q[5] < 0) return false;
+      out[i] = {q[0], q[1], q[2], q[3], q[4], q[5], total};
+      total += (int64_t)q[4] * q[5];
+    }
+    return true;
+  };
+  std::vector<WSec> ow, nwv;
+  int64_t dold = 0, dnew = 0;
+  if (!sectors(nw_old, wsec_old, ow, dold) || !sectors(nw_new, wsec_new, nwv, dnew)) {
+    msg = "negative sector dimension";
+    return TJSQ_ERR_ARG;
+  }
+  const int64_t eoff = sys_trun.ndata;   // env_trun's data follows sys_trun's
+  pg.x_elems = dold;
+  pg.ops_elems = sys_trun.ndata + env_trun.ndata;
+  pg.normalize_out = true;
+  // output: one sub-block per new sector; work blocks are the fused
+  // sub-blocks along the side that gained the site (LR: system rows, RL:
+  // environment columns), which tile each sector exactly
+  const BasisView& FB = dir == 0 ? SB : EB;
+  std::vector<std::vector<int>> ob_of(nw_new);
+  for (int w = 0; w < nw_new; ++w) {
+    const WSec& n = nwv[w];
+    pg.out.sub.insert(pg.out.sub.end(), {n.sys_up, n.sys_down, 0, n.sys_dim, n.env_dim});
+    pg.out.off.push_back(n.off);
+    const int fs = dir == 0 ? first_sector(FB, n.sys_up, n.sys_down) : first_sector(FB, n.env_up, n.env_down);
+    const int extent = dir == 0 ? n.sys_dim : n.env_dim;
+    int covered = 0;
+    if (fs >= 0)
+      for (const MapSub& m : FB.sec[fs].sub) {
+        if (m.spos != covered || m.sdim < 0) { covered = -1; break; }
+        covered += m.sdim;
+      }
+    if (fs < 0 || covered != extent) {
+      msg = "a new wavefunction sector disagrees with its fused basis sector";
+      return TJSQ_ERR_ARG;
+    }
+    for (size_t z = 0; z < FB.sec[fs].sub.size(); ++z) {
+      const MapSub& m = FB.sec[fs].sub[z];
+      ob_of[w].push_back((int)pg.hp.out_blocks.size());
+      if (dir == 0)
+        pg.hp.out_blocks.push_back({n.off + m.spos, std::max(1, n.sys_dim), m.sdim, n.env_dim, w});
+      else
+        pg.hp.out_blocks.push_back({n.off + (int64_t)m.spos * n.sys_dim, std::max(1, n.sys_dim), n.sys_dim, m.sdim, w});
+    }
+  }
+  pg.out.nops = 1;
+  pg.out.hdr = {0, 0, 0, nw_new};
+  pg.out.ndata = dnew;
+  auto find_new = [&](int su, int sd, int eu, int ed) -> int {
+    for (int w = 0; w < nw_new; ++w)
+      if (nwv[w].sys_up == su && nwv[w].sys_down == sd && nwv[w].env_up == eu && nwv[w].env_down == ed) return w;
+    return -1;
+  };
+  const int SUs = 2;   // spin step (common.f90:11)
+  for (int i = 0; i < nw_old; ++i) {
+    const WSec& o = ow[i];
+    if (dir == 0) {
+      const int sid = first_sub(S, o.sys_up, o.sys_down);
+      if (sid < 0) continue;
+      for (const MapSec& k : EB.sec) {
+        if (k.new_up != o.env_up || k.new_down != o.env_down) continue;
+        for (const MapSub& x : k.sub) {
+          const int eid = first_sub(E, x.bl_up, x.bl_down);
+          for (int nsd = std::abs(o.sys_down - x.st_down); nsd <= o.sys_down + x.st_down; nsd += SUs) {
+            const int nsu = o.sys_up + x.st_up;
+            int y = -1, z = -1;
+            const bool fused = find_fused(SB, nsu, nsd, o.sys_up, o.sys_down, x.st_up, x.st_down, y, z);
+            const int w = find_new(nsu, nsd, x.bl_up, x.bl_down);
+            if (eid < 0 || !fused || w < 0) continue;
+            const double c = coef_lr(o.sys_down, x.st_down, nsd, x.bl_down, o.env_down);
+            if (c == 0.0) continue;
+            const OpSub& A = S.sub[sid];
+            const OpSub& B = E.sub[eid];
+            const MapSub& f = SB.sec[y].sub[z];
+            if (A.row_dim != o.sys_dim || A.sdim != f.sdim || B.row_dim != nwv[w].env_dim || B.sdim != x.sdim ||
+                x.spos + x.sdim > o.env_dim) {
+              msg = "truncation operator dimensions disagree with the wavefunction sectors";
+              return TJSQ_ERR_ARG;
+            }
+            if (f.sdim == 0 || o.sys_dim == 0 || x.sdim == 0 || nwv[w].env_dim == 0) continue;
+            Contrib q{};
+            q.kind = C_BOTH;
+            q.out_block = ob_of[w][z];
+            q.c = c;
+            q.M = f.sdim; q.K1 = o.sys_dim; q.K2 = x.sdim; q.N = nwv[w].env_dim;
+            q.a_off = A.off; q.a_ld = std::max(1, A.row_dim); q.a_t = 1;
+            q.x_off = o.off + (int64_t)x.spos * o.sys_dim; q.x_ld = std::max(1, o.sys_dim); q.x_t = 0;
+            q.b_off = eoff + B.off; q.b_ld = std::max(1, B.row_dim); q.b_t = 1;
+            pg.hp.contrib.push_back(q);
+          }
+        }
+      }
+    } else {
+      const int eid = first_sub(E, o.env_up, o.env_down);
+      if (eid < 0) continue;
+      for (const MapSec& k : SB.sec) {
+        if (k.new_up != o.sys_up || k.new_down != o.sys_down) continue;
+        for (const MapSub& x : k.sub) {
+          for (int ned = std::abs(o.env_down - x.st_down); ned <= o.env_down + x.st_down; ned += SUs) {
+            const int neu = o.env_up + x.st_up;
+            const int sid = first_sub(S, x.bl_up, x.bl_down);
+            int y = -1, z = -1;
+            const bool fused = find_fused(EB, neu, ned, o.env_up, o.env_down, x.st_up, x.st_down, y, z);
+            const int w = find_new(x.bl_up, x.bl_down, neu, ned);
+            if (sid < 0 || !fused || w < 0) continue;
+            const double c = coef_rl(o.env_down, x.st_down, x.bl_down, ned, o.sys_down);
+            if (c == 0.0) continue;
+            const OpSub& A = S.sub[sid];
+            const OpSub& B = E.sub[eid];
+            const MapSub& f = EB.sec[y].sub[z];
+            if (A.row_dim != nwv[w].sys_dim || A.sdim != x.sdim || B.row_dim != o.env_dim || B.sdim != f.sdim ||
+                x.spos + x.sdim > o.sys_dim) {
+              msg = "truncation operator dimensions disagree with the wavefunction sectors";
+              return TJSQ_ERR_ARG;
+            }
+            if (nwv[w].sys_dim == 0 || x.sdim == 0 || o.env_dim == 0 || f.sdim == 0) continue;
+            Contrib q{};
+            q.kind = C_BOTH;
+            q.out_block = ob_of[w][z];
+            q.c = c;
+            q.M = nwv[w].sys_dim; q.K1 = x.sdim; q.K2 = o.env_dim; q.N = f.sdim;
+            q.a_off = A.off; q.a_ld = std::max(1, A.row_dim); q.a_t = 0;
+            q.x_off = o.off + x.spos; q.x_ld = std::max(1, o.sys_dim); q.x_t = 0;
+            q.b_off = eoff + B.off; q.b_ld = std::max(1, B.row_dim); q.b_t = 0;
+            pg.hp.contrib.push_back(q);
+          }
+        }
+      }
+    }
+  }
   Ctx cx;
   return finish(pg, shape, cx, msg);
 }

@@ -43,7 +43,22 @@ struct Program {
   int64_t x_elems = 0;        // elements the X operand must hold
   int64_t ops_elems = 0;      // elements the SP_OPS operand must hold
   int levels = 1;             // density programs: states summed before use
+  bool normalize_out = false; // wave transforms: the output is normalised
 };
+
+// wave_trans_LR (dir 0, code/tjsq/dmrg_sub.f90:1093-1238) / wave_trans_RL
+// (dir 1, :1241-1384): the ground state of one bond carried to the next bond
+// of a sweep, the start vector of the next solve.  sys_bs / env_bs are the
+// fused bases the reference passes (LR: the new system basis and the old
+// environment basis; RL: the old system basis and the new environment
+// basis), sys_trun / env_trun one operator each, wsec_old / wsec_new the
+// sector lists (6 ints each, Get_super_basis order) of the two wavefunctions.
+// The program's operator space is sys_trun's data followed by env_trun's,
+// its input the old wavefunction, its output the new one, normalised.
+int build_wave_transform_program(int dir, const tjsq_basis& sys_bs, const tjsq_basis& env_bs,
+                                 const tjsq_opset& sys_trun, const tjsq_opset& env_trun, int nw_old,
+                                 const int* wsec_old, int nw_new, const int* wsec_new, const Shape& shape,
+                                 Program& pg, std::string& msg);
 
 // Get_Hamiltonian_Sys (side 0) / Get_Hamiltonian_Env (side 1): pre is the
 // block model at length L (nops = 1 + 4 * active sites), basis the fused

@@ -320,6 +320,19 @@ int tjsq_trun_project_create(const tjsq_opset* ops, const int* ndia, const tjsq_
  * 4-134).  The output operator has one sub-block per wavefunction sector. */
 int tjsq_density_create(int side, int len, int nw, const int* wsec, int levels, tjsq_program** prog);
 
+/* wave_trans_LR (dir 0) / wave_trans_RL (dir 1) (code/tjsq/dmrg_sub.f90:
+ * 1093-1238, 1241-1384): the ground state of one bond carried to the next
+ * bond of a sweep, the start vector of the next solve (the "initial guess"
+ * step before lanczos_memo_new).  sys_bs / env_bs are the fused bases the
+ * reference routine takes (LR: new system, old environment; RL: old system,
+ * new environment); sys_trun / env_trun hold one operator each;
+ * wsec_old / wsec_new are the sector lists (6 ints each) of the old and the
+ * new wavefunction.  Run with ops = sys_trun data followed by env_trun data,
+ * x = the old wavefunction, y = the new one (normalised, wave_normalize). */
+int tjsq_wave_transform_create(int dir, const tjsq_basis* sys_bs, const tjsq_basis* env_bs, const tjsq_opset* sys_trun,
+                               const tjsq_opset* env_trun, int nw_old, const int* wsec_old, int nw_new,
+                               const int* wsec_new, tjsq_program** prog);
+
 int tjsq_program_sizes(const tjsq_program* prog, int64_t* sizes);
 int tjsq_program_layout(const tjsq_program* prog, int* hdr, int* sub, int64_t* off);
 int tjsq_program_tables(const tjsq_program* prog, void* seg1, void* tile1, void* seg2, void* tile2, void* rtile);

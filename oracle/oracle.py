@@ -95,6 +95,10 @@ def oracle_lib():
             lib.oracle_density_matrix.argtypes = [C.c_int, C.c_int, C.c_int, I32P, C.c_int, F64P]
             lib.oracle_quicksort_double.restype = None
             lib.oracle_quicksort_double.argtypes = [F64P, I32P, C.c_int]
+            lib.oracle_wave_trans.restype = C.c_int
+            lib.oracle_wave_trans.argtypes = ([C.c_int, C.c_int, I32P, I32P, C.c_int, I32P, I32P]
+                                              + [I32P, I32P, I64P, F64P] * 2
+                                              + [C.c_int, I32P, F64P, C.c_int, I32P, F64P])
             lib.oracle_truncation_select.restype = C.c_int
             lib.oracle_truncation_select.argtypes = [C.c_int, I32P, F64P, C.c_int, C.c_int, C.c_double, I32P, F64P,
                                                      F64P, I32P]
@@ -143,6 +147,10 @@ def ref_lib():
                                             C.POINTER(C.c_int), C.POINTER(C.c_double)]
             lib.refb_quicksort.restype = C.c_int
             lib.refb_quicksort.argtypes = [C.c_int, F64P, I32P]
+            lib.refb_wave_trans.restype = C.c_int
+            lib.refb_wave_trans.argtypes = ([C.c_int, C.c_int, I32P, I32P, C.c_int, I32P, I32P]
+                                            + [I32P, I32P, I64P, F64P] * 2
+                                            + [C.c_int, I32P, F64P, C.c_int, I32P, F64P])
             lib.refb_flush.restype = C.c_int
             lib.refb_flush.argtypes = []
             _libs["ref"] = lib
@@ -367,6 +375,24 @@ def density_matrix(side, length, wsec, psi, levels=1):
     return _take_oracle(h, length)
 
 
+def _wave_trans_args(direction, sys_bs, env_bs, sys_trun, env_trun, wsec_old, x, wsec_new):
+    wo = np.ascontiguousarray(wsec_old, np.int32).reshape(-1, 6)
+    wn = np.ascontiguousarray(wsec_new, np.int32).reshape(-1, 6)
+    dn = int((wn[:, 4].astype(np.int64) * wn[:, 5]).sum())
+    y = np.zeros(max(dn, 1))
+    args = ([direction] + _bs(sys_bs) + _bs(env_bs) + _one(sys_trun) + _one(env_trun)
+            + [wo.shape[0], wo, np.ascontiguousarray(x, np.float64), wn.shape[0], wn, y])
+    return args, y[:dn]
+
+
+def wave_trans(direction, sys_bs, env_bs, sys_trun, env_trun, wsec_old, x, wsec_new):
+    """Oracle wave_trans_LR (direction 0) / wave_trans_RL (1) + wave_normalize."""
+    args, y = _wave_trans_args(direction, sys_bs, env_bs, sys_trun, env_trun, wsec_old, x, wsec_new)
+    if oracle_lib().oracle_wave_trans(*args) != 0:
+        raise ValueError("truncation operator dimensions disagree with the wavefunction sectors")
+    return y
+
+
 def quicksort_double(arr):
     """Oracle QuickSort_Double: (values, 1-based permutation)."""
     a = np.ascontiguousarray(arr, np.float64).copy()
@@ -436,6 +462,14 @@ def ref_update_trun(ops, ndia, trun):
         lib = ref_lib()
         lib.refb_update_trun(*_m(ops), np.ascontiguousarray(ndia, np.int32), *_one(trun))
         return _take_ref()
+
+
+def ref_wave_trans(direction, sys_bs, env_bs, sys_trun, env_trun, wsec_old, x, wsec_new):
+    """wave_trans_LR / wave_trans_RL of the reference build."""
+    args, y = _wave_trans_args(direction, sys_bs, env_bs, sys_trun, env_trun, wsec_old, x, wsec_new)
+    with _lock:
+        ref_lib().refb_wave_trans(*args)
+    return y
 
 
 def ref_density_matrix(side, sys_len, env_len, wsec, psi, levels=1):

@@ -633,3 +633,48 @@ integer(c_int) function refb_flush() bind(C, name="refb_flush")
   flush(6)
   refb_flush = 0
 end function refb_flush
+
+! wave_trans_LR (dir 0) / wave_trans_RL (dir 1) (dmrg_sub.f90:1093-1384) from
+! the old wavefunction x into y (sectors wsec_new, normalised by the routine).
+integer(c_int) function refb_wave_trans(dir, sys_nsec, sys_bsec, sys_bsub, env_nsec, env_bsec, env_bsub, &
+    st_hdr, st_sub, st_off, st_data, et_hdr, et_sub, et_off, et_data, nw_old, wsec_old, x, nw_new, wsec_new, y) &
+    bind(C, name="refb_wave_trans")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: dir, sys_nsec, env_nsec, nw_old, nw_new
+  integer(c_int), intent(in) :: sys_bsec(4, *), sys_bsub(6, *), env_bsec(4, *), env_bsub(6, *)
+  integer(c_int), intent(in) :: st_hdr(4), st_sub(5, *), et_hdr(4), et_sub(5, *), wsec_old(6, *), wsec_new(6, *)
+  integer(c_int64_t), intent(in) :: st_off(*), et_off(*)
+  real(c_double), intent(in) :: st_data(*), et_data(*), x(*)
+  real(c_double), intent(inout) :: y(*)
+  type(Total_Basis) :: sbs, ebs
+  type(Total_Block) :: st, et
+  type(Wavefunction) :: ow, nw
+  real(8), allocatable :: zero(:)
+  integer(8) :: dn
+  integer :: i
+  call put_basis(sbs, 0, sys_nsec, sys_bsec, sys_bsub)
+  call put_basis(ebs, 0, env_nsec, env_bsec, env_bsub)
+  call put_block(st, st_hdr(1), st_hdr, st_sub, st_off, st_data)
+  call put_block(et, et_hdr(1), et_hdr, et_sub, et_off, et_data)
+  call put_wave(ow, 0, 0, nw_old, wsec_old, x)
+  dn = 0
+  do i = 1, nw_new
+    dn = dn + int(wsec_new(5, i), 8) * wsec_new(6, i)
+  end do
+  allocate(zero(max(dn, 1_8)))
+  zero = 0.0d0
+  call put_wave(nw, 0, 0, nw_new, wsec_new, zero)
+  if (dir == 0) then
+    call wave_trans_LR(sbs, ebs, st, et, ow, nw)
+  else
+    call wave_trans_RL(sbs, ebs, st, et, ow, nw)
+  end if
+  call get_wave(nw, y)
+  call free_block(st)
+  call free_block(et)
+  deallocate(sbs%sub, ebs%sub, ow%sub, nw%sub, zero)
+  refb_wave_trans = 0
+end function refb_wave_trans

@@ -1885,3 +1885,176 @@ int oracle_truncation_select(int nsec, const int* den_sub, const double* eigval,
   free(ev); free(idx); free(eigs); free(sec);
   return num;
 }
+
+/* ------------------------------------------------------------------------
+ * wave_trans_LR / wave_trans_RL (dmrg_sub.f90:1093-1238, 1241-1384) and
+ * wave_normalize (block.f90:2397-2432)
+ * ------------------------------------------------------------------------ */
+static double wt_negpow(int n) { return (n % 2 == 0) ? 1.0 : -1.0; }
+
+/* first sub-block of operator (hdr, sub) with ket (up, down); -1 if none */
+static int wt_first_sub(const int* hdr, const int* sub, int up, int down) {
+  for (int k = 0; k < hdr[3]; ++k)
+    if (sub[5 * k] == up && sub[5 * k + 1] == down) return k;
+  return -1;
+}
+
+/* first sector of a basis with (up, down) holding sub-block (bl, st); the
+ * sub-block's row in bsub is returned through *q */
+static int wt_find_fused(int nsec, const int* bsec, const int* bsub, int up, int down, int blu, int bld, int stu,
+                         int std_, int* q) {
+  int first = 0;
+  for (int y = 0; y < nsec; ++y) {
+    const int* s = bsec + 4 * y;
+    if (s[0] == up && s[1] == down)
+      for (int z = 0; z < s[3]; ++z) {
+        const int* m = bsub + 6 * (first + z);
+        if (m[0] == blu && m[1] == bld && m[2] == stu && m[3] == std_) {
+          *q = first + z;
+          return y;
+        }
+      }
+    first += s[3];
+  }
+  return -1;
+}
+
+int oracle_wave_trans(int dir, int sys_nsec, const int* sys_bsec, const int* sys_bsub, int env_nsec,
+                      const int* env_bsec, const int* env_bsub, const int* st_hdr, const int* st_sub,
+                      const int64_t* st_off, const double* st_data, const int* et_hdr, const int* et_sub,
+                      const int64_t* et_off, const double* et_data, int nw_old, const int* wsec_old,
+                      const double* x, int nw_new, const int* wsec_new, double* y) {
+  int64_t* ooff = (int64_t*)malloc(sizeof(int64_t) * (nw_old > 0 ? nw_old : 1));
+  int64_t* noff = (int64_t*)malloc(sizeof(int64_t) * (nw_new > 0 ? nw_new : 1));
+  int64_t t = 0;
+  for (int i = 0; i < nw_old; ++i) { ooff[i] = t; t += (int64_t)wsec_old[6 * i + 4] * wsec_old[6 * i + 5]; }
+  int64_t dn = 0;
+  for (int i = 0; i < nw_new; ++i) { noff[i] = dn; dn += (int64_t)wsec_new[6 * i + 4] * wsec_new[6 * i + 5]; }
+  for (int64_t e = 0; e < dn; ++e) y[e] = 0.0;
+  int rc = 0;
+  for (int i = 0; i < nw_old && rc == 0; ++i) {
+    const int* o = wsec_old + 6 * i;
+    const int su = o[0], sd = o[1], eu = o[2], ed = o[3], osd = o[4], oed = o[5];
+    /* the side whose truncation operator is keyed by the old sector, and the
+       fused basis walked for the moving site */
+    const int tid = dir == 0 ? wt_first_sub(st_hdr, st_sub, su, sd) : wt_first_sub(et_hdr, et_sub, eu, ed);
+    if (tid < 0) continue;
+    const int wnsec = dir == 0 ? env_nsec : sys_nsec;
+    const int* wbsec = dir == 0 ? env_bsec : sys_bsec;
+    const int* wbsub = dir == 0 ? env_bsub : sys_bsub;
+    int first = 0;
+    for (int k = 0; k < wnsec && rc == 0; first += wbsec[4 * k + 3], ++k) {
+      const int* sec = wbsec + 4 * k;
+      if (dir == 0 ? (sec[0] != eu || sec[1] != ed) : (sec[0] != su || sec[1] != sd)) continue;
+      for (int xx = 0; xx < sec[3] && rc == 0; ++xx) {
+        const int* m = wbsub + 6 * (first + xx);
+        const int blu = m[0], bld = m[1], stu = m[2], std_ = m[3], opos = m[4], osub = m[5];
+        const int other = dir == 0 ? ed : sd;   /* 2S that couples with the site on the receiving side */
+        for (int nd = abs(dir == 0 ? sd - std_ : ed - std_); nd <= (dir == 0 ? sd : ed) + std_; nd += 2) {
+          (void)other;
+          int q = -1, w = -1, id2, y2;
+          if (dir == 0) {
+            const int nsu = su + stu;
+            id2 = wt_first_sub(et_hdr, et_sub, blu, bld);
+            y2 = wt_find_fused(sys_nsec, sys_bsec, sys_bsub, nsu, nd, su, sd, stu, std_, &q);
+            for (int v = 0; v < nw_new; ++v) {
+              const int* n = wsec_new + 6 * v;
+              if (n[0] == nsu && n[1] == nd && n[2] == blu && n[3] == bld) { w = v; break; }
+            }
+          } else {
+            const int neu = eu + stu;
+            id2 = wt_first_sub(st_hdr, st_sub, blu, bld);
+            y2 = wt_find_fused(env_nsec, env_bsec, env_bsub, neu, nd, eu, ed, stu, std_, &q);
+            for (int v = 0; v < nw_new; ++v) {
+              const int* n = wsec_new + 6 * v;
+              if (n[0] == blu && n[1] == bld && n[2] == neu && n[3] == nd) { w = v; break; }
+            }
+          }
+          if (id2 < 0 || y2 < 0 || w < 0) continue;
+          const int* n = wsec_new + 6 * w;
+          const int nsd_ = n[4], ned_ = n[5];
+          const int npos = (dir == 0 ? sys_bsub : env_bsub)[6 * q + 4];
+          const int nsub = (dir == 0 ? sys_bsub : env_bsub)[6 * q + 5];
+          double c;
+          if (dir == 0) {   /* j12 = sd, j3 = std_, j123 = nd, j4 = bld, j34 = ed (:1201-1214) */
+            const int j12 = sd, j3 = std_, j123 = nd, j4 = bld, j34 = ed;
+            c = wt_negpow((j12 + j3 + j123) / 2) / sqrt((1.0 + j12) * (1.0 + j123));
+            if (c == 0.0) continue;
+            c = c * sqrt((1.0 + j123) * (j34 + 1.0));
+            if (((j12 + j3 + j4) / 2) % 2 != 0) c = -c;
+            if (((j3 + j4 - j34) / 2) % 2 != 0) c = -c;
+          } else {          /* j12 = ed, j3 = std_, j4 = bld, j123 = nd, j34 = sd (:1348-1361) */
+            const int j12 = ed, j3 = std_, j4 = bld, j123 = nd, j34 = sd;
+            c = wt_negpow((j34 + j4 + j3) / 2) / sqrt((1.0 + j12) * (1.0 + j4));
+            if (c == 0.0) continue;
+            c = c * sqrt((1.0 + j123) * (j34 + 1.0));
+            if (((j4 + j3 + j12) / 2) % 2 != 0) c = -c;
+            if (((j12 + j3 - j123) / 2) % 2 != 0) c = -c;
+          }
+          const int* A = dir == 0 ? st_sub + 5 * tid : st_sub + 5 * id2;   /* sys_trun sub-block */
+          const int* B = dir == 0 ? et_sub + 5 * id2 : et_sub + 5 * tid;   /* env_trun sub-block */
+          const double* Ad = st_data + (dir == 0 ? st_off[tid] : st_off[id2]);
+          const double* Bd = et_data + (dir == 0 ? et_off[id2] : et_off[tid]);
+          if (dir == 0) {
+            /* sub = c A^T X(:, opos:opos+osub), A: osd x nsub; new(npos.., :) += sub B^T, B: ned_ x osub */
+            if (A[3] != osd || A[4] != nsub || B[3] != ned_ || B[4] != osub) { rc = -1; break; }
+            double* sub = (double*)calloc((size_t)nsub * osub + 1, sizeof(double));
+            const double* X = x + ooff[i] + (int64_t)opos * osd;
+            for (int cc = 0; cc < osub; ++cc)
+              for (int r = 0; r < nsub; ++r) {
+                double s = 0.0;
+                for (int k2 = 0; k2 < osd; ++k2) s += Ad[k2 + (int64_t)r * osd] * X[k2 + (int64_t)cc * osd];
+                sub[r + (int64_t)cc * nsub] = c * s;
+              }
+            double* Y = y + noff[w];
+            for (int cc = 0; cc < ned_; ++cc)
+              for (int r = 0; r < nsub; ++r) {
+                double s = 0.0;
+                for (int k2 = 0; k2 < osub; ++k2) s += sub[r + (int64_t)k2 * nsub] * Bd[cc + (int64_t)k2 * ned_];
+                Y[npos + r + (int64_t)cc * nsd_] += s;
+              }
+            free(sub);
+          } else {
+            /* sub = c A X(opos:opos+osub, :), A: nsd_ x osub; new(:, npos..) += sub B, B: oed x nsub */
+            if (A[3] != nsd_ || A[4] != osub || B[3] != oed || B[4] != nsub) { rc = -1; break; }
+            double* sub = (double*)calloc((size_t)nsd_ * oed + 1, sizeof(double));
+            const double* X = x + ooff[i] + opos;
+            for (int cc = 0; cc < oed; ++cc)
+              for (int r = 0; r < nsd_; ++r) {
+                double s = 0.0;
+                for (int k2 = 0; k2 < osub; ++k2) s += Ad[r + (int64_t)k2 * nsd_] * X[k2 + (int64_t)cc * osd];
+                sub[r + (int64_t)cc * nsd_] = c * s;
+              }
+            double* Y = y + noff[w];
+            for (int cc = 0; cc < nsub; ++cc)
+              for (int r = 0; r < nsd_; ++r) {
+                double s = 0.0;
+                for (int k2 = 0; k2 < oed; ++k2) s += sub[r + (int64_t)k2 * nsd_] * Bd[k2 + (int64_t)cc * oed];
+                Y[r + (int64_t)(npos + cc) * nsd_] += s;
+              }
+            free(sub);
+          }
+        }
+      }
+    }
+  }
+  if (rc == 0) {   /* wave_normalize: norm summed sector by sector, column by column */
+    double nrm = 0.0;
+    for (int v = 0; v < nw_new; ++v) {
+      const int rows = wsec_new[6 * v + 4], cols = wsec_new[6 * v + 5];
+      for (int cc = 0; cc < cols; ++cc) {
+        double dcol = 0.0;
+        for (int r = 0; r < rows; ++r) {
+          const double a = y[noff[v] + r + (int64_t)cc * rows];
+          dcol += a * a;
+        }
+        nrm += dcol;
+      }
+    }
+    nrm = sqrt(nrm);
+    for (int64_t e = 0; e < dn; ++e) y[e] = y[e] / nrm;
+  }
+  free(ooff);
+  free(noff);
+  return rc;
+}

@@ -119,6 +119,18 @@ void oracle_quicksort_double(double* arr, int* idx, int n);
 int oracle_truncation_select(int nsec, const int* den_sub, const double* eigval, int kept_min, int kept_max,
                              double trun_err, int* counts, double* info, double* sorted, int* order);
 
+/* wave_trans_LR (dir 0, dmrg_sub.f90:1093-1238) / wave_trans_RL (dir 1,
+ * :1241-1384) followed by wave_normalize (block.f90:2418-2432): the new
+ * wavefunction y (sectors wsec_new) from the old one x (sectors wsec_old),
+ * the fused bases the reference routine takes and the two truncation
+ * operators (st_* = sys_trun, et_* = env_trun, one operator each).
+ * Returns 0, or -1 when operator dimensions disagree with the sectors. */
+int oracle_wave_trans(int dir, int sys_nsec, const int* sys_bsec, const int* sys_bsub, int env_nsec,
+                      const int* env_bsec, const int* env_bsub, const int* st_hdr, const int* st_sub,
+                      const int64_t* st_off, const double* st_data, const int* et_hdr, const int* et_sub,
+                      const int64_t* et_off, const double* et_data, int nw_old, const int* wsec_old,
+                      const double* x, int nw_new, const int* wsec_new, double* y);
+
 #ifdef __cplusplus
 }
 #endif
