@@ -106,8 +106,9 @@ def timed_solves(plan, x0_t, steps, warmup, precision, tol, max_iter, torch, ene
 def block_update_step(model, sb, psi, m_keep, torch, reps=5):
     """The step after each solve on the device: Get_density_matrix_sys ->
     Get_truncation_operator (kept_min = m/2, kept_max = m, trun_err 1e-8) ->
-    Get_Hamiltonian_Sys of the system block at this bond.  Times each part
-    with events on the launching stream (the current torch stream)."""
+    Get_Hamiltonian_Sys of the system block at this bond -> wave_trans_LR
+    of the ground state to the next bond.  Times each part with events on
+    the launching stream (the current torch stream)."""
     from grand_sql_tj_amd import capi
     st = torch.cuda.current_stream()
 
@@ -147,7 +148,37 @@ def block_update_step(model, sb, psi, m_keep, torch, reps=5):
     pgu.run(rho, pre, out)
     ms_update = timed(lambda: pgu.run(rho, pre, out), reps)
     fl = pgu.stats()["flops_exec"]
+    # wave_trans_LR to the next bond: the system truncation just computed,
+    # an orthogonal env truncation per env sector (square: the synthetic
+    # environment's previous block has the same sector dims), psi as input
+    from grand_sql_tj_amd.blocks import Basis, get_basis, get_super_basis, site_sectors
+    kept = np.stack([tsub[:, 0], tsub[:, 1], tsub[:, 4]], axis=1).astype(np.int32)
+    sys_new_bs = get_basis(sb.sys_len + 1, kept, site_sectors(model.up_bias))
+    env_secs = sb.env.sectors()
+    env_new = Basis(sb.env_len - 1, np.column_stack([env_secs, np.zeros(len(env_secs), np.int32)]).astype(np.int32),
+                    np.zeros((0, 6), np.int32))
+    wsec_new = get_super_basis(sys_new_bs, env_new, int(sb.meta["total_up"]), 0)
+    rng = np.random.default_rng(5)
+    e_sub, e_off, e_dat, nd = [], [], [], 0
+    for u, d, n in env_secs:
+        q, _ = np.linalg.qr(rng.normal(size=(int(n), int(n))))
+        e_sub.append((u, d, 0, n, n))
+        e_off.append(nd)
+        e_dat.append(q.T.reshape(-1))
+        nd += int(n) * int(n)
+    env_trun = OpSet(sb.env_len, np.array([[sb.env_len, 0, 0, len(e_sub)]], np.int32), np.array(e_sub, np.int32),
+                     np.array(e_off, np.int64), np.concatenate(e_dat))
+    nt = int((toff + tsub[:, 3].astype(np.int64) * tsub[:, 4]).max()) if tsub.shape[0] else 0
+    sys_trun = OpSet(sb.sys_len + 1, trun.hdr, tsub, toff, np.zeros(nt))
+    pgw = capi.Program.wave_transform(0, sys_new_bs, sb.env_bs, sys_trun, env_trun, sb.wsec, wsec_new)
+    wops = torch.cat([rho[:nt], torch.from_numpy(env_trun.data).cuda()])
+    psi_new = torch.empty(max(1, pgw.sizes["ndata"]), dtype=torch.float64, device="cuda")
+    pgw.run(wops, psi, psi_new)
+    ms_wave = timed(lambda: pgw.run(wops, psi, psi_new), reps)
+    wfl = pgw.stats()["flops_exec"]
     return {"density_ms": ms_density, "truncation_ms": ms_trunc, "update_ms": ms_update,
+            "wave_transform_ms": ms_wave, "wave_transform_flops": wfl, "wave_new_dim": pgw.sizes["ndata"],
+            "wave_transform_tflops": wfl / (ms_wave * 1e-3) / 1e12,
             "update_build_s": build_s, "kept": info["num_state"], "rho_dim": info["dim"],
             "trunc_err": info["trunc_err"], "update_flops": fl,
             "update_tflops": fl / (ms_update * 1e-3) / 1e12, "update_frac_fp64_peak": fl / (ms_update * 1e-3) / PEAK["fp64"],
