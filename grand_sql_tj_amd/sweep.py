@@ -1,0 +1,267 @@
+"""Two-site SU(2) DMRG on the device: the warm-up and the finite sweeps of
+the reference (warm_up_point, left_to_right, right_to_left;
+code/tjsq/dmrg_sub.f90:556-1087, scheduled as Get_Ground_State,
+code/tjsq/main.f90:96-187), assembled from the library's device calls.
+
+Each step at a bond [sys | site | site | env]:
+
+    tjsq_plan_create (hmerge)          -> tjsq_lanczos_solve (lanczos_memo_new)
+    tjsq_density_create / _run         -> tjsq_truncation_operator (or identity)
+    tjsq_block_update_create / _run    (Get_Hamiltonian_Sys / _Env)
+    tjsq_wave_transform_create / _run  (wave_trans_LR / _RL, the next start vector)
+
+Block models, truncation operators and the fused bases are kept per block
+length, as the reference keeps them on disk (model_to_disk, truns_to_disk,
+basis_to_disk); here the operator data stay in device memory and only their
+layouts live on the host.  The reference's random start vectors
+(wave_initialize, unseeded random_number) are seeded here.
+
+The schedule is the reference's with fixed settings: warm-up to half the
+lattice (the blocks grow together, with truncation once an enlarged block
+exceeds kept_max), then left-to-right and right-to-left half sweeps between
+the block lengths 1 and N - 3, `sweeps` times.  The tolerance and kept-state
+ramps of Get_Ground_State are left out: every step uses kept_max, kept_min
+and trun_err as given.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+from .blocks import Basis, OpSet, OpSetBuilder, Superblock, get_basis, get_super_basis, site_sectors
+
+
+def site_model(model) -> OpSet:
+    """Block model of one site (Get_hamiltonian_site with the site operators
+    of Get_site_operator_su2_U0, dmrg_sub.f90:370-517): the Hamiltonian is
+    esite * n (warm_up_point, dmrg_sub.f90:576-580), one active site."""
+    ub = model.up_bias
+    s2 = math.sqrt(2.0)
+    b = OpSetBuilder(1)
+    b.add(0, 0, [(0, 0, 0, [[0.0]]), (ub, 1, 0, [[model.esite * s2]])])
+    b.add(ub, 1, [(0, 0, 1, [[-s2]])])                 # elec_up: c+ on the empty site
+    b.add(-ub, 1, [(ub, 1, -1, [[-s2]])])              # elec_down
+    b.add(0, 2, [(ub, 1, 0, [[math.sqrt(1.5)]])])      # spin_sd
+    b.add(0, 0, [(0, 0, 0, [[0.0]]), (ub, 1, 0, [[s2]])])   # num_sn
+    return b.build()
+
+
+@dataclass
+class DeviceBlock:
+    """A block model: operator-set layout on the host, data on the device."""
+    ops: OpSet            # layout (its data array is a placeholder of the right size)
+    data: object          # torch float64 CUDA tensor
+
+    @property
+    def length(self):
+        return self.ops.length
+
+
+@dataclass
+class DeviceTrun:
+    ops: OpSet            # one-operator layout
+    data: object
+
+
+@dataclass
+class StepRecord:
+    phase: str
+    sys_len: int
+    env_len: int
+    dim: int
+    energy: float
+    iterations: int
+    kept: int
+    trunc_err: float
+    seconds: float
+
+
+@dataclass
+class DMRG:
+    model: object
+    total_up: int
+    kept_max: int
+    kept_min: int = 0
+    trun_err: float = 1e-8
+    tol: float = 1e-10
+    max_iter: int = 40
+    precision: str = "fp64"
+    seed: int = 1
+    sys: dict = field(default_factory=dict)       # length -> DeviceBlock
+    env: dict = field(default_factory=dict)
+    sys_trun: dict = field(default_factory=dict)  # length L -> truncation that made block L
+    env_trun: dict = field(default_factory=dict)
+    sys_basis: dict = field(default_factory=dict)  # length L -> fused basis (block L-1 + site)
+    env_basis: dict = field(default_factory=dict)
+    log: list = field(default_factory=list)
+
+    def __post_init__(self):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("the device DMRG needs a GPU")
+        self.torch = torch
+        self.nsite = self.model.nsite
+        self.site = site_sectors(self.model.up_bias)
+        self.rng = np.random.default_rng(self.seed)
+        self.psi = None          # ground state of the last bond (torch float64)
+        self.wsec = None         # its sector list
+        if self.kept_min <= 0:
+            self.kept_min = min(3 * self.kept_max // 4, 3600)
+
+    # ---- helpers ---------------------------------------------------------
+    def _device(self, a):
+        return self.torch.from_numpy(np.ascontiguousarray(a, np.float64)).to("cuda")
+
+    def _block(self, ops: OpSet) -> DeviceBlock:
+        return DeviceBlock(OpSet(ops.length, ops.hdr, ops.sub, ops.off, np.zeros(ops.data.size)), self._device(ops.data))
+
+    def _basis(self, blk: DeviceBlock) -> Basis:
+        """Get_basis of the block with one site (the basis carries length + 1)."""
+        return get_basis(blk.length, blk.ops.sectors(), self.site)
+
+    def _solve(self, blk_s, blk_e, sys_bs, env_bs, wsec, psi0, phase):
+        import time
+        torch = self.torch
+        sb = Superblock(blk_s.length, blk_e.length, blk_s.ops, blk_e.ops, sys_bs, env_bs, wsec, meta={})
+        t0 = time.perf_counter()
+        plan = capi.Plan(self.model, sb, precisions=("fp64", self.precision), device_ops=(blk_s.data, blk_e.data))
+        if psi0 is None:
+            psi = self._device(self.rng.standard_normal(sb.dim))
+        else:
+            psi = psi0.clone()
+        psi /= psi.norm()
+        info = plan.lanczos(psi, tol=self.tol, max_iter=self.max_iter, precision=self.precision)
+        torch.cuda.synchronize()
+        self.psi, self.wsec = psi, wsec
+        return info, time.perf_counter() - t0
+
+    def _truncate(self, side, sys_len, env_len, wsec, basis, psi, allow_identity):
+        """rho of one side, then its truncation operator (identity while the
+        enlarged block fits in kept_max, as the warm-up does)."""
+        torch = self.torch
+        length = (sys_len if side == 0 else env_len) + 1
+        if allow_identity and int(basis.sec[:, 2].sum()) <= self.kept_max:
+            tsub, toff, nd = capi.identity_truncation(basis)
+            data = torch.empty(max(1, nd), dtype=torch.float64, device="cuda")
+            capi.identity_truncation(basis, data)
+            kept, err = int(basis.sec[:, 2].sum()), 0.0
+        else:
+            den = capi.Program.density(side, length, wsec, 1)
+            rho = torch.empty(max(1, den.sizes["ndata"]), dtype=torch.float64, device="cuda")
+            den.run(None, psi, rho)
+            _, dsub, doff = den.layout()
+            tsub, toff, _, info = capi.truncation_operator(dsub, doff, rho, self.kept_min, self.kept_max,
+                                                           self.trun_err)
+            data, kept, err = rho, info["num_state"], info["trunc_err"]
+        n = int((toff + tsub[:, 3].astype(np.int64) * tsub[:, 4]).max()) if tsub.shape[0] else 0
+        ops = OpSet(length, np.array([[length, 0, 0, tsub.shape[0]]], np.int32), tsub.astype(np.int32),
+                    toff.astype(np.int64), np.zeros(max(1, n)))
+        return DeviceTrun(ops, data), kept, err
+
+    def _grow(self, side, blk, basis, trun, sys_len, env_len) -> DeviceBlock:
+        """Get_Hamiltonian_Sys (side 0) / _Env (side 1): the block one site longer."""
+        torch = self.torch
+        pg = capi.Program.block_update(self.model, side, sys_len, env_len, blk.ops, basis, trun.ops)
+        hdr, sub, off = pg.layout()
+        out = torch.empty(max(1, pg.sizes["ndata"]), dtype=torch.float64, device="cuda")
+        pg.run(trun.data, blk.data, out)
+        ops = OpSet(blk.length + 1, hdr, sub, off, np.zeros(max(1, pg.sizes["ndata"])))
+        return DeviceBlock(ops, out)
+
+    def _transform(self, direction, sys_bs, env_bs, sys_trun, env_trun, wsec_new):
+        torch = self.torch
+        pg = capi.Program.wave_transform(direction, sys_bs, env_bs, sys_trun.ops, env_trun.ops, self.wsec, wsec_new)
+        ns = sys_trun.ops.data.size
+        ops = torch.cat([sys_trun.data[:ns], env_trun.data[:env_trun.ops.data.size]])
+        out = torch.empty(max(1, pg.sizes["ndata"]), dtype=torch.float64, device="cuda")
+        pg.run(ops, self.psi, out)
+        return out[:pg.sizes["ndata"]]
+
+    def _record(self, phase, sl, el, wsec, info, kept, err, secs):
+        d = int((wsec[:, 4].astype(np.int64) * wsec[:, 5]).sum())
+        self.log.append(StepRecord(phase, sl, el, d, info["e0"], info["iterations"], kept, err, secs))
+
+    # ---- warm_up_point (dmrg_sub.f90:556-725), bg_id = 1 ------------------
+    def warm_up(self):
+        site = site_model(self.model)
+        self.sys[1] = self._block(site)
+        self.env[1] = self._block(site)
+        label, end = 1, self.nsite // 2
+        while label + 1 <= end:
+            s, e = self.sys[label], self.env[label]
+            sys_bs, env_bs = self._basis(s), self._basis(e)
+            tot = (sys_bs.length + env_bs.length)
+            tot_up = int(1.0 * tot / self.nsite * self.total_up + 0.001)
+            tot_up -= tot_up % 2
+            tot_up = min(tot_up, self.total_up)
+            wsec = get_super_basis(sys_bs, env_bs, tot_up, 0)
+            info, secs = self._solve(s, e, sys_bs, env_bs, wsec, None, "warm-up")
+            t_s, k_s, err = self._truncate(0, label, label, wsec, sys_bs, self.psi, True)
+            t_e, _, _ = self._truncate(1, label, label, wsec, env_bs, self.psi, True)
+            self._record("warm-up", label, label, wsec, info, k_s, err, secs)
+            self.sys[label + 1] = self._grow(0, s, sys_bs, t_s, label, label)
+            self.env[label + 1] = self._grow(1, e, env_bs, t_e, label, label)
+            self.sys_trun[label + 1], self.env_trun[label + 1] = t_s, t_e
+            self.sys_basis[label + 1], self.env_basis[label + 1] = sys_bs, env_bs
+            label += 1
+
+    # ---- left_to_right (dmrg_sub.f90:728-907) -----------------------------
+    def left_to_right(self, bg_id, end_id):
+        N = self.nsite
+        label = bg_id
+        trun = None
+        while label + 1 <= end_id:
+            s, e = self.sys[label], self.env[N - label - 2]
+            sys_bs = self._basis(s)
+            env_bs = self.env_basis.get(N - label - 1) or self._basis(e)
+            wsec = get_super_basis(sys_bs, env_bs, self.total_up, 0)
+            psi0 = None
+            if self.wsec is not None and np.array_equal(self.wsec, wsec):
+                psi0 = self.psi               # the same bond as the last solve
+            elif trun is not None:
+                # wave_trans_LR(sys_bs, env_basis, sys_trun, env_trun, old, new)
+                psi0 = self._transform(0, sys_bs, self.env_basis[N - label], trun, self.env_trun[N - label - 1], wsec)
+            info, secs = self._solve(s, e, sys_bs, env_bs, wsec, psi0, "left-to-right")
+            trun, kept, err = self._truncate(0, label, N - label - 2, wsec, sys_bs, self.psi, False)
+            self._record("left-to-right", label, N - label - 2, wsec, info, kept, err, secs)
+            self.sys[label + 1] = self._grow(0, s, sys_bs, trun, label, N - label - 2)
+            self.sys_trun[label + 1], self.sys_basis[label + 1] = trun, sys_bs
+            label += 1
+
+    # ---- right_to_left (dmrg_sub.f90:910-1087) ----------------------------
+    def right_to_left(self, bg_id, end_id):
+        N = self.nsite
+        label = bg_id
+        trun = None
+        while label - 1 >= end_id:
+            s, e = self.sys[label - 1], self.env[N - label - 1]
+            sys_bs = self.sys_basis.get(label) or self._basis(s)
+            env_bs = self._basis(e)
+            wsec = get_super_basis(sys_bs, env_bs, self.total_up, 0)
+            psi0 = None
+            if self.wsec is not None and np.array_equal(self.wsec, wsec):
+                psi0 = self.psi
+            elif trun is not None:
+                # wave_trans_RL(sys_basis, env_bs, sys_trun, env_trun, old, new)
+                psi0 = self._transform(1, self.sys_basis[label + 1], env_bs, self.sys_trun[label], trun, wsec)
+            info, secs = self._solve(s, e, sys_bs, env_bs, wsec, psi0, "right-to-left")
+            trun, kept, err = self._truncate(1, label - 1, N - label - 1, wsec, env_bs, self.psi, False)
+            self._record("right-to-left", label - 1, N - label - 1, wsec, info, kept, err, secs)
+            self.env[N - label] = self._grow(1, e, env_bs, trun, label - 1, N - label - 1)
+            self.env_trun[N - label], self.env_basis[N - label] = trun, env_bs
+            label -= 1
+
+    def run(self, sweeps=2):
+        """Warm-up, then `sweeps` pairs of half sweeps (Get_Ground_State with
+        trun_idx = 1).  Returns the last ground-state energy."""
+        N = self.nsite
+        self.warm_up()
+        self.left_to_right(N // 2 - 1, N - 2)
+        self.right_to_left(N - 2, 1)
+        for _ in range(sweeps):
+            self.left_to_right(1, N - 2)
+            self.right_to_left(N - 2, 1)
+        return self.log[-1].energy

@@ -1,0 +1,108 @@
+"""Exact diagonalisation of the t-J model on the reference's square cylinder
+(test helper): the physics pin of the device DMRG.
+
+H = sum_<ij> [ -t_ij sum_s (c+_is c_js + h.c.) + J_ij (S_i.S_j - n_i n_j / 4) ]
+    - mu sum_i n_i,
+nearest-neighbour bonds (t, J) and next-nearest (t', J' = J (t'/t)^2) taken
+from the neighbour table of dmrg_setup (code/tjsq/dmrg_sub.f90:87-143: y
+periodic, x open, no doubled rung on a two-leg ladder), no double occupancy,
+fixed N and S_z = 0.
+"""
+import itertools
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from grand_sql_tj_amd.lattice import square_neighbours
+
+
+def bonds(model):
+    n = model.nsite
+    neib = square_neighbours(model.ny, model.nx, n)
+    out = []
+    for i in range(1, n + 1):
+        for k, (t, J) in ((0, (model.t, model.J)), (2, (model.t, model.J)), (4, (model.tp, model.jprime)),
+                          (6, (model.tp, model.jprime))):
+            j = int(neib[i, k])
+            if j and (t != 0.0 or J != 0.0):
+                out.append((i - 1, j - 1, t, J))
+    return out
+
+
+def _states(n, nup, ndn):
+    ups = [sum(1 << b for b in c) for c in itertools.combinations(range(n), nup)]
+    out = []
+    for u in ups:
+        free = [b for b in range(n) if not (u >> b) & 1]
+        for c in itertools.combinations(free, ndn):
+            out.append((u << n) | sum(1 << b for b in c))
+    return np.array(sorted(out), dtype=np.int64)
+
+
+def _popcount_between(m, a, b):
+    """number of set bits of m strictly between positions a and b"""
+    lo, hi = min(a, b), max(a, b)
+    mask = ((1 << hi) - 1) & ~((1 << (lo + 1)) - 1)
+    x = m & mask
+    c = np.zeros_like(x)
+    while True:
+        nz = x != 0
+        if not nz.any():
+            return c
+        c += nz
+        x = x & (x - 1)
+
+
+def hamiltonian(model, n_elec):
+    """Sparse H on the (N = n_elec, S_z = 0) sector; returns (H, states)."""
+    n = model.nsite
+    st = _states(n, n_elec // 2, n_elec // 2)
+    full = (1 << n) - 1
+    up, dn = st >> n, st & full
+    rows, cols, vals = [], [], []
+    diag = np.full(st.size, -model.mu * n_elec, dtype=np.float64)
+    for i, j, t, J in bonds(model):
+        bi, bj = 1 << i, 1 << j
+        ui, uj = (up & bi) != 0, (up & bj) != 0
+        di, dj = (dn & bi) != 0, (dn & bj) != 0
+        # exchange, diagonal: J (Sz Sz - n n / 4): antiparallel -J/2, parallel 0
+        anti = (ui & dj) | (di & uj)
+        diag += np.where(anti, -0.5 * J, 0.0)
+        # spin flip J/2 (S+_i S-_j + h.c.): (i up, j dn) <-> (i dn, j up).
+        # S-_i S+_j = c+_id c_iu c+_ju c_jd = -(c+_id c_jd)(c+_ju c_iu): minus
+        # the product of the two hops, each with its string sign
+        for sel, nu, nd in (((ui & dj), up ^ bi ^ bj, dn ^ bi ^ bj), ((di & uj), up ^ bi ^ bj, dn ^ bi ^ bj)):
+            idx = np.nonzero(sel)[0]
+            if idx.size == 0 or J == 0.0:
+                continue
+            sign = (-1.0) ** (_popcount_between(up[idx], i, j) + _popcount_between(dn[idx], i, j))
+            tgt = np.searchsorted(st, (nu[idx] << n) | nd[idx])
+            rows.append(tgt); cols.append(idx); vals.append(-0.5 * J * sign)
+        # hopping -t (c+_is c_js + h.c.) for s = up, down
+        if t != 0.0:
+            for spin in (0, 1):
+                m = up if spin == 0 else dn
+                occ_i, occ_j = (m & bi) != 0, (m & bj) != 0
+                empty_i = ((up | dn) & bi) == 0
+                empty_j = ((up | dn) & bj) == 0
+                for src, dst, ok in ((j, i, occ_j & empty_i), (i, j, occ_i & empty_j)):
+                    idx = np.nonzero(ok)[0]
+                    if idx.size == 0:
+                        continue
+                    mm = m[idx] ^ (1 << src) ^ (1 << dst)
+                    sign = (-1.0) ** _popcount_between(m[idx], src, dst)
+                    new = (mm << n) | dn[idx] if spin == 0 else (up[idx] << n) | mm
+                    tgt = np.searchsorted(st, new)
+                    rows.append(tgt); cols.append(idx); vals.append(-t * sign)
+    rows.append(np.arange(st.size)); cols.append(np.arange(st.size)); vals.append(diag)
+    H = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(st.size, st.size))
+    return H, st
+
+
+def ground_energy(model, n_elec):
+    H, _ = hamiltonian(model, n_elec)
+    assert abs(H - H.T).max() < 1e-12
+    if H.shape[0] < 400:
+        return float(np.linalg.eigvalsh(H.toarray())[0])
+    return float(spla.eigsh(H, k=1, which="SA", tol=1e-13)[0][0])
