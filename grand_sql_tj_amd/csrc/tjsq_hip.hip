@@ -140,9 +140,9 @@ struct TileCfg {
   // operand reads of 8 consecutive lanes fall in distinct banks
   static constexpr int HS = MODE == MODE_F64 ? KC + 2 : KC + 4;
   // waves per SIMD asked of the compiler (the kernel is latency-bound):
-  // 4 for the FP64 and BF16x3 64-tiles (119 / 105 VGPRs; FP64 measured 10 %
-  // faster than at 3, BF16x3 slower at 5 where 10 VGPRs spill), 5 for FP32
-  static constexpr int WAVES = TSZ == 64 ? (MODE == MODE_BF16X3 || MODE == MODE_F64 ? 4 : 5) : 4;
+  // 4 for the FP64 64-tiles (119 VGPRs; measured 10 % faster than at 3),
+  // 5 for FP32 and BF16x3 (87 VGPRs; at 6 it spills and runs slower)
+  static constexpr int WAVES = TSZ == 64 ? (MODE == MODE_F64 ? 4 : 5) : 4;
 };
 
 constexpr uint32_t OOB = 0x80000000u;   // at or above the record count: reads as zero
@@ -329,13 +329,13 @@ __global__ __launch_bounds__((TileCfg<MODE, TSZ>::NT), (TileCfg<MODE, TSZ>::WAVE
         }
   }
   if (si < se) {
-    // BF16x3 sums each segment in its own accumulators and scales the sum by
-    // the coefficient (its operand words cannot be scaled)
-    Acc sacc[2][2];
-    if constexpr (MODE == MODE_BF16X3)
-      for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b)
-          for (int r = 0; r < 4; ++r) sacc[a][b][r] = 0.f;
+    // BF16x3 operand words cannot be scaled by the segment coefficient, so
+    // the running sum is kept as scale * acc: at a segment's first chunk acc
+    // is multiplied by scale / c and scale becomes c (coefficients are never
+    // 0: zero terms are not emitted), and the MFMAs then add A.B unscaled;
+    // acc *= scale at the end.  One accumulator set instead of two keeps the
+    // kernel at 87 VGPRs, 5 waves per SIMD.
+    double scale = 1.0;
     Seg cur = segs[si];
     // descriptor of the segment after `cur`, loaded one segment ahead so the
     // chunk loads at a segment boundary do not wait on it
@@ -361,6 +361,7 @@ __global__ __launch_bounds__((TileCfg<MODE, TSZ>::NT), (TileCfg<MODE, TSZ>::WAVE
       // the chunk in LDS is (cur, k0); the loads below fetch the next one
       // (the sides move on to the next segment at a segment's last chunk)
       const int kvalid = min(KC, cur.K - k0);
+      const bool first_chunk = k0 == 0;
       const bool last_chunk = k0 + KC >= cur.K;
       const double cseg = cur.c;
       if (last_chunk) {
@@ -387,6 +388,13 @@ __global__ __launch_bounds__((TileCfg<MODE, TSZ>::NT), (TileCfg<MODE, TSZ>::WAVE
       const bool r0 = wm < tl.mt, r1 = wm + 16 < tl.mt, c0 = wn < tl.nt, c1 = wn + 16 < tl.nt;
       const bool rr_[2] = {r0, r1}, cc_[2] = {c0, c1};
       if constexpr (MODE == MODE_BF16X3) {
+        if (first_chunk) {
+          const float rs = (float)(scale / cseg);
+          scale = cseg;
+          for (int bi = 0; bi < 2; ++bi)
+            for (int bj = 0; bj < 2; ++bj)
+              for (int r = 0; r < 4; ++r) acc[bi][bj][r] *= rs;
+        }
         // k step g of 32: lane (lr, lq) supplies k = 32 g + 8 lq + t, t < 8
         for (int g = 0; g * 32 < kvalid; ++g) {
           bf8 bh[2], bl[2];
@@ -401,20 +409,11 @@ __global__ __launch_bounds__((TileCfg<MODE, TSZ>::NT), (TileCfg<MODE, TSZ>::WAVE
             split_pairs(*(const u4*)la, *(const u4*)(la + 4), ah, al);
             for (int bj = 0; bj < 2; ++bj) {
               if (!cc_[bj]) continue;
-              sacc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[bj], sacc[bi][bj], 0, 0, 0);
-              sacc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[bj], sacc[bi][bj], 0, 0, 0);
-              sacc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[bj], sacc[bi][bj], 0, 0, 0);
+              acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[bj], acc[bi][bj], 0, 0, 0);
+              acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[bj], acc[bi][bj], 0, 0, 0);
+              acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[bj], acc[bi][bj], 0, 0, 0);
             }
           }
-        }
-        if (last_chunk) {
-          const float cf = (float)cseg;
-          for (int bi = 0; bi < 2; ++bi)
-            for (int bj = 0; bj < 2; ++bj)
-              for (int r = 0; r < 4; ++r) {
-                acc[bi][bj][r] += cf * sacc[bi][bj][r];
-                sacc[bi][bj][r] = 0.f;
-              }
         }
       } else {
         // k group g of 16: lane (lr, lq) supplies k = 16 g + 4 lq + s in step s
@@ -448,6 +447,12 @@ __global__ __launch_bounds__((TileCfg<MODE, TSZ>::NT), (TileCfg<MODE, TSZ>::WAVE
       }
       __syncthreads();
       if (!more) break;
+    }
+    if constexpr (MODE == MODE_BF16X3) {
+      const float rs = (float)scale;
+      for (int bi = 0; bi < 2; ++bi)
+        for (int bj = 0; bj < 2; ++bj)
+          for (int r = 0; r < 4; ++r) acc[bi][bj][r] *= rs;
     }
   }
   if (tl.slot >= 0) {            // partial tile: slot counts units of 32x32 elements
