@@ -22,7 +22,7 @@ HIPCC_FLAGS = [
 def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     # rocSOLVER (with rocBLAS) diagonalises the density-matrix sectors
-    libs = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+    libs = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-lpthread", "-Wl,-rpath,/opt/rocm/lib"]
     cmd = [hipcc] + HIPCC_FLAGS + SOURCES + libs + ["-o", OUT + ".tmp"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

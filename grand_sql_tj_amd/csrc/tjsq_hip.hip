@@ -28,6 +28,8 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "tjsq_capi.h"
@@ -1998,20 +2000,46 @@ int tjsq_truncation_operator(int nsec, const int* den_sub, const int64_t* den_of
     bool ok = hipMemcpy(djobs, jobs.data(), sizeof(PadJob) * njobs, hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipEventRecord(S->ev[NSOLVE], st) == hipSuccess;
     for (int i = 0; ok && i < NSOLVE; ++i) ok = hipStreamWaitEvent(S->s[i], S->ev[NSOLVE], 0) == hipSuccess;
-    for (size_t b = 0; ok && b < batches.size(); ++b) {
-      const Batch& B = batches[b];
-      hipStream_t sb = S->s[b % NSOLVE];
-      dim3 grid((unsigned)std::min<int64_t>(((int64_t)B.N * B.N + 255) / 256, 64), (unsigned)B.count);
-      hipLaunchKernelGGL(k_pad_in, grid, dim3(256), 0, sb, den, pad, djobs + B.first);
-      ok = hipGetLastError() == hipSuccess;
-      ok = ok && rocsolver_dsyevd_strided_batched(S->h[b % NSOLVE], rocblas_evect_original, rocblas_fill_upper, B.N,
+    // A class is a few hundred to a thousand small rocsolver launches per
+    // call, so issuing the classes from one host thread serialises their
+    // launch cost; each solver stream gets its own host thread (three
+    // classes: 11.7 -> 9.7 ms on MI355X).  The HIP device is per host
+    // thread, so every worker selects the caller's.
+    int dev = 0;
+    ok = ok && hipGetDevice(&dev) == hipSuccess;
+    const int nthr = (int)std::min<size_t>(batches.size(), NSOLVE);
+    std::vector<char> tok(std::max(1, nthr), 1);
+    auto worker = [&](int t) {
+      bool k = hipSetDevice(dev) == hipSuccess;
+      for (size_t b = t; k && b < batches.size(); b += nthr) {
+        const Batch& B = batches[b];
+        hipStream_t sb = S->s[t];
+        dim3 grid((unsigned)std::min<int64_t>(((int64_t)B.N * B.N + 255) / 256, 64), (unsigned)B.count);
+        hipLaunchKernelGGL(k_pad_in, grid, dim3(256), 0, sb, den, pad, djobs + B.first);
+        k = hipGetLastError() == hipSuccess;
+        k = k && rocsolver_dsyevd_strided_batched(S->h[t], rocblas_evect_original, rocblas_fill_upper, B.N,
                                                   pad + B.pad_off, B.N, (rocblas_stride)B.N * B.N, D + B.d_off,
                                                   B.N, E + B.d_off, B.N, dinfo + B.first,
                                                   B.count) == rocblas_status_success;
-      if (ok) {
-        hipLaunchKernelGGL(k_pad_out, dim3(grid.x, B.count), dim3(256), 0, sb, den, pad, djobs + B.first);
-        ok = hipGetLastError() == hipSuccess;
+        if (k) {
+          hipLaunchKernelGGL(k_pad_out, dim3(grid.x, B.count), dim3(256), 0, sb, den, pad, djobs + B.first);
+          k = hipGetLastError() == hipSuccess;
+        }
       }
+      tok[t] = k;
+    };
+    if (ok && nthr > 0) {
+      std::vector<std::thread> th;
+      int started = 1;
+      try {
+        for (; started < nthr; ++started) th.emplace_back(worker, started);
+      } catch (const std::system_error&) {
+        // no thread to spare: the remaining streams are issued from this one
+      }
+      worker(0);
+      for (auto& x : th) x.join();
+      for (int t = started; t < nthr; ++t) worker(t);
+      for (int t = 0; t < nthr; ++t) ok = ok && tok[t];
     }
     for (int i = 0; ok && i < NSOLVE; ++i)
       ok = hipEventRecord(S->ev[i], S->s[i]) == hipSuccess && hipStreamWaitEvent(st, S->ev[i], 0) == hipSuccess;
