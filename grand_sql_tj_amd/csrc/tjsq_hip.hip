@@ -133,18 +133,19 @@ __device__ __forceinline__ int acc_row(int lq, int r) {
 template <int MODE, int TSZ>
 struct TileCfg {
   static constexpr int NT = TSZ == 64 ? 256 : 64;    // threads
-  // K chunk staged per step.  A chunk of 64 for the 4-wave FP32 / BF16x3
-  // items cost one wave per SIMD of occupancy and measured slower.
-  static constexpr int KC = 32;
+  // K chunk staged per step: 32 (FP32 / BF16x3), 16 (FP64, half the
+  // prefetch registers: 5 waves per SIMD instead of 4, 6 % faster).  A chunk
+  // of 64 for FP32 / BF16x3 cost one wave per SIMD and measured slower.
+  static constexpr int KC = MODE == MODE_F64 ? 16 : 32;
   static constexpr int PER = TSZ * KC / NT;          // elements per thread per panel and chunk
-  static_assert(KC % 32 == 0 && PER >= 1, "K chunk must be a multiple of 32");
+  static_assert(KC % (MODE == MODE_BF16X3 ? 32 : 16) == 0 && PER >= 1, "K chunk must be a multiple of the MFMA k group");
   // LDS row stride in operand words: rows start 16-byte aligned and the
   // operand reads of 8 consecutive lanes fall in distinct banks
   static constexpr int HS = MODE == MODE_F64 ? KC + 2 : KC + 4;
-  // waves per SIMD asked of the compiler (the kernel is latency-bound):
-  // 4 for the FP64 64-tiles (119 VGPRs; measured 10 % faster than at 3),
-  // 5 for FP32 and BF16x3 (87 VGPRs; at 6 it spills and runs slower)
-  static constexpr int WAVES = TSZ == 64 ? (MODE == MODE_F64 ? 4 : 5) : 4;
+  // waves per SIMD asked of the compiler for the 64-tiles (the kernel is
+  // latency-bound): 5 (BF16x3 87 VGPRs; 6 spills and runs slower; FP64
+  // 96 VGPRs with 5 spilled, still faster than 4 waves without spills)
+  static constexpr int WAVES = TSZ == 64 ? 5 : 4;
 };
 
 constexpr uint32_t OOB = 0x80000000u;   // at or above the record count: reads as zero
