@@ -536,8 +536,9 @@ class Program:
         return OpSet(length, hdr, sub, off, np.ascontiguousarray(data, np.float64))
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().tjsq_program_destroy(self._h)
+        # at interpreter exit the module's globals may already be gone
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.tjsq_program_destroy(self._h)
             self._h = None
 
 
@@ -649,6 +650,7 @@ class Batch:
         return ms, n
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().tjsq_batch_destroy(self._h)
+        # at interpreter exit the module's globals may already be gone
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.tjsq_batch_destroy(self._h)
             self._h = None
