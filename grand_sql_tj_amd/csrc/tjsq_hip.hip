@@ -1646,11 +1646,37 @@ __global__ void k_identity(double* __restrict__ a, int64_t n) {
 // its own rocBLAS handle, so batched eigensolves of different size classes
 // overlap.  Created on first use and kept until the process ends.
 constexpr int NSOLVE = 4;
+// one sector inside its class buffer: rho (n x n, ld n) at src, padded
+// N x N matrix at dst
+struct PadJob { int64_t src, dst; int n, N; };
 struct SolverState {
   rocblas_handle h[NSOLVE] = {};
   hipStream_t s[NSOLVE] = {};
   hipEvent_t ev[NSOLVE + 1] = {};
+  // work buffers of the batched eigensolve, grown on demand and reused: a
+  // hipMalloc / hipFree pair per call costs several ms in a process that
+  // holds many large allocations (18 ms against 10 ms per call measured)
+  double* pad = nullptr;
+  double* D = nullptr;
+  double* E = nullptr;
+  rocblas_int* info = nullptr;
+  PadJob* jobs = nullptr;
+  int64_t pad_cap = 0, D_cap = 0, E_cap = 0, info_cap = 0, jobs_cap = 0;
 };
+
+// grows a device buffer to hold `need` elements (its old contents are not
+// kept); false when the allocation fails
+template <typename T>
+bool grow_buffer(T** p, int64_t* cap, int64_t need) {
+  if (need <= *cap) return true;
+  (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  const int64_t n = std::max<int64_t>(1, need + need / 4);
+  if (hipMalloc((void**)p, sizeof(T) * n) != hipSuccess) return false;
+  *cap = n;
+  return true;
+}
 std::mutex g_solver_mu;
 std::vector<SolverState*> g_solver;
 
@@ -1661,7 +1687,7 @@ int solver_state(SolverState** out) {
   if (!g_solver[dev]) {
     SolverState* S = new SolverState();
     for (int i = 0; i < NSOLVE; ++i) {
-      HIPCHK(hipStreamCreate(&S->s[i]));
+      HIPCHK(hipStreamCreateWithFlags(&S->s[i], hipStreamNonBlocking));
       if (rocblas_create_handle(&S->h[i]) != rocblas_status_success)
         return fail(TJSQ_ERR_HIP, "rocblas_create_handle failed");
       if (rocblas_set_stream(S->h[i], S->s[i]) != rocblas_status_success)
@@ -1714,9 +1740,6 @@ std::vector<int> eig_classes(const std::vector<int>& dims, int groups) {
   return cls;
 }
 
-// one sector inside its class buffer: rho (n x n, ld n) at src, padded
-// N x N matrix at dst
-struct PadJob { int64_t src, dst; int n, N; };
 
 // dst = [[rho, 0], [0, diag(-(2 + 2j))]]: the padding block decouples exactly
 // in the Householder tridiagonalisation (its reflectors are identities and the
@@ -1970,35 +1993,24 @@ int tjsq_truncation_operator(int nsec, const int* den_sub, const int64_t* den_of
     pad_elems += (int64_t)N * N;
     d_elems += N;
   }
-  double* pad = nullptr;
-  double* D = nullptr;
-  double* E = nullptr;
-  rocblas_int* dinfo = nullptr;
-  PadJob* djobs = nullptr;
   const int njobs = (int)jobs.size();
-  auto release = [&]() {
-    (void)hipFree(pad);
-    (void)hipFree(D);
-    (void)hipFree(E);
-    (void)hipFree(dinfo);
-    (void)hipFree(djobs);
-  };
-  if (hipMalloc(&pad, sizeof(double) * std::max<int64_t>(1, pad_elems)) != hipSuccess ||
-      hipMalloc(&D, sizeof(double) * std::max<int64_t>(1, d_elems)) != hipSuccess ||
-      hipMalloc(&E, sizeof(double) * std::max<int64_t>(1, d_elems)) != hipSuccess ||
-      hipMalloc(&dinfo, sizeof(rocblas_int) * std::max(1, njobs)) != hipSuccess ||
-      hipMalloc(&djobs, sizeof(PadJob) * std::max(1, njobs)) != hipSuccess) {
-    release();
-    return fail(TJSQ_ERR_NOMEM, "tjsq_truncation_operator: work allocation failed");
-  }
   std::vector<double> hD(d_elems);
   std::vector<rocblas_int> hinfo(std::max(1, njobs), 0);
   if (njobs > 0) {
     std::lock_guard<std::mutex> lk(g_solver_mu);
     SolverState* S = nullptr;
     int rc = solver_state(&S);
-    if (rc) { release(); return rc; }
-    bool ok = hipMemcpy(djobs, jobs.data(), sizeof(PadJob) * njobs, hipMemcpyHostToDevice) == hipSuccess;
+    if (rc) return rc;
+    if (!grow_buffer(&S->pad, &S->pad_cap, pad_elems) || !grow_buffer(&S->D, &S->D_cap, d_elems) ||
+        !grow_buffer(&S->E, &S->E_cap, d_elems) || !grow_buffer(&S->info, &S->info_cap, (int64_t)njobs) ||
+        !grow_buffer(&S->jobs, &S->jobs_cap, (int64_t)njobs))
+      return fail(TJSQ_ERR_NOMEM, "tjsq_truncation_operator: work allocation failed");
+    double* pad = S->pad;
+    double* D = S->D;
+    double* E = S->E;
+    rocblas_int* dinfo = S->info;
+    PadJob* djobs = S->jobs;
+    bool ok = hipMemcpyAsync(djobs, jobs.data(), sizeof(PadJob) * njobs, hipMemcpyHostToDevice, st) == hipSuccess;
     ok = ok && hipEventRecord(S->ev[NSOLVE], st) == hipSuccess;
     for (int i = 0; ok && i < NSOLVE; ++i) ok = hipStreamWaitEvent(S->s[i], S->ev[NSOLVE], 0) == hipSuccess;
     // A class is a few hundred to a thousand small rocsolver launches per
@@ -2010,11 +2022,20 @@ int tjsq_truncation_operator(int nsec, const int* den_sub, const int64_t* den_of
     ok = ok && hipGetDevice(&dev) == hipSuccess;
     const int nthr = (int)std::min<size_t>(batches.size(), NSOLVE);
     std::vector<char> tok(std::max(1, nthr), 1);
+    // Streams only overlap on different hardware queues, and a process has
+    // few (GPU_MAX_HW_QUEUES, 4 by default): streams made after the caller's
+    // own ones share them, and two classes on one queue run one after the
+    // other (17.7 ms per call measured in a process holding a side stream,
+    // 9.8 ms without).  The largest class therefore runs on the caller's
+    // stream, which has a queue already, and the others on the first
+    // solver streams, the ones made earliest.
+    const size_t nb = batches.size();
     auto worker = [&](int t) {
       bool k = hipSetDevice(dev) == hipSuccess;
-      for (size_t b = t; k && b < batches.size(); b += nthr) {
-        const Batch& B = batches[b];
-        hipStream_t sb = S->s[t];
+      hipStream_t sb = t == 0 ? st : S->s[t - 1];
+      k = k && rocblas_set_stream(S->h[t], sb) == rocblas_status_success;
+      for (size_t i = t; k && i < nb; i += nthr) {
+        const Batch& B = batches[nb - 1 - i];   // largest class first
         dim3 grid((unsigned)std::min<int64_t>(((int64_t)B.N * B.N + 255) / 256, 64), (unsigned)B.count);
         hipLaunchKernelGGL(k_pad_in, grid, dim3(256), 0, sb, den, pad, djobs + B.first);
         k = hipGetLastError() == hipSuccess;
@@ -2047,12 +2068,8 @@ int tjsq_truncation_operator(int nsec, const int* den_sub, const int64_t* den_of
     ok = ok && hipMemcpyAsync(hD.data(), D, sizeof(double) * d_elems, hipMemcpyDeviceToHost, st) == hipSuccess;
     ok = ok && hipMemcpyAsync(hinfo.data(), dinfo, sizeof(rocblas_int) * njobs, hipMemcpyDeviceToHost, st) == hipSuccess;
     ok = ok && hipStreamSynchronize(st) == hipSuccess;
-    if (!ok) {
-      release();
-      return fail(TJSQ_ERR_HIP, "tjsq_truncation_operator: batched eigensolve failed");
-    }
+    if (!ok) return fail(TJSQ_ERR_HIP, "tjsq_truncation_operator: batched eigensolve failed");
   }
-  release();
   for (int j = 0; j < njobs; ++j)
     if (hinfo[j] != 0) return fail(TJSQ_ERR_HIP, "tjsq_truncation_operator: eigensolver did not converge");
   // eigenvalues per sector in density-matrix order; the padding eigenvalues
