@@ -407,6 +407,47 @@ def serve_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def sweep_bench(args):
+    """Whole ground-state runs on the device (grand_sql_tj_amd.sweep): the
+    reference's warm-up and one pair of finite half sweeps on a t-J cylinder
+    (t = 1, t' = -0.2, J = 0.4), every bond's plan, solve, truncation,
+    block update and wave transform through the library.  Reports bonds per
+    second over the whole run and the time per phase."""
+    import torch
+    from grand_sql_tj_amd import sweep
+    from grand_sql_tj_amd.lattice import TJModel
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
+    ny, nx, n_up, m = (int(v) for v in args.sweep_shape.split(","))
+    model = TJModel(ny=ny, nx=nx, t=1.0, tp=-0.2, J=0.4, up_bias=1)
+
+    def run():
+        d = sweep.DMRG(model, total_up=n_up, kept_max=m, trun_err=1e-9, tol=1e-9, precision=args.precision)
+        t0 = time.perf_counter()
+        e = d.run(sweeps=1)
+        return d, e, time.perf_counter() - t0
+
+    run_small = sweep.DMRG(TJModel(ny=2, nx=4, t=1.0, J=0.4, up_bias=1), total_up=6, kept_max=64)
+    run_small.run(sweeps=1)          # loads the library and the solver handles
+    d, e, wall = run()
+    bonds = len(d.log)
+    line = {
+        "metric": "device DMRG: bonds/s over warm-up + one sweep (plan, solve, truncation, block update, transform)",
+        "value": bonds / wall, "unit": "bonds/s", "n_gpus": 1, "steps": bonds, "warmup": 1,
+        "ms_per_step": 1e3 * wall / bonds, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32+f64", "bf16x3": "bf16x3->f32+f64"}[args.precision], "data": "synthetic",
+        "config": {"workload": f"t-J {ny}x{nx} cylinder (t=1, t'=-0.2, J=0.4), N={n_up}, kept_max={m}, "
+                               "trun_err 1e-9, Lanczos tol 1e-9, warm-up + 1 sweep", "precision": args.precision},
+        "energy": e, "energy_per_site": e / (ny * nx), "wall_s": wall,
+        "phase_s": {k: round(v, 4) for k, v in d.phase_s.items()},
+        "host_other_s": wall - sum(d.phase_s.values()),
+        "max_superblock_dim": max(r.dim for r in d.log), "max_kept": max(r.kept for r in d.log),
+        "last_trunc_err": d.log[-1].trunc_err,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(model, sb, x0, n_iter, budget_s=20.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -454,6 +495,9 @@ def main():
                     help="arrival rate as a fraction of the one-at-a-time service rate")
     ap.add_argument("--warm-iter", type=int, default=200)
     ap.add_argument("--noise", type=float, default=0.01)
+    ap.add_argument("--sweep", action="store_true",
+                    help="instead of the solve benchmark: a whole device DMRG run (warm-up + one sweep)")
+    ap.add_argument("--sweep-shape", default="4,12,44,1500", help="ny,nx,N,kept_max of the --sweep run")
     ap.add_argument("--apply-only", type=int, default=0,
                     help="profiling mode: a 512 MiB device copy (counter calibration) then N isolated applies")
     args = ap.parse_args()
@@ -467,6 +511,9 @@ def main():
         return
     if args.serve:
         serve_bench(args)
+        return
+    if args.sweep:
+        sweep_bench(args)
         return
     import torch
     from grand_sql_tj_amd import capi

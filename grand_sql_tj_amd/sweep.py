@@ -26,6 +26,7 @@ and trun_err as given.
 from __future__ import annotations
 
 import math
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -97,6 +98,11 @@ class DMRG:
     sys_basis: dict = field(default_factory=dict)  # length L -> fused basis (block L-1 + site)
     env_basis: dict = field(default_factory=dict)
     log: list = field(default_factory=list)
+    # seconds per phase over the run: plan (host compile + table upload),
+    # solve (Lanczos), truncate (density matrix + eigensolve + selection),
+    # grow (block update), transform (wave_trans_LR/RL); each phase ends
+    # with a device synchronisation
+    phase_s: dict = field(default_factory=lambda: {k: 0.0 for k in ("plan", "solve", "truncate", "grow", "transform")})
 
     def __post_init__(self):
         import torch
@@ -122,19 +128,26 @@ class DMRG:
         """Get_basis of the block with one site (the basis carries length + 1)."""
         return get_basis(blk.length, blk.ops.sectors(), self.site)
 
+    def _timed(self, name, fn, *args):
+        t0 = time.perf_counter()
+        out = fn(*args)
+        self.torch.cuda.synchronize()
+        self.phase_s[name] += time.perf_counter() - t0
+        return out
+
     def _solve(self, blk_s, blk_e, sys_bs, env_bs, wsec, psi0, phase):
-        import time
         torch = self.torch
         sb = Superblock(blk_s.length, blk_e.length, blk_s.ops, blk_e.ops, sys_bs, env_bs, wsec, meta={})
         t0 = time.perf_counter()
-        plan = capi.Plan(self.model, sb, precisions=("fp64", self.precision), device_ops=(blk_s.data, blk_e.data))
+        plan = self._timed("plan", lambda: capi.Plan(self.model, sb, precisions=("fp64", self.precision),
+                                                     device_ops=(blk_s.data, blk_e.data)))
         if psi0 is None:
             psi = self._device(self.rng.standard_normal(sb.dim))
         else:
             psi = psi0.clone()
         psi /= psi.norm()
-        info = plan.lanczos(psi, tol=self.tol, max_iter=self.max_iter, precision=self.precision)
-        torch.cuda.synchronize()
+        info = self._timed("solve", lambda: plan.lanczos(psi, tol=self.tol, max_iter=self.max_iter,
+                                                         precision=self.precision))
         self.psi, self.wsec = psi, wsec
         return info, time.perf_counter() - t0
 
@@ -199,11 +212,11 @@ class DMRG:
             tot_up = min(tot_up, self.total_up)
             wsec = get_super_basis(sys_bs, env_bs, tot_up, 0)
             info, secs = self._solve(s, e, sys_bs, env_bs, wsec, None, "warm-up")
-            t_s, k_s, err = self._truncate(0, label, label, wsec, sys_bs, self.psi, True)
-            t_e, _, _ = self._truncate(1, label, label, wsec, env_bs, self.psi, True)
+            t_s, k_s, err = self._timed("truncate", self._truncate, 0, label, label, wsec, sys_bs, self.psi, True)
+            t_e, _, _ = self._timed("truncate", self._truncate, 1, label, label, wsec, env_bs, self.psi, True)
             self._record("warm-up", label, label, wsec, info, k_s, err, secs)
-            self.sys[label + 1] = self._grow(0, s, sys_bs, t_s, label, label)
-            self.env[label + 1] = self._grow(1, e, env_bs, t_e, label, label)
+            self.sys[label + 1] = self._timed("grow", self._grow, 0, s, sys_bs, t_s, label, label)
+            self.env[label + 1] = self._timed("grow", self._grow, 1, e, env_bs, t_e, label, label)
             self.sys_trun[label + 1], self.env_trun[label + 1] = t_s, t_e
             self.sys_basis[label + 1], self.env_basis[label + 1] = sys_bs, env_bs
             label += 1
@@ -223,11 +236,13 @@ class DMRG:
                 psi0 = self.psi               # the same bond as the last solve
             elif trun is not None:
                 # wave_trans_LR(sys_bs, env_basis, sys_trun, env_trun, old, new)
-                psi0 = self._transform(0, sys_bs, self.env_basis[N - label], trun, self.env_trun[N - label - 1], wsec)
+                psi0 = self._timed("transform", self._transform, 0, sys_bs, self.env_basis[N - label], trun,
+                                   self.env_trun[N - label - 1], wsec)
             info, secs = self._solve(s, e, sys_bs, env_bs, wsec, psi0, "left-to-right")
-            trun, kept, err = self._truncate(0, label, N - label - 2, wsec, sys_bs, self.psi, False)
+            trun, kept, err = self._timed("truncate", self._truncate, 0, label, N - label - 2, wsec, sys_bs, self.psi,
+                                          False)
             self._record("left-to-right", label, N - label - 2, wsec, info, kept, err, secs)
-            self.sys[label + 1] = self._grow(0, s, sys_bs, trun, label, N - label - 2)
+            self.sys[label + 1] = self._timed("grow", self._grow, 0, s, sys_bs, trun, label, N - label - 2)
             self.sys_trun[label + 1], self.sys_basis[label + 1] = trun, sys_bs
             label += 1
 
@@ -246,11 +261,13 @@ class DMRG:
                 psi0 = self.psi
             elif trun is not None:
                 # wave_trans_RL(sys_basis, env_bs, sys_trun, env_trun, old, new)
-                psi0 = self._transform(1, self.sys_basis[label + 1], env_bs, self.sys_trun[label], trun, wsec)
+                psi0 = self._timed("transform", self._transform, 1, self.sys_basis[label + 1], env_bs,
+                                   self.sys_trun[label], trun, wsec)
             info, secs = self._solve(s, e, sys_bs, env_bs, wsec, psi0, "right-to-left")
-            trun, kept, err = self._truncate(1, label - 1, N - label - 1, wsec, env_bs, self.psi, False)
+            trun, kept, err = self._timed("truncate", self._truncate, 1, label - 1, N - label - 1, wsec, env_bs,
+                                          self.psi, False)
             self._record("right-to-left", label - 1, N - label - 1, wsec, info, kept, err, secs)
-            self.env[N - label] = self._grow(1, e, env_bs, trun, label - 1, N - label - 1)
+            self.env[N - label] = self._timed("grow", self._grow, 1, e, env_bs, trun, label - 1, N - label - 1)
             self.env_trun[N - label], self.env_basis[N - label] = trun, env_bs
             label -= 1
 
