@@ -152,3 +152,47 @@ def test_update_step_chain_gauge_invariant(side):
             assert abs(np.linalg.norm(a[k]) - np.linalg.norm(b[k])) < 1e-9 * scale
             if op == 0:
                 assert np.abs(np.linalg.eigvalsh(a[k]) - np.linalg.eigvalsh(b[k])).max() < 1e-9 * scale
+
+
+def test_truncation_classes_on_any_caller_stream():
+    """Sectors of very different sizes (several padded size classes, one of
+    them run on the caller's stream, the others on the library's solver
+    streams): the result is the same bitwise from the default stream and
+    from a side stream, and the spectrum is numpy's."""
+    rng = np.random.default_rng(3)
+    dims = [7, 33, 90, 150, 260, 12, 64]
+    sub, off, blocks, o = [], [], [], 0
+    for k, n in enumerate(dims):
+        q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+        lam = np.exp(-0.15 * np.arange(n)) * rng.uniform(0.5, 1.0, n)
+        blocks.append((q * lam) @ q.T)
+        sub.append((2 * k, 0, 0, n, n))
+        off.append(o)
+        o += n * n
+    tr = sum(np.trace(b) for b in blocks)
+    den = np.concatenate([(b / tr).T.reshape(-1) for b in blocks])
+    den_sub, den_off = np.array(sub, np.int32), np.array(off, np.int64)
+    outs = []
+    for side in (False, True):
+        d = _cuda(den)
+        torch.cuda.synchronize()
+        if side:
+            with torch.cuda.stream(torch.cuda.Stream()):
+                res = capi.truncation_operator(den_sub, den_off, d, 50, 400, 1e-10)
+        else:
+            res = capi.truncation_operator(den_sub, den_off, d, 50, 400, 1e-10)
+        torch.cuda.synchronize()
+        outs.append((res, d.cpu().numpy()))
+    (t0, o0, s0, i0), e0 = outs[0]
+    (t1, o1, s1, i1), e1 = outs[1]
+    assert np.array_equal(e0, e1) and np.array_equal(s0, s1)
+    assert np.array_equal(t0, t1) and np.array_equal(o0, o1) and i0 == i1
+    ref = np.sort(np.concatenate([np.linalg.eigvalsh(b / tr) for b in blocks]))
+    assert np.abs(np.sort(s0) - ref).max() < 1e-14
+    for s in range(t0.shape[0]):
+        n, c = int(t0[s, 3]), int(t0[s, 4])
+        t = e0[o0[s]:o0[s] + n * c].reshape(c, n).T
+        b = blocks[[k for k in range(len(dims)) if sub[k][0] == t0[s, 0]][0]] / tr
+        assert np.abs(t.T @ t - np.eye(c)).max() < 1e-12
+        w = np.sort(np.linalg.eigvalsh(b))[::-1][:c]
+        assert np.abs(b @ t - t * w[::-1]).max() < 1e-13 * max(1.0, w.max()) * 10
