@@ -79,11 +79,13 @@ def workload(m_mult: int, seed: int):
 
 
 def timed_solves(plan, x0_t, steps, warmup, precision, tol, max_iter, torch, energy_tol=0.0, rtol=0.0, stream=None):
-    """Wall time of each solve (the solve synchronises its stream at the end).
-    On a side stream with the plan's graphs enabled, each Lanczos iteration is
-    replayed from its captured hipGraph (captured during the warm-up)."""
+    """Wall time of each solve (the solve synchronises its stream at the end),
+    the last solve's result and its ground state.  On a side stream with the
+    plan's graphs enabled, each Lanczos iteration is replayed from its
+    captured hipGraph (captured during the warm-up)."""
     lat = []
     res = None
+    psi = None
     ctx = torch.cuda.stream(stream) if stream is not None else None
     if ctx is not None:
         ctx.__enter__()
@@ -100,7 +102,7 @@ def timed_solves(plan, x0_t, steps, warmup, precision, tol, max_iter, torch, ene
     finally:
         if ctx is not None:
             ctx.__exit__(None, None, None)
-    return np.array(lat), res
+    return np.array(lat), res, psi
 
 
 def block_update_step(model, sb, psi, m_keep, torch, reps=5):
@@ -409,10 +411,11 @@ def serve_bench(args):
 
 def sweep_bench(args):
     """Whole ground-state runs on the device (grand_sql_tj_amd.sweep): the
-    reference's warm-up and one pair of finite half sweeps on a t-J cylinder
-    (t = 1, t' = -0.2, J = 0.4), every bond's plan, solve, truncation,
-    block update and wave transform through the library.  Reports bonds per
-    second over the whole run and the time per phase."""
+    reference's warm-up and `--sweeps` pairs of finite half sweeps on a t-J
+    cylinder (t = 1, J = 0.4, t' = -0.2 unless --sweep-tp), every bond's
+    plan, solve, truncation, block update and wave transform through the
+    library.  Reports bonds per second over the whole run, the time per
+    phase and the energy at the end of every sweep."""
     import torch
     from grand_sql_tj_amd import sweep
     from grand_sql_tj_amd.lattice import TJModel
@@ -420,26 +423,29 @@ def sweep_bench(args):
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
     ny, nx, n_up, m = (int(v) for v in args.sweep_shape.split(","))
-    model = TJModel(ny=ny, nx=nx, t=1.0, tp=-0.2, J=0.4, up_bias=1)
+    model = TJModel(ny=ny, nx=nx, t=1.0, tp=args.sweep_tp, J=0.4, up_bias=1)
 
     def run():
         d = sweep.DMRG(model, total_up=n_up, kept_max=m, trun_err=1e-9, tol=1e-9, precision=args.precision)
         t0 = time.perf_counter()
-        e = d.run(sweeps=1)
+        e = d.run(sweeps=args.sweeps)
         return d, e, time.perf_counter() - t0
 
     run_small = sweep.DMRG(TJModel(ny=2, nx=4, t=1.0, J=0.4, up_bias=1), total_up=6, kept_max=64)
     run_small.run(sweeps=1)          # loads the library and the solver handles
     d, e, wall = run()
     bonds = len(d.log)
+    ends = [r.energy for i, r in enumerate(d.log)
+            if r.phase == "right-to-left" and (i + 1 == bonds or d.log[i + 1].phase != "right-to-left")]
     line = {
-        "metric": "device DMRG: bonds/s over warm-up + one sweep (plan, solve, truncation, block update, transform)",
+        "metric": "device DMRG: bonds/s over warm-up + sweeps (plan, solve, truncation, block update, transform)",
         "value": bonds / wall, "unit": "bonds/s", "n_gpus": 1, "steps": bonds, "warmup": 1,
         "ms_per_step": 1e3 * wall / bonds, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32+f64", "bf16x3": "bf16x3->f32+f64"}[args.precision], "data": "synthetic",
-        "config": {"workload": f"t-J {ny}x{nx} cylinder (t=1, t'=-0.2, J=0.4), N={n_up}, kept_max={m}, "
-                               "trun_err 1e-9, Lanczos tol 1e-9, warm-up + 1 sweep", "precision": args.precision},
-        "energy": e, "energy_per_site": e / (ny * nx), "wall_s": wall,
+        "config": {"workload": f"t-J {ny}x{nx} cylinder (t=1, t'={args.sweep_tp}, J=0.4), N={n_up}, kept_max={m}, "
+                               f"trun_err 1e-9, Lanczos tol 1e-9, warm-up + {args.sweeps} sweep(s)",
+                   "precision": args.precision},
+        "energy": e, "energy_per_site": e / (ny * nx), "energy_after_each_sweep": ends, "wall_s": wall,
         "phase_s": {k: round(v, 4) for k, v in d.phase_s.items()},
         "host_other_s": wall - sum(d.phase_s.values()),
         "max_superblock_dim": max(r.dim for r in d.log), "max_kept": max(r.kept for r in d.log),
@@ -498,6 +504,8 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="instead of the solve benchmark: a whole device DMRG run (warm-up + one sweep)")
     ap.add_argument("--sweep-shape", default="4,12,44,1500", help="ny,nx,N,kept_max of the --sweep run")
+    ap.add_argument("--sweeps", type=int, default=1, help="finite sweeps of the --sweep run after the warm-up")
+    ap.add_argument("--sweep-tp", type=float, default=-0.2, help="t'/t of the --sweep run")
     ap.add_argument("--apply-only", type=int, default=0,
                     help="profiling mode: a 512 MiB device copy (counter calibration) then N isolated applies")
     args = ap.parse_args()
@@ -564,14 +572,14 @@ def main():
     if args.graphs:
         side = torch.cuda.Stream()
         plan.graphs(True)
-    lat64, r64 = timed_solves(plan, x0_t, args.steps, args.warmup, "fp64", args.tol, args.max_iter, torch,
+    lat64, r64, psi64 = timed_solves(plan, x0_t, args.steps, args.warmup, "fp64", args.tol, args.max_iter, torch,
                               stream=side)
     print(f"[bench] fp64 solve {np.median(lat64)*1e3:.1f} ms E0={r64['e0']:.12f} it={r64['iterations']}",
           file=sys.stderr, flush=True)
     # the mixed solve restarts in FP64 only when its estimated energy error
     # exceeds a tenth of the 1e-8 per-site target
     etol = 0.1 * 1e-8 * nsite
-    latmp, rmp = timed_solves(plan, x0_t, args.steps, args.warmup, args.precision, args.tol, args.max_iter, torch,
+    latmp, rmp, psimp = timed_solves(plan, x0_t, args.steps, args.warmup, args.precision, args.tol, args.max_iter, torch,
                               energy_tol=etol, stream=side)
     print(f"[bench] {args.precision} solve {np.median(latmp)*1e3:.1f} ms E0={rmp['e0']:.12f} it={rmp['iterations']} "
           f"refine={rmp['refinements']}", file=sys.stderr, flush=True)
@@ -579,7 +587,7 @@ def main():
     # FP64 with the mixed pass's stopping rule (|dE| <= 1e-7 |E|): the FP64
     # rate at matched iteration counts, reported beside the reference-tolerance
     # FP64 rate
-    latm, rm = timed_solves(plan, x0_t, args.steps, args.warmup, "fp64", args.tol, args.max_iter, torch, rtol=1e-7,
+    latm, rm, _ = timed_solves(plan, x0_t, args.steps, args.warmup, "fp64", args.tol, args.max_iter, torch, rtol=1e-7,
                             stream=side)
     plan.graphs(False)
     print(f"[bench] fp64 (rtol 1e-7) solve {np.median(latm)*1e3:.1f} ms E0={rm['e0']:.12f} it={rm['iterations']}",
@@ -601,6 +609,10 @@ def main():
     value = 1.0 / float(np.mean(latmp))
     v64 = 1.0 / float(np.mean(lat64))
     de_site = abs(rmp["e0"] - r64["e0"]) / nsite
+    # wavefunction infidelity of the mixed-precision ground state against the
+    # FP64 one (both normalised; the sign of a ground state is arbitrary)
+    ov = float(torch.dot(psimp.double(), psi64.double()) / (psimp.double().norm() * psi64.double().norm()))
+    infidelity = 1.0 - abs(ov)
     cpu = None if args.no_cpu else cpu_baseline(model, sb, x0, max(1, r64["applies"]))
     line = {
         "metric": "Lanczos solves/s vs plain FP64 GPU at dE/site<=1e-8; p50/p99 solve latency",
@@ -612,6 +624,7 @@ def main():
                                f"centre bond 47|s|s|47, m={args.m} SU(2) multiplets, synthetic symmetric blocks",
                    "D": sb.dim, "precision": args.precision, "lanczos_tol": args.tol, "max_iter": args.max_iter},
         "fp64_solves_per_s": v64, "speedup_vs_fp64": value / v64, "dE_per_site": de_site,
+        "infidelity_vs_fp64": infidelity,
         "fp64_rtol_solves_per_s": 1.0 / float(np.mean(latm)),
         "speedup_vs_fp64_rtol": value * float(np.mean(latm)),
         "dE_per_site_fp64_rtol": abs(rm["e0"] - r64["e0"]) / nsite, "iterations_fp64_rtol": rm["iterations"],
