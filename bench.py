@@ -454,7 +454,14 @@ def sweep_bench(args):
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(model, sb, x0, n_iter, budget_s=20.0):
+def cpu_baseline(model, sb, x0, n_iter, threads=1, budget_s=20.0):
+    """The reference's own model_wave (hmerge + 14 kernels, compiled from its
+    sources) timed on this host on a bounded sample: a few applies of the same
+    workload, scaled by the applies of one FP64 solve.  The reference builds
+    with threaded MKL (code/tjsq/makefile:3, -qmkl=parallel) and has no
+    OpenMP of its own; its DGEMMs are small, and on the MI355X box's host one
+    MKL thread was fastest (an m=2000 apply: 0.136 s on 1 thread, 0.155 /
+    0.316 / 0.330 / 0.371 s on 2 / 4 / 8 / 16), so one thread is the default."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     if not oracle.ref_available():
@@ -462,17 +469,19 @@ def cpu_baseline(model, sb, x0, n_iter, budget_s=20.0):
     cwd = os.getcwd()
     os.chdir(os.environ.get("TMPDIR", "/tmp"))
     try:
+        oracle.ref_threads(threads)
         times = []
         while sum(times) < budget_s and len(times) < 3:
             t0 = time.perf_counter()
             oracle.ref_model_wave(model, sb, x0)
             times.append(time.perf_counter() - t0)
     finally:
+        oracle.ref_threads(1)
         os.chdir(cwd)
     t_apply = float(np.median(times))
-    return {"value": 1.0 / (t_apply * n_iter), "unit": "solves/s", "cores": 1, "kind": "reference",
-            "sample": f"{len(times)} reference model_wave applies (hmerge + 14 kernels, MKL sequential) of this "
-                      f"workload, {t_apply:.3f} s each, scaled by the {n_iter} applies of one FP64 solve"}
+    return {"value": 1.0 / (t_apply * n_iter), "unit": "solves/s", "cores": threads, "kind": "reference",
+            "sample": f"{len(times)} reference model_wave applies (hmerge + 14 kernels, MKL on {threads} thread(s)) "
+                      f"of this workload, {t_apply:.3f} s each, scaled by the {n_iter} applies of one FP64 solve"}
 
 
 def main():
@@ -486,6 +495,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=1, help="MKL threads of the reference CPU baseline")
     ap.add_argument("--no-update", action="store_true", help="skip the density/truncation/update step")
     ap.add_argument("--batch", type=int, default=0,
                     help="instead of the solve benchmark: N independent applies in one grouped launch")
@@ -613,7 +623,6 @@ def main():
     # FP64 one (both normalised; the sign of a ground state is arbitrary)
     ov = float(torch.dot(psimp.double(), psi64.double()) / (psimp.double().norm() * psi64.double().norm()))
     infidelity = 1.0 - abs(ov)
-    cpu = None if args.no_cpu else cpu_baseline(model, sb, x0, max(1, r64["applies"]))
     line = {
         "metric": "Lanczos solves/s vs plain FP64 GPU at dE/site<=1e-8; p50/p99 solve latency",
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -649,13 +658,16 @@ def main():
                         "unit": "TFLOP/s", "frac": achieved / peak,
                         "traffic": measured_traffic(kname, args.precision, sb.dim),
                         "algorithmic_flops_per_launch": fl[dom], "avg_launch_ms": kavg[dom]}
-    line["cpu_baseline"] = cpu
     if tuned is not None:
         line["work_item_shape"] = tuned["shape"]
     line["lanczos_iteration_graphs"] = bool(args.graphs)
     if not args.no_update:
         line["block_update"] = block_update_step(model, sb, g, args.m, torch)
         print(f"[bench] block update {line['block_update']}", file=sys.stderr, flush=True)
+    # the CPU baseline last, so MKL's worker threads cannot compete with the
+    # host side of the GPU measurements
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(model, sb, x0, max(1, r64["applies"]),
+                                                                 threads=args.cpu_threads)
     print(json.dumps(line), flush=True)
 
 

@@ -114,10 +114,21 @@ def ref_available() -> bool:
     return os.path.exists(ref_path())
 
 
+def ref_threads(n: int) -> None:
+    """Threads MKL uses inside the reference's DGEMM / DSYEV calls (the
+    reference builds with -qmkl=parallel, code/tjsq/makefile:3)."""
+    ref_lib().MKL_Set_Num_Threads(int(n))
+
+
 def ref_lib():
     with _lock:
         if "ref" not in _libs:
             lib = C.CDLL(ref_path(), mode=C.RTLD_GLOBAL)
+            # MKL through mkl_rt: one thread unless a caller asks for more
+            # (parity runs stay deterministic)
+            lib.MKL_Set_Num_Threads.restype = None
+            lib.MKL_Set_Num_Threads.argtypes = [C.c_int]
+            lib.MKL_Set_Num_Threads(1)
             lib.refb_init.restype = C.c_int
             lib.refb_init.argtypes = [C.c_int, C.c_int] + [C.c_double] * 8
             margs = [C.c_int, I32P, I32P, I64P, F64P]
@@ -279,6 +290,9 @@ def dropin_lib():
         if "dropin" not in _libs:
             ref_lib()
             lib = C.CDLL(dropin_path(), mode=C.RTLD_GLOBAL)
+            lib.MKL_Set_Num_Threads.restype = None
+            lib.MKL_Set_Num_Threads.argtypes = [C.c_int]
+            lib.MKL_Set_Num_Threads(1)     # the reference routines inside it, as in ref_lib
             margs = [C.c_int, I32P, I32P, I64P, F64P]
             common = [C.c_int, C.c_int] + margs + margs + [C.c_int, I32P, I32P, C.c_int, I32P, I32P, C.c_int, I32P]
             lib.dropin_model_wave.restype = C.c_int
