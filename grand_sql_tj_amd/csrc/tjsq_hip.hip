@@ -1706,9 +1706,9 @@ int solver_state(SolverState** out) {
 // that the size's Householder steps set.  The sectors, sorted by size, are
 // cut into at most `groups` contiguous classes (each padded to its largest
 // sector, rounded up to 16) minimising the largest class cost; the classes
-// run concurrently on the solver streams.  At the m = 8000 bond (91 sectors
-// up to 360) three classes take 13.8 ms against 18.9 ms for classes of
-// multiples of 64.
+// run concurrently (the largest on the caller's stream, the others on solver
+// streams).  At the m = 8000 bond (91 sectors up to 360) three classes took
+// 13.8 ms against 18.9 ms for classes of multiples of 64.
 std::vector<int> eig_classes(const std::vector<int>& dims, int groups) {
   const int n = (int)dims.size();
   std::vector<int> idx(n);
@@ -1966,7 +1966,7 @@ int tjsq_truncation_operator(int nsec, const int* den_sub, const int64_t* den_of
   }
   // sectors grouped into padded size classes, one strided-batched DSYEV-like
   // call (ascending eigenvalues, eigenvectors in place; den_mat.f90:192) per
-  // class, classes spread over the solver streams
+  // class, the classes run concurrently
   std::vector<int> dims, sec_of;
   for (int k = 0; k < nsec; ++k)
     if (den_sub[5 * k + 4] > 0) {
