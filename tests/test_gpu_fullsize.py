@@ -1,0 +1,131 @@
+"""The device path at the full sizes of the benchmark configurations, where
+no CPU oracle finishes in seconds, through properties that hold at any size:
+the isolated FP64 apply of a 6x12 cylinder at m = 4000 per side (t = 1,
+J = 0.4, 66 electrons, centre bond 35|s|s|35) and the 6x16, m = 8000 bond of
+the mixed-precision solve (t' = -0.2, 84 electrons, bond 47|s|s|47).
+
+- H_eff is symmetric: <x, H y> = <H x, y> (model_wave adds every hermitian
+  conjugate, get_sys.f90:2811-2823), and linear.
+- Applies are bitwise repeatable in every precision.
+- FP32 and BF16x3 applies stay within their stated tolerances of FP64
+  (2e-6 and 2e-5 relative).
+- The mixed-precision solve reaches the FP64 energy within 1e-8 per site,
+  with a ground state of infidelity below 1e-6.
+- The density matrix of a normalised state has trace 1 and symmetric
+  sector blocks (den_mat.f90:4-134).
+"""
+import numpy as np
+import pytest
+
+from grand_sql_tj_amd import capi, synth
+from grand_sql_tj_amd.lattice import TJModel
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def _bond_6x12_m4000():
+    model = TJModel(ny=6, nx=12, t=1.0, tp=0.0, J=0.4, mu=0.0, up_bias=1)
+    sb = synth.superblock(model, 35, 35, 4000, 4000, seed=7, median=40, sigma=0.8, maxdim=600, nsec=120,
+                          smax=12, filling=66 / 72, total_up=66)
+    return model, sb
+
+
+def _bond_6x16_m8000():
+    model = TJModel(ny=6, nx=16, t=1.0, tp=-0.2, J=0.4, mu=0.0, up_bias=1)
+    sb = synth.superblock(model, 47, 47, 8000, 8000, seed=2024, median=40, sigma=0.8, maxdim=600, nsec=140,
+                          smax=12, filling=0.875, total_up=84)
+    return model, sb
+
+
+@pytest.fixture(scope="module")
+def bond4000():
+    model, sb = _bond_6x12_m4000()
+    return model, sb, capi.Plan(model, sb, precisions=("fp64", "fp32", "bf16x3"))
+
+
+@pytest.fixture(scope="module")
+def bond8000():
+    model, sb = _bond_6x16_m8000()
+    return model, sb, capi.Plan(model, sb, precisions=("fp64", "bf16x3"))
+
+
+def _vec(sb, seed):
+    return torch.from_numpy(synth.random_vector(sb, seed)).cuda()
+
+
+def _apply(plan, x, precision="fp64"):
+    xx = x if precision == "fp64" else x.float()
+    y = torch.empty_like(xx)
+    plan.apply(xx, y, precision=precision)
+    torch.cuda.synchronize()
+    return y
+
+
+@pytest.mark.parametrize("which", ["bond4000", "bond8000"])
+def test_fp64_apply_symmetric_linear_repeatable(which, request):
+    model, sb, plan = request.getfixturevalue(which)
+    assert sb.dim > 300_000
+    x, y = _vec(sb, 1), _vec(sb, 2)
+    hx, hy = _apply(plan, x), _apply(plan, y)
+    assert torch.isfinite(hx).all() and hx.norm() > 0
+    lhs, rhs = float(torch.dot(x, hy)), float(torch.dot(hx, y))
+    assert abs(lhs - rhs) <= 1e-12 * float(hx.norm() * y.norm())
+    a, b = 0.75, -1.25
+    hz = _apply(plan, a * x + b * y)
+    assert float((hz - (a * hx + b * hy)).norm()) <= 1e-13 * float(hz.norm())
+    assert torch.equal(_apply(plan, x), hx)
+
+
+@pytest.mark.parametrize("which,precision,tol", [("bond4000", "fp32", 2e-6), ("bond4000", "bf16x3", 2e-5),
+                                                 ("bond8000", "bf16x3", 2e-5)])
+def test_mixed_apply_within_tolerance_and_repeatable(which, precision, tol, request):
+    model, sb, plan = request.getfixturevalue(which)
+    x = _vec(sb, 3)
+    h64 = _apply(plan, x)
+    hm = _apply(plan, x, precision)
+    rel = float((hm.double() - h64).norm() / h64.norm())
+    assert rel <= tol
+    assert torch.equal(_apply(plan, x, precision), hm)
+
+
+def test_mixed_solve_energy_per_site(bond8000):
+    model, sb, plan = bond8000
+    nsite = model.nsite
+    g = _vec(sb, 2024)
+    r64 = plan.lanczos(g, tol=1e-12, max_iter=200, precision="fp64")
+    noise = _vec(sb, 2025)
+    x0 = g + 0.01 * noise
+    x0 /= x0.norm()
+    psi = x0.clone()
+    rmp = plan.lanczos(psi, tol=1e-10, max_iter=40, precision="bf16x3", energy_tol=0.1 * 1e-8 * nsite)
+    assert abs(rmp["e0"] - r64["e0"]) / nsite <= 1e-8
+    ov = float(torch.dot(psi, g) / (psi.norm() * g.norm()))
+    assert 1.0 - abs(ov) <= 1e-6
+    assert rmp["residual"] < 1e-2 and rmp["energy_err_est"] <= 0.1 * 1e-8 * nsite
+
+
+def test_density_matrix_trace_and_symmetry(bond8000):
+    model, sb, plan = bond8000
+    psi = _vec(sb, 5)
+    for side, length in ((0, sb.sys_len + 1), (1, sb.env_len + 1)):
+        pg = capi.Program.density(side, length, sb.wsec, 1)
+        rho = torch.empty(pg.sizes["ndata"], dtype=torch.float64, device="cuda")
+        pg.run(None, psi, rho)
+        torch.cuda.synchronize()
+        _, sub, off = pg.layout()
+        tr, asym, big = 0.0, 0.0, 0.0
+        for (u, d, _, n, m), o in zip(sub, off):
+            blk = rho[int(o):int(o) + int(n) * int(m)].view(int(m), int(n)).T
+            tr += float(torch.trace(blk))
+            asym = max(asym, float((blk - blk.T).abs().max()))
+            big = max(big, float(blk.abs().max()))
+        assert abs(tr - 1.0) <= 1e-12
+        assert asym <= 1e-13 * max(1.0, big)
