@@ -82,6 +82,7 @@ EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_t
            "tjsq_block_update_create", "tjsq_trun_project_create", "tjsq_density_create", "tjsq_wave_transform_create",
            "tjsq_program_sizes",
            "tjsq_program_layout", "tjsq_program_tables", "tjsq_program_stats", "tjsq_program_run",
+           "tjsq_program_run_host", "tjsq_truncation_operator_host",
            "tjsq_program_profile", "tjsq_program_profile_read", "tjsq_program_destroy", "tjsq_truncation_operator",
            "tjsq_truncation_select", "tjsq_identity_truncation", "tjsq_batch_create", "tjsq_batch_offsets",
            "tjsq_batch_stats", "tjsq_batch_graph", "tjsq_batch_apply", "tjsq_batch_apply_f32", "tjsq_batch_profile",
@@ -135,6 +136,8 @@ def lib():
             "tjsq_program_profile_read": [P, P, P],
             "tjsq_program_destroy": [P],
             "tjsq_truncation_operator": [I, P, P, P, I, I, C.c_double, P, P, P, P, P, P],
+            "tjsq_truncation_operator_host": [I, P, P, P, I, I, C.c_double, P, P, P, P, P],
+            "tjsq_program_run_host": [P, P, P, P],
             "tjsq_truncation_select": [I, P, P, I, I, C.c_double, P, P, P, P],
             "tjsq_identity_truncation": [P, P, P, P, P, P],
             "tjsq_batch_create": [P, I, I, P, P],

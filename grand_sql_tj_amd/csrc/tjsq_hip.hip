@@ -1930,6 +1930,40 @@ int tjsq_program_profile_read(tjsq_program* P, double* ms, int64_t* launches) {
   return tjsq_plan_profile_read(P->dev, ms, launches);
 }
 
+// host-buffer form of tjsq_program_run for callers that keep their data on
+// the host (the Fortran drop-in): ops and x are copied to the device, the
+// program runs on the null stream and y comes back
+int tjsq_program_run_host(tjsq_program* P, const double* ops, const double* x, double* y) {
+  if (!P || !y || (!x && P->x_elems > 0) || (!ops && P->ops_elems > 0 && P->kind != 1))
+    return fail(TJSQ_ERR_ARG, "tjsq_program_run_host: null argument");
+  // density programs read no operators (ops is ignored for them) and take
+  // `levels` states of x_elems values each
+  const int64_t no = (ops && P->kind != 1) ? P->ops_elems : 0;
+  const int64_t nx = P->x_elems * (P->kind == 1 ? std::max(1, P->levels) : 1), ny = P->out.ndata;
+  double *dops = nullptr, *dx = nullptr, *dy = nullptr;
+  auto release = [&]() {
+    (void)hipFree(dops);
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+  };
+  if ((no > 0 && hipMalloc(&dops, sizeof(double) * no) != hipSuccess) ||
+      (nx > 0 && hipMalloc(&dx, sizeof(double) * nx) != hipSuccess) ||
+      hipMalloc(&dy, sizeof(double) * std::max<int64_t>(1, ny)) != hipSuccess) {
+    release();
+    return fail(TJSQ_ERR_NOMEM, "tjsq_program_run_host: staging allocation failed");
+  }
+  int rc = TJSQ_OK;
+  if (no > 0 && hipMemcpy(dops, ops, sizeof(double) * no, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(TJSQ_ERR_HIP, "tjsq_program_run_host: copy of ops to the device failed");
+  if (rc == TJSQ_OK && nx > 0 && hipMemcpy(dx, x, sizeof(double) * nx, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(TJSQ_ERR_HIP, "tjsq_program_run_host: copy of x to the device failed");
+  if (rc == TJSQ_OK) rc = tjsq_program_run(P, dops, dx, dy, nullptr);
+  if (rc == TJSQ_OK && ny > 0 && hipMemcpy(y, dy, sizeof(double) * ny, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(TJSQ_ERR_HIP, "tjsq_program_run_host: copy of y to the host failed");
+  release();
+  return rc;
+}
+
 int tjsq_program_destroy(tjsq_program* P) {
   if (!P) return TJSQ_OK;
   tjsq_plan_destroy(P->dev);
@@ -1950,6 +1984,31 @@ int tjsq_truncation_select(int nsec, const int* den_sub, const double* eigval, i
   if (order) std::memcpy(order, sel.order.data(), sizeof(int) * sel.order.size());
   fill_info(sel, info);
   return TJSQ_OK;
+}
+
+// host-buffer form of tjsq_truncation_operator: den (host) is copied to the
+// device, diagonalised there and the eigenvectors copied back over it
+int tjsq_truncation_operator_host(int nsec, const int* den_sub, const int64_t* den_off, double* den, int kept_min,
+                                  int kept_max, double trun_err, int* trun_sub, int64_t* trun_off, int* trun_num,
+                                  double* spectrum, tjsq_trunc_info* info) {
+  if (nsec < 1 || !den_sub || !den_off || !den)
+    return fail(TJSQ_ERR_ARG, "tjsq_truncation_operator_host: bad argument");
+  int64_t n = 0;
+  for (int k = 0; k < nsec; ++k)
+    n = std::max<int64_t>(n, den_off[k] + (int64_t)std::max(0, den_sub[5 * k + 3]) * std::max(0, den_sub[5 * k + 4]));
+  double* d = nullptr;
+  if (hipMalloc(&d, sizeof(double) * std::max<int64_t>(1, n)) != hipSuccess)
+    return fail(TJSQ_ERR_NOMEM, "tjsq_truncation_operator_host: staging allocation failed");
+  int rc = TJSQ_OK;
+  if (n > 0 && hipMemcpy(d, den, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(TJSQ_ERR_HIP, "tjsq_truncation_operator_host: copy to the device failed");
+  if (rc == TJSQ_OK)
+    rc = tjsq_truncation_operator(nsec, den_sub, den_off, d, kept_min, kept_max, trun_err, trun_sub, trun_off,
+                                  trun_num, spectrum, info, nullptr);
+  if (rc == TJSQ_OK && n > 0 && hipMemcpy(den, d, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(TJSQ_ERR_HIP, "tjsq_truncation_operator_host: copy to the host failed");
+  (void)hipFree(d);
+  return rc;
 }
 
 int tjsq_truncation_operator(int nsec, const int* den_sub, const int64_t* den_off, double* den, int kept_min,

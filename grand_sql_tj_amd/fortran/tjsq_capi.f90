@@ -40,7 +40,79 @@ module tjsq_capi
     real(c_double) :: gap, energy_err_est, seconds
   end type tjsq_lanczos_result
 
+  type, bind(C) :: tjsq_trunc_info
+    integer(c_int) :: num_state, dim
+    real(c_double) :: trunc_err, entropy, entropy1, entropy2
+  end type tjsq_trunc_info
+
   interface
+    integer(c_int) function tjsq_density_create(side, len, nw, wsec, levels, prog) bind(C, name="tjsq_density_create")
+      import :: c_int, c_ptr
+      integer(c_int), value :: side, len, nw, levels
+      integer(c_int), intent(in) :: wsec(*)
+      type(c_ptr), intent(out) :: prog
+    end function
+
+    integer(c_int) function tjsq_block_update_create(lat, side, sys_len, env_len, pre, basis, trun, prog) &
+        bind(C, name="tjsq_block_update_create")
+      import :: c_int, c_ptr, tjsq_lattice, tjsq_opset, tjsq_basis
+      type(tjsq_lattice), intent(in) :: lat
+      integer(c_int), value :: side, sys_len, env_len
+      type(tjsq_opset), intent(in) :: pre, trun
+      type(tjsq_basis), intent(in) :: basis
+      type(c_ptr), intent(out) :: prog
+    end function
+
+    integer(c_int) function tjsq_wave_transform_create(dir, sys_bs, env_bs, sys_trun, env_trun, nw_old, wsec_old, &
+        nw_new, wsec_new, prog) bind(C, name="tjsq_wave_transform_create")
+      import :: c_int, c_ptr, tjsq_opset, tjsq_basis
+      integer(c_int), value :: dir, nw_old, nw_new
+      type(tjsq_basis), intent(in) :: sys_bs, env_bs
+      type(tjsq_opset), intent(in) :: sys_trun, env_trun
+      integer(c_int), intent(in) :: wsec_old(*), wsec_new(*)
+      type(c_ptr), intent(out) :: prog
+    end function
+
+    integer(c_int) function tjsq_program_sizes(prog, sizes) bind(C, name="tjsq_program_sizes")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: prog
+      integer(c_int64_t), intent(out) :: sizes(14)
+    end function
+
+    integer(c_int) function tjsq_program_layout(prog, hdr, sub, off) bind(C, name="tjsq_program_layout")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: prog
+      integer(c_int), intent(out) :: hdr(*), sub(*)
+      integer(c_int64_t), intent(out) :: off(*)
+    end function
+
+    integer(c_int) function tjsq_program_run_host(prog, ops, x, y) bind(C, name="tjsq_program_run_host")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: prog
+      real(c_double), intent(in) :: ops(*), x(*)
+      real(c_double), intent(out) :: y(*)
+    end function
+
+    integer(c_int) function tjsq_program_destroy(prog) bind(C, name="tjsq_program_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: prog
+    end function
+
+    integer(c_int) function tjsq_truncation_operator_host(nsec, den_sub, den_off, den, kept_min, kept_max, trun_err, &
+        trun_sub, trun_off, trun_num, spectrum, info) bind(C, name="tjsq_truncation_operator_host")
+      import :: c_int, c_ptr, c_double, c_int64_t, tjsq_trunc_info
+      integer(c_int), value :: nsec, kept_min, kept_max
+      integer(c_int), intent(in) :: den_sub(*)
+      integer(c_int64_t), intent(in) :: den_off(*)
+      real(c_double), intent(inout) :: den(*)
+      real(c_double), value :: trun_err
+      integer(c_int), intent(out) :: trun_sub(*)
+      integer(c_int64_t), intent(out) :: trun_off(*)
+      integer(c_int), intent(out) :: trun_num
+      type(c_ptr), value :: spectrum
+      type(tjsq_trunc_info), intent(out) :: info
+    end function
+
     integer(c_int) function tjsq_plan_create(lat, sys_len, env_len, sys, env, sys_bs, env_bs, nw, wsec, &
         precision_mask, stream, plan) bind(C, name="tjsq_plan_create")
       import :: c_int, c_ptr, tjsq_lattice, tjsq_opset, tjsq_basis

@@ -6,6 +6,28 @@
 !   gpu_model_wave(sys,env,sys_bs,env_bs,invec,outvec)
 !        replaces model_wave (with hmerge), code/tjsq/get_sys.f90:4127-4134
 !
+! and the step after a solve (code/tjsq/dmrg_sub.f90:656-693, 778, 960):
+!
+!   gpu_Get_density_matrix_sys_new(sys_den,wave,levels) / _env_new
+!        replace Get_density_matrix_sys_new / _env_new, den_mat.f90:84-134
+!   gpu_Get_truncation_operator(den_mat,trun_op,trun_err,num_state)
+!        replaces Get_truncation_operator, den_mat.f90:141-383 (kept_min,
+!        kept_max from pubdata; sets Entropys; writes no log files)
+!   gpu_Get_Hamiltonian_Sys(pre,next,basis,trun) / _Env
+!        replace Get_Hamiltonian_Sys / _Env, get_sys.f90:4-478: pre becomes
+!        the projected model one site longer, next is left untouched (the
+!        reference uses it as scratch).  Operators spin_sz, which nothing on
+!        the solve path reads, are not carried.
+!   gpu_wave_trans_LR / gpu_wave_trans_RL(sys_bs,env_bs,sys_trun,env_trun,
+!        old_wave,new_wave) replace wave_trans_LR / _RL, dmrg_sub.f90:
+!        1093-1384; new_wave comes allocated with the new sectors.
+!
+! The block-update routines work at the bond of the last gpu_lanczos_memo_new
+! or gpu_model_wave call (the reference calls dmrg_setup(sys%len, env%len)
+! and lanczos_memo_new before each bond's update), or the one gpu_set_bond
+! sets.  Data stay on the host between calls, as in the reference: each call
+! copies its inputs to the device and its outputs back.
+!
 ! It is compiled inside the reference's build (it uses module pubdata,
 ! code/tjsq/common.f90) and calls libtjsq_hip.so through module tjsq_capi.
 ! The derived types are flattened into the C-ABI layout; the lattice
@@ -18,10 +40,15 @@ module tjsq_dropin
   use tjsq_capi
   implicit none
   private
-  public :: gpu_lanczos_memo_new, gpu_model_wave, tjsq_dropin_precision
+  public :: gpu_lanczos_memo_new, gpu_model_wave, tjsq_dropin_precision, gpu_set_bond
+  public :: gpu_Get_density_matrix_sys_new, gpu_Get_density_matrix_env_new, gpu_Get_truncation_operator
+  public :: gpu_Get_Hamiltonian_Sys, gpu_Get_Hamiltonian_Env, gpu_wave_trans_LR, gpu_wave_trans_RL
 
   ! contraction precision of the drop-in solve (TJSQ_FP64 by default)
   integer(c_int), save :: tjsq_dropin_precision = TJSQ_FP64
+
+  ! block lengths of the current bond
+  integer, save :: bond_sys_len = -1, bond_env_len = -1
 
   type flat_model
     integer(c_int), allocatable :: hdr(:, :), sub(:, :)
@@ -236,6 +263,7 @@ contains
       error stop 1
     end if
     call wave_sectors(eigvec(1), wsec)
+    call gpu_set_bond(sys%len, env%len)
     call make_plan(sys, env, sys_bs, env_bs, wsec, eigvec(1)%num, plan, fs, fe, bs, be)
     allocate(psi(eigvec(1)%dim))
     call pack_wave(eigvec(1), psi)
@@ -264,6 +292,7 @@ contains
     real(c_double), allocatable :: x(:), y(:)
     type(c_ptr) :: plan
     call wave_sectors(invec, wsec)
+    call gpu_set_bond(sys%len, env%len)
     call make_plan(sys, env, sys_bs, env_bs, wsec, invec%num, plan, fs, fe, bs, be)
     allocate(x(invec%dim), y(invec%dim))
     call pack_wave(invec, x)
@@ -271,4 +300,333 @@ contains
     call unpack_wave(y, outvec)
     call check(tjsq_plan_destroy(plan), 'tjsq_plan_destroy')
   end subroutine gpu_model_wave
+
+  subroutine gpu_set_bond(sys_len, env_len)
+    integer, intent(in) :: sys_len, env_len
+    bond_sys_len = sys_len
+    bond_env_len = env_len
+  end subroutine gpu_set_bond
+
+  ! one operator as a one-operator flat set
+  subroutine flatten_block(b, f)
+    type(Total_Block), intent(in) :: b
+    type(flat_model), intent(out) :: f
+    integer :: s, r, c, n, nd
+    n = count_subs(b)
+    nd = 0
+    do s = 1, n
+      nd = nd + b%sub(s)%row_dim * b%sub(s)%sdim
+    end do
+    allocate(f%hdr(4, 1), f%sub(5, max(1, n)), f%off(max(1, n)), f%data(max(1, nd)))
+    f%hdr(:, 1) = [b%len, b%up_dif, b%down_dif, n]
+    nd = 0
+    do s = 1, n
+      f%sub(:, s) = [b%sub(s)%num_up, b%sub(s)%num_down, b%sub(s)%down_dif, b%sub(s)%row_dim, b%sub(s)%sdim]
+      f%off(s) = nd
+      do c = 1, b%sub(s)%sdim
+        do r = 1, b%sub(s)%row_dim
+          f%data(nd + (c - 1) * b%sub(s)%row_dim + r) = b%sub(s)%mat(r, c)
+        end do
+      end do
+      nd = nd + b%sub(s)%row_dim * b%sub(s)%sdim
+    end do
+  end subroutine flatten_block
+
+  function opset_of(f, length, nops) result(o)
+    type(flat_model), intent(in), target :: f
+    integer, intent(in) :: length, nops
+    type(tjsq_opset) :: o
+    o = tjsq_opset(length, nops, c_loc(f%hdr), c_loc(f%sub), c_loc(f%off), c_loc(f%data), &
+                   int(size(f%data), c_int64_t), 0)
+  end function opset_of
+
+  function basis_of(bs, f) result(o)
+    type(Total_Basis), intent(in) :: bs
+    type(flat_basis), intent(in), target :: f
+    type(tjsq_basis) :: o
+    o = tjsq_basis(bs%len, bs%num, c_loc(f%sec), c_loc(f%sub))
+  end function basis_of
+
+  ! one operator of a flat layout (header hdr, sub-blocks s0 + 1 .. s0 + hdr(4)
+  ! of sub / off into data) into a Total_Block of total dimension dim
+  subroutine block_from_flat(b, hdr, sub, off, data, s0, dim)
+    type(Total_Block), intent(inout) :: b
+    integer(c_int), intent(in) :: hdr(4), sub(:, :)
+    integer(c_int64_t), intent(in) :: off(:)
+    real(c_double), intent(in) :: data(:)
+    integer, intent(in) :: s0, dim
+    integer :: s, r, c, rows, cols
+    integer(c_int64_t) :: o
+    if (allocated(b%sub)) deallocate(b%sub)
+    b%len = hdr(1)
+    b%up_dif = hdr(2)
+    b%down_dif = hdr(3)
+    b%num = hdr(4)
+    b%dim = dim
+    allocate(b%sub(b%num))
+    do s = 1, b%num
+      rows = sub(4, s0 + s)
+      cols = sub(5, s0 + s)
+      b%sub(s)%num_up = sub(1, s0 + s)
+      b%sub(s)%num_down = sub(2, s0 + s)
+      b%sub(s)%down_dif = sub(3, s0 + s)
+      b%sub(s)%row_dim = rows
+      b%sub(s)%sdim = cols
+      allocate(b%sub(s)%mat(rows, cols))
+      o = off(s0 + s)
+      do c = 1, cols
+        do r = 1, rows
+          b%sub(s)%mat(r, c) = data(o + (c - 1) * rows + r)
+        end do
+      end do
+    end do
+  end subroutine block_from_flat
+
+  ! a program's output layout and data on the host
+  subroutine program_output(prog, ops, x, hdr, sub, off, y)
+    type(c_ptr), intent(in) :: prog
+    real(c_double), intent(in) :: ops(:), x(:)
+    integer(c_int), allocatable, intent(out) :: hdr(:, :), sub(:, :)
+    integer(c_int64_t), allocatable, intent(out) :: off(:)
+    real(c_double), allocatable, intent(out) :: y(:)
+    integer(c_int64_t) :: sz(14)
+    call check(tjsq_program_sizes(prog, sz), 'tjsq_program_sizes')
+    allocate(hdr(4, max(1_c_int64_t, sz(1))), sub(5, max(1_c_int64_t, sz(2))), off(max(1_c_int64_t, sz(2))), &
+             y(max(1_c_int64_t, sz(3))))
+    call check(tjsq_program_layout(prog, hdr, sub, off), 'tjsq_program_layout')
+    call check(tjsq_program_run_host(prog, ops, x, y), 'tjsq_program_run_host')
+  end subroutine program_output
+
+  ! Get_density_matrix_sys_new / _env_new (den_mat.f90:84-134): the levels
+  ! states summed and normalised, rho = psi psi^T per sector (side 0) or
+  ! psi^T psi (side 1)
+  subroutine den_matrix(side, den, wave, levels)
+    integer, intent(in) :: side, levels
+    type(Total_Block), intent(inout) :: den
+    type(Wavefunction), intent(in) :: wave(levels)
+    integer(c_int), allocatable :: wsec(:, :), hdr(:, :), sub(:, :)
+    integer(c_int64_t), allocatable :: off(:)
+    real(c_double), allocatable :: psi(:), y(:)
+    real(c_double) :: none(1)
+    type(c_ptr) :: prog
+    integer :: l, n, s, dim, length
+    call wave_sectors(wave(1), wsec)
+    n = wave(1)%dim
+    allocate(psi(max(1, n * levels)))
+    do l = 1, levels
+      call pack_wave(wave(l), psi((l - 1) * n + 1:l * n))
+    end do
+    length = wave(1)%sys_len
+    if (side == 1) length = wave(1)%env_len
+    call check(tjsq_density_create(int(side, c_int), int(length, c_int), int(wave(1)%num, c_int), wsec, &
+                                   int(levels, c_int), prog), 'tjsq_density_create')
+    none = 0.0d0
+    call program_output(prog, none, psi, hdr, sub, off, y)
+    dim = 0
+    do s = 1, hdr(4, 1)
+      dim = dim + sub(5, s)
+    end do
+    call block_from_flat(den, hdr(:, 1), sub, off, y, 0, dim)
+    call check(tjsq_program_destroy(prog), 'tjsq_program_destroy')
+  end subroutine den_matrix
+
+  subroutine gpu_Get_density_matrix_sys_new(sys_den, wave, levels)
+    integer, intent(in) :: levels
+    type(Wavefunction), intent(in) :: wave(levels)
+    type(Total_Block), intent(inout) :: sys_den
+    call den_matrix(0, sys_den, wave, levels)
+  end subroutine gpu_Get_density_matrix_sys_new
+
+  subroutine gpu_Get_density_matrix_env_new(env_den, wave, levels)
+    integer, intent(in) :: levels
+    type(Wavefunction), intent(in) :: wave(levels)
+    type(Total_Block), intent(inout) :: env_den
+    call den_matrix(1, env_den, wave, levels)
+  end subroutine gpu_Get_density_matrix_env_new
+
+  ! Get_truncation_operator (den_mat.f90:141-383): every sector diagonalised
+  ! on the device, the kept states chosen as the reference chooses them
+  ! (kept_min, kept_max of pubdata, trun_err), their eigenvectors copied out
+  subroutine gpu_Get_truncation_operator(den_mat, trun_op, trun_err, num_state)
+    real(8), intent(in) :: trun_err
+    type(Total_Block), intent(in) :: den_mat
+    type(Total_Block), intent(inout) :: trun_op
+    integer, intent(inout) :: num_state
+    type(flat_model) :: f
+    integer(c_int), allocatable :: tsub(:, :)
+    integer(c_int64_t), allocatable :: toff(:)
+    integer(c_int) :: nt, hdr(4)
+    type(tjsq_trunc_info) :: info
+    integer :: n, s, dim
+    call flatten_block(den_mat, f)
+    n = max(1, count_subs(den_mat))
+    allocate(tsub(5, n), toff(n))
+    call check(tjsq_truncation_operator_host(int(count_subs(den_mat), c_int), f%sub, f%off, f%data, &
+                                             int(kept_min, c_int), int(kept_max, c_int), real(trun_err, c_double), &
+                                             tsub, toff, nt, c_null_ptr, info), 'tjsq_truncation_operator_host')
+    dim = 0
+    do s = 1, nt
+      dim = dim + tsub(5, s)
+    end do
+    hdr = [int(den_mat%len, c_int), int(den_mat%up_dif, c_int), int(den_mat%down_dif, c_int), nt]
+    call block_from_flat(trun_op, hdr, tsub, toff, f%data, 0, dim)
+    num_state = info%num_state
+    Entropys = info%entropy
+  end subroutine gpu_Get_truncation_operator
+
+  ! Get_Hamiltonian_Sys (side 0) / _Env (side 1), get_sys.f90:4-478
+  subroutine enlarge_block(side, pre, basis, trun)
+    integer, intent(in) :: side
+    type(Total_Model), intent(inout), target :: pre
+    type(Total_Basis), intent(in) :: basis
+    type(Total_Block), intent(in) :: trun
+    type(flat_model), target :: fp, ft
+    type(flat_basis), target :: fb
+    type(tjsq_lattice) :: lat
+    type(c_ptr) :: prog
+    integer(c_int), allocatable :: hdr(:, :), sub(:, :)
+    integer(c_int64_t), allocatable :: off(:)
+    real(c_double), allocatable :: y(:)
+    integer :: nops, k, s0, len0
+    len0 = bond_sys_len
+    if (side == 1) len0 = bond_env_len
+    if (bond_sys_len < 0 .or. pre%len /= len0) then
+      write(*, *) 'tjsq: the block does not belong to the current bond (gpu_set_bond)'
+      error stop 1
+    end if
+    call flatten_model(pre, fp, nops)
+    call flatten_block(trun, ft)
+    call flatten_basis(basis, fb)
+    lat = lattice_from_pubdata()
+    call check(tjsq_block_update_create(lat, int(side, c_int), int(bond_sys_len, c_int), int(bond_env_len, c_int), &
+                                        opset_of(fp, pre%len, nops), basis_of(basis, fb), opset_of(ft, trun%len, 1), &
+                                        prog), 'tjsq_block_update_create')
+    call program_output(prog, ft%data, fp%data, hdr, sub, off, y)
+    call check(tjsq_program_destroy(prog), 'tjsq_program_destroy')
+    ! the new model replaces pre (ham, then elec_up, elec_down, spin_sd,
+    ! num_sn of each active site; the other sites' operators are released)
+    do k = 1, nxx
+      call drop_block(pre%sub(k)%elec_up)
+      call drop_block(pre%sub(k)%elec_down)
+      call drop_block(pre%sub(k)%spin_sd)
+      call drop_block(pre%sub(k)%spin_sz)
+      call drop_block(pre%sub(k)%num_sn)
+    end do
+    if ((size(hdr, 2) - 1) / 4 > nxx) then
+      write(*, *) 'tjsq: the new block has more active sites than nxx'
+      error stop 1
+    end if
+    s0 = 0
+    do k = 1, size(hdr, 2)
+      call set_op(pre, k, hdr(:, k), sub, off, y, s0, trun_dim(trun))
+      s0 = s0 + hdr(4, k)
+    end do
+    pre%len = pre%len + 1
+  end subroutine enlarge_block
+
+  ! operator k of a model (the order of flatten_model) from a flat layout
+  subroutine set_op(m, k, hdr, sub, off, data, s0, dim)
+    type(Total_Model), intent(inout) :: m
+    integer, intent(in) :: k, s0, dim
+    integer(c_int), intent(in) :: hdr(4), sub(:, :)
+    integer(c_int64_t), intent(in) :: off(:)
+    real(c_double), intent(in) :: data(:)
+    integer :: site
+    if (k == 1) then
+      call block_from_flat(m%ham, hdr, sub, off, data, s0, dim)
+      return
+    end if
+    site = (k - 2) / 4 + 1
+    select case (mod(k - 2, 4))
+    case (0)
+      call block_from_flat(m%sub(site)%elec_up, hdr, sub, off, data, s0, dim)
+    case (1)
+      call block_from_flat(m%sub(site)%elec_down, hdr, sub, off, data, s0, dim)
+    case (2)
+      call block_from_flat(m%sub(site)%spin_sd, hdr, sub, off, data, s0, dim)
+    case default
+      call block_from_flat(m%sub(site)%num_sn, hdr, sub, off, data, s0, dim)
+    end select
+  end subroutine set_op
+
+  integer function trun_dim(t)
+    type(Total_Block), intent(in) :: t
+    integer :: s
+    trun_dim = 0
+    do s = 1, count_subs(t)
+      trun_dim = trun_dim + t%sub(s)%sdim
+    end do
+  end function trun_dim
+
+  subroutine drop_block(b)
+    type(Total_Block), intent(inout) :: b
+    if (allocated(b%sub)) deallocate(b%sub)
+    b%num = 0
+    b%dim = 0
+  end subroutine drop_block
+
+  subroutine gpu_Get_Hamiltonian_Sys(pre, next, basis, trun)
+    type(Total_Basis), intent(in) :: basis
+    type(Total_Model), intent(inout), target :: pre
+    type(Total_Model), intent(inout) :: next
+    type(Total_Block), intent(in) :: trun
+    call enlarge_block(0, pre, basis, trun)
+  end subroutine gpu_Get_Hamiltonian_Sys
+
+  subroutine gpu_Get_Hamiltonian_Env(pre, next, basis, trun)
+    type(Total_Basis), intent(in) :: basis
+    type(Total_Model), intent(inout), target :: pre
+    type(Total_Model), intent(inout) :: next
+    type(Total_Block), intent(in) :: trun
+    call enlarge_block(1, pre, basis, trun)
+  end subroutine gpu_Get_Hamiltonian_Env
+
+  ! wave_trans_LR (dir 0) / wave_trans_RL (dir 1), dmrg_sub.f90:1093-1384,
+  ! with wave_normalize of the result
+  subroutine move_wave(dir, sys_bs, env_bs, sys_trun, env_trun, old_wave, new_wave)
+    integer, intent(in) :: dir
+    type(Total_Basis), intent(in) :: sys_bs, env_bs
+    type(Total_Block), intent(in) :: sys_trun, env_trun
+    type(Wavefunction), intent(in) :: old_wave
+    type(Wavefunction), intent(inout) :: new_wave
+    type(flat_basis), target :: fs, fe
+    type(flat_model), target :: ts, te
+    integer(c_int), allocatable :: wold(:, :), wnew(:, :), hdr(:, :), sub(:, :)
+    integer(c_int64_t), allocatable :: off(:)
+    real(c_double), allocatable :: x(:), y(:), ops(:)
+    type(c_ptr) :: prog
+    call flatten_basis(sys_bs, fs)
+    call flatten_basis(env_bs, fe)
+    call flatten_block(sys_trun, ts)
+    call flatten_block(env_trun, te)
+    call wave_sectors(old_wave, wold)
+    call wave_sectors(new_wave, wnew)
+    call check(tjsq_wave_transform_create(int(dir, c_int), basis_of(sys_bs, fs), basis_of(env_bs, fe), &
+                                          opset_of(ts, sys_trun%len, 1), opset_of(te, env_trun%len, 1), &
+                                          int(old_wave%num, c_int), wold, int(new_wave%num, c_int), wnew, prog), &
+               'tjsq_wave_transform_create')
+    allocate(x(max(1, old_wave%dim)), ops(size(ts%data) + size(te%data)))
+    call pack_wave(old_wave, x)
+    ops(1:size(ts%data)) = ts%data
+    ops(size(ts%data) + 1:) = te%data
+    call program_output(prog, ops, x, hdr, sub, off, y)
+    call check(tjsq_program_destroy(prog), 'tjsq_program_destroy')
+    call unpack_wave(y, new_wave)
+  end subroutine move_wave
+
+  subroutine gpu_wave_trans_LR(sys_bs, env_bs, sys_trun, env_trun, old_wave, new_wave)
+    type(Total_Basis), intent(in) :: sys_bs, env_bs
+    type(Total_Block), intent(in) :: sys_trun, env_trun
+    type(Wavefunction), intent(in) :: old_wave
+    type(Wavefunction), intent(inout) :: new_wave
+    call move_wave(0, sys_bs, env_bs, sys_trun, env_trun, old_wave, new_wave)
+  end subroutine gpu_wave_trans_LR
+
+  subroutine gpu_wave_trans_RL(sys_bs, env_bs, sys_trun, env_trun, old_wave, new_wave)
+    type(Total_Basis), intent(in) :: sys_bs, env_bs
+    type(Total_Block), intent(in) :: sys_trun, env_trun
+    type(Wavefunction), intent(in) :: old_wave
+    type(Wavefunction), intent(inout) :: new_wave
+    call move_wave(1, sys_bs, env_bs, sys_trun, env_trun, old_wave, new_wave)
+  end subroutine gpu_wave_trans_RL
 end module tjsq_dropin

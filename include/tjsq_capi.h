@@ -341,6 +341,14 @@ int tjsq_program_run(tjsq_program* prog, const double* ops, const double* x, dou
 int tjsq_program_profile(tjsq_program* prog, int enable);
 int tjsq_program_profile_read(tjsq_program* prog, double* ms, int64_t* launches);
 int tjsq_program_destroy(tjsq_program* prog);
+/* tjsq_program_run on host buffers (ops, x in; y out), for callers whose data
+ * stay on the host, as the reference's own routines keep theirs: the Fortran
+ * drop-in of Get_density_matrix_*_new, Get_Hamiltonian_Sys/Env and
+ * wave_trans_LR/RL (code/tjsq/den_mat.f90:84-134, get_sys.f90:4-478,
+ * dmrg_sub.f90:1093-1384).  x holds what tjsq_program_run's x holds (for a
+ * density program, `levels` states back to back).  Runs on the null stream;
+ * returns when y is on the host. */
+int tjsq_program_run_host(tjsq_program* prog, const double* ops, const double* x, double* y);
 
 typedef struct tjsq_trunc_info {
   int num_state;       /* kept states (Num_state) */
@@ -363,6 +371,14 @@ typedef struct tjsq_trunc_info {
 int tjsq_truncation_operator(int nsec, const int* den_sub, const int64_t* den_off, double* den, int kept_min,
                              int kept_max, double trun_err, int* trun_sub, int64_t* trun_off, int* trun_num,
                              double* spectrum, tjsq_trunc_info* info, void* stream);
+
+/* tjsq_truncation_operator with den in host memory (the Fortran drop-in of
+ * Get_truncation_operator, code/tjsq/den_mat.f90:141-383): den is copied to
+ * the device, diagonalised there, and the eigenvectors are copied back over
+ * it; trun_off indexes into the host den. */
+int tjsq_truncation_operator_host(int nsec, const int* den_sub, const int64_t* den_off, double* den, int kept_min,
+                                  int kept_max, double trun_err, int* trun_sub, int64_t* trun_off, int* trun_num,
+                                  double* spectrum, tjsq_trunc_info* info);
 
 /* Host part of Get_truncation_operator on given per-sector ascending
  * eigenvalues (no GPU): kept count per sector, sorted spectrum and the

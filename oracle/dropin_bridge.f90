@@ -79,3 +79,153 @@ integer(c_int) function dropin_lanczos(sys_len, env_len, sys_nops, sys_hdr, sys_
   call free_model(env)
   dropin_lanczos = 0
 end function dropin_lanczos
+
+! Get_density_matrix_sys_new / _env_new through the drop-in (as refb_density);
+! result held for refb_result_sizes / refb_result_copy.
+integer(c_int) function dropin_density(side, sys_len, env_len, nw, wsec, levels, psi) bind(C, name="dropin_density")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  use tjsq_dropin
+  implicit none
+  integer(c_int), value :: side, sys_len, env_len, nw, levels
+  integer(c_int), intent(in) :: wsec(6, *)
+  real(c_double), intent(in) :: psi(*)
+  type(Wavefunction), allocatable :: w(:)
+  integer :: l, i
+  integer(8) :: d
+  d = 0
+  do i = 1, nw
+    d = d + int(wsec(5, i), 8) * wsec(6, i)
+  end do
+  allocate(w(levels))
+  do l = 1, levels
+    call put_wave(w(l), sys_len, env_len, nw, wsec, psi((l - 1) * d + 1))
+  end do
+  call set_result(1)
+  if (side == 0) then
+    call gpu_Get_density_matrix_sys_new(res(1), w, levels)
+  else
+    call gpu_Get_density_matrix_env_new(res(1), w, levels)
+  end if
+  do l = 1, levels
+    call deallocate_wave(w(l))
+  end do
+  dropin_density = 0
+end function dropin_density
+
+! Get_truncation_operator through the drop-in (as refb_truncation).
+integer(c_int) function dropin_truncation(den_hdr, den_sub, den_off, den_data, kmin, kmax, terr, num_state, &
+    entropy) bind(C, name="dropin_truncation")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  use tjsq_dropin
+  implicit none
+  integer(c_int), intent(in) :: den_hdr(4), den_sub(5, *)
+  integer(c_int64_t), intent(in) :: den_off(*)
+  real(c_double), intent(in) :: den_data(*)
+  integer(c_int), value :: kmin, kmax
+  real(c_double), value :: terr
+  integer(c_int), intent(out) :: num_state
+  real(c_double), intent(out) :: entropy
+  type(Total_Block) :: den
+  integer :: ns
+  call put_block(den, den_hdr(1), den_hdr, den_sub, den_off, den_data)
+  kept_min = kmin
+  kept_max = kmax
+  call set_result(1)
+  ns = 0
+  call gpu_Get_truncation_operator(den, res(1), terr, ns)
+  num_state = ns
+  entropy = Entropys
+  call free_block(den)
+  dropin_truncation = 0
+end function dropin_truncation
+
+! Get_Hamiltonian_Sys / _Env through the drop-in (as refb_get_hamiltonian).
+integer(c_int) function dropin_get_hamiltonian(side, sys_len, env_len, pre_nops, pre_hdr, pre_sub, pre_off, &
+    pre_data, blen, nsec, bsec, bsub, trun_hdr, trun_sub, trun_off, trun_data) bind(C, name="dropin_get_hamiltonian")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  use tjsq_dropin
+  implicit none
+  integer(c_int), value :: side, sys_len, env_len, pre_nops, blen, nsec
+  integer(c_int), intent(in) :: pre_hdr(4, *), pre_sub(5, *), bsec(4, *), bsub(6, *), trun_hdr(4), trun_sub(5, *)
+  integer(c_int64_t), intent(in) :: pre_off(*), trun_off(*)
+  real(c_double), intent(in) :: pre_data(*), trun_data(*)
+  type(Total_Model) :: pre, nxt
+  type(Total_Basis) :: basis
+  type(Total_Block) :: trun
+  integer :: plen, n1, ii
+  plen = sys_len
+  if (side == 1) plen = env_len
+  call put_model(pre, plen, pre_nops, pre_hdr, pre_sub, pre_off, pre_data)
+  call put_basis(basis, blen, nsec, bsec, bsub)
+  call put_block(trun, blen, trun_hdr, trun_sub, trun_off, trun_data)
+  call dmrg_setup(sys_len, env_len)
+  call gpu_set_bond(sys_len, env_len)
+  if (side == 0) then
+    call gpu_Get_Hamiltonian_Sys(pre, nxt, basis, trun)
+  else
+    call gpu_Get_Hamiltonian_Env(pre, nxt, basis, trun)
+  end if
+  n1 = nleg11(plen + 1)
+  call set_result(1 + 4 * n1)
+  call block_transfer(pre%ham, res(1))
+  do ii = 1, n1
+    call block_transfer(pre%sub(ii)%elec_up, res(2 + 4 * (ii - 1)))
+    call block_transfer(pre%sub(ii)%elec_down, res(3 + 4 * (ii - 1)))
+    call block_transfer(pre%sub(ii)%spin_sd, res(4 + 4 * (ii - 1)))
+    call block_transfer(pre%sub(ii)%num_sn, res(5 + 4 * (ii - 1)))
+  end do
+  call free_model(pre)
+  call free_block(trun)
+  dropin_get_hamiltonian = 0
+end function dropin_get_hamiltonian
+
+! wave_trans_LR / wave_trans_RL through the drop-in (as refb_wave_trans).
+integer(c_int) function dropin_wave_trans(dir, sys_nsec, sys_bsec, sys_bsub, env_nsec, env_bsec, env_bsub, &
+    st_hdr, st_sub, st_off, st_data, et_hdr, et_sub, et_off, et_data, nw_old, wsec_old, x, nw_new, wsec_new, y) &
+    bind(C, name="dropin_wave_trans")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  use tjsq_dropin
+  implicit none
+  integer(c_int), value :: dir, sys_nsec, env_nsec, nw_old, nw_new
+  integer(c_int), intent(in) :: sys_bsec(4, *), sys_bsub(6, *), env_bsec(4, *), env_bsub(6, *)
+  integer(c_int), intent(in) :: st_hdr(4), st_sub(5, *), et_hdr(4), et_sub(5, *), wsec_old(6, *), wsec_new(6, *)
+  integer(c_int64_t), intent(in) :: st_off(*), et_off(*)
+  real(c_double), intent(in) :: st_data(*), et_data(*), x(*)
+  real(c_double), intent(inout) :: y(*)
+  type(Total_Basis) :: sbs, ebs
+  type(Total_Block) :: st, et
+  type(Wavefunction) :: ow, nw
+  real(8), allocatable :: zero(:)
+  integer(8) :: dn
+  integer :: i
+  call put_basis(sbs, 0, sys_nsec, sys_bsec, sys_bsub)
+  call put_basis(ebs, 0, env_nsec, env_bsec, env_bsub)
+  call put_block(st, st_hdr(1), st_hdr, st_sub, st_off, st_data)
+  call put_block(et, et_hdr(1), et_hdr, et_sub, et_off, et_data)
+  call put_wave(ow, 0, 0, nw_old, wsec_old, x)
+  dn = 0
+  do i = 1, nw_new
+    dn = dn + int(wsec_new(5, i), 8) * wsec_new(6, i)
+  end do
+  allocate(zero(max(dn, 1_8)))
+  zero = 0.0d0
+  call put_wave(nw, 0, 0, nw_new, wsec_new, zero)
+  if (dir == 0) then
+    call gpu_wave_trans_LR(sbs, ebs, st, et, ow, nw)
+  else
+    call gpu_wave_trans_RL(sbs, ebs, st, et, ow, nw)
+  end if
+  call get_wave(nw, y)
+  call free_block(st)
+  call free_block(et)
+  deallocate(sbs%sub, ebs%sub, ow%sub, nw%sub, zero)
+  dropin_wave_trans = 0
+end function dropin_wave_trans

@@ -299,6 +299,15 @@ def dropin_lib():
             lib.dropin_model_wave.argtypes = common + [F64P, F64P]
             lib.dropin_lanczos.restype = C.c_int
             lib.dropin_lanczos.argtypes = common + [F64P, C.POINTER(C.c_double)]
+            ref = ref_lib()
+            # the drop-in test entry points take the reference bridge's
+            # arguments (dropin_truncation without system_block)
+            for name in ("density", "truncation", "get_hamiltonian", "wave_trans"):
+                getattr(lib, "dropin_" + name).restype = C.c_int
+            lib.dropin_density.argtypes = ref.refb_density.argtypes
+            lib.dropin_get_hamiltonian.argtypes = ref.refb_get_hamiltonian.argtypes
+            lib.dropin_wave_trans.argtypes = ref.refb_wave_trans.argtypes
+            lib.dropin_truncation.argtypes = ref.refb_truncation.argtypes[:7] + ref.refb_truncation.argtypes[8:]
             _libs["dropin"] = lib
         return _libs["dropin"]
 
@@ -512,6 +521,39 @@ def ref_truncation(den, kept_min, kept_max, trun_err, workdir, system_block=True
         finally:
             os.chdir(cwd)
         return _take_ref(), ns.value, ent.value
+
+
+def dropin_density_matrix(side, sys_len, env_len, wsec, psi, levels=1):
+    """Get_density_matrix_sys/env_new through the product's Fortran drop-in
+    (as ref_density_matrix)."""
+    w = np.ascontiguousarray(wsec, np.int32).reshape(-1, 6)
+    with _lock:
+        dropin_lib().dropin_density(side, sys_len, env_len, w.shape[0], w, levels, np.ascontiguousarray(psi, np.float64))
+        return _take_ref()
+
+
+def dropin_truncation(den, kept_min, kept_max, trun_err):
+    """Get_truncation_operator through the drop-in (as ref_truncation)."""
+    ns, ent = C.c_int(0), C.c_double(0.0)
+    with _lock:
+        dropin_lib().dropin_truncation(*_one(den), kept_min, kept_max, trun_err, C.byref(ns), C.byref(ent))
+        return _take_ref(), ns.value, ent.value
+
+
+def dropin_get_hamiltonian(model, side, sys_len, env_len, pre, basis, trun):
+    """Get_Hamiltonian_Sys / _Env through the drop-in (as ref_get_hamiltonian)."""
+    with _lock:
+        _ref_init(model)
+        dropin_lib().dropin_get_hamiltonian(side, sys_len, env_len, *_m(pre), basis.length, *_bs(basis), *_one(trun))
+        return _take_ref()
+
+
+def dropin_wave_trans(direction, sys_bs, env_bs, sys_trun, env_trun, wsec_old, x, wsec_new):
+    """wave_trans_LR / wave_trans_RL through the drop-in (as ref_wave_trans)."""
+    args, y = _wave_trans_args(direction, sys_bs, env_bs, sys_trun, env_trun, wsec_old, x, wsec_new)
+    with _lock:
+        dropin_lib().dropin_wave_trans(*args)
+    return y
 
 
 def ref_quicksort(arr):
