@@ -134,3 +134,19 @@ def test_serving_footprint_counts_the_plan():
     assert f64 >= 8 * (hp.arena_elems + hp.scratch_elems) + 42 * big.dim * 8
     assert f32 > f64 - 42 * big.dim * 4          # extra FP32 operands outweigh the smaller basis
     assert serve.footprint(m, small, "fp64", 40) < f64
+
+
+def test_host_buffer_entry_points_reject_bad_arguments():
+    """The host-buffer forms the Fortran drop-in calls check their arguments
+    before any device work (no GPU is touched here)."""
+    lib = capi.lib()
+    lib.tjsq_program_run_host.restype = C.c_int
+    lib.tjsq_program_run_host.argtypes = [C.c_void_p] * 4
+    assert lib.tjsq_program_run_host(None, None, None, None) != 0
+    assert b"tjsq_program_run_host" in lib.tjsq_last_error()
+    lib.tjsq_truncation_operator_host.restype = C.c_int
+    sub = np.array([0, 0, 0, 2, 2], np.int32)
+    off = np.zeros(1, np.int64)
+    assert lib.tjsq_truncation_operator_host(0, sub.ctypes.data, off.ctypes.data, None, 1, 2, C.c_double(0.0),
+                                             None, None, None, None, None) != 0
+    assert b"tjsq_truncation_operator_host" in lib.tjsq_last_error()
