@@ -1,5 +1,4 @@
 """Batched grouped launch, hipGraph replay and work-item shapes on the GPU."""
-import numpy as np
 import pytest
 
 import oracle

@@ -14,7 +14,6 @@ the mixed-precision solve (t' = -0.2, 84 electrons, bond 47|s|s|47).
 - The density matrix of a normalised state has trace 1 and symmetric
   sector blocks (den_mat.f90:4-134).
 """
-import numpy as np
 import pytest
 
 from grand_sql_tj_amd import capi, synth

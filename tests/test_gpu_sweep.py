@@ -11,7 +11,6 @@ reference."""
 import json
 import os
 
-import numpy as np
 import pytest
 
 from grand_sql_tj_amd.lattice import TJModel
