@@ -26,6 +26,8 @@ and trun_err as given.
 from __future__ import annotations
 
 import math
+import os
+import re
 import time
 from dataclasses import dataclass, field
 
@@ -114,6 +116,7 @@ class DMRG:
         self.rng = np.random.default_rng(self.seed)
         self.psi = None          # ground state of the last bond (torch float64)
         self.wsec = None         # its sector list
+        self.bond = None         # (sys_len, env_len) of that bond
         if self.kept_min <= 0:
             self.kept_min = min(3 * self.kept_max // 4, 3600)
 
@@ -149,6 +152,7 @@ class DMRG:
         info = self._timed("solve", lambda: plan.lanczos(psi, tol=self.tol, max_iter=self.max_iter,
                                                          precision=self.precision))
         self.psi, self.wsec = psi, wsec
+        self.bond = (blk_s.length, blk_e.length)
         return info, time.perf_counter() - t0
 
     def _truncate(self, side, sys_len, env_len, wsec, basis, psi, allow_identity):
@@ -270,6 +274,70 @@ class DMRG:
             self.env[N - label] = self._timed("grow", self._grow, 1, e, env_bs, trun, label - 1, N - label - 1)
             self.env_trun[N - label], self.env_basis[N - label] = trun, env_bs
             label -= 1
+
+    # ---- checkpoint files (block.f90:2760-3097) ---------------------------
+    @staticmethod
+    def _host(dev) -> OpSet:
+        n = int(dev.ops.data.size)
+        return OpSet(dev.ops.length, dev.ops.hdr, dev.ops.sub, dev.ops.off, dev.data[:n].cpu().numpy())
+
+    def save_checkpoint(self, directory):
+        """Write what the run holds as the reference's checkpoint files:
+        sysblocks<L> / envblocks<L> for every block model, systruns<L> /
+        envtruns<L> for the truncation that made block L, sysbs<L> / envbs<L>
+        for the fused basis (block L-1 + site), and eigvec1 for the ground
+        state of the last bond (model_to_disk, truns_to_disk, basis_to_disk,
+        wave_to_disk, block.f90:2760-3045; names of Get_model_name,
+        dmrg_sub.f90:294).  The reference, or load_checkpoint, resumes from
+        them."""
+        from . import checkpoint as ck
+        os.makedirs(directory, exist_ok=True)
+        sides = ((0, self.sys, self.sys_trun, self.sys_basis), (1, self.env, self.env_trun, self.env_basis))
+        for side, blocks, truns, bases in sides:
+            for L, blk in blocks.items():
+                ck.save_model(os.path.join(directory, ck.file_name(side, "blocks", L)), self._host(blk))
+            for L, t in truns.items():
+                ck.save_trun(os.path.join(directory, ck.file_name(side, "truns", L)), self._host(t))
+            for L, b in bases.items():
+                ck.save_basis(os.path.join(directory, ck.file_name(side, "bs", L)), b)
+        if self.psi is not None:
+            ck.save_wave(os.path.join(directory, ck.wave_file(1)), self.bond[0] + 1, self.bond[1] + 1, self.wsec,
+                         self.psi.cpu().numpy())
+
+    def load_checkpoint(self, directory):
+        """Read checkpoint files written by save_checkpoint or by the
+        reference (model_from_disk, truns_from_disk, basis_from_disk,
+        wave_from_disk) into the run: block models and truncation operators
+        go to device memory, bases stay on the host, and eigvec1 becomes the
+        start vector of the next solve at its bond.  A sweep
+        (left_to_right / right_to_left) then continues from them without the
+        warm-up."""
+        from . import checkpoint as ck
+        pat = re.compile(r"^(sys|env)(blocks|truns|bs)([0-9]+)$")
+        for name in sorted(os.listdir(directory)):
+            m = pat.match(name)
+            if not m:
+                continue
+            side, kind, L = 0 if m.group(1) == "sys" else 1, m.group(2), int(m.group(3))
+            path = os.path.join(directory, name)
+            if kind == "blocks":
+                ops, _ = ck.load_model(path)
+                if ops.length != L:
+                    raise ck.CheckpointError(f"{path}: holds a block of length {ops.length}")
+                (self.sys if side == 0 else self.env)[L] = self._block(ops)
+            elif kind == "truns":
+                t, _ = ck.load_trun(path)
+                trun = DeviceTrun(OpSet(L, t.hdr, t.sub, t.off, np.zeros(max(1, t.data.size))),
+                                  self._device(t.data if t.data.size else np.zeros(1)))
+                (self.sys_trun if side == 0 else self.env_trun)[L] = trun
+            else:
+                b, _ = ck.load_basis(path)
+                (self.sys_basis if side == 0 else self.env_basis)[L] = b
+        wave = os.path.join(directory, ck.wave_file(1))
+        if os.path.exists(wave):
+            w = ck.load_wave(wave)
+            self.psi, self.wsec = self._device(w.x), w.wsec
+            self.bond = (w.sys_len - 1, w.env_len - 1)
 
     def run(self, sweeps=2):
         """Warm-up, then `sweeps` pairs of half sweeps (Get_Ground_State with

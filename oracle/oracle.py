@@ -164,6 +164,28 @@ def ref_lib():
                                             + [C.c_int, I32P, F64P, C.c_int, I32P, F64P])
             lib.refb_flush.restype = C.c_int
             lib.refb_flush.argtypes = []
+            lib.refb_result_dims.restype = C.c_int
+            lib.refb_result_dims.argtypes = [I32P]
+            lib.refb_model_to_disk.restype = C.c_int
+            lib.refb_model_to_disk.argtypes = [C.c_int] + margs + [I32P, C.c_int, C.c_int]
+            lib.refb_model_from_disk.restype = C.c_int
+            lib.refb_model_from_disk.argtypes = [C.c_int, C.c_int]
+            lib.refb_truns_to_disk.restype = C.c_int
+            lib.refb_truns_to_disk.argtypes = [I32P, I32P, I64P, F64P, C.c_int, C.c_int, C.c_int]
+            lib.refb_truns_from_disk.restype = C.c_int
+            lib.refb_truns_from_disk.argtypes = [C.c_int, C.c_int]
+            lib.refb_basis_to_disk.restype = C.c_int
+            lib.refb_basis_to_disk.argtypes = [C.c_int, C.c_int, I32P, I32P, C.c_int, C.c_int]
+            lib.refb_basis_from_disk.restype = C.c_int
+            lib.refb_basis_from_disk.argtypes = [C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 3
+            lib.refb_basis_copy.restype = C.c_int
+            lib.refb_basis_copy.argtypes = [I32P, I32P, I32P]
+            lib.refb_wave_to_disk.restype = C.c_int
+            lib.refb_wave_to_disk.argtypes = [C.c_int, C.c_int, C.c_int, I32P, F64P, C.c_char_p, C.c_int]
+            lib.refb_wave_from_disk.restype = C.c_int
+            lib.refb_wave_from_disk.argtypes = [C.c_char_p, C.c_int] + [C.POINTER(C.c_int)] * 4
+            lib.refb_wave_copy.restype = C.c_int
+            lib.refb_wave_copy.argtypes = [I32P, F64P]
             _libs["ref"] = lib
         return _libs["ref"]
 
@@ -554,6 +576,103 @@ def dropin_wave_trans(direction, sys_bs, env_bs, sys_trun, env_trun, wsec_old, x
     with _lock:
         dropin_lib().dropin_wave_trans(*args)
     return y
+
+
+# ---------------------------------------------------------------------------
+# checkpoint files: the reference's own writers and readers (block.f90:2760-3097)
+# ---------------------------------------------------------------------------
+class _cwd:
+    def __init__(self, path):
+        self.path = os.fspath(path)
+
+    def __enter__(self):
+        self.saved = os.getcwd()
+        os.chdir(self.path)
+
+    def __exit__(self, *exc):
+        os.chdir(self.saved)
+
+
+def _result_dims(n):
+    d = np.zeros(max(1, n), np.int32)
+    ref_lib().refb_result_dims(d)
+    return d[:n].astype(np.int64)
+
+
+def ref_result_dims():
+    """The `dim` fields of the operators the last reference call produced."""
+    with _lock:
+        no, ns, nd = C.c_int(0), C.c_int(0), C.c_int64(0)
+        ref_lib().refb_result_sizes(C.byref(no), C.byref(ns), C.byref(nd))
+        return _result_dims(no.value)
+
+
+def ref_model_to_disk(model, workdir, ops, dims, idx, system=True):
+    """model_to_disk of the reference build into workdir (the model of a
+    lattice like ``model``'s: 1 + 4 nleg11(len) operators)."""
+    with _lock, _cwd(workdir):
+        lib = _ref_init(model)
+        rc = lib.refb_model_to_disk(ops.length, *_m(ops), np.ascontiguousarray(dims, np.int32), idx, int(system))
+    if rc != 0:
+        raise RuntimeError("model does not hold 1 + 4 nleg11(len) operators")
+
+
+def ref_model_from_disk(model, workdir, idx, system=True):
+    """model_from_disk of the reference build: (ops, dims)."""
+    with _lock, _cwd(workdir):
+        lib = _ref_init(model)
+        length = lib.refb_model_from_disk(idx, int(system))
+        ops = _take_ref(length)
+        return ops, _result_dims(ops.nops)
+
+
+def ref_truns_to_disk(workdir, trun, dim, idx, system=True):
+    with _lock, _cwd(workdir):
+        ref_lib().refb_truns_to_disk(*_one(trun), int(dim), idx, int(system))
+
+
+def ref_truns_from_disk(workdir, idx, system=True):
+    with _lock, _cwd(workdir):
+        ref_lib().refb_truns_from_disk(idx, int(system))
+        t = _take_ref()
+        return t, int(_result_dims(1)[0])
+
+
+def ref_basis_to_disk(workdir, basis, idx, system=True):
+    with _lock, _cwd(workdir):
+        ref_lib().refb_basis_to_disk(basis.length, *_bs(basis), idx, int(system))
+
+
+def ref_basis_from_disk(workdir, idx, system=True):
+    """basis_from_disk of the reference build: (length, sec, sub, idx)."""
+    ln, nsec, nsub = C.c_int(0), C.c_int(0), C.c_int(0)
+    with _lock, _cwd(workdir):
+        lib = ref_lib()
+        lib.refb_basis_from_disk(idx, int(system), C.byref(ln), C.byref(nsec), C.byref(nsub))
+        sec = np.zeros((max(1, nsec.value), 4), np.int32)
+        sub = np.zeros((max(1, nsub.value), 6), np.int32)
+        bidx = np.zeros(max(1, nsec.value), np.int32)
+        lib.refb_basis_copy(sec, sub, bidx)
+    return ln.value, sec[:nsec.value], sub[:nsub.value], bidx[:nsec.value]
+
+
+def ref_wave_to_disk(workdir, name, sys_len, env_len, wsec, x):
+    w = np.ascontiguousarray(wsec, np.int32).reshape(-1, 6)
+    with _lock, _cwd(workdir):
+        ref_lib().refb_wave_to_disk(sys_len, env_len, w.shape[0], w, np.ascontiguousarray(x, np.float64),
+                                    name.encode(), len(name))
+
+
+def ref_wave_from_disk(workdir, name):
+    """wave_from_disk of the reference build: (sys_len, env_len, wsec, x)."""
+    sl, el, nw, dim = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+    with _lock, _cwd(workdir):
+        lib = ref_lib()
+        lib.refb_wave_from_disk(name.encode(), len(name), C.byref(sl), C.byref(el), C.byref(nw), C.byref(dim))
+        wsec = np.zeros((max(1, nw.value), 6), np.int32)
+        y = np.zeros(max(1, dim.value), np.float64)
+        lib.refb_wave_copy(wsec, y)
+    return sl.value, el.value, wsec[:nw.value], y[:dim.value]
 
 
 def ref_quicksort(arr):

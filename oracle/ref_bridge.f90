@@ -17,6 +17,9 @@ module ref_bridge_util
   implicit none
   ! operators returned by the last refb_* call that produces operators
   type(Total_Block), allocatable, save :: res(:)
+  ! basis and wavefunction read by the last refb_basis_from_disk / refb_wave_from_disk
+  type(Total_Basis), save :: bres
+  type(Wavefunction), save :: wres
 contains
   subroutine set_result(n)
     integer, intent(in) :: n
@@ -678,3 +681,235 @@ integer(c_int) function refb_wave_trans(dir, sys_nsec, sys_bsec, sys_bsub, env_n
   deallocate(sbs%sub, ebs%sub, ow%sub, nw%sub, zero)
   refb_wave_trans = 0
 end function refb_wave_trans
+
+! ---------------------------------------------------------------------------
+! Checkpoint files (block.f90:2760-3097), written into and read from the
+! current directory under the names of Get_model_name (dmrg_sub.f90:294).
+! flag /= 0 selects the system names, 0 the environment names.
+! ---------------------------------------------------------------------------
+
+! The `dim` field of each operator held from the last producing call.
+integer(c_int) function refb_result_dims(dims) bind(C, name="refb_result_dims")
+  use iso_c_binding
+  use ref_bridge_util
+  implicit none
+  integer(c_int), intent(inout) :: dims(*)
+  integer :: o
+  do o = 1, size(res)
+    dims(o) = res(o)%dim
+  end do
+  refb_result_dims = 0
+end function refb_result_dims
+
+! model_to_disk of a flat model of length len with the given dim fields;
+! nops must be 1 + 4 nleg11(len).
+integer(c_int) function refb_model_to_disk(len, nops, hdr, sub, off, data, dims, idx, flag) &
+    bind(C, name="refb_model_to_disk")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: len, nops, idx, flag
+  integer(c_int), intent(in) :: hdr(4, *), sub(5, *), dims(*)
+  integer(c_int64_t), intent(in) :: off(*)
+  real(c_double), intent(in) :: data(*)
+  type(Total_Model) :: m
+  integer :: ii
+  if (1 + 4 * nleg11(len) /= nops) then
+    refb_model_to_disk = -1
+    return
+  end if
+  call Get_model_name(sysname, envname)
+  call put_model(m, len, nops, hdr, sub, off, data)
+  m%ham%dim = dims(1)
+  do ii = 1, nleg11(len)
+    m%sub(ii)%elec_up%dim = dims(2 + 4 * (ii - 1))
+    m%sub(ii)%elec_down%dim = dims(3 + 4 * (ii - 1))
+    m%sub(ii)%spin_sd%dim = dims(4 + 4 * (ii - 1))
+    m%sub(ii)%num_sn%dim = dims(5 + 4 * (ii - 1))
+  end do
+  call model_to_disk(m, 1001, idx, flag /= 0)
+  call free_model(m)
+  refb_model_to_disk = 0
+end function refb_model_to_disk
+
+! model_from_disk; the model is held as the result operators.
+integer(c_int) function refb_model_from_disk(idx, flag) bind(C, name="refb_model_from_disk")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: idx, flag
+  type(Total_Model) :: m
+  integer :: n1, ii
+  call Get_model_name(sysname, envname)
+  call model_from_disk(m, 1001, idx, flag /= 0)
+  n1 = nleg11(m%len)
+  call set_result(1 + 4 * n1)
+  call block_transfer(m%ham, res(1))
+  do ii = 1, n1
+    call block_transfer(m%sub(ii)%elec_up, res(2 + 4 * (ii - 1)))
+    call block_transfer(m%sub(ii)%elec_down, res(3 + 4 * (ii - 1)))
+    call block_transfer(m%sub(ii)%spin_sd, res(4 + 4 * (ii - 1)))
+    call block_transfer(m%sub(ii)%num_sn, res(5 + 4 * (ii - 1)))
+  end do
+  call free_model(m)
+  refb_model_from_disk = m%len
+end function refb_model_from_disk
+
+! truns_to_disk of one flat block with the given dim field.
+integer(c_int) function refb_truns_to_disk(hdr, sub, off, data, dim, idx, flag) bind(C, name="refb_truns_to_disk")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), intent(in) :: hdr(4), sub(5, *)
+  integer(c_int64_t), intent(in) :: off(*)
+  real(c_double), intent(in) :: data(*)
+  integer(c_int), value :: dim, idx, flag
+  type(Total_Block) :: b
+  call Get_model_name(sysname, envname)
+  call put_block(b, hdr(1), hdr, sub, off, data)
+  b%dim = dim
+  call truns_to_disk(b, 1001, idx, flag /= 0)
+  call free_block(b)
+  refb_truns_to_disk = 0
+end function refb_truns_to_disk
+
+! truns_from_disk; the block is held as the single result operator.
+integer(c_int) function refb_truns_from_disk(idx, flag) bind(C, name="refb_truns_from_disk")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: idx, flag
+  call Get_model_name(sysname, envname)
+  call set_result(1)
+  call truns_from_disk(res(1), 1001, idx, flag /= 0)
+  refb_truns_from_disk = 0
+end function refb_truns_from_disk
+
+! basis_to_disk of a flat basis.
+integer(c_int) function refb_basis_to_disk(len, nsec, bsec, bsub, idx, flag) bind(C, name="refb_basis_to_disk")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: len, nsec, idx, flag
+  integer(c_int), intent(in) :: bsec(4, *), bsub(6, *)
+  type(Total_Basis) :: bs
+  call Get_model_name(sysname, envname)
+  call put_basis(bs, len, nsec, bsec, bsub)
+  call basis_to_disk(bs, 1001, idx, flag /= 0)
+  call deallocate_basis(bs)
+  refb_basis_to_disk = 0
+end function refb_basis_to_disk
+
+! basis_from_disk; returns the length and the table sizes, the tables follow
+! from refb_basis_copy.
+integer(c_int) function refb_basis_from_disk(idx, flag, len, nsec, nsub) bind(C, name="refb_basis_from_disk")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: idx, flag
+  integer(c_int), intent(out) :: len, nsec, nsub
+  integer :: i
+  call Get_model_name(sysname, envname)
+  if (allocated(bres%sub)) call deallocate_basis(bres)
+  call basis_from_disk(bres, 1001, idx, flag /= 0)
+  len = bres%len
+  nsec = bres%num
+  nsub = 0
+  do i = 1, bres%num
+    nsub = nsub + bres%sub(i)%num
+  end do
+  refb_basis_from_disk = 0
+end function refb_basis_from_disk
+
+integer(c_int) function refb_basis_copy(bsec, bsub, bidx) bind(C, name="refb_basis_copy")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), intent(inout) :: bsec(4, *), bsub(6, *), bidx(*)
+  integer :: i, j, q
+  q = 0
+  do i = 1, bres%num
+    bsec(1, i) = bres%sub(i)%new_num_up; bsec(2, i) = bres%sub(i)%new_num_down
+    bsec(3, i) = bres%sub(i)%dim; bsec(4, i) = bres%sub(i)%num
+    bidx(i) = bres%sub(i)%idx
+    do j = 1, bres%sub(i)%num
+      q = q + 1
+      bsub(1, q) = bres%sub(i)%sub(j)%bl_num_up; bsub(2, q) = bres%sub(i)%sub(j)%bl_num_down
+      bsub(3, q) = bres%sub(i)%sub(j)%st_num_up; bsub(4, q) = bres%sub(i)%sub(j)%st_num_down
+      bsub(5, q) = bres%sub(i)%sub(j)%spos; bsub(6, q) = bres%sub(i)%sub(j)%sdim
+    end do
+  end do
+  refb_basis_copy = 0
+end function refb_basis_copy
+
+! wave_to_disk of a flat wavefunction into the file `name`.
+integer(c_int) function refb_wave_to_disk(sys_len, env_len, nw, wsec, x, name, name_len) &
+    bind(C, name="refb_wave_to_disk")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: sys_len, env_len, nw, name_len
+  integer(c_int), intent(in) :: wsec(6, *)
+  real(c_double), intent(in) :: x(*)
+  character(kind=c_char), intent(in) :: name(*)
+  character(len=name_len) :: fname
+  type(Wavefunction) :: w
+  integer :: i
+  do i = 1, name_len
+    fname(i:i) = name(i)
+  end do
+  call put_wave(w, sys_len, env_len, nw, wsec, x)
+  call wave_to_disk(w, 1001, fname, name_len)
+  call deallocate_wave(w)
+  refb_wave_to_disk = 0
+end function refb_wave_to_disk
+
+! wave_from_disk of the file `name`; returns sizes, the data follow from
+! refb_wave_copy.
+integer(c_int) function refb_wave_from_disk(name, name_len, sys_len, env_len, nw, dim) &
+    bind(C, name="refb_wave_from_disk")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: name_len
+  character(kind=c_char), intent(in) :: name(*)
+  integer(c_int), intent(out) :: sys_len, env_len, nw, dim
+  character(len=name_len) :: fname
+  integer :: i
+  do i = 1, name_len
+    fname(i:i) = name(i)
+  end do
+  if (allocated(wres%sub)) call deallocate_wave(wres)
+  call wave_from_disk(wres, 1001, fname, name_len)
+  sys_len = wres%sys_len
+  env_len = wres%env_len
+  nw = wres%num
+  dim = wres%dim
+  refb_wave_from_disk = 0
+end function refb_wave_from_disk
+
+integer(c_int) function refb_wave_copy(wsec, y) bind(C, name="refb_wave_copy")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), intent(inout) :: wsec(6, *)
+  real(c_double), intent(inout) :: y(*)
+  integer :: i
+  do i = 1, wres%num
+    wsec(1, i) = wres%sub(i)%sys_num_up; wsec(2, i) = wres%sub(i)%sys_num_down
+    wsec(3, i) = wres%sub(i)%env_num_up; wsec(4, i) = wres%sub(i)%env_num_down
+    wsec(5, i) = wres%sub(i)%sys_dim; wsec(6, i) = wres%sub(i)%env_dim
+  end do
+  call get_wave(wres, y)
+  refb_wave_copy = 0
+end function refb_wave_copy

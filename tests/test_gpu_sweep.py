@@ -66,3 +66,42 @@ def test_mixed_precision_sweep():
     case = next(c for c in ED if c["ny"] == 2 and c["nx"] == 6 and c["N"] == 10 and c["tp"] == 0.0 and c["mu"] == 0.0)
     e, _ = _run(case, kept_max=1000, precision="bf16x3")
     assert abs(e - case["E0"]) < 1e-7
+
+
+def test_checkpoint_resume(tmp_path):
+    """save_checkpoint writes the run as the reference's checkpoint files
+    (grand_sql_tj_amd.checkpoint, pinned byte for byte against the
+    reference's writers on the CPU); a new run that loads them continues the
+    sweep exactly as the run that wrote them."""
+    import numpy as np
+    from grand_sql_tj_amd import checkpoint as ck
+    from grand_sql_tj_amd import sweep
+    case = next(c for c in ED if c["ny"] == 2 and c["nx"] == 8)
+    m = _model(case)
+    kw = dict(total_up=case["N"], kept_max=48, trun_err=1e-12, tol=1e-12)
+    a = sweep.DMRG(m, **kw)
+    N = a.nsite
+    a.warm_up()
+    a.left_to_right(N // 2 - 1, N - 2)
+    a.right_to_left(N - 2, 1)
+    a.save_checkpoint(tmp_path)
+    for side, blocks in ((0, a.sys), (1, a.env)):
+        for L, blk in blocks.items():
+            ops, dims = ck.load_model(tmp_path / ck.file_name(side, "blocks", L))
+            assert ops.length == L and np.array_equal(ops.sub, blk.ops.sub) and np.array_equal(ops.off, blk.ops.off)
+            assert np.array_equal(ops.data, blk.data[:ops.data.size].cpu().numpy())
+            assert np.array_equal(dims, ck.default_dims(ops))
+    assert any(t.ops.sub[:, 4].sum() < t.ops.sub[:, 3].sum() for t in a.env_trun.values())   # truncated
+    b = sweep.DMRG(m, **kw)
+    b.load_checkpoint(tmp_path)
+    assert sorted(b.sys) == sorted(a.sys) and sorted(b.env) == sorted(a.env)
+    assert sorted(b.env_trun) == sorted(a.env_trun) and sorted(b.env_basis) == sorted(a.env_basis)
+    assert b.bond == a.bond and np.array_equal(b.wsec, a.wsec)
+    n0 = len(a.log)
+    a.left_to_right(1, N - 2)
+    b.left_to_right(1, N - 2)
+    ea = [r.energy for r in a.log[n0:]]
+    eb = [r.energy for r in b.log]
+    assert len(ea) == len(eb) == N - 3
+    assert max(abs(x - y) for x, y in zip(ea, eb)) <= 1e-12 * abs(case["E0"])
+    assert min(eb) >= case["E0"] - 1e-9
