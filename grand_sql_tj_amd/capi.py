@@ -80,7 +80,7 @@ EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_t
            "tjsq_hostplan_stats", "tjsq_hostplan_destroy", "tjsq_plan_profile", "tjsq_plan_profile_read",
            "tjsq_heff_apply_host", "tjsq_lanczos_solve_host",
            "tjsq_block_update_create", "tjsq_trun_project_create", "tjsq_density_create", "tjsq_wave_transform_create",
-           "tjsq_program_sizes",
+           "tjsq_dia_transport_create", "tjsq_measure_create", "tjsq_program_sizes",
            "tjsq_program_layout", "tjsq_program_tables", "tjsq_program_stats", "tjsq_program_run",
            "tjsq_program_run_host", "tjsq_truncation_operator_host",
            "tjsq_program_profile", "tjsq_program_profile_read", "tjsq_program_destroy", "tjsq_truncation_operator",
@@ -125,6 +125,8 @@ def lib():
             "tjsq_lanczos_solve_host": [P, P, P, P],
             "tjsq_block_update_create": [P, I, I, I, P, P, P, P],
             "tjsq_trun_project_create": [P, P, P, P],
+            "tjsq_dia_transport_create": [P, P, P, P, P],
+            "tjsq_measure_create": [I, P, P, P, I, P, P],
             "tjsq_density_create": [I, I, I, P, I, P],
             "tjsq_wave_transform_create": [I, P, P, P, P, I, P, I, P, P],
             "tjsq_program_sizes": [P, P],
@@ -462,6 +464,39 @@ class Program:
         h = C.c_void_p()
         check(lib().tjsq_trun_project_create(C.byref(os_), nd.ctypes.data, C.byref(ts), C.byref(h)),
               "tjsq_trun_project_create")
+        return cls(h)
+
+    @classmethod
+    def dia_transport(cls, ops, site, basis, trun):
+        """One step of the transport of diagonal operators along a block:
+        ops (diagonal, block length L, or None) embedded into basis (length
+        L + 1), the site operator (host data, or None) embedded after them,
+        each projected with trun.  Run with ops = trun data, x = ops data."""
+        keep = _Keep()
+        os_ = opset_struct(ops, keep) if ops is not None and ops.nops > 0 else None
+        ss = opset_struct(site, keep) if site is not None else None
+        bs, ts = basis_struct(basis, keep), opset_struct(trun, keep)
+        h = C.c_void_p()
+        check(lib().tjsq_dia_transport_create(C.byref(os_) if os_ is not None else None,
+                                              C.byref(ss) if ss is not None else None, C.byref(bs), C.byref(ts),
+                                              C.byref(h)), "tjsq_dia_transport_create")
+        return cls(h)
+
+    @classmethod
+    def measure(cls, side, ops, site, basis, wsec):
+        """<psi|O|psi> of diagonal block operators (or None) and the site
+        operator of one side: the output is nops + 1 wavefunction-sized
+        vectors y_j with <O_j> = psi . y_j.  Run with ops = the operators'
+        data, x = psi, y zeroed."""
+        keep = _Keep()
+        os_ = opset_struct(ops, keep) if ops is not None and ops.nops > 0 else None
+        ss = opset_struct(site, keep) if site is not None else None
+        bs = basis_struct(basis, keep)
+        w = np.ascontiguousarray(wsec, np.int32).reshape(-1, 6)
+        h = C.c_void_p()
+        check(lib().tjsq_measure_create(side, C.byref(os_) if os_ is not None else None,
+                                        C.byref(ss) if ss is not None else None, C.byref(bs), w.shape[0],
+                                        w.ctypes.data, C.byref(h)), "tjsq_measure_create")
         return cls(h)
 
     @classmethod

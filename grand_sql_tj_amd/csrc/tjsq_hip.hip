@@ -1803,6 +1803,28 @@ int tjsq_trun_project_create(const tjsq_opset* ops, const int* ndia, const tjsq_
   return make_program(pg, 0, 1, prog);
 }
 
+int tjsq_dia_transport_create(const tjsq_opset* ops, const tjsq_opset* site, const tjsq_basis* basis,
+                              const tjsq_opset* trun, tjsq_program** prog) {
+  if (!basis || !trun || !prog) return fail(TJSQ_ERR_ARG, "tjsq_dia_transport_create: null argument");
+  *prog = nullptr;
+  tjsq::Program pg;
+  std::string msg;
+  int rc = tjsq::build_dia_transport_program(ops, site, *basis, *trun, tjsq::Shape(), pg, msg);
+  if (rc) return fail(rc, "tjsq_dia_transport_create: " + msg);
+  return make_program(pg, 0, 1, prog);
+}
+
+int tjsq_measure_create(int side, const tjsq_opset* ops, const tjsq_opset* site, const tjsq_basis* basis, int nw,
+                        const int* wsec, tjsq_program** prog) {
+  if (!basis || !wsec || !prog) return fail(TJSQ_ERR_ARG, "tjsq_measure_create: null argument");
+  *prog = nullptr;
+  tjsq::Program pg;
+  std::string msg;
+  int rc = tjsq::build_measure_program(side, ops, site, *basis, nw, wsec, tjsq::Shape(), pg, msg);
+  if (rc) return fail(rc, "tjsq_measure_create: " + msg);
+  return make_program(pg, 0, 1, prog);
+}
+
 int tjsq_density_create(int side, int len, int nw, const int* wsec, int levels, tjsq_program** prog) {
   if (!wsec || !prog || levels < 1) return fail(TJSQ_ERR_ARG, "tjsq_density_create: bad argument");
   *prog = nullptr;

@@ -576,6 +576,160 @@ int build_projection_program(const tjsq_opset& ops_in, const int* ndia, const tj
   return finish(pg, shape, cx, msg);
 }
 
+namespace {
+
+// a site operator given as a one-operator set of 1x1 sub-blocks with host
+// data (the layout of Get_site_operator's Total_Block, dmrg_sub.f90:370-473)
+int read_site(const tjsq_opset& in, Site& st, std::string& msg) {
+  std::vector<Op> ops;
+  int rc = read_ops(in, ops, msg);
+  if (rc) return rc;
+  if (in.nops != 1 || !in.data || in.data_on_device) {
+    msg = "the site operator must be one operator with host data";
+    return TJSQ_ERR_ARG;
+  }
+  st = Site();
+  st.up_dif = ops[0].up_dif;
+  st.down_dif = ops[0].down_dif;
+  for (const OpSub& q : ops[0].sub) {
+    if (q.row_dim != 1 || q.sdim != 1) { msg = "site operator sub-blocks must be 1 x 1"; return TJSQ_ERR_ARG; }
+    st.sub.push_back(q);
+    st.val.push_back(in.data[q.off]);
+  }
+  return TJSQ_OK;
+}
+
+bool all_dia(const std::vector<Op>& ops) {
+  for (const Op& o : ops) {
+    if (o.up_dif != 0 || o.down_dif != 0) return false;
+    for (const OpSub& q : o.sub)
+      if (q.down_dif != 0 || q.row_dim != q.sdim) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+// One step of Get_sys_oper_dia / Get_env_oper_dia (get_sys.f90:764-891):
+// every diagonal operator of the block at length L embedded into the fused
+// basis (update_block_dia), the new site's operator embedded
+// (update_site_dia), each projected with T (update_trun_dia).
+int build_dia_transport_program(const tjsq_opset* ops_in, const tjsq_opset* site, const tjsq_basis& basis,
+                                const tjsq_opset& trun, const Shape& shape, Program& pg, std::string& msg) {
+  pg = Program();
+  std::vector<Op> ops, tops;
+  int rc;
+  if (ops_in && ops_in->nops > 0) {
+    if ((rc = read_ops(*ops_in, ops, msg))) return rc;
+    if (!all_dia(ops)) { msg = "transported operators must be diagonal (square, down_dif 0)"; return TJSQ_ERR_ARG; }
+  }
+  Site st;
+  if (site && (rc = read_site(*site, st, msg))) return rc;
+  if (site && (st.up_dif != 0 || st.down_dif != 0)) { msg = "the site operator must be diagonal"; return TJSQ_ERR_ARG; }
+  if (ops.empty() && !site) { msg = "nothing to transport"; return TJSQ_ERR_ARG; }
+  if ((rc = read_ops(trun, tops, msg))) return rc;
+  if (trun.nops != 1) { msg = "the truncation operator set must hold one operator"; return TJSQ_ERR_ARG; }
+  BasisView bs;
+  read_basis(basis, bs);
+  Ctx cx;
+  Projector pj{pg, tops[0], cx};
+  for (const Op& o : ops) pj.project(upd_block_dia(o, bs, basis.length, cx), true);
+  if (site) pj.project(upd_site_dia(st, bs, basis.length), true);
+  pg.x_elems = ops.empty() ? 0 : max_extent(*ops_in);
+  pg.ops_elems = max_extent(trun);
+  return finish(pg, shape, cx, msg);
+}
+
+// measure_sys_block / measure_sys_site (side 0, measure.f90:1241-1355) and
+// measure_env_block / measure_env_site (side 1, :1359-1470) as products
+// laid out like the wavefunction: output copy j (of nops + 1, each D long)
+// holds Y_j with <psi|O_j|psi> = psi . Y_j, where per wavefunction sector
+// and fused sub-sector x with block part bl (site part st)
+//   side 0: Y_j[rows of x]    = O_j[bl] psi[rows of x] / sqrt(1 + 2S_bl)
+//   side 1: Y_j[columns of x] = psi[columns of x] O_j[bl] / sqrt(1 + 2S_bl)
+// and the last copy Y_site = site[st] psi[x] / sqrt(1 + 2S_st).  Entries no
+// term reaches are not written (the caller clears y).
+int build_measure_program(int side, const tjsq_opset* ops_in, const tjsq_opset* site, const tjsq_basis& basis,
+                          int nw, const int* wsec, const Shape& shape, Program& pg, std::string& msg) {
+  pg = Program();
+  if ((side != 0 && side != 1) || nw < 1 || !wsec) { msg = "bad measurement arguments"; return TJSQ_ERR_ARG; }
+  std::vector<Op> ops;
+  int rc;
+  if (ops_in && ops_in->nops > 0) {
+    if ((rc = read_ops(*ops_in, ops, msg))) return rc;
+    if (!all_dia(ops)) { msg = "measured block operators must be diagonal (square, down_dif 0)"; return TJSQ_ERR_ARG; }
+  }
+  Site st;
+  if (site && (rc = read_site(*site, st, msg))) return rc;
+  BasisView bs;
+  read_basis(basis, bs);
+  int64_t D = 0;
+  std::vector<int64_t> woff(nw);
+  for (int i = 0; i < nw; ++i) {
+    const int* w = wsec + 6 * i;
+    if (w[4] < 0 || w[5] < 0) { msg = "negative sector dimension"; return TJSQ_ERR_ARG; }
+    woff[i] = D;
+    D += (int64_t)w[4] * w[5];
+  }
+  const int ncopy = (int)ops.size() + 1;
+  auto out_block = [&](int64_t off, int ld, int M, int N) {
+    pg.hp.out_blocks.push_back({off, std::max(1, ld), M, N, (int)pg.hp.out_blocks.size()});
+    return (int)pg.hp.out_blocks.size() - 1;
+  };
+  for (int i = 0; i < nw; ++i) {
+    const int* w = wsec + 6 * i;
+    const int ds = w[4], de = w[5];
+    if (ds == 0 || de == 0) continue;
+    const int k = find_sec(bs, side == 0 ? w[0] : w[2], side == 0 ? w[1] : w[3]);
+    if (k < 0) continue;
+    if (bs.sec[k].dim != (side == 0 ? ds : de)) { msg = "fused basis sector disagrees with the wavefunction"; return TJSQ_ERR_ARG; }
+    for (const MapSub& m : bs.sec[k].sub) {
+      if (m.sdim == 0) continue;
+      // the rows (side 0) or columns (side 1) of sub-sector x inside sector i
+      const int64_t xo = woff[i] + (side == 0 ? (int64_t)m.spos : (int64_t)m.spos * ds);
+      const int M = side == 0 ? m.sdim : ds, N = side == 0 ? de : m.sdim;
+      for (size_t j = 0; j < ops.size(); ++j) {
+        for (const OpSub& q : ops[j].sub) {
+          if (q.num_up != m.bl_up || q.num_down != m.bl_down) continue;
+          if (q.row_dim != m.sdim) { msg = "operator sub-block disagrees with the fused basis"; return TJSQ_ERR_ARG; }
+          Contrib c{};
+          c.kind = side == 0 ? C_LEFT : C_RIGHT;
+          c.out_block = out_block((int64_t)j * D + xo, ds, M, N);
+          c.x_off = xo; c.x_ld = ds;
+          c.M = M; c.N = N;
+          c.c = 1.0 / std::sqrt(1.0 + m.bl_down);
+          if (side == 0) { c.a_off = q.off; c.a_ld = q.row_dim; c.a_t = 0; c.K1 = m.sdim; }
+          else { c.b_off = q.off; c.b_ld = q.row_dim; c.b_t = 0; c.K2 = m.sdim; }
+          pg.hp.contrib.push_back(c);
+          break;
+        }
+      }
+      if (site) {
+        for (size_t y = 0; y < st.sub.size(); ++y) {
+          if (st.sub[y].num_up != m.st_up || st.sub[y].num_down != m.st_down) continue;
+          Contrib c{};
+          c.kind = C_AXPY;
+          c.out_block = out_block((int64_t)(ncopy - 1) * D + xo, ds, M, N);
+          c.x_off = xo; c.x_ld = ds;
+          c.M = M; c.N = N;
+          c.c = st.val[y] / std::sqrt(1.0 + m.st_down);
+          if (c.c != 0.0) pg.hp.contrib.push_back(c);
+          else pg.hp.out_blocks.pop_back();
+          break;
+        }
+      }
+    }
+  }
+  const int len = basis.length;
+  pg.out.nops = ncopy;
+  for (int j = 0; j < ncopy; ++j) pg.out.hdr.insert(pg.out.hdr.end(), {len, 0, 0, 0});
+  pg.out.ndata = (int64_t)ncopy * D;
+  pg.x_elems = D;
+  pg.ops_elems = ops.empty() ? 0 : max_extent(*ops_in);
+  Ctx cx;
+  return finish(pg, shape, cx, msg);
+}
+
 // Get_density_matrix_sys (den_mat.f90:4-38): rho = psi psi^T per sector;
 // Get_density_matrix_env (:44-78): rho = psi^T psi.
 int build_density_program(int side, int len, int nw, const int* wsec, const Shape& shape, Program& pg,

@@ -74,6 +74,25 @@ int build_update_program(const tjsq_lattice& lat, int side, int sys_len, int env
 int build_projection_program(const tjsq_opset& ops, const int* ndia, const tjsq_opset& trun, const Shape& shape,
                              Program& pg, std::string& msg);
 
+// One step of the transport of diagonal operators to a longer block
+// (Get_sys_oper_dia / Get_env_oper_dia, code/tjsq/get_sys.f90:764-891):
+// each operator of ops (diagonal, at block length L; may be NULL) embedded
+// into `basis` (update_block_dia), then the site operator `site` (one
+// operator of 1x1 sub-blocks with host data; may be NULL) embedded
+// (update_site_dia), each projected with `trun` (update_trun_dia).  Run
+// with ops = trun data, x = the data of ops.
+int build_dia_transport_program(const tjsq_opset* ops, const tjsq_opset* site, const tjsq_basis& basis,
+                                const tjsq_opset& trun, const Shape& shape, Program& pg, std::string& msg);
+
+// Expectation values of diagonal block operators and of a site operator in
+// a wavefunction (measure_sys_block / _site, measure_env_block / _site,
+// code/tjsq/measure.f90:1241-1470): the output is nops + 1 vectors of the
+// wavefunction's length, Y_j for every operator of ops and Y_site last,
+// with <O_j> = psi . Y_j.  basis is the fused basis of the measured side.
+// Run with ops = the operators' data, x = psi, y cleared beforehand.
+int build_measure_program(int side, const tjsq_opset* ops, const tjsq_opset* site, const tjsq_basis& basis,
+                          int nw, const int* wsec, const Shape& shape, Program& pg, std::string& msg);
+
 // Get_density_matrix_sys (side 0) / _env (side 1) of a wavefunction with
 // sectors wsec; `len` is the block length recorded in the output operator.
 int build_density_program(int side, int len, int nw, const int* wsec, const Shape& shape, Program& pg,

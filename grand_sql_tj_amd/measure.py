@@ -1,0 +1,80 @@
+"""Site expectation values on the device: <n_i> over the lattice, as the
+reference's measure_operator_dia computes it (code/tjsq/measure.f90:3-57,
+called by Measurement, code/tjsq/main.f90:191-332).
+
+Every site operator is carried from its site to the block next to the bond
+(Get_sys_oper_dia / Get_env_oper_dia, code/tjsq/get_sys.f90:764-891): at
+each block length it is embedded into the fused basis (update_block_dia),
+the new site's operator is embedded beside it (update_site_dia), and all are
+projected with that length's truncation operator (update_trun_dia).  The
+reference carries each site's operator separately, O(N^2) transports per
+side; here all operators of a side travel together, one device program per
+block length (tjsq_dia_transport_create).  The contraction with the
+wavefunction (measure_sys_block / _site, measure_env_block / _site,
+code/tjsq/measure.f90:1241-1470) is one more program (tjsq_measure_create)
+whose outputs y_j give <O_j> = psi . y_j.
+
+The inputs are what a run holds, or what its checkpoint files hold: the
+fused bases by length (sysbs<L> / envbs<L>), the truncation operators by
+length (systruns<L> / envtruns<L>) with their data on the device, and the
+wavefunction with its sector list (eigvec1).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import capi
+from .blocks import OpSet
+
+
+def num_elec(up_bias: int) -> OpSet:
+    """num_elec of Get_site_operator_su2_U0 (dmrg_sub.f90:370-473): the
+    reduced matrix element sqrt(2) on the occupied site, 0 on the hole, as a
+    one-operator set of 1x1 sub-blocks."""
+    return OpSet(1, np.array([[1, 0, 0, 2]], np.int32),
+                 np.array([[0, 0, 0, 1, 1], [up_bias, 1, 0, 1, 1]], np.int32),
+                 np.array([0, 1], np.int64), np.array([0.0, math.sqrt(2.0)]))
+
+
+def side_values(side, site: OpSet, bases, truns, length, wsec, psi):
+    """<O> of the site operator on the `length` sites of one side of the
+    bond: the block's sites 1..length-1 (counted from the block's outer end),
+    then the site next to the bond.  bases[L] (L = 2..length) are the fused
+    bases, truns[L] (L = 2..length-1) objects with `.ops` (layout) and
+    `.data` (device tensor) of the truncation that made block L."""
+    import torch
+    if length < 2:
+        raise ValueError("a side holds at least the block site and the bond site")
+    dev = psi.device
+    cur = site
+    cur_data = torch.from_numpy(np.ascontiguousarray(site.data, np.float64)).to(dev)
+    for L in range(2, length):
+        t = truns[L]
+        pg = capi.Program.dia_transport(cur, site, bases[L], t.ops)
+        hdr, sub, off = pg.layout()
+        n = max(1, pg.sizes["ndata"])
+        out = torch.empty(n, dtype=torch.float64, device=dev)
+        pg.run(t.data, cur_data, out)
+        cur, cur_data = OpSet(L, hdr, sub, off, np.zeros(n)), out
+    mp = capi.Program.measure(side, cur, site, bases[length], wsec)
+    D = psi.numel()
+    y = torch.zeros(max(1, mp.sizes["ndata"]), dtype=torch.float64, device=dev)
+    mp.run(cur_data, psi, y)
+    return (y[:mp.sizes["ndata"]].view(-1, D) @ psi).cpu().numpy()
+
+
+def density_profile(up_bias, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns, wsec, psi):
+    """measure_operator_dia (measure.f90:3-57) of num_elec: <n_i> for the
+    sys_len + env_len lattice sites of a wavefunction whose system part is
+    sys_len sites long (block + site) and environment part env_len.
+    Environment block site i is lattice site N - i + 1."""
+    site = num_elec(up_bias)
+    n = sys_len + env_len
+    ave = np.zeros(n)
+    ave[:sys_len] = side_values(0, site, sys_bases, sys_truns, sys_len, wsec, psi)
+    ev = side_values(1, site, env_bases, env_truns, env_len, wsec, psi)
+    for i in range(1, env_len + 1):
+        ave[n - i] = ev[i - 1]
+    return ave

@@ -339,6 +339,17 @@ class DMRG:
             self.psi, self.wsec = self._device(w.x), w.wsec
             self.bond = (w.sys_len - 1, w.env_len - 1)
 
+    # ---- measurements (measure_operator_dia, measure.f90:3-57) -----------
+    def measure_density(self):
+        """<n_i> of every lattice site in the ground state of the last bond,
+        carried through the run's bases and truncation operators."""
+        from . import measure
+        if self.psi is None:
+            raise RuntimeError("no ground state yet")
+        sl, el = self.bond[0] + 1, self.bond[1] + 1
+        return measure.density_profile(self.model.up_bias, sl, el, self.sys_basis, self.env_basis, self.sys_trun,
+                                       self.env_trun, self.wsec, self.psi)
+
     def run(self, sweeps=2):
         """Warm-up, then `sweeps` pairs of half sweeps (Get_Ground_State with
         trun_idx = 1).  Returns the last ground-state energy."""

@@ -315,6 +315,29 @@ int tjsq_block_update_create(const tjsq_lattice* lat, int side, int sys_len, int
  * 1311-1503); ndia may be NULL (all diagonal). */
 int tjsq_trun_project_create(const tjsq_opset* ops, const int* ndia, const tjsq_opset* trun, tjsq_program** prog);
 
+/* One step of the transport of diagonal operators along a block
+ * (Get_sys_oper_dia / Get_env_oper_dia, code/tjsq/get_sys.f90:764-891, the
+ * operator transport of the measurements, code/tjsq/measure.f90:3-57): every
+ * operator of `ops` (diagonal, at block length L; may be NULL) embedded into
+ * the fused basis `basis` of length L + 1 (update_block_dia), then the new
+ * site's operator `site` (one operator of 1x1 sub-blocks, host data; may be
+ * NULL) embedded (update_site_dia), each projected with `trun`
+ * (update_trun_dia).  Run with ops = trun data, x = the data of `ops`. */
+int tjsq_dia_transport_create(const tjsq_opset* ops, const tjsq_opset* site, const tjsq_basis* basis,
+                              const tjsq_opset* trun, tjsq_program** prog);
+
+/* Expectation values in a wavefunction of diagonal block operators and of
+ * the site operator next to the block (measure_sys_block / measure_sys_site
+ * for side 0, measure_env_block / measure_env_site for side 1,
+ * code/tjsq/measure.f90:1241-1470).  `basis` is the fused basis of the
+ * measured side, wsec the wavefunction's sector list.  The output is
+ * nops + 1 vectors of the wavefunction's length, one per operator of `ops`
+ * and the site operator's last: <O_j> = psi . y_j.  Run with ops = the
+ * operators' data, x = psi, and y cleared first (entries no term reaches
+ * are not written). */
+int tjsq_measure_create(int side, const tjsq_opset* ops, const tjsq_opset* site, const tjsq_basis* basis, int nw,
+                        const int* wsec, tjsq_program** prog);
+
 /* Get_density_matrix_sys (side 0) / _env (side 1), levels >= 1 states summed
  * and normalised first (Get_density_matrix_*_new, code/tjsq/den_mat.f90:
  * 4-134).  The output operator has one sub-block per wavefunction sector. */

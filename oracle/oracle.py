@@ -186,6 +186,12 @@ def ref_lib():
             lib.refb_wave_from_disk.argtypes = [C.c_char_p, C.c_int] + [C.POINTER(C.c_int)] * 4
             lib.refb_wave_copy.restype = C.c_int
             lib.refb_wave_copy.argtypes = [I32P, F64P]
+            lib.refb_set_bsm.restype = C.c_int
+            lib.refb_set_bsm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, I32P, I32P]
+            lib.refb_set_truns.restype = C.c_int
+            lib.refb_set_truns.argtypes = [C.c_int, C.c_int, I32P, I32P, I64P, F64P]
+            lib.refb_measure_density.restype = C.c_int
+            lib.refb_measure_density.argtypes = [C.c_int, C.c_int, C.c_int, I32P, F64P, C.c_int, F64P]
             _libs["ref"] = lib
         return _libs["ref"]
 
@@ -673,6 +679,31 @@ def ref_wave_from_disk(workdir, name):
         y = np.zeros(max(1, dim.value), np.float64)
         lib.refb_wave_copy(wsec, y)
     return sl.value, el.value, wsec[:nw.value], y[:dim.value]
+
+
+def ref_measure_density(model, workdir, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns, wsec, x):
+    """measure_operator_dia of num_elec in the reference build (it appends
+    to num_elec.dat in workdir): <n_i> of the sys_len + env_len sites of the
+    wavefunction, in lattice order.  Bases keyed by length 2..sys_len
+    (env_len), truncation operators by 2..sys_len - 1 (env_len - 1)."""
+    w = np.ascontiguousarray(wsec, np.int32).reshape(-1, 6)
+    with _lock, _cwd(workdir):
+        lib = _ref_init(model)
+        for side, bases, truns in ((0, sys_bases, sys_truns), (1, env_bases, env_truns)):
+            for L, b in bases.items():
+                lib.refb_set_bsm(side, L, b.length, *_bs(b))
+            for L, t in truns.items():
+                lib.refb_set_truns(side, L, *_one(t))
+        ave = np.zeros(4096)
+        with _quiet():
+            nref = lib.refb_measure_density(sys_len, env_len, w.shape[0], w, np.ascontiguousarray(x, np.float64), 1,
+                                            ave)
+    n = sys_len + env_len
+    out = np.zeros(n)
+    out[:sys_len] = ave[:sys_len]
+    for i in range(1, env_len + 1):
+        out[n - i] = ave[nref - i]
+    return out
 
 
 def ref_quicksort(arr):

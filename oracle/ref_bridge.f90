@@ -913,3 +913,66 @@ integer(c_int) function refb_wave_copy(wsec, y) bind(C, name="refb_wave_copy")
   call get_wave(wres, y)
   refb_wave_copy = 0
 end function refb_wave_copy
+
+! ---------------------------------------------------------------------------
+! Measurements: measure_operator_dia (measure.f90:3-57) of the site operator
+! num_elec, over bases and truncation operators stored into the reference's
+! globals sys_bsm / env_bsm / systruns / envtruns (common.f90:159-160), as
+! Measurement reads them from disk (main.f90:220-238).
+! ---------------------------------------------------------------------------
+integer(c_int) function refb_set_bsm(side, idx, len, nsec, bsec, bsub) bind(C, name="refb_set_bsm")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: side, idx, len, nsec
+  integer(c_int), intent(in) :: bsec(4, *), bsub(6, *)
+  if (side == 0) then
+    if (allocated(sys_bsm(idx)%sub)) call deallocate_basis(sys_bsm(idx))
+    call put_basis(sys_bsm(idx), len, nsec, bsec, bsub)
+  else
+    if (allocated(env_bsm(idx)%sub)) call deallocate_basis(env_bsm(idx))
+    call put_basis(env_bsm(idx), len, nsec, bsec, bsub)
+  end if
+  refb_set_bsm = 0
+end function refb_set_bsm
+
+integer(c_int) function refb_set_truns(side, idx, hdr, sub, off, data) bind(C, name="refb_set_truns")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: side, idx
+  integer(c_int), intent(in) :: hdr(4), sub(5, *)
+  integer(c_int64_t), intent(in) :: off(*)
+  real(c_double), intent(in) :: data(*)
+  if (side == 0) then
+    call put_block(systruns(idx), hdr(1), hdr, sub, off, data)
+  else
+    call put_block(envtruns(idx), hdr(1), hdr, sub, off, data)
+  end if
+  refb_set_truns = 0
+end function refb_set_truns
+
+! <n_i> of every lattice site in the wavefunction (sys_len, env_len are the
+! wavefunction's lengths, block + site); ave(Num_site) receives ave_dia.
+integer(c_int) function refb_measure_density(sys_len, env_len, nw, wsec, x, trun_idx, ave) &
+    bind(C, name="refb_measure_density")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: sys_len, env_len, nw, trun_idx
+  integer(c_int), intent(in) :: wsec(6, *)
+  real(c_double), intent(in) :: x(*)
+  real(c_double), intent(inout) :: ave(*)
+  type(Wavefunction) :: w
+  real(8) :: ave_dia(Num_site)
+  character(len=30) :: fname
+  fname = "num_elec.dat"
+  call put_wave(w, sys_len, env_len, nw, wsec, x)
+  call measure_operator_dia(ave_dia, num_elec, w, trun_idx, fname)
+  ave(1:Num_site) = ave_dia
+  call deallocate_wave(w)
+  refb_measure_density = Num_site
+end function refb_measure_density

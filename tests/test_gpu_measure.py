@@ -1,0 +1,82 @@
+"""Density measurement on the device (grand_sql_tj_amd.measure: the
+transport programs tjsq_dia_transport_create and the contraction
+tjsq_measure_create) against the numpy oracle of measure_operator_dia
+(oracle/measure_oracle.py, pinned against the reference on the CPU) on
+synthetic chains with random truncations, and against exact
+diagonalisation through the device DMRG: untruncated, <n_i> of the DMRG
+ground state is the exact ground state's density, site by site."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import ed
+import measure_oracle as mo
+from test_measure_cpu import GEOMS, case
+from grand_sql_tj_amd.lattice import TJModel
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+with open(os.path.join(HERE, "golden", "ed_energies.json")) as f:
+    ED = json.load(f)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+class _Trun:
+    def __init__(self, t):
+        from grand_sql_tj_amd.blocks import OpSet
+        self.ops = OpSet(int(t.length), np.asarray(t.hdr), np.asarray(t.sub), np.asarray(t.off),
+                         np.zeros(max(1, np.asarray(t.data).size)))
+        self.data = torch.from_numpy(np.ascontiguousarray(t.data, np.float64)).cuda()
+
+
+@pytest.mark.parametrize("sys_len,env_len", GEOMS)
+def test_device_density_matches_oracle(sys_len, env_len):
+    from grand_sql_tj_amd import measure
+    sb, eb, st, et, wsec, x, N = case(sys_len, env_len, 3 + sys_len)
+    want = mo.density_profile(1, sys_len + env_len, sys_len, env_len, sb, eb, st, et, wsec, x)
+    psi = torch.from_numpy(x).cuda()
+    got = measure.density_profile(1, sys_len, env_len, sb, eb, {L: _Trun(t) for L, t in st.items()},
+                                  {L: _Trun(t) for L, t in et.items()}, wsec, psi)
+    assert np.abs(got - want).max() < 1e-13
+    assert abs(got.sum() - N) < 1e-12 * N
+
+
+def _ed_density(model, n_elec):
+    import scipy.sparse.linalg as spla
+    H, states = ed.hamiltonian(model, n_elec)
+    if H.shape[0] <= 2000:
+        w, v = np.linalg.eigh(H.toarray())
+        g = v[:, 0]
+    else:
+        w, v = spla.eigsh(H, k=1, which="SA", tol=1e-13)
+        g = v[:, 0]
+    n = model.nsite
+    up, dn = states >> n, states & ((1 << n) - 1)
+    occ = np.array([((up >> b) & 1) + ((dn >> b) & 1) for b in range(n)], dtype=np.float64)
+    return occ @ (g * g)
+
+
+@pytest.mark.parametrize("case_", [c for c in ED if c["nx"] * c["ny"] <= 12][:3],
+                         ids=lambda c: f"{c['ny']}x{c['nx']}N{c['N']}")
+def test_untruncated_dmrg_density_is_exact(case_):
+    from grand_sql_tj_amd import sweep
+    m = TJModel(ny=case_["ny"], nx=case_["nx"], t=case_["t"], tp=case_["tp"], J=case_["J"], mu=case_["mu"],
+                up_bias=1)
+    d = sweep.DMRG(m, total_up=case_["N"], kept_max=1000, trun_err=1e-12, tol=1e-12)
+    d.run(sweeps=1)
+    got = d.measure_density()
+    want = _ed_density(m, case_["N"])
+    assert abs(got.sum() - case_["N"]) < 1e-10
+    # the state converges to the Lanczos tolerance on the energy (1e-12), so to
+    # ~1e-6 in norm: densities agree to that order
+    assert np.abs(got - want).max() < 2e-5

@@ -1,0 +1,147 @@
+"""The density measurement's oracle (oracle/measure_oracle.py, a numpy
+restatement of measure_operator_dia, code/tjsq/measure.f90:3-57) against
+the reference's own measure_operator_dia (compiled into oracle/_ref) on
+synthetic chains: bases grown site by site with random orthonormal
+truncations, and a random normalised wavefunction on the final bond.
+
+The oracle is also pinned without the reference by a sum rule: the site
+densities add up to the particle number of the wavefunction's sectors,
+sum_i <n_i> = N, whatever the truncations (each one is an isometry inside
+a sector, so the block's number operator survives it).  A wrong embedding
+or contraction coefficient breaks the sum."""
+import numpy as np
+import pytest
+
+import measure_oracle as mo
+import oracle
+from helpers import model, opset, random_trun
+from grand_sql_tj_amd.blocks import get_basis, get_super_basis, site_sectors
+
+needs_ref = pytest.mark.skipif(not oracle.ref_available(), reason="reference build not present")
+
+
+def chain(up_bias, length, seed, frac=0.7, drop=0.1):
+    """Fused bases (lengths 2..length) and truncation operators (lengths
+    2..length-1) of a block grown from one site."""
+    site = site_sectors(up_bias)
+    blk = site.copy()
+    bases, truns = {}, {}
+    for L in range(2, length + 1):
+        b = get_basis(L - 1, blk, site)
+        bases[L] = b
+        if L < length:
+            t = random_trun(b, seed * 100 + L, frac=frac, drop=drop)
+            truns[L] = t
+            blk = np.array([(s[0], s[1], s[4]) for s in t.sub], np.int32)
+    return bases, truns
+
+
+def case(sys_len, env_len, seed, up_bias=1):
+    sb, st = chain(up_bias, sys_len, seed)
+    eb, et = chain(up_bias, env_len, seed + 7)
+    n = sys_len + env_len
+    N = int(round(0.875 * n))
+    N -= N % 2
+    wsec = get_super_basis(sb[sys_len], eb[env_len], N if up_bias else 0, 0)
+    assert wsec.shape[0] > 0
+    rng = np.random.default_rng(seed)
+    d = int((wsec[:, 4].astype(np.int64) * wsec[:, 5]).sum())
+    x = rng.standard_normal(d)
+    x /= np.linalg.norm(x)
+    return sb, eb, st, et, wsec, x, N
+
+
+GEOMS = [(3, 4), (5, 5), (6, 8), (9, 7)]
+
+
+@pytest.mark.parametrize("sys_len,env_len", GEOMS)
+def test_oracle_sum_rule(sys_len, env_len):
+    sb, eb, st, et, wsec, x, N = case(sys_len, env_len, 3 + sys_len)
+    ave = mo.density_profile(1, sys_len + env_len, sys_len, env_len, sb, eb, st, et, wsec, x)
+    assert abs(ave.sum() - N) < 1e-12 * N
+    assert (ave > -1e-12).all() and (ave < 1 + 1e-12).all()
+
+
+def identity_chain(up_bias, length):
+    site = site_sectors(up_bias)
+    blk = site.copy()
+    bases, truns = {}, {}
+    for L in range(2, length + 1):
+        b = get_basis(L - 1, blk, site)
+        bases[L] = b
+        if L < length:
+            sub, off, chunks, nd = [], [], [], 0
+            for u, d, n, _ in b.sec:
+                sub.append((u, d, 0, n, n))
+                off.append(nd)
+                chunks.append(np.eye(int(n)).reshape(-1))
+                nd += int(n) * int(n)
+            truns[L] = opset(L, [[L, 0, 0, len(sub)]], sub, off, np.concatenate(chunks))
+            blk = np.array([(u, d, n) for u, d, n, _ in b.sec], np.int32)
+    return bases, truns
+
+
+@pytest.mark.parametrize("sys_len,env_len", [(3, 3), (4, 5)])
+def test_oracle_basis_states_have_definite_occupations(sys_len, env_len):
+    """Untruncated, every fused-basis state has a definite occupation of
+    each site (each enlargement fuses a block state with a site state of
+    definite occupation), so a wavefunction on one basis state measures
+    0 or 1 on every site, N in total.  This pins the SU(2) factors of the
+    embeddings (sqrt((1 + 2S') / (1 + 2S))) and of the contraction
+    (1 / sqrt(1 + 2S)) exactly."""
+    sb, st = identity_chain(1, sys_len)
+    eb, et = identity_chain(1, env_len)
+    n = sys_len + env_len
+    N = n - 2
+    wsec = get_super_basis(sb[sys_len], eb[env_len], N, 0)
+    d = int((wsec[:, 4].astype(np.int64) * wsec[:, 5]).sum())
+    for k in range(0, d, max(1, d // 7)):
+        x = np.zeros(d)
+        x[k] = 1.0
+        ave = mo.density_profile(1, n, sys_len, env_len, sb, eb, st, et, wsec, x)
+        assert np.all(np.minimum(np.abs(ave), np.abs(ave - 1.0)) < 1e-13)
+        assert abs(ave.sum() - N) < 1e-12
+
+
+@needs_ref
+@pytest.mark.parametrize("sys_len,env_len", GEOMS)
+def test_oracle_matches_reference(tmp_path, sys_len, env_len):
+    sb, eb, st, et, wsec, x, N = case(sys_len, env_len, 3 + sys_len)
+    ave = mo.density_profile(1, sys_len + env_len, sys_len, env_len, sb, eb, st, et, wsec, x)
+    ref = oracle.ref_measure_density(model(up_bias=1), tmp_path, sys_len, env_len, sb, eb, st, et, wsec, x)
+    assert np.abs(ave - ref).max() < 1e-13
+
+
+def _tables_profile(sys_len, env_len, sb, eb, st, et, wsec, x):
+    """The device programs' task tables (transport per block length, then
+    the contraction) executed on the CPU by plan_eval, which asserts that
+    every read and write stays inside its buffer."""
+    import plan_eval
+    from grand_sql_tj_amd import capi, measure
+    from grand_sql_tj_amd.blocks import OpSet
+    site = measure.num_elec(1)
+    n = sys_len + env_len
+    ave = np.zeros(n)
+    for side, bases, truns, length in ((0, sb, st, sys_len), (1, eb, et, env_len)):
+        cur, cur_data = site, site.data
+        for L in range(2, length):
+            pg = capi.Program.dia_transport(cur, site, bases[L], truns[L])
+            hdr, sub, off = pg.layout()
+            y = plan_eval.evaluate_program(pg, truns[L].data, cur_data)
+            cur, cur_data = OpSet(L, hdr, sub, off, y), y
+        mp = capi.Program.measure(side, cur, site, bases[length], wsec)
+        vals = plan_eval.evaluate_program(mp, cur_data, x).reshape(-1, x.size) @ x
+        if side == 0:
+            ave[:sys_len] = vals
+        else:
+            for i in range(1, env_len + 1):
+                ave[n - i] = vals[i - 1]
+    return ave
+
+
+@pytest.mark.parametrize("sys_len,env_len", GEOMS)
+def test_program_tables_match_oracle(sys_len, env_len):
+    sb, eb, st, et, wsec, x, N = case(sys_len, env_len, 3 + sys_len)
+    want = mo.density_profile(1, sys_len + env_len, sys_len, env_len, sb, eb, st, et, wsec, x)
+    got = _tables_profile(sys_len, env_len, sb, eb, st, et, wsec, x)
+    assert np.abs(got - want).max() < 1e-13
