@@ -415,7 +415,8 @@ def sweep_bench(args):
     cylinder (t = 1, J = 0.4, t' = -0.2 unless --sweep-tp), every bond's
     plan, solve, truncation, block update and wave transform through the
     library.  Reports bonds per second over the whole run, the time per
-    phase and the energy at the end of every sweep."""
+    phase, the energy at the end of every sweep, and the time of the final
+    state's density profile (DMRG.measure_density)."""
     import torch
     from grand_sql_tj_amd import sweep
     from grand_sql_tj_amd.lattice import TJModel
@@ -435,6 +436,14 @@ def sweep_bench(args):
     run_small.run(sweeps=1)          # loads the library and the solver handles
     d, e, wall = run()
     bonds = len(d.log)
+    # the density profile of the final state (measure_operator_dia), timed
+    # after the run: every site operator carried through the run's
+    # truncations on the device, then contracted with the ground state
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dens = d.measure_density()
+    torch.cuda.synchronize()
+    t_meas = time.perf_counter() - t0
     ends = [r.energy for i, r in enumerate(d.log)
             if r.phase == "right-to-left" and (i + 1 == bonds or d.log[i + 1].phase != "right-to-left")]
     line = {
@@ -450,6 +459,8 @@ def sweep_bench(args):
         "host_other_s": wall - sum(d.phase_s.values()),
         "max_superblock_dim": max(r.dim for r in d.log), "max_kept": max(r.kept for r in d.log),
         "last_trunc_err": d.log[-1].trunc_err,
+        "density_profile_s": t_meas, "density_sum": float(dens.sum()),
+        "density_min_max": [float(dens.min()), float(dens.max())],
     }
     print(json.dumps(line), flush=True)
 
