@@ -444,6 +444,10 @@ def sweep_bench(args):
     dens = d.measure_density()
     torch.cuda.synchronize()
     t_meas = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cor = d.measure_cross_correlation()
+    torch.cuda.synchronize()
+    t_cor = time.perf_counter() - t0
     ends = [r.energy for i, r in enumerate(d.log)
             if r.phase == "right-to-left" and (i + 1 == bonds or d.log[i + 1].phase != "right-to-left")]
     line = {
@@ -461,6 +465,7 @@ def sweep_bench(args):
         "last_trunc_err": d.log[-1].trunc_err,
         "density_profile_s": t_meas, "density_sum": float(dens.sum()),
         "density_min_max": [float(dens.min()), float(dens.max())],
+        "cross_correlation_s": t_cor, "cross_correlation_shape": list(cor.shape),
     }
     print(json.dumps(line), flush=True)
 

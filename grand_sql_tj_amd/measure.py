@@ -12,7 +12,9 @@ side; here all operators of a side travel together, one device program per
 block length (tjsq_dia_transport_create).  The contraction with the
 wavefunction (measure_sys_block / _site, measure_env_block / _site,
 code/tjsq/measure.f90:1241-1470) is one more program (tjsq_measure_create)
-whose outputs y_j give <O_j> = psi . y_j.
+whose outputs y_j give <O_j> = psi . y_j.  The same vectors give the
+two-point functions across the bond, <n_i n_j> = y_i . y_j for i on the
+system side and j on the environment side (cross_correlation).
 
 The inputs are what a run holds, or what its checkpoint files hold: the
 fused bases by length (sysbs<L> / envbs<L>), the truncation operators by
@@ -38,12 +40,15 @@ def num_elec(up_bias: int) -> OpSet:
                  np.array([0, 1], np.int64), np.array([0.0, math.sqrt(2.0)]))
 
 
-def side_values(side, site: OpSet, bases, truns, length, wsec, psi):
-    """<O> of the site operator on the `length` sites of one side of the
-    bond: the block's sites 1..length-1 (counted from the block's outer end),
-    then the site next to the bond.  bases[L] (L = 2..length) are the fused
-    bases, truns[L] (L = 2..length-1) objects with `.ops` (layout) and
-    `.data` (device tensor) of the truncation that made block L."""
+def side_vectors(side, site: OpSet, bases, truns, length, wsec, psi):
+    """The vectors y_j (a (length, D) device tensor) with <O_j> = psi . y_j
+    for the site operator on the `length` sites of one side of the bond:
+    the block's sites 1..length-1 (counted from the block's outer end), then
+    the site next to the bond.  Side 0: y_j = O_j psi / sqrt(1 + 2S) on the
+    system index; side 1: psi O_j / sqrt(1 + 2S) on the environment index.
+    bases[L] (L = 2..length) are the fused bases, truns[L] (L = 2..length-1)
+    objects with `.ops` (layout) and `.data` (device tensor) of the
+    truncation that made block L."""
     import torch
     if length < 2:
         raise ValueError("a side holds at least the block site and the bond site")
@@ -62,7 +67,13 @@ def side_values(side, site: OpSet, bases, truns, length, wsec, psi):
     D = psi.numel()
     y = torch.zeros(max(1, mp.sizes["ndata"]), dtype=torch.float64, device=dev)
     mp.run(cur_data, psi, y)
-    return (y[:mp.sizes["ndata"]].view(-1, D) @ psi).cpu().numpy()
+    return y[:mp.sizes["ndata"]].view(-1, D)
+
+
+def side_values(side, site: OpSet, bases, truns, length, wsec, psi):
+    """<O> of the site operator on the `length` sites of one side of the
+    bond, in the order of side_vectors."""
+    return (side_vectors(side, site, bases, truns, length, wsec, psi) @ psi).cpu().numpy()
 
 
 def density_profile(up_bias, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns, wsec, psi):
@@ -78,3 +89,19 @@ def density_profile(up_bias, sys_len, env_len, sys_bases, env_bases, sys_truns, 
     for i in range(1, env_len + 1):
         ave[n - i] = ev[i - 1]
     return ave
+
+
+def cross_correlation(up_bias, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns, wsec, psi):
+    """<n_i n_j> for every system-side site i (rows, lattice sites
+    1..sys_len) and environment-side site j (columns, lattice sites
+    N - j + 1 for j = 1..env_len): the pairs across the bond of
+    Get_oper_cor_dia (code/tjsq/measure.f90:468-634; sys_block_env_block,
+    sys_block_env_site, sys_site_env_block and sys_site_env_site_cor_dia,
+    :1638-2247).  Each of those contractions is
+    sum over sectors and fused sub-sectors of c_i c_j <O_i psi, psi O_j>
+    with one factor per side, so the whole block is the Gram matrix of the
+    two sides' measurement vectors, Y_sys Y_env^T: one GEMM."""
+    site = num_elec(up_bias)
+    ys = side_vectors(0, site, sys_bases, sys_truns, sys_len, wsec, psi)
+    ye = side_vectors(1, site, env_bases, env_truns, env_len, wsec, psi)
+    return (ys @ ye.T).cpu().numpy()

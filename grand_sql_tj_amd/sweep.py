@@ -350,6 +350,16 @@ class DMRG:
         return measure.density_profile(self.model.up_bias, sl, el, self.sys_basis, self.env_basis, self.sys_trun,
                                        self.env_trun, self.wsec, self.psi)
 
+    def measure_cross_correlation(self):
+        """<n_i n_j> for the system-side sites i (lattice 1..sys_len) and the
+        environment-side sites j (lattice N - j + 1) of the last bond."""
+        from . import measure
+        if self.psi is None:
+            raise RuntimeError("no ground state yet")
+        sl, el = self.bond[0] + 1, self.bond[1] + 1
+        return measure.cross_correlation(self.model.up_bias, sl, el, self.sys_basis, self.env_basis, self.sys_trun,
+                                         self.env_trun, self.wsec, self.psi)
+
     def run(self, sweeps=2):
         """Warm-up, then `sweeps` pairs of half sweeps (Get_Ground_State with
         trun_idx = 1).  Returns the last ground-state energy."""

@@ -181,3 +181,50 @@ def density_profile(up_bias, nsite, sys_len, env_len, sys_bases, env_bases, sys_
         ave[nsite - i] = measure_block(1, op, env_bases[env_len], wsec, x)
     ave[nsite - env_len] = measure_site(1, site, env_bases[env_len], wsec, x)
     return ave
+
+
+def _cross_pair(sys_op, sys_is_site, env_op, env_is_site, sys_bs, env_bs, wsec, x):
+    """sys_block_env_block_cor_dia (measure.f90:1638-1735) and its site
+    variants sys_block_env_site (:1736-1834), sys_site_env_block
+    (:1835-1932), sys_site_env_site (:2155-2247): per wavefunction sector,
+    system sub-sector and environment sub-sector, the first matching
+    sub-block of each side's operator (block part, or site part for a site
+    operator), coefficient 1 / sqrt((1 + 2S_sys)(1 + 2S_env)) times the
+    site operators' values."""
+    ssecs, esecs = _secs_by_qn(sys_bs), _secs_by_qn(env_bs)
+    val = 0.0
+    for su, sd, eu, ed, v in _wave_blocks(wsec, x):
+        for bu, bd, tu, td, spos, sdim in ssecs.get((su, sd), []):
+            qs = (tu, td) if sys_is_site else (bu, bd)
+            ms = next((m for u, d, m in sys_op if (u, d) == qs), None)
+            if ms is None:
+                continue
+            for ebu, ebd, etu, etd, epos, edim in esecs.get((eu, ed), []):
+                qe = (etu, etd) if env_is_site else (ebu, ebd)
+                me = next((m for u, d, m in env_op if (u, d) == qe), None)
+                if me is None:
+                    continue
+                c = 1.0 / math.sqrt((1.0 + qs[1]) * (1.0 + qe[1]))
+                vb = v[spos:spos + sdim, epos:epos + edim]
+                left = ms[0, 0] * vb if sys_is_site else ms @ vb
+                right = me[0, 0] * vb if env_is_site else vb @ me
+                val += float(np.sum(left * right)) * c
+    return val
+
+
+def cross_correlation(up_bias, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns, wsec, x):
+    """The across-the-bond block of Get_oper_cor_dia (measure.f90:468-634)
+    for num_elec: <n_i n_j> for system-side sites i = 1..sys_len (rows) and
+    environment-side sites j = 1..env_len (columns; j = env_len is the
+    environment's bond site, lattice N - env_len + 1)."""
+    site = site_num_elec(up_bias)
+    sops = [transport(site, i, sys_bases, sys_truns, sys_len - 1) for i in range(1, sys_len)]
+    eops = [transport(site, j, env_bases, env_truns, env_len - 1) for j in range(1, env_len)]
+    sb, eb = sys_bases[sys_len], env_bases[env_len]
+    out = np.zeros((sys_len, env_len))
+    for i in range(sys_len):
+        so, ss = (sops[i], False) if i < sys_len - 1 else (site, True)
+        for j in range(env_len):
+            eo, es = (eops[j], False) if j < env_len - 1 else (site, True)
+            out[i, j] = _cross_pair(so, ss, eo, es, sb, eb, wsec, x)
+    return out

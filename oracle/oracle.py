@@ -190,6 +190,8 @@ def ref_lib():
             lib.refb_set_bsm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, I32P, I32P]
             lib.refb_set_truns.restype = C.c_int
             lib.refb_set_truns.argtypes = [C.c_int, C.c_int, I32P, I32P, I64P, F64P]
+            lib.refb_cross_cor.restype = C.c_int
+            lib.refb_cross_cor.argtypes = [C.c_int, C.c_int, C.c_int, I32P, F64P, F64P]
             lib.refb_measure_density.restype = C.c_int
             lib.refb_measure_density.argtypes = [C.c_int, C.c_int, C.c_int, I32P, F64P, C.c_int, F64P]
             _libs["ref"] = lib
@@ -704,6 +706,24 @@ def ref_measure_density(model, workdir, sys_len, env_len, sys_bases, env_bases, 
     for i in range(1, env_len + 1):
         out[n - i] = ave[nref - i]
     return out
+
+
+def ref_cross_correlation(model, workdir, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns, wsec, x):
+    """The across-the-bond pairs of Get_oper_cor_dia for num_elec, computed
+    with the reference's own transports and cor_dia routines: a
+    (sys_len, env_len) matrix, rows system-side sites 1..sys_len, columns
+    environment-side sites 1..env_len (the last of each the bond site)."""
+    w = np.ascontiguousarray(wsec, np.int32).reshape(-1, 6)
+    with _lock, _cwd(workdir):
+        lib = _ref_init(model)
+        for side, bases, truns in ((0, sys_bases, sys_truns), (1, env_bases, env_truns)):
+            for L, b in bases.items():
+                lib.refb_set_bsm(side, L, b.length, *_bs(b))
+            for L, t in truns.items():
+                lib.refb_set_truns(side, L, *_one(t))
+        cor = np.zeros(sys_len * env_len)
+        lib.refb_cross_cor(sys_len, env_len, w.shape[0], w, np.ascontiguousarray(x, np.float64), cor)
+    return cor.reshape(env_len, sys_len).T
 
 
 def ref_quicksort(arr):

@@ -976,3 +976,45 @@ integer(c_int) function refb_measure_density(sys_len, env_len, nw, wsec, x, trun
   call deallocate_wave(w)
   refb_measure_density = Num_site
 end function refb_measure_density
+
+! The across-the-bond pairs of Get_oper_cor_dia (measure.f90:468-634) for
+! num_elec, with the reference's own routines: every system-side site i
+! (1..sys_len, the last the bond site) with every environment-side site j
+! (1..env_len, the last the bond site); cor(sys_len, env_len).
+integer(c_int) function refb_cross_cor(sys_len, env_len, nw, wsec, x, cor) bind(C, name="refb_cross_cor")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: sys_len, env_len, nw
+  integer(c_int), intent(in) :: wsec(6, *)
+  real(c_double), intent(in) :: x(*)
+  real(c_double), intent(inout) :: cor(sys_len, env_len)
+  type(Wavefunction) :: w
+  type(Total_Block) :: so, eo
+  integer :: i, j
+  real(8) :: v
+  call put_wave(w, sys_len, env_len, nw, wsec, x)
+  do i = 1, sys_len - 1
+    call Get_sys_oper_dia(num_elec, i, so, sys_len - 1, 1, .true.)
+    do j = 1, env_len - 1
+      call Get_env_oper_dia(num_elec, j, eo, env_len - 1, 1, .true.)
+      call sys_block_env_block_cor_dia(v, so, eo, sys_bsm(sys_len), env_bsm(env_len), w)
+      cor(i, j) = v
+      call deallocate_block(eo)
+    end do
+    call sys_block_env_site_cor_dia(v, so, num_elec, sys_bsm(sys_len), env_bsm(env_len), w)
+    cor(i, env_len) = v
+    call deallocate_block(so)
+  end do
+  do j = 1, env_len - 1
+    call Get_env_oper_dia(num_elec, j, eo, env_len - 1, 1, .true.)
+    call sys_site_env_block_cor_dia(v, num_elec, eo, sys_bsm(sys_len), env_bsm(env_len), w)
+    cor(sys_len, j) = v
+    call deallocate_block(eo)
+  end do
+  call sys_site_env_site_cor_dia(v, num_elec, num_elec, sys_bsm(sys_len), env_bsm(env_len), w)
+  cor(sys_len, env_len) = v
+  call deallocate_wave(w)
+  refb_cross_cor = 0
+end function refb_cross_cor

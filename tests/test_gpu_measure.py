@@ -80,3 +80,40 @@ def test_untruncated_dmrg_density_is_exact(case_):
     # the state converges to the Lanczos tolerance on the energy (1e-12), so to
     # ~1e-6 in norm: densities agree to that order
     assert np.abs(got - want).max() < 2e-5
+
+
+@pytest.mark.parametrize("sys_len,env_len", GEOMS)
+def test_device_cross_correlation_matches_oracle(sys_len, env_len):
+    from grand_sql_tj_amd import measure
+    sb, eb, st, et, wsec, x, N = case(sys_len, env_len, 3 + sys_len)
+    want = mo.cross_correlation(1, sys_len, env_len, sb, eb, st, et, wsec, x)
+    psi = torch.from_numpy(x).cuda()
+    got = measure.cross_correlation(1, sys_len, env_len, sb, eb, {L: _Trun(t) for L, t in st.items()},
+                                    {L: _Trun(t) for L, t in et.items()}, wsec, psi)
+    assert got.shape == (sys_len, env_len)
+    assert np.abs(got - want).max() < 1e-13
+
+
+def _ed_pair_density(model, n_elec):
+    H, states = ed.hamiltonian(model, n_elec)
+    import scipy.sparse.linalg as spla
+    g = spla.eigsh(H, k=1, which="SA", tol=1e-13)[1][:, 0]
+    n = model.nsite
+    up, dn = states >> n, states & ((1 << n) - 1)
+    occ = np.array([((up >> b) & 1) + ((dn >> b) & 1) for b in range(n)], dtype=np.float64)
+    return (occ * (g * g)) @ occ.T
+
+
+def test_untruncated_dmrg_cross_correlation_is_exact():
+    from grand_sql_tj_amd import sweep
+    c = next(c for c in ED if c["ny"] == 2 and c["nx"] == 6 and c["N"] == 10 and c["mu"] == 0.0)
+    m = TJModel(ny=c["ny"], nx=c["nx"], t=c["t"], tp=c["tp"], J=c["J"], mu=c["mu"], up_bias=1)
+    d = sweep.DMRG(m, total_up=c["N"], kept_max=1000, trun_err=1e-12, tol=1e-12)
+    d.run(sweeps=1)
+    got = d.measure_cross_correlation()
+    sl, el = d.bond[0] + 1, d.bond[1] + 1
+    nn = _ed_pair_density(m, c["N"])
+    n = m.nsite
+    want = np.array([[nn[i, n - j] for j in range(1, el + 1)] for i in range(sl)])
+    assert got.shape == (sl, el)
+    assert np.abs(got - want).max() < 2e-5

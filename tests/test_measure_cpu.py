@@ -145,3 +145,77 @@ def test_program_tables_match_oracle(sys_len, env_len):
     want = mo.density_profile(1, sys_len + env_len, sys_len, env_len, sb, eb, st, et, wsec, x)
     got = _tables_profile(sys_len, env_len, sb, eb, st, et, wsec, x)
     assert np.abs(got - want).max() < 1e-13
+
+
+# ---- two-point functions across the bond ------------------------------------------------
+
+def _nsys_nenv(wsec, x):
+    """<N_sys N_env>: every wavefunction sector has definite particle
+    numbers on both sides (num_up with up_bias = 1)."""
+    out, o = 0.0, 0
+    for su, sd, eu, ed, ns, ne in np.asarray(wsec):
+        n = int(ns) * int(ne)
+        out += float(su) * float(eu) * float(np.dot(x[o:o + n], x[o:o + n]))
+        o += n
+    return out
+
+
+@pytest.mark.parametrize("sys_len,env_len", GEOMS)
+def test_cross_correlation_sum_rule(sys_len, env_len):
+    """sum over i (system side) and j (environment side) of <n_i n_j> is
+    <N_sys N_env>, fixed by the sector weights."""
+    sb, eb, st, et, wsec, x, N = case(sys_len, env_len, 3 + sys_len)
+    cor = mo.cross_correlation(1, sys_len, env_len, sb, eb, st, et, wsec, x)
+    assert abs(cor.sum() - _nsys_nenv(wsec, x)) < 1e-12 * N * N
+
+
+@pytest.mark.parametrize("sys_len,env_len", [(3, 3), (4, 5)])
+def test_cross_correlation_of_basis_states_is_product(sys_len, env_len):
+    """Untruncated, a wavefunction on one basis state has definite
+    occupations, so <n_i n_j> = <n_i><n_j>, each 0 or 1."""
+    sb, st = identity_chain(1, sys_len)
+    eb, et = identity_chain(1, env_len)
+    n = sys_len + env_len
+    wsec = get_super_basis(sb[sys_len], eb[env_len], n - 2, 0)
+    d = int((wsec[:, 4].astype(np.int64) * wsec[:, 5]).sum())
+    for k in range(0, d, max(1, d // 5)):
+        x = np.zeros(d)
+        x[k] = 1.0
+        ave = mo.density_profile(1, n, sys_len, env_len, sb, eb, st, et, wsec, x)
+        cor = mo.cross_correlation(1, sys_len, env_len, sb, eb, st, et, wsec, x)
+        env_side = np.array([ave[n - j] for j in range(1, env_len + 1)])
+        assert np.abs(cor - np.outer(ave[:sys_len], env_side)).max() < 1e-13
+
+
+@needs_ref
+@pytest.mark.parametrize("sys_len,env_len", GEOMS)
+def test_cross_correlation_matches_reference(tmp_path, sys_len, env_len):
+    sb, eb, st, et, wsec, x, N = case(sys_len, env_len, 3 + sys_len)
+    cor = mo.cross_correlation(1, sys_len, env_len, sb, eb, st, et, wsec, x)
+    ref = oracle.ref_cross_correlation(model(up_bias=1), tmp_path, sys_len, env_len, sb, eb, st, et, wsec, x)
+    assert np.abs(cor - ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("sys_len,env_len", GEOMS)
+def test_cross_correlation_tables_match_oracle(sys_len, env_len):
+    """The Gram form Y_sys Y_env^T of the device path, with the programs'
+    tables run on the CPU (bounds asserted), against the oracle's
+    restatement of the four reference routines."""
+    import plan_eval
+    from grand_sql_tj_amd import capi, measure
+    from grand_sql_tj_amd.blocks import OpSet
+    sb, eb, st, et, wsec, x, N = case(sys_len, env_len, 3 + sys_len)
+    site = measure.num_elec(1)
+    ys = []
+    for side, bases, truns, length in ((0, sb, st, sys_len), (1, eb, et, env_len)):
+        cur, cur_data = site, site.data
+        for L in range(2, length):
+            pg = capi.Program.dia_transport(cur, site, bases[L], truns[L])
+            hdr, sub, off = pg.layout()
+            y = plan_eval.evaluate_program(pg, truns[L].data, cur_data)
+            cur, cur_data = OpSet(L, hdr, sub, off, y), y
+        mp = capi.Program.measure(side, cur, site, bases[length], wsec)
+        ys.append(plan_eval.evaluate_program(mp, cur_data, x).reshape(-1, x.size))
+    got = ys[0] @ ys[1].T
+    want = mo.cross_correlation(1, sys_len, env_len, sb, eb, st, et, wsec, x)
+    assert np.abs(got - want).max() < 1e-13
