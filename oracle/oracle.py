@@ -190,6 +190,8 @@ def ref_lib():
             lib.refb_set_bsm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, I32P, I32P]
             lib.refb_set_truns.restype = C.c_int
             lib.refb_set_truns.argtypes = [C.c_int, C.c_int, I32P, I32P, I64P, F64P]
+            lib.refb_oper_cor_dia.restype = C.c_int
+            lib.refb_oper_cor_dia.argtypes = [C.c_int, C.c_int, C.c_int, I32P, F64P, F64P]
             lib.refb_cross_cor.restype = C.c_int
             lib.refb_cross_cor.argtypes = [C.c_int, C.c_int, C.c_int, I32P, F64P, F64P]
             lib.refb_measure_density.restype = C.c_int
@@ -724,6 +726,27 @@ def ref_cross_correlation(model, workdir, sys_len, env_len, sys_bases, env_bases
         cor = np.zeros(sys_len * env_len)
         lib.refb_cross_cor(sys_len, env_len, w.shape[0], w, np.ascontiguousarray(x, np.float64), cor)
     return cor.reshape(env_len, sys_len).T
+
+
+def ref_oper_cor_dia(model, workdir, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns, wsec, x):
+    """The reference's whole Get_oper_cor_dia (measure.f90:468-634) of
+    num_elec: the (n, n) matrix oper_dia over the wavefunction's
+    sys_len + env_len sites in lattice order (it appends to two files in
+    workdir)."""
+    w = np.ascontiguousarray(wsec, np.int32).reshape(-1, 6)
+    with _lock, _cwd(workdir):
+        lib = _ref_init(model)
+        for side, bases, truns in ((0, sys_bases, sys_truns), (1, env_bases, env_truns)):
+            for L, b in bases.items():
+                lib.refb_set_bsm(side, L, b.length, *_bs(b))
+            for L, t in truns.items():
+                lib.refb_set_truns(side, L, *_one(t))
+        cor = np.zeros(4096 * 4096 // 64)
+        with _quiet():
+            nref = lib.refb_oper_cor_dia(sys_len, env_len, w.shape[0], w, np.ascontiguousarray(x, np.float64), cor)
+    full = cor[:nref * nref].reshape(nref, nref).T
+    idx = list(range(sys_len)) + [nref - j for j in range(env_len, 0, -1)]
+    return full[np.ix_(idx, idx)]
 
 
 def ref_quicksort(arr):

@@ -1018,3 +1018,28 @@ integer(c_int) function refb_cross_cor(sys_len, env_len, nw, wsec, x, cor) bind(
   call deallocate_wave(w)
   refb_cross_cor = 0
 end function refb_cross_cor
+
+! Get_oper_cor_dia (measure.f90:468-634) of num_elec in full: cor(Num_site,
+! Num_site) receives oper_dia (it appends to oper_dia_tmp.dat and the
+! given file in the current directory).
+integer(c_int) function refb_oper_cor_dia(sys_len, env_len, nw, wsec, x, cor) bind(C, name="refb_oper_cor_dia")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: sys_len, env_len, nw
+  integer(c_int), intent(in) :: wsec(6, *)
+  real(c_double), intent(in) :: x(*)
+  real(c_double), intent(inout) :: cor(Num_site, Num_site)
+  type(Wavefunction) :: w
+  real(8), allocatable :: od(:, :)
+  character(len=30) :: fname
+  fname = "num_elec_cor.dat"
+  allocate(od(Num_site, Num_site))
+  call put_wave(w, sys_len, env_len, nw, wsec, x)
+  call Get_oper_cor_dia(od, num_elec, w, 1, fname)
+  cor = od
+  call deallocate_wave(w)
+  deallocate(od)
+  refb_oper_cor_dia = Num_site
+end function refb_oper_cor_dia

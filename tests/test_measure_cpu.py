@@ -219,3 +219,23 @@ def test_cross_correlation_tables_match_oracle(sys_len, env_len):
     got = ys[0] @ ys[1].T
     want = mo.cross_correlation(1, sys_len, env_len, sb, eb, st, et, wsec, x)
     assert np.abs(got - want).max() < 1e-13
+
+
+@needs_ref
+@pytest.mark.parametrize("sys_len,env_len", [(5, 5), (6, 8)])
+def test_reference_full_correlation_routine(tmp_path, sys_len, env_len):
+    """The reference's whole Get_oper_cor_dia: its across-the-bond entries
+    are the oracle's cross_correlation.  Its same-side entries are not
+    physical correlations: block_mul_block_dia (block.f90:1924-1954)
+    multiplies two reduced matrices without the SU(2) factor, so its
+    diagonal <n_i^2> comes out as sqrt(2) <n_i> where n_i^2 = n_i; those
+    entries are therefore not restated (this test records the quirk)."""
+    sb, eb, st, et, wsec, x, N = case(sys_len, env_len, 3 + sys_len)
+    n = sys_len + env_len
+    full = oracle.ref_oper_cor_dia(model(up_bias=1), tmp_path, sys_len, env_len, sb, eb, st, et, wsec, x)
+    cross = mo.cross_correlation(1, sys_len, env_len, sb, eb, st, et, wsec, x)
+    env_cols = [n - j for j in range(1, env_len + 1)]
+    assert np.abs(full[:sys_len][:, env_cols] - cross).max() < 1e-13
+    assert np.abs(full[env_cols][:, :sys_len] - cross.T).max() < 1e-13
+    dens = mo.density_profile(1, n, sys_len, env_len, sb, eb, st, et, wsec, x)
+    assert np.abs(np.diag(full) - np.sqrt(2.0) * dens).max() < 1e-12
