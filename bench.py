@@ -448,6 +448,10 @@ def sweep_bench(args):
     cor = d.measure_cross_correlation()
     torch.cuda.synchronize()
     t_cor = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    full = d.measure_correlations()
+    torch.cuda.synchronize()
+    t_full = time.perf_counter() - t0
     ends = [r.energy for i, r in enumerate(d.log)
             if r.phase == "right-to-left" and (i + 1 == bonds or d.log[i + 1].phase != "right-to-left")]
     line = {
@@ -466,6 +470,7 @@ def sweep_bench(args):
         "density_profile_s": t_meas, "density_sum": float(dens.sum()),
         "density_min_max": [float(dens.min()), float(dens.max())],
         "cross_correlation_s": t_cor, "cross_correlation_shape": list(cor.shape),
+        "correlation_matrix_s": t_full, "correlation_matrix_sum": float(full.sum()),
     }
     print(json.dumps(line), flush=True)
 

@@ -80,7 +80,8 @@ EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_t
            "tjsq_hostplan_stats", "tjsq_hostplan_destroy", "tjsq_plan_profile", "tjsq_plan_profile_read",
            "tjsq_heff_apply_host", "tjsq_lanczos_solve_host",
            "tjsq_block_update_create", "tjsq_trun_project_create", "tjsq_density_create", "tjsq_wave_transform_create",
-           "tjsq_dia_transport_create", "tjsq_measure_create", "tjsq_program_sizes",
+           "tjsq_dia_transport_create", "tjsq_dia_transport_pairs_create", "tjsq_measure_create",
+           "tjsq_program_sizes",
            "tjsq_program_layout", "tjsq_program_tables", "tjsq_program_stats", "tjsq_program_run",
            "tjsq_program_run_host", "tjsq_truncation_operator_host",
            "tjsq_program_profile", "tjsq_program_profile_read", "tjsq_program_destroy", "tjsq_truncation_operator",
@@ -126,6 +127,7 @@ def lib():
             "tjsq_block_update_create": [P, I, I, I, P, P, P, P],
             "tjsq_trun_project_create": [P, P, P, P],
             "tjsq_dia_transport_create": [P, P, P, P, P],
+            "tjsq_dia_transport_pairs_create": [P, I, P, P, P, P],
             "tjsq_measure_create": [I, P, P, P, I, P, P],
             "tjsq_density_create": [I, I, I, P, I, P],
             "tjsq_wave_transform_create": [I, P, P, P, P, I, P, I, P, P],
@@ -480,6 +482,21 @@ class Program:
         check(lib().tjsq_dia_transport_create(C.byref(os_) if os_ is not None else None,
                                               C.byref(ss) if ss is not None else None, C.byref(bs), C.byref(ts),
                                               C.byref(h)), "tjsq_dia_transport_create")
+        return cls(h)
+
+    @classmethod
+    def dia_transport_pairs(cls, ops, nprod, site, basis, trun):
+        """dia_transport with the products of the first nprod operators and
+        the site operator appended; output order (nprod > 0): ops[:nprod],
+        the site operator, ops[nprod:], the products."""
+        keep = _Keep()
+        os_ = opset_struct(ops, keep) if ops is not None and ops.nops > 0 else None
+        ss = opset_struct(site, keep)
+        bs, ts = basis_struct(basis, keep), opset_struct(trun, keep)
+        h = C.c_void_p()
+        check(lib().tjsq_dia_transport_pairs_create(C.byref(os_) if os_ is not None else None, int(nprod),
+                                                    C.byref(ss), C.byref(bs), C.byref(ts), C.byref(h)),
+              "tjsq_dia_transport_pairs_create")
         return cls(h)
 
     @classmethod

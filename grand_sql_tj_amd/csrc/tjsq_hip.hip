@@ -1814,6 +1814,17 @@ int tjsq_dia_transport_create(const tjsq_opset* ops, const tjsq_opset* site, con
   return make_program(pg, 0, 1, prog);
 }
 
+int tjsq_dia_transport_pairs_create(const tjsq_opset* ops, int nprod, const tjsq_opset* site, const tjsq_basis* basis,
+                                    const tjsq_opset* trun, tjsq_program** prog) {
+  if (!basis || !trun || !prog) return fail(TJSQ_ERR_ARG, "tjsq_dia_transport_pairs_create: null argument");
+  *prog = nullptr;
+  tjsq::Program pg;
+  std::string msg;
+  int rc = tjsq::build_dia_transport_program(ops, site, nprod, *basis, *trun, tjsq::Shape(), pg, msg);
+  if (rc) return fail(rc, "tjsq_dia_transport_pairs_create: " + msg);
+  return make_program(pg, 0, 1, prog);
+}
+
 int tjsq_measure_create(int side, const tjsq_opset* ops, const tjsq_opset* site, const tjsq_basis* basis, int nw,
                         const int* wsec, tjsq_program** prog) {
   if (!basis || !wsec || !prog) return fail(TJSQ_ERR_ARG, "tjsq_measure_create: null argument");

@@ -614,7 +614,44 @@ bool all_dia(const std::vector<Op>& ops) {
 // every diagonal operator of the block at length L embedded into the fused
 // basis (update_block_dia), the new site's operator embedded
 // (update_site_dia), each projected with T (update_trun_dia).
+namespace {
+
+// the product A (x) s of a diagonal block operator A and the new site's
+// operator s in the fused basis, as a reduced operator: per fused sector k
+// and sub-sector x (block part bl, site part st) the placement
+// sqrt((1 + 2S_k) / (1 + 2S_bl)) * s(st) / sqrt(1 + 2S_st) * A(bl), the
+// embedding of update_block_dia times the site operator's value
+SymOp upd_block_site_product(const Op& blk, const Site& st, const BasisView& bs, int len, Ctx& cx) {
+  SymOp o;
+  o.present = true; o.len = len;
+  for (const MapSec& k : bs.sec) o.sub.push_back({k.new_up, k.new_down, 0, k.dim, k.dim, {}});
+  for (size_t k = 0; k < bs.sec.size(); ++k)
+    for (const MapSub& m : bs.sec[k].sub) {
+      double sv = 0.0;
+      for (size_t i = 0; i < st.sub.size(); ++i)
+        if (st.sub[i].num_up == m.st_up && st.sub[i].num_down == m.st_down) {
+          sv = st.val[i] / std::sqrt(1.0 + st.sub[i].num_down);
+          break;
+        }
+      if (sv == 0.0) continue;
+      for (const OpSub& q : blk.sub)
+        if (q.num_up == m.bl_up && q.num_down == m.bl_down) {
+          const double c = std::sqrt((1.0 + bs.sec[k].new_down) / (1.0 + q.num_down)) * sv;
+          place_block(o.sub[k], m.spos, m.spos, m.sdim, m.sdim, c, q, false, cx);
+          break;
+        }
+    }
+  return o;
+}
+
+}  // namespace
+
 int build_dia_transport_program(const tjsq_opset* ops_in, const tjsq_opset* site, const tjsq_basis& basis,
+                                const tjsq_opset& trun, const Shape& shape, Program& pg, std::string& msg) {
+  return build_dia_transport_program(ops_in, site, 0, basis, trun, shape, pg, msg);
+}
+
+int build_dia_transport_program(const tjsq_opset* ops_in, const tjsq_opset* site, int nprod, const tjsq_basis& basis,
                                 const tjsq_opset& trun, const Shape& shape, Program& pg, std::string& msg) {
   pg = Program();
   std::vector<Op> ops, tops;
@@ -633,8 +670,17 @@ int build_dia_transport_program(const tjsq_opset* ops_in, const tjsq_opset* site
   read_basis(basis, bs);
   Ctx cx;
   Projector pj{pg, tops[0], cx};
-  for (const Op& o : ops) pj.project(upd_block_dia(o, bs, basis.length, cx), true);
-  if (site) pj.project(upd_site_dia(st, bs, basis.length), true);
+  if (nprod < 0 || nprod > (int)ops.size() || (nprod > 0 && !site)) {
+    msg = "product count must lie within the operators and needs the site operator";
+    return TJSQ_ERR_ARG;
+  }
+  // output order: ops[0, nprod), the site operator, ops[nprod, end), then
+  // the products (with nprod = 0: ops, then the site operator)
+  for (int i = 0; i < nprod; ++i) pj.project(upd_block_dia(ops[i], bs, basis.length, cx), true);
+  if (site && nprod > 0) pj.project(upd_site_dia(st, bs, basis.length), true);
+  for (size_t i = nprod; i < ops.size(); ++i) pj.project(upd_block_dia(ops[i], bs, basis.length, cx), true);
+  if (site && nprod == 0) pj.project(upd_site_dia(st, bs, basis.length), true);
+  for (int i = 0; i < nprod; ++i) pj.project(upd_block_site_product(ops[i], st, bs, basis.length, cx), true);
   pg.x_elems = ops.empty() ? 0 : max_extent(*ops_in);
   pg.ops_elems = max_extent(trun);
   return finish(pg, shape, cx, msg);

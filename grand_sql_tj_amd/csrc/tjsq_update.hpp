@@ -83,6 +83,13 @@ int build_projection_program(const tjsq_opset& ops, const int* ndia, const tjsq_
 // with ops = trun data, x = the data of ops.
 int build_dia_transport_program(const tjsq_opset* ops, const tjsq_opset* site, const tjsq_basis& basis,
                                 const tjsq_opset& trun, const Shape& shape, Program& pg, std::string& msg);
+// The same with the products of the first nprod operators of ops with the
+// site operator appended (each embedded as a reduced operator of the fused
+// basis, then projected): the two-point operators n_i n_j of a block formed
+// when site j joins it.  Output order for nprod > 0: ops[0, nprod), the
+// site operator, ops[nprod, end), the products.
+int build_dia_transport_program(const tjsq_opset* ops, const tjsq_opset* site, int nprod, const tjsq_basis& basis,
+                                const tjsq_opset& trun, const Shape& shape, Program& pg, std::string& msg);
 
 // Expectation values of diagonal block operators and of a site operator in
 // a wavefunction (measure_sys_block / _site, measure_env_block / _site,

@@ -105,3 +105,84 @@ def cross_correlation(up_bias, sys_len, env_len, sys_bases, env_bases, sys_truns
     ys = side_vectors(0, site, sys_bases, sys_truns, sys_len, wsec, psi)
     ye = side_vectors(1, site, env_bases, env_truns, env_len, wsec, psi)
     return (ys @ ye.T).cpu().numpy()
+
+
+def _trivial_basis(basis):
+    """The fused basis read as a block basis of one sub-sector per sector,
+    for measuring operators that already live in the fused basis."""
+    from .blocks import Basis
+    sec = np.asarray(basis.sec)
+    return Basis(basis.length, np.stack([sec[:, 0], sec[:, 1], sec[:, 2], np.ones(len(sec), np.int32)], 1)
+                 .astype(np.int32),
+                 np.stack([sec[:, 0], sec[:, 1], np.zeros(len(sec), np.int32), np.zeros(len(sec), np.int32),
+                           np.zeros(len(sec), np.int32), sec[:, 2]], 1).astype(np.int32))
+
+
+def side_pair_vectors(side, site: OpSet, bases, truns, length, wsec, psi):
+    """Measurement vectors of every single-site operator n_i and every
+    same-side product n_i n_j (i < j) of one side, all carried to the
+    wavefunction's fused basis: returns (Y, labels), Y a (len(labels), D)
+    device tensor with <O> = psi . y, labels (i,) for singles and (i, j)
+    for pairs, sites counted from the side's outer end (the bond site is
+    `length`).  Each product is formed when its second site joins the block,
+    as a reduced operator (tjsq_dia_transport_pairs_create), then carried on
+    like the singles."""
+    import torch
+    dev = psi.device
+    cur = site
+    cur_data = torch.from_numpy(np.ascontiguousarray(site.data, np.float64)).to(dev)
+    singles, pairs = [(1,)], []
+    for L in range(2, length + 1):
+        if L < length:
+            t_ops, t_data = truns[L].ops, truns[L].data
+        else:                      # the fused basis of the wavefunction itself: no truncation
+            tsub, toff, nd = capi.identity_truncation(bases[L])
+            t_data = torch.empty(max(1, nd), dtype=torch.float64, device=dev)
+            capi.identity_truncation(bases[L], t_data)
+            t_ops = OpSet(L, np.array([[L, 0, 0, tsub.shape[0]]], np.int32), tsub.astype(np.int32),
+                          toff.astype(np.int64), np.zeros(max(1, nd)))
+        pg = capi.Program.dia_transport_pairs(cur, len(singles), site, bases[L], t_ops)
+        hdr, sub, off = pg.layout()
+        n = max(1, pg.sizes["ndata"])
+        out = torch.empty(n, dtype=torch.float64, device=dev)
+        pg.run(t_data, cur_data, out)
+        pairs = pairs + [(i[0], L) for i in singles]
+        singles = singles + [(L,)]
+        cur, cur_data = OpSet(L, hdr, sub, off, np.zeros(n)), out
+    mp = capi.Program.measure(side, cur, None, _trivial_basis(bases[length]), wsec)
+    D = psi.numel()
+    y = torch.zeros(max(1, mp.sizes["ndata"]), dtype=torch.float64, device=dev)
+    mp.run(cur_data, psi, y)
+    Y = y[:mp.sizes["ndata"]].view(-1, D)
+    return Y[:-1], singles + pairs
+
+
+def correlation_matrix(up_bias, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns, wsec, psi):
+    """<n_i n_j> for all pairs of the sys_len + env_len lattice sites (lattice
+    order; environment site j from the outer end is lattice N - j + 1):
+    - across the bond, the Gram matrix of the two sides' single-site vectors
+      (the entries of the reference's Get_oper_cor_dia, measure.f90:468-634);
+    - on one side, the carried products n_i n_j, formed with their SU(2)
+      factor (where the reference's block_mul_block_dia leaves one factor
+      sqrt(1 + 2S) too many, those entries here are the physical ones);
+    - on the diagonal <n_i^2> = <n_i> (no double occupancy)."""
+    site = num_elec(up_bias)
+    n = sys_len + env_len
+    cor = np.zeros((n, n))
+    lat = [lambda i: i - 1, lambda j: n - j]
+    vecs = []
+    for side, bases, truns, length in ((0, sys_bases, sys_truns, sys_len), (1, env_bases, env_truns, env_len)):
+        Y, labels = side_pair_vectors(side, site, bases, truns, length, wsec, psi)
+        vals = (Y @ psi).cpu().numpy()
+        f = lat[side]
+        for lab, v in zip(labels, vals):
+            if len(lab) == 1:
+                cor[f(lab[0]), f(lab[0])] = v
+            else:
+                cor[f(lab[0]), f(lab[1])] = cor[f(lab[1]), f(lab[0])] = v
+        vecs.append(Y[:length])
+    cross = (vecs[0] @ vecs[1].T).cpu().numpy()
+    for i in range(sys_len):
+        for j in range(1, env_len + 1):
+            cor[i, n - j] = cor[n - j, i] = cross[i, j - 1]
+    return cor

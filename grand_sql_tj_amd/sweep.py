@@ -360,6 +360,16 @@ class DMRG:
         return measure.cross_correlation(self.model.up_bias, sl, el, self.sys_basis, self.env_basis, self.sys_trun,
                                          self.env_trun, self.wsec, self.psi)
 
+    def measure_correlations(self):
+        """The whole <n_i n_j> matrix (lattice order) in the ground state of
+        the last bond (measure.correlation_matrix)."""
+        from . import measure
+        if self.psi is None:
+            raise RuntimeError("no ground state yet")
+        sl, el = self.bond[0] + 1, self.bond[1] + 1
+        return measure.correlation_matrix(self.model.up_bias, sl, el, self.sys_basis, self.env_basis, self.sys_trun,
+                                          self.env_trun, self.wsec, self.psi)
+
     def run(self, sweeps=2):
         """Warm-up, then `sweeps` pairs of half sweeps (Get_Ground_State with
         trun_idx = 1).  Returns the last ground-state energy."""

@@ -326,6 +326,19 @@ int tjsq_trun_project_create(const tjsq_opset* ops, const int* ndia, const tjsq_
 int tjsq_dia_transport_create(const tjsq_opset* ops, const tjsq_opset* site, const tjsq_basis* basis,
                               const tjsq_opset* trun, tjsq_program** prog);
 
+/* tjsq_dia_transport_create with the products of the first nprod
+ * operators of `ops` with the site operator appended to the output (output
+ * order for nprod > 0: ops[0, nprod), the site operator, ops[nprod, end),
+ * the products), each product a reduced operator of the fused basis
+ * (sqrt((1+2S)/(1+2S_bl)) A(bl) times the site operator's value
+ * s(st)/sqrt(1+2S_st)), then projected: the two-point operators n_i n_j
+ * of a block, formed when site j joins it.  No reference counterpart: the
+ * reference forms such products with block_mul_block_dia
+ * (code/tjsq/block.f90:1924-1954), which multiplies reduced matrices
+ * without the SU(2) factor. */
+int tjsq_dia_transport_pairs_create(const tjsq_opset* ops, int nprod, const tjsq_opset* site, const tjsq_basis* basis,
+                                    const tjsq_opset* trun, tjsq_program** prog);
+
 /* Expectation values in a wavefunction of diagonal block operators and of
  * the site operator next to the block (measure_sys_block / measure_sys_site
  * for side 0, measure_env_block / measure_env_site for side 1,

@@ -117,3 +117,36 @@ def test_untruncated_dmrg_cross_correlation_is_exact():
     want = np.array([[nn[i, n - j] for j in range(1, el + 1)] for i in range(sl)])
     assert got.shape == (sl, el)
     assert np.abs(got - want).max() < 2e-5
+
+
+@pytest.mark.parametrize("case_", [c for c in ED if c["nx"] * c["ny"] <= 12][:3],
+                         ids=lambda c: f"{c['ny']}x{c['nx']}N{c['N']}")
+def test_untruncated_dmrg_correlation_matrix_is_exact(case_):
+    """The whole <n_i n_j> matrix (same-side products formed with their SU(2)
+    factor, cross-bond pairs as a Gram matrix) against exact
+    diagonalisation, entry by entry."""
+    from grand_sql_tj_amd import measure, sweep
+    m = TJModel(ny=case_["ny"], nx=case_["nx"], t=case_["t"], tp=case_["tp"], J=case_["J"], mu=case_["mu"],
+                up_bias=1)
+    d = sweep.DMRG(m, total_up=case_["N"], kept_max=1000, trun_err=1e-12, tol=1e-12)
+    d.run(sweeps=1)
+    sl, el = d.bond[0] + 1, d.bond[1] + 1
+    got = measure.correlation_matrix(1, sl, el, d.sys_basis, d.env_basis, d.sys_trun, d.env_trun, d.wsec, d.psi)
+    want = _ed_pair_density(m, case_["N"])
+    assert np.abs(got - want).max() < 2e-5
+    assert abs(got.sum() - case_["N"] ** 2) < 1e-9 * case_["N"] ** 2
+
+
+def test_device_pair_vectors_match_tables():
+    """side_pair_vectors on the device against the same programs' tables run
+    on the CPU, on a truncated synthetic chain."""
+    from grand_sql_tj_amd import measure
+    from test_measure_cpu import _tables_pairs
+    sb, eb, st, et, wsec, x, N = case(6, 8, 9)
+    psi = torch.from_numpy(x).cuda()
+    for side, bases, truns, length in ((0, sb, st, 6), (1, eb, et, 8)):
+        dev_truns = {L: _Trun(t) for L, t in truns.items()}
+        Y, labels = measure.side_pair_vectors(side, measure.num_elec(1), bases, dev_truns, length, wsec, psi)
+        Yc, lc = _tables_pairs(side, bases, truns, length, wsec, x)
+        assert labels == lc
+        assert np.abs((Y @ psi).cpu().numpy() - Yc @ x).max() < 1e-13

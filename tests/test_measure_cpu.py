@@ -239,3 +239,82 @@ def test_reference_full_correlation_routine(tmp_path, sys_len, env_len):
     assert np.abs(full[env_cols][:, :sys_len] - cross.T).max() < 1e-13
     dens = mo.density_profile(1, n, sys_len, env_len, sb, eb, st, et, wsec, x)
     assert np.abs(np.diag(full) - np.sqrt(2.0) * dens).max() < 1e-12
+
+
+# ---- same-side two-point functions (products formed with their SU(2) factor) -------------
+
+def _tables_pairs(side, bases, truns, length, wsec, x):
+    """side_pair_vectors of grand_sql_tj_amd.measure with the programs'
+    tables run on the CPU by plan_eval (bounds asserted)."""
+    import plan_eval
+    from grand_sql_tj_amd import capi, measure
+    from grand_sql_tj_amd.blocks import OpSet
+    site = measure.num_elec(1)
+    cur, cur_data = site, site.data
+    singles, pairs = [(1,)], []
+    for L in range(2, length + 1):
+        if L < length:
+            t = truns[L]
+        else:
+            tsub, toff, nd = capi.identity_truncation(bases[L])
+            data = np.zeros(max(1, nd))
+            for (u, d, _, r, c), o in zip(tsub, toff):
+                data[o:o + r * c] = np.eye(r).reshape(-1)
+            t = opset(L, [[L, 0, 0, tsub.shape[0]]], tsub, toff, data)
+        pg = capi.Program.dia_transport_pairs(cur, len(singles), site, bases[L], t)
+        hdr, sub, off = pg.layout()
+        y = plan_eval.evaluate_program(pg, t.data, cur_data)
+        pairs = pairs + [(i[0], L) for i in singles]
+        singles = singles + [(L,)]
+        cur, cur_data = OpSet(L, hdr, sub, off, y), y
+    mp = capi.Program.measure(side, cur, None, measure._trivial_basis(bases[length]), wsec)
+    Y = plan_eval.evaluate_program(mp, cur_data, x).reshape(-1, x.size)[:-1]
+    return Y, singles + pairs
+
+
+def _side_pair_sum(side, wsec, x):
+    """sum over i < j on one side of <n_i n_j> = <N_side (N_side - 1) / 2>:
+    the side's particle number is fixed in each wavefunction sector."""
+    out, o = 0.0, 0
+    for su, sd, eu, ed, ns, ne in np.asarray(wsec):
+        n = int(ns) * int(ne)
+        N = float(su if side == 0 else eu)
+        out += N * (N - 1) / 2 * float(np.dot(x[o:o + n], x[o:o + n]))
+        o += n
+    return out
+
+
+@pytest.mark.parametrize("sys_len,env_len", GEOMS)
+def test_pair_tables_singles_and_sum_rule(sys_len, env_len):
+    sb, eb, st, et, wsec, x, N = case(sys_len, env_len, 3 + sys_len)
+    n = sys_len + env_len
+    dens = mo.density_profile(1, n, sys_len, env_len, sb, eb, st, et, wsec, x)
+    for side, bases, truns, length in ((0, sb, st, sys_len), (1, eb, et, env_len)):
+        Y, labels = _tables_pairs(side, bases, truns, length, wsec, x)
+        vals = Y @ x
+        single = {lab[0]: v for lab, v in zip(labels, vals) if len(lab) == 1}
+        for i in range(1, length + 1):
+            want = dens[i - 1] if side == 0 else dens[n - i]
+            assert abs(single[i] - want) < 1e-13
+        psum = sum(v for lab, v in zip(labels, vals) if len(lab) == 2)
+        assert len(labels) == length + length * (length - 1) // 2
+        assert abs(psum - _side_pair_sum(side, wsec, x)) < 1e-11 * N * N
+
+
+@pytest.mark.parametrize("sys_len,env_len", [(3, 3), (4, 5)])
+def test_pair_tables_of_basis_states_are_products(sys_len, env_len):
+    sb, st = identity_chain(1, sys_len)
+    eb, et = identity_chain(1, env_len)
+    n = sys_len + env_len
+    wsec = get_super_basis(sb[sys_len], eb[env_len], n - 2, 0)
+    d = int((wsec[:, 4].astype(np.int64) * wsec[:, 5]).sum())
+    for k in range(0, d, max(1, d // 5)):
+        x = np.zeros(d)
+        x[k] = 1.0
+        dens = mo.density_profile(1, n, sys_len, env_len, sb, eb, st, et, wsec, x)
+        for side, bases, truns, length in ((0, sb, st, sys_len), (1, eb, et, env_len)):
+            Y, labels = _tables_pairs(side, bases, truns, length, wsec, x)
+            occ = {i: (dens[i - 1] if side == 0 else dens[n - i]) for i in range(1, length + 1)}
+            for lab, v in zip(labels, Y @ x):
+                want = occ[lab[0]] if len(lab) == 1 else occ[lab[0]] * occ[lab[1]]
+                assert abs(v - want) < 1e-13
