@@ -186,3 +186,48 @@ def correlation_matrix(up_bias, sys_len, env_len, sys_bases, env_bases, sys_trun
         for j in range(1, env_len + 1):
             cor[i, n - j] = cor[n - j, i] = cross[i, j - 1]
     return cor
+
+
+class _DeviceTrun:
+    def __init__(self, ops, data):
+        self.ops, self.data = ops, data
+
+
+def load_run(directory, device="cuda"):
+    """What Measurement reads from a run's checkpoint files
+    (code/tjsq/main.f90:213-238): eigvec1, the truncation operators
+    systruns<L> / envtruns<L> (data moved to the device) and the fused bases
+    sysbs<L> / envbs<L>.  Returns a dict with sys_len, env_len, wsec, psi,
+    sys_bases, env_bases, sys_truns, env_truns, ready for density_profile /
+    correlation_matrix."""
+    import os
+    import re
+    import torch
+    from . import checkpoint as ck
+    w = ck.load_wave(os.path.join(directory, ck.wave_file(1)))
+    out = {"sys_len": w.sys_len, "env_len": w.env_len, "wsec": w.wsec,
+           "psi": torch.from_numpy(w.x).to(device), "sys_bases": {}, "env_bases": {}, "sys_truns": {}, "env_truns": {}}
+    pat = re.compile(r"^(sys|env)(truns|bs)([0-9]+)$")
+    for name in sorted(os.listdir(directory)):
+        m = pat.match(name)
+        if not m:
+            continue
+        side, L = m.group(1), int(m.group(3))
+        path = os.path.join(directory, name)
+        if m.group(2) == "bs":
+            out[side + "_bases"][L], _ = ck.load_basis(path)
+        else:
+            t, _ = ck.load_trun(path)
+            data = torch.from_numpy(t.data if t.data.size else np.zeros(1)).to(device)
+            out[side + "_truns"][L] = _DeviceTrun(OpSet(L, t.hdr, t.sub, t.off, np.zeros(max(1, t.data.size))), data)
+    return out
+
+
+def measure_run(directory, up_bias=1):
+    """The density profile and the whole <n_i n_j> matrix of the ground state
+    stored in a checkpoint directory (the reference's or one written by
+    DMRG.save_checkpoint), as Measurement computes them from the same files."""
+    r = load_run(directory)
+    args = (up_bias, r["sys_len"], r["env_len"], r["sys_bases"], r["env_bases"], r["sys_truns"], r["env_truns"],
+            r["wsec"], r["psi"])
+    return density_profile(*args), correlation_matrix(*args)

@@ -150,3 +150,19 @@ def test_device_pair_vectors_match_tables():
         Yc, lc = _tables_pairs(side, bases, truns, length, wsec, x)
         assert labels == lc
         assert np.abs((Y @ psi).cpu().numpy() - Yc @ x).max() < 1e-13
+
+
+def test_measure_from_checkpoint_files(tmp_path):
+    """Measurement's flow: the density profile and correlation matrix from a
+    checkpoint directory (written here by DMRG.save_checkpoint in the
+    reference's format) equal the ones measured on the live run."""
+    from grand_sql_tj_amd import measure, sweep
+    c = next(c for c in ED if c["ny"] == 2 and c["nx"] == 8)
+    m = TJModel(ny=c["ny"], nx=c["nx"], t=c["t"], tp=c["tp"], J=c["J"], mu=c["mu"], up_bias=1)
+    d = sweep.DMRG(m, total_up=c["N"], kept_max=48, trun_err=1e-12, tol=1e-12)
+    d.run(sweeps=1)
+    d.save_checkpoint(tmp_path)
+    dens, cor = measure.measure_run(tmp_path)
+    assert np.abs(dens - d.measure_density()).max() < 1e-13
+    assert np.abs(cor - d.measure_correlations()).max() < 1e-13
+    assert abs(dens.sum() - c["N"]) < 1e-10 and abs(cor.sum() - c["N"] ** 2) < 1e-8
