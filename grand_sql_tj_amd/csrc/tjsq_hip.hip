@@ -1990,6 +1990,9 @@ int tjsq_program_run_host(tjsq_program* P, const double* ops, const double* x, d
     rc = fail(TJSQ_ERR_HIP, "tjsq_program_run_host: copy of ops to the device failed");
   if (rc == TJSQ_OK && nx > 0 && hipMemcpy(dx, x, sizeof(double) * nx, hipMemcpyHostToDevice) != hipSuccess)
     rc = fail(TJSQ_ERR_HIP, "tjsq_program_run_host: copy of x to the device failed");
+  // y starts cleared: measurement programs leave entries no term reaches
+  if (rc == TJSQ_OK && hipMemset(dy, 0, sizeof(double) * std::max<int64_t>(1, ny)) != hipSuccess)
+    rc = fail(TJSQ_ERR_HIP, "tjsq_program_run_host: clearing y failed");
   if (rc == TJSQ_OK) rc = tjsq_program_run(P, dops, dx, dy, nullptr);
   if (rc == TJSQ_OK && ny > 0 && hipMemcpy(y, dy, sizeof(double) * ny, hipMemcpyDeviceToHost) != hipSuccess)
     rc = fail(TJSQ_ERR_HIP, "tjsq_program_run_host: copy of y to the host failed");

@@ -53,6 +53,24 @@ module tjsq_capi
       type(c_ptr), intent(out) :: prog
     end function
 
+    integer(c_int) function tjsq_dia_transport_create(ops, site, basis, trun, prog) &
+        bind(C, name="tjsq_dia_transport_create")
+      import :: c_int, c_ptr, tjsq_opset, tjsq_basis
+      type(tjsq_opset), intent(in) :: ops, site, trun
+      type(tjsq_basis), intent(in) :: basis
+      type(c_ptr), intent(out) :: prog
+    end function
+
+    integer(c_int) function tjsq_measure_create(side, ops, site, basis, nw, wsec, prog) &
+        bind(C, name="tjsq_measure_create")
+      import :: c_int, c_ptr, tjsq_opset, tjsq_basis
+      integer(c_int), value :: side, nw
+      type(tjsq_opset), intent(in) :: ops, site
+      type(tjsq_basis), intent(in) :: basis
+      integer(c_int), intent(in) :: wsec(*)
+      type(c_ptr), intent(out) :: prog
+    end function
+
     integer(c_int) function tjsq_block_update_create(lat, side, sys_len, env_len, pre, basis, trun, prog) &
         bind(C, name="tjsq_block_update_create")
       import :: c_int, c_ptr, tjsq_lattice, tjsq_opset, tjsq_basis

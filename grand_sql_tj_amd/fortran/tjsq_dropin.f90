@@ -43,6 +43,7 @@ module tjsq_dropin
   public :: gpu_lanczos_memo_new, gpu_model_wave, tjsq_dropin_precision, gpu_set_bond
   public :: gpu_Get_density_matrix_sys_new, gpu_Get_density_matrix_env_new, gpu_Get_truncation_operator
   public :: gpu_Get_Hamiltonian_Sys, gpu_Get_Hamiltonian_Env, gpu_wave_trans_LR, gpu_wave_trans_RL
+  public :: gpu_measure_operator_dia
 
   ! contraction precision of the drop-in solve (TJSQ_FP64 by default)
   integer(c_int), save :: tjsq_dropin_precision = TJSQ_FP64
@@ -629,4 +630,134 @@ contains
     type(Wavefunction), intent(inout) :: new_wave
     call move_wave(1, sys_bs, env_bs, sys_trun, env_trun, old_wave, new_wave)
   end subroutine gpu_wave_trans_RL
+  ! an identity truncation over the sectors of a fused basis (the reference
+  ! skips update_trun_dia at lengths up to trun_idx)
+  subroutine identity_flat(bs, f)
+    type(Total_Basis), intent(in) :: bs
+    type(flat_model), intent(out) :: f
+    integer :: k, r, n, nd
+    nd = 0
+    do k = 1, bs%num
+      nd = nd + bs%sub(k)%dim * bs%sub(k)%dim
+    end do
+    allocate(f%hdr(4, 1), f%sub(5, max(1, bs%num)), f%off(max(1, bs%num)), f%data(max(1, nd)))
+    f%hdr(:, 1) = [bs%len, 0, 0, bs%num]
+    f%data = 0.0d0
+    nd = 0
+    do k = 1, bs%num
+      n = bs%sub(k)%dim
+      f%sub(:, k) = [bs%sub(k)%new_num_up, bs%sub(k)%new_num_down, 0, n, n]
+      f%off(k) = nd
+      do r = 1, n
+        f%data(nd + (r - 1) * n + r) = 1.0d0
+      end do
+      nd = nd + n * n
+    end do
+  end subroutine identity_flat
+
+  ! <st_oper> on the sites of one side of the wavefunction: every site's
+  ! operator carried along the block (Get_sys_oper_dia / Get_env_oper_dia,
+  ! get_sys.f90:764-891), all of them in one device program per block
+  ! length, then contracted with the wavefunction (measure_sys_block /
+  ! _site, measure_env_block / _site, measure.f90:1241-1470).  v(i) is site
+  ! i counted from the side's outer end; v(length) the bond site.
+  subroutine side_measure(side, st, wave, trun_idx, v)
+    integer, intent(in) :: side, trun_idx
+    type(Total_Block), intent(in) :: st
+    type(Wavefunction), intent(in) :: wave
+    real(8), allocatable, intent(out) :: v(:)
+    type(flat_model), target :: cur, fs, ft
+    type(flat_basis), target :: fb
+    integer(c_int), allocatable :: hdr(:, :), sub(:, :), wsec(:, :)
+    integer(c_int64_t), allocatable :: off(:)
+    real(c_double), allocatable :: y(:), psi(:)
+    type(c_ptr) :: prog
+    integer :: length, L, nops, j, D
+    length = wave%sys_len
+    if (side == 1) length = wave%env_len
+    call flatten_block(st, fs)
+    call flatten_block(st, cur)
+    nops = 1
+    do L = 2, length - 1
+      if (side == 0) then
+        call flatten_basis(sys_bsm(L), fb)
+        if (L > trun_idx) then
+          call flatten_block(systruns(L), ft)
+        else
+          call identity_flat(sys_bsm(L), ft)
+        end if
+        call check(tjsq_dia_transport_create(opset_of(cur, L - 1, nops), opset_of(fs, 1, 1), &
+                                             basis_of(sys_bsm(L), fb), opset_of(ft, L, 1), prog), &
+                   'tjsq_dia_transport_create')
+      else
+        call flatten_basis(env_bsm(L), fb)
+        if (L > trun_idx) then
+          call flatten_block(envtruns(L), ft)
+        else
+          call identity_flat(env_bsm(L), ft)
+        end if
+        call check(tjsq_dia_transport_create(opset_of(cur, L - 1, nops), opset_of(fs, 1, 1), &
+                                             basis_of(env_bsm(L), fb), opset_of(ft, L, 1), prog), &
+                   'tjsq_dia_transport_create')
+      end if
+      call program_output(prog, ft%data, cur%data, hdr, sub, off, y)
+      call check(tjsq_program_destroy(prog), 'tjsq_program_destroy')
+      nops = nops + 1
+      call move_alloc(hdr, cur%hdr)
+      call move_alloc(sub, cur%sub)
+      call move_alloc(off, cur%off)
+      call move_alloc(y, cur%data)
+    end do
+    call wave_sectors(wave, wsec)
+    D = wave%dim
+    allocate(psi(max(1, D)))
+    call pack_wave(wave, psi)
+    if (side == 0) then
+      call flatten_basis(sys_bsm(length), fb)
+      call check(tjsq_measure_create(0_c_int, opset_of(cur, length - 1, nops), opset_of(fs, 1, 1), &
+                                     basis_of(sys_bsm(length), fb), int(wave%num, c_int), wsec, prog), &
+                 'tjsq_measure_create')
+    else
+      call flatten_basis(env_bsm(length), fb)
+      call check(tjsq_measure_create(1_c_int, opset_of(cur, length - 1, nops), opset_of(fs, 1, 1), &
+                                     basis_of(env_bsm(length), fb), int(wave%num, c_int), wsec, prog), &
+                 'tjsq_measure_create')
+    end if
+    call program_output(prog, cur%data, psi, hdr, sub, off, y)
+    call check(tjsq_program_destroy(prog), 'tjsq_program_destroy')
+    allocate(v(length))
+    do j = 1, length
+      v(j) = dot_product(y((j - 1) * D + 1:j * D), psi(1:D))
+    end do
+  end subroutine side_measure
+
+  ! measure_operator_dia (measure.f90:3-57): <st_oper> on every site of the
+  ! lattice in the wavefunction, appended to Filename as the reference
+  ! writes it
+  subroutine gpu_measure_operator_dia(ave_dia, st_oper, wave, trun_idx, Filename)
+    character(len=30) :: Filename
+    integer, intent(in) :: trun_idx
+    type(Total_Block), intent(in) :: st_oper
+    type(Wavefunction), intent(in) :: wave
+    real(8), intent(inout) :: ave_dia(Num_site)
+    real(8), allocatable :: v(:)
+    integer :: i
+    ave_dia = 0.0d0
+    call side_measure(0, st_oper, wave, trun_idx, v)
+    ave_dia(1:wave%sys_len) = v(1:wave%sys_len)
+    call side_measure(1, st_oper, wave, trun_idx, v)
+    do i = 1, wave%env_len
+      ave_dia(Num_site - i + 1) = v(i)
+    end do
+    open(10, file=Filename, position='append')
+    write(10, *) "Nx=", Nx, "Ny=", Ny, "N_up=", tot_num_up, "N_down=", tot_num_down, "M1=", kept_min, &
+      "M2=", kept_max
+    do i = 1, Num_site
+      write(10, 111) Lattice(1, i), Lattice(2, i), ave_dia(i)
+    end do
+    write(10, *)
+    close(10)
+111 format(I4,1X,I2,1X,F21.15)
+  end subroutine gpu_measure_operator_dia
+
 end module tjsq_dropin

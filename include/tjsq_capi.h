@@ -382,8 +382,9 @@ int tjsq_program_destroy(tjsq_program* prog);
  * drop-in of Get_density_matrix_*_new, Get_Hamiltonian_Sys/Env and
  * wave_trans_LR/RL (code/tjsq/den_mat.f90:84-134, get_sys.f90:4-478,
  * dmrg_sub.f90:1093-1384).  x holds what tjsq_program_run's x holds (for a
- * density program, `levels` states back to back).  Runs on the null stream;
- * returns when y is on the host. */
+ * density program, `levels` states back to back).  The device copy of y
+ * starts cleared (a measurement program's y needs it).  Runs on the null
+ * stream; returns when y is on the host. */
 int tjsq_program_run_host(tjsq_program* prog, const double* ops, const double* x, double* y);
 
 typedef struct tjsq_trunc_info {

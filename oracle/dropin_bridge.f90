@@ -229,3 +229,28 @@ integer(c_int) function dropin_wave_trans(dir, sys_nsec, sys_bsec, sys_bsub, env
   deallocate(sbs%sub, ebs%sub, ow%sub, nw%sub, zero)
   dropin_wave_trans = 0
 end function dropin_wave_trans
+
+! measure_operator_dia of num_elec through the drop-in, over the bases and
+! truncation operators stored by refb_set_bsm / refb_set_truns (as
+! refb_measure_density).
+integer(c_int) function dropin_measure_density(sys_len, env_len, nw, wsec, x, trun_idx, ave) &
+    bind(C, name="dropin_measure_density")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  use tjsq_dropin
+  implicit none
+  integer(c_int), value :: sys_len, env_len, nw, trun_idx
+  integer(c_int), intent(in) :: wsec(6, *)
+  real(c_double), intent(in) :: x(*)
+  real(c_double), intent(inout) :: ave(*)
+  type(Wavefunction) :: w
+  real(8) :: ave_dia(Num_site)
+  character(len=30) :: fname
+  fname = "num_elec.dat"
+  call put_wave(w, sys_len, env_len, nw, wsec, x)
+  call gpu_measure_operator_dia(ave_dia, num_elec, w, trun_idx, fname)
+  ave(1:Num_site) = ave_dia
+  call deallocate_wave(w)
+  dropin_measure_density = Num_site
+end function dropin_measure_density

@@ -329,6 +329,8 @@ def dropin_lib():
             common = [C.c_int, C.c_int] + margs + margs + [C.c_int, I32P, I32P, C.c_int, I32P, I32P, C.c_int, I32P]
             lib.dropin_model_wave.restype = C.c_int
             lib.dropin_model_wave.argtypes = common + [F64P, F64P]
+            lib.dropin_measure_density.restype = C.c_int
+            lib.dropin_measure_density.argtypes = [C.c_int, C.c_int, C.c_int, I32P, F64P, C.c_int, F64P]
             lib.dropin_lanczos.restype = C.c_int
             lib.dropin_lanczos.argtypes = common + [F64P, C.POINTER(C.c_double)]
             ref = ref_lib()
@@ -747,6 +749,30 @@ def ref_oper_cor_dia(model, workdir, sys_len, env_len, sys_bases, env_bases, sys
     full = cor[:nref * nref].reshape(nref, nref).T
     idx = list(range(sys_len)) + [nref - j for j in range(env_len, 0, -1)]
     return full[np.ix_(idx, idx)]
+
+
+def dropin_measure_density(model, workdir, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns, wsec, x,
+                           trun_idx=1):
+    """measure_operator_dia through the product's Fortran drop-in (as
+    ref_measure_density, whose globals the drop-in shares)."""
+    w = np.ascontiguousarray(wsec, np.int32).reshape(-1, 6)
+    with _lock, _cwd(workdir):
+        lib = _ref_init(model)
+        for side, bases, truns in ((0, sys_bases, sys_truns), (1, env_bases, env_truns)):
+            for L, b in bases.items():
+                lib.refb_set_bsm(side, L, b.length, *_bs(b))
+            for L, t in truns.items():
+                lib.refb_set_truns(side, L, *_one(t))
+        ave = np.zeros(4096)
+        with _quiet():
+            nref = dropin_lib().dropin_measure_density(sys_len, env_len, w.shape[0], w,
+                                                      np.ascontiguousarray(x, np.float64), trun_idx, ave)
+    n = sys_len + env_len
+    out = np.zeros(n)
+    out[:sys_len] = ave[:sys_len]
+    for i in range(1, env_len + 1):
+        out[n - i] = ave[nref - i]
+    return out
 
 
 def ref_quicksort(arr):

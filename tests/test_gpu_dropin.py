@@ -114,3 +114,19 @@ def test_dropin_wave_trans_matches_reference(direction, seed):
     r = oracle.ref_wave_trans(*wave_args(c))
     assert d.shape == r.shape and d.size > 0
     assert rel(d, r) < 1e-13
+
+
+@needs
+@pytest.mark.parametrize("sys_len,env_len", [(5, 5), (6, 8), (9, 7)])
+def test_dropin_measure_density_matches_reference(tmp_path, sys_len, env_len):
+    """measure_operator_dia through the drop-in (gpu_measure_operator_dia)
+    against the reference's own, on synthetic chains with random
+    truncations (trun_idx = 1: every truncation applies)."""
+    from test_measure_cpu import case
+    sb, eb, st, et, wsec, x, N = case(sys_len, env_len, 3 + sys_len)
+    args = (model(up_bias=1), tmp_path, sys_len, env_len, sb, eb, st, et, wsec, x)
+    got = oracle.dropin_measure_density(*args)
+    want = oracle.ref_measure_density(*args)
+    assert np.abs(got - want).max() < 1e-13
+    assert abs(got.sum() - N) < 1e-12 * N
+    assert os.path.exists(tmp_path / "num_elec.dat")
