@@ -166,3 +166,31 @@ def test_measure_from_checkpoint_files(tmp_path):
     assert np.abs(dens - d.measure_density()).max() < 1e-13
     assert np.abs(cor - d.measure_correlations()).max() < 1e-13
     assert abs(dens.sum() - c["N"]) < 1e-10 and abs(cor.sum() - c["N"] ** 2) < 1e-8
+
+
+def test_measure_run_on_reference_written_files(tmp_path):
+    """Interop end to end: the reference's own writers (truns_to_disk,
+    basis_to_disk, wave_to_disk) store a synthetic run's files; the device
+    measurement of that directory equals the reference's
+    measure_operator_dia of the same data."""
+    import oracle
+    from helpers import model
+    from grand_sql_tj_amd import checkpoint as ck, measure
+    if not oracle.ref_available():
+        pytest.skip("reference build not present")
+    sys_len, env_len = 6, 8
+    sb, eb, st, et, wsec, x, N = case(sys_len, env_len, 9)
+    run = tmp_path / "run"
+    run.mkdir()
+    for side, bases, truns in ((0, sb, st), (1, eb, et)):
+        for L, b in bases.items():
+            oracle.ref_basis_to_disk(run, b, L, side == 0)
+        for L, t in truns.items():
+            oracle.ref_truns_to_disk(run, t, int(np.asarray(t.sub)[:, 4].sum()), L, side == 0)
+    oracle.ref_wave_to_disk(run, ck.wave_file(1), sys_len, env_len, wsec, x)
+    dens, cor = measure.measure_run(run)
+    log = tmp_path / "log"
+    log.mkdir()
+    want = oracle.ref_measure_density(model(up_bias=1), log, sys_len, env_len, sb, eb, st, et, wsec, x)
+    assert np.abs(dens - want).max() < 1e-13
+    assert abs(cor.sum() - N * N) < 1e-10 * N * N
