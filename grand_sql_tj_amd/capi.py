@@ -6,6 +6,7 @@ solver.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 
@@ -88,7 +89,7 @@ EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_t
            "tjsq_truncation_select", "tjsq_identity_truncation", "tjsq_batch_create", "tjsq_batch_offsets",
            "tjsq_batch_stats", "tjsq_batch_graph", "tjsq_batch_apply", "tjsq_batch_apply_f32", "tjsq_batch_profile",
            "tjsq_batch_profile_read", "tjsq_batch_destroy", "tjsq_plan_create_ex", "tjsq_hostplan_create_ex",
-           "tjsq_plan_reshape", "tjsq_plan_graphs", "tjsq_last_error"]
+           "tjsq_plan_reshape", "tjsq_plan_graphs", "tjsq_shutdown", "tjsq_last_error"]
 
 _lib = None
 
@@ -157,6 +158,7 @@ def lib():
             "tjsq_hostplan_create_ex": [P, I, I, P, P, P, P, I, P, P, P],
             "tjsq_plan_reshape": [P, P, P],
             "tjsq_plan_graphs": [P, I],
+            "tjsq_shutdown": [],
         }
         for name in EXPORTS:
             f = getattr(L, name)
@@ -165,7 +167,28 @@ def lib():
                 f.argtypes = sig[name]
         del I64
         _lib = L
+        # release the library's device state while the GPU runtime is still
+        # up: atexit handlers run last-registered first, so this one runs
+        # before those of torch and the HIP runtime registered earlier
+        atexit.register(_shutdown)
     return _lib
+
+
+_shut = False
+
+
+def _shutdown():
+    """tjsq_shutdown at interpreter exit; afterwards the finalizers of
+    plans, programs and batches leave their handles alone (the process is
+    ending, and the GPU runtime may already be gone when they run)."""
+    global _shut
+    if _lib is not None and not _shut:
+        _shut = True
+        _lib.tjsq_shutdown()
+
+
+def _alive() -> bool:
+    return _lib is not None and not _shut
 
 
 def check(rc: int, what: str):
@@ -321,7 +344,7 @@ class HostPlan:
         self.stats = st.as_dict()
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
+        if getattr(self, "h", None) and _alive():
             _lib.tjsq_hostplan_destroy(self.h)
             self.h = None
 
@@ -419,7 +442,7 @@ class Plan:
         return ms, n
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
+        if getattr(self, "h", None) and _alive():
             _lib.tjsq_plan_destroy(self.h)
             self.h = None
 
@@ -592,7 +615,7 @@ class Program:
 
     def __del__(self):
         # at interpreter exit the module's globals may already be gone
-        if getattr(self, "_h", None) and _lib is not None:
+        if getattr(self, "_h", None) and _alive():
             _lib.tjsq_program_destroy(self._h)
             self._h = None
 
@@ -706,6 +729,6 @@ class Batch:
 
     def __del__(self):
         # at interpreter exit the module's globals may already be gone
-        if getattr(self, "_h", None) and _lib is not None:
+        if getattr(self, "_h", None) and _alive():
             _lib.tjsq_batch_destroy(self._h)
             self._h = None

@@ -1700,6 +1700,23 @@ int solver_state(SolverState** out) {
   return TJSQ_OK;
 }
 
+// releases one device's solver state; the caller has made that device current
+void solver_release(SolverState* S) {
+  (void)hipDeviceSynchronize();
+  for (int i = 0; i < NSOLVE; ++i) {
+    if (S->h[i]) (void)rocblas_destroy_handle(S->h[i]);
+    if (S->s[i]) (void)hipStreamDestroy(S->s[i]);
+  }
+  for (int i = 0; i <= NSOLVE; ++i)
+    if (S->ev[i]) (void)hipEventDestroy(S->ev[i]);
+  (void)hipFree(S->pad);
+  (void)hipFree(S->D);
+  (void)hipFree(S->E);
+  (void)hipFree(S->info);
+  (void)hipFree(S->jobs);
+  delete S;
+}
+
 // Padded size classes of the batched eigensolver.  A strided-batched
 // rocsolver_dsyevd call costs about a n + c n^2 b for b matrices of size n
 // (measured on MI355X: a = 0.0225 ms, c = 1.4e-6 ms), a fixed cost per size
@@ -2202,6 +2219,20 @@ int tjsq_truncation_operator(int nsec, const int* den_sub, const int64_t* den_of
   *trun_num = nt;
   if (spectrum) std::memcpy(spectrum, sel.sorted.data(), sizeof(double) * sel.sorted.size());
   fill_info(sel, info);
+  return TJSQ_OK;
+}
+
+int tjsq_shutdown(void) {
+  std::lock_guard<std::mutex> lk(g_solver_mu);
+  int cur = 0;
+  const bool have = hipGetDevice(&cur) == hipSuccess;
+  for (size_t d = 0; d < g_solver.size(); ++d) {
+    if (!g_solver[d]) continue;
+    if (hipSetDevice((int)d) == hipSuccess) solver_release(g_solver[d]);
+    g_solver[d] = nullptr;
+  }
+  g_solver.clear();
+  if (have) (void)hipSetDevice(cur);
   return TJSQ_OK;
 }
 

@@ -733,7 +733,10 @@ contains
 
   ! measure_operator_dia (measure.f90:3-57): <st_oper> on every site of the
   ! lattice in the wavefunction, appended to Filename as the reference
-  ! writes it
+  ! writes it.  Only trun_idx = 1 is taken, the value the reference's driver
+  ! passes (main.f90:80,250): with a larger trun_idx the reference still
+  ! projects with systruns(idx_two) at the last length (get_sys.f90:817-822),
+  ! a case this routine does not restate, so it stops instead.
   subroutine gpu_measure_operator_dia(ave_dia, st_oper, wave, trun_idx, Filename)
     character(len=30) :: Filename
     integer, intent(in) :: trun_idx
@@ -742,6 +745,10 @@ contains
     real(8), intent(inout) :: ave_dia(Num_site)
     real(8), allocatable :: v(:)
     integer :: i
+    if (trun_idx /= 1) then
+      write(*, *) 'tjsq: gpu_measure_operator_dia takes trun_idx = 1 only, got ', trun_idx
+      error stop 1
+    end if
     ave_dia = 0.0d0
     call side_measure(0, st_oper, wave, trun_idx, v)
     ave_dia(1:wave%sys_len) = v(1:wave%sys_len)

@@ -452,6 +452,14 @@ int tjsq_batch_profile(tjsq_batch* batch, int enable);
 int tjsq_batch_profile_read(tjsq_batch* batch, double* ms, int64_t* launches);
 int tjsq_batch_destroy(tjsq_batch* batch);
 
+/* Releases the library's per-device state that outlives single calls: the
+ * eigensolver's rocBLAS handles, streams, events and work buffers of
+ * tjsq_truncation_operator.  No counterpart in the reference, whose process
+ * exit frees everything; a host program calls it before its GPU runtime is
+ * torn down (the Python binding registers it at interpreter exit).  Later
+ * calls that need the state create it again. */
+int tjsq_shutdown(void);
+
 /* Last error message of the calling thread (static storage). */
 const char* tjsq_last_error(void);
 

@@ -137,6 +137,9 @@ def ref_lib():
             lib.refb_model_wave.argtypes = common + [F64P, F64P]
             lib.refb_lanczos.restype = C.c_int
             lib.refb_lanczos.argtypes = common + [F64P, C.POINTER(C.c_double)]
+            lib.refb_get_super_basis.restype = C.c_int
+            lib.refb_get_super_basis.argtypes = ([C.c_int, C.c_int, I32P, I32P] * 2
+                                                 + [C.c_int, C.c_int, I32P, C.POINTER(C.c_int), C.POINTER(C.c_int)])
             lib.refb_w6js.restype = C.c_double
             lib.refb_w6js.argtypes = [C.c_int] * 6
             lib.refb_get_basis.restype = C.c_int
@@ -252,9 +255,10 @@ def ref_model_wave(model, sb, x: np.ndarray) -> np.ndarray:
     with _lock:
         lib = _ref_init(model)
         y = np.zeros(sb.dim)
-        lib.refb_model_wave(sb.sys_len, sb.env_len, *_m(sb.sys), *_m(sb.env), *_bs(sb.sys_bs), *_bs(sb.env_bs),
-                            int(sb.wsec.shape[0]), np.ascontiguousarray(sb.wsec),
-                            np.ascontiguousarray(x, dtype=np.float64), y)
+        with _quiet():
+            lib.refb_model_wave(sb.sys_len, sb.env_len, *_m(sb.sys), *_m(sb.env), *_bs(sb.sys_bs), *_bs(sb.env_bs),
+                                int(sb.wsec.shape[0]), np.ascontiguousarray(sb.wsec),
+                                np.ascontiguousarray(x, dtype=np.float64), y)
     return y
 
 
@@ -295,9 +299,39 @@ def ref_get_basis(block_len, blk, site):
     sub = np.zeros(6 * cap, np.int32)
     ns, nq = C.c_int(0), C.c_int(0)
     with _lock:
-        ref_lib().refb_get_basis(block_len, blk.shape[0], blk, site.shape[0], site, sec, sub, C.byref(ns),
-                                 C.byref(nq))
+        lib = ref_lib()
+        with _quiet():
+            lib.refb_get_basis(block_len, blk.shape[0], blk, site.shape[0], site, sec, sub, C.byref(ns),
+                               C.byref(nq))
     return sec[:4 * ns.value].reshape(-1, 4), sub[:6 * nq.value].reshape(-1, 6)
+
+
+def oracle_get_super_basis(sys_bs, env_bs, total_up, total_down):
+    """The C restatement's Get_super_basis: the wavefunction sector table."""
+    ssec = np.ascontiguousarray(sys_bs.sec, dtype=np.int32)
+    esec = np.ascontiguousarray(env_bs.sec, dtype=np.int32)
+    w = np.zeros(6 * max(1, ssec.shape[0] * esec.shape[0]), np.int32)
+    n = oracle_lib().oracle_get_super_basis(ssec.shape[0], ssec, esec.shape[0], esec, int(total_up),
+                                            int(total_down), w)
+    return w[:6 * n].reshape(-1, 6)
+
+
+def ref_get_super_basis(sys_bs, env_bs, total_up, total_down):
+    """Get_super_basis of the reference build (block.f90:110-161): the
+    wavefunction sector table (sys_up, sys_down, env_up, env_down, sys_dim,
+    env_dim) in the routine's order, and the routine's total dimension."""
+    ssec = np.ascontiguousarray(sys_bs.sec, dtype=np.int32)
+    ssub = np.ascontiguousarray(sys_bs.sub, dtype=np.int32)
+    esec = np.ascontiguousarray(env_bs.sec, dtype=np.int32)
+    esub = np.ascontiguousarray(env_bs.sub, dtype=np.int32)
+    w = np.zeros(6 * max(1, ssec.shape[0] * esec.shape[0]), np.int32)
+    nw, dim = C.c_int(0), C.c_int(0)
+    with _lock:
+        lib = ref_lib()
+        with _quiet():
+            lib.refb_get_super_basis(sys_bs.length, ssec.shape[0], ssec, ssub, env_bs.length, esec.shape[0], esec,
+                                     esub, int(total_up), int(total_down), w, C.byref(nw), C.byref(dim))
+    return w[:6 * nw.value].reshape(-1, 6), dim.value
 
 
 def oracle_get_basis(block_len, blk, site):

@@ -424,6 +424,35 @@ integer(c_int) function refb_get_basis(blen, nb, blk, ns, site, sec, sub, nsec, 
   refb_get_basis = 0
 end function refb_get_basis
 
+! Get_super_basis of the reference (block.f90:110-161) over two fused bases
+! given as flat (sec, sub) tables; the sector list comes back as rows of
+! (sys_up, sys_down, env_up, env_down, sys_dim, env_dim) in the routine's order.
+integer(c_int) function refb_get_super_basis(slen, snsec, ssec, ssub, elen, ensec, esec, esub, total_up, &
+    total_down, wsec, nw, dim) bind(C, name="refb_get_super_basis")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: slen, snsec, elen, ensec, total_up, total_down
+  integer(c_int), intent(in) :: ssec(4, *), ssub(6, *), esec(4, *), esub(6, *)
+  integer(c_int), intent(inout) :: wsec(6, *)
+  integer(c_int), intent(out) :: nw, dim
+  type(Total_Basis) :: sbs, ebs
+  type(Super_Basis) :: super
+  integer :: i
+  call put_basis(sbs, slen, snsec, ssec, ssub)
+  call put_basis(ebs, elen, ensec, esec, esub)
+  call Get_super_basis(sbs, ebs, super, total_up, total_down)
+  nw = super%num
+  dim = super%dim
+  do i = 1, super%num
+    wsec(1, i) = super%sub(i)%sys_num_up; wsec(2, i) = super%sub(i)%sys_num_down
+    wsec(3, i) = super%sub(i)%env_num_up; wsec(4, i) = super%sub(i)%env_num_down
+    wsec(5, i) = super%sub(i)%sys_dim; wsec(6, i) = super%sub(i)%env_dim
+  end do
+  refb_get_super_basis = 0
+end function refb_get_super_basis
+
 ! Sizes of the operators held from the last producing call.
 integer(c_int) function refb_result_sizes(nops, nsub, ndata) bind(C, name="refb_result_sizes")
   use iso_c_binding

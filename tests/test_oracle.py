@@ -79,6 +79,34 @@ def test_get_basis_bit_exact_with_reference(ub):
         assert keys == sorted(keys)
 
 
+@needs_ref
+@pytest.mark.parametrize("ub", [0, 1])
+def test_get_super_basis_bit_exact_with_reference(ub):
+    # distinct sys and env bases, several particle numbers and total spins:
+    # the sector list and its order (sys sector outer, env inner) are the
+    # layout of every wavefunction, eigvec<k> checkpoint files included
+    from grand_sql_tj_amd import capi
+    rng = np.random.default_rng(31 + ub)
+    site = blocks.site_sectors(ub)
+    hits = 0
+    for _ in range(40):
+        sbs = blocks.get_basis(5, _random_blocks(rng, ub), site)
+        ebs = blocks.get_basis(4, _random_blocks(rng, ub), site)
+        ups = sorted({int(a) + int(b) for a in sbs.sec[:, 0] for b in ebs.sec[:, 0]}) if ub else [0]
+        for tot_up in ups[:4] + [ups[-1] + 2]:
+            for tot_down in (0, 1, 2):
+                rw, rdim = oracle.ref_get_super_basis(sbs, ebs, tot_up, tot_down)
+                ow = oracle.oracle_get_super_basis(sbs, ebs, tot_up, tot_down)
+                cw = capi.get_super_basis(sbs, ebs, tot_up, tot_down)
+                pw = blocks.get_super_basis(sbs, ebs, tot_up, tot_down)
+                assert np.array_equal(rw, ow) and np.array_equal(rw, cw) and np.array_equal(rw, pw)
+                assert rdim == int((rw[:, 4].astype(np.int64) * rw[:, 5]).sum())
+                for su_, sd, eu, ed, sdim, edim in rw:
+                    assert su_ + eu == tot_up and abs(sd - ed) <= tot_down <= sd + ed
+                hits += rw.shape[0]
+    assert hits > 100
+
+
 # ---- H_eff apply ------------------------------------------------------------
 
 @needs_ref
@@ -175,3 +203,21 @@ def test_lanczos_breakdown_exit():
     A = np.diag(np.arange(1.0, 6.0))
     e, psi, k, brk = olanczos.lanczos(lambda v: A @ v, np.eye(5)[2], 1e-10, 110)
     assert brk and k == 1 and e == pytest.approx(3.0)
+
+
+@needs_ref
+def test_cpu_baseline_leaves_standard_output_to_the_caller():
+    # the reference prints FACTRL's last factorial (block.f90:1514) and its
+    # Lanczos log lines; bench.py's standard output must stay one JSON line,
+    # so the baseline leg, run first in a fresh process, prints nothing there
+    import subprocess
+    import sys
+    from helpers import ROOT
+    code = ("import sys; sys.path.insert(0, 'tests'); import numpy as np, bench; from helpers import model, "
+            "small_superblock; m = model(); sb = small_superblock(m, 5, 5, seed=3); "
+            "r = bench.cpu_baseline(m, sb, np.ones(sb.dim), 4, budget_s=0.1); "
+            "sys.stderr.write('rate %g\\n' % r['value'])")
+    p = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert p.stdout == "", p.stdout
+    assert "rate" in p.stderr
