@@ -38,6 +38,10 @@ HBM_PEAK = 8.0e12
 KERNEL_KINDS = ("stage1_64", "stage1_32", "stage2_64", "stage2_32", "reduce")
 KERNEL_SYMBOL = {"stage1_64": "k_tiles<{mode}, 64, 1>", "stage1_32": "k_tiles<{mode}, 32, 1>",
                  "stage2_64": "k_tiles<{mode}, 64, 2>", "stage2_32": "k_tiles<{mode}, 32, 2>", "reduce": "k_reduce"}
+# the packed BF16x3 apply: its kernels in the same profiling slots (slot 1 =
+# the per-apply packing of x, slot 3 unused)
+PACKED_SYMBOL = {"stage1_64": "k_packed<1>", "stage1_32": "k_pack<float>", "stage2_64": "k_packed<2>",
+                 "stage2_32": "-", "reduce": "k_preduce"}
 MODE_ID = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "heff_apply_m8000_traffic.json")
 
@@ -635,7 +639,9 @@ def main():
     ms_k, n_k = plan.profile_read()
     plan.profile(False)
     kavg = {k: float(ms_k[i] / max(1, n_k[i])) for i, k in enumerate(KERNEL_KINDS)}   # ms per launch
-    dom = max(KERNEL_KINDS[:4], key=lambda k: kavg[k])
+    packed = args.precision == "bf16x3" and bool(st.get("packed"))
+    # (the packed apply's slot 1 is the packing of x, not a GEMM stage)
+    dom = max(("stage1_64", "stage2_64") if packed else KERNEL_KINDS[:4], key=lambda k: kavg[k])
 
     value = 1.0 / float(np.mean(latmp))
     v64 = 1.0 / float(np.mean(lat64))
@@ -664,6 +670,8 @@ def main():
         "p50_ms": 1e3 * float(np.percentile(latmp, 50)), "p99_ms": 1e3 * float(np.percentile(latmp, 99)),
         "p50_ms_fp64": 1e3 * float(np.percentile(lat64, 50)), "p99_ms_fp64": 1e3 * float(np.percentile(lat64, 99)),
         "apply_kernel_ms": kavg,
+        "apply_kernels": PACKED_SYMBOL if packed else {k: v.format(mode=MODE_ID[args.precision])
+                                                       for k, v in KERNEL_SYMBOL.items()},
         "apply_ms": float(sum(ms_k[:5]) / max(1, n_k[2])),
         "apply_tflops_reference_order": st["flops_ref"] / (float(sum(ms_k[:5]) / max(1, n_k[2])) * 1e-3) / 1e12,
     }
@@ -672,9 +680,11 @@ def main():
     # recomputation) over its average launch time (events on the launching
     # stream, isolated applies of the mixed precision)
     fl = kernel_flops(capi.HostPlan(model, sb))
+    if packed:   # one packed kernel per stage does both tile sizes' work
+        fl = {"stage1_64": fl["stage1_64"] + fl["stage1_32"], "stage2_64": fl["stage2_64"] + fl["stage2_32"]}
     achieved = fl[dom] / (kavg[dom] * 1e-3)
     peak = PEAK[args.precision]
-    kname = KERNEL_SYMBOL[dom].format(mode=MODE_ID[args.precision])
+    kname = (PACKED_SYMBOL if packed else KERNEL_SYMBOL)[dom].format(mode=MODE_ID[args.precision])
     line["roofline"] = {"bound": "mfma", "kernel": kname, "achieved": achieved / 1e12, "peak": peak / 1e12,
                         "unit": "TFLOP/s", "frac": achieved / peak,
                         "traffic": measured_traffic(kname, args.precision, sb.dim),

@@ -8,6 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SOURCES = [os.path.join(HERE, "csrc", "tjsq_plan.cpp"), os.path.join(HERE, "csrc", "tjsq_update.cpp"),
+           os.path.join(HERE, "csrc", "tjsq_packed.cpp"),
            os.path.join(HERE, "csrc", "tjsq_hip.hip")]
 OUT = os.path.join(HERE, "libtjsq_hip.so")
 

@@ -38,7 +38,8 @@ class PlanStats(C.Structure):
     _fields_ = [("dim", C.c_int64), ("n_contrib", C.c_int64), ("n_chain", C.c_int64), ("n_stage1", C.c_int64),
                 ("n_out_blocks", C.c_int64), ("n_tiles1", C.c_int64), ("n_tiles2", C.c_int64),
                 ("flops_ref", C.c_double), ("flops_exec", C.c_double), ("bytes_fp64", C.c_double),
-                ("op_elems", C.c_int64), ("scratch_elems", C.c_int64), ("n_slots", C.c_int64)]
+                ("op_elems", C.c_int64), ("scratch_elems", C.c_int64), ("n_slots", C.c_int64),
+                ("packed", C.c_int64), ("flops_packed", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -74,6 +75,17 @@ TILE_DTYPE = np.dtype([("out_off", "<i8"), ("out_ld", "<i4"), ("m0", "<i4"), ("n
                        ("group", "<i4")])
 MERGE_DTYPE = np.dtype([("dst_off", "<i8"), ("src_off", "<i8"), ("n", "<i8"), ("coef", "<f8")])
 assert SEG_DTYPE.itemsize == 40 and TILE_DTYPE.itemsize == 48 and MERGE_DTYPE.itemsize == 32
+# packed BF16x3 tables (tjsq_hostplan_packed)
+PACKJOB_DTYPE = np.dtype([("src_off", "<i8"), ("dst_off", "<i8"), ("src_ld", "<i4"), ("rows", "<i4"),
+                          ("cols", "<i4"), ("trans", "<i4"), ("lines", "<i4"), ("kpad", "<i4")])
+PSEG_DTYPE = np.dtype([("l_off", "<i8"), ("r_off", "<i8"), ("l_lim", "<u4"), ("r_lim", "<u4"),
+                       ("l_stride", "<i4"), ("r_stride", "<i4"), ("K", "<i4"), ("spaces", "<i4"), ("c", "<f8")])
+PTILE_DTYPE = np.dtype([("out_off", "<i8"), ("out_ld", "<i4"), ("m0", "<i4"), ("n0", "<i4"), ("mb", "<i4"),
+                        ("nb", "<i4"), ("mlim", "<i4"), ("nlim", "<i4"), ("seg_begin", "<i4"), ("seg_end", "<i4"),
+                        ("out_kind", "<i4"), ("slot", "<i4"), ("pad0", "<i4")])
+PREDUCE_DTYPE = np.dtype([("out_off", "<i8"), ("out_ld", "<i4"), ("m0", "<i4"), ("n0", "<i4"), ("mt", "<i4"),
+                          ("nt", "<i4"), ("s0", "<i4"), ("s1", "<i4"), ("pad0", "<i4")])
+assert (PACKJOB_DTYPE.itemsize, PSEG_DTYPE.itemsize, PTILE_DTYPE.itemsize, PREDUCE_DTYPE.itemsize) == (40, 48, 56, 40)
 
 EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_tables", "tjsq_active_sites",
            "tjsq_plan_create", "tjsq_plan_stats_get", "tjsq_plan_destroy", "tjsq_heff_apply", "tjsq_heff_apply_f32",
@@ -89,7 +101,8 @@ EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_t
            "tjsq_truncation_select", "tjsq_identity_truncation", "tjsq_batch_create", "tjsq_batch_offsets",
            "tjsq_batch_stats", "tjsq_batch_graph", "tjsq_batch_apply", "tjsq_batch_apply_f32", "tjsq_batch_profile",
            "tjsq_batch_profile_read", "tjsq_batch_destroy", "tjsq_plan_create_ex", "tjsq_hostplan_create_ex",
-           "tjsq_plan_reshape", "tjsq_plan_graphs", "tjsq_shutdown", "tjsq_last_error"]
+           "tjsq_plan_reshape", "tjsq_plan_graphs", "tjsq_shutdown", "tjsq_hostplan_packed",
+           "tjsq_last_error"]
 
 _lib = None
 
@@ -159,6 +172,7 @@ def lib():
             "tjsq_plan_reshape": [P, P, P],
             "tjsq_plan_graphs": [P, I],
             "tjsq_shutdown": [],
+            "tjsq_hostplan_packed": [P, P, P, P, P, P, P, P, P],
         }
         for name in EXPORTS:
             f = getattr(L, name)
@@ -343,8 +357,25 @@ class HostPlan:
         check(lib().tjsq_hostplan_stats(h, C.byref(st)), "tjsq_hostplan_stats")
         self.stats = st.as_dict()
 
+    def packed(self):
+        """The packed BF16x3 tables (dict of numpy arrays and byte sizes), or
+        None where that layout does not apply."""
+        sizes = np.zeros(11, np.int64)
+        if lib().tjsq_hostplan_packed(self.h, sizes.ctypes.data, *([None] * 7)) != TJSQ_OK:
+            return None
+        n = [int(v) for v in sizes]
+        out = {"ops_jobs": np.zeros(n[0], PACKJOB_DTYPE), "x_jobs": np.zeros(n[1], PACKJOB_DTYPE),
+               "seg1": np.zeros(n[2], PSEG_DTYPE), "tile1": np.zeros(n[3], PTILE_DTYPE),
+               "seg2": np.zeros(n[4], PSEG_DTYPE), "tile2": np.zeros(n[5], PTILE_DTYPE),
+               "rtile": np.zeros(n[6], PREDUCE_DTYPE)}
+        check(lib().tjsq_hostplan_packed(self.h, sizes.ctypes.data,
+                                         *[out[k].ctypes.data for k in ("ops_jobs", "x_jobs", "seg1", "tile1", "seg2",
+                                                                        "tile2", "rtile")]), "tjsq_hostplan_packed")
+        out.update(nslots=n[7], ops_bytes=n[8], x_bytes=n[9], t_bytes=n[10])
+        return out
+
     def __del__(self):
-        if getattr(self, "h", None) and _alive():
+        if getattr(self, "h", None) and _alive and _alive():
             _lib.tjsq_hostplan_destroy(self.h)
             self.h = None
 
@@ -442,7 +473,7 @@ class Plan:
         return ms, n
 
     def __del__(self):
-        if getattr(self, "h", None) and _alive():
+        if getattr(self, "h", None) and _alive and _alive():
             _lib.tjsq_plan_destroy(self.h)
             self.h = None
 
@@ -615,7 +646,7 @@ class Program:
 
     def __del__(self):
         # at interpreter exit the module's globals may already be gone
-        if getattr(self, "_h", None) and _alive():
+        if getattr(self, "_h", None) and _alive and _alive():
             _lib.tjsq_program_destroy(self._h)
             self._h = None
 
@@ -729,6 +760,6 @@ class Batch:
 
     def __del__(self):
         # at interpreter exit the module's globals may already be gone
-        if getattr(self, "_h", None) and _alive():
+        if getattr(self, "_h", None) and _alive and _alive():
             _lib.tjsq_batch_destroy(self._h)
             self._h = None

@@ -26,6 +26,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <system_error>
@@ -35,6 +36,7 @@
 #include "tjsq_capi.h"
 #include "tjsq_plan.hpp"
 #include "tjsq_update.hpp"
+#include "tjsq_packed.hpp"
 
 using tjsq::HostPlan;
 using tjsq::Seg;
@@ -517,6 +519,279 @@ __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ rt, con
 }
 
 // ---------------------------------------------------------------------------
+// Packed BF16x3 apply (tables of tjsq_packed.cpp)
+//
+// Operands are lines of k in groups of eight: 16 bytes of bf16 high halves,
+// then 16 bytes of low halves.  One 64-lane wave owns one output tile of up
+// to 4 x 4 blocks of 16 x 16 and loads each MFMA operand straight into its
+// registers: lane (i = lane % 16, q = lane / 16) reads line i of a block,
+// group q of the 32-wide K chunk, as two aligned 16-byte buffer loads.
+// Everything is zero-padded to multiples of 16, so the loop has no bounds
+// tests; a K chunk that is only 16 long (the padded K is a multiple of 16)
+// zeroes the left operand of lanes q >= 2 (the right one reads finite data:
+// the next line or zeros past the buffer end).  The next chunk's operands,
+// also across segments, are loaded while the current chunk's MFMAs run.
+// ---------------------------------------------------------------------------
+// the vectors of one packed launch: x itself (axpy segments) and y; the
+// operand copies and the stage-1 products are addressed by the tables
+struct PSpaces {
+  const float* xf;
+  float* y;
+};
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc_n(const void* base, uint32_t nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)nbytes, 0x00020000);
+}
+
+// one packed copy per workgroup (grid-stride over jobs); a thread packs one
+// group of eight k of one line (device jobs hold absolute destinations)
+template <typename S>
+__global__ __launch_bounds__(256) void k_pack(const tjsq::PackJob* __restrict__ jobs, int njobs, const S* __restrict__ src) {
+  for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
+    const tjsq::PackJob J = jobs[jb];
+    const int groups = J.kpad / 8;
+    const int n = J.lines * groups;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+      const int l = e / groups, g = e % groups;
+      u4 hi, lo;
+      unsigned short h[8], w[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int k = 8 * g + t;
+        const int r = J.trans ? l : k, c = J.trans ? k : l;
+        double v = 0.0;
+        if (r < J.rows && c < J.cols) v = (double)src[J.src_off + r + (int64_t)c * J.src_ld];
+        // FP64 operators keep their FP64 residual in the low half (as the
+        // unpacked operand words do); FP32 x splits its own value
+        h[t] = bf16_rne((float)v);
+        w[t] = bf16_rne((float)(v - (double)bf16_to_f(h[t])));
+      }
+      hi.x = h[0] | ((uint32_t)h[1] << 16); hi.y = h[2] | ((uint32_t)h[3] << 16);
+      hi.z = h[4] | ((uint32_t)h[5] << 16); hi.w = h[6] | ((uint32_t)h[7] << 16);
+      lo.x = w[0] | ((uint32_t)w[1] << 16); lo.y = w[2] | ((uint32_t)w[3] << 16);
+      lo.z = w[4] | ((uint32_t)w[5] << 16); lo.w = w[6] | ((uint32_t)w[7] << 16);
+      u4* d = (u4*)((uint8_t*)J.dst_off + (int64_t)l * J.kpad * 4 + (int64_t)g * 32);   // absolute address
+      d[0] = hi;
+      d[1] = lo;
+    }
+  }
+}
+
+// the operand lines of one segment as a lane reads them: the buffer from
+// the segment's copy to the end of its space, and per 16-line block the
+// lane's byte offset (line origin + 16 b + lane % 16, group lane / 16), or
+// an offset past the record count (reads zero, no memory access) for blocks
+// outside the tile
+struct PSide {
+  __amdgpu_buffer_rsrc_t r;
+  uint32_t v[4];
+};
+
+__device__ __forceinline__ PSide packed_side(int64_t addr, uint32_t lim, int stride, int origin, int nblk, int li,
+                                             int lq) {
+  PSide s;
+  s.r = make_rsrc_n((const void*)addr, lim);
+  const uint32_t v0 = (uint32_t)(origin + li) * (uint32_t)stride + (uint32_t)lq * 32u;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) s.v[b] = b < nblk ? v0 + (uint32_t)(16 * b) * (uint32_t)stride : OOB;
+  return s;
+}
+
+// high and low halves of the lane's eight k of block b, ko bytes into the
+// lines (all offsets in the lane's VGPR, where the range check sees them)
+__device__ __forceinline__ void ld_op(const PSide& s, int b, uint32_t ko, u4& hi, u4& lo) {
+  hi = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(s.r, (int)(s.v[b] + ko), 0, 0));
+  lo = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(s.r, (int)(s.v[b] + ko + 16u), 0, 0));
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(64, 2) void k_packed(const tjsq::PTile* __restrict__ tiles, const tjsq::PSeg* __restrict__ segs,
+                                                 PSpaces S, float* __restrict__ partial, int ntiles) {
+  const int bid = blockIdx.x;
+  if (bid >= ntiles) return;
+  const tjsq::PTile tl = tiles[bid];
+  const int lane = threadIdx.x, li = lane & 15, lq = lane >> 4;
+  const int mb = tl.mb, nb = tl.nb;
+  f4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  // the lane's rows (4 consecutive, + 16 bi) and column (+ 16 bj) in the tile
+  const int row0 = tl.m0 + 4 * lq, col0 = tl.n0 + li;
+  int si = tl.seg_begin;
+  const int se = tl.seg_end;
+  for (; si < se; ++si) {   // axpy segments lead the list; they add x itself
+    const tjsq::PSeg s = segs[si];
+    if ((s.spaces & 0xff) != tjsq::PK_XF) break;
+    const float c = (float)s.c;
+    const float* xb = S.xf + s.l_off;
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj) {
+        if (bi >= mb || bj >= nb) continue;
+        const int col = col0 + 16 * bj;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = row0 + 16 * bi + r;
+          if (row < tl.mlim && col < tl.nlim) acc[bi][bj][r] += c * xb[row + (int64_t)col * s.l_stride];
+        }
+      }
+  }
+  double scale = 1.0;
+  if (si < se) {
+    tjsq::PSeg cur = segs[si];
+    tjsq::PSeg nxt = segs[min(si + 1, se - 1)];
+    PSide L = packed_side(cur.l_off, cur.l_lim, cur.l_stride, tl.m0, mb, li, lq);
+    PSide R = packed_side(cur.r_off, cur.r_lim, cur.r_stride, tl.n0, nb, li, lq);
+    int k0 = 0;
+    // two register sets, A and B: one chunk is multiplied while the next
+    // one loads into the other set
+    u4 aAh[4], aAl[4], bAh[4], bAl[4], aBh[4], aBl[4], bBh[4], bBl[4];
+    auto issue = [&](uint32_t ko, u4 (&ah)[4], u4 (&al)[4], u4 (&bh)[4], u4 (&bl)[4]) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        ld_op(L, b, ko, ah[b], al[b]);
+        ld_op(R, b, ko, bh[b], bl[b]);
+      }
+    };
+    // moves the sides past the chunk (cur, k0): false after the last one,
+    // when the sides read nothing (every block out of range)
+    auto advance = [&](uint32_t& ko) -> bool {
+      int kn = k0 + 32;
+      bool more = true;
+      if (kn >= cur.K) {
+        ++si;
+        kn = 0;
+        more = si < se;
+        cur = nxt;
+        nxt = segs[min(si + 1, se - 1)];
+        L = packed_side(cur.l_off, cur.l_lim, cur.l_stride, tl.m0, more ? mb : 0, li, lq);
+        R = packed_side(cur.r_off, cur.r_lim, cur.r_stride, tl.n0, more ? nb : 0, li, lq);
+      }
+      k0 = kn;
+      ko = (uint32_t)kn * 4u;
+      return more;
+    };
+    auto compute = [&](bool first, bool half, double cseg, u4 (&ah)[4], u4 (&al)[4], const u4 (&bh)[4],
+                       const u4 (&bl)[4]) {
+      if (first) {
+        // the running sum is kept as scale * acc (operand words cannot be
+        // scaled): acc *= scale / c, scale = c; coefficients are never 0
+        const float rs = (float)(scale / cseg);
+        scale = cseg;
+        if (rs != 1.0f) {
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] *= rs;
+        }
+      }
+      if (half && lq >= 2) {   // a 16-long chunk: lanes supplying k 16..31 add nothing
+#pragma unroll
+        for (int b = 0; b < 4; ++b) { ah[b] = u4{0u, 0u, 0u, 0u}; al[b] = u4{0u, 0u, 0u, 0u}; }
+      }
+#pragma unroll
+      for (int bi = 0; bi < 4; ++bi) {
+        if (bi >= mb) continue;
+        const bf8 xh = __builtin_bit_cast(bf8, ah[bi]), xl = __builtin_bit_cast(bf8, al[bi]);
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj) {
+          if (bj >= nb) continue;
+          const bf8 yh = __builtin_bit_cast(bf8, bh[bj]), yl = __builtin_bit_cast(bf8, bl[bj]);
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, yh, acc[bi][bj], 0, 0, 0);
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, yl, acc[bi][bj], 0, 0, 0);
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, yh, acc[bi][bj], 0, 0, 0);
+        }
+      }
+    };
+    issue(0u, aAh, aAl, bAh, bAl);
+    for (;;) {
+      bool first = k0 == 0, half = cur.K - k0 == 16;
+      double cseg = cur.c;
+      uint32_t ko;
+      bool more = advance(ko);
+      issue(ko, aBh, aBl, bBh, bBl);
+      compute(first, half, cseg, aAh, aAl, bAh, bAl);
+      if (!more) break;
+      first = k0 == 0;
+      half = cur.K - k0 == 16;
+      cseg = cur.c;
+      more = advance(ko);
+      issue(ko, aAh, aAl, bAh, bAl);
+      compute(first, half, cseg, aBh, aBl, bBh, bBl);
+      if (!more) break;
+    }
+    const float rs = (float)scale;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] *= rs;
+  }
+  if (tl.out_kind == tjsq::PK_OUT_T) {
+    // lane (li, lq) holds D[u][v] = T[v][u] for u = m0 + 16 bi + 4 lq + r,
+    // v = n0 + 16 bj + li: four consecutive k of line v of T, in group
+    // m0 / 8 + 2 bi + lq / 2, half lq % 2 of its eight
+    uint8_t* tb = (uint8_t*)tl.out_off + (int64_t)col0 * tl.out_ld + (tl.m0 / 8 + lq / 2) * 32 + (lq % 2) * 8;
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj) {
+        if (bi >= mb || bj >= nb) continue;
+        unsigned short h[4], w[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          h[r] = bf16_rne(acc[bi][bj][r]);
+          w[r] = bf16_rne(acc[bi][bj][r] - bf16_to_f(h[r]));
+        }
+        uint8_t* d = tb + (int64_t)(16 * bj) * tl.out_ld + 64 * bi;
+        *(uint2*)d = uint2{h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16)};
+        *(uint2*)(d + 16) = uint2{w[0] | ((uint32_t)w[1] << 16), w[2] | ((uint32_t)w[3] << 16)};
+      }
+    return;
+  }
+  if (tl.slot >= 0) {   // partial tile: 64 x 64, column-major, tile coordinates
+    float* pb = partial + (int64_t)tl.slot * 4096 + 4 * lq + li * 64;
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj) {
+        if (bi >= mb || bj >= nb) continue;
+        *(f4*)(pb + 16 * bi + 16 * 64 * bj) = acc[bi][bj];
+      }
+    return;
+  }
+  float* yb = S.y + tl.out_off + (int64_t)col0 * tl.out_ld + row0;
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) {
+      if (bi >= mb || bj >= nb) continue;
+      if (col0 + 16 * bj >= tl.nlim) continue;
+      float* d = yb + (int64_t)(16 * bj) * tl.out_ld + 16 * bi;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (row0 + 16 * bi + r < tl.mlim) d[r] = acc[bi][bj][r];
+    }
+}
+
+// ordered sum of the partial slots of split packed tiles
+__global__ __launch_bounds__(256) void k_preduce(const tjsq::PReduce* __restrict__ rt, const float* __restrict__ partial,
+                                                 float* __restrict__ y, int n) {
+  const int t = blockIdx.x;
+  if (t >= n) return;
+  const tjsq::PReduce R = rt[t];
+  for (int e = threadIdx.x; e < R.mt * R.nt; e += blockDim.x) {
+    const int row = e % R.mt, col = e / R.mt;
+    const float* p = partial + (int64_t)R.s0 * 4096 + row + col * 64;
+    float acc = 0.f;
+    for (int s = 0; s < R.s1 - R.s0; ++s) acc += p[(int64_t)s * 4096];
+    y[R.out_off + (R.m0 + row) + (int64_t)(R.n0 + col) * R.out_ld] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // merged operators: dst[e] = sum_t coef_t * src_t[e] in recipe order
 // ---------------------------------------------------------------------------
 struct MergeDst { int64_t dst_off, n; int t_begin, t_end; };
@@ -687,9 +962,30 @@ struct tjsq_plan {
   // per-launch HIP events of the two apply stages while profiling is on
   bool profiling = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof;
+  // packed BF16x3 apply (tjsq_packed.hpp); used for BF16x3 when `packed`
+  bool packed = false;
+  tjsq::PackedPlan pk;           // device form of the tables (absolute addresses)
+  tjsq::PackedPlan pk_host;      // host form (offsets into the plan's spaces)
+  std::vector<void*> pk_owned;   // further buffers the tables address (batch copies)
+  uint8_t* pk_ops = nullptr;     // packed operator copies
+  uint8_t* pk_x = nullptr;       // packed copies of an apply's x
+  uint8_t* pk_t = nullptr;       // stage-1 products
+  tjsq::PackJob* pk_xjobs = nullptr;
+  tjsq::PSeg* pk_seg1 = nullptr;
+  tjsq::PSeg* pk_seg2 = nullptr;
+  tjsq::PTile* pk_tile1 = nullptr;
+  tjsq::PTile* pk_tile2 = nullptr;
+  tjsq::PReduce* pk_rtile = nullptr;
+  float* pk_part = nullptr;
 };
 
 namespace {
+
+// a switch read from the environment (set and not "0")
+bool getenv_flag(const char* name) {
+  const char* v = std::getenv(name);
+  return v && *v && std::strcmp(v, "0") != 0;
+}
 
 int grid_for(int64_t n, int threads = 256) {
   int64_t g = (n + threads - 1) / threads;
@@ -817,6 +1113,162 @@ int launch_apply(const tjsq_plan* p, const Spaces<MODE>& S, typename KT<MODE>::T
   return TJSQ_OK;
 }
 
+// event-timed launch of one kernel kind while profiling is on
+template <typename F>
+int timed_launch(const tjsq_plan* p, int kind, hipStream_t st, F&& fn) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (p->profiling) {
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, st));
+  }
+  fn();
+  HIPCHK(hipGetLastError());
+  if (p->profiling) {
+    HIPCHK(hipEventRecord(e1, st));
+    const_cast<tjsq_plan*>(p)->prof.push_back({kind, {e0, e1}});
+  }
+  return TJSQ_OK;
+}
+
+PSpaces packed_spaces(const float* x, float* y) { return PSpaces{x, y}; }
+
+// packed BF16x3 apply: pack x, stage 1, stage 2, ordered partial sums
+// (profiling kinds: 1 the x pack, 0 stage 1, 2 stage 2, 4 the reduction);
+// p->pk holds the device form of the tables (counts only are read here)
+int packed_apply(const tjsq_plan* p, const float* x, float* y, hipStream_t st) {
+  const tjsq::PackedPlan& K = p->pk;
+  const PSpaces S = packed_spaces(x, y);
+  const int nj = (int)K.x_jobs.size();
+  int rc = TJSQ_OK;
+  if (nj)
+    rc = timed_launch(p, 1, st, [&] {
+      hipLaunchKernelGGL(k_pack<float>, dim3(std::min(nj, 8192)), dim3(256), 0, st, p->pk_xjobs, nj, x);
+    });
+  const int n1 = (int)K.tile1.size(), n2 = (int)K.tile2.size(), nr = (int)K.rtile.size();
+  if (!rc && n1)
+    rc = timed_launch(p, 0, st, [&] {
+      hipLaunchKernelGGL(k_packed<1>, dim3(n1), dim3(64), 0, st, p->pk_tile1, p->pk_seg1, S, p->pk_part, n1);
+    });
+  if (!rc && n2)
+    rc = timed_launch(p, 2, st, [&] {
+      hipLaunchKernelGGL(k_packed<2>, dim3(n2), dim3(64), 0, st, p->pk_tile2, p->pk_seg2, S, p->pk_part, n2);
+    });
+  if (!rc && nr)
+    rc = timed_launch(p, 4, st, [&] {
+      hipLaunchKernelGGL(k_preduce, dim3(nr), dim3(256), 0, st, p->pk_rtile, p->pk_part, y, nr);
+    });
+  return rc;
+}
+
+// Device form of one plan's packed tables, appended to D: absolute addresses
+// of its operand copies (ops, x) and stage-1 products (t), element offsets
+// into x / y shifted by xy, segment indices and partial slots shifted past
+// what D already holds.  A plan's own tables use xy = 0; a batch appends
+// every member.
+void pk_append_device(const tjsq::PackedPlan& K, uint8_t* ops, uint8_t* x, uint8_t* t, int64_t xy,
+                      tjsq::PackedPlan& D) {
+  const int sb1 = (int)D.seg1.size(), sb2 = (int)D.seg2.size();
+  const int64_t slot0 = D.nslots;
+  auto fix = [&](tjsq::PSeg g) {
+    if (g.l_space() == tjsq::PK_XF) {
+      g.l_off += xy;
+      return g;
+    }
+    auto one = [&](int space, int64_t& off, uint32_t& lim) {
+      uint8_t* b = space == tjsq::PK_OPS ? ops : space == tjsq::PK_X ? x : t;
+      const int64_t n = space == tjsq::PK_OPS ? K.ops_bytes : space == tjsq::PK_X ? K.x_bytes : K.t_bytes;
+      lim = (uint32_t)(n - off);
+      off = (int64_t)(uintptr_t)(b + off);
+    };
+    one(g.l_space(), g.l_off, g.l_lim);
+    one(g.r_space(), g.r_off, g.r_lim);
+    return g;
+  };
+  for (const tjsq::PSeg& g : K.seg1) D.seg1.push_back(fix(g));
+  for (const tjsq::PSeg& g : K.seg2) D.seg2.push_back(fix(g));
+  for (tjsq::PTile tl : K.tile1) {
+    tl.seg_begin += sb1; tl.seg_end += sb1;
+    tl.out_off = (int64_t)(uintptr_t)(t + tl.out_off);
+    D.tile1.push_back(tl);
+  }
+  for (tjsq::PTile tl : K.tile2) {
+    tl.seg_begin += sb2; tl.seg_end += sb2;
+    tl.out_off += xy;
+    if (tl.slot >= 0) tl.slot += (int32_t)slot0;
+    D.tile2.push_back(tl);
+  }
+  for (tjsq::PReduce r : K.rtile) {
+    r.out_off += xy;
+    r.s0 += (int32_t)slot0; r.s1 += (int32_t)slot0;
+    D.rtile.push_back(r);
+  }
+  for (tjsq::PackJob j : K.x_jobs) {
+    j.src_off += xy;
+    j.dst_off = (int64_t)(uintptr_t)(x + j.dst_off);
+    D.x_jobs.push_back(j);
+  }
+  D.nslots += K.nslots;
+  D.flops_exec += K.flops_exec;
+}
+
+// device tables of the packed form D (and its partial slots) into p
+int pk_upload(tjsq_plan* p, const tjsq::PackedPlan& D, hipStream_t st) {
+  auto put = [&](auto** dptr, const auto& vec) -> bool {
+    using E = typename std::remove_reference<decltype(vec)>::type::value_type;
+    if (hipMalloc((void**)dptr, sizeof(E) * std::max<size_t>(1, vec.size())) != hipSuccess) return false;
+    return vec.empty() ||
+           hipMemcpyAsync(*dptr, vec.data(), sizeof(E) * vec.size(), hipMemcpyHostToDevice, st) == hipSuccess;
+  };
+  if (hipMalloc(&p->pk_part, sizeof(float) * 4096 * std::max<int64_t>(1, D.nslots)) != hipSuccess ||
+      !put(&p->pk_xjobs, D.x_jobs) || !put(&p->pk_seg1, D.seg1) || !put(&p->pk_seg2, D.seg2) ||
+      !put(&p->pk_tile1, D.tile1) || !put(&p->pk_tile2, D.tile2) || !put(&p->pk_rtile, D.rtile))
+    return fail(TJSQ_ERR_NOMEM, "packed BF16x3 table allocation failed");
+  HIPCHK(hipStreamSynchronize(st));   // the host vectors may go after this
+  return TJSQ_OK;
+}
+
+// the packed BF16x3 tables and operator copies of a plan; ok = false (and
+// the plan keeps the unpacked BF16x3 operands) where the layout does not
+// apply or a space would exceed the 2 GiB a buffer resource addresses
+int setup_packed(tjsq_plan* p, hipStream_t st, bool& ok) {
+  ok = false;
+  std::string msg;
+  tjsq::PackedPlan K;
+  if (tjsq::compile_packed(p->hp, tjsq::PackShape(), K, msg) != TJSQ_OK) return TJSQ_OK;
+  const int64_t lim = (int64_t)0x7FFFFFFF - 4096;
+  if (K.ops_bytes > lim || K.x_bytes > lim || K.t_bytes > lim) return TJSQ_OK;
+  if (hipMalloc(&p->pk_ops, std::max<int64_t>(16, K.ops_bytes)) != hipSuccess ||
+      hipMalloc(&p->pk_x, std::max<int64_t>(16, K.x_bytes)) != hipSuccess ||
+      hipMalloc(&p->pk_t, std::max<int64_t>(16, K.t_bytes)) != hipSuccess)
+    return fail(TJSQ_ERR_NOMEM, "packed BF16x3 allocation failed");
+  // the operator copies, packed once from the FP64 arena
+  std::vector<tjsq::PackJob> oj = K.ops_jobs;
+  for (tjsq::PackJob& j : oj) j.dst_off = (int64_t)(uintptr_t)(p->pk_ops + j.dst_off);
+  tjsq::PackJob* ojobs = nullptr;
+  if (!oj.empty()) {
+    if (hipMalloc(&ojobs, sizeof(tjsq::PackJob) * oj.size()) != hipSuccess)
+      return fail(TJSQ_ERR_NOMEM, "packed BF16x3 allocation failed");
+    HIPCHK(hipMemcpyAsync(ojobs, oj.data(), sizeof(tjsq::PackJob) * oj.size(), hipMemcpyHostToDevice, st));
+    const int nj = (int)oj.size();
+    hipLaunchKernelGGL(k_pack<double>, dim3(std::min(nj, 8192)), dim3(256), 0, st, ojobs, nj, p->arena);
+  }
+  const hipError_t e = hipGetLastError();
+  const hipError_t e2 = hipStreamSynchronize(st);
+  (void)hipFree(ojobs);
+  if (e != hipSuccess || e2 != hipSuccess) return fail(TJSQ_ERR_HIP, "packing the BF16x3 operators failed");
+  tjsq::PackedPlan D;
+  D.ops_bytes = K.ops_bytes; D.x_bytes = K.x_bytes; D.t_bytes = K.t_bytes;
+  pk_append_device(K, p->pk_ops, p->pk_x, p->pk_t, 0, D);
+  D.ops_jobs = K.ops_jobs;   // (host form, kept for batches that copy this plan's operators)
+  int rc = pk_upload(p, D, st);
+  if (rc) return rc;
+  p->pk_host = std::move(K);
+  p->pk = std::move(D);
+  ok = true;
+  return TJSQ_OK;
+}
+
 // apply of plan p (or a batch's merged plan) in one precision: FP64 on
 // double vectors, FP32 / BF16x3 on float vectors
 int apply_spaces(const tjsq_plan* p, int precision, const void* x, void* y, hipStream_t st) {
@@ -827,6 +1279,7 @@ int apply_spaces(const tjsq_plan* p, int precision, const void* x, void* y, hipS
   const float* xf = (const float*)x;
   if (precision == TJSQ_FP32)
     return launch_apply<MODE_F32>(p, Spaces<MODE_F32>{p->arena32, xf, xf, p->t32, (float*)y}, p->part32, st);
+  if (p->packed) return packed_apply(p, xf, (float*)y, st);
   const int64_t n = p->hp.dim;
   hipLaunchKernelGGL(k_f32_to_split, dim3(grid_for(n)), dim3(256), 0, st, xf, p->xw, n);
   HIPCHK(hipGetLastError());
@@ -1045,12 +1498,22 @@ int tjsq_plan_create_ex(const tjsq_lattice* lat, int sys_len, int env_len, const
     HIPCHK(hipGetLastError());
   }
   if (precision_mask & (1 << TJSQ_BF16X3)) {
-    if (hipMalloc(&p->arenab, sizeof(uint32_t) * std::max<int64_t>(1, hp.arena_elems)) != hipSuccess ||
-        hipMalloc(&p->xw, sizeof(uint32_t) * std::max<int64_t>(1, hp.dim)) != hipSuccess)
-      return cleanup(TJSQ_ERR_NOMEM, "BF16x3 operand allocation failed");
-    hipLaunchKernelGGL(k_f64_to_split, dim3(grid_for(hp.arena_elems)), dim3(256), 0, st, p->arena, p->arenab,
-                       hp.arena_elems);
-    HIPCHK(hipGetLastError());
+    bool ok = false;
+    if (!getenv_flag("TJSQ_UNPACKED_BF16X3") && (rc = setup_packed(p, st, ok))) {
+      tjsq_plan_destroy(p);
+      return rc;
+    }
+    p->packed = ok;
+    hp.stats.packed = ok ? 1 : 0;
+    hp.stats.flops_packed = ok ? p->pk.flops_exec : 0.0;
+    if (!ok) {
+      if (hipMalloc(&p->arenab, sizeof(uint32_t) * std::max<int64_t>(1, hp.arena_elems)) != hipSuccess ||
+          hipMalloc(&p->xw, sizeof(uint32_t) * std::max<int64_t>(1, hp.dim)) != hipSuccess)
+        return cleanup(TJSQ_ERR_NOMEM, "BF16x3 operand allocation failed");
+      hipLaunchKernelGGL(k_f64_to_split, dim3(grid_for(hp.arena_elems)), dim3(256), 0, st, p->arena, p->arenab,
+                         hp.arena_elems);
+      HIPCHK(hipGetLastError());
+    }
   }
   if ((rc = upload_tables(p, (precision_mask & 1) != 0, p->arena32 || p->arenab, st))) {
     tjsq_plan_destroy(p);
@@ -1063,6 +1526,9 @@ int tjsq_plan_create_ex(const tjsq_lattice* lat, int sys_len, int env_len, const
 
 struct tjsq_hostplan {
   HostPlan hp;
+  bool have_pk = false;
+  int pk_rc = TJSQ_OK;
+  tjsq::PackedPlan pk;
 };
 
 int tjsq_hostplan_create(const tjsq_lattice* lat, int sys_len, int env_len, const tjsq_opset* sys,
@@ -1124,6 +1590,32 @@ int tjsq_hostplan_destroy(tjsq_hostplan* h) {
   return TJSQ_OK;
 }
 
+int tjsq_hostplan_packed(const tjsq_hostplan* hc, int64_t* sizes, void* ops_jobs, void* x_jobs, void* seg1,
+                         void* tile1, void* seg2, void* tile2, void* rtile) {
+  if (!hc || !sizes) return fail(TJSQ_ERR_ARG, "tjsq_hostplan_packed: null argument");
+  tjsq_hostplan* h = const_cast<tjsq_hostplan*>(hc);
+  if (!h->have_pk) {
+    std::string msg;
+    h->pk_rc = tjsq::compile_packed(h->hp, tjsq::PackShape(), h->pk, msg);
+    h->have_pk = true;
+    if (h->pk_rc) return fail(h->pk_rc, "tjsq_hostplan_packed: " + msg);
+  }
+  if (h->pk_rc) return fail(h->pk_rc, "tjsq_hostplan_packed: layout does not apply");
+  const tjsq::PackedPlan& K = h->pk;
+  const int64_t v[] = {(int64_t)K.ops_jobs.size(), (int64_t)K.x_jobs.size(), (int64_t)K.seg1.size(),
+                       (int64_t)K.tile1.size(), (int64_t)K.seg2.size(), (int64_t)K.tile2.size(),
+                       (int64_t)K.rtile.size(), K.nslots, K.ops_bytes, K.x_bytes, K.t_bytes};
+  std::copy(v, v + 11, sizes);
+  if (ops_jobs) std::memcpy(ops_jobs, K.ops_jobs.data(), sizeof(tjsq::PackJob) * K.ops_jobs.size());
+  if (x_jobs) std::memcpy(x_jobs, K.x_jobs.data(), sizeof(tjsq::PackJob) * K.x_jobs.size());
+  if (seg1) std::memcpy(seg1, K.seg1.data(), sizeof(tjsq::PSeg) * K.seg1.size());
+  if (tile1) std::memcpy(tile1, K.tile1.data(), sizeof(tjsq::PTile) * K.tile1.size());
+  if (seg2) std::memcpy(seg2, K.seg2.data(), sizeof(tjsq::PSeg) * K.seg2.size());
+  if (tile2) std::memcpy(tile2, K.tile2.data(), sizeof(tjsq::PTile) * K.tile2.size());
+  if (rtile) std::memcpy(rtile, K.rtile.data(), sizeof(tjsq::PReduce) * K.rtile.size());
+  return TJSQ_OK;
+}
+
 int tjsq_plan_reshape(tjsq_plan* p, const tjsq_shape* shape, void* stream) {
   if (!p) return fail(TJSQ_ERR_ARG, "tjsq_plan_reshape: null plan");
   tjsq::Shape sh;
@@ -1173,6 +1665,17 @@ int tjsq_plan_destroy(tjsq_plan* p) {
   (void)hipFree(p->x32);
   (void)hipFree(p->x64a);
   (void)hipFree(p->x64b);
+  (void)hipFree(p->pk_ops);
+  (void)hipFree(p->pk_x);
+  (void)hipFree(p->pk_t);
+  (void)hipFree(p->pk_xjobs);
+  (void)hipFree(p->pk_seg1);
+  (void)hipFree(p->pk_seg2);
+  (void)hipFree(p->pk_tile1);
+  (void)hipFree(p->pk_tile2);
+  (void)hipFree(p->pk_rtile);
+  (void)hipFree(p->pk_part);
+  for (void* b : p->pk_owned) (void)hipFree(b);
   delete p;
   return TJSQ_OK;
 }
@@ -1220,7 +1723,8 @@ int tjsq_heff_apply_f32(tjsq_plan* p, int precision, const float* x, float* y, v
   if (!p || !x || !y) return fail(TJSQ_ERR_ARG, "tjsq_heff_apply_f32: null argument");
   if (precision != TJSQ_FP32 && precision != TJSQ_BF16X3) return fail(TJSQ_ERR_ARG, "precision must be FP32 or BF16X3");
   if (precision == TJSQ_FP32 && !p->arena32) return fail(TJSQ_ERR_ARG, "plan was created without FP32 operands");
-  if (precision == TJSQ_BF16X3 && !p->arenab) return fail(TJSQ_ERR_ARG, "plan was created without BF16x3 operands");
+  if (precision == TJSQ_BF16X3 && !p->arenab && !p->packed)
+    return fail(TJSQ_ERR_ARG, "plan was created without BF16x3 operands");
   return apply_spaces(p, precision, x, y, (hipStream_t)stream);
 }
 
@@ -1465,7 +1969,8 @@ extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_
   if (!p || !psi || !opts || !res) return fail(TJSQ_ERR_ARG, "tjsq_lanczos_solve: null argument");
   if (opts->max_iter < 2 || opts->max_iter > 250) return fail(TJSQ_ERR_ARG, "max_iter must be in [2, 250]");
   if (opts->precision == TJSQ_FP32 && !p->arena32) return fail(TJSQ_ERR_ARG, "plan has no FP32 operands");
-  if (opts->precision == TJSQ_BF16X3 && !p->arenab) return fail(TJSQ_ERR_ARG, "plan has no BF16x3 operands");
+  if (opts->precision == TJSQ_BF16X3 && !p->arenab && !p->packed)
+    return fail(TJSQ_ERR_ARG, "plan has no BF16x3 operands");
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t st = (hipStream_t)stream;
   std::memset(res, 0, sizeof(*res));
@@ -2327,9 +2832,12 @@ int tjsq_batch_create(tjsq_plan* const* plans, int n, int precision_mask, void* 
     if (!plans[i]) return fail(TJSQ_ERR_ARG, "tjsq_batch_create: null plan");
     if (withf && !plans[i]->arena32)
       return fail(TJSQ_ERR_ARG, "tjsq_batch_create: a member plan was created without FP32 operands");
-    if (withb && !plans[i]->arenab)
+    if (withb && !plans[i]->arenab && !plans[i]->packed)
       return fail(TJSQ_ERR_ARG, "tjsq_batch_create: a member plan was created without BF16x3 operands");
   }
+  // BF16x3 on the packed layout when every member has it
+  bool all_packed = withb;
+  for (int i = 0; i < n; ++i) all_packed = all_packed && plans[i]->packed;
   hipStream_t st = (hipStream_t)stream;
   tjsq_batch* B = new tjsq_batch();
   B->dev = new tjsq_plan();
@@ -2405,9 +2913,34 @@ int tjsq_batch_create(tjsq_plan* const* plans, int n, int precision_mask, void* 
     return cleanup(TJSQ_ERR_NOMEM, "batch arena allocation failed");
   if (withf && hipMalloc(&B->dev->arena32, sizeof(float) * std::max<int64_t>(1, arena)) != hipSuccess)
     return cleanup(TJSQ_ERR_NOMEM, "batch FP32 arena allocation failed");
-  if (withb && (hipMalloc(&B->dev->arenab, sizeof(uint32_t) * std::max<int64_t>(1, arena)) != hipSuccess ||
-                hipMalloc(&B->dev->xw, sizeof(uint32_t) * std::max<int64_t>(1, M.dim)) != hipSuccess))
+  if (withb && !all_packed &&
+      (hipMalloc(&B->dev->arenab, sizeof(uint32_t) * std::max<int64_t>(1, arena)) != hipSuccess ||
+       hipMalloc(&B->dev->xw, sizeof(uint32_t) * std::max<int64_t>(1, M.dim)) != hipSuccess))
     return cleanup(TJSQ_ERR_NOMEM, "batch BF16x3 operand allocation failed");
+  if (all_packed) {
+    // each member's operator copies, x copies and stage-1 products in
+    // buffers of the batch's own (members may be destroyed afterwards)
+    tjsq::PackedPlan D;
+    for (int i = 0; i < n; ++i) {
+      const tjsq::PackedPlan& K = plans[i]->pk_host;
+      uint8_t *o = nullptr, *x = nullptr, *t = nullptr;
+      const bool okm = hipMalloc(&o, std::max<int64_t>(16, K.ops_bytes)) == hipSuccess &&
+                       (B->dev->pk_owned.push_back(o), hipMalloc(&x, std::max<int64_t>(16, K.x_bytes)) == hipSuccess) &&
+                       (B->dev->pk_owned.push_back(x), hipMalloc(&t, std::max<int64_t>(16, K.t_bytes)) == hipSuccess);
+      if (t) B->dev->pk_owned.push_back(t);
+      if (!okm) return cleanup(TJSQ_ERR_NOMEM, "batch packed operand allocation failed");
+      if (K.ops_bytes && hipMemcpyAsync(o, plans[i]->pk_ops, K.ops_bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
+        return cleanup(TJSQ_ERR_HIP, "batch packed operand copy failed");
+      pk_append_device(K, o, x, t, B->x_off[i], D);
+    }
+    int rc = pk_upload(B->dev, D, st);
+    if (rc) {
+      tjsq_batch_destroy(B);
+      return rc;
+    }
+    B->dev->pk = std::move(D);
+    B->dev->packed = true;
+  }
   for (int i = 0; i < n; ++i) {
     const int64_t e = plans[i]->hp.arena_elems;
     if (e == 0) continue;
@@ -2415,11 +2948,18 @@ int tjsq_batch_create(tjsq_plan* const* plans, int n, int precision_mask, void* 
                                   hipMemcpyDeviceToDevice, st) != hipSuccess) ||
         (withf && hipMemcpyAsync(B->dev->arena32 + abase[i], plans[i]->arena32, sizeof(float) * e,
                                  hipMemcpyDeviceToDevice, st) != hipSuccess) ||
-        (withb && hipMemcpyAsync(B->dev->arenab + abase[i], plans[i]->arenab, sizeof(uint32_t) * e,
-                                 hipMemcpyDeviceToDevice, st) != hipSuccess))
+        (withb && !all_packed && plans[i]->arenab &&
+         hipMemcpyAsync(B->dev->arenab + abase[i], plans[i]->arenab, sizeof(uint32_t) * e,
+                        hipMemcpyDeviceToDevice, st) != hipSuccess))
       return cleanup(TJSQ_ERR_HIP, "batch arena copy failed");
+    if (withb && !all_packed && !plans[i]->arenab) {
+      // a packed member keeps no operand words: split its FP64 operators
+      hipLaunchKernelGGL(k_f64_to_split, dim3(grid_for(e)), dim3(256), 0, st, plans[i]->arena,
+                         B->dev->arenab + abase[i], e);
+      if (hipGetLastError() != hipSuccess) return cleanup(TJSQ_ERR_HIP, "batch operand split failed");
+    }
   }
-  int rc = upload_tables(B->dev, with64, withf || withb, st);
+  int rc = upload_tables(B->dev, with64, withf || (withb && !all_packed), st);
   if (rc) {
     tjsq_batch_destroy(B);
     return rc;
@@ -2458,7 +2998,7 @@ int tjsq_batch_apply_f32(tjsq_batch* B, int precision, const float* x, float* y,
     return fail(TJSQ_ERR_ARG, "tjsq_batch_apply_f32: bad argument");
   if (precision == TJSQ_FP32 && !B->dev->arena32)
     return fail(TJSQ_ERR_ARG, "tjsq_batch_apply_f32: the batch was created without FP32");
-  if (precision == TJSQ_BF16X3 && !B->dev->arenab)
+  if (precision == TJSQ_BF16X3 && !B->dev->arenab && !B->dev->packed)
     return fail(TJSQ_ERR_ARG, "tjsq_batch_apply_f32: the batch was created without BF16x3");
   return batch_run<float>(B, precision, x, y, (hipStream_t)stream);
 }

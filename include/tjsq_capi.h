@@ -113,6 +113,10 @@ typedef struct tjsq_plan_stats {
   int64_t op_elems;         /* operator elements referenced */
   int64_t scratch_elems;    /* stage-1 scratch elements */
   int64_t n_slots;          /* partial output tiles of split stage-2 work */
+  int64_t packed;           /* 1: the BF16x3 apply runs on the packed layout
+                               (tjsq_hostplan_packed), 0: on operand words */
+  double flops_packed;      /* MFMA flops per product the packed tiles execute
+                               (padding included; BF16x3 issues 3 products) */
 } tjsq_plan_stats;
 
 typedef struct tjsq_lanczos_opts {
@@ -273,6 +277,22 @@ int tjsq_hostplan_tables(const tjsq_hostplan* h, void* seg1, void* tile1, void* 
                          void* rtile);
 int tjsq_hostplan_stats(const tjsq_hostplan* h, tjsq_plan_stats* stats);
 int tjsq_hostplan_destroy(tjsq_hostplan* h);
+/* The packed tables of the BF16x3 apply (the layout the plan uses for
+ * BF16x3 when it applies): operands as lines of k in groups of eight (eight
+ * bf16 high halves, then eight low halves), every sub-block zero-padded to
+ * multiples of 16.  sizes[10] = {n_ops_jobs, n_x_jobs, n_seg1, n_tile1,
+ * n_seg2, n_tile2, n_rtile, n_slots, ops_bytes, x_bytes} plus sizes[10] =
+ * t_bytes; then the tables are copied to the non-null pointers.  Layouts:
+ * pack job 40 bytes {int64 src_off, dst_off; int32 src_ld, rows, cols,
+ * trans, lines, kpad}; segment 48 bytes {int64 l_off, r_off; uint32 l_lim,
+ * r_lim; int32 l_stride, r_stride, K, spaces; double c}; tile 56 bytes
+ * {int64 out_off; int32 out_ld, m0, n0, mb, nb, mlim, nlim, seg_begin,
+ * seg_end, out_kind, slot, pad}; reduce 40 bytes {int64 out_off; int32
+ * out_ld, m0, n0, mt, nt, s0, s1, pad}.  Returns TJSQ_ERR_ARG where the
+ * layout does not apply (the plan then keeps the unpacked BF16x3 apply).
+ * No counterpart in the reference: test-side inspection of the tables. */
+int tjsq_hostplan_packed(const tjsq_hostplan* h, int64_t* sizes, void* ops_jobs, void* x_jobs, void* seg1,
+                         void* tile1, void* seg2, void* tile2, void* rtile);
 
 /* ---- host-buffer entry points (Fortran callers holding data in host
  * memory): the same operations with x, y, psi in host memory; the library

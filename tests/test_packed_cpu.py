@@ -1,0 +1,54 @@
+"""The packed BF16x3 tables on the CPU: evaluated byte for byte in numpy
+(packed_eval.py, every read and write bounds-checked) against the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+import packed_eval
+from helpers import GEOMS, model, rel, small_superblock
+from grand_sql_tj_amd import capi, synth
+
+
+@pytest.mark.parametrize("ub", [1, 0])
+@pytest.mark.parametrize("geom", GEOMS)
+def test_packed_tables_reproduce_oracle(ub, geom):
+    m = model(up_bias=ub)
+    sb = small_superblock(m, *geom, seed=7 * geom[0] + geom[1] + ub, msys=40, menv=44, median=5)
+    x = synth.random_vector(sb, 3).astype(np.float32)
+    hp = capi.HostPlan(m, sb)
+    y = packed_eval.evaluate(hp, sb, x)
+    yo = oracle.model_wave(m, sb, x.astype(np.float64))
+    # BF16x3: each product to ~2^-16 relative, sums of O(100) terms
+    assert rel(y, yo) < 2e-5
+
+
+def test_packed_tables_split_long_blocks():
+    # blocks wider than a tile and K longer than a piece: several tiles per
+    # block, partial slots and their ordered reduction
+    m = model(up_bias=1)
+    sb = small_superblock(m, 11, 11, seed=5, msys=200, menv=210, median=90, nsec=3)
+    hp = capi.HostPlan(m, sb)
+    pk = hp.packed()
+    assert pk["nslots"] > 0 and len(pk["rtile"]) > 0
+    assert (pk["tile2"]["mb"] == 4).any() and (pk["tile1"]["mb"] == 4).any()
+    x = synth.random_vector(sb, 4).astype(np.float32)
+    assert rel(packed_eval.evaluate(hp, sb, x), oracle.model_wave(m, sb, x.astype(np.float64))) < 2e-5
+
+
+def test_packed_layout_geometry():
+    m = model(up_bias=1)
+    sb = small_superblock(m, 7, 9, seed=2, msys=60, menv=60, median=6)
+    pk = capi.HostPlan(m, sb).packed()
+    for jobs in (pk["ops_jobs"], pk["x_jobs"]):
+        assert (jobs["lines"] % 16 == 0).all() and (jobs["kpad"] % 16 == 0).all()
+        assert (jobs["dst_off"] % 64 == 0).all()
+        end = jobs["dst_off"] + jobs["lines"].astype(np.int64) * jobs["kpad"] * 4
+        order = np.argsort(jobs["dst_off"])
+        assert (end[order][:-1] <= jobs["dst_off"][order][1:]).all()   # copies do not overlap
+    for segs in (pk["seg1"], pk["seg2"]):
+        g = segs[(segs["spaces"] & 0xff) != 3]
+        assert (g["K"] % 16 == 0).all() and (g["K"] > 0).all()
+        assert (g["l_off"] % 64 == 0).all() and (g["r_off"] % 64 == 0).all()
+    for t in (pk["tile1"], pk["tile2"]):
+        assert ((t["mb"] >= 1) & (t["mb"] <= 4) & (t["nb"] >= 1) & (t["nb"] <= 4)).all()
+        assert (t["m0"] % 64 == 0).all() and (t["n0"] % 64 == 0).all()
