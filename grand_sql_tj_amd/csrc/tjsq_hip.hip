@@ -544,7 +544,10 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc_n(const void* base, 
 }
 
 // one packed copy per workgroup (grid-stride over jobs); a thread packs one
-// group of eight k of one line (device jobs hold absolute destinations)
+// group of eight k of one line (device jobs hold absolute destinations).
+// Consecutive threads take consecutive source elements: along a line where
+// k runs down the source rows (trans = 0), across lines where it runs along
+// them (trans = 1).
 template <typename S>
 __global__ __launch_bounds__(256) void k_pack(const tjsq::PackJob* __restrict__ jobs, int njobs, const S* __restrict__ src) {
   for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
@@ -552,20 +555,26 @@ __global__ __launch_bounds__(256) void k_pack(const tjsq::PackJob* __restrict__ 
     const int groups = J.kpad / 8;
     const int n = J.lines * groups;
     for (int e = threadIdx.x; e < n; e += blockDim.x) {
-      const int l = e / groups, g = e % groups;
-      u4 hi, lo;
+      const int l = J.trans ? e % J.lines : e / groups, g = J.trans ? e / J.lines : e % groups;
       unsigned short h[8], w[8];
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
         const int k = 8 * g + t;
         const int r = J.trans ? l : k, c = J.trans ? k : l;
-        double v = 0.0;
-        if (r < J.rows && c < J.cols) v = (double)src[J.src_off + r + (int64_t)c * J.src_ld];
-        // FP64 operators keep their FP64 residual in the low half (as the
-        // unpacked operand words do); FP32 x splits its own value
-        h[t] = bf16_rne((float)v);
-        w[t] = bf16_rne((float)(v - (double)bf16_to_f(h[t])));
+        const bool in = r < J.rows && c < J.cols;
+        if constexpr (sizeof(S) == 8) {
+          // FP64 operators keep their FP64 residual in the low half (as the
+          // unpacked operand words do)
+          const double v = in ? (double)src[J.src_off + r + (int64_t)c * J.src_ld] : 0.0;
+          h[t] = bf16_rne((float)v);
+          w[t] = bf16_rne((float)(v - (double)bf16_to_f(h[t])));
+        } else {
+          const float v = in ? (float)src[J.src_off + r + (int64_t)c * J.src_ld] : 0.0f;
+          h[t] = bf16_rne(v);
+          w[t] = bf16_rne(v - bf16_to_f(h[t]));   // exact in FP32
+        }
       }
+      u4 hi, lo;
       hi.x = h[0] | ((uint32_t)h[1] << 16); hi.y = h[2] | ((uint32_t)h[3] << 16);
       hi.z = h[4] | ((uint32_t)h[5] << 16); hi.w = h[6] | ((uint32_t)h[7] << 16);
       lo.x = w[0] | ((uint32_t)w[1] << 16); lo.y = w[2] | ((uint32_t)w[3] << 16);
@@ -577,11 +586,6 @@ __global__ __launch_bounds__(256) void k_pack(const tjsq::PackJob* __restrict__ 
   }
 }
 
-// the operand lines of one segment as a lane reads them: the buffer from
-// the segment's copy to the end of its space, and per 16-line block the
-// lane's byte offset (line origin + 16 b + lane % 16, group lane / 16), or
-// an offset past the record count (reads zero, no memory access) for blocks
-// outside the tile
 struct PSide {
   __amdgpu_buffer_rsrc_t r;
   uint32_t v[4];
