@@ -194,8 +194,10 @@ def block_update_step(model, sb, psi, m_keep, torch, reps=5):
 def batch_member(args):
     """One member of the batched-apply workload: a bond in the middle of a
     width-4 (4 x 16) or width-6 (6 x 12) cylinder, m multiplets per block,
-    sector dims log-normal (median 32, max 512)."""
-    i, seed = args
+    sector dims log-normal (median 32, max 512).  args = (i, seed) or
+    (i, seed, fast): fast draws the operator entries with FastNormals."""
+    i, seed = args[0], args[1]
+    fast = len(args) > 2 and bool(args[2])
     from grand_sql_tj_amd import synth
     from grand_sql_tj_amd.lattice import TJModel
     r = np.random.default_rng(seed + 7919 * i)
@@ -209,7 +211,7 @@ def batch_member(args):
                     up_bias=1)
     nsec = int(r.integers(4, 9))
     sb = synth.superblock(model, sys_len, env_len, m, m, seed=seed + i, median=32, sigma=0.8, maxdim=512,
-                          nsec=nsec, smax=10, filling=0.875)
+                          nsec=nsec, smax=10, filling=0.875, fast=fast)
     return model, sb, {"ny": ny, "m": m, "sys_len": sys_len, "env_len": env_len}
 
 
@@ -302,8 +304,10 @@ def batch_bench(args):
 def serve_request(args):
     """One request of the serving scan: the centre bond of a width-4/6,
     length 8-16 t-J cylinder, J/t ~ U[0.2, 0.6], t'/t in {0, -0.2}, doping in
-    {1/12, 1/8, 1/6}, m ~ U[500, 6000]."""
-    i, seed = args
+    {1/12, 1/8, 1/6}, m ~ U[500, 6000].  args = (i, seed) or (i, seed, fast)
+    (fast: operator entries drawn with FastNormals)."""
+    i, seed = args[0], args[1]
+    fast = len(args) > 2 and bool(args[2])
     from grand_sql_tj_amd import synth
     from grand_sql_tj_amd.lattice import TJModel
     r = np.random.default_rng(seed + 104729 * i)
@@ -319,7 +323,7 @@ def serve_request(args):
     sys_len = n // 2 - 1
     env_len = n - 2 - sys_len
     sb = synth.superblock(model, sys_len, env_len, m, m, seed=seed + i, median=32, sigma=0.8, maxdim=512,
-                          nsec=int(np.clip(m // 400, 4, 14)), smax=10, filling=ne / n, total_up=ne)
+                          nsec=int(np.clip(m // 400, 4, 14)), smax=10, filling=ne / n, total_up=ne, fast=fast)
     psi0 = synth.random_vector(sb, seed + i)
     return model, sb, psi0, {"ny": ny, "nx": nx, "m": m, "N": ne, "J": model.J, "D": sb.dim}
 

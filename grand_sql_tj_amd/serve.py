@@ -78,7 +78,12 @@ def footprint(model, sb, precision: str, max_iter: int) -> int:
     tables = 40 * (hp.seg1.size + hp.seg2.size) + 48 * (hp.tile1.size + hp.tile2.size + hp.rtile.size)
     b = 8 * (hp.arena_elems + hp.scratch_elems + slots) + tables
     w = 8
-    if precision != "fp64":
+    pk = hp.packed() if precision == "bf16x3" else None
+    if pk is not None:   # the packed BF16x3 operands, products, partial tiles and tables
+        b += (pk["ops_bytes"] + pk["x_bytes"] + pk["t_bytes"] + 4 * 4096 * pk["nslots"]
+              + 48 * (pk["seg1"].size + pk["seg2"].size) + 56 * (pk["tile1"].size + pk["tile2"].size))
+        w = 4
+    elif precision != "fp64":
         b += 4 * (hp.arena_elems + hp.scratch_elems + slots)
         w = 4
     b += (max_iter + 2) * D * w + 6 * D * 8

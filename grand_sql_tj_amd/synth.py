@@ -63,6 +63,20 @@ class SplitMix64:
         return z[:n]
 
 
+class FastNormals(SplitMix64):
+    """SplitMix64 for the structure draws (sector dimensions), numpy's PCG64
+    for the operator entries: the same block structure as SplitMix64 with
+    the same seed, different (and ~10x cheaper) entries.  For tests at the
+    benchmark sizes, where only the structure and the symmetries matter."""
+
+    def __init__(self, seed: int):
+        super().__init__(seed)
+        self.fast = np.random.default_rng(seed & _MASK)
+
+    def entries(self, n: int) -> np.ndarray:
+        return self.fast.standard_normal(n)
+
+
 def block_sectors(rng: SplitMix64, length: int, up_bias: int, n_center: float, m: int,
                   median: float, sigma: float, maxdim: int, nsec: int, smax: int):
     """Sector list [(N, 2S, dim)] of a block, ordered like the reference's
@@ -91,7 +105,8 @@ def block_sectors(rng: SplitMix64, length: int, up_bias: int, n_center: float, m
 
 
 def _rand(rng: SplitMix64, r: int, c: int, scale: float) -> np.ndarray:
-    return (rng.normal(r * c) * scale).reshape(c, r).T.copy()
+    draw = rng.entries if isinstance(rng, FastNormals) else rng.normal
+    return (draw(r * c) * scale).reshape(c, r).T.copy()
 
 
 def _sym(rng: SplitMix64, d: int, scale: float) -> np.ndarray:
@@ -154,9 +169,10 @@ def random_model(rng: SplitMix64, length: int, secs: np.ndarray, nact: int, up_b
 def superblock(model: TJModel, sys_len: int, env_len: int, m_sys: int, m_env: int, seed: int,
                median: float = 4.0, sigma: float = 0.6, maxdim: int = 600, nsec: int = 8,
                smax: int = 6, filling: float | None = None, total_up: int | None = None,
-               scale: float = 1.0, nact: int | None = None) -> Superblock:
-    """A random but symmetric two-site superblock [sys . site . site . env]."""
-    rng = SplitMix64(seed)
+               scale: float = 1.0, nact: int | None = None, fast: bool = False) -> Superblock:
+    """A random but symmetric two-site superblock [sys . site . site . env]
+    (fast: operator entries from FastNormals)."""
+    rng = FastNormals(seed) if fast else SplitMix64(seed)
     ub = model.up_bias
     if filling is None:
         filling = 0.875
