@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtjsq_hip.so")
 
 TJSQ_OK = 0
+TJSQ_ERR_ARG = -1
 FP64, FP32, BF16X3 = 0, 1, 2
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
 
@@ -101,7 +102,7 @@ EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_t
            "tjsq_truncation_select", "tjsq_identity_truncation", "tjsq_batch_create", "tjsq_batch_offsets",
            "tjsq_batch_stats", "tjsq_batch_graph", "tjsq_batch_apply", "tjsq_batch_apply_f32", "tjsq_batch_profile",
            "tjsq_batch_profile_read", "tjsq_batch_destroy", "tjsq_plan_create_ex", "tjsq_hostplan_create_ex",
-           "tjsq_plan_reshape", "tjsq_plan_graphs", "tjsq_shutdown", "tjsq_hostplan_packed",
+           "tjsq_plan_reshape", "tjsq_plan_graphs", "tjsq_shutdown", "tjsq_hostplan_packed", "tjsq_heff_apply_noalloc",
            "tjsq_last_error"]
 
 _lib = None
@@ -172,6 +173,7 @@ def lib():
             "tjsq_plan_reshape": [P, P, P],
             "tjsq_plan_graphs": [P, I],
             "tjsq_shutdown": [],
+            "tjsq_heff_apply_noalloc": [P, I, P, P, P],
             "tjsq_hostplan_packed": [P, P, P, P, P, P, P, P, P],
         }
         for name in EXPORTS:

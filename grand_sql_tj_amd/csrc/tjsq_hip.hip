@@ -1723,6 +1723,78 @@ int tjsq_heff_apply(tjsq_plan* p, const double* x, double* y, void* stream) {
   return apply_spaces(p, TJSQ_FP64, x, y, st);
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// H_eff apply that allocates nothing: the stage-1 products and the partial
+// tiles of split work items live in one file-scope device array, `scratch`
+// (TJSQ_NOALLOC_SCRATCH_BYTES, held by every GPU the library runs on until
+// the process ends).  In host code `scratch` names a host-side stand-in, not
+// a device address: its device address is looked up once per device with
+// hipGetSymbolAddress (that device current) and only that address reaches
+// kernels.  Calls on one GPU share the array: calls that overlap on one GPU
+// race.  Nothing in it needs clearing: every stage-1 product and every
+// partial tile is written in full before it is read.
+// ---------------------------------------------------------------------------
+namespace {
+
+static __device__ __align__(256) unsigned char scratch[TJSQ_NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_addr;   // device address of `scratch` per device
+
+int scratch_address(void** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_addr.size() <= dev) g_scratch_addr.resize(dev + 1, nullptr);
+  if (!g_scratch_addr[dev]) {
+    void* a = nullptr;
+    HIPCHK(hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)));
+    if (!a) return fail(TJSQ_ERR_HIP, "tjsq_heff_apply_noalloc: no device address for the scratch array");
+    g_scratch_addr[dev] = a;
+  }
+  *out = g_scratch_addr[dev];
+  return TJSQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tjsq_heff_apply_noalloc(tjsq_plan* p, int precision, const void* x, void* y, void* stream) {
+  if (!p || !x || !y) return fail(TJSQ_ERR_ARG, "tjsq_heff_apply_noalloc: null argument");
+  if (precision != TJSQ_FP64 && precision != TJSQ_FP32)
+    return fail(TJSQ_ERR_ARG, "tjsq_heff_apply_noalloc: precision must be FP64 or FP32 (BF16x3 applies read "
+                              "their packed copies through the plan's own buffers)");
+  if (precision == TJSQ_FP32 && !p->arena32)
+    return fail(TJSQ_ERR_ARG, "tjsq_heff_apply_noalloc: plan was created without FP32 operands");
+  if (p->profiling) return fail(TJSQ_ERR_ARG, "tjsq_heff_apply_noalloc: profiling records events; switch it off");
+  const HostPlan& hp = p->hp;
+  const int64_t w = precision == TJSQ_FP64 ? 8 : 4;
+  // stage-1 products, then the partial tiles (units of 32 x 32 elements),
+  // each start 256-byte aligned
+  const int64_t t_bytes = (w * std::max<int64_t>(1, hp.scratch_elems) + 255) / 256 * 256;
+  const int64_t need = t_bytes + w * std::max<int64_t>(1, hp.nslots) * 1024;
+  if (need > (int64_t)TJSQ_NOALLOC_SCRATCH_BYTES)
+    return fail(TJSQ_ERR_ARG, "tjsq_heff_apply_noalloc: the plan needs " + std::to_string(need) +
+                                  " bytes of scratch, the static scratch holds " +
+                                  std::to_string((int64_t)TJSQ_NOALLOC_SCRATCH_BYTES));
+  void* base = nullptr;
+  int rc = scratch_address(&base);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  uint8_t* b = (uint8_t*)base;
+  if (precision == TJSQ_FP64) {
+    const double* xd = (const double*)x;
+    return launch_apply<MODE_F64>(p, Spaces<MODE_F64>{p->arena, xd, xd, (double*)b, (double*)y},
+                                  (double*)(b + t_bytes), st);
+  }
+  const float* xf = (const float*)x;
+  return launch_apply<MODE_F32>(p, Spaces<MODE_F32>{p->arena32, xf, xf, (float*)b, (float*)y},
+                                (float*)(b + t_bytes), st);
+}
+
 int tjsq_heff_apply_f32(tjsq_plan* p, int precision, const float* x, float* y, void* stream) {
   if (!p || !x || !y) return fail(TJSQ_ERR_ARG, "tjsq_heff_apply_f32: null argument");
   if (precision != TJSQ_FP32 && precision != TJSQ_BF16X3) return fail(TJSQ_ERR_ARG, "precision must be FP32 or BF16X3");

@@ -227,6 +227,19 @@ int tjsq_heff_apply(tjsq_plan* plan, const double* x, double* y, void* stream);
  * TJSQ_BF16X3. */
 int tjsq_heff_apply_f32(tjsq_plan* plan, int precision, const float* x, float* y, void* stream);
 
+/* tjsq_heff_apply (precision TJSQ_FP64, double vectors) or
+ * tjsq_heff_apply_f32 with TJSQ_FP32 (float vectors), allocating nothing:
+ * the stage-1 products and partial tiles live in a static device array of
+ * TJSQ_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until
+ * the process ends.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).  Plans
+ * needing more scratch than the array holds are rejected (TJSQ_ERR_ARG), as
+ * are BF16x3 and a plan with profiling on.  Calls that overlap on one GPU
+ * share the array and race.  Results are bitwise those of the allocating
+ * entry points. */
+#define TJSQ_NOALLOC_SCRATCH_BYTES (64 * 1024 * 1024)
+int tjsq_heff_apply_noalloc(tjsq_plan* plan, int precision, const void* x, void* y, void* stream);
+
 /* Per-launch timing of the apply kernels with HIP events recorded on the
  * launching stream.  profile_read returns the summed milliseconds and launch
  * counts since the last read in ms[5] / launches[5]: stage-1 64x64 items,
