@@ -955,7 +955,8 @@ struct tjsq_plan {
   void* kry = nullptr;           // Krylov basis
   int64_t kry_bytes = 0;
   double* dwork = nullptr;       // partials + dot outputs + coefficients
-  float* x32 = nullptr;          // FP32 staging for a residual check
+  float* x32 = nullptr;          // FP32 staging of the mixed solve's applies (x and y)
+  float* y32 = nullptr;
   double* x64a = nullptr;
   double* x64b = nullptr;
   // Lanczos iterations captured as hipGraphs, keyed by (precision, basis
@@ -1667,6 +1668,7 @@ int tjsq_plan_destroy(tjsq_plan* p) {
   (void)hipFree(p->kry);
   (void)hipFree(p->dwork);
   (void)hipFree(p->x32);
+  (void)hipFree(p->y32);
   (void)hipFree(p->x64a);
   (void)hipFree(p->x64b);
   (void)hipFree(p->pk_ops);
@@ -1881,6 +1883,21 @@ int apply_prec(tjsq_plan* p, int precision, const void* x, void* y, hipStream_t 
   return tjsq_heff_apply_f32(p, precision, (const float*)x, (float*)y, st);
 }
 
+// y = H x on FP64 vectors with the contractions in `precision`: FP32 /
+// BF16x3 applies read an FP32 copy of x and write an FP32 y, widened to FP64
+// (the mixed solve's Krylov basis, and so its reorthogonalisation, is FP64)
+int apply_f64(tjsq_plan* p, int precision, const double* x, double* y, hipStream_t st) {
+  if (precision == TJSQ_FP64) return tjsq_heff_apply(p, x, y, st);
+  const int64_t n = p->hp.dim;
+  hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(n)), dim3(256), 0, st, x, p->x32, n);
+  HIPCHK(hipGetLastError());
+  int rc = tjsq_heff_apply_f32(p, precision, p->x32, p->y32, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_f32_to_f64, dim3(grid_for(n)), dim3(256), 0, st, p->y32, y, n);
+  HIPCHK(hipGetLastError());
+  return TJSQ_OK;
+}
+
 // One Lanczos pass from v0 (already in slot 0, normalised).  Returns the
 // number of vectors, the Ritz value and the tridiagonal data.
 // One Lanczos iteration at basis size `last` with every scalar on the device
@@ -1906,7 +1923,8 @@ int iteration_launches(Krylov<T>& K, int precision, int last) {
   hipLaunchKernelGGL(k_combine<T>, dim3(grid_for(K.n)), dim3(256), 0, K.st, w, K.V, K.n, last, K.coef, K.n, 1.0, 0);
   K.normalize_dev(w, K.tmp + 9);
   HIPCHK(hipGetLastError());
-  int rc = apply_prec(K.p, precision, w, K.vec(last + 1), K.st);
+  int rc = std::is_same<T, double>::value ? apply_f64(K.p, precision, (const double*)w, (double*)K.vec(last + 1), K.st)
+                                          : apply_prec(K.p, precision, w, K.vec(last + 1), K.st);
   if (rc) return rc;
   K.dots_dev(1, w, K.vec(last + 1), false, K.ab + 2 * last + 1);
   HIPCHK(hipGetLastError());
@@ -1960,7 +1978,9 @@ int lanczos_pass(Krylov<T>& K, int precision, double tol, int max_iter, int& las
     int rc0 = ensure_fp64_scratch(K.p);   // no allocation may happen inside a capture
     if (rc0) return rc0;
   }
-  int rc = apply_prec(K.p, precision, K.vec(0), K.vec(1), K.st);
+  int rc = std::is_same<T, double>::value
+               ? apply_f64(K.p, precision, (const double*)K.vec(0), (double*)K.vec(1), K.st)
+               : apply_prec(K.p, precision, K.vec(0), K.vec(1), K.st);
   if (rc) return rc;
   ++applies;
   K.dots_dev(1, K.vec(0), K.vec(1), false, K.ab + 1);
@@ -2074,9 +2094,16 @@ extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_
     res->e_ritz = e0;
     res->iterations = last;
   } else {
-    Krylov<float> K;
+    // mixed precision: FP32 / BF16x3 contractions, the Krylov basis, its
+    // dots and the Gram-Schmidt subtraction in FP64 (apply_f64)
+    if (!p->y32) {
+      if ((!p->x32 && hipMalloc(&p->x32, sizeof(float) * n) != hipSuccess) ||
+          hipMalloc(&p->y32, sizeof(float) * n) != hipSuccess)
+        return fail(TJSQ_ERR_NOMEM, "FP32 staging allocation failed");
+    }
+    Krylov<double> K;
     if ((rc = ensure_krylov(p, opts->max_iter, K, st))) return rc;
-    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(n)), dim3(256), 0, st, psi, K.vec(0), n);
+    HIPCHK(hipMemcpyAsync(K.vec(0), psi, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     double nrm;
     if ((rc = K.norm(K.vec(0), nrm))) return rc;
     if ((rc = K.scale(K.vec(0), 1.0 / nrm))) return rc;
@@ -2084,9 +2111,7 @@ extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_
     if ((rc = lanczos_pass(K, opts->precision, opts->tol, opts->max_iter, last, ritz, alpha, beta, brk, applies, rtol,
                            &theta2)))
       return rc;
-    float* out32 = K.vec(opts->max_iter + 1);
-    if ((rc = assemble(K, last, alpha, beta, out32, e0))) return rc;
-    hipLaunchKernelGGL(k_f32_to_f64, dim3(grid_for(n)), dim3(256), 0, st, out32, psi, n);
+    if ((rc = assemble(K, last, alpha, beta, psi, e0))) return rc;
     res->e_ritz = e0;
     res->iterations = last;
   }
