@@ -836,13 +836,10 @@ __global__ void k_f32_to_split(const float* __restrict__ a, uint32_t* __restrict
 // ---------------------------------------------------------------------------
 // Lanczos vector kernels (FP64 arithmetic on FP64 or FP32 storage)
 // ---------------------------------------------------------------------------
-constexpr int RED_BLOCKS = 512;
+constexpr int RED_BLOCKS = 2048;
 constexpr int RED_THREADS = 256;
+constexpr int RED_MAXV = 256;     // vectors one k_multidot launch takes (max_iter + 1 <= 251)
 
-// partial[b * nv + j] = sum over block b's slice of V_j . w  (j < nv), and,
-// when with_self, partial[(RED_BLOCKS + b) * nv + j] = V_j . V_j.  Each block
-// owns one contiguous slice of the vectors; the slice of w stays in cache
-// while the block walks the nv vectors.
 template <typename T>
 __device__ __forceinline__ double block_sum(double v, double* red) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -856,27 +853,53 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return t;
 }
 
+// partial[b * nv + j] = sum over block b's slice of V_j . w  (j < nv), and,
+// when with_self, partial[(RED_BLOCKS + b) * nv + j] = V_j . V_j.  Each block
+// owns one contiguous slice of the vectors and walks them four at a time
+// (w's slice stays in cache); each wave reduces its sums by shuffles into
+// LDS, and after one barrier the block's four wave sums are added in a fixed
+// order, so the result does not depend on timing.
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_multidot(const T* __restrict__ V, int64_t ldv, int nv,
                                                           const T* __restrict__ w, int64_t n, double* partial,
                                                           int with_self) {
-  __shared__ double red[RED_THREADS / 64];
+  constexpr int G = 4;
+  __shared__ double red[2][RED_THREADS / 64][RED_MAXV];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int64_t per = (n + gridDim.x - 1) / gridDim.x;
   const int64_t lo = (int64_t)blockIdx.x * per, hi = min(n, lo + per);
-  for (int j = 0; j < nv; ++j) {
-    const T* v = V + (int64_t)j * ldv;
-    double s = 0.0, ss = 0.0;
+  for (int j0 = 0; j0 < nv; j0 += G) {
+    double s[G], ss[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) s[g] = ss[g] = 0.0;
     for (int64_t i = lo + threadIdx.x; i < hi; i += RED_THREADS) {
-      const double vi = (double)v[i];
-      s += vi * (double)w[i];
-      ss += vi * vi;
+      const double wi = (double)w[i];
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        if (j0 + g < nv) {
+          const double vi = (double)V[(int64_t)(j0 + g) * ldv + i];
+          s[g] += vi * wi;
+          ss[g] += vi * vi;
+        }
     }
-    const double ts = block_sum<T>(s, red);
-    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * nv + j] = ts;
-    if (with_self) {
-      const double tss = block_sum<T>(ss, red);
-      if (threadIdx.x == 0) partial[((int64_t)RED_BLOCKS + blockIdx.x) * nv + j] = tss;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      double a = s[g], b = ss[g];
+      for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_down(a, o, 64);
+        b += __shfl_down(b, o, 64);
+      }
+      if (lane == 0 && j0 + g < nv) {
+        red[0][wid][j0 + g] = a;
+        red[1][wid][j0 + g] = b;
+      }
     }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < nv; j += RED_THREADS) {
+    partial[(int64_t)blockIdx.x * nv + j] = ((red[0][0][j] + red[0][1][j]) + red[0][2][j]) + red[0][3][j];
+    if (with_self)
+      partial[((int64_t)RED_BLOCKS + blockIdx.x) * nv + j] = ((red[1][0][j] + red[1][1][j]) + red[1][2][j]) + red[1][3][j];
   }
 }
 
@@ -2063,6 +2086,7 @@ int ensure_krylov(tjsq_plan* p, int max_iter, Krylov<T>& K, hipStream_t st) {
 extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_opts* opts,
                                   tjsq_lanczos_result* res, void* stream) {
   if (!p || !psi || !opts || !res) return fail(TJSQ_ERR_ARG, "tjsq_lanczos_solve: null argument");
+  static_assert(250 + 1 <= RED_MAXV, "a Gram-Schmidt dot takes up to max_iter + 1 vectors");
   if (opts->max_iter < 2 || opts->max_iter > 250) return fail(TJSQ_ERR_ARG, "max_iter must be in [2, 250]");
   if (opts->precision == TJSQ_FP32 && !p->arena32) return fail(TJSQ_ERR_ARG, "plan has no FP32 operands");
   if (opts->precision == TJSQ_BF16X3 && !p->arenab && !p->packed)

@@ -159,6 +159,26 @@ int compile_packed(const HostPlan& P, const PackShape& shape, PackedPlan& out, s
     }
   }
 
+  if (shape.order1 && !out.tile1.empty()) {
+    // locality order: x block of the first segment (by its offset in x: sys
+    // sector outer, env sector inner), then the operator copy
+    std::map<int64_t, int64_t> x_src;
+    for (const PackJob& j : out.x_jobs) x_src.emplace(j.dst_off, j.src_off);
+    const size_t n = out.tile1.size();
+    std::vector<int> order(n);
+    std::vector<std::tuple<int64_t, int64_t, int, int>> key(n);
+    for (size_t t = 0; t < n; ++t) {
+      const PTile& tl = out.tile1[t];
+      const PSeg& s0 = out.seg1[tl.seg_begin];
+      key[t] = std::make_tuple(x_src.at(s0.l_off), s0.r_off, tl.n0, tl.m0);
+      order[t] = (int)t;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+    std::vector<PTile> sorted(n);
+    for (size_t i = 0; i < n; ++i) sorted[i] = out.tile1[order[i]];
+    out.tile1.swap(sorted);
+  }
+
   // ---- stage 2: per output block, axpy segments first, then GEMM pieces --
   std::vector<int> order(P.contrib.size());
   for (size_t q = 0; q < order.size(); ++q) order[q] = (int)q;

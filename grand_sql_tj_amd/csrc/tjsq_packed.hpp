@@ -85,6 +85,10 @@ struct PackedPlan {
 struct PackShape {
   int k_split = 256;       // K pieces of stage-2 segments (a multiple of 32)
   int item_chunks = 32;    // K chunks of 32 per stage-2 work item
+  // stage-1 tiles sorted by their x block (sys sector outer, so the
+  // operators of one sys sector are read while the sector's x blocks are
+  // swept) and then by operator copy; 0: in compile order
+  int order1 = 1;
 };
 
 // Builds the packed tables from P.contrib / P.out_blocks (filled by

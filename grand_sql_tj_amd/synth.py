@@ -198,6 +198,17 @@ def superblock(model: TJModel, sys_len: int, env_len: int, m_sys: int, m_env: in
                       meta={"seed": seed, "total_up": total_up, "model": model})
 
 
+def block(model: TJModel, length: int, m: int, seed: int, median: float = 4.0, sigma: float = 0.6,
+          maxdim: int = 600, nsec: int = 8, smax: int = 6, filling: float = 0.875, scale: float = 1.0,
+          nact: int | None = None, fast: bool = False):
+    """One random symmetric block model of `length` sites and about m
+    multiplets (the sys side of superblock() with the same arguments)."""
+    rng = FastNormals(seed) if fast else SplitMix64(seed)
+    _, nleg11 = active_sites(model)
+    secs = block_sectors(rng, length, model.up_bias, filling * length, m, median, sigma, maxdim, nsec, smax)
+    return random_model(rng, length, secs, nact or max(1, int(nleg11[length])), model.up_bias, scale)
+
+
 def random_vector(sb: Superblock, seed: int) -> np.ndarray:
     rng = SplitMix64(seed ^ 0x5DEECE66D)
     x = rng.normal(sb.dim)

@@ -52,3 +52,9 @@ def test_packed_layout_geometry():
     for t in (pk["tile1"], pk["tile2"]):
         assert ((t["mb"] >= 1) & (t["mb"] <= 4) & (t["nb"] >= 1) & (t["nb"] <= 4)).all()
         assert (t["m0"] % 64 == 0).all() and (t["n0"] % 64 == 0).all()
+    # stage-1 tiles in locality order: x block of the first segment, then
+    # the operator copy
+    src = {int(j["dst_off"]): int(j["src_off"]) for j in pk["x_jobs"]}
+    first = pk["seg1"][pk["tile1"]["seg_begin"]]
+    key = [(src[int(s["l_off"])], int(s["r_off"])) for s in first]
+    assert key == sorted(key)
