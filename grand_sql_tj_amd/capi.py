@@ -103,7 +103,7 @@ EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_t
            "tjsq_batch_stats", "tjsq_batch_graph", "tjsq_batch_apply", "tjsq_batch_apply_f32", "tjsq_batch_profile",
            "tjsq_batch_profile_read", "tjsq_batch_destroy", "tjsq_plan_create_ex", "tjsq_hostplan_create_ex",
            "tjsq_plan_reshape", "tjsq_plan_graphs", "tjsq_shutdown", "tjsq_hostplan_packed", "tjsq_heff_apply_noalloc",
-           "tjsq_last_error"]
+           "tjsq_plan_set_ops", "tjsq_last_error"]
 
 _lib = None
 
@@ -171,6 +171,7 @@ def lib():
             "tjsq_plan_create_ex": [P, I, I, P, P, P, P, I, P, I, P, P, P],
             "tjsq_hostplan_create_ex": [P, I, I, P, P, P, P, I, P, P, P],
             "tjsq_plan_reshape": [P, P, P],
+            "tjsq_plan_set_ops": [P, P, P, P],
             "tjsq_plan_graphs": [P, I],
             "tjsq_shutdown": [],
             "tjsq_heff_apply_noalloc": [P, I, P, P, P],
@@ -416,6 +417,17 @@ class Plan:
 
     def _stream(self):
         return C.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+
+    def set_ops(self, sys_ops, env_ops, device_ops=None):
+        """New block-operator data with the plan's layouts (tjsq_plan_set_ops):
+        sys_ops / env_ops are OpSets whose hdr, sub and off equal those the
+        plan was made with; their data (or device_ops, two CUDA tensors) are
+        uploaded, hmerge and the FP32 / BF16x3 copies are redone."""
+        keep = _Keep()
+        dsys = device_ops[0].data_ptr() if device_ops is not None else None
+        denv = device_ops[1].data_ptr() if device_ops is not None else None
+        s, e = opset_struct(sys_ops, keep, dsys), opset_struct(env_ops, keep, denv)
+        check(lib().tjsq_plan_set_ops(self.h, C.byref(s), C.byref(e), self._stream()), "tjsq_plan_set_ops")
 
     def graphs(self, enable=True):
         """Replay each Lanczos iteration from a captured hipGraph."""

@@ -1005,6 +1005,8 @@ struct tjsq_plan {
   tjsq::PTile* pk_tile2 = nullptr;
   tjsq::PReduce* pk_rtile = nullptr;
   float* pk_part = nullptr;
+  // the operator-set layouts the plan was compiled for (tjsq_plan_set_ops)
+  uint64_t layout_hash = 0;
 };
 
 namespace {
@@ -1259,6 +1261,25 @@ int pk_upload(tjsq_plan* p, const tjsq::PackedPlan& D, hipStream_t st) {
 // the packed BF16x3 tables and operator copies of a plan; ok = false (and
 // the plan keeps the unpacked BF16x3 operands) where the layout does not
 // apply or a space would exceed the 2 GiB a buffer resource addresses
+// the packed operator copies, packed from the FP64 arena (jobs: host form,
+// destinations relative to pk_ops)
+int pack_ops(tjsq_plan* p, const std::vector<tjsq::PackJob>& jobs, hipStream_t st) {
+  if (jobs.empty()) return TJSQ_OK;
+  std::vector<tjsq::PackJob> oj = jobs;
+  for (tjsq::PackJob& j : oj) j.dst_off = (int64_t)(uintptr_t)(p->pk_ops + j.dst_off);
+  tjsq::PackJob* ojobs = nullptr;
+  if (hipMalloc(&ojobs, sizeof(tjsq::PackJob) * oj.size()) != hipSuccess)
+    return fail(TJSQ_ERR_NOMEM, "packed BF16x3 allocation failed");
+  HIPCHK(hipMemcpyAsync(ojobs, oj.data(), sizeof(tjsq::PackJob) * oj.size(), hipMemcpyHostToDevice, st));
+  const int nj = (int)oj.size();
+  hipLaunchKernelGGL(k_pack<double>, dim3(std::min(nj, 8192)), dim3(256), 0, st, ojobs, nj, p->arena);
+  const hipError_t e = hipGetLastError();
+  const hipError_t e2 = hipStreamSynchronize(st);
+  (void)hipFree(ojobs);
+  if (e != hipSuccess || e2 != hipSuccess) return fail(TJSQ_ERR_HIP, "packing the BF16x3 operators failed");
+  return TJSQ_OK;
+}
+
 int setup_packed(tjsq_plan* p, hipStream_t st, bool& ok) {
   ok = false;
   std::string msg;
@@ -1270,31 +1291,96 @@ int setup_packed(tjsq_plan* p, hipStream_t st, bool& ok) {
       hipMalloc(&p->pk_x, std::max<int64_t>(16, K.x_bytes)) != hipSuccess ||
       hipMalloc(&p->pk_t, std::max<int64_t>(16, K.t_bytes)) != hipSuccess)
     return fail(TJSQ_ERR_NOMEM, "packed BF16x3 allocation failed");
-  // the operator copies, packed once from the FP64 arena
-  std::vector<tjsq::PackJob> oj = K.ops_jobs;
-  for (tjsq::PackJob& j : oj) j.dst_off = (int64_t)(uintptr_t)(p->pk_ops + j.dst_off);
-  tjsq::PackJob* ojobs = nullptr;
-  if (!oj.empty()) {
-    if (hipMalloc(&ojobs, sizeof(tjsq::PackJob) * oj.size()) != hipSuccess)
-      return fail(TJSQ_ERR_NOMEM, "packed BF16x3 allocation failed");
-    HIPCHK(hipMemcpyAsync(ojobs, oj.data(), sizeof(tjsq::PackJob) * oj.size(), hipMemcpyHostToDevice, st));
-    const int nj = (int)oj.size();
-    hipLaunchKernelGGL(k_pack<double>, dim3(std::min(nj, 8192)), dim3(256), 0, st, ojobs, nj, p->arena);
-  }
-  const hipError_t e = hipGetLastError();
-  const hipError_t e2 = hipStreamSynchronize(st);
-  (void)hipFree(ojobs);
-  if (e != hipSuccess || e2 != hipSuccess) return fail(TJSQ_ERR_HIP, "packing the BF16x3 operators failed");
+  int rc = pack_ops(p, K.ops_jobs, st);
+  if (rc) return rc;
   tjsq::PackedPlan D;
   D.ops_bytes = K.ops_bytes; D.x_bytes = K.x_bytes; D.t_bytes = K.t_bytes;
   pk_append_device(K, p->pk_ops, p->pk_x, p->pk_t, 0, D);
   D.ops_jobs = K.ops_jobs;   // (host form, kept for batches that copy this plan's operators)
-  int rc = pk_upload(p, D, st);
+  rc = pk_upload(p, D, st);
   if (rc) return rc;
   p->pk_host = std::move(K);
   p->pk = std::move(D);
   ok = true;
   return TJSQ_OK;
+}
+
+// hmerge (get_sys.f90:4383-4593) on the device: the merged operators of the
+// plan's recipe, summed into the arena in the recipe's accumulation order
+int run_merge(tjsq_plan* p, hipStream_t st) {
+  const HostPlan& hp = p->hp;
+  std::vector<MergeDst> dsts;
+  std::vector<tjsq::MergeTerm> terms = hp.merge;
+  // group recipe terms by destination (recipes are emitted in accumulation
+  // order; stable grouping keeps that order per destination)
+  std::stable_sort(terms.begin(), terms.end(),
+                   [](const tjsq::MergeTerm& a, const tjsq::MergeTerm& b) { return a.dst_off < b.dst_off; });
+  // every merged sub-block (also those without terms) must be zeroed
+  const int64_t mstart = hp.merged_base, mend = hp.arena_elems;
+  if (mend > mstart) HIPCHK(hipMemsetAsync(p->arena + mstart, 0, sizeof(double) * (mend - mstart), st));
+  for (size_t t = 0; t < terms.size();) {
+    size_t u = t;
+    while (u < terms.size() && terms[u].dst_off == terms[t].dst_off) ++u;
+    dsts.push_back({terms[t].dst_off, terms[t].n, (int)t, (int)u});
+    t = u;
+  }
+  if (!dsts.empty()) {
+    MergeDst* dd = nullptr;
+    tjsq::MergeTerm* dt = nullptr;
+    if (hipMalloc(&dd, sizeof(MergeDst) * dsts.size()) != hipSuccess ||
+        hipMalloc(&dt, sizeof(tjsq::MergeTerm) * terms.size()) != hipSuccess) {
+      (void)hipFree(dd);
+      return fail(TJSQ_ERR_NOMEM, "merge table allocation failed");
+    }
+    HIPCHK(hipMemcpyAsync(dd, dsts.data(), sizeof(MergeDst) * dsts.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dt, terms.data(), sizeof(tjsq::MergeTerm) * terms.size(), hipMemcpyHostToDevice, st));
+    int64_t maxn = 1;
+    for (auto& d : dsts) maxn = std::max(maxn, d.n);
+    dim3 grid((unsigned)std::min<int64_t>((maxn + 255) / 256, 64), (unsigned)std::min<size_t>(dsts.size(), 32768));
+    hipLaunchKernelGGL(k_merge, grid, dim3(256), 0, st, p->arena, dd, dt, (int)dsts.size());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    (void)hipFree(dd);
+    (void)hipFree(dt);
+  }
+  return TJSQ_OK;
+}
+
+// the FP32 / BF16x3 operand copies of the FP64 arena a plan holds
+int refresh_copies(tjsq_plan* p, hipStream_t st) {
+  const HostPlan& hp = p->hp;
+  if (p->arena32) {
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(hp.arena_elems)), dim3(256), 0, st, p->arena, p->arena32,
+                       hp.arena_elems);
+    HIPCHK(hipGetLastError());
+  }
+  if (p->packed) {
+    int rc = pack_ops(p, p->pk_host.ops_jobs, st);
+    if (rc) return rc;
+  }
+  if (p->arenab) {
+    hipLaunchKernelGGL(k_f64_to_split, dim3(grid_for(hp.arena_elems)), dim3(256), 0, st, p->arena, p->arenab,
+                       hp.arena_elems);
+    HIPCHK(hipGetLastError());
+  }
+  return TJSQ_OK;
+}
+
+// FNV-1a over an operator set's layout (length, header, sub-blocks, offsets)
+uint64_t layout_hash(const tjsq_opset& o, uint64_t h) {
+  auto mix = [&](const void* d, size_t n) {
+    const unsigned char* b = (const unsigned char*)d;
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+  };
+  mix(&o.length, sizeof(int));
+  mix(&o.nops, sizeof(int));
+  mix(&o.ndata, sizeof(int64_t));
+  int64_t nsub = 0;
+  for (int k = 0; k < o.nops; ++k) nsub += o.hdr[4 * k + 3];
+  mix(o.hdr, sizeof(int) * 4 * (size_t)o.nops);
+  mix(o.sub, sizeof(int) * 5 * (size_t)nsub);
+  mix(o.off, sizeof(int64_t) * (size_t)nsub);
+  return h;
 }
 
 // apply of plan p (or a batch's merged plan) in one precision: FP64 on
@@ -1477,46 +1563,16 @@ int tjsq_plan_create_ex(const tjsq_lattice* lat, int sys_len, int env_len, const
     return fail(code, m);
   };
   (void)hipGetDevice(&p->device);
+  p->layout_hash = layout_hash(*env, layout_hash(*sys, 1469598103934665603ull));
   if (hipMalloc(&p->arena, sizeof(double) * std::max<int64_t>(1, hp.arena_elems)) != hipSuccess)
     return cleanup(TJSQ_ERR_NOMEM, "operator arena allocation failed");
   if ((rc = upload(p, *sys, hp.sys_base, st)) || (rc = upload(p, *env, hp.env_base, st))) {
     tjsq_plan_destroy(p);
     return rc;
   }
-  // merged operators
-  {
-    std::vector<MergeDst> dsts;
-    std::vector<tjsq::MergeTerm> terms = hp.merge;
-    // group recipe terms by destination (recipes are emitted in accumulation
-    // order; stable grouping keeps that order per destination)
-    std::stable_sort(terms.begin(), terms.end(),
-                     [](const tjsq::MergeTerm& a, const tjsq::MergeTerm& b) { return a.dst_off < b.dst_off; });
-    // every merged sub-block (also those without terms) must be zeroed
-    const int64_t mstart = hp.merged_base, mend = hp.arena_elems;
-    if (mend > mstart) HIPCHK(hipMemsetAsync(p->arena + mstart, 0, sizeof(double) * (mend - mstart), st));
-    for (size_t t = 0; t < terms.size();) {
-      size_t u = t;
-      while (u < terms.size() && terms[u].dst_off == terms[t].dst_off) ++u;
-      dsts.push_back({terms[t].dst_off, terms[t].n, (int)t, (int)u});
-      t = u;
-    }
-    if (!dsts.empty()) {
-      MergeDst* dd = nullptr;
-      tjsq::MergeTerm* dt = nullptr;
-      if (hipMalloc(&dd, sizeof(MergeDst) * dsts.size()) != hipSuccess ||
-          hipMalloc(&dt, sizeof(tjsq::MergeTerm) * terms.size()) != hipSuccess)
-        return cleanup(TJSQ_ERR_NOMEM, "merge table allocation failed");
-      HIPCHK(hipMemcpyAsync(dd, dsts.data(), sizeof(MergeDst) * dsts.size(), hipMemcpyHostToDevice, st));
-      HIPCHK(hipMemcpyAsync(dt, terms.data(), sizeof(tjsq::MergeTerm) * terms.size(), hipMemcpyHostToDevice, st));
-      int64_t maxn = 1;
-      for (auto& d : dsts) maxn = std::max(maxn, d.n);
-      dim3 grid((unsigned)std::min<int64_t>((maxn + 255) / 256, 64), (unsigned)std::min<size_t>(dsts.size(), 32768));
-      hipLaunchKernelGGL(k_merge, grid, dim3(256), 0, st, p->arena, dd, dt, (int)dsts.size());
-      HIPCHK(hipGetLastError());
-      HIPCHK(hipStreamSynchronize(st));
-      (void)hipFree(dd);
-      (void)hipFree(dt);
-    }
+  if ((rc = run_merge(p, st))) {
+    tjsq_plan_destroy(p);
+    return rc;
   }
   if (precision_mask & (1 << TJSQ_FP32)) {
     if (hipMalloc(&p->arena32, sizeof(float) * std::max<int64_t>(1, hp.arena_elems)) != hipSuccess)
@@ -1549,6 +1605,18 @@ int tjsq_plan_create_ex(const tjsq_lattice* lat, int sys_len, int env_len, const
   }
   HIPCHK(hipStreamSynchronize(st));
   *plan = p;
+  return TJSQ_OK;
+}
+
+int tjsq_plan_set_ops(tjsq_plan* p, const tjsq_opset* sys, const tjsq_opset* env, void* stream) {
+  if (!p || !sys || !env) return fail(TJSQ_ERR_ARG, "tjsq_plan_set_ops: null argument");
+  if (layout_hash(*env, layout_hash(*sys, 1469598103934665603ull)) != p->layout_hash)
+    return fail(TJSQ_ERR_ARG, "tjsq_plan_set_ops: the operator layouts differ from the plan's");
+  hipStream_t st = (hipStream_t)stream;
+  int rc;
+  if ((rc = upload(p, *sys, p->hp.sys_base, st)) || (rc = upload(p, *env, p->hp.env_base, st))) return rc;
+  if ((rc = run_merge(p, st)) || (rc = refresh_copies(p, st))) return rc;
+  HIPCHK(hipStreamSynchronize(st));
   return TJSQ_OK;
 }
 

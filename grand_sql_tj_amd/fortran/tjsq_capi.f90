@@ -143,6 +143,13 @@ module tjsq_capi
       type(c_ptr), intent(out) :: plan
     end function
 
+    integer(c_int) function tjsq_plan_set_ops(plan, sys, env, stream) bind(C, name="tjsq_plan_set_ops")
+      import :: c_int, c_ptr, tjsq_opset
+      type(c_ptr), value :: plan
+      type(tjsq_opset), intent(in) :: sys, env
+      type(c_ptr), value :: stream
+    end function
+
     integer(c_int) function tjsq_plan_destroy(plan) bind(C, name="tjsq_plan_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: plan

@@ -12,7 +12,8 @@
 !        replace Get_density_matrix_sys_new / _env_new, den_mat.f90:84-134
 !   gpu_Get_truncation_operator(den_mat,trun_op,trun_err,num_state)
 !        replaces Get_truncation_operator, den_mat.f90:141-383 (kept_min,
-!        kept_max from pubdata; sets Entropys; writes no log files)
+!        kept_max from pubdata; sets Entropys; appends the reference's log
+!        lines)
 !   gpu_Get_Hamiltonian_Sys(pre,next,basis,trun) / _Env
 !        replace Get_Hamiltonian_Sys / _Env, get_sys.f90:4-478: pre becomes
 !        the projected model one site longer, next is left untouched (the
@@ -28,12 +29,30 @@
 ! sets.  Data stay on the host between calls, as in the reference: each call
 ! copies its inputs to the device and its outputs back.
 !
+! Plan cache: gpu_lanczos_memo_new and gpu_model_wave keep the last device
+! plan, keyed by the block lengths, the couplings and the layouts of the two
+! block models, their fused bases and the wavefunction's sectors.  A call
+! with the same key uploads the models' data into that plan
+! (tjsq_plan_set_ops: no host compile); any other key compiles a new plan
+! and frees the old one.  A reference build that swaps only model_wave, which
+! lanczos_memo_new calls once per iteration, so compiles once per bond.
+! gpu_release_plan frees the cached plan; tjsq_dropin_plan_builds counts the
+! compiles.
+!
+! Run logs as the reference writes them: gpu_lanczos_memo_new appends its
+! Energy_All.dat line (and prints it, eigen.f90:166-171; the time is the
+! solve's wall time), gpu_Get_truncation_operator appends to Entropy.dat,
+! entropy1.dat and Num_State.dat (den_mat.f90:299-317).
+!
 ! It is compiled inside the reference's build (it uses module pubdata,
 ! code/tjsq/common.f90) and calls libtjsq_hip.so through module tjsq_capi.
 ! The derived types are flattened into the C-ABI layout; the lattice
-! couplings are read from pubdata (uniform per bond class, as
-! code/tjsq/main.f90:35-47 sets them).  A nonzero status from the library
-! ends the program with `error stop`, where the reference would `stop`.
+! couplings are read from pubdata, where they must be uniform per bond class
+! (directions 1-4 and 5-8, directions 9-12 zero, esite the same on every
+! site), as code/tjsq/main.f90:35-47 sets them: other values end the program
+! with a message (pind / pinz / pint, which no routine on the path reads,
+! are not checked).  A nonzero status from the library ends the program with
+! `error stop`, where the reference would `stop`.
 module tjsq_dropin
   use iso_c_binding
   use pubdata
@@ -43,10 +62,20 @@ module tjsq_dropin
   public :: gpu_lanczos_memo_new, gpu_model_wave, tjsq_dropin_precision, gpu_set_bond
   public :: gpu_Get_density_matrix_sys_new, gpu_Get_density_matrix_env_new, gpu_Get_truncation_operator
   public :: gpu_Get_Hamiltonian_Sys, gpu_Get_Hamiltonian_Env, gpu_wave_trans_LR, gpu_wave_trans_RL
-  public :: gpu_measure_operator_dia
+  public :: gpu_measure_operator_dia, gpu_release_plan, tjsq_dropin_plan_builds
 
   ! contraction precision of the drop-in solve (TJSQ_FP64 by default)
   integer(c_int), save :: tjsq_dropin_precision = TJSQ_FP64
+
+  ! plan cache (one plan) and its key
+  type(c_ptr), save :: cached_plan = c_null_ptr
+  integer, save :: tjsq_dropin_plan_builds = 0
+  integer, save :: key_len(2) = -1
+  real(c_double), save :: key_coup(7) = 0.0d0
+  integer(c_int), allocatable, save :: key_shdr(:, :), key_ssub(:, :), key_ehdr(:, :), key_esub(:, :)
+  integer(c_int64_t), allocatable, save :: key_soff(:), key_eoff(:)
+  integer(c_int), allocatable, save :: key_bsec(:, :), key_bsub(:, :), key_besec(:, :), key_besub(:, :)
+  integer(c_int), allocatable, save :: key_wsec(:, :)
 
   ! block lengths of the current bond
   integer, save :: bond_sys_len = -1, bond_env_len = -1
@@ -163,10 +192,45 @@ contains
     end do
   end subroutine flatten_basis
 
+  ! the couplings the library takes: uniform per bond class on every site
+  ! (common.f90:36-39 holds them per direction and site)
+  subroutine check_uniform_couplings()
+    integer :: i, k
+    logical :: ok
+    ok = .true.
+    do i = 1, Num_site
+      do k = 1, neibt
+        if (k <= 4) then
+          ok = ok .and. jt(k, i) == jt(1, 1) .and. jd(k, i) == jd(1, 1)
+        else if (k <= 8) then
+          ok = ok .and. jt(k, i) == jt(5, 1) .and. jd(k, i) == jd(5, 1)
+        else
+          ok = ok .and. jt(k, i) == 0.0d0 .and. jd(k, i) == 0.0d0
+        end if
+      end do
+      ok = ok .and. esite(i) == esite(1)
+    end do
+    do k = 1, neibt
+      if (k <= 4) then
+        ok = ok .and. jn(k) == jn(1)
+      else if (k <= 8) then
+        ok = ok .and. jn(k) == jn(5)
+      else
+        ok = ok .and. jn(k) == 0.0d0
+      end if
+    end do
+    if (.not. ok) then
+      write(*, *) 'tjsq: the couplings jt / jd / jn / esite are not uniform per bond class; ', &
+                  'the GPU path takes directions 1-4 and 5-8 with one value each, 9-12 zero, one esite'
+      error stop 1
+    end if
+  end subroutine check_uniform_couplings
+
   ! couplings of pubdata (main.f90:35-47): bond class 1 = directions 1-4,
   ! class 2 = directions 5-8
   function lattice_from_pubdata() result(lat)
     type(tjsq_lattice) :: lat
+    call check_uniform_couplings()
     lat%ny = Ny
     lat%nx_tables = Nx
     lat%jt = [jt(1, 1), jt(5, 1)]
@@ -178,7 +242,27 @@ contains
     lat%up_bias = st_elec_up%up_dif
   end function lattice_from_pubdata
 
-  subroutine make_plan(sys, env, sys_bs, env_bs, wsec, nw, plan, fs, fe, bs, be)
+  logical function same_i2(a, b)
+    integer(c_int), allocatable, intent(in) :: a(:, :)
+    integer(c_int), intent(in) :: b(:, :)
+    same_i2 = .false.
+    if (.not. allocated(a)) return
+    if (any(shape(a) /= shape(b))) return
+    same_i2 = all(a == b)
+  end function same_i2
+
+  logical function same_i8(a, b)
+    integer(c_int64_t), allocatable, intent(in) :: a(:)
+    integer(c_int64_t), intent(in) :: b(:)
+    same_i8 = .false.
+    if (.not. allocated(a)) return
+    if (size(a) /= size(b)) return
+    same_i8 = all(a == b)
+  end function same_i8
+
+  ! the device plan of a bond: the cached one when the key matches (with the
+  ! models' data uploaded into it), else a new one that replaces it
+  subroutine get_plan(sys, env, sys_bs, env_bs, wsec, nw, plan, fs, fe, bs, be)
     type(Total_Model), intent(in) :: sys, env
     type(Total_Basis), intent(in) :: sys_bs, env_bs
     integer(c_int), intent(in), target :: wsec(:, :)
@@ -189,21 +273,57 @@ contains
     type(tjsq_lattice) :: lat
     type(tjsq_opset) :: os, oe
     type(tjsq_basis) :: cbs, cbe
+    real(c_double) :: coup(7)
     integer :: ns, ne
+    logical :: hit
     call flatten_model(sys, fs, ns)
     call flatten_model(env, fe, ne)
     call flatten_basis(sys_bs, bs)
     call flatten_basis(env_bs, be)
     lat = lattice_from_pubdata()
+    coup = [lat%jt(1), lat%jt(2), lat%jd(1), lat%jd(2), lat%jn(1), lat%jn(2), lat%esite]
     os = tjsq_opset(sys%len, ns, c_loc(fs%hdr), c_loc(fs%sub), c_loc(fs%off), c_loc(fs%data), &
                     int(size(fs%data), c_int64_t), 0)
     oe = tjsq_opset(env%len, ne, c_loc(fe%hdr), c_loc(fe%sub), c_loc(fe%off), c_loc(fe%data), &
                     int(size(fe%data), c_int64_t), 0)
+    hit = c_associated(cached_plan) .and. key_len(1) == sys%len .and. key_len(2) == env%len
+    if (hit) hit = all(key_coup == coup) .and. same_i2(key_shdr, fs%hdr) .and. same_i2(key_ssub, fs%sub)
+    if (hit) hit = same_i8(key_soff, fs%off) .and. same_i2(key_ehdr, fe%hdr) .and. same_i2(key_esub, fe%sub)
+    if (hit) hit = same_i8(key_eoff, fe%off) .and. same_i2(key_bsec, bs%sec) .and. same_i2(key_bsub, bs%sub)
+    if (hit) hit = same_i2(key_besec, be%sec) .and. same_i2(key_besub, be%sub) .and. same_i2(key_wsec, wsec(:, 1:nw))
+    if (hit) then
+      call check(tjsq_plan_set_ops(cached_plan, os, oe, c_null_ptr), 'tjsq_plan_set_ops')
+      plan = cached_plan
+      return
+    end if
+    call gpu_release_plan()
     cbs = tjsq_basis(sys_bs%len, sys_bs%num, c_loc(bs%sec), c_loc(bs%sub))
     cbe = tjsq_basis(env_bs%len, env_bs%num, c_loc(be%sec), c_loc(be%sub))
     call check(tjsq_plan_create(lat, sys%len, env%len, os, oe, cbs, cbe, int(nw, c_int), wsec, &
                                 int(7, c_int), c_null_ptr, plan), 'tjsq_plan_create')
-  end subroutine make_plan
+    tjsq_dropin_plan_builds = tjsq_dropin_plan_builds + 1
+    cached_plan = plan
+    key_len = [sys%len, env%len]
+    key_coup = coup
+    key_shdr = fs%hdr
+    key_ssub = fs%sub
+    key_soff = fs%off
+    key_ehdr = fe%hdr
+    key_esub = fe%sub
+    key_eoff = fe%off
+    key_bsec = bs%sec
+    key_bsub = bs%sub
+    key_besec = be%sec
+    key_besub = be%sub
+    key_wsec = wsec(:, 1:nw)
+  end subroutine get_plan
+
+  ! frees the cached plan (the next solve or apply compiles a new one)
+  subroutine gpu_release_plan()
+    if (c_associated(cached_plan)) call check(tjsq_plan_destroy(cached_plan), 'tjsq_plan_destroy')
+    cached_plan = c_null_ptr
+    key_len = -1
+  end subroutine gpu_release_plan
 
   subroutine pack_wave(w, x)
     type(Wavefunction), intent(in) :: w
@@ -265,7 +385,7 @@ contains
     end if
     call wave_sectors(eigvec(1), wsec)
     call gpu_set_bond(sys%len, env%len)
-    call make_plan(sys, env, sys_bs, env_bs, wsec, eigvec(1)%num, plan, fs, fe, bs, be)
+    call get_plan(sys, env, sys_bs, env_bs, wsec, eigvec(1)%num, plan, fs, fe, bs, be)
     allocate(psi(eigvec(1)%dim))
     call pack_wave(eigvec(1), psi)
     opts%tol = lanzero
@@ -278,7 +398,12 @@ contains
     call check(tjsq_lanczos_solve_host(plan, psi, opts, res), 'tjsq_lanczos_solve_host')
     call unpack_wave(psi, eigvec(1))
     Energys(1) = res%e0
-    call check(tjsq_plan_destroy(plan), 'tjsq_plan_destroy')
+    ! the reference's output line (eigen.f90:166-171)
+    open(11, file="Energy_All.dat", position='append')
+    write(11, 111) "Sys_len=", eigvec(1)%sys_len, "E0=", Energys(1), "Iter=", res%iterations, "time=", res%seconds
+    write(*, 111) "Sys_len=", eigvec(1)%sys_len, "E0=", Energys(1), "Iter=", res%iterations, "time=", res%seconds
+    close(11)
+111 format(A8,I3,2X,A3,2X,F18.12,2X,A5,I4,2X,A5,2X,F12.6)
   end subroutine gpu_lanczos_memo_new
 
   ! hmerge + model_wave (get_sys.f90:4383, 4127): outvec = H_eff invec
@@ -294,12 +419,11 @@ contains
     type(c_ptr) :: plan
     call wave_sectors(invec, wsec)
     call gpu_set_bond(sys%len, env%len)
-    call make_plan(sys, env, sys_bs, env_bs, wsec, invec%num, plan, fs, fe, bs, be)
+    call get_plan(sys, env, sys_bs, env_bs, wsec, invec%num, plan, fs, fe, bs, be)
     allocate(x(invec%dim), y(invec%dim))
     call pack_wave(invec, x)
     call check(tjsq_heff_apply_host(plan, x, y), 'tjsq_heff_apply_host')
     call unpack_wave(y, outvec)
-    call check(tjsq_plan_destroy(plan), 'tjsq_plan_destroy')
   end subroutine gpu_model_wave
 
   subroutine gpu_set_bond(sys_len, env_len)
@@ -458,7 +582,7 @@ contains
     integer(c_int64_t), allocatable :: toff(:)
     integer(c_int) :: nt, hdr(4)
     type(tjsq_trunc_info) :: info
-    integer :: n, s, dim
+    integer :: n, s, dim, slen
     call flatten_block(den_mat, f)
     n = max(1, count_subs(den_mat))
     allocate(tsub(5, n), toff(n))
@@ -473,6 +597,19 @@ contains
     call block_from_flat(trun_op, hdr, tsub, toff, f%data, 0, dim)
     num_state = info%num_state
     Entropys = info%entropy
+    ! the reference's logs (den_mat.f90:299-317)
+    slen = den_mat%len
+    if (.not. system_block) slen = Num_site - den_mat%len
+    open(60, file="Entropy.dat", position='append')
+    write(60, *) "t=", jt(1, 1), jt(7, 1), "J=", jz(1, 1), jz(7, 1), "N=", Num_site
+    close(60)
+    open(60, file="entropy1.dat", position='append')
+    write(60, 22) slen, info%entropy1, info%entropy2, info%entropy, Num_site, tot_num_up, tot_num_down
+    close(60)
+22  format(i8, 3f18.8, 3i9)
+    open(60, file="Num_State.dat", position='append')
+    write(60, *) "Sys_len=", slen, "Num_State=", info%num_state, "Err=", info%trunc_err
+    close(60)
   end subroutine gpu_Get_truncation_operator
 
   ! Get_Hamiltonian_Sys (side 0) / _Env (side 1), get_sys.f90:4-478

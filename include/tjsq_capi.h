@@ -217,6 +217,17 @@ int tjsq_plan_create_ex(const tjsq_lattice* lat, int sys_len, int env_len, const
  * operators on the device (no launch of the plan may be in flight). */
 int tjsq_plan_reshape(tjsq_plan* plan, const tjsq_shape* shape, void* stream);
 
+/* New operator data for a plan whose block layouts are unchanged (the same
+ * hdr / sub / off of both sets as at tjsq_plan_create; TJSQ_ERR_ARG
+ * otherwise): the data are uploaded, hmerge is run again and the FP32 /
+ * BF16x3 copies are rebuilt, with no host compile.  No reference
+ * counterpart: the reference's model_wave and lanczos_memo_new
+ * (code/tjsq/get_sys.f90:4127-4134, code/tjsq/eigen.f90:2-10) read the block
+ * models on every call; this lets a caller that holds a plan per bond
+ * follow the models as a sweep rebuilds them (the Fortran drop-in's plan
+ * cache). */
+int tjsq_plan_set_ops(tjsq_plan* plan, const tjsq_opset* sys, const tjsq_opset* env, void* stream);
+
 int tjsq_plan_stats_get(const tjsq_plan* plan, tjsq_plan_stats* stats);
 int tjsq_plan_destroy(tjsq_plan* plan);
 

@@ -115,8 +115,8 @@ integer(c_int) function dropin_density(side, sys_len, env_len, nw, wsec, levels,
 end function dropin_density
 
 ! Get_truncation_operator through the drop-in (as refb_truncation).
-integer(c_int) function dropin_truncation(den_hdr, den_sub, den_off, den_data, kmin, kmax, terr, num_state, &
-    entropy) bind(C, name="dropin_truncation")
+integer(c_int) function dropin_truncation(den_hdr, den_sub, den_off, den_data, kmin, kmax, terr, sysblk, &
+    num_state, entropy) bind(C, name="dropin_truncation")
   use iso_c_binding
   use pubdata
   use ref_bridge_util
@@ -125,7 +125,7 @@ integer(c_int) function dropin_truncation(den_hdr, den_sub, den_off, den_data, k
   integer(c_int), intent(in) :: den_hdr(4), den_sub(5, *)
   integer(c_int64_t), intent(in) :: den_off(*)
   real(c_double), intent(in) :: den_data(*)
-  integer(c_int), value :: kmin, kmax
+  integer(c_int), value :: kmin, kmax, sysblk
   real(c_double), value :: terr
   integer(c_int), intent(out) :: num_state
   real(c_double), intent(out) :: entropy
@@ -134,6 +134,7 @@ integer(c_int) function dropin_truncation(den_hdr, den_sub, den_off, den_data, k
   call put_block(den, den_hdr(1), den_hdr, den_sub, den_off, den_data)
   kept_min = kmin
   kept_max = kmax
+  system_block = sysblk /= 0
   call set_result(1)
   ns = 0
   call gpu_Get_truncation_operator(den, res(1), terr, ns)
@@ -254,3 +255,30 @@ integer(c_int) function dropin_measure_density(sys_len, env_len, nw, wsec, x, tr
   call deallocate_wave(w)
   dropin_measure_density = Num_site
 end function dropin_measure_density
+
+! the drop-in's plan cache: compiles so far, and release of the cached plan
+integer(c_int) function dropin_plan_builds() bind(C, name="dropin_plan_builds")
+  use iso_c_binding
+  use tjsq_dropin
+  implicit none
+  dropin_plan_builds = tjsq_dropin_plan_builds
+end function dropin_plan_builds
+
+integer(c_int) function dropin_release_plan() bind(C, name="dropin_release_plan")
+  use iso_c_binding
+  use tjsq_dropin
+  implicit none
+  call gpu_release_plan()
+  dropin_release_plan = 0
+end function dropin_release_plan
+
+! sets jt(k, i) of pubdata (a non-uniform coupling for the rejection test)
+integer(c_int) function dropin_set_jt(k, i, v) bind(C, name="dropin_set_jt")
+  use iso_c_binding
+  use pubdata
+  implicit none
+  integer(c_int), value :: k, i
+  real(c_double), value :: v
+  jt(k, i) = v
+  dropin_set_jt = 0
+end function dropin_set_jt

@@ -375,7 +375,13 @@ def dropin_lib():
             lib.dropin_density.argtypes = ref.refb_density.argtypes
             lib.dropin_get_hamiltonian.argtypes = ref.refb_get_hamiltonian.argtypes
             lib.dropin_wave_trans.argtypes = ref.refb_wave_trans.argtypes
-            lib.dropin_truncation.argtypes = ref.refb_truncation.argtypes[:7] + ref.refb_truncation.argtypes[8:]
+            lib.dropin_truncation.argtypes = ref.refb_truncation.argtypes
+            lib.dropin_plan_builds.restype = C.c_int
+            lib.dropin_plan_builds.argtypes = []
+            lib.dropin_release_plan.restype = C.c_int
+            lib.dropin_release_plan.argtypes = []
+            lib.dropin_set_jt.restype = C.c_int
+            lib.dropin_set_jt.argtypes = [C.c_int, C.c_int, C.c_double]
             _libs["dropin"] = lib
         return _libs["dropin"]
 
@@ -600,12 +606,37 @@ def dropin_density_matrix(side, sys_len, env_len, wsec, psi, levels=1):
         return _take_ref()
 
 
-def dropin_truncation(den, kept_min, kept_max, trun_err):
-    """Get_truncation_operator through the drop-in (as ref_truncation)."""
+def dropin_truncation(den, kept_min, kept_max, trun_err, workdir=None, system_block=True):
+    """Get_truncation_operator through the drop-in (as ref_truncation; its
+    log files go to ``workdir``, default the current directory)."""
     ns, ent = C.c_int(0), C.c_double(0.0)
     with _lock:
-        dropin_lib().dropin_truncation(*_one(den), kept_min, kept_max, trun_err, C.byref(ns), C.byref(ent))
+        cwd = os.getcwd()
+        if workdir:
+            os.chdir(workdir)
+        try:
+            dropin_lib().dropin_truncation(*_one(den), kept_min, kept_max, trun_err, int(system_block), C.byref(ns),
+                                           C.byref(ent))
+        finally:
+            os.chdir(cwd)
         return _take_ref(), ns.value, ent.value
+
+
+def dropin_plan_builds() -> int:
+    """Plans the drop-in has compiled so far (its plan cache's misses)."""
+    with _lock:
+        return int(dropin_lib().dropin_plan_builds())
+
+
+def dropin_release_plan():
+    with _lock:
+        dropin_lib().dropin_release_plan()
+
+
+def dropin_set_jt(k, i, v):
+    """jt(k, i) of the reference's pubdata (1-based direction and site)."""
+    with _lock:
+        dropin_lib().dropin_set_jt(k, i, v)
 
 
 def dropin_get_hamiltonian(model, side, sys_len, env_len, pre, basis, trun):

@@ -50,6 +50,59 @@ def test_dropin_lanczos_matches_reference(tmp_path):
     assert abs(abs(float(np.dot(psi_gpu, psi_ref))) - 1.0) < 1e-8
 
 
+@needs
+def test_dropin_lanczos_writes_the_reference_energy_line(tmp_path):
+    # Energy_All.dat (eigen.f90:166-171): Sys_len, E0, iterations, time
+    m = model()
+    sb = small_superblock(m, 11, 11, seed=4, msys=30, menv=30, median=4)
+    x0 = synth.random_vector(sb, 1)
+    cwd = os.getcwd()
+    lines = {}
+    for who in ("ref", "gpu"):
+        (tmp_path / who).mkdir()
+        os.chdir(tmp_path / who)
+        try:
+            (oracle.ref_lanczos if who == "ref" else oracle.dropin_lanczos)(m, sb, x0, 1e-10)
+        finally:
+            os.chdir(cwd)
+        lines[who] = (tmp_path / who / "Energy_All.dat").read_text().splitlines()
+    assert len(lines["gpu"]) == 1 and len(lines["ref"]) == 1
+    g, r = lines["gpu"][0], lines["ref"][0]
+    # format (A8,I3,2X,A3,2X,F18.12,2X,A5,I4,2X,A5,2X,F12.6)
+    assert len(g) == len(r) and g[:18] == r[:18]                 # "Sys_len= 12  E0=  "
+    assert abs(float(g[18:36]) - float(r[18:36])) <= 1e-10 * abs(float(r[18:36]))
+    assert g[38:43] == "Iter=" and int(g[43:47]) > 0 and g[49:54] == "time="
+
+
+@needs
+def test_dropin_plan_cache_compiles_once_per_bond():
+    # lanczos_memo_new calls model_wave once per iteration at one bond: the
+    # drop-in compiles one plan there, then only uploads the models' data
+    import dataclasses
+    from grand_sql_tj_amd.blocks import OpSet
+    m = model()
+    sb = small_superblock(m, 11, 11, seed=61)
+    x = synth.random_vector(sb, 3)
+    oracle.dropin_release_plan()
+    n0 = oracle.dropin_plan_builds()
+    ys = [oracle.dropin_model_wave(m, sb, x) for _ in range(3)]
+    assert oracle.dropin_plan_builds() == n0 + 1
+    assert all(np.array_equal(y, ys[0]) for y in ys)
+    assert rel(ys[0], oracle.ref_model_wave(m, sb, x)) < 1e-13
+    # the same bond with rebuilt models (new data, same layouts): no compile
+    scaled = lambda o, f: OpSet(o.length, o.hdr, o.sub, o.off, o.data * f)
+    sb2 = dataclasses.replace(sb, sys=scaled(sb.sys, 0.7), env=scaled(sb.env, 1.3))
+    y2 = oracle.dropin_model_wave(m, sb2, x)
+    assert oracle.dropin_plan_builds() == n0 + 1
+    assert rel(y2, oracle.ref_model_wave(m, sb2, x)) < 1e-13
+    # another bond: a new plan replaces the cached one
+    sb3 = small_superblock(m, 7, 9, seed=62)
+    x3 = synth.random_vector(sb3, 1)
+    assert rel(oracle.dropin_model_wave(m, sb3, x3), oracle.ref_model_wave(m, sb3, x3)) < 1e-13
+    assert oracle.dropin_plan_builds() == n0 + 2
+    oracle.dropin_release_plan()
+
+
 def _same_layout(a, b):
     assert np.asarray(a.hdr).shape == np.asarray(b.hdr).shape
     assert (np.asarray(a.hdr) == np.asarray(b.hdr)).all()
@@ -70,14 +123,28 @@ def test_dropin_density_matrix_matches_reference(side, levels):
 
 
 @needs
-@pytest.mark.parametrize("kmin,kmax,terr", [(20, 60, 1e-6), (5, 200, 1e-3)])
-def test_dropin_truncation_matches_reference(kmin, kmax, terr, tmp_path):
+@pytest.mark.parametrize("kmin,kmax,terr,sysblk", [(20, 60, 1e-6, True), (5, 200, 1e-3, False)])
+def test_dropin_truncation_matches_reference(kmin, kmax, terr, sysblk, tmp_path):
     m = model()
     sb = small_superblock(m, 11, 11, seed=9, msys=160, menv=150, median=14, nsec=10)
     den = oracle.ref_density_matrix(0, 12, 12, sb.wsec, synth.random_vector(sb, 4), 1)
-    r, ns_r, ent_r = oracle.ref_truncation(den, kmin, kmax, terr, str(tmp_path))
-    d, ns_d, ent_d = oracle.dropin_truncation(den, kmin, kmax, terr)
+    (tmp_path / "ref").mkdir()
+    (tmp_path / "gpu").mkdir()
+    r, ns_r, ent_r = oracle.ref_truncation(den, kmin, kmax, terr, str(tmp_path / "ref"), system_block=sysblk)
+    d, ns_d, ent_d = oracle.dropin_truncation(den, kmin, kmax, terr, str(tmp_path / "gpu"), system_block=sysblk)
     assert ns_d == ns_r
+    # the reference's three log files (den_mat.f90:299-317), line for line
+    for name in ("Entropy.dat", "entropy1.dat", "Num_State.dat"):
+        lr = (tmp_path / "ref" / name).read_text().split()
+        lg = (tmp_path / "gpu" / name).read_text().split()
+        assert len(lg) == len(lr) and lg, name
+        for a, b in zip(lg, lr):
+            try:
+                fa, fb = float(a), float(b)
+            except ValueError:
+                assert a == b, name
+                continue
+            assert abs(fa - fb) <= 1e-7 * max(1.0, abs(fb)) + 1e-12, (name, a, b)
     assert abs(ent_d - ent_r) <= 1e-12 * max(1.0, abs(ent_r))
     _same_layout(d, r)
     for s in range(np.asarray(d.sub).shape[0]):
