@@ -16,12 +16,19 @@ basis_to_disk); here the operator data stay in device memory and only their
 layouts live on the host.  The reference's random start vectors
 (wave_initialize, unseeded random_number) are seeded here.
 
-The schedule is the reference's with fixed settings: warm-up to half the
-lattice (the blocks grow together, with truncation once an enlarged block
-exceeds kept_max), then left-to-right and right-to-left half sweeps between
-the block lengths 1 and N - 3, `sweeps` times.  The tolerance and kept-state
-ramps of Get_Ground_State are left out: every step uses kept_max, kept_min
-and trun_err as given.
+Two schedules:
+- run(sweeps): warm-up to half the lattice (the blocks grow together, with
+  truncation once an enlarged block exceeds kept_max), then left-to-right
+  and right-to-left half sweeps between the block lengths 1 and N - 3,
+  `sweeps` times, every step with the tol, kept_max, kept_min and trun_err
+  given;
+- ground_state(...): Get_Ground_State's schedule (main.f90:96-187, a fresh
+  run: restart1 = restart2 = 0, iter1 = 0) with its ramps of the Lanczos
+  tolerance (lanerr x 1000 in the warm-up, x 30 in the first two half sweeps,
+  then lanerr), of the truncation error (trun_err x 200, x 15, then
+  trun_err) and of kept_max (kept_max, then 2/3 kept_max + 1/3 kept_maxf,
+  then kept_maxf, with kept_min = min(3 kept_max / 4, 3600)), and its closing
+  left-to-right pass to the block length Num_site0 / 2 + 3.
 """
 from __future__ import annotations
 
@@ -369,6 +376,39 @@ class DMRG:
         sl, el = self.bond[0] + 1, self.bond[1] + 1
         return measure.correlation_matrix(self.model.up_bias, sl, el, self.sys_basis, self.env_basis, self.sys_trun,
                                           self.env_trun, self.wsec, self.psi)
+
+    def ground_state(self, lanerr, trun_err, kept_max, kept_maxf, sweep_num, nsite0=None):
+        """Get_Ground_State (main.f90:96-187) for a fresh run (restart1 =
+        restart2 = 0, iter1 = 0), with the kept-state settings of main.f90:
+        13-33 (kept_min = min(1000, 3 kept_max / 4), kept_max capped at 1200).
+        nsite0: Num_site0 of the reference, Nleg (Nx + 2) (default), which
+        sets where the closing left-to-right pass ends.  Returns the last
+        ground-state energy."""
+        N = self.nsite
+        if nsite0 is None:
+            nsite0 = self.model.ny * (self.model.nx + 2)
+        self.kept_max = min(1200, kept_max)
+        self.kept_min = min(1000, kept_max * 3 // 4)
+        # warm-up
+        self.tol, self.trun_err = lanerr * 1000, trun_err * 200
+        self.warm_up()
+        self.tol, self.trun_err = lanerr * 30, trun_err * 15
+        self.kept_max = self.kept_max * 2 // 3 + kept_maxf // 3
+        self.kept_min = min(3 * self.kept_max // 4, 3600)
+        # the first half sweep starts from a random vector, as left_to_right
+        # does with restart1 = 0 (dmrg_sub.f90:763-766)
+        self.wsec = None
+        self.left_to_right(N // 2 - 1, N - 2)
+        self.kept_min = min(3 * self.kept_max // 4, 3600)
+        self.right_to_left(N - 2, 1)
+        self.kept_max = kept_maxf
+        self.kept_min = min(3 * self.kept_max // 4, 3600)
+        self.tol, self.trun_err = lanerr, trun_err
+        for _ in range(sweep_num):
+            self.left_to_right(1, N - 2)
+            self.right_to_left(N - 2, 1)
+        self.left_to_right(1, nsite0 // 2 + 3)
+        return self.log[-1].energy
 
     def run(self, sweeps=2):
         """Warm-up, then `sweeps` pairs of half sweeps (Get_Ground_State with

@@ -141,3 +141,24 @@ def test_apply_work_item_shapes_match_oracle(shape):
     assert rel(p.apply(_cuda(x)).cpu().numpy(), ref) < 1e-13
     y = p.apply(_cuda(x, torch.float32), precision="bf16x3").double().cpu().numpy()
     assert rel(y, ref) < 2e-5
+
+
+def test_plan_set_ops_equals_a_new_plan():
+    # new block data with the same layouts (a later sweep at the same bond):
+    # every precision's apply bitwise that of a plan compiled for the new data
+    import dataclasses
+    from grand_sql_tj_amd.blocks import OpSet
+    m = model()
+    sb = small_superblock(m, 11, 11, seed=13, msys=80, menv=90, median=8)
+    scaled = lambda o, f: OpSet(o.length, o.hdr, o.sub, o.off, o.data * f)
+    sb2 = dataclasses.replace(sb, sys=scaled(sb.sys, 0.6), env=scaled(sb.env, -1.2))
+    p = capi.Plan(m, sb)
+    p.set_ops(sb2.sys, sb2.env)
+    q = capi.Plan(m, sb2)
+    x = _cuda(synth.random_vector(sb, 2))
+    for prec in ("fp64", "fp32", "bf16x3"):
+        xx = x if prec == "fp64" else x.float()
+        assert torch.equal(p.apply(xx, precision=prec), q.apply(xx, precision=prec)), prec
+    other = small_superblock(m, 7, 9, seed=14)
+    with pytest.raises(RuntimeError, match="layouts differ"):
+        p.set_ops(other.sys, other.env)
