@@ -608,12 +608,10 @@ __device__ __forceinline__ void ld_op(const PSide& s, int b, uint32_t ko, u4& hi
   lo = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(s.r, (int)(s.v[b] + ko + 16u), 0, 0));
 }
 
+// one output tile of a packed stage (64 lanes)
 template <int STAGE>
-__global__ __launch_bounds__(64, 2) void k_packed(const tjsq::PTile* __restrict__ tiles, const tjsq::PSeg* __restrict__ segs,
-                                                 PSpaces S, float* __restrict__ partial, int ntiles) {
-  const int bid = blockIdx.x;
-  if (bid >= ntiles) return;
-  const tjsq::PTile tl = tiles[bid];
+__device__ __forceinline__ void packed_tile(const tjsq::PTile& tl, const tjsq::PSeg* __restrict__ segs, const PSpaces& S,
+                                            float* __restrict__ partial) {
   const int lane = threadIdx.x, li = lane & 15, lq = lane >> 4;
   const int mb = tl.mb, nb = tl.nb;
   f4 acc[4][4];
@@ -778,6 +776,16 @@ __global__ __launch_bounds__(64, 2) void k_packed(const tjsq::PTile* __restrict_
       for (int r = 0; r < 4; ++r)
         if (row0 + 16 * bi + r < tl.mlim) d[r] = acc[bi][bj][r];
     }
+}
+
+// one workgroup per tile
+template <int STAGE>
+__global__ __launch_bounds__(64, 2) void k_packed(const tjsq::PTile* __restrict__ tiles, const tjsq::PSeg* __restrict__ segs,
+                                                 PSpaces S, float* __restrict__ partial, int ntiles) {
+  const int bid = blockIdx.x;
+  if (bid >= ntiles) return;
+  const tjsq::PTile tl = tiles[bid];
+  packed_tile<STAGE>(tl, segs, S, partial);
 }
 
 // ordered sum of the partial slots of split packed tiles
@@ -1258,9 +1266,6 @@ int pk_upload(tjsq_plan* p, const tjsq::PackedPlan& D, hipStream_t st) {
   return TJSQ_OK;
 }
 
-// the packed BF16x3 tables and operator copies of a plan; ok = false (and
-// the plan keeps the unpacked BF16x3 operands) where the layout does not
-// apply or a space would exceed the 2 GiB a buffer resource addresses
 // the packed operator copies, packed from the FP64 arena (jobs: host form,
 // destinations relative to pk_ops)
 int pack_ops(tjsq_plan* p, const std::vector<tjsq::PackJob>& jobs, hipStream_t st) {
@@ -1280,6 +1285,9 @@ int pack_ops(tjsq_plan* p, const std::vector<tjsq::PackJob>& jobs, hipStream_t s
   return TJSQ_OK;
 }
 
+// the packed BF16x3 tables and operator copies of a plan; ok = false (and
+// the plan keeps the unpacked BF16x3 operands) where the layout does not
+// apply or a space would exceed the 2 GiB a buffer resource addresses
 int setup_packed(tjsq_plan* p, hipStream_t st, bool& ok) {
   ok = false;
   std::string msg;
