@@ -532,6 +532,57 @@ def sweep_bench(args):
     print(json.dumps(line), flush=True)
 
 
+C1 = dict(ny=4, nx=8, t=1.0, tp=0.0, J=0.4, electrons=28, kept_max=500, kept_maxf=500, sweeps=6, lanerr=1e-10,
+          trun_err=1e-18)
+
+
+def c1_bench(args):
+    """C1 on the device: t-J 4 x 8 cylinder, t = 1, J = 0.4, 28 electrons,
+    S = 0, kept_max = kept_maxf = 500, 6 sweeps with Get_Ground_State's
+    schedule (sweep.DMRG.ground_state), FP64.  Reports the wall time and the
+    energies beside those of the reference's own C1 run (its routines built
+    for the 4 x 8 lattice, tests/golden/c1_reference.json, timed on one core
+    where that fixture was written)."""
+    import torch
+    from grand_sql_tj_amd import sweep
+    from grand_sql_tj_amd.lattice import TJModel
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
+    P = C1
+    model = TJModel(ny=P["ny"], nx=P["nx"], t=P["t"], tp=P["tp"], J=P["J"], mu=0.0, up_bias=1)
+    warm = sweep.DMRG(TJModel(ny=2, nx=4, t=1.0, J=0.4, up_bias=1), total_up=6, kept_max=64)
+    warm.run(sweeps=1)               # loads the library and the solver handles
+    d = sweep.DMRG(model, total_up=P["electrons"], kept_max=P["kept_max"], max_iter=110, seed=5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e = d.ground_state(P["lanerr"], P["trun_err"], P["kept_max"], P["kept_maxf"], P["sweeps"])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ref = None
+    path = os.path.join(ROOT, "tests", "golden", "c1_reference.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            ref = json.load(f)
+    energies = [r.energy for r in d.log]
+    line = {
+        "metric": "C1 ground-state run on the device (warm-up + 6 sweeps + closing pass, Get_Ground_State schedule)",
+        "value": len(d.log) / wall, "unit": "bonds/s", "n_gpus": 1, "steps": len(d.log), "warmup": 1,
+        "ms_per_step": 1e3 * wall / len(d.log), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "C1: t-J 4x8 cylinder (y periodic, x open), t=1, J=0.4, N=28, S=0, kept_max = "
+                               "kept_maxf = 500, 6 sweeps, lanerr 1e-10, trun_err 1e-18"},
+        "wall_s": wall, "final_e0": e, "min_e0": min(energies), "solves": len(d.log),
+        "phase_s": {k: round(v, 4) for k, v in d.phase_s.items()},
+        "reference_final_e0": ref["final_e0"] if ref else None, "reference_min_e0": ref["min_e0"] if ref else None,
+        "reference_solves": len(ref["steps"]) if ref else None,
+        "reference_seconds_1_core": ref["reference_seconds_1_core"] if ref else None,
+        "rel_diff_final": abs(e - ref["final_e0"]) / abs(ref["final_e0"]) if ref else None,
+        "max_superblock_dim": max(r.dim for r in d.log),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(model, sb, x0, n_iter, threads=1, budget_s=20.0):
     """The reference's own model_wave (hmerge + 14 kernels, compiled from its
     sources) timed on this host on a bounded sample: a few applies of the same
@@ -592,6 +643,9 @@ def main():
     ap.add_argument("--sweep-shape", default="4,12,44,1500", help="ny,nx,N,kept_max of the --sweep run")
     ap.add_argument("--sweeps", type=int, default=1, help="finite sweeps of the --sweep run after the warm-up")
     ap.add_argument("--sweep-tp", type=float, default=-0.2, help="t'/t of the --sweep run")
+    ap.add_argument("--c1", action="store_true",
+                    help="instead of the solve benchmark: the C1 ground-state run (4x8, N=28, m=500, 6 sweeps, "
+                         "Get_Ground_State schedule) on the device, against the reference run's energies")
     ap.add_argument("--apply-only", type=int, default=0,
                     help="profiling mode: a 512 MiB device copy (counter calibration) then N isolated applies")
     args = ap.parse_args()
@@ -608,6 +662,9 @@ def main():
         return
     if args.sweep:
         sweep_bench(args)
+        return
+    if args.c1:
+        c1_bench(args)
         return
     import torch
     from grand_sql_tj_amd import capi
