@@ -867,9 +867,9 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // (w's slice stays in cache); each wave reduces its sums by shuffles into
 // LDS, and after one barrier the block's four wave sums are added in a fixed
 // order, so the result does not depend on timing.
-template <typename T>
+template <typename T, typename TW = T>
 __global__ __launch_bounds__(RED_THREADS) void k_multidot(const T* __restrict__ V, int64_t ldv, int nv,
-                                                          const T* __restrict__ w, int64_t n, double* partial,
+                                                          const TW* __restrict__ w, int64_t n, double* partial,
                                                           int with_self) {
   constexpr int G = 4;
   __shared__ double red[2][RED_THREADS / 64][RED_MAXV];
@@ -925,14 +925,26 @@ __global__ __launch_bounds__(RED_THREADS) void k_finish_dots(const double* parti
 }
 
 // w = (w - sum_j coef_j V_j) * post_scale ; coefficients from device memory
-template <typename T>
-__global__ void k_combine(T* __restrict__ w, const T* __restrict__ V, int64_t ldv, int nv, const double* coef,
+template <typename T, typename TV = T>
+__global__ void k_combine(T* __restrict__ w, const TV* __restrict__ V, int64_t ldv, int nv, const double* coef,
                           int64_t n, double post_scale, int accumulate_into_zero) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double acc = accumulate_into_zero ? 0.0 : (double)w[i];
     double sub = 0.0;
     for (int j = 0; j < nv; ++j) sub += coef[j] * (double)V[(int64_t)j * ldv + i];
     w[i] = (T)((acc - sub) * post_scale);
+  }
+}
+
+// w = src - sum_j coef_j V_j in FP64 into an FP64 w (src and V in the
+// basis type)
+template <typename TV>
+__global__ void k_combine_from(double* __restrict__ w, const TV* __restrict__ src, const TV* __restrict__ V,
+                               int64_t ldv, int nv, const double* coef, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double sub = 0.0;
+    for (int j = 0; j < nv; ++j) sub += coef[j] * (double)V[(int64_t)j * ldv + i];
+    w[i] = (double)src[i] - sub;
   }
 }
 
@@ -1923,6 +1935,7 @@ struct Krylov {
   double* coef;      // device coefficients
   double* ab;        // device alpha_j at 2j-1, beta_j at 2j (the three-term coefficients are adjacent)
   double* tmp;       // device scratch scalars
+  double* W = nullptr;   // FP64 working vector of a solve whose basis is FP32
   std::vector<double> hbuf;
 
   T* vec(int j) { return V + (int64_t)j * n; }
@@ -1982,19 +1995,11 @@ int apply_prec(tjsq_plan* p, int precision, const void* x, void* y, hipStream_t 
   return tjsq_heff_apply_f32(p, precision, (const float*)x, (float*)y, st);
 }
 
-// y = H x on FP64 vectors with the contractions in `precision`: FP32 /
-// BF16x3 applies read an FP32 copy of x and write an FP32 y, widened to FP64
-// (the mixed solve's Krylov basis, and so its reorthogonalisation, is FP64)
+// y = H x on FP64 vectors (the FP64 solve and the FP64 restarts; a mixed
+// solve's basis is FP32 and its apply reads it directly)
 int apply_f64(tjsq_plan* p, int precision, const double* x, double* y, hipStream_t st) {
-  if (precision == TJSQ_FP64) return tjsq_heff_apply(p, x, y, st);
-  const int64_t n = p->hp.dim;
-  hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(n)), dim3(256), 0, st, x, p->x32, n);
-  HIPCHK(hipGetLastError());
-  int rc = tjsq_heff_apply_f32(p, precision, p->x32, p->y32, st);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_f32_to_f64, dim3(grid_for(n)), dim3(256), 0, st, p->y32, y, n);
-  HIPCHK(hipGetLastError());
-  return TJSQ_OK;
+  if (precision != TJSQ_FP64) return fail(TJSQ_ERR_ARG, "an FP64 Krylov basis is applied in FP64 only");
+  return tjsq_heff_apply(p, x, y, st);
 }
 
 // One Lanczos pass from v0 (already in slot 0, normalised).  Returns the
@@ -2007,8 +2012,11 @@ int apply_f64(tjsq_plan* p, int precision, const double* x, double* y, hipStream
 // The operations are those of the host-scalar form (same kernels, the same
 // IEEE divisions and square roots), so results are bitwise the same.  A
 // beta at or below tol ends the pass; the rest of that iteration is unused.
+int iteration_launches_fp32_basis(Krylov<float>& K, int precision, int last);
+
 template <typename T>
 int iteration_launches(Krylov<T>& K, int precision, int last) {
+  if constexpr (std::is_same<T, float>::value) return iteration_launches_fp32_basis(K, precision, last);
   T* w = K.vec(last);
   if (last < 2)
     hipLaunchKernelGGL(k_combine<T>, dim3(grid_for(K.n)), dim3(256), 0, K.st, w, K.vec(0), K.n, 1, K.ab + 1, K.n,
@@ -2026,6 +2034,43 @@ int iteration_launches(Krylov<T>& K, int precision, int last) {
                                           : apply_prec(K.p, precision, w, K.vec(last + 1), K.st);
   if (rc) return rc;
   K.dots_dev(1, w, K.vec(last + 1), false, K.ab + 2 * last + 1);
+  HIPCHK(hipGetLastError());
+  return TJSQ_OK;
+}
+
+// The same iteration for a mixed-precision solve on an FP32-stored basis:
+// the new direction is formed, normalised and reorthogonalised in an FP64
+// working vector W (FP64 coefficients, FP64 subtraction of the FP32 basis
+// vectors), then stored to the basis in FP32, which the FP32 / BF16x3 apply
+// reads directly; alpha = <v, H v> of the stored vector, FP64 accumulation.
+int iteration_launches_fp32_basis(Krylov<float>& K, int precision, int last) {
+  double* W = K.W;
+  const int64_t n = K.n;
+  const float* y = K.vec(last);   // H v_{last-1}
+  if (last < 2)
+    hipLaunchKernelGGL(k_combine_from<float>, dim3(grid_for(n)), dim3(256), 0, K.st, W, y, K.vec(0), n, 1, K.ab + 1, n);
+  else
+    hipLaunchKernelGGL(k_combine_from<float>, dim3(grid_for(n)), dim3(256), 0, K.st, W, y, K.vec(last - 2), n, 2,
+                       K.ab + 2 * last - 2, n);
+  auto normalize = [&](double* nrm) {
+    hipLaunchKernelGGL(k_multidot<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, K.st, W, n, 1, W, n, K.partial, 0);
+    hipLaunchKernelGGL(k_finish_dots, dim3(1), dim3(RED_THREADS), 0, K.st, K.partial, 1, RED_BLOCKS, 0, K.tmp + 8);
+    hipLaunchKernelGGL(k_sqrt_to, dim3(1), dim3(64), 0, K.st, nrm, K.tmp + 8);
+    hipLaunchKernelGGL(k_scale_inv_dev<double>, dim3(grid_for(n)), dim3(256), 0, K.st, W, n, nrm);
+  };
+  normalize(K.ab + 2 * last);
+  hipLaunchKernelGGL((k_multidot<float, double>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, K.st, K.V, n, last, W, n,
+                     K.partial, 1);
+  hipLaunchKernelGGL(k_finish_dots, dim3(2 * last), dim3(RED_THREADS), 0, K.st, K.partial, last, RED_BLOCKS, 1, K.dots);
+  hipLaunchKernelGGL(k_gs_coef, dim3((last + 63) / 64), dim3(64), 0, K.st, K.coef, K.dots, last);
+  hipLaunchKernelGGL((k_combine<double, float>), dim3(grid_for(n)), dim3(256), 0, K.st, W, K.V, n, last, K.coef, n, 1.0,
+                     0);
+  normalize(K.tmp + 9);
+  hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(n)), dim3(256), 0, K.st, W, K.vec(last), n);
+  HIPCHK(hipGetLastError());
+  int rc = apply_prec(K.p, precision, K.vec(last), K.vec(last + 1), K.st);
+  if (rc) return rc;
+  K.dots_dev(1, K.vec(last), K.vec(last + 1), false, K.ab + 2 * last + 1);
   HIPCHK(hipGetLastError());
   return TJSQ_OK;
 }
@@ -2130,6 +2175,28 @@ int assemble(Krylov<T>& K, int last, const std::vector<double>& alpha, const std
   return K.scale(out, 1.0 / nrm);
 }
 
+// psi = normalise(sum_i z_i v_i) in FP64 from an FP32-stored basis
+int assemble_f64(Krylov<float>& K, int last, const std::vector<double>& alpha, const std::vector<double>& beta,
+                 double* out, double& e0) {
+  std::vector<double> dia(alpha.begin() + 1, alpha.begin() + last + 1);
+  std::vector<double> off(beta.begin() + 1, beta.begin() + last + 1);
+  std::vector<double> z;
+  tridiag_eig(last, dia, off, &z);
+  e0 = dia[0];
+  std::vector<double> c(last);
+  for (int i = 0; i < last; ++i) c[i] = -z[i];  // combine subtracts
+  HIPCHK(hipMemcpyAsync(K.coef, c.data(), sizeof(double) * last, hipMemcpyHostToDevice, K.st));
+  hipLaunchKernelGGL((k_combine<double, float>), dim3(grid_for(K.n)), dim3(256), 0, K.st, out, K.V, K.n, last, K.coef,
+                     K.n, 1.0, 1);
+  HIPCHK(hipGetLastError());
+  Krylov<double> D;
+  D.p = K.p; D.st = K.st; D.n = K.n; D.V = out; D.partial = K.partial; D.dots = K.dots; D.coef = K.coef;
+  double nrm;
+  int rc = D.norm(out, nrm);
+  if (rc) return rc;
+  return D.scale(out, 1.0 / nrm);
+}
+
 template <typename T>
 int ensure_krylov(tjsq_plan* p, int max_iter, Krylov<T>& K, hipStream_t st) {
   const int64_t n = p->hp.dim;
@@ -2194,16 +2261,15 @@ extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_
     res->e_ritz = e0;
     res->iterations = last;
   } else {
-    // mixed precision: FP32 / BF16x3 contractions, the Krylov basis, its
-    // dots and the Gram-Schmidt subtraction in FP64 (apply_f64)
-    if (!p->y32) {
-      if ((!p->x32 && hipMalloc(&p->x32, sizeof(float) * n) != hipSuccess) ||
-          hipMalloc(&p->y32, sizeof(float) * n) != hipSuccess)
-        return fail(TJSQ_ERR_NOMEM, "FP32 staging allocation failed");
-    }
-    Krylov<double> K;
+    // mixed precision: FP32 / BF16x3 contractions on an FP32-stored Krylov
+    // basis; each new direction is orthogonalised in FP64 (FP64 dots,
+    // coefficients and subtraction into an FP64 working vector) before it is
+    // stored, and the Ritz vector is assembled in FP64
+    Krylov<float> K;
     if ((rc = ensure_krylov(p, opts->max_iter, K, st))) return rc;
-    HIPCHK(hipMemcpyAsync(K.vec(0), psi, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    K.W = p->x64b;
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(n)), dim3(256), 0, st, psi, K.vec(0), n);
+    HIPCHK(hipGetLastError());
     double nrm;
     if ((rc = K.norm(K.vec(0), nrm))) return rc;
     if ((rc = K.scale(K.vec(0), 1.0 / nrm))) return rc;
@@ -2211,7 +2277,7 @@ extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_
     if ((rc = lanczos_pass(K, opts->precision, opts->tol, opts->max_iter, last, ritz, alpha, beta, brk, applies, rtol,
                            &theta2)))
       return rc;
-    if ((rc = assemble(K, last, alpha, beta, psi, e0))) return rc;
+    if ((rc = assemble_f64(K, last, alpha, beta, psi, e0))) return rc;
     res->e_ritz = e0;
     res->iterations = last;
   }
