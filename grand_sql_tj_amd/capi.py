@@ -95,6 +95,7 @@ EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_t
            "tjsq_heff_apply_host", "tjsq_lanczos_solve_host",
            "tjsq_block_update_create", "tjsq_trun_project_create", "tjsq_density_create", "tjsq_wave_transform_create",
            "tjsq_dia_transport_create", "tjsq_dia_transport_pairs_create", "tjsq_measure_create",
+           "tjsq_w3js", "tjsq_ndia_transport_create", "tjsq_pair_ndia_create", "tjsq_cor_ndia_create",
            "tjsq_program_sizes",
            "tjsq_program_layout", "tjsq_program_tables", "tjsq_program_stats", "tjsq_program_run",
            "tjsq_program_run_host", "tjsq_truncation_operator_host",
@@ -116,6 +117,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.tjsq_w6js.restype = C.c_double
         L.tjsq_w6js.argtypes = [C.c_int] * 6
+        L.tjsq_w3js.restype = C.c_double
+        L.tjsq_w3js.argtypes = [C.c_int] * 6
         L.tjsq_last_error.restype = C.c_char_p
         L.tjsq_last_error.argtypes = []
         P, I, I64 = C.c_void_p, C.c_int, C.c_int64
@@ -144,6 +147,9 @@ def lib():
             "tjsq_dia_transport_create": [P, P, P, P, P],
             "tjsq_dia_transport_pairs_create": [P, I, P, P, P, P],
             "tjsq_measure_create": [I, P, P, P, I, P, P],
+            "tjsq_ndia_transport_create": [I, I, P, P, P, P, P],
+            "tjsq_pair_ndia_create": [I, I, P, P, P, P, P],
+            "tjsq_cor_ndia_create": [I, P, P, I, P, I, P, I, P, P, I, P, P],
             "tjsq_density_create": [I, I, I, P, I, P],
             "tjsq_wave_transform_create": [I, P, P, P, P, I, P, I, P, P],
             "tjsq_program_sizes": [P, P],
@@ -266,6 +272,10 @@ def basis_struct(bs, keep: _Keep) -> BasisC:
 
 def w6js(*j) -> float:
     return lib().tjsq_w6js(*[int(v) for v in j])
+
+
+def w3js(*j) -> float:
+    return lib().tjsq_w3js(*[int(v) for v in j])
 
 
 def get_basis(block_len, blk, site):
@@ -582,6 +592,60 @@ class Program:
         check(lib().tjsq_measure_create(side, C.byref(os_) if os_ is not None else None,
                                         C.byref(ss) if ss is not None else None, C.byref(bs), w.shape[0],
                                         w.ctypes.data, C.byref(h)), "tjsq_measure_create")
+        return cls(h)
+
+    @classmethod
+    def ndia_transport(cls, side, flag, ops, site, basis, trun):
+        """One step of the transport of rank-k operators along a block (side
+        0 system, 1 environment; flag 'B' or 'F'): ops (block length L, or
+        None) embedded into basis (length L + 1), the site operator (host
+        data, or None) embedded after them, each projected with trun.  Run
+        with ops = trun data, x = ops data."""
+        keep = _Keep()
+        os_ = opset_struct(ops, keep) if ops is not None and ops.nops > 0 else None
+        ss = opset_struct(site, keep) if site is not None else None
+        bs, ts = basis_struct(basis, keep), opset_struct(trun, keep)
+        h = C.c_void_p()
+        check(lib().tjsq_ndia_transport_create(int(side), ord(flag), C.byref(os_) if os_ is not None else None,
+                                               C.byref(ss) if ss is not None else None, C.byref(bs), C.byref(ts),
+                                               C.byref(h)), "tjsq_ndia_transport_create")
+        return cls(h)
+
+    @classmethod
+    def pair_ndia(cls, side, flag, ops, site, basis, trun):
+        """The rank-0 products [O x s]^0 of every rank-k block operator of
+        ops (length L - 1) with the site operator joining at length L,
+        projected with trun: diagonal operators at length L.  Run with
+        ops = trun data, x = ops data."""
+        keep = _Keep()
+        os_, ss = opset_struct(ops, keep), opset_struct(site, keep)
+        bs, ts = basis_struct(basis, keep), opset_struct(trun, keep)
+        h = C.c_void_p()
+        check(lib().tjsq_pair_ndia_create(int(side), ord(flag), C.byref(os_), C.byref(ss), C.byref(bs),
+                                          C.byref(ts), C.byref(h)), "tjsq_pair_ndia_create")
+        return cls(h)
+
+    @classmethod
+    def cor_ndia(cls, flag, sys_op, env_ops, env_trans, env_site, env_site_trans, sys_site, sys_site_trans,
+                 sys_basis, env_basis, wsec):
+        """Correlations across the bond of one system block operator with
+        environment block operators, the environment bond-site operator and
+        the system bond-site operator (each optional): one
+        wavefunction-sized output vector y_q per partner, value = psi . y_q.
+        Run with ops = concat(sys_op data, env_ops data), x = psi, y zeroed."""
+        keep = _Keep()
+        so = opset_struct(sys_op, keep)
+        eo = opset_struct(env_ops, keep) if env_ops is not None and env_ops.nops > 0 else None
+        es = opset_struct(env_site, keep) if env_site is not None else None
+        ss = opset_struct(sys_site, keep) if sys_site is not None else None
+        sb, eb = basis_struct(sys_basis, keep), basis_struct(env_basis, keep)
+        w = np.ascontiguousarray(wsec, np.int32).reshape(-1, 6)
+        ref = lambda v: C.byref(v) if v is not None else None
+        h = C.c_void_p()
+        check(lib().tjsq_cor_ndia_create(ord(flag), C.byref(so), ref(eo), int(env_trans), ref(es),
+                                         int(env_site_trans), ref(ss), int(sys_site_trans), C.byref(sb),
+                                         C.byref(eb), w.shape[0], w.ctypes.data, C.byref(h)),
+              "tjsq_cor_ndia_create")
         return cls(h)
 
     @classmethod

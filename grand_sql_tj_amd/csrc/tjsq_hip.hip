@@ -1503,6 +1503,7 @@ extern "C" {
 const char* tjsq_last_error(void) { return g_err.c_str(); }
 
 double tjsq_w6js(int j1, int j2, int j3, int l1, int l2, int l3) { return tjsq::w6js(j1, j2, j3, l1, l2, l3); }
+double tjsq_w3js(int j1, int j2, int j3, int m1, int m2, int m3) { return tjsq::w3js(j1, j2, j3, m1, m2, m3); }
 
 int tjsq_get_basis(int block_len, int nb, const int* blk, int nsite, const int* site, int* sec, int* sub, int* nsec,
                    int* nsub) {
@@ -2622,6 +2623,44 @@ int tjsq_measure_create(int side, const tjsq_opset* ops, const tjsq_opset* site,
   std::string msg;
   int rc = tjsq::build_measure_program(side, ops, site, *basis, nw, wsec, tjsq::Shape(), pg, msg);
   if (rc) return fail(rc, "tjsq_measure_create: " + msg);
+  return make_program(pg, 0, 1, prog);
+}
+
+int tjsq_ndia_transport_create(int side, int flag, const tjsq_opset* ops, const tjsq_opset* site,
+                               const tjsq_basis* basis, const tjsq_opset* trun, tjsq_program** prog) {
+  if (!basis || !trun || !prog) return fail(TJSQ_ERR_ARG, "tjsq_ndia_transport_create: null argument");
+  *prog = nullptr;
+  tjsq::Program pg;
+  std::string msg;
+  int rc = tjsq::build_ndia_transport_program(side, (char)flag, ops, site, *basis, *trun, tjsq::Shape(), pg, msg);
+  if (rc) return fail(rc, "tjsq_ndia_transport_create: " + msg);
+  return make_program(pg, 0, 1, prog);
+}
+
+int tjsq_pair_ndia_create(int side, int flag, const tjsq_opset* ops, const tjsq_opset* site, const tjsq_basis* basis,
+                          const tjsq_opset* trun, tjsq_program** prog) {
+  if (!ops || !site || !basis || !trun || !prog) return fail(TJSQ_ERR_ARG, "tjsq_pair_ndia_create: null argument");
+  *prog = nullptr;
+  tjsq::Program pg;
+  std::string msg;
+  int rc = tjsq::build_pair_ndia_program(side, (char)flag, *ops, *site, *basis, *trun, tjsq::Shape(), pg, msg);
+  if (rc) return fail(rc, "tjsq_pair_ndia_create: " + msg);
+  return make_program(pg, 0, 1, prog);
+}
+
+int tjsq_cor_ndia_create(int flag, const tjsq_opset* sys_op, const tjsq_opset* env_ops, int env_trans,
+                         const tjsq_opset* env_site, int env_site_trans, const tjsq_opset* sys_site, int sys_site_trans,
+                         const tjsq_basis* sys_basis, const tjsq_basis* env_basis, int nw, const int* wsec,
+                         tjsq_program** prog) {
+  if (!sys_op || !sys_basis || !env_basis || !wsec || !prog)
+    return fail(TJSQ_ERR_ARG, "tjsq_cor_ndia_create: null argument");
+  *prog = nullptr;
+  tjsq::Program pg;
+  std::string msg;
+  int rc = tjsq::build_cor_ndia_program(*sys_op, env_ops, env_trans, env_site, env_site_trans, sys_site,
+                                        sys_site_trans, *sys_basis, *env_basis, nw, wsec, (char)flag, tjsq::Shape(),
+                                        pg, msg);
+  if (rc) return fail(rc, "tjsq_cor_ndia_create: " + msg);
   return make_program(pg, 0, 1, prog);
 }
 

@@ -63,6 +63,32 @@ double w6js(int j1, int j2, int j3, int l1, int l2, int l3) {
   return omega * std::sqrt(t1 * t2 * t3 * t4);
 }
 
+// 3j symbol (j1 j2 j3; m1 m2 m3), every argument twice the angular momentum
+// (W3JS, block.f90:1522-1560): Racah's single sum over z,
+//   (-1)^(j1-j2-m3) sqrt(D(j1 j2 j3) prod (j_i + m_i)! (j_i - m_i)!)
+//   sum_z (-1)^z / [z! (j1+j2-j3-z)! (j1-m1-z)! (j2+m2-z)! (j3-j2+m1+z)! (j3-j1-m2+z)!]
+// with D = (j1+j2-j3)! (j1-j2+j3)! (-j1+j2+j3)! / (j1+j2+j3+1)!; 0 where the
+// selection rules fail
+double w3js(int j1, int j2, int j3, int m1, int m2, int m3) {
+  const double* F = fact_table();
+  if (m1 + m2 + m3 != 0 || j3 > j1 + j2 || j3 < std::abs(j1 - j2)) return 0.0;
+  if (((j1 + j2 + j3) | (j1 - m1) | (j2 - m2)) & 1) return 0.0;
+  if (std::abs(m1) > j1 || std::abs(m2) > j2 || std::abs(m3) > j3) return 0.0;
+  const int a = j1 + j2 - j3, b = j1 - m1, c = j2 + m2, d = j3 - j2 + m1, e = j3 - j1 - m2;
+  const int zmin = std::max(0, std::max(-d, -e)), zmax = std::min(a, std::min(b, c));
+  double sum = 0.0;
+  for (int z = zmin; z <= zmax; z += 2) {
+    double den = F[z / 2] * F[(a - z) / 2] * F[(b - z) / 2] * F[(c - z) / 2] * F[(d + z) / 2] * F[(e + z) / 2];
+    if (z % 4 != 0) den = -den;
+    sum = sum + 1.0 / den;
+  }
+  const double tri = F[a / 2] * F[(j3 + j1 - j2) / 2] * F[(j3 - j1 + j2) / 2] / F[(j1 + j2 + j3 + 2) / 2];
+  const double mm = F[(j1 + m1) / 2] * F[b / 2] * F[c / 2] * F[(j2 - m2) / 2] * F[(j3 - m3) / 2] * F[(j3 + m3) / 2];
+  double v = sum * std::sqrt(tri * mm);
+  if ((j1 - j2 - m3) % 4 != 0) v = -v;
+  return v;
+}
+
 static inline double par(int n) { return (n % 2 == 0) ? 1.0 : -1.0; }
 
 // --------------------------------------------------------------------------

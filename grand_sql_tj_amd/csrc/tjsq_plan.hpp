@@ -65,6 +65,7 @@ struct Couplings {
 void hmerge_tables(const LatticeTables& L, int sys_len, int env_len, Couplings& C);
 
 double w6js(int j1, int j2, int j3, int l1, int l2, int l3);
+double w3js(int j1, int j2, int j3, int m1, int m2, int m3);
 
 // ---- merged operators (hmerge's snc / sub_s) ------------------------------
 // A merged operator copies the structure of active site 1's operator and is

@@ -2,6 +2,9 @@
 // (see tjsq_update.hpp for the scheme).
 #include "tjsq_update.hpp"
 
+#include <map>
+#include <tuple>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -773,6 +776,386 @@ int build_measure_program(int side, const tjsq_opset* ops_in, const tjsq_opset* 
   pg.x_elems = D;
   pg.ops_elems = ops.empty() ? 0 : max_extent(*ops_in);
   Ctx cx;
+  return finish(pg, shape, cx, msg);
+}
+
+// One step of Get_sys_oper_ndia / Get_env_oper_ndia and Change_*_oper_ndia
+// (get_sys.f90:998-1238): every operator of the block at length L embedded
+// into the fused basis (update_block_ndia_sys / _env), the new site's
+// operator embedded (update_site_ndia_sys / _env), each projected with T
+// (update_trun_ndia); flag 'B' or 'F' selects the fermion signs of the
+// embeddings.
+int build_ndia_transport_program(int side, char flag, const tjsq_opset* ops_in, const tjsq_opset* site,
+                                 const tjsq_basis& basis, const tjsq_opset& trun, const Shape& shape, Program& pg,
+                                 std::string& msg) {
+  pg = Program();
+  if ((side != 0 && side != 1) || (flag != 'B' && flag != 'F')) { msg = "bad side or sign flag"; return TJSQ_ERR_ARG; }
+  std::vector<Op> ops, tops;
+  int rc;
+  if (ops_in && ops_in->nops > 0 && (rc = read_ops(*ops_in, ops, msg))) return rc;
+  Site st;
+  if (site && (rc = read_site(*site, st, msg))) return rc;
+  if (ops.empty() && !site) { msg = "nothing to transport"; return TJSQ_ERR_ARG; }
+  if ((rc = read_ops(trun, tops, msg))) return rc;
+  if (trun.nops != 1) { msg = "the truncation operator set must hold one operator"; return TJSQ_ERR_ARG; }
+  BasisView bs;
+  read_basis(basis, bs);
+  Ctx cx;
+  Projector pj{pg, tops[0], cx};
+  for (const Op& o : ops) pj.project(upd_block_ndia(o, bs, basis.length, flag, side, cx), false);
+  if (site) pj.project(upd_site_ndia(st, bs, basis.length, flag, side), false);
+  pg.x_elems = ops.empty() ? 0 : max_extent(*ops_in);
+  pg.ops_elems = max_extent(trun);
+  return finish(pg, shape, cx, msg);
+}
+
+namespace {
+
+// an operator's sub-blocks as a contraction routine sees them, optionally
+// after block_transfer_trans (block.f90:1676-1706): the transpose's quantum
+// numbers (ket + the sub-block's shift, shift negated, rows and columns
+// swapped); the data stay in place and are read transposed (t)
+struct VSub { int num_up, num_down, down_dif, row_dim, sdim; int64_t off; double val; };
+struct VOp { int up_dif = 0, down_dif = 0; bool t = false; std::vector<VSub> sub; };
+VOp view_op(const Op& o, bool trans, int64_t base) {
+  VOp v;
+  v.up_dif = trans ? -o.up_dif : o.up_dif;
+  v.down_dif = o.down_dif;
+  v.t = trans;
+  for (const OpSub& q : o.sub)
+    v.sub.push_back(trans ? VSub{q.num_up + o.up_dif, q.num_down + q.down_dif, -q.down_dif, q.sdim, q.row_dim,
+                                 base + q.off, 0.0}
+                          : VSub{q.num_up, q.num_down, q.down_dif, q.row_dim, q.sdim, base + q.off, 0.0});
+  return v;
+}
+VOp view_site(const Site& st, bool trans) {
+  VOp v;
+  v.up_dif = trans ? -st.up_dif : st.up_dif;
+  v.down_dif = st.down_dif;
+  for (size_t i = 0; i < st.sub.size(); ++i) {
+    const OpSub& q = st.sub[i];
+    v.sub.push_back(trans ? VSub{q.num_up + st.up_dif, q.num_down + q.down_dif, -q.down_dif, 1, 1, 0, st.val[i]}
+                          : VSub{q.num_up, q.num_down, q.down_dif, 1, 1, 0, st.val[i]});
+  }
+  return v;
+}
+int first_vsub(const VOp& o, int up, int down, int dd) {
+  for (size_t m = 0; m < o.sub.size(); ++m)
+    if (o.sub[m].num_up == up && o.sub[m].num_down == down && o.sub[m].down_dif == dd) return (int)m;
+  return -1;
+}
+// first sub-sector of the first sector (up, down) of b with the given block
+// and site parts (the reference's nested searches with goto on a match);
+// bl_up < -100000 leaves the block's particle label unchecked
+const MapSub* first_sub(const BasisView& b, int up, int down, int bl_up, int bl_down, int st_up, int st_down) {
+  for (const MapSec& m : b.sec) {
+    if (m.new_up != up || m.new_down != down) continue;
+    for (const MapSub& n : m.sub)
+      if ((bl_up < -100000 || n.bl_up == bl_up) && n.bl_down == bl_down && n.st_up == st_up && n.st_down == st_down)
+        return &n;
+  }
+  return nullptr;
+}
+double fsign(char flag, int n) { return flag == 'F' && n % 2 != 0 ? -1.0 : 1.0; }
+
+}  // namespace
+
+// sys_block_env_block_cor_ndia (measure.f90:5241-5480) for each environment
+// block operator, sys_block_env_site_cor_ndia (:5482-5706) for the
+// environment bond-site operator and sys_block_site_cor_ndia (:4939-5083)
+// for the system bond-site operator, each against the one system block
+// operator A.  Output copy q (environment block operators first, then the
+// environment site, then the system site) holds Y_q with the routine's value
+// = psi . Y_q.  The program's operator space is A's data followed by the
+// environment operators' data; *_trans applies block_transfer_trans to those
+// operands first, as the callers do for fermions.
+int build_cor_ndia_program(const tjsq_opset& sys_op, const tjsq_opset* env_ops, int env_trans,
+                           const tjsq_opset* env_site, int env_site_trans, const tjsq_opset* sys_site,
+                           int sys_site_trans, const tjsq_basis& sys_basis, const tjsq_basis& env_basis, int nw,
+                           const int* wsec, char flag, const Shape& shape, Program& pg, std::string& msg) {
+  pg = Program();
+  if (nw < 1 || !wsec || (flag != 'B' && flag != 'F')) { msg = "bad correlation arguments"; return TJSQ_ERR_ARG; }
+  std::vector<Op> av, ev;
+  int rc;
+  if ((rc = read_ops(sys_op, av, msg))) return rc;
+  if (sys_op.nops != 1) { msg = "one system block operator"; return TJSQ_ERR_ARG; }
+  if (env_ops && env_ops->nops > 0 && (rc = read_ops(*env_ops, ev, msg))) return rc;
+  Site est, sst;
+  if (env_site && (rc = read_site(*env_site, est, msg))) return rc;
+  if (sys_site && (rc = read_site(*sys_site, sst, msg))) return rc;
+  const int64_t ebase = max_extent(sys_op);
+  const VOp A = view_op(av[0], false, 0);
+  std::vector<VOp> B;
+  for (const Op& o : ev) B.push_back(view_op(o, env_trans != 0, ebase));
+  const VOp ET = view_site(est, env_site_trans != 0), STv = view_site(sst, sys_site_trans != 0);
+  BasisView sb, eb;
+  read_basis(sys_basis, sb);
+  read_basis(env_basis, eb);
+  std::vector<WSec> W(nw);
+  int64_t D = 0;
+  for (int i = 0; i < nw; ++i) {
+    const int* w = wsec + 6 * i;
+    W[i] = {w[0], w[1], w[2], w[3], w[4], w[5], D};
+    D += (int64_t)w[4] * w[5];
+  }
+  auto wave_of = [&](int su, int sd, int eu, int ed) {
+    for (int m = 0; m < nw; ++m)
+      if (W[m].sys_up == su && W[m].sys_down == sd && W[m].env_up == eu && W[m].env_down == ed) return m;
+    return -1;
+  };
+  const int ncopy = (int)B.size() + (env_site ? 1 : 0) + (sys_site ? 1 : 0);
+  // every region a term reads or writes lies inside its wavefunction sector
+  bool bad = false;
+  auto inside = [&](int w, int r0, int M, int c0, int N) {
+    if (r0 < 0 || c0 < 0 || r0 + M > W[w].sys_dim || c0 + N > W[w].env_dim) bad = true;
+    return !bad;
+  };
+  // one output block per (copy, sector, row origin, column origin): the
+  // regions of one copy are the tiles of the fused sub-sector grid (sector
+  // wide row strips for the system-site copy), each with one size
+  std::map<std::tuple<int, int, int, int>, int> region;
+  auto out_block = [&](int q, int w, int r0, int c0, int M, int N) {
+    auto key = std::make_tuple(q, w, r0, c0);
+    auto it = region.find(key);
+    if (it != region.end()) {
+      const OutBlock& b = pg.hp.out_blocks[it->second];
+      if (b.M != M || b.N != N) bad = true;
+      return it->second;
+    }
+    pg.hp.out_blocks.push_back({(int64_t)q * D + W[w].off + r0 + (int64_t)c0 * W[w].sys_dim,
+                                std::max(1, W[w].sys_dim), M, N, (int)pg.hp.out_blocks.size()});
+    const int id = (int)pg.hp.out_blocks.size() - 1;
+    region.emplace(key, id);
+    return id;
+  };
+  const int ud = A.up_dif, k1 = A.down_dif;
+  for (int i = 0; i < nw; ++i) {
+    const WSec& wi = W[i];
+    if (wi.sys_dim == 0 || wi.env_dim == 0) continue;
+    const int j = find_sec(sb, wi.sys_up, wi.sys_down), k = find_sec(eb, wi.env_up, wi.env_down);
+    if (j < 0 || k < 0) continue;
+    for (const MapSub& x : sb.sec[j].sub)
+      for (const MapSub& y : eb.sec[k].sub) {
+        const int64_t xoff = wi.off + x.spos + (int64_t)y.spos * wi.sys_dim;
+        // (a) A (x) env block operator q
+        for (size_t q = 0; q < B.size(); ++q)
+          for (int dd = -k1; dd <= k1; dd += SU) {
+            const int nbl_up = x.bl_up + ud, nbl_down = x.bl_down + dd;
+            const int sid = first_vsub(A, x.bl_up, x.bl_down, dd);
+            for (int edd = -k1; edd <= k1; edd += SU) {
+              const int nebl_up = y.bl_up - ud, nebl_down = y.bl_down - edd;
+              const int eid = first_vsub(B[q], nebl_up, nebl_down, edd);
+              for (int nsd = std::abs(wi.sys_down - k1); nsd <= wi.sys_down + k1; nsd += SU) {
+                const int nsu = wi.sys_up + ud, neu = wi.env_up - ud, ned = nsd;
+                const MapSub* xp = first_sub(sb, nsu, nsd, nbl_up, nbl_down, x.st_up, x.st_down);
+                const MapSub* yp = first_sub(eb, neu, ned, nebl_up, nebl_down, y.st_up, y.st_down);
+                const int w = wave_of(nsu, nsd, neu, ned);
+                if (sid < 0 || eid < 0 || !xp || !yp || w < 0) continue;
+                double c1 = w6js(nsd, k1, wi.sys_down, x.bl_down, x.st_down, nbl_down);
+                if (c1 == 0.0) continue;
+                c1 *= w6js(ned, k1, wi.env_down, y.bl_down, y.st_down, nebl_down);
+                if (c1 == 0.0) continue;
+                c1 *= std::sqrt((1.0 + ned) * (1.0 + wi.env_down) / (1.0 + k1));
+                if (((nbl_down + x.st_down + wi.sys_down + k1) / 2 + (nebl_down + y.st_down + wi.env_down + k1) / 2) %
+                        2 != 0)
+                  c1 = -c1;
+                const VSub& as = A.sub[sid];
+                const VSub& bs = B[q].sub[eid];
+                if (as.row_dim != xp->sdim || as.sdim != x.sdim || bs.row_dim != y.sdim || bs.sdim != yp->sdim ||
+                    !inside(i, x.spos, x.sdim, y.spos, y.sdim) || !inside(w, xp->spos, xp->sdim, yp->spos, yp->sdim)) {
+                  msg = "operator sub-block disagrees with the fused bases";
+                  return TJSQ_ERR_ARG;
+                }
+                Contrib c{};
+                c.kind = C_BOTH;
+                c.c = fsign(flag, wi.sys_down + y.st_down) * c1;
+                c.a_off = as.off; c.a_ld = as.row_dim; c.a_t = 0;
+                c.x_off = xoff; c.x_ld = wi.sys_dim; c.x_t = 0;
+                c.b_off = bs.off; c.b_ld = B[q].t ? bs.sdim : bs.row_dim; c.b_t = B[q].t ? 1 : 0;
+                c.M = xp->sdim; c.K1 = x.sdim; c.K2 = y.sdim; c.N = yp->sdim;
+                c.out_block = out_block((int)q, w, xp->spos, yp->spos, c.M, c.N);
+                if (c.M && c.N && c.K1 && c.K2) pg.hp.contrib.push_back(c);
+              }
+            }
+          }
+        // (b) A (x) the environment bond-site operator
+        if (env_site) {
+          const int q = (int)B.size();
+          for (int dd = -k1; dd <= k1; dd += SU) {
+            const int nbl_up = x.bl_up + ud, nbl_down = x.bl_down + dd;
+            const int sid = first_vsub(A, x.bl_up, x.bl_down, dd);
+            for (int sdd = -ET.down_dif; sdd <= ET.down_dif; sdd += SU) {
+              const int nst_up = y.st_up - ud, nst_down = y.st_down - sdd;
+              const int eid = first_vsub(ET, nst_up, nst_down, sdd);
+              if (sid < 0 || eid < 0) continue;
+              for (int nsd = std::abs(wi.sys_down - k1); nsd <= wi.sys_down + k1; nsd += SU) {
+                const int nsu = wi.sys_up + ud, neu = wi.env_up - ud, ned = nsd;
+                const MapSub* xp = first_sub(sb, nsu, nsd, nbl_up, nbl_down, x.st_up, x.st_down);
+                const MapSub* yp = first_sub(eb, neu, ned, -1000000, y.bl_down, nst_up, nst_down);
+                const int w = wave_of(nsu, nsd, neu, ned);
+                if (!xp || !yp || w < 0) continue;
+                double c1 = w6js(nsd, k1, wi.sys_down, x.bl_down, x.st_down, nbl_down);
+                if (c1 == 0.0) continue;
+                c1 *= w6js(ned, k1, wi.env_down, y.st_down, y.bl_down, nst_down);
+                if (c1 == 0.0) continue;
+                c1 *= std::sqrt((1.0 + ned) * (1.0 + wi.env_down) / (1.0 + k1));
+                if (((nbl_down + x.st_down + wi.sys_down + k1) / 2 + (y.bl_down + y.st_down + ned + k1) / 2) % 2 != 0)
+                  c1 = -c1;
+                const VSub& as = A.sub[sid];
+                if (as.row_dim != xp->sdim || as.sdim != x.sdim || yp->sdim != y.sdim ||
+                    !inside(i, x.spos, x.sdim, y.spos, y.sdim) || !inside(w, xp->spos, xp->sdim, yp->spos, y.sdim)) {
+                  msg = "operator sub-block disagrees with the fused bases";
+                  return TJSQ_ERR_ARG;
+                }
+                Contrib c{};
+                c.kind = C_LEFT;
+                c.c = fsign(flag, wi.sys_down) * ET.sub[eid].val * c1;
+                c.a_off = as.off; c.a_ld = as.row_dim; c.a_t = 0;
+                c.x_off = xoff; c.x_ld = wi.sys_dim;
+                c.M = xp->sdim; c.K1 = x.sdim; c.N = y.sdim;
+                c.out_block = out_block(q, w, xp->spos, yp->spos, c.M, c.N);
+                if (c.c != 0.0 && c.M && c.N && c.K1) pg.hp.contrib.push_back(c);
+              }
+            }
+          }
+        }
+      }
+    // (c) A (x) the system bond-site operator, inside the sector
+    if (sys_site) {
+      const int q = ncopy - 1;
+      for (const MapSub& x : sb.sec[j].sub)
+        for (int dd = -k1; dd <= k1; dd += SU)
+          for (int sdd = -k1; sdd <= k1; sdd += SU) {
+            const int nbl_up = x.bl_up + ud, nbl_down = x.bl_down + dd;
+            const int nst_up = x.st_up - ud, nst_down = x.st_down - sdd;
+            const int bid = first_vsub(A, x.bl_up, x.bl_down, dd);
+            const int tid = first_vsub(STv, nst_up, nst_down, sdd);
+            const MapSub* xp = nullptr;
+            for (const MapSub& m : sb.sec[j].sub)
+              if (m.bl_up == nbl_up && m.bl_down == nbl_down && m.st_up == nst_up && m.st_down == nst_down) {
+                xp = &m;
+                break;
+              }
+            if (bid < 0 || tid < 0 || !xp) continue;
+            double c1 = w6js(nbl_down, x.bl_down, k1, x.st_down, nst_down, wi.sys_down) / std::sqrt(1.0 + k1);
+            if (c1 == 0.0) continue;
+            if (((x.bl_down + nst_down + k1 + wi.sys_down) / 2) % 2 != 0) c1 = -c1;
+            const VSub& as = A.sub[bid];
+            if (as.row_dim != xp->sdim || as.sdim != x.sdim || !inside(i, x.spos, x.sdim, 0, wi.env_dim) ||
+                !inside(i, xp->spos, xp->sdim, 0, wi.env_dim)) {
+              msg = "operator sub-block disagrees with the fused basis";
+              return TJSQ_ERR_ARG;
+            }
+            Contrib c{};
+            c.kind = C_LEFT;
+            c.c = fsign(flag, x.bl_down) * STv.sub[tid].val * c1;
+            c.a_off = as.off; c.a_ld = as.row_dim; c.a_t = 0;
+            c.x_off = wi.off + x.spos; c.x_ld = wi.sys_dim;
+            c.M = xp->sdim; c.K1 = x.sdim; c.N = wi.env_dim;
+            c.out_block = out_block(q, i, xp->spos, 0, c.M, c.N);
+            if (c.c != 0.0 && c.M && c.N && c.K1) pg.hp.contrib.push_back(c);
+          }
+    }
+  }
+  if (bad) { msg = "a correlation term falls outside its wavefunction sector"; return TJSQ_ERR_ARG; }
+  pg.out.nops = ncopy;
+  for (int q = 0; q < ncopy; ++q) pg.out.hdr.insert(pg.out.hdr.end(), {sys_basis.length, 0, 0, 0});
+  pg.out.ndata = (int64_t)ncopy * D;
+  pg.x_elems = D;
+  pg.ops_elems = ebase + (env_ops && env_ops->nops > 0 ? max_extent(*env_ops) : 0);
+  Ctx cx;
+  return finish(pg, shape, cx, msg);
+}
+
+namespace {
+
+// block_mul_block_ndia (block.f90:1956-2197) with Flag1 = Flag2 = 'N', coef 1
+// and new_oper%down_dif = 0: the rank-0 coupling P = [O1 x O2]^0 of two
+// embedded operators over one fused basis.  Per basis sector i (ket) and
+// intermediate 2S m, P[i] += c(i, m) O1[m -> i] O2[i -> m] with
+//   c = sum over mu of (J k1 m; -J J-mu mu) (m k2 J; -mu mu-J J) (-1)^((m-mu)/2)
+//       (k1 k2 0; J-mu mu-J 0) (-1)^((k2-k1)/2) / (J 0 J; -J 0 J)
+// (J = 2S of sector i, k1, k2 the ranks).  The product of a placement of
+// O1 with an identity placement of O2 over the same sub-sector is one
+// placement of O1's data; products of two data placements are not formed.
+SymOp mul_ndia_scalar(const SymOp& o1, const SymOp& o2, const BasisView& bs, int len, Ctx& cx) {
+  SymOp p;
+  p.present = true; p.len = len; p.up_dif = o1.up_dif + o2.up_dif; p.down_dif = 0;
+  const int k1 = o1.down_dif, k2 = o2.down_dif;
+  auto find = [](const SymOp& o, int up, int down, int dd) {
+    for (size_t x = 0; x < o.sub.size(); ++x)
+      if (o.sub[x].num_up == up && o.sub[x].num_down == down && o.sub[x].down_dif == dd) return (int)x;
+    return -1;
+  };
+  for (const MapSec& sec : bs.sec) {
+    const int J = sec.new_down;
+    const int bsi = find_sec(bs, sec.new_up + p.up_dif, J);
+    if (bsi < 0) continue;
+    p.sub.push_back({sec.new_up, J, 0, bs.sec[bsi].dim, sec.dim, {}});
+    SymSub& d = p.sub.back();
+    for (int m = std::abs(J - k2); m <= J + k2; m += SU) {
+      const int i2 = find(o2, sec.new_up, J, m - J);
+      const int i1 = find(o1, sec.new_up + o2.up_dif, m, J - m);
+      if (i1 < 0 || i2 < 0) continue;
+      const double c11 = w3js(J, 0, J, -J, 0, J);
+      if (c11 == 0.0) { cx.fail("vanishing normalising 3j symbol"); return p; }
+      double c1 = 0.0;
+      for (int mu = -m; mu <= m; mu += SU) {
+        double c12 = w3js(J, k1, m, -J, J - mu, mu);
+        c12 = c12 * w3js(m, k2, J, -mu, mu - J, J) * par((m - mu) / 2);
+        c12 = c12 * w3js(k1, k2, 0, J - mu, mu - J, 0) * par((-k1 + k2 + 0) / 2) * std::sqrt(0 + 1.0);
+        c1 = c1 + c12;
+      }
+      if (c1 == 0.0) continue;
+      c1 = 1.0 * c1 / c11;
+      for (const Place& a : o1.sub[i1].pl)
+        for (const Place& b : o2.sub[i2].pl) {
+          if (a.c0 + a.w <= b.r0 || b.r0 + b.h <= a.c0) continue;
+          if (a.c0 != b.r0 || a.w != b.h) { cx.fail("operator product over misaligned placements"); return p; }
+          if (a.src >= 0 && b.src >= 0) { cx.fail("operator product of two data blocks is not supported"); return p; }
+          const double c = c1 * a.coef * b.coef;
+          if (c == 0.0) continue;
+          if (b.src < 0)
+            d.pl.push_back({a.r0, b.c0, a.h, b.w, c, a.src, a.ld, a.t});
+          else
+            d.pl.push_back({a.r0, b.c0, a.h, b.w, c, b.src, b.ld, b.t});
+        }
+    }
+  }
+  return p;
+}
+
+}  // namespace
+
+// The same-side pair operators of Get_sys2b / Get_env2b (measure.f90:
+// 2748-2819) at the block length where the second site joins: every
+// operator of ops (rank k, block length L - 1) embedded into `basis`
+// (update_block_ndia_*), the site operator embedded (update_site_ndia_*),
+// their rank-0 coupling (block_mul_block_ndia), projected with T
+// (update_trun_dia).  The outputs are diagonal operators at length L.
+int build_pair_ndia_program(int side, char flag, const tjsq_opset& ops_in, const tjsq_opset& site,
+                            const tjsq_basis& basis, const tjsq_opset& trun, const Shape& shape, Program& pg,
+                            std::string& msg) {
+  pg = Program();
+  if ((side != 0 && side != 1) || (flag != 'B' && flag != 'F')) { msg = "bad side or sign flag"; return TJSQ_ERR_ARG; }
+  std::vector<Op> ops, tops;
+  int rc;
+  if ((rc = read_ops(ops_in, ops, msg))) return rc;
+  Site st;
+  if ((rc = read_site(site, st, msg))) return rc;
+  if ((rc = read_ops(trun, tops, msg))) return rc;
+  if (trun.nops != 1) { msg = "the truncation operator set must hold one operator"; return TJSQ_ERR_ARG; }
+  BasisView bs;
+  read_basis(basis, bs);
+  Ctx cx;
+  Projector pj{pg, tops[0], cx};
+  const SymOp so = upd_site_ndia(st, bs, basis.length, flag, side);
+  for (const Op& o : ops) {
+    if (o.up_dif + st.up_dif != 0) { msg = "the pair operator must conserve the particle number"; return TJSQ_ERR_ARG; }
+    pj.project(mul_ndia_scalar(upd_block_ndia(o, bs, basis.length, flag, side, cx), so, bs, basis.length, cx), true);
+  }
+  pg.x_elems = max_extent(ops_in);
+  pg.ops_elems = max_extent(trun);
   return finish(pg, shape, cx, msg);
 }
 

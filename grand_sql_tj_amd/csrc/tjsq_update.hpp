@@ -100,6 +100,44 @@ int build_dia_transport_program(const tjsq_opset* ops, const tjsq_opset* site, i
 int build_measure_program(int side, const tjsq_opset* ops, const tjsq_opset* site, const tjsq_basis& basis,
                           int nw, const int* wsec, const Shape& shape, Program& pg, std::string& msg);
 
+// One step of the transport of rank-k operators (Get_sys_oper_ndia /
+// Get_env_oper_ndia / Change_*_oper_ndia, code/tjsq/get_sys.f90:998-1238):
+// each operator of ops (block length L; may be NULL) embedded into `basis`
+// (update_block_ndia_sys / _env, side 0 / 1), then the site operator `site`
+// (1x1 sub-blocks, host data; may be NULL) embedded (update_site_ndia_*),
+// each projected with `trun` (update_trun_ndia).  flag: 'B' or 'F' (the
+// fermion signs of the embeddings).  Run with ops = trun data, x = ops data.
+int build_ndia_transport_program(int side, char flag, const tjsq_opset* ops, const tjsq_opset* site,
+                                 const tjsq_basis& basis, const tjsq_opset& trun, const Shape& shape, Program& pg,
+                                 std::string& msg);
+
+// Correlations of one system block operator A (block length sys_len - 1)
+// with environment block operators (sys_block_env_block_cor_ndia,
+// code/tjsq/measure.f90:5241-5480), the environment bond-site operator
+// (sys_block_env_site_cor_ndia, :5482-5706) and the system bond-site
+// operator (sys_block_site_cor_ndia, :4939-5083): output copy q holds Y_q,
+// the routine's value = psi . Y_q (environment block operators first, then
+// env_site, then sys_site).  sys_basis / env_basis are the wavefunction's
+// fused bases; *_trans applies block_transfer_trans to that operand first.
+// Run with ops = A's data followed by the environment operators' data,
+// x = psi, y cleared beforehand.
+int build_cor_ndia_program(const tjsq_opset& sys_op, const tjsq_opset* env_ops, int env_trans,
+                           const tjsq_opset* env_site, int env_site_trans, const tjsq_opset* sys_site,
+                           int sys_site_trans, const tjsq_basis& sys_basis, const tjsq_basis& env_basis, int nw,
+                           const int* wsec, char flag, const Shape& shape, Program& pg, std::string& msg);
+
+// The same-side pair operators of Get_sys2b / Get_env2b (side 0 / 1,
+// code/tjsq/measure.f90:2748-2819) formed when the second site joins the
+// block: each operator of ops (rank k, block length L - 1) embedded into
+// `basis` (update_block_ndia_*), the site operator (1x1 sub-blocks, host
+// data) embedded (update_site_ndia_*), their rank-0 coupling
+// (block_mul_block_ndia, code/tjsq/block.f90:1956-2197) projected with
+// `trun` (update_trun_dia): one diagonal operator at length L per operator
+// of ops.  Run with ops = trun data, x = ops data.
+int build_pair_ndia_program(int side, char flag, const tjsq_opset& ops, const tjsq_opset& site,
+                            const tjsq_basis& basis, const tjsq_opset& trun, const Shape& shape, Program& pg,
+                            std::string& msg);
+
 // Get_density_matrix_sys (side 0) / _env (side 1) of a wavefunction with
 // sectors wsec; `len` is the block length recorded in the output operator.
 int build_density_program(int side, int len, int nw, const int* wsec, const Shape& shape, Program& pg,

@@ -188,6 +188,185 @@ def correlation_matrix(up_bias, sys_len, env_len, sys_bases, env_bases, sys_trun
     return cor
 
 
+# ---- correlations of rank-k site operators (Get_density_cor) ------------------------------
+
+def spin_sd(up_bias: int) -> OpSet:
+    """st_sd of Get_site_operator_su2_U0 (dmrg_sub.f90:370-473): the spin
+    vector S, rank 1 (down_dif 2), reduced element sqrt(3/2) on the singly
+    occupied site."""
+    return OpSet(1, np.array([[1, 0, 2, 1]], np.int32), np.array([[up_bias, 1, 0, 1, 1]], np.int32),
+                 np.array([0], np.int64), np.array([math.sqrt(1.5)]))
+
+
+def elec_up(up_bias: int) -> OpSet:
+    """st_elec_up: the creation operator c^+, rank 1/2 (down_dif 1), from the
+    hole to the singly occupied site, reduced element -sqrt(2)."""
+    return OpSet(1, np.array([[1, up_bias, 1, 1]], np.int32), np.array([[0, 0, 1, 1, 1]], np.int32),
+                 np.array([0], np.int64), np.array([-math.sqrt(2.0)]))
+
+
+def elec_down(up_bias: int) -> OpSet:
+    """st_elec_down: the annihilation operator c, rank 1/2, from the singly
+    occupied site to the hole, reduced element -sqrt(2)."""
+    return OpSet(1, np.array([[1, -up_bias, 1, 1]], np.int32), np.array([[up_bias, 1, -1, 1, 1]], np.int32),
+                 np.array([0], np.int64), np.array([-math.sqrt(2.0)]))
+
+
+# the three calls of Get_density_cor in Measurement (main.f90:274-299):
+# (first operator, second operator, sign flag)
+COR_KINDS = {"density": (num_elec, num_elec, "B"), "spin": (spin_sd, spin_sd, "B"),
+             "hopping": (elec_up, elec_down, "F")}
+
+
+class DeviceRunner:
+    """Runs programs on torch device tensors (the product path).  The
+    correlation driver takes any object with these five methods, so that the
+    same driver can execute the programs' task tables elsewhere."""
+
+    def __init__(self, device):
+        self.device = device
+
+    def put(self, a):
+        import torch
+        return torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(self.device)
+
+    def run(self, pg, ops, x, clear=False):
+        import torch
+        n = max(1, pg.sizes["ndata"])
+        y = (torch.zeros if clear else torch.empty)(n, dtype=torch.float64, device=self.device)
+        pg.run(ops, x, y)
+        return y
+
+    def cat(self, a, b):
+        import torch
+        return torch.cat([a, b])
+
+    def head(self, a, n):
+        return a[:n]
+
+    def values(self, y, nvec, psi):
+        D = psi.numel()
+        return (y[:nvec * D].view(nvec, D) @ psi).cpu().numpy()
+
+
+def _extent(ops: OpSet) -> int:
+    sub, off = np.asarray(ops.sub).reshape(-1, 5), np.asarray(ops.off)
+    return int((off + sub[:, 3].astype(np.int64) * sub[:, 4]).max()) if off.size else 0
+
+
+def _concat(a: OpSet, ad, b: OpSet, bd, runner):
+    """Operator sets a and b as one (a's operators first) with their data."""
+    na = _extent(a)
+    hdr = np.concatenate([np.asarray(a.hdr).reshape(-1, 4), np.asarray(b.hdr).reshape(-1, 4)]).astype(np.int32)
+    sub = np.concatenate([np.asarray(a.sub).reshape(-1, 5), np.asarray(b.sub).reshape(-1, 5)]).astype(np.int32)
+    off = np.concatenate([np.asarray(a.off), np.asarray(b.off) + na]).astype(np.int64)
+    data = runner.cat(runner.head(ad, na), runner.head(bd, _extent(b)))
+    return OpSet(b.length, hdr, sub, off, np.zeros(na + _extent(b))), data
+
+
+def _step(pg, length, runner, ops_data, x):
+    """Run a transport-like program; returns (output OpSet layout, data)."""
+    hdr, sub, off = pg.layout()
+    y = runner.run(pg, ops_data, x)
+    return OpSet(length, hdr, sub, off, np.zeros(max(1, pg.sizes["ndata"]))), y
+
+
+def pair_correlations(kind, up_bias, sid1, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns, wsec, psi,
+                      runner=None):
+    """The two-point functions of Get_density_cor (code/tjsq/measure.f90:
+    3110-3223) between system-side site sid1 and every later site of the
+    sys_len + env_len sites: returns cor (lattice order, entry sid - 1; NaN
+    for sid <= sid1), each value scaled as that routine scales it
+    (sqrt(1 + 2k), negated for k = 1).  kind (COR_KINDS): "density"
+    <n_i n_j>, "spin" <S_i . S_j>, "hopping" the spin-summed
+    <c^+_i c_j>.
+
+    The operator of site sid1 is formed at its block length and carried to
+    the block next to the bond (Get_sys_oper_ndia, Change_sys_oper_ndia,
+    code/tjsq/get_sys.f90:998-1238), one ndia transport program per length.
+    On the way, at every length L the product with site L's operator is
+    formed (Get_sys2b, measure.f90:2785-2819: the rank-0 coupling of
+    block_mul_block_ndia, projected) and carried on as a diagonal operator;
+    all products travel together and are measured at the bond
+    (measure_sys_block).  The environment's operators of every site travel
+    together too (the reference carries each one separately), and one
+    correlation program contracts the system operator with all of them, the
+    environment bond site and the system bond site (sys_block_env_block_,
+    sys_block_env_site_ and sys_block_site_cor_ndia)."""
+    if runner is None:
+        runner = DeviceRunner(psi.device)
+    f1, f2, flag = COR_KINDS[kind]
+    o1, o2 = f1(up_bias), f2(up_bias)
+    k1, k2 = int(o1.hdr[0][2]), int(o2.hdr[0][2])
+    if not 1 <= sid1 <= sys_len - 1 or env_len < 2:
+        raise ValueError("sid1 must lie in the system block (1..sys_len-1) and the environment hold a block")
+    n = sys_len + env_len
+    scale = math.sqrt(1.0 + k1) * (-1.0 if k1 == 2 else 1.0)
+    cor = np.full(n, np.nan)
+    one = runner.put(np.zeros(1))
+    # the system operator at block length sid1
+    if sid1 == 1:
+        cur, cd = o1, runner.put(o1.data)
+    else:
+        t = sys_truns[sid1]
+        cur, cd = _step(capi.Program.ndia_transport(0, flag, None, o1, sys_bases[sid1], t.ops), sid1, runner,
+                        t.data, one)
+    prods, pdata, labels = None, None, []
+    for L in range(sid1 + 1, sys_len):
+        t = sys_truns[L]
+        new, nd = _step(capi.Program.pair_ndia(0, flag, cur, o2, sys_bases[L], t.ops), L, runner, t.data, cd)
+        if prods is not None:
+            prods, pdata = _step(capi.Program.dia_transport(prods, None, sys_bases[L], t.ops), L, runner, t.data,
+                                 pdata)
+            prods, pdata = _concat(prods, pdata, new, nd, runner)
+        else:
+            prods, pdata = new, nd
+        labels.append(L)
+        cur, cd = _step(capi.Program.ndia_transport(0, flag, cur, None, sys_bases[L], t.ops), L, runner, t.data, cd)
+    if prods is not None:
+        mp = capi.Program.measure(0, prods, None, sys_bases[sys_len], wsec)
+        vals = runner.values(runner.run(mp, pdata, psi, clear=True), len(labels), psi)
+        for sid2, v in zip(labels, vals):
+            cor[sid2 - 1] = scale * v
+    # the environment operators of sites 1..env_len-1 (counted from the outer end)
+    ecur, ed = o2, runner.put(o2.data)
+    for L in range(2, env_len):
+        t = env_truns[L]
+        ecur, ed = _step(capi.Program.ndia_transport(1, flag, ecur, o2, env_bases[L], t.ops), L, runner, t.data, ed)
+    ne = env_len - 1
+    cp = capi.Program.cor_ndia(flag, cur, ecur, k2 != 0, o2, k2 == 1, o2, k2 == 1, sys_bases[sys_len],
+                               env_bases[env_len], wsec)
+    sys_ext = _extent(cur)
+    ops = runner.cat(runner.head(cd, sys_ext), runner.head(ed, _extent(ecur)))
+    vals = runner.values(runner.run(cp, ops, psi, clear=True), ne + 2, psi)
+    for q in range(ne):
+        cor[n - q - 1] = scale * vals[q]
+    cor[sys_len] = scale * vals[ne]
+    cor[sys_len - 1] = scale * vals[ne + 1]
+    return cor
+
+
+def density_cor(kind, up_bias, xi, xj, ny, ij, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns, wsec,
+                psi, runner=None):
+    """Get_density_cor's matrix oper_ndia (measure.f90:3110-3223) on the
+    sys_len + env_len sites: for every sid1 in xi..xj the same-side sites
+    sid1 + ny + ij, + 2 ny, ... below sys_len and the sites from sys_len on
+    whose distance from sid1 + ij is a multiple of ny (the routine's
+    mod(sid1 + ij - sid3, ny) >= 0 test); other entries 0."""
+    n = sys_len + env_len
+    out = np.zeros((n, n))
+    for sid1 in range(xi, xj + 1):
+        cor = pair_correlations(kind, up_bias, sid1, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns,
+                                wsec, psi, runner)
+        for sid2 in range(sid1 + ny + ij, sys_len, ny):
+            out[sid1 - 1, sid2 - 1] = cor[sid2 - 1]
+        for sid3 in range(sys_len, n + 1):
+            a = sid1 + ij - sid3
+            if a >= 0 or a % ny == 0:
+                out[sid1 - 1, sid3 - 1] = cor[sid3 - 1]
+    return out
+
+
 class _DeviceTrun:
     def __init__(self, ops, data):
         self.ops, self.data = ops, data

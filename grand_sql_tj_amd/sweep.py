@@ -377,6 +377,30 @@ class DMRG:
         return measure.correlation_matrix(self.model.up_bias, sl, el, self.sys_basis, self.env_basis, self.sys_trun,
                                           self.env_trun, self.wsec, self.psi)
 
+    def measure_pair_correlations(self, kind, sid1):
+        """Get_density_cor's two-point functions (measure.pair_correlations)
+        of lattice site sid1 on the system side with every later site, in
+        the ground state of the last bond: kind "density" <n_i n_j>, "spin"
+        <S_i . S_j>, "hopping" sum over s of <c^+_is c_js>."""
+        from . import measure
+        if self.psi is None:
+            raise RuntimeError("no ground state yet")
+        sl, el = self.bond[0] + 1, self.bond[1] + 1
+        return measure.pair_correlations(kind, self.model.up_bias, sid1, sl, el, self.sys_basis, self.env_basis,
+                                         self.sys_trun, self.env_trun, self.wsec, self.psi)
+
+    def measure_line_correlations(self, kind, xi, xj, ij=0):
+        """Get_density_cor's matrix oper_ndia (measure.density_cor) for the
+        sites xi..xj: the pairs along the lattice's legs, as Measurement
+        writes them to density_cor.dat, spin_cor.dat and elec_cor.dat
+        (main.f90:272-318)."""
+        from . import measure
+        if self.psi is None:
+            raise RuntimeError("no ground state yet")
+        sl, el = self.bond[0] + 1, self.bond[1] + 1
+        return measure.density_cor(kind, self.model.up_bias, xi, xj, self.model.ny, ij, sl, el, self.sys_basis,
+                                   self.env_basis, self.sys_trun, self.env_trun, self.wsec, self.psi)
+
     def ground_state(self, lanerr, trun_err, kept_max, kept_maxf, sweep_num, nsite0=None):
         """Get_Ground_State (main.f90:96-187) for a fresh run (restart1 =
         restart2 = 0, iter1 = 0), with the kept-state settings of main.f90:

@@ -155,6 +155,9 @@ typedef struct tjsq_lanczos_result {
 
 /* 6j symbol on doubled arguments (W6JS). */
 double tjsq_w6js(int j1, int j2, int j3, int l1, int l2, int l3);
+/* 3j symbol (j1 j2 j3; m1 m2 m3), all arguments twice the angular momentum
+ * (W3JS, code/tjsq/block.f90:1522-1560). */
+double tjsq_w3js(int j1, int j2, int j3, int m1, int m2, int m3);
 
 /* Get_basis: fuse a block with a site.  blk[3*i] = {num_up, num_down, dim}
  * of the block's sectors in order, site[3*j] likewise.  sec must hold
@@ -394,6 +397,53 @@ int tjsq_dia_transport_pairs_create(const tjsq_opset* ops, int nprod, const tjsq
  * are not written). */
 int tjsq_measure_create(int side, const tjsq_opset* ops, const tjsq_opset* site, const tjsq_basis* basis, int nw,
                         const int* wsec, tjsq_program** prog);
+
+/* One step of the transport of rank-k operators along a block (the steps of
+ * Get_sys_oper_ndia / Get_env_oper_ndia and Change_sys_oper_ndia /
+ * Change_env_oper_ndia, code/tjsq/get_sys.f90:998-1238, that Get_density_cor,
+ * code/tjsq/measure.f90:3110-3223, runs per site): every operator of `ops`
+ * (at block length L; may be NULL) embedded into the fused basis `basis` of
+ * length L + 1 (update_block_ndia_sys for side 0, update_block_ndia_env for
+ * side 1, code/tjsq/block.f90:503-920), then the new site's operator `site`
+ * (one operator of 1x1 sub-blocks, host data; may be NULL) embedded
+ * (update_site_ndia_sys / _env, :925-1305), each projected with `trun`
+ * (update_trun_ndia, :1360-1503).  flag is 'B' or 'F', the reference's
+ * Types argument (fermion signs of the embeddings).  Run with ops = trun
+ * data, x = the data of `ops`. */
+int tjsq_ndia_transport_create(int side, int flag, const tjsq_opset* ops, const tjsq_opset* site,
+                               const tjsq_basis* basis, const tjsq_opset* trun, tjsq_program** prog);
+
+/* The same-side two-site operators of Get_sys2b (side 0) / Get_env2b
+ * (side 1) (code/tjsq/measure.f90:2748-2819) at the block length L where the
+ * second site joins: each operator of `ops` (rank k, block length L - 1)
+ * embedded into `basis` (update_block_ndia_*), the site operator `site`
+ * embedded (update_site_ndia_*), their rank-0 coupling (block_mul_block_ndia
+ * with down_dif 0, code/tjsq/block.f90:1956-2197), projected with `trun`
+ * (update_trun_dia).  Output: one diagonal operator at length L per operator
+ * of `ops`, to be carried on with tjsq_dia_transport_create and measured
+ * with tjsq_measure_create.  Run with ops = trun data, x = the data of
+ * `ops`. */
+int tjsq_pair_ndia_create(int side, int flag, const tjsq_opset* ops, const tjsq_opset* site, const tjsq_basis* basis,
+                          const tjsq_opset* trun, tjsq_program** prog);
+
+/* The across-the-bond correlations of Get_density_cor (code/tjsq/measure.f90:
+ * 3110-3223) for one system block operator `sys_op` (block length
+ * sys_len - 1): with every operator of `env_ops` (environment block length
+ * env_len - 1; may be NULL; sys_block_env_block_cor_ndia, :5241-5480), with
+ * the environment bond-site operator `env_site` (may be NULL;
+ * sys_block_env_site_cor_ndia, :5482-5706) and with the system bond-site
+ * operator `sys_site` (may be NULL; sys_block_site_cor_ndia, :4939-5083).
+ * *_trans != 0 applies block_transfer_trans (code/tjsq/block.f90:1676-1706)
+ * to that operand first, as Get_density_cor does.  sys_basis / env_basis are
+ * the wavefunction's fused bases (sys_bsm(sys_len), env_bsm(env_len)), wsec
+ * its sector list.  The output is one vector of the wavefunction's length per
+ * operator of env_ops, then env_site's, then sys_site's: each routine's
+ * value = psi . y_q.  Run with ops = the data of sys_op followed by the data
+ * of env_ops, x = psi, y cleared first. */
+int tjsq_cor_ndia_create(int flag, const tjsq_opset* sys_op, const tjsq_opset* env_ops, int env_trans,
+                         const tjsq_opset* env_site, int env_site_trans, const tjsq_opset* sys_site, int sys_site_trans,
+                         const tjsq_basis* sys_basis, const tjsq_basis* env_basis, int nw, const int* wsec,
+                         tjsq_program** prog);
 
 /* Get_density_matrix_sys (side 0) / _env (side 1), levels >= 1 states summed
  * and normalised first (Get_density_matrix_*_new, code/tjsq/den_mat.f90:

@@ -199,6 +199,10 @@ def ref_lib():
             lib.refb_cross_cor.argtypes = [C.c_int, C.c_int, C.c_int, I32P, F64P, F64P]
             lib.refb_measure_density.restype = C.c_int
             lib.refb_measure_density.argtypes = [C.c_int, C.c_int, C.c_int, I32P, F64P, C.c_int, F64P]
+            lib.refb_w3js.restype = C.c_double
+            lib.refb_w3js.argtypes = [C.c_int] * 6
+            lib.refb_density_cor.restype = C.c_int
+            lib.refb_density_cor.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, I32P, F64P, F64P]
             _libs["ref"] = lib
         return _libs["ref"]
 
@@ -793,6 +797,47 @@ def ref_cross_correlation(model, workdir, sys_len, env_len, sys_bases, env_bases
         cor = np.zeros(sys_len * env_len)
         lib.refb_cross_cor(sys_len, env_len, w.shape[0], w, np.ascontiguousarray(x, np.float64), cor)
     return cor.reshape(env_len, sys_len).T
+
+
+DENSITY_COR_KINDS = {"density": 0, "spin": 1, "hopping": 2}
+
+
+def ref_w3js(*j) -> float:
+    """W3JS of the reference build."""
+    global _ref_ready
+    with _lock:
+        lib = ref_lib()
+        if not _ref_ready:
+            with _quiet():
+                lib.refb_init(6, 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1e-10)
+            _ref_ready = True
+        return lib.refb_w3js(*[int(v) for v in j])
+
+
+def ref_density_cor(model, workdir, kind, sid1, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns, wsec,
+                    x):
+    """The pairs of Get_density_cor (measure.f90:3110-3223) of system-side
+    site sid1 with every later site of the chain, computed with the
+    reference's own transports, products and cor_ndia routines (kind
+    "density": num_elec x num_elec, "spin": st_sd x st_sd, "hopping":
+    st_elec_up x st_elec_down with fermion signs).  Returns cor (length
+    sys_len + env_len, lattice order, entry sid - 1; NaN for sid <= sid1)."""
+    w = np.ascontiguousarray(wsec, np.int32).reshape(-1, 6)
+    n = sys_len + env_len
+    with _lock, _cwd(workdir):
+        lib = _ref_init(model)
+        for side, bases, truns in ((0, sys_bases, sys_truns), (1, env_bases, env_truns)):
+            for L, b in bases.items():
+                lib.refb_set_bsm(side, L, b.length, *_bs(b))
+            for L, t in truns.items():
+                lib.refb_set_truns(side, L, *_one(t))
+        cor = np.full(n, np.nan)
+        rc = lib.refb_density_cor(DENSITY_COR_KINDS[kind], sid1, sys_len, env_len, w.shape[0], w,
+                                  np.ascontiguousarray(x, np.float64), cor)
+    if rc != n:
+        raise RuntimeError(f"refb_density_cor returned {rc}")
+    cor[:sid1] = np.nan
+    return cor
 
 
 def ref_oper_cor_dia(model, workdir, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns, wsec, x):

@@ -1072,3 +1072,120 @@ integer(c_int) function refb_oper_cor_dia(sys_len, env_len, nw, wsec, x, cor) bi
   deallocate(od)
   refb_oper_cor_dia = Num_site
 end function refb_oper_cor_dia
+
+! W3JS of the reference (block.f90:1522-1560); refb_init must have run FACTRL.
+real(c_double) function refb_w3js(j1, j2, j3, m1, m2, m3) bind(C, name="refb_w3js")
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: j1, j2, j3, m1, m2, m3
+  real(8), external :: w3js
+  refb_w3js = w3js(j1, j2, j3, m1, m2, m3)
+end function refb_w3js
+
+! The pairs of Get_density_cor (measure.f90:3110-3223) of system-side site
+! sid1 with every later site of a chain of sys_len + env_len sites, with the
+! reference's own routines in the order that routine calls them (trun_idx
+! 1): cor(sid2) for the same-side sites sid2 = sid1+1 .. sys_len-1
+! (Get_sys2b + measure_sys_block), cor(sys_len) (sys_block_site_cor_ndia),
+! cor(sys_len+1) (sys_block_env_site_cor_ndia) and cor(sid3) for
+! sid3 >= sys_len+2, environment site n - sid3 + 1
+! (sys_block_env_block_cor_ndia), each scaled as Get_density_cor scales it.
+! kind 0: num_elec x num_elec ('B'), 1: st_sd x st_sd ('B'),
+! 2: st_elec_up x st_elec_down ('F').  The reference's routine walks sid3 to
+! its compile-time Num_site and picks the sites in line with sid1; this one
+! walks the chain's own sites and returns them all.
+integer(c_int) function refb_density_cor(kind, sid1, sys_len, env_len, nw, wsec, x, cor) &
+    bind(C, name="refb_density_cor")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: kind, sid1, sys_len, env_len, nw
+  integer(c_int), intent(in) :: wsec(6, *)
+  real(c_double), intent(in) :: x(*)
+  real(c_double), intent(inout) :: cor(*)
+  type(Wavefunction) :: w
+  type(Total_Block) :: o1, o2, so, eo, nw1, nw2, tmp1
+  character(len=1) :: flags
+  integer :: n, sid2, sid3, eid1, trun_idx
+  real(8) :: value
+  refb_density_cor = -1
+  if (sid1 < 1 .or. sid1 > sys_len - 1 .or. env_len < 2) return
+  select case (kind)
+  case (0)
+    call block_transfer(num_elec, o1); call block_transfer(num_elec, o2); flags = 'B'
+  case (1)
+    call block_transfer(st_sd, o1); call block_transfer(st_sd, o2); flags = 'B'
+  case (2)
+    call block_transfer(st_elec_up, o1); call block_transfer(st_elec_down, o2); flags = 'F'
+  case default
+    return
+  end select
+  n = sys_len + env_len
+  trun_idx = 1
+  call put_wave(w, sys_len, env_len, nw, wsec, x)
+  do sid2 = sid1 + 1, sys_len - 1
+    call Get_sys2b(sid1, sid2, o1, o2, so, trun_idx, w, flags)
+    call measure_sys_block(value, so, sys_bsm(sys_len), w)
+    value = value * dsqrt(1.0d0 + o1%down_dif)
+    if (o1%down_dif == 2) value = -value
+    cor(sid2) = value
+    call deallocate_block(so)
+  end do
+  call Get_sys_oper_ndia(o1, sid1, nw1, sid1, trun_idx, .false., flags)
+  if (sid1 <= trun_idx) then
+    call block_transfer(nw1, nw2)
+  else
+    call update_trun_ndia(nw1, nw2, systruns(sid1))
+  end if
+  call deallocate_block(nw1)
+  call Change_sys_oper_ndia(nw2, sid1, so, sys_len - 1, trun_idx, .true., flags)
+  call deallocate_block(nw2)
+  do sid3 = sys_len, n
+    value = 0.0d0
+    if (sid3 == sys_len) then
+      if (o2%down_dif == 1) then
+        call block_transfer_trans(o2, tmp1)
+        call sys_block_site_cor_ndia(value, so, tmp1, sys_bsm(sys_len), w, flags)
+        call deallocate_block(tmp1)
+      else
+        call sys_block_site_cor_ndia(value, so, o2, sys_bsm(sys_len), w, flags)
+      end if
+    else if (sid3 == sys_len + 1) then
+      if (o2%down_dif == 1) then
+        call block_transfer_trans(o2, tmp1)
+        call sys_block_env_site_cor_ndia(value, so, tmp1, sys_bsm(sys_len), env_bsm(env_len), w, flags)
+        call deallocate_block(tmp1)
+      else
+        call sys_block_env_site_cor_ndia(value, so, o2, sys_bsm(sys_len), env_bsm(env_len), w, flags)
+      end if
+    else
+      eid1 = n - sid3 + 1
+      call Get_env_oper_ndia(o2, eid1, nw1, eid1, trun_idx, .false., flags)
+      if (eid1 <= trun_idx) then
+        call block_transfer(nw1, nw2)
+      else
+        call update_trun_ndia(nw1, nw2, envtruns(eid1))
+      end if
+      call deallocate_block(nw1)
+      call Change_env_oper_ndia(nw2, eid1, eo, env_len - 1, trun_idx, .true., flags)
+      call deallocate_block(nw2)
+      if (o2%down_dif /= 0) then
+        call block_transfer_trans(eo, tmp1)
+        call sys_block_env_block_cor_ndia(value, so, tmp1, sys_bsm(sys_len), env_bsm(env_len), w, flags)
+        call deallocate_block(tmp1)
+      else
+        call sys_block_env_block_cor_ndia(value, so, eo, sys_bsm(sys_len), env_bsm(env_len), w, flags)
+      end if
+      call deallocate_block(eo)
+    end if
+    value = value * dsqrt(1.0d0 + o1%down_dif)
+    if (o1%down_dif == 2) value = -value
+    cor(sid3) = value
+  end do
+  call deallocate_block(so)
+  call deallocate_block(o1)
+  call deallocate_block(o2)
+  call deallocate_wave(w)
+  refb_density_cor = n
+end function refb_density_cor

@@ -106,3 +106,77 @@ def ground_energy(model, n_elec):
     if H.shape[0] < 400:
         return float(np.linalg.eigvalsh(H.toarray())[0])
     return float(spla.eigsh(H, k=1, which="SA", tol=1e-13)[0][0])
+
+
+def ground_state(model, n_elec):
+    """(E0, g, states) of the (N = n_elec, S_z = 0) sector."""
+    H, st = hamiltonian(model, n_elec)
+    if H.shape[0] <= 2000:
+        w, v = np.linalg.eigh(H.toarray())
+    else:
+        w, v = spla.eigsh(H, k=1, which="SA", tol=1e-13)
+    return float(w[0]), v[:, 0], st
+
+
+def spin_correlation(model, g, st):
+    """<S_i . S_j> (n x n) in the state g over the basis st: S^z S^z on the
+    diagonal of the basis, (S^+_i S^-_j + S^-_i S^+_j) / 2 with the string
+    signs of the exchange term of hamiltonian()."""
+    n = model.nsite
+    full = (1 << n) - 1
+    up, dn = st >> n, st & full
+    w = g * g
+    out = np.zeros((n, n))
+    sz = np.array([0.5 * (((up >> b) & 1) - ((dn >> b) & 1)) for b in range(n)], np.float64)
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                out[i, i] = 0.75 * float((((up | dn) >> i) & 1) @ w)
+                continue
+            v = float((sz[i] * sz[j]) @ w)
+            bi, bj = 1 << i, 1 << j
+            ui, uj, di, dj = (up & bi) != 0, (up & bj) != 0, (dn & bi) != 0, (dn & bj) != 0
+            for sel in ((ui & dj), (di & uj)):
+                idx = np.nonzero(sel)[0]
+                if idx.size == 0:
+                    continue
+                sign = (-1.0) ** (_popcount_between(up[idx], i, j) + _popcount_between(dn[idx], i, j))
+                tgt = np.searchsorted(st, ((up[idx] ^ bi ^ bj) << n) | (dn[idx] ^ bi ^ bj))
+                v += float((-0.5 * sign * g[tgt]) @ g[idx])
+            out[i, j] = v
+    return out
+
+
+def hopping_correlation(model, g, st):
+    """sum over s of <c^+_is c_js> (n x n, i != j) in the state g, with the
+    string signs of the hopping term of hamiltonian()."""
+    n = model.nsite
+    full = (1 << n) - 1
+    up, dn = st >> n, st & full
+    out = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                continue
+            bi, bj = 1 << i, 1 << j
+            v = 0.0
+            for spin in (0, 1):
+                m = up if spin == 0 else dn
+                ok = ((m & bj) != 0) & (((up | dn) & bi) == 0)
+                idx = np.nonzero(ok)[0]
+                if idx.size == 0:
+                    continue
+                mm = m[idx] ^ bj ^ bi
+                sign = (-1.0) ** _popcount_between(m[idx], i, j)
+                new = (mm << n) | dn[idx] if spin == 0 else (up[idx] << n) | mm
+                v += float((sign * g[np.searchsorted(st, new)]) @ g[idx])
+            out[i, j] = v
+    return out
+
+
+def density_correlation(model, g, st):
+    n = model.nsite
+    full = (1 << n) - 1
+    up, dn = st >> n, st & full
+    occ = np.array([((up >> b) & 1) + ((dn >> b) & 1) for b in range(n)], dtype=np.float64)
+    return (occ * (g * g)) @ occ.T
