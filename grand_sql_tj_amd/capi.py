@@ -96,6 +96,7 @@ EXPORTS = ["tjsq_w6js", "tjsq_get_basis", "tjsq_get_super_basis", "tjsq_hmerge_t
            "tjsq_block_update_create", "tjsq_trun_project_create", "tjsq_density_create", "tjsq_wave_transform_create",
            "tjsq_dia_transport_create", "tjsq_dia_transport_pairs_create", "tjsq_measure_create",
            "tjsq_w3js", "tjsq_ndia_transport_create", "tjsq_pair_ndia_create", "tjsq_cor_ndia_create",
+           "tjsq_product_ndia_create", "tjsq_site_cor_create",
            "tjsq_program_sizes",
            "tjsq_program_layout", "tjsq_program_tables", "tjsq_program_stats", "tjsq_program_run",
            "tjsq_program_run_host", "tjsq_truncation_operator_host",
@@ -150,6 +151,8 @@ def lib():
             "tjsq_ndia_transport_create": [I, I, P, P, P, P, P],
             "tjsq_pair_ndia_create": [I, I, P, P, P, P, P],
             "tjsq_cor_ndia_create": [I, P, P, I, P, I, P, I, P, P, I, P, P],
+            "tjsq_product_ndia_create": [I, I, I, P, P, P, P, P],
+            "tjsq_site_cor_create": [I, I, P, I, P, I, P, I, P, P, I, P, P],
             "tjsq_density_create": [I, I, I, P, I, P],
             "tjsq_wave_transform_create": [I, P, P, P, P, I, P, I, P, P],
             "tjsq_program_sizes": [P, P],
@@ -646,6 +649,39 @@ class Program:
                                          int(env_site_trans), ref(ss), int(sys_site_trans), C.byref(sb),
                                          C.byref(eb), w.shape[0], w.ctypes.data, C.byref(h)),
               "tjsq_cor_ndia_create")
+        return cls(h)
+
+    @classmethod
+    def product_ndia(cls, side, flag, rank, ops, site, basis, trun):
+        """The couplings [O x s]^rank of every operator of ops (length L - 1)
+        with the site operator joining at length L, projected with trun:
+        operators at length L.  Run with ops = trun data, x = ops data."""
+        keep = _Keep()
+        os_, ss = opset_struct(ops, keep), opset_struct(site, keep)
+        bs, ts = basis_struct(basis, keep), opset_struct(trun, keep)
+        h = C.c_void_p()
+        check(lib().tjsq_product_ndia_create(int(side), ord(flag), int(rank), C.byref(os_), C.byref(ss),
+                                             C.byref(bs), C.byref(ts), C.byref(h)), "tjsq_product_ndia_create")
+        return cls(h)
+
+    @classmethod
+    def site_cor(cls, kind, flag, sys_site, sys_site_trans, env_ops, env_trans, env_site, env_site_trans, sys_basis,
+                 env_basis, wsec):
+        """The bond-site contractions of Get_SC_Order: kind 0 system site x
+        environment site, 1 system site x each environment block operator,
+        2 each environment block operator x environment site.  Run with
+        ops = env_ops data (kind 0: any), x = psi, y zeroed."""
+        keep = _Keep()
+        ss = opset_struct(sys_site, keep) if sys_site is not None else None
+        eo = opset_struct(env_ops, keep) if env_ops is not None and env_ops.nops > 0 else None
+        es = opset_struct(env_site, keep) if env_site is not None else None
+        sb, eb = basis_struct(sys_basis, keep), basis_struct(env_basis, keep)
+        w = np.ascontiguousarray(wsec, np.int32).reshape(-1, 6)
+        ref = lambda v: C.byref(v) if v is not None else None
+        h = C.c_void_p()
+        check(lib().tjsq_site_cor_create(int(kind), ord(flag), ref(ss), int(sys_site_trans), ref(eo), int(env_trans),
+                                         ref(es), int(env_site_trans), C.byref(sb), C.byref(eb), w.shape[0],
+                                         w.ctypes.data, C.byref(h)), "tjsq_site_cor_create")
         return cls(h)
 
     @classmethod

@@ -2664,6 +2664,31 @@ int tjsq_cor_ndia_create(int flag, const tjsq_opset* sys_op, const tjsq_opset* e
   return make_program(pg, 0, 1, prog);
 }
 
+int tjsq_product_ndia_create(int side, int flag, int rank, const tjsq_opset* ops, const tjsq_opset* site,
+                             const tjsq_basis* basis, const tjsq_opset* trun, tjsq_program** prog) {
+  if (!ops || !site || !basis || !trun || !prog) return fail(TJSQ_ERR_ARG, "tjsq_product_ndia_create: null argument");
+  *prog = nullptr;
+  tjsq::Program pg;
+  std::string msg;
+  int rc = tjsq::build_product_ndia_program(side, (char)flag, rank, *ops, *site, *basis, *trun, tjsq::Shape(), pg,
+                                            msg);
+  if (rc) return fail(rc, "tjsq_product_ndia_create: " + msg);
+  return make_program(pg, 0, 1, prog);
+}
+
+int tjsq_site_cor_create(int kind, int flag, const tjsq_opset* sys_site, int sys_site_trans, const tjsq_opset* env_ops,
+                         int env_trans, const tjsq_opset* env_site, int env_site_trans, const tjsq_basis* sys_basis,
+                         const tjsq_basis* env_basis, int nw, const int* wsec, tjsq_program** prog) {
+  if (!sys_basis || !env_basis || !wsec || !prog) return fail(TJSQ_ERR_ARG, "tjsq_site_cor_create: null argument");
+  *prog = nullptr;
+  tjsq::Program pg;
+  std::string msg;
+  int rc = tjsq::build_site_cor_program(kind, sys_site, sys_site_trans, env_ops, env_trans, env_site, env_site_trans,
+                                        *sys_basis, *env_basis, nw, wsec, (char)flag, tjsq::Shape(), pg, msg);
+  if (rc) return fail(rc, "tjsq_site_cor_create: " + msg);
+  return make_program(pg, 0, 1, prog);
+}
+
 int tjsq_density_create(int side, int len, int nw, const int* wsec, int levels, tjsq_program** prog) {
   if (!wsec || !prog || levels < 1) return fail(TJSQ_ERR_ARG, "tjsq_density_create: bad argument");
   *prog = nullptr;

@@ -1069,57 +1069,64 @@ int build_cor_ndia_program(const tjsq_opset& sys_op, const tjsq_opset* env_ops, 
 
 namespace {
 
-// block_mul_block_ndia (block.f90:1956-2197) with Flag1 = Flag2 = 'N', coef 1
-// and new_oper%down_dif = 0: the rank-0 coupling P = [O1 x O2]^0 of two
-// embedded operators over one fused basis.  Per basis sector i (ket) and
-// intermediate 2S m, P[i] += c(i, m) O1[m -> i] O2[i -> m] with
-//   c = sum over mu of (J k1 m; -J J-mu mu) (m k2 J; -mu mu-J J) (-1)^((m-mu)/2)
-//       (k1 k2 0; J-mu mu-J 0) (-1)^((k2-k1)/2) / (J 0 J; -J 0 J)
-// (J = 2S of sector i, k1, k2 the ranks).  The product of a placement of
-// O1 with an identity placement of O2 over the same sub-sector is one
-// placement of O1's data; products of two data placements are not formed.
-SymOp mul_ndia_scalar(const SymOp& o1, const SymOp& o2, const BasisView& bs, int len, Ctx& cx) {
+// block_mul_block_ndia (block.f90:1956-2197) with Flag1 = Flag2 = 'N' and
+// coef 1: the rank-dd coupling P = [O1 x O2]^dd of two embedded operators
+// over one fused basis (new_oper%down_dif = dd, set by the caller there).
+// Per basis sector i (ket, 2S = J), bra 2S nJ in |J - dd| .. J + dd and
+// intermediate 2S m in |J - k2| .. J + k2,
+//   P[i, nJ - J] += c O1[(m) -> nJ] O2[i -> m],
+//   c = sum over mu of (nJ k1 m; -nJ nJ-mu mu) (m k2 J; -mu mu-J J) (-1)^((m-mu)/2)
+//       (k1 k2 dd; nJ-mu mu-J J-nJ) (-1)^((k2-k1+nJ-J)/2) sqrt(dd+1)
+//       / (nJ dd J; -nJ nJ-J J)
+// (k1, k2 the ranks).  The product of a placement of O1 with an identity
+// placement of O2 over the same sub-sector is one placement of O1's data;
+// products of two data placements are not formed.
+SymOp mul_ndia(const SymOp& o1, const SymOp& o2, int dd, const BasisView& bs, int len, Ctx& cx) {
   SymOp p;
-  p.present = true; p.len = len; p.up_dif = o1.up_dif + o2.up_dif; p.down_dif = 0;
+  p.present = true; p.len = len; p.up_dif = o1.up_dif + o2.up_dif; p.down_dif = dd;
   const int k1 = o1.down_dif, k2 = o2.down_dif;
-  auto find = [](const SymOp& o, int up, int down, int dd) {
+  auto find = [](const SymOp& o, int up, int down, int d) {
     for (size_t x = 0; x < o.sub.size(); ++x)
-      if (o.sub[x].num_up == up && o.sub[x].num_down == down && o.sub[x].down_dif == dd) return (int)x;
+      if (o.sub[x].num_up == up && o.sub[x].num_down == down && o.sub[x].down_dif == d) return (int)x;
     return -1;
   };
   for (const MapSec& sec : bs.sec) {
     const int J = sec.new_down;
-    const int bsi = find_sec(bs, sec.new_up + p.up_dif, J);
-    if (bsi < 0) continue;
-    p.sub.push_back({sec.new_up, J, 0, bs.sec[bsi].dim, sec.dim, {}});
-    SymSub& d = p.sub.back();
-    for (int m = std::abs(J - k2); m <= J + k2; m += SU) {
-      const int i2 = find(o2, sec.new_up, J, m - J);
-      const int i1 = find(o1, sec.new_up + o2.up_dif, m, J - m);
-      if (i1 < 0 || i2 < 0) continue;
-      const double c11 = w3js(J, 0, J, -J, 0, J);
-      if (c11 == 0.0) { cx.fail("vanishing normalising 3j symbol"); return p; }
-      double c1 = 0.0;
-      for (int mu = -m; mu <= m; mu += SU) {
-        double c12 = w3js(J, k1, m, -J, J - mu, mu);
-        c12 = c12 * w3js(m, k2, J, -mu, mu - J, J) * par((m - mu) / 2);
-        c12 = c12 * w3js(k1, k2, 0, J - mu, mu - J, 0) * par((-k1 + k2 + 0) / 2) * std::sqrt(0 + 1.0);
-        c1 = c1 + c12;
-      }
-      if (c1 == 0.0) continue;
-      c1 = 1.0 * c1 / c11;
-      for (const Place& a : o1.sub[i1].pl)
-        for (const Place& b : o2.sub[i2].pl) {
-          if (a.c0 + a.w <= b.r0 || b.r0 + b.h <= a.c0) continue;
-          if (a.c0 != b.r0 || a.w != b.h) { cx.fail("operator product over misaligned placements"); return p; }
-          if (a.src >= 0 && b.src >= 0) { cx.fail("operator product of two data blocks is not supported"); return p; }
-          const double c = c1 * a.coef * b.coef;
-          if (c == 0.0) continue;
-          if (b.src < 0)
-            d.pl.push_back({a.r0, b.c0, a.h, b.w, c, a.src, a.ld, a.t});
-          else
-            d.pl.push_back({a.r0, b.c0, a.h, b.w, c, b.src, b.ld, b.t});
+    for (int nJ = std::abs(J - dd); nJ <= J + dd; nJ += SU) {
+      const int bsi = find_sec(bs, sec.new_up + p.up_dif, nJ);
+      if (bsi < 0) continue;
+      p.sub.push_back({sec.new_up, J, nJ - J, bs.sec[bsi].dim, sec.dim, {}});
+      SymSub& d = p.sub.back();
+      const int kq = nJ - J;
+      for (int m = std::abs(J - k2); m <= J + k2; m += SU) {
+        const int i2 = find(o2, sec.new_up, J, m - J);
+        const int i1 = find(o1, sec.new_up + o2.up_dif, m, nJ - m);
+        if (i1 < 0 || i2 < 0) continue;
+        const double c11 = w3js(nJ, dd, J, -nJ, kq, J);
+        if (c11 == 0.0) { cx.fail("vanishing normalising 3j symbol"); return p; }
+        double c1 = 0.0;
+        for (int mu = -m; mu <= m; mu += SU) {
+          const int j5 = nJ - mu;
+          double c12 = w3js(nJ, k1, m, -nJ, j5, mu);
+          c12 = c12 * w3js(m, k2, J, -mu, mu - J, J) * par((m - mu) / 2);
+          c12 = c12 * w3js(k1, k2, dd, j5, mu - J, -kq) * par((-k1 + k2 + kq) / 2) * std::sqrt(dd + 1.0);
+          c1 = c1 + c12;
         }
+        if (c1 == 0.0) continue;
+        c1 = 1.0 * c1 / c11;
+        for (const Place& a : o1.sub[i1].pl)
+          for (const Place& b : o2.sub[i2].pl) {
+            if (a.c0 + a.w <= b.r0 || b.r0 + b.h <= a.c0) continue;
+            if (a.c0 != b.r0 || a.w != b.h) { cx.fail("operator product over misaligned placements"); return p; }
+            if (a.src >= 0 && b.src >= 0) { cx.fail("operator product of two data blocks is not supported"); return p; }
+            const double c = c1 * a.coef * b.coef;
+            if (c == 0.0) continue;
+            if (b.src < 0)
+              d.pl.push_back({a.r0, b.c0, a.h, b.w, c, a.src, a.ld, a.t});
+            else
+              d.pl.push_back({a.r0, b.c0, a.h, b.w, c, b.src, b.ld, b.t});
+          }
+      }
     }
   }
   return p;
@@ -1152,10 +1159,276 @@ int build_pair_ndia_program(int side, char flag, const tjsq_opset& ops_in, const
   const SymOp so = upd_site_ndia(st, bs, basis.length, flag, side);
   for (const Op& o : ops) {
     if (o.up_dif + st.up_dif != 0) { msg = "the pair operator must conserve the particle number"; return TJSQ_ERR_ARG; }
-    pj.project(mul_ndia_scalar(upd_block_ndia(o, bs, basis.length, flag, side, cx), so, bs, basis.length, cx), true);
+    pj.project(mul_ndia(upd_block_ndia(o, bs, basis.length, flag, side, cx), so, 0, bs, basis.length, cx), true);
   }
   pg.x_elems = max_extent(ops_in);
   pg.ops_elems = max_extent(trun);
+  return finish(pg, shape, cx, msg);
+}
+
+// The general product of Get_sys_two / Get_sys_two1 / Get_SCOP_Operator_Env
+// (measure.f90:3798-3889, 6211-6263): each operator of ops (block length
+// L - 1) embedded into `basis` (update_block_ndia_*), the site operator
+// embedded (update_site_ndia_*), their rank-`rank` coupling
+// (block_mul_block_ndia), projected with T: update_trun_dia when the product
+// is a scalar that conserves the particle label, update_trun_ndia otherwise.
+int build_product_ndia_program(int side, char flag, int rank, const tjsq_opset& ops_in, const tjsq_opset& site,
+                               const tjsq_basis& basis, const tjsq_opset& trun, const Shape& shape, Program& pg,
+                               std::string& msg) {
+  pg = Program();
+  if ((side != 0 && side != 1) || (flag != 'B' && flag != 'F') || rank < 0 || rank > 4) {
+    msg = "bad side, sign flag or rank";
+    return TJSQ_ERR_ARG;
+  }
+  std::vector<Op> ops, tops;
+  int rc;
+  if ((rc = read_ops(ops_in, ops, msg))) return rc;
+  Site st;
+  if ((rc = read_site(site, st, msg))) return rc;
+  if ((rc = read_ops(trun, tops, msg))) return rc;
+  if (trun.nops != 1) { msg = "the truncation operator set must hold one operator"; return TJSQ_ERR_ARG; }
+  BasisView bs;
+  read_basis(basis, bs);
+  Ctx cx;
+  Projector pj{pg, tops[0], cx};
+  const SymOp so = upd_site_ndia(st, bs, basis.length, flag, side);
+  for (const Op& o : ops) {
+    if ((o.down_dif + st.down_dif + rank) % 2 != 0 || rank > o.down_dif + st.down_dif ||
+        rank < std::abs(o.down_dif - st.down_dif)) {
+      msg = "the ranks of the factors cannot couple to the requested rank";
+      return TJSQ_ERR_ARG;
+    }
+    const SymOp prod = mul_ndia(upd_block_ndia(o, bs, basis.length, flag, side, cx), so, rank, bs, basis.length, cx);
+    pj.project(prod, rank == 0 && prod.up_dif == 0);
+  }
+  pg.x_elems = max_extent(ops_in);
+  pg.ops_elems = max_extent(trun);
+  return finish(pg, shape, cx, msg);
+}
+
+namespace {
+
+// first sub-sector of the first sector (up, down) of b whose fields equal
+// the given ones; a field given as ANY is not compared (the reference's
+// searches compare three of the four block / site labels)
+constexpr int ANY = -1000000;
+const MapSub* find_sub(const BasisView& b, int up, int down, int bl_up, int bl_down, int st_up, int st_down) {
+  for (const MapSec& m : b.sec) {
+    if (m.new_up != up || m.new_down != down) continue;
+    for (const MapSub& n : m.sub)
+      if ((bl_up == ANY || n.bl_up == bl_up) && (bl_down == ANY || n.bl_down == bl_down) &&
+          (st_up == ANY || n.st_up == st_up) && (st_down == ANY || n.st_down == st_down))
+        return &n;
+  }
+  return nullptr;
+}
+
+}  // namespace
+
+// The bond-site contractions of Get_SC_Order (measure.f90:3226-3376):
+//   kind 0 sys_site_env_site_cor_ndia (:5709-5924)   sys_site x env_site
+//   kind 1 sys_site_env_block_cor_ndia (:5926-6154)  sys_site x each env_ops operator
+//   kind 2 env_block_site_cor_ndia (:5085-5238)      each env_ops operator x env_site
+// Output copy q holds Y_q with the routine's value = psi . Y_q (one copy for
+// kind 0, one per operator of env_ops otherwise).  *_trans applies
+// block_transfer_trans to that operand first.  The program's operator space
+// is env_ops' data (kinds 1, 2); x = psi, y cleared first.
+int build_site_cor_program(int kind, const tjsq_opset* sys_site, int sys_site_trans, const tjsq_opset* env_ops,
+                           int env_trans, const tjsq_opset* env_site, int env_site_trans,
+                           const tjsq_basis& sys_basis, const tjsq_basis& env_basis, int nw, const int* wsec,
+                           char flag, const Shape& shape, Program& pg, std::string& msg) {
+  pg = Program();
+  if (kind < 0 || kind > 2 || nw < 1 || !wsec || (flag != 'B' && flag != 'F')) {
+    msg = "bad correlation arguments";
+    return TJSQ_ERR_ARG;
+  }
+  const bool need_sys = kind <= 1, need_ops = kind >= 1, need_est = kind != 1;
+  if ((need_sys && !sys_site) || (need_ops && (!env_ops || env_ops->nops < 1)) || (need_est && !env_site)) {
+    msg = "missing operand for this contraction";
+    return TJSQ_ERR_ARG;
+  }
+  int rc;
+  Site sst, est;
+  if (need_sys && (rc = read_site(*sys_site, sst, msg))) return rc;
+  if (need_est && (rc = read_site(*env_site, est, msg))) return rc;
+  std::vector<Op> ev;
+  if (need_ops && (rc = read_ops(*env_ops, ev, msg))) return rc;
+  std::vector<VOp> B;
+  for (const Op& o : ev) B.push_back(view_op(o, env_trans != 0, 0));
+  const VOp S = view_site(sst, sys_site_trans != 0), E = view_site(est, env_site_trans != 0);
+  BasisView sb, eb;
+  read_basis(sys_basis, sb);
+  read_basis(env_basis, eb);
+  std::vector<WSec> W(nw);
+  int64_t D = 0;
+  for (int i = 0; i < nw; ++i) {
+    const int* w = wsec + 6 * i;
+    if (w[4] < 0 || w[5] < 0) { msg = "negative sector dimension"; return TJSQ_ERR_ARG; }
+    W[i] = {w[0], w[1], w[2], w[3], w[4], w[5], D};
+    D += (int64_t)w[4] * w[5];
+  }
+  auto wave_of = [&](int su, int sd, int eu, int ed) {
+    for (int m = 0; m < nw; ++m)
+      if (W[m].sys_up == su && W[m].sys_down == sd && W[m].env_up == eu && W[m].env_down == ed) return m;
+    return -1;
+  };
+  bool bad = false;
+  auto inside = [&](int w, int r0, int M, int c0, int N) {
+    if (r0 < 0 || c0 < 0 || r0 + M > W[w].sys_dim || c0 + N > W[w].env_dim) bad = true;
+    return !bad;
+  };
+  std::map<std::tuple<int, int, int, int>, int> region;
+  auto out_block = [&](int q, int w, int r0, int c0, int M, int N) {
+    auto key = std::make_tuple(q, w, r0, c0);
+    auto it = region.find(key);
+    if (it != region.end()) {
+      const OutBlock& b = pg.hp.out_blocks[it->second];
+      if (b.M != M || b.N != N) bad = true;
+      return it->second;
+    }
+    pg.hp.out_blocks.push_back({(int64_t)q * D + W[w].off + r0 + (int64_t)c0 * W[w].sys_dim,
+                                std::max(1, W[w].sys_dim), M, N, (int)pg.hp.out_blocks.size()});
+    const int id = (int)pg.hp.out_blocks.size() - 1;
+    region.emplace(key, id);
+    return id;
+  };
+  const int ncopy = kind == 0 ? 1 : (int)B.size();
+  for (int i = 0; i < nw && !bad; ++i) {
+    const WSec& wi = W[i];
+    if (wi.sys_dim == 0 || wi.env_dim == 0) continue;
+    const int j = find_sec(sb, wi.sys_up, wi.sys_down), k = find_sec(eb, wi.env_up, wi.env_down);
+    if (k < 0 || (kind <= 1 && j < 0)) continue;
+    if (kind == 2) {
+      // env_block_site_cor_ndia: the environment block operator and site
+      // operator on one sector; Y[:, new sub-sector] += c psi[:, x] B
+      for (size_t q = 0; q < B.size(); ++q) {
+        const int ud = B[q].up_dif, k1 = B[q].down_dif;
+        for (const MapSub& x : eb.sec[k].sub)
+          for (int bdd = -k1; bdd <= k1; bdd += SU) {
+            const int nbu = x.bl_up - ud, nbd = x.bl_down - bdd;
+            const int bid = first_vsub(B[q], nbu, nbd, bdd);
+            for (int sdd = -k1; sdd <= k1; sdd += SU) {
+              const int ntu = x.st_up + ud, ntd = x.st_down + sdd;
+              const int tid = first_vsub(E, x.st_up, x.st_down, sdd);
+              const MapSub* np = nullptr;
+              for (const MapSub& n : eb.sec[k].sub)
+                if (n.bl_up == nbu && n.bl_down == nbd && n.st_up == ntu && n.st_down == ntd) { np = &n; break; }
+              if (bid < 0 || tid < 0 || !np) continue;
+              const double sign = fsign(flag, 2 * wi.sys_down + x.st_down);
+              double c1 = 1.0 / std::sqrt(1.0 + k1) * w6js(nbd, x.bl_down, k1, x.st_down, ntd, wi.env_down);
+              c1 = c1 * par((x.bl_down + ntd + k1 + wi.env_down) / 2);
+              if (c1 == 0.0) continue;
+              const VSub& bs = B[q].sub[bid];
+              if (bs.row_dim != x.sdim || bs.sdim != np->sdim || !inside(i, 0, wi.sys_dim, x.spos, x.sdim) ||
+                  !inside(i, 0, wi.sys_dim, np->spos, np->sdim)) {
+                msg = "operator sub-block disagrees with the fused basis";
+                return TJSQ_ERR_ARG;
+              }
+              Contrib c{};
+              c.kind = C_RIGHT;
+              c.c = sign * E.sub[tid].val * c1;
+              c.x_off = wi.off + (int64_t)x.spos * wi.sys_dim; c.x_ld = wi.sys_dim;
+              c.b_off = bs.off; c.b_ld = B[q].t ? bs.sdim : bs.row_dim; c.b_t = B[q].t ? 1 : 0;
+              c.M = wi.sys_dim; c.K2 = x.sdim; c.N = np->sdim;
+              c.out_block = out_block((int)q, i, 0, np->spos, c.M, c.N);
+              if (c.c != 0.0 && c.M && c.N && c.K2) pg.hp.contrib.push_back(c);
+            }
+          }
+      }
+      continue;
+    }
+    const int ud = S.up_dif, k1 = S.down_dif;
+    for (const MapSub& x : sb.sec[j].sub)
+      for (const MapSub& y : eb.sec[k].sub)
+        for (int sdd = -k1; sdd <= k1; sdd += SU) {
+          const int nsu_st = x.st_up + ud, nsd_st = x.st_down + sdd;
+          const int sid = first_vsub(S, x.st_up, x.st_down, sdd);
+          const int nq = kind == 0 ? 1 : (int)B.size();
+          for (int q = 0; q < nq; ++q) {
+            const VOp& R = kind == 0 ? E : B[q];
+            // kind 0 walks st_down_dif2 over the system site's rank, kind 1
+            // bl_down_dif over the environment block operator's rank
+            const int rk = kind == 0 ? k1 : R.down_dif;
+            for (int edd = -rk; edd <= rk; edd += SU) {
+              int eid;
+              int nebu = 0, nebd = 0, netu = 0, netd = 0;
+              if (kind == 0) {
+                netu = y.st_up - ud; netd = y.st_down - edd;
+                eid = first_vsub(E, netu, netd, edd);
+              } else {
+                nebu = y.bl_up - ud; nebd = y.bl_down - edd;
+                eid = first_vsub(R, nebu, nebd, edd);
+              }
+              if (sid < 0 || eid < 0) continue;
+              for (int nsd = std::abs(wi.sys_down - k1); nsd <= wi.sys_down + k1; nsd += SU) {
+                const int nsu = wi.sys_up + ud, neu = wi.env_up - ud, ned = nsd;
+                const MapSub* xp = find_sub(sb, nsu, nsd, ANY, x.bl_down, nsu_st, nsd_st);
+                const MapSub* yp = kind == 0 ? find_sub(eb, neu, ned, ANY, y.bl_down, netu, netd)
+                                             : find_sub(eb, neu, ned, nebu, nebd, ANY, y.st_down);
+                const int w = wave_of(nsu, nsd, neu, ned);
+                if (!xp || !yp || w < 0) continue;
+                double c1, sign;
+                if (kind == 0) {
+                  sign = fsign(flag, wi.sys_down + x.bl_down);
+                  c1 = w6js(nsd, k1, wi.sys_down, x.st_down, x.bl_down, nsd_st);
+                  if (c1 == 0.0) continue;
+                  c1 *= w6js(ned, k1, wi.env_down, y.st_down, y.bl_down, netd);
+                  if (c1 == 0.0) continue;
+                  c1 *= std::sqrt((1.0 + ned) * (1.0 + wi.env_down) / (1.0 + k1));
+                  if (((y.bl_down + y.st_down + ned + k1) / 2 + (x.bl_down + x.st_down + nsd + k1) / 2) % 2 != 0)
+                    c1 = -c1;
+                } else {
+                  sign = fsign(flag, wi.sys_down + y.st_down + x.bl_down);
+                  c1 = w6js(nsd, k1, wi.sys_down, x.st_down, x.bl_down, nsd_st);
+                  if (c1 == 0.0) continue;
+                  c1 *= w6js(ned, k1, wi.env_down, y.bl_down, y.st_down, nebd);
+                  if (c1 == 0.0) continue;
+                  c1 *= std::sqrt((1.0 + ned) * (wi.env_down + 1.0) / (1.0 + k1));
+                  if (((nebd + y.st_down + wi.env_down + k1) / 2 + (x.bl_down + x.st_down + nsd + k1) / 2) % 2 != 0)
+                    c1 = -c1;
+                }
+                Contrib c{};
+                c.out_block = -1;
+                if (kind == 0) {
+                  if (xp->sdim != x.sdim || yp->sdim != y.sdim || !inside(i, x.spos, x.sdim, y.spos, y.sdim) ||
+                      !inside(w, xp->spos, x.sdim, yp->spos, y.sdim)) {
+                    msg = "site operator moves between sub-sectors of different sizes";
+                    return TJSQ_ERR_ARG;
+                  }
+                  c.kind = C_AXPY;
+                  c.c = sign * S.sub[sid].val * E.sub[eid].val * c1;
+                  c.x_off = W[w].off + xp->spos + (int64_t)yp->spos * W[w].sys_dim; c.x_ld = W[w].sys_dim;
+                  c.M = x.sdim; c.N = y.sdim;
+                  c.out_block = out_block(0, i, x.spos, y.spos, c.M, c.N);
+                  if (c.c != 0.0 && c.M && c.N) pg.hp.contrib.push_back(c);
+                } else {
+                  const VSub& bs = B[q].sub[eid];
+                  if (xp->sdim != x.sdim || bs.row_dim != y.sdim || bs.sdim != yp->sdim ||
+                      !inside(i, x.spos, x.sdim, y.spos, y.sdim) || !inside(w, xp->spos, x.sdim, yp->spos, yp->sdim)) {
+                    msg = "operator sub-block disagrees with the fused bases";
+                    return TJSQ_ERR_ARG;
+                  }
+                  // Y[x, y] += c psi_w[x', y'] B^T  (B maps y' -> y)
+                  c.kind = C_RIGHT;
+                  c.c = sign * S.sub[sid].val * c1;
+                  c.x_off = W[w].off + xp->spos + (int64_t)yp->spos * W[w].sys_dim; c.x_ld = W[w].sys_dim;
+                  c.b_off = bs.off; c.b_ld = B[q].t ? bs.sdim : bs.row_dim; c.b_t = B[q].t ? 0 : 1;
+                  c.M = x.sdim; c.K2 = yp->sdim; c.N = y.sdim;
+                  c.out_block = out_block(q, i, x.spos, y.spos, c.M, c.N);
+                  if (c.c != 0.0 && c.M && c.N && c.K2) pg.hp.contrib.push_back(c);
+                }
+              }
+            }
+          }
+        }
+  }
+  if (bad) { msg = "a correlation term falls outside its wavefunction sector"; return TJSQ_ERR_ARG; }
+  pg.out.nops = ncopy;
+  for (int q = 0; q < ncopy; ++q) pg.out.hdr.insert(pg.out.hdr.end(), {sys_basis.length, 0, 0, 0});
+  pg.out.ndata = (int64_t)ncopy * D;
+  pg.x_elems = D;
+  pg.ops_elems = need_ops ? max_extent(*env_ops) : 0;
+  Ctx cx;
   return finish(pg, shape, cx, msg);
 }
 

@@ -138,6 +138,28 @@ int build_pair_ndia_program(int side, char flag, const tjsq_opset& ops, const tj
                             const tjsq_basis& basis, const tjsq_opset& trun, const Shape& shape, Program& pg,
                             std::string& msg);
 
+// The general product of two operators of one side (Get_sys_two,
+// Get_sys_two1, Get_SCOP_Operator_Env, code/tjsq/measure.f90:3798-3889,
+// 6211-6263): each operator of ops (block length L - 1) embedded into
+// `basis`, the site operator embedded, their rank-`rank` coupling
+// (block_mul_block_ndia, code/tjsq/block.f90:1956-2197), projected with
+// `trun` (update_trun_dia for a particle-conserving scalar, update_trun_ndia
+// otherwise).  Run with ops = trun data, x = ops data.
+int build_product_ndia_program(int side, char flag, int rank, const tjsq_opset& ops, const tjsq_opset& site,
+                               const tjsq_basis& basis, const tjsq_opset& trun, const Shape& shape, Program& pg,
+                               std::string& msg);
+
+// The bond-site contractions of Get_SC_Order (code/tjsq/measure.f90:
+// 3226-3376): kind 0 sys_site_env_site_cor_ndia (:5709-5924), kind 1
+// sys_site_env_block_cor_ndia (:5926-6154) with every operator of env_ops,
+// kind 2 env_block_site_cor_ndia (:5085-5238) with every operator of
+// env_ops.  Output copy q holds Y_q with the routine's value = psi . Y_q.
+// Run with ops = env_ops' data (kinds 1, 2), x = psi, y cleared first.
+int build_site_cor_program(int kind, const tjsq_opset* sys_site, int sys_site_trans, const tjsq_opset* env_ops,
+                           int env_trans, const tjsq_opset* env_site, int env_site_trans,
+                           const tjsq_basis& sys_basis, const tjsq_basis& env_basis, int nw, const int* wsec,
+                           char flag, const Shape& shape, Program& pg, std::string& msg);
+
 // Get_density_matrix_sys (side 0) / _env (side 1) of a wavefunction with
 // sectors wsec; `len` is the block length recorded in the output operator.
 int build_density_program(int side, int len, int nw, const int* wsec, const Shape& shape, Program& pg,

@@ -367,6 +367,264 @@ def density_cor(kind, up_bias, xi, xj, ny, ij, sys_len, env_len, sys_bases, env_
     return out
 
 
+# ---- pairing: Get_SC_Order and Get_mSCOP_Operator_Cor0 -----------------------------------
+
+def _select(ops: OpSet, idx) -> OpSet:
+    """The operators idx of ops, reading the same data array (offsets kept)."""
+    hdr = np.asarray(ops.hdr).reshape(-1, 4)
+    sub, off = np.asarray(ops.sub).reshape(-1, 5), np.asarray(ops.off)
+    first = np.concatenate([[0], np.cumsum(hdr[:, 3])])
+    rows = [np.arange(first[i], first[i + 1]) for i in idx]
+    rows = np.concatenate(rows) if rows else np.zeros(0, np.int64)
+    return OpSet(ops.length, hdr[list(idx)].astype(np.int32), sub[rows].astype(np.int32),
+                 off[rows].astype(np.int64), ops.data)
+
+
+def _carry(side, flag, ops, od, L0, L1, bases, truns, runner):
+    """Change_sys_oper_ndia / Change_env_oper_ndia with Truns (get_sys.f90:
+    1134-1238): ops at block length L0 carried to block length L1 >= L0,
+    embedded into each fused basis and projected with its truncation."""
+    for L in range(L0 + 1, L1 + 1):
+        t = truns[L]
+        ops, od = _step(capi.Program.ndia_transport(side, flag, ops, None, bases[L], t.ops), L, runner, t.data, od)
+    return ops, od
+
+
+def _pair_chain(side, up_bias, length, bases, truns, pairs, runner):
+    """The creation operators c^+_i of the sites i = 1 .. length - 1 of one
+    side and the singlet pair operators [c^+_i c^+_j]^0 of the requested
+    (i, j), i < j <= length - 1, all at block length length - 1: the
+    operators of Get_sys_two (measure.f90:3798-3832) and
+    Get_SCOP_Operator_Env (:6211-6263).  c^+_i is the site operator at
+    length 1 and otherwise formed at length i (update_site_ndia_*, projected)
+    and carried with the fermion signs ('F'); the pair is formed when site j
+    joins (block_mul_block_ndia with down_dif 0 of c^+_i at length j - 1 and
+    the site's c^+, projected) and carried as a boson ('B').  All operators
+    of one kind travel together, one program per block length.  Returns
+    (cdag, cdag_data, pair_ops, pair_data, pair_labels); pair_ops is None
+    when no pair was requested."""
+    cd = elec_up(up_bias)
+    want = {}
+    for i, j in pairs:
+        if not 1 <= i < j <= length - 1:
+            raise ValueError(f"pair ({i}, {j}) does not lie in the block (sites 1..{length - 1})")
+        want.setdefault(j, []).append(i)
+    cur, cdat = cd, runner.put(cd.data)          # c^+_1 at length 1
+    pr, pdat, labels = None, None, []
+    for L in range(2, length):
+        t = truns[L]
+        if pr is not None:
+            pr, pdat = _step(capi.Program.ndia_transport(side, "B", pr, None, bases[L], t.ops), L, runner, t.data,
+                             pdat)
+        firsts = sorted(set(want.get(L, [])))
+        if firsts:
+            sel = _select(cur, [i - 1 for i in firsts])
+            new, nd = _step(capi.Program.product_ndia(side, "F", 0, sel, cd, bases[L], t.ops), L, runner, t.data,
+                            cdat)
+            if pr is None:
+                pr, pdat = new, nd
+            else:
+                pr, pdat = _concat(pr, pdat, new, nd, runner)
+            labels += [(i, L) for i in firsts]
+        cur, cdat = _step(capi.Program.ndia_transport(side, "F", cur, cd, bases[L], t.ops), L, runner, t.data, cdat)
+    return cur, cdat, pr, pdat, labels
+
+
+def _measure_ops(side, ops, od, basis, wsec, psi, runner):
+    """measure_sys_block / measure_env_block of every operator of ops."""
+    mp = capi.Program.measure(side, ops, None, basis, wsec)
+    return runner.values(runner.run(mp, od, psi, clear=True), ops.nops, psi)
+
+
+def _check_bond(sys_len, env_len):
+    if sys_len < 3 or env_len < 3:
+        raise ValueError("the pairing measurements need at least two block sites on each side of the bond")
+
+
+def sc_order(up_bias, neib, ny, xc1, xc2, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns, wsec, psi,
+             runner=None):
+    """Get_SC_Order (code/tjsq/measure.f90:3226-3376): the singlet pair
+    amplitude <[c^+_i c^+_j]^0> of the bonds of sites sid1 = xc1, xc1 + ny,
+    ... <= xc2 with their neighbours neib[sid1, 0] (direction 1, along y) and
+    neib[sid1, 2] (direction 3, along x), on a chain of n = sys_len +
+    env_len sites (environment site e is lattice site n - e + 1).  Returns
+    rows (sid1, neib[sid1, 2], value of direction 1, value of direction 3),
+    the first four columns the routine writes to Sc_order.dat (its last two
+    columns are never assigned there).
+
+    Each bond takes the routine's branch for where its sites lie: both in
+    the system block, the pair operator (Get_sys_two) measured with
+    measure_sys_block; both in the environment block, Get_SCOP_Operator_Env
+    with measure_env_block; across the bond, the contraction of c^+ with the
+    transposed c^+ (or with st_elec_down at a bond site) of the other side
+    (sys_site_env_site_, sys_site_env_block_, env_block_site_ (negated),
+    sys_block_site_, sys_block_env_site_ and sys_block_env_block_cor_ndia).
+    The amplitude connects sectors of different particle number, which a
+    wavefunction has only without the particle label: up_bias must be 0, as
+    the reference configures it (common.f90:11)."""
+    if up_bias != 0:
+        raise ValueError("the pair amplitude needs the SU(2)-only basis (up_bias 0)")
+    _check_bond(sys_len, env_len)
+    if runner is None:
+        runner = DeviceRunner(psi.device)
+    n = sys_len + env_len
+    neib = np.asarray(neib)
+    sids = list(range(xc1, xc2 + 1, ny))
+    bonds = []
+    for sid1 in sids:
+        for ij in (1, 3):
+            sid2 = int(neib[sid1, ij - 1])
+            if not 1 <= sid1 <= n or not 1 <= sid2 <= n:
+                raise ValueError(f"site {sid1} has no neighbour in direction {ij} on the {n}-site chain")
+            bonds.append((sid1, ij, min(sid1, sid2), max(sid1, sid2)))
+    sys_pairs = {(a, b) for _, _, a, b in bonds if b < sys_len}
+    env_pairs = {(n - b + 1, n - a + 1) for _, _, a, b in bonds if a > sys_len + 1 and n - a + 1 <= env_len - 1}
+    scd, scdat, sp, spdat, slab = _pair_chain(0, up_bias, sys_len, sys_bases, sys_truns, sys_pairs, runner)
+    ecd, ecdat, ep, epdat, elab = _pair_chain(1, up_bias, env_len, env_bases, env_truns, env_pairs, runner)
+    sb, eb = sys_bases[sys_len], env_bases[env_len]
+    val = {}
+    if sp is not None:
+        for lab, v in zip(slab, _measure_ops(0, sp, spdat, sb, wsec, psi, runner)):
+            val[("s",) + lab] = v
+    if ep is not None:
+        for lab, v in zip(elab, _measure_ops(1, ep, epdat, eb, wsec, psi, runner)):
+            val[("e",) + lab] = v
+    up, dn = elec_up(up_bias), elec_down(up_bias)
+    one = runner.put(np.zeros(1))
+    # across the bond: environment c^+ of site e sits at index e - 1 of ecd
+    cross_e = sorted({n - b + 1 for _, _, a, b in bonds if a >= sys_len and b > sys_len + 1} |
+                     {n - b + 1 for _, _, a, b in bonds if a < sys_len and b > sys_len + 1})
+    e_ops = _select(ecd, [e - 1 for e in cross_e]) if cross_e else None
+    epos = {e: q for q, e in enumerate(cross_e)}
+    for sid1, ij, a, b in bonds:
+        key = (sid1, ij)
+        if b < sys_len:
+            val[key] = val[("s", a, b)]
+        elif a > sys_len + 1:
+            val[key] = val[("e", n - b + 1, n - a + 1)]
+        elif a == sys_len and b == sys_len + 1:
+            pg = capi.Program.site_cor(0, "F", up, 0, None, 0, dn, 0, sb, eb, wsec)
+            val[key] = runner.values(runner.run(pg, one, psi, clear=True), 1, psi)[0]
+        elif a == sys_len:
+            pg = capi.Program.site_cor(1, "F", up, 0, _select(e_ops, [epos[n - b + 1]]), 1, None, 0, sb, eb, wsec)
+            val[key] = runner.values(runner.run(pg, ecdat, psi, clear=True), 1, psi)[0]
+        elif a == sys_len + 1:
+            pg = capi.Program.site_cor(2, "F", None, 0, _select(e_ops, [epos[n - b + 1]]), 1, up, 0, sb, eb, wsec)
+            val[key] = -runner.values(runner.run(pg, ecdat, psi, clear=True), 1, psi)[0]
+        else:                                  # a < sys_len <= b
+            sop = _select(scd, [a - 1])
+            if b == sys_len:
+                pg = capi.Program.cor_ndia("F", sop, None, 0, None, 0, dn, 0, sb, eb, wsec)
+                ops = scdat
+            elif b == sys_len + 1:
+                pg = capi.Program.cor_ndia("F", sop, None, 0, dn, 0, None, 0, sb, eb, wsec)
+                ops = scdat
+            else:
+                eo = _select(e_ops, [epos[n - b + 1]])
+                pg = capi.Program.cor_ndia("F", sop, eo, 1, None, 0, None, 0, sb, eb, wsec)
+                ops = runner.cat(runner.head(scdat, _extent(sop)), runner.head(ecdat, _extent(eo)))
+            val[key] = runner.values(runner.run(pg, ops, psi, clear=True), 1, psi)[0]
+    return [(sid1, int(neib[sid1, 2]), val[(sid1, 1)], val[(sid1, 3)]) for sid1 in sids]
+
+
+def sc_pair_cor0(up_bias, neib, ny, xc1, ij, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns, wsec,
+                 psi, runner=None):
+    """Get_mSCOP_Operator_Cor0 (code/tjsq/measure.f90:3377-3507) with
+    xc1 = xc2: the correlations of the singlet pair operator of bond
+    (sid1, neib[sid1, ij - 1]), sid1 = xc1, with the pair operators of the
+    bonds (sid3, neib[sid3, i - 1]), i = 1, 3, 5, further along the chain
+    (n = sys_len + env_len sites).  Returns the rows the routine writes to its
+    file: (sid1, sid3, mod(sid3 - sid1 + n, ny), (sid3 - 1)//ny -
+    (sid1 - 1)//ny, v1, v3, v5) (its last three columns come from a table
+    the reference never fills and are 0):
+    - sid3 = sid1 + ny, ... <= sys_len - 3 on the system side: the
+      four-fermion operator of Get_sys_two1 (:3835-3889: the pair carried to
+      min(sid3, sid4), coupled with st_elec_down to 2S = 1, carried, coupled
+      with st_elec_down to 2S = 0, carried as a scalar) measured with
+      measure_sys_block; bonds reaching back to the first pair or past
+      sys_len - 1 stay 0;
+    - when ny >= 6, the bond of sid1 + 4 with sid3 = sys_len + 2 ..
+      sys_len + ny + 1 in line with it, and then the first bond with
+      sid3 = sys_len + 2 .. n - ny in line with it, on the environment side:
+      sys_block_env_block_cor_ndia of the two pair operators
+      (Get_SCOP_Operator_Env), bonds that leave the environment block stay 0.
+    Valid for both particle labels."""
+    _check_bond(sys_len, env_len)
+    if runner is None:
+        runner = DeviceRunner(psi.device)
+    n = sys_len + env_len
+    neib = np.asarray(neib)
+    dn = elec_down(up_bias)
+    sb, eb = sys_bases[sys_len], env_bases[env_len]
+
+    def row(s1, s3, v):
+        return (s1, s3, (s3 - s1 + n) % ny, (s3 - 1) // ny - (s1 - 1) // ny, v[0], v[1], v[2])
+
+    def sys_pair(s1):
+        s2 = int(neib[s1, ij - 1])
+        if s2 == 0 or not s1 < s2 <= sys_len - 1:
+            raise ValueError(f"the bond ({s1}, {s2}) must point forward inside the system block")
+        return s1, s2
+
+    s1, s2 = sys_pair(xc1)
+    _, _, pr, pdat, _ = _pair_chain(0, up_bias, s2 + 1, sys_bases, sys_truns, [(s1, s2)], runner)
+    rows = []
+    # (a) both pairs on the system side
+    for s3 in range(s1 + ny, sys_len - 2, ny):
+        v = [0.0, 0.0, 0.0]
+        for k, i in enumerate((1, 3, 5)):
+            s4 = int(neib[s3, i - 1])
+            if min(s3, s4) <= max(s1, s2) or max(s3, s4) > sys_len - 1:
+                continue
+            i3, i4 = min(s3, s4), max(s3, s4)
+            q, qd = _carry(0, "B", pr, pdat, s2, i3 - 1, sys_bases, sys_truns, runner)
+            t = sys_truns[i3]
+            q, qd = _step(capi.Program.product_ndia(0, "F", 1, q, dn, sys_bases[i3], t.ops), i3, runner, t.data, qd)
+            q, qd = _carry(0, "F", q, qd, i3, i4 - 1, sys_bases, sys_truns, runner)
+            t = sys_truns[i4]
+            q, qd = _step(capi.Program.product_ndia(0, "F", 0, q, dn, sys_bases[i4], t.ops), i4, runner, t.data, qd)
+            for L in range(i4 + 1, sys_len):
+                t = sys_truns[L]
+                q, qd = _step(capi.Program.dia_transport(q, None, sys_bases[L], t.ops), L, runner, t.data, qd)
+            v[k] = _measure_ops(0, q, qd, sb, wsec, psi, runner)[0]
+        rows.append(row(s1, s3, v))
+    # (b), (c) the second pair on the environment side
+    parts = []
+    if ny >= 6:
+        parts.append((sys_pair(xc1 + 4), range(sys_len + 2, sys_len + ny + 2)))
+    parts.append(((s1, s2), range(sys_len + 2, n - ny + 1)))
+    work = []
+    for (a1, a2), s3s in parts:
+        bl = []
+        for s3 in s3s:
+            if (s3 - a1) % ny != 0:
+                continue
+            if s3 > n:
+                raise ValueError(f"site {s3} lies past the end of the {n}-site chain")
+            e1 = n - s3 + 1
+            es = []
+            for i in (1, 3, 5):
+                s4 = int(neib[s3, i - 1])
+                e2 = n - s4 + 1
+                es.append((min(e1, e2), max(e1, e2)) if s4 != 0 and max(e1, e2) <= env_len - 1 else None)
+            bl.append((s3, es))
+        work.append(((a1, a2), bl))
+    env_pairs = {p for _, bl in work for _, es in bl for p in es if p is not None}
+    if env_pairs:
+        _, _, ep, epdat, elab = _pair_chain(1, up_bias, env_len, env_bases, env_truns, env_pairs, runner)
+    for (a1, a2), bl in work:
+        _, _, sp, spdat, _ = _pair_chain(0, up_bias, a2 + 1, sys_bases, sys_truns, [(a1, a2)], runner)
+        sp, spdat = _carry(0, "B", sp, spdat, a2, sys_len - 1, sys_bases, sys_truns, runner)
+        vals = {}
+        if env_pairs:
+            pg = capi.Program.cor_ndia("B", sp, ep, 0, None, 0, None, 0, sb, eb, wsec)
+            ops = runner.cat(runner.head(spdat, _extent(sp)), runner.head(epdat, _extent(ep)))
+            vals = dict(zip(elab, runner.values(runner.run(pg, ops, psi, clear=True), ep.nops, psi)))
+        for s3, es in bl:
+            rows.append(row(a1, s3, [vals[p] if p is not None else 0.0 for p in es]))
+    return rows
+
+
 class _DeviceTrun:
     def __init__(self, ops, data):
         self.ops, self.data = ops, data

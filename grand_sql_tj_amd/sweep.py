@@ -401,6 +401,36 @@ class DMRG:
         return measure.density_cor(kind, self.model.up_bias, xi, xj, self.model.ny, ij, sl, el, self.sys_basis,
                                    self.env_basis, self.sys_trun, self.env_trun, self.wsec, self.psi)
 
+    def _neighbours(self):
+        from .lattice import square_neighbours
+        m = self.model
+        return square_neighbours(m.ny, m.nx, m.nsite)
+
+    def measure_sc_order(self, xc1, xc2):
+        """Get_SC_Order (measure.sc_order) in the ground state of the last
+        bond: the singlet pair amplitude of the y and x bonds of the sites
+        xc1, xc1 + ny, ... <= xc2, as Measurement writes Sc_order.dat
+        (main.f90:243-246 with xc1 = 1 and 2, xc2 = N - ny).  Needs the
+        SU(2)-only basis (up_bias 0)."""
+        from . import measure
+        if self.psi is None:
+            raise RuntimeError("no ground state yet")
+        sl, el = self.bond[0] + 1, self.bond[1] + 1
+        return measure.sc_order(self.model.up_bias, self._neighbours(), self.model.ny, xc1, xc2, sl, el,
+                                self.sys_basis, self.env_basis, self.sys_trun, self.env_trun, self.wsec, self.psi)
+
+    def measure_sc_pair_cor0(self, xc1, ij):
+        """Get_mSCOP_Operator_Cor0 (measure.sc_pair_cor0) in the ground state
+        of the last bond: the pair-pair correlations of the bond of site xc1
+        in direction ij (1: y, 3: x) with the bonds further along, as
+        Measurement writes Sc_cor_y.dat / Sc_cor_x.dat (main.f90:255-259)."""
+        from . import measure
+        if self.psi is None:
+            raise RuntimeError("no ground state yet")
+        sl, el = self.bond[0] + 1, self.bond[1] + 1
+        return measure.sc_pair_cor0(self.model.up_bias, self._neighbours(), self.model.ny, xc1, ij, sl, el,
+                                    self.sys_basis, self.env_basis, self.sys_trun, self.env_trun, self.wsec, self.psi)
+
     def ground_state(self, lanerr, trun_err, kept_max, kept_maxf, sweep_num, nsite0=None):
         """Get_Ground_State (main.f90:96-187) for a fresh run (restart1 =
         restart2 = 0, iter1 = 0), with the kept-state settings of main.f90:

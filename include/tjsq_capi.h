@@ -445,6 +445,34 @@ int tjsq_cor_ndia_create(int flag, const tjsq_opset* sys_op, const tjsq_opset* e
                          const tjsq_basis* sys_basis, const tjsq_basis* env_basis, int nw, const int* wsec,
                          tjsq_program** prog);
 
+/* The general product of two operators of one side, as the pairing
+ * measurements form them (Get_sys_two and Get_sys_two1, code/tjsq/measure.f90:
+ * 3798-3889; Get_SCOP_Operator_Env, :6211-6263): each operator of `ops`
+ * (block length L - 1) embedded into `basis` (update_block_ndia_*), the site
+ * operator `site` embedded (update_site_ndia_*), their coupling to total
+ * 2S = `rank` (block_mul_block_ndia with new_oper%down_dif = rank,
+ * code/tjsq/block.f90:1956-2197), projected with `trun` (update_trun_dia for
+ * a particle-conserving scalar, update_trun_ndia otherwise,
+ * code/tjsq/block.f90:1311-1503).  One operator at length L per operator of
+ * `ops`.  Run with ops = trun data, x = the data of `ops`. */
+int tjsq_product_ndia_create(int side, int flag, int rank, const tjsq_opset* ops, const tjsq_opset* site,
+                             const tjsq_basis* basis, const tjsq_opset* trun, tjsq_program** prog);
+
+/* The bond-site contractions of Get_SC_Order (code/tjsq/measure.f90:
+ * 3226-3376): kind 0 sys_site_env_site_cor_ndia (:5709-5924) of `sys_site`
+ * and `env_site`; kind 1 sys_site_env_block_cor_ndia (:5926-6154) of
+ * `sys_site` with every operator of `env_ops` (environment block length
+ * env_len - 1); kind 2 env_block_site_cor_ndia (:5085-5238) of every
+ * operator of `env_ops` with `env_site`.  *_trans != 0 applies
+ * block_transfer_trans (code/tjsq/block.f90:1676-1706) to that operand
+ * first.  The output is one vector of the wavefunction's length per
+ * operator of env_ops (one for kind 0): each routine's value = psi . y_q.
+ * Run with ops = the data of env_ops (any one-element array for kind 0),
+ * x = psi, y cleared first. */
+int tjsq_site_cor_create(int kind, int flag, const tjsq_opset* sys_site, int sys_site_trans, const tjsq_opset* env_ops,
+                         int env_trans, const tjsq_opset* env_site, int env_site_trans, const tjsq_basis* sys_basis,
+                         const tjsq_basis* env_basis, int nw, const int* wsec, tjsq_program** prog);
+
 /* Get_density_matrix_sys (side 0) / _env (side 1), levels >= 1 states summed
  * and normalised first (Get_density_matrix_*_new, code/tjsq/den_mat.f90:
  * 4-134).  The output operator has one sub-block per wavefunction sector. */

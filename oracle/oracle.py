@@ -203,6 +203,10 @@ def ref_lib():
             lib.refb_w3js.argtypes = [C.c_int] * 6
             lib.refb_density_cor.restype = C.c_int
             lib.refb_density_cor.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, I32P, F64P, F64P]
+            lib.refb_sc_order.restype = C.c_int
+            lib.refb_sc_order.argtypes = [C.c_int] * 6 + [I32P, C.c_int, I32P, F64P, F64P]
+            lib.refb_sc_pair_cor0.restype = C.c_int
+            lib.refb_sc_pair_cor0.argtypes = [C.c_int] * 6 + [I32P, C.c_int, I32P, F64P, C.c_int, F64P]
             _libs["ref"] = lib
         return _libs["ref"]
 
@@ -838,6 +842,56 @@ def ref_density_cor(model, workdir, kind, sid1, sys_len, env_len, sys_bases, env
         raise RuntimeError(f"refb_density_cor returned {rc}")
     cor[:sid1] = np.nan
     return cor
+
+
+def _ref_chain(lib, sys_bases, env_bases, sys_truns, env_truns):
+    for side, bases, truns in ((0, sys_bases, sys_truns), (1, env_bases, env_truns)):
+        for L, b in bases.items():
+            lib.refb_set_bsm(side, L, b.length, *_bs(b))
+        for L, t in truns.items():
+            lib.refb_set_truns(side, L, *_one(t))
+
+
+def ref_sc_order(model, workdir, neib, ny, xc1, xc2, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns,
+                 wsec, x):
+    """Get_SC_Order (measure.f90:3226-3376) over a chain of sys_len +
+    env_len sites with neighbour table neib (neib[i, k - 1] = neib(i, k)),
+    computed with the reference's own routines (refb_sc_order).  Returns
+    rows (sid1, neib[sid1, 2], val1(1), val1(2))."""
+    w = np.ascontiguousarray(wsec, np.int32).reshape(-1, 6)
+    n = sys_len + env_len
+    nb = np.ascontiguousarray(np.asarray(neib, np.int32)[:n + 1, :12])
+    sids = list(range(xc1, xc2 + 1, ny))
+    out = np.zeros(2 * max(1, len(sids)))
+    with _lock, _cwd(workdir):
+        lib = _ref_init(model)
+        _ref_chain(lib, sys_bases, env_bases, sys_truns, env_truns)
+        with _quiet():
+            rc = lib.refb_sc_order(xc1, xc2, ny, sys_len, env_len, n, nb, w.shape[0], w,
+                                   np.ascontiguousarray(x, np.float64), out)
+    if rc != len(sids):
+        raise RuntimeError(f"refb_sc_order returned {rc}")
+    return [(s, int(nb[s, 2]), out[2 * k], out[2 * k + 1]) for k, s in enumerate(sids)]
+
+
+def ref_sc_pair_cor0(model, workdir, neib, ny, xc1, ij, sys_len, env_len, sys_bases, env_bases, sys_truns,
+                     env_truns, wsec, x, maxrows=256):
+    """Get_mSCOP_Operator_Cor0 (measure.f90:3377-3507) with xc1 = xc2 over a
+    chain of sys_len + env_len sites, computed with the reference's own
+    routines (refb_sc_pair_cor0): rows (sid1, sid3, mod, dx, v1, v3, v5)."""
+    w = np.ascontiguousarray(wsec, np.int32).reshape(-1, 6)
+    n = sys_len + env_len
+    nb = np.ascontiguousarray(np.asarray(neib, np.int32)[:n + 1, :12])
+    out = np.zeros((maxrows, 7))
+    with _lock, _cwd(workdir):
+        lib = _ref_init(model)
+        _ref_chain(lib, sys_bases, env_bases, sys_truns, env_truns)
+        with _quiet():
+            rc = lib.refb_sc_pair_cor0(xc1, ij, ny, sys_len, env_len, n, nb, w.shape[0], w,
+                                       np.ascontiguousarray(x, np.float64), maxrows, out)
+    if rc < 0 or rc > maxrows:
+        raise RuntimeError(f"refb_sc_pair_cor0 returned {rc}")
+    return [tuple(int(v) for v in r[:4]) + tuple(float(v) for v in r[4:]) for r in out[:rc]]
 
 
 def ref_oper_cor_dia(model, workdir, sys_len, env_len, sys_bases, env_bases, sys_truns, env_truns, wsec, x):

@@ -1189,3 +1189,236 @@ integer(c_int) function refb_density_cor(kind, sid1, sys_len, env_len, nw, wsec,
   call deallocate_wave(w)
   refb_density_cor = n
 end function refb_density_cor
+
+! Get_SC_Order (measure.f90:3226-3376) over a chain of nb = sys_len + env_len
+! sites with the neighbour table nbt (column k of site i = neib(i, k)), with
+! the reference's own routines in the order and branches of that routine
+! (trun_idx 1).  The routine addresses environment sites through its
+! compile-time Num_site; here n = nb takes its place.  out(2k-1), out(2k) are
+! val1(1), val1(2) of the k-th sid11 = xc1, xc1 + ny_, ...; the routine never
+! assigns its val1(3:4).
+integer(c_int) function refb_sc_order(xc1, xc2, ny_, sys_len, env_len, nb, nbt, nw, wsec, x, out) &
+    bind(C, name="refb_sc_order")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: xc1, xc2, ny_, sys_len, env_len, nb, nw
+  integer(c_int), intent(in) :: nbt(12, 0:nb), wsec(6, *)
+  real(c_double), intent(in) :: x(*)
+  real(c_double), intent(inout) :: out(*)
+  type(Wavefunction) :: w
+  type(Total_Basis) :: sys_bs, env_bs
+  type(Total_Block) :: sys_oper, sys_oper1, sys_oper2, env_oper, new_oper, new_oper1, new_oper2, tmp1
+  integer :: sid11, sid1, sid2, sid3, sid4, eid1, eid2, ij, k, n, trun_idx, lo, hi
+  real(8) :: value, val1(2)
+  refb_sc_order = -1
+  n = nb
+  trun_idx = 1
+  call put_wave(w, sys_len, env_len, nw, wsec, x)
+  call basis_transfer(sys_bsm(sys_len), sys_bs)
+  call basis_transfer(env_bsm(env_len), env_bs)
+  val1 = 0.0d0
+  k = 0
+  do sid11 = xc1, xc2, ny_
+    sid1 = sid11
+    do ij = 1, 3, 2
+      sid2 = nbt(ij, sid1)
+      lo = min(sid1, sid2)
+      hi = max(sid1, sid2)
+      value = 0.0d0
+      if (hi < sys_len) then
+        call Get_sys_two(lo, hi, sys_oper2, trun_idx, w)
+        call Change_sys_oper_ndia(sys_oper2, hi, sys_oper, sys_len - 1, trun_idx, .true., 'B')
+        call deallocate_block(sys_oper2)
+        call measure_sys_block(value, sys_oper, sys_bsm(sys_len), w)
+        call deallocate_block(sys_oper)
+        val1((ij + 1) / 2) = value
+      end if
+      if (lo == sys_len) then
+        if (hi == sys_len + 1) then
+          call sys_site_env_site_cor_ndia(value, st_elec_up, st_elec_down, sys_bs, env_bs, w, 'F')
+          val1((ij + 1) / 2) = value
+        end if
+        if (hi > sys_len + 1) then
+          eid1 = n - hi + 1
+          call env_cdag(eid1, env_oper)
+          call block_transfer_trans(env_oper, tmp1)
+          call sys_site_env_block_cor_ndia(value, st_elec_up, tmp1, sys_bs, env_bs, w, 'F')
+          call deallocate_block(tmp1)
+          call deallocate_block(env_oper)
+        end if
+        val1((ij + 1) / 2) = value
+      end if
+      if (lo == sys_len + 1) then
+        if (hi > sys_len + 1) then
+          eid1 = n - hi + 1
+          call env_cdag(eid1, env_oper)
+          call block_transfer_trans(env_oper, tmp1)
+          call env_block_site_cor_ndia(value, tmp1, st_elec_up, env_bs, w, 'F')
+          value = -value
+          call deallocate_block(tmp1)
+          call deallocate_block(env_oper)
+        end if
+        val1((ij + 1) / 2) = value
+      end if
+      if (lo < sys_len .and. hi >= sys_len) then
+        call Get_sys_oper_ndia(st_elec_up, lo, new_oper, lo, trun_idx, .false., 'F')
+        if (lo <= trun_idx) then
+          call block_transfer(new_oper, new_oper1)
+        else
+          call update_trun_ndia(new_oper, new_oper1, systruns(lo))
+        end if
+        call deallocate_block(new_oper)
+        call Change_sys_oper_ndia(new_oper1, lo, sys_oper1, sys_len - 1, trun_idx, .true., 'F')
+        call deallocate_block(new_oper1)
+        if (hi == sys_len) then
+          call sys_block_site_cor_ndia(value, sys_oper1, st_elec_down, sys_bs, w, 'F')
+        else if (hi == sys_len + 1) then
+          call sys_block_env_site_cor_ndia(value, sys_oper1, st_elec_down, sys_bs, env_bs, w, 'F')
+        else
+          eid1 = n - hi + 1
+          call env_cdag(eid1, env_oper)
+          call block_transfer_trans(env_oper, tmp1)
+          call sys_block_env_block_cor_ndia(value, sys_oper1, tmp1, sys_bs, env_bs, w, 'F')
+          call deallocate_block(tmp1)
+          call deallocate_block(env_oper)
+        end if
+        call deallocate_block(sys_oper1)
+        val1((ij + 1) / 2) = value
+      end if
+      sid3 = sid1
+      eid1 = n - sid3 + 1
+      sid4 = nbt(ij, sid3)
+      eid2 = n - sid4 + 1
+      if (max(eid1, eid2) <= env_len - 1) then
+        call Get_SCOP_Operator_Env(min(eid1, eid2), max(eid1, eid2), env_oper, env_len - 1, trun_idx, .true., 'S0')
+        call measure_env_block(value, env_oper, env_bsm(env_len), w)
+        call deallocate_block(env_oper)
+        val1((ij + 1) / 2) = value
+      end if
+    end do
+    out(2 * k + 1) = val1(1)
+    out(2 * k + 2) = val1(2)
+    k = k + 1
+  end do
+  call deallocate_wave(w)
+  refb_sc_order = k
+contains
+  ! c^+ of environment site e at env_len - 1, as Get_SC_Order forms it
+  subroutine env_cdag(e, op)
+    integer, intent(in) :: e
+    type(Total_Block), intent(inout) :: op
+    type(Total_Block) :: a, b
+    call Get_env_oper_ndia(st_elec_up, e, a, e, trun_idx, .false., 'F')
+    if (e <= trun_idx) then
+      call block_transfer(a, b)
+    else
+      call update_trun_ndia(a, b, envtruns(e))
+    end if
+    call deallocate_block(a)
+    call Change_env_oper_ndia(b, e, op, env_len - 1, trun_idx, .true., 'F')
+    call deallocate_block(b)
+  end subroutine env_cdag
+end function refb_sc_order
+
+! Get_mSCOP_Operator_Cor0 (measure.f90:3377-3507) with xc1 = xc2 over a chain
+! of nb = sys_len + env_len sites with the neighbour table nbt, with the
+! reference's own routines in that routine's order (trun_idx 1, Flags 'S0',
+! n = nb in place of Num_site).  Row r of out (7 doubles) holds sid1, sid3,
+! mod(sid3 - sid1 + n, ny), (sid3-1)/ny - (sid1-1)/ny and val1(1:3) of each
+! line the routine writes (its val2 columns come from the never-filled
+! cor11 table).  Returns the number of rows, at most maxrows.
+integer(c_int) function refb_sc_pair_cor0(xc1, ij, ny_, sys_len, env_len, nb, nbt, nw, wsec, x, maxrows, out) &
+    bind(C, name="refb_sc_pair_cor0")
+  use iso_c_binding
+  use pubdata
+  use ref_bridge_util
+  implicit none
+  integer(c_int), value :: xc1, ij, ny_, sys_len, env_len, nb, nw, maxrows
+  integer(c_int), intent(in) :: nbt(12, 0:nb), wsec(6, *)
+  real(c_double), intent(in) :: x(*)
+  real(c_double), intent(inout) :: out(7, *)
+  type(Wavefunction) :: w
+  type(Total_Basis) :: sys_bs, env_bs
+  type(Total_Block) :: sys_oper, sys_oper1, sys_oper2, env_oper
+  integer :: sid11, sid1, sid2, sid3, sid4, eid1, eid2, i, n, trun_idx, nrow
+  real(8) :: value, val1(3)
+  refb_sc_pair_cor0 = -1
+  n = nb
+  trun_idx = 1
+  nrow = 0
+  call put_wave(w, sys_len, env_len, nw, wsec, x)
+  call basis_transfer(sys_bsm(sys_len), sys_bs)
+  call basis_transfer(env_bsm(env_len), env_bs)
+  sid11 = xc1
+  sid1 = sid11
+  sid2 = nbt(ij, sid1)
+  call Get_sys_two(min(sid1, sid2), max(sid1, sid2), sys_oper2, trun_idx, w)
+  do sid3 = sid1 + ny_, sys_len - 3, ny_
+    val1 = 0.0d0
+    do i = 1, 5, 2
+      sid4 = nbt(i, sid3)
+      if (min(sid4, sid3) <= max(sid1, sid2)) cycle
+      if (max(sid3, sid4) <= sys_len - 1) then
+        call Get_sys_two1(max(sid1, sid2), min(sid3, sid4), max(sid3, sid4), sys_oper2, sys_oper, trun_idx, w)
+        call measure_sys_block(value, sys_oper, sys_bsm(sys_len), w)
+        val1((i + 1) / 2) = value
+        call deallocate_block(sys_oper)
+      end if
+    end do
+    call add_row(sid1, sid3)
+  end do
+  if (ny_ >= 6) then
+    sid1 = sid11 + 4
+    sid2 = nbt(ij, sid1)
+    call Get_sys_two(min(sid1, sid2), max(sid1, sid2), sys_oper1, trun_idx, w)
+    call Change_sys_oper_ndia(sys_oper1, sid2, sys_oper, sys_len - 1, trun_idx, .true., 'B')
+    call deallocate_block(sys_oper1)
+    do sid3 = sys_len + 2, sys_len + ny_ + 1
+      if (mod(sid3 - sid1, ny_) /= 0) cycle
+      call env_part(sid3)
+      call add_row(sid1, sid3)
+    end do
+    call deallocate_block(sys_oper)
+  end if
+  sid1 = sid11
+  sid2 = nbt(ij, sid1)
+  call Change_sys_oper_ndia(sys_oper2, sid2, sys_oper, sys_len - 1, trun_idx, .true., 'B')
+  call deallocate_block(sys_oper2)
+  do sid3 = sys_len + 2, n - ny_
+    if (mod(sid3 - sid1, ny_) /= 0) cycle
+    call env_part(sid3)
+    call add_row(sid1, sid3)
+  end do
+  call deallocate_block(sys_oper)
+  call deallocate_wave(w)
+  refb_sc_pair_cor0 = nrow
+contains
+  subroutine env_part(s3)
+    integer, intent(in) :: s3
+    integer :: k, s4, e1, e2
+    val1 = 0.0d0
+    e1 = n - s3 + 1
+    do k = 1, 5, 2
+      s4 = nbt(k, s3)
+      e2 = n - s4 + 1
+      if (max(e1, e2) <= env_len - 1) then
+        call Get_SCOP_Operator_Env(min(e1, e2), max(e1, e2), env_oper, env_len - 1, trun_idx, .true., 'S0')
+        call sys_block_env_block_cor_ndia(value, sys_oper, env_oper, sys_bs, env_bs, w, 'B')
+        call deallocate_block(env_oper)
+        val1((k + 1) / 2) = value
+      end if
+    end do
+  end subroutine env_part
+  subroutine add_row(s1, s3)
+    integer, intent(in) :: s1, s3
+    nrow = nrow + 1
+    if (nrow > maxrows) return
+    out(1, nrow) = s1
+    out(2, nrow) = s3
+    out(3, nrow) = mod(s3 - s1 + n, ny_)
+    out(4, nrow) = (s3 - 1) / ny_ - (s1 - 1) / ny_
+    out(5:7, nrow) = val1
+  end subroutine add_row
+end function refb_sc_pair_cor0

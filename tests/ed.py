@@ -180,3 +180,62 @@ def density_correlation(model, g, st):
     up, dn = st >> n, st & full
     occ = np.array([((up >> b) & 1) + ((dn >> b) & 1) for b in range(n)], dtype=np.float64)
     return (occ * (g * g)) @ occ.T
+
+
+def _pair_annihilate(n, st, g, k, l, target):
+    """Delta_kl |g> = (c_l,dn c_k,up - c_l,up c_k,dn) / sqrt(2) |g> on the
+    states `target`, modes ordered all up spins (by site), then all down
+    spins: the ordering whose string signs hamiltonian() uses."""
+    full = (1 << n) - 1
+    up, dn = st >> n, st & full
+    out = np.zeros(target.size)
+
+    def pc(x):
+        c = np.zeros_like(x)
+        while True:
+            nz = x != 0
+            if not nz.any():
+                return c
+            c += nz
+            x = x & (x - 1)
+
+    for first_up, sgn in ((True, 1.0), (False, -1.0)):
+        if first_up:       # c_k,up then c_l,dn
+            ok = ((up >> k) & 1 == 1)
+            u1 = up ^ (1 << k)
+            s1 = pc(up & ((1 << k) - 1))
+            ok &= ((dn >> l) & 1 == 1)
+            d1 = dn ^ (1 << l)
+            s2 = pc(u1) + pc(dn & ((1 << l) - 1))
+            nu, nd = u1, d1
+        else:              # c_k,dn then c_l,up
+            ok = ((dn >> k) & 1 == 1)
+            d1 = dn ^ (1 << k)
+            s1 = pc(up) + pc(dn & ((1 << k) - 1))
+            ok &= ((up >> l) & 1 == 1)
+            u1 = up ^ (1 << l)
+            s2 = pc(up & ((1 << l) - 1))
+            nu, nd = u1, d1
+        idx = np.nonzero(ok)[0]
+        if idx.size == 0:
+            continue
+        tgt = np.searchsorted(target, (nu[idx] << n) | nd[idx])
+        np.add.at(out, tgt, sgn * (-1.0) ** (s1[idx] + s2[idx]) * g[idx] / np.sqrt(2.0))
+    return out
+
+
+def pair_correlation(model, g, st, n_elec, i, j, k, l):
+    """<Delta^+_ij Delta_kl> in the state g of the (N = n_elec, S_z = 0)
+    sector over st, with the singlet pair Delta^+_ij = (c^+_i,up c^+_j,dn -
+    c^+_i,dn c^+_j,up) / sqrt(2) (sites 0-based)."""
+    n = model.nsite
+    target = _states(n, n_elec // 2 - 1, n_elec // 2 - 1)
+    a = _pair_annihilate(n, st, g, i, j, target)
+    b = _pair_annihilate(n, st, g, k, l, target)
+    return float(a @ b)
+
+
+def pair_amplitude(model, g_lo, st_lo, g_hi, st_hi, i, j):
+    """<g_hi| Delta^+_ij |g_lo> between a state of N electrons (g_lo over
+    st_lo) and one of N + 2 (g_hi over st_hi), both S_z = 0 (sites 0-based)."""
+    return float(_pair_annihilate(model.nsite, st_hi, g_hi, i, j, st_lo) @ g_lo)
