@@ -297,14 +297,14 @@ __device__ __forceinline__ void split_pairs(u4 x0, u4 x1, bf8& hi, bf8& lo) {
 template <int MODE, int TSZ, int STAGE>
 __global__ __launch_bounds__((TileCfg<MODE, TSZ>::NT), (TileCfg<MODE, TSZ>::WAVES)) void k_tiles(
     const Tile* __restrict__ tiles, const Seg* __restrict__ segs, Spaces<MODE> S, typename KT<MODE>::T* partial,
-    int ntiles) {
+    int ntiles, const int* __restrict__ gate) {
   using T = typename KT<MODE>::T;
   using W = typename KT<MODE>::W;
   using Acc = typename KT<MODE>::Acc;
   using C = TileCfg<MODE, TSZ>;
   constexpr int PER = C::PER, KC = C::KC, HS = C::HS;
   const int bid = blockIdx.x;
-  if (bid >= ntiles) return;
+  if (bid >= ntiles || (gate && *gate)) return;
   const Tile tl = tiles[bid];
   const int tid = threadIdx.x;
   // the wave index is uniform per wave: read it into a scalar register so
@@ -493,9 +493,9 @@ __global__ __launch_bounds__((TileCfg<MODE, TSZ>::NT), (TileCfg<MODE, TSZ>::WAVE
 // (slot = 1 for 32x32 partials, 4 for 64x64) with leading dimension out_space
 template <typename T>
 __global__ __launch_bounds__(256) void k_reduce(const Tile* __restrict__ rt, const T* __restrict__ partial, T* y,
-                                                int n) {
+                                                int n, const int* __restrict__ gate) {
   const int t = blockIdx.x;
-  if (t >= n) return;
+  if (t >= n || (gate && *gate)) return;
   const Tile tl = rt[t];
   const int units = tl.slot, pld = tl.out_space;
   const int np = (tl.seg_end - tl.seg_begin) / units;
@@ -549,7 +549,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc_n(const void* base, 
 // k runs down the source rows (trans = 0), across lines where it runs along
 // them (trans = 1).
 template <typename S>
-__global__ __launch_bounds__(256) void k_pack(const tjsq::PackJob* __restrict__ jobs, int njobs, const S* __restrict__ src) {
+__global__ __launch_bounds__(256) void k_pack(const tjsq::PackJob* __restrict__ jobs, int njobs, const S* __restrict__ src,
+                                              const int* __restrict__ gate) {
+  if (gate && *gate) return;
   for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
     const tjsq::PackJob J = jobs[jb];
     const int groups = J.kpad / 8;
@@ -781,18 +783,19 @@ __device__ __forceinline__ void packed_tile(const tjsq::PTile& tl, const tjsq::P
 // one workgroup per tile
 template <int STAGE>
 __global__ __launch_bounds__(64, 2) void k_packed(const tjsq::PTile* __restrict__ tiles, const tjsq::PSeg* __restrict__ segs,
-                                                 PSpaces S, float* __restrict__ partial, int ntiles) {
+                                                 PSpaces S, float* __restrict__ partial, int ntiles,
+                                                 const int* __restrict__ gate) {
   const int bid = blockIdx.x;
-  if (bid >= ntiles) return;
+  if (bid >= ntiles || (gate && *gate)) return;
   const tjsq::PTile tl = tiles[bid];
   packed_tile<STAGE>(tl, segs, S, partial);
 }
 
 // ordered sum of the partial slots of split packed tiles
 __global__ __launch_bounds__(256) void k_preduce(const tjsq::PReduce* __restrict__ rt, const float* __restrict__ partial,
-                                                 float* __restrict__ y, int n) {
+                                                 float* __restrict__ y, int n, const int* __restrict__ gate) {
   const int t = blockIdx.x;
-  if (t >= n) return;
+  if (t >= n || (gate && *gate)) return;
   const tjsq::PReduce R = rt[t];
   for (int e = threadIdx.x; e < R.mt * R.nt; e += blockDim.x) {
     const int row = e % R.mt, col = e / R.mt;
@@ -836,7 +839,9 @@ __global__ void k_f64_to_split(const double* __restrict__ a, uint32_t* __restric
     b[i] = ((uint32_t)h << 16) | bf16_rne((float)(v - (double)bf16_to_f(h)));
   }
 }
-__global__ void k_f32_to_split(const float* __restrict__ a, uint32_t* __restrict__ b, int64_t n) {
+__global__ void k_f32_to_split(const float* __restrict__ a, uint32_t* __restrict__ b, int64_t n,
+                               const int* __restrict__ gate) {
+  if (gate && *gate) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     b[i] = split_word(a[i]);
 }
@@ -973,6 +978,351 @@ __global__ void k_gs_coef(double* coef, const double* dots, int nv) {
   if (j < nv) coef[j] = dots[j] / dots[nv + j];
 }
 
+// ---- one Lanczos iteration in five streaming passes ------------------------
+// The flags of a pass (device ints): LZ_DONE set once the pass has converged
+// or broken down (every later kernel of the pass, the applies included,
+// returns at once), LZ_LAST the number of basis vectors, LZ_BRK breakdown.
+constexpr int LZ_DONE = 0, LZ_LAST = 1, LZ_BRK = 2;
+// slices of the Gram-Schmidt dot kernel and vectors per block
+constexpr int LZ_DOT_BLOCKS = 1024, LZ_DOT_GROUP = 16;
+// threads (bisection points per round) of the convergence check
+constexpr int LZ_CHECK_THREADS = 256;
+
+// sum of this block's values of v (one per thread) in a fixed order, given
+// to every thread (red: RED_THREADS / 64 + 1 doubles)
+__device__ __forceinline__ double block_sum_fixed(double v, double* red) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < RED_THREADS / 64; ++q) t += red[q];
+    red[RED_THREADS / 64] = t;
+  }
+  __syncthreads();
+  const double t = red[RED_THREADS / 64];
+  __syncthreads();
+  return t;
+}
+
+// W = y - sum_q c_q V_q (nv = 1 or 2 vectors, c from device memory), in
+// FP64; partial[b] = this block's part of |W|^2
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_lz_start(double* __restrict__ W, const T* __restrict__ y,
+                                                          const T* __restrict__ V, int64_t ldv, int nv,
+                                                          const double* __restrict__ c, int64_t n,
+                                                          double* __restrict__ partial, const int* __restrict__ flags) {
+  __shared__ double red[RED_THREADS / 64 + 1];
+  if (flags[LZ_DONE]) return;
+  const double c0 = c[0], c1 = nv > 1 ? c[1] : 0.0;
+  double ss = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double sub = c0 * (double)V[i];
+    if (nv > 1) sub += c1 * (double)V[ldv + i];
+    const double w = (double)y[i] - sub;
+    W[i] = w;
+    ss += w * w;
+  }
+  const double t = block_sum_fixed(ss, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// partial[b * nv + j] = slice b's part of V_j . W (j < nv): block (b, q)
+// takes the vectors 16q .. 16q + 15 over slice b of the elements, each
+// thread one accumulator per vector, so W is read once per 16 vectors and
+// every block reduces once
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_lz_dots(const T* __restrict__ V, int64_t ldv, int nv,
+                                                         const double* __restrict__ W, int64_t n,
+                                                         double* __restrict__ partial, const int* __restrict__ flags) {
+  constexpr int G = LZ_DOT_GROUP;
+  __shared__ double red[RED_THREADS / 64][G];
+  if (flags[LZ_DONE]) return;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int j0 = blockIdx.y * G;
+  const int gn = min(G, nv - j0);
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * per, hi = min(n, lo + per);
+  double s[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) s[g] = 0.0;
+  const T* v0 = V + (int64_t)j0 * ldv;
+  int64_t i = lo + threadIdx.x;
+  for (; i + RED_THREADS < hi; i += 2 * RED_THREADS) {
+    const double w0 = W[i], w1 = W[i + RED_THREADS];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (g < gn) {
+        const T* v = v0 + (int64_t)g * ldv + i;
+        s[g] += (double)v[0] * w0;
+        s[g] += (double)v[RED_THREADS] * w1;
+      }
+  }
+  if (i < hi) {
+    const double w0 = W[i];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (g < gn) s[g] += (double)v0[(int64_t)g * ldv + i] * w0;
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    double a = s[g];
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+    if (lane == 0) red[wid][g] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < gn)
+    partial[(int64_t)blockIdx.x * nv + j0 + threadIdx.x] =
+        ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+}
+
+// beta = |W| from the partials of k_lz_start (every block forms it the same
+// way; block 0 stores it, and a beta at or below tol, tmp[5], ends the pass:
+// eigen.f90:74-77) and the Gram-Schmidt coefficients of the normalised
+// direction w = W / beta (block.f90:2515-2551):
+// coef_j = (V_j . W / beta) / (V_j . V_j), block j
+__global__ __launch_bounds__(RED_THREADS) void k_lz_coefs(const double* __restrict__ partial, int nv, int nb,
+                                                          const double* __restrict__ wpart, int nbw,
+                                                          double* __restrict__ beta, const double* __restrict__ vv,
+                                                          double* __restrict__ coef, const double* __restrict__ tmp,
+                                                          int* __restrict__ flags) {
+  __shared__ double red[RED_THREADS / 64 + 1];
+  if (flags[LZ_DONE]) return;
+  const int j = blockIdx.x;
+  double t = 0.0;
+  for (int b = threadIdx.x; b < nbw; b += RED_THREADS) t += wpart[b];
+  const double bt = sqrt(block_sum_fixed(t, red));
+  t = 0.0;
+  for (int b = threadIdx.x; b < nb; b += RED_THREADS) t += partial[(int64_t)b * nv + j];
+  const double u = block_sum_fixed(t, red);
+  if (threadIdx.x == 0) {
+    coef[j] = (u / bt) / vv[j];
+    if (j == 0) {
+      beta[0] = bt;
+      if (fabs(bt) <= tmp[5]) {
+        flags[LZ_BRK] = 1;
+        flags[LZ_DONE] = 1;
+      }
+    }
+  }
+}
+
+// W = W / beta - sum_j coef_j V_j in FP64; partial[b] = block part of |W|^2
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_lz_subtract(double* __restrict__ W, const T* __restrict__ V,
+                                                             int64_t ldv, int nv, const double* __restrict__ coef,
+                                                             const double* __restrict__ beta, int64_t n,
+                                                             double* __restrict__ partial,
+                                                             const int* __restrict__ flags) {
+  __shared__ double red[RED_THREADS / 64 + 1];
+  __shared__ double cs[RED_MAXV];
+  if (flags[LZ_DONE]) return;
+  for (int j = threadIdx.x; j < nv; j += RED_THREADS) cs[j] = coef[j];
+  __syncthreads();
+  const double inv = 1.0 / beta[0];
+  double ss = 0.0;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * step) {
+    double acc[4], sub[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      acc[u] = i0 + u * step < n ? W[i0 + u * step] * inv : 0.0;
+      sub[u] = 0.0;
+    }
+    for (int j = 0; j < nv; ++j) {
+      const T* v = V + (int64_t)j * ldv + i0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (i0 + u * step < n) sub[u] += cs[j] * (double)v[u * step];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (i0 + u * step < n) {
+        const double w = acc[u] - sub[u];
+        W[i0 + u * step] = w;
+        ss += w * w;
+      }
+  }
+  const double t = block_sum_fixed(ss, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// v = W / |W| stored in the basis type (|W| from the partials of
+// k_lz_subtract, formed the same way by every block); partial[b] = block
+// part of v . v (of the stored values, the denominator later Gram-Schmidt
+// steps use)
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_lz_store(const double* __restrict__ W, T* __restrict__ v, int64_t n,
+                                                          const double* __restrict__ wpart, int nbw,
+                                                          double* __restrict__ partial, const int* __restrict__ flags) {
+  __shared__ double red[RED_THREADS / 64 + 1];
+  if (flags[LZ_DONE]) return;
+  double t0 = 0.0;
+  for (int b = threadIdx.x; b < nbw; b += RED_THREADS) t0 += wpart[b];
+  const double inv = 1.0 / sqrt(block_sum_fixed(t0, red));
+  double ss = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const T x = (T)(W[i] * inv);
+    v[i] = x;
+    ss += (double)x * (double)x;
+  }
+  const double t = block_sum_fixed(ss, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// out = sum of nb partials (a dot or a self dot of the pass)
+__global__ __launch_bounds__(RED_THREADS) void k_lz_sum(const double* __restrict__ partial, int nb,
+                                                        double* __restrict__ out, const int* __restrict__ flags) {
+  __shared__ double red[RED_THREADS / 64 + 1];
+  if (flags[LZ_DONE]) return;
+  double t = 0.0;
+  for (int b = threadIdx.x; b < nb; b += RED_THREADS) t += partial[b];
+  const double u = block_sum_fixed(t, red);
+  if (threadIdx.x == 0) out[0] = u;
+}
+
+// partial[b] = block part of a . b (two basis-type vectors)
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_lz_dot2(const T* __restrict__ a, const T* __restrict__ b, int64_t n,
+                                                         double* __restrict__ partial, const int* __restrict__ flags) {
+  __shared__ double red[RED_THREADS / 64 + 1];
+  if (flags[LZ_DONE]) return;
+  double ss = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    ss += (double)a[i] * (double)b[i];
+  const double t = block_sum_fixed(ss, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// Sturm count: eigenvalues below x of the m x m tridiagonal with diagonal
+// ab[2i - 1] and off-diagonal ab[2i] (i = 1..m), LDL^T pivots guarded by
+// pivmin (the recurrence of LAPACK's DLAEBZ)
+__device__ int lz_sturm(const double* __restrict__ ab, int m, double x, double pivmin) {
+  int c = 0;
+  double q = ab[1] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  if (q < 0.0) ++c;
+  for (int i = 2; i <= m; ++i) {
+    const double e = ab[2 * i - 2];
+    q = (ab[2 * i - 1] - x) - e * e / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q < 0.0) ++c;
+  }
+  return c;
+}
+
+// alpha_m = sum of the partials (<v, H v>) and v . v of the stored vector
+// from its own partials, then the lowest eigenvalue of
+// the m x m tridiagonal by multisection (LZ_CHECK_THREADS points a round) inside
+// [theta_prev - w, theta_prev] (interlacing: the lowest Ritz value only
+// falls), widened to the Gershgorin interval when the counts reject it; the
+// pass converges when it moved by at most max(tol, rtol |theta|)
+// (eigen.f90:93-96) or the basis is full.  tmp[0] holds the previous Ritz
+// value, tmp[1] its last change, tmp[5..7] tol, rtol and the basis limit.
+__global__ __launch_bounds__(LZ_CHECK_THREADS) void k_lz_check(const double* __restrict__ partial, int nb,
+                                                               const double* __restrict__ vpart,
+                                                               double* __restrict__ vv_out, double* __restrict__ ab,
+                                                               int m, double* __restrict__ tmp,
+                                                               int* __restrict__ flags) {
+  constexpr int NT = LZ_CHECK_THREADS, NW = NT / 64;
+  __shared__ double sh[4];
+  __shared__ double red[2][NW];
+  __shared__ int first_w[NW];
+  if (flags[LZ_DONE]) return;
+  const double tol = tmp[5], rtol = tmp[6];
+  const int max_iter = (int)tmp[7];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  double t = 0.0, tv = 0.0;
+  for (int b = tid; b < nb; b += NT) {
+    t += partial[b];
+    tv += vpart[b];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    t += __shfl_down(t, o, 64);
+    tv += __shfl_down(tv, o, 64);
+  }
+  if (lane == 0) {
+    red[0][wid] = t;
+    red[1][wid] = tv;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double a = 0.0, v = 0.0;
+    for (int w = 0; w < NW; ++w) {
+      a += red[0][w];
+      v += red[1][w];
+    }
+    vv_out[0] = v;
+    ab[2 * m - 1] = a;
+    // Gershgorin interval and the pivot guard
+    double gl = 1e300, gu = -1e300, emax = 1.0;
+    for (int i = 1; i <= m; ++i) {
+      const double el = i > 1 ? fabs(ab[2 * i - 2]) : 0.0, er = i < m ? fabs(ab[2 * i]) : 0.0;
+      const double d = i == m ? a : ab[2 * i - 1];
+      gl = fmin(gl, d - el - er);
+      gu = fmax(gu, d + el + er);
+      if (i < m) emax = fmax(emax, ab[2 * i] * ab[2 * i]);
+    }
+    sh[0] = gl;
+    sh[1] = gu;
+    sh[2] = 2.2250738585072014e-308 * emax;   // pivmin
+  }
+  __syncthreads();
+  const double gl = sh[0], gu = sh[1], pivmin = sh[2];
+  const double prev = tmp[0], moved = tmp[1];
+  const double tiny = 4.0 * 2.220446049250313e-16 * fmax(fabs(gl), fabs(gu)) + pivmin;
+  double hi = fmin(gu, prev + tiny), lo = fmax(gl, prev - fmax(4.0 * moved, 1e-9 * (1.0 + fabs(prev))));
+  // accept the bracket only if it holds exactly the lowest eigenvalue
+  if (lz_sturm(ab, m, lo, pivmin) != 0) lo = gl - tiny;
+  if (lz_sturm(ab, m, hi, pivmin) < 1) hi = gu + tiny;
+  for (int round = 0; round < 64; ++round) {
+    const double width = hi - lo;
+    if (width <= 2.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+    // NT points split [lo, hi] into NT + 1 parts; the first point with the
+    // eigenvalue below it bounds the new interval
+    const double x = lo + width * (double)(tid + 1) / (double)(NT + 1);
+    const unsigned long long mask = __ballot(lz_sturm(ab, m, x, pivmin) >= 1);
+    if (lane == 0) first_w[wid] = mask ? __ffsll((long long)mask) - 1 + 64 * wid : NT;
+    __syncthreads();
+    int first = NT;
+    for (int w = 0; w < NW; ++w) first = min(first, first_w[w]);
+    __syncthreads();
+    const double nlo = first == 0 ? lo : lo + width * (double)first / (double)(NT + 1);
+    const double nhi = first == NT ? hi : lo + width * (double)(first + 1) / (double)(NT + 1);
+    lo = nlo;
+    hi = nhi;
+  }
+  if (tid == 0) {
+    const double theta = 0.5 * (lo + hi);
+    const bool done = fabs(theta - prev) <= fmax(tol, rtol * fabs(theta)) || m >= max_iter;
+    tmp[1] = fabs(theta - prev);
+    tmp[0] = theta;
+    flags[LZ_LAST] = m;
+    if (done) flags[LZ_DONE] = 1;
+  }
+}
+
+// start of a pass: flags cleared, the stopping rule stored
+__global__ void k_lz_init(double* __restrict__ tmp, int* __restrict__ flags, double tol, double rtol, int max_iter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    tmp[5] = tol;
+    tmp[6] = rtol;
+    tmp[7] = (double)max_iter;
+    flags[LZ_DONE] = 0;
+    flags[LZ_LAST] = 1;
+    flags[LZ_BRK] = 0;
+  }
+}
+
+// the first Ritz value alpha_1, and a first "change" as wide as it
+__global__ void k_lz_ritz0(const double* __restrict__ ab, double* __restrict__ tmp) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    tmp[0] = ab[1];
+    tmp[1] = fabs(ab[1]) + 1.0;
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -994,6 +1344,11 @@ struct tjsq_plan {
   Tile* rtile = nullptr;
   double* part64 = nullptr;      // partial tiles of split stage-2 work
   float* part32 = nullptr;
+  // while a Lanczos pass launches its iterations: the pass's "done" flag;
+  // every apply kernel of the plan then returns at once after convergence
+  const int* gate = nullptr;
+  int* lz_host = nullptr;        // pinned copies of the pass flags (two chunks)
+  hipEvent_t lz_ev[2] = {nullptr, nullptr};
   // Lanczos workspace
   void* kry = nullptr;           // Krylov basis
   int64_t kry_bytes = 0;
@@ -1111,9 +1466,9 @@ constexpr int NPROF = 5;
 
 template <int MODE, int TSZ, int STAGE>
 void launch_items(int n, const Tile* tiles, const Seg* segs, const Spaces<MODE>& S, typename KT<MODE>::T* part,
-                  hipStream_t st) {
+                  hipStream_t st, const int* gate) {
   hipLaunchKernelGGL((k_tiles<MODE, TSZ, STAGE>), dim3(n), dim3(TileCfg<MODE, TSZ>::NT), 0, st, tiles, segs, S, part,
-                     n);
+                     n, gate);
 }
 
 template <int MODE, int STAGE>
@@ -1138,11 +1493,12 @@ int launch_stage(const tjsq_plan* p, int n, int n64, const Tile* tiles, const Se
   // 64x64 work items first in the table, then 32x32 ones
   const int k0 = STAGE == 1 ? 0 : 2;
   int rc = TJSQ_OK;
-  if (n64) rc = timed(k0, [&] { launch_items<MODE, 64, STAGE>(n64, tiles, segs, S, part, st); });
-  if (!rc && n > n64) rc = timed(k0 + 1, [&] { launch_items<MODE, 32, STAGE>(n - n64, tiles + n64, segs, S, part, st); });
+  if (n64) rc = timed(k0, [&] { launch_items<MODE, 64, STAGE>(n64, tiles, segs, S, part, st, p->gate); });
+  if (!rc && n > n64)
+    rc = timed(k0 + 1, [&] { launch_items<MODE, 32, STAGE>(n - n64, tiles + n64, segs, S, part, st, p->gate); });
   if (!rc && STAGE == 2 && !p->hp.rtile.empty()) {
     const int nr = (int)p->hp.rtile.size();
-    rc = timed(4, [&] { hipLaunchKernelGGL(k_reduce<T>, dim3(nr), dim3(256), 0, st, p->rtile, part, S.y, nr); });
+    rc = timed(4, [&] { hipLaunchKernelGGL(k_reduce<T>, dim3(nr), dim3(256), 0, st, p->rtile, part, S.y, nr, p->gate); });
   }
   return rc;
 }
@@ -1193,20 +1549,20 @@ int packed_apply(const tjsq_plan* p, const float* x, float* y, hipStream_t st) {
   int rc = TJSQ_OK;
   if (nj)
     rc = timed_launch(p, 1, st, [&] {
-      hipLaunchKernelGGL(k_pack<float>, dim3(std::min(nj, 8192)), dim3(256), 0, st, p->pk_xjobs, nj, x);
+      hipLaunchKernelGGL(k_pack<float>, dim3(std::min(nj, 8192)), dim3(256), 0, st, p->pk_xjobs, nj, x, p->gate);
     });
   const int n1 = (int)K.tile1.size(), n2 = (int)K.tile2.size(), nr = (int)K.rtile.size();
   if (!rc && n1)
     rc = timed_launch(p, 0, st, [&] {
-      hipLaunchKernelGGL(k_packed<1>, dim3(n1), dim3(64), 0, st, p->pk_tile1, p->pk_seg1, S, p->pk_part, n1);
+      hipLaunchKernelGGL(k_packed<1>, dim3(n1), dim3(64), 0, st, p->pk_tile1, p->pk_seg1, S, p->pk_part, n1, p->gate);
     });
   if (!rc && n2)
     rc = timed_launch(p, 2, st, [&] {
-      hipLaunchKernelGGL(k_packed<2>, dim3(n2), dim3(64), 0, st, p->pk_tile2, p->pk_seg2, S, p->pk_part, n2);
+      hipLaunchKernelGGL(k_packed<2>, dim3(n2), dim3(64), 0, st, p->pk_tile2, p->pk_seg2, S, p->pk_part, n2, p->gate);
     });
   if (!rc && nr)
     rc = timed_launch(p, 4, st, [&] {
-      hipLaunchKernelGGL(k_preduce, dim3(nr), dim3(256), 0, st, p->pk_rtile, p->pk_part, y, nr);
+      hipLaunchKernelGGL(k_preduce, dim3(nr), dim3(256), 0, st, p->pk_rtile, p->pk_part, y, nr, p->gate);
     });
   return rc;
 }
@@ -1289,7 +1645,7 @@ int pack_ops(tjsq_plan* p, const std::vector<tjsq::PackJob>& jobs, hipStream_t s
     return fail(TJSQ_ERR_NOMEM, "packed BF16x3 allocation failed");
   HIPCHK(hipMemcpyAsync(ojobs, oj.data(), sizeof(tjsq::PackJob) * oj.size(), hipMemcpyHostToDevice, st));
   const int nj = (int)oj.size();
-  hipLaunchKernelGGL(k_pack<double>, dim3(std::min(nj, 8192)), dim3(256), 0, st, ojobs, nj, p->arena);
+  hipLaunchKernelGGL(k_pack<double>, dim3(std::min(nj, 8192)), dim3(256), 0, st, ojobs, nj, p->arena, nullptr);
   const hipError_t e = hipGetLastError();
   const hipError_t e2 = hipStreamSynchronize(st);
   (void)hipFree(ojobs);
@@ -1415,7 +1771,7 @@ int apply_spaces(const tjsq_plan* p, int precision, const void* x, void* y, hipS
     return launch_apply<MODE_F32>(p, Spaces<MODE_F32>{p->arena32, xf, xf, p->t32, (float*)y}, p->part32, st);
   if (p->packed) return packed_apply(p, xf, (float*)y, st);
   const int64_t n = p->hp.dim;
-  hipLaunchKernelGGL(k_f32_to_split, dim3(grid_for(n)), dim3(256), 0, st, xf, p->xw, n);
+  hipLaunchKernelGGL(k_f32_to_split, dim3(grid_for(n)), dim3(256), 0, st, xf, p->xw, n, p->gate);
   HIPCHK(hipGetLastError());
   return launch_apply<MODE_BF16X3>(
       p, Spaces<MODE_BF16X3>{p->arenab, p->xw, xf, (uint32_t*)p->t32, (float*)y}, p->part32, st);
@@ -1764,6 +2120,9 @@ int tjsq_plan_destroy(tjsq_plan* p) {
   if (!p) return TJSQ_OK;
   for (auto& g : p->iter_graphs) (void)hipGraphExecDestroy(g.second);
   p->iter_graphs.clear();
+  for (auto& e : p->lz_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (p->lz_host) (void)hipHostFree(p->lz_host);
   (void)hipFree(p->arena);
   (void)hipFree(p->arena32);
   (void)hipFree(p->arenab);
@@ -1940,6 +2299,9 @@ struct Krylov {
   std::vector<double> hbuf;
 
   T* vec(int j) { return V + (int64_t)j * n; }
+  // pass state: flags (LZ_*), and V_j . V_j of the stored basis vectors
+  int* flags() { return (int*)(tmp + 32); }
+  double* vv() { return tmp + 64; }
 
   // host <- [V_j . w for j < nv] (+ [V_j . V_j] when self)
   int dots_with(int nv, const T* w, bool self, std::vector<double>& out, const T* Vbase = nullptr) {
@@ -2003,75 +2365,50 @@ int apply_f64(tjsq_plan* p, int precision, const double* x, double* y, hipStream
   return tjsq_heff_apply(p, x, y, st);
 }
 
-// One Lanczos pass from v0 (already in slot 0, normalised).  Returns the
-// number of vectors, the Ritz value and the tridiagonal data.
-// One Lanczos iteration at basis size `last` with every scalar on the device
-// (eigen.f90:63-90 with Gram_Schmidt_process, block.f90:2515-2551):
-//   w = H v_last - alpha_last v_last - beta_{last-1} v_{last-1}; beta_last = |w|;
-//   w /= beta_last; w -= sum_j (<v_j,w> / <v_j,v_j>) v_j; w /= |w|;
-//   v_{last+1} = H w; alpha_{last+1} = <w, H w>.
-// The operations are those of the host-scalar form (same kernels, the same
-// IEEE divisions and square roots), so results are bitwise the same.  A
-// beta at or below tol ends the pass; the rest of that iteration is unused.
-int iteration_launches_fp32_basis(Krylov<float>& K, int precision, int last);
-
+// One Lanczos iteration at basis size `last` (eigen.f90:63-96 with
+// Gram_Schmidt_process, block.f90:2515-2551), every scalar on the device, in
+// five streaming passes over the vectors:
+//   W = H v_{last-1} - alpha_last v_{last-1} - beta_{last-1} v_{last-2} (FP64),
+//       beta_last = |W| (a beta at or below tol ends the pass);
+//   c_j = (v_j . W / beta) / (v_j . v_j) for j < last;
+//   W = W / beta - sum_j c_j v_j, v_last = W / |W| stored in the basis type;
+//   v_{last+1} = H v_last, alpha_{last+1} = v_last . v_{last+1};
+//   the lowest eigenvalue of the tridiagonal and the stopping rule.
+// W is an FP64 working vector for both basis types, so the FP64 and the
+// mixed solves run the same kernels.  Once the pass is done every kernel of
+// a later iteration, the applies included, returns at once (the flags).
 template <typename T>
 int iteration_launches(Krylov<T>& K, int precision, int last) {
-  if constexpr (std::is_same<T, float>::value) return iteration_launches_fp32_basis(K, precision, last);
-  T* w = K.vec(last);
-  if (last < 2)
-    hipLaunchKernelGGL(k_combine<T>, dim3(grid_for(K.n)), dim3(256), 0, K.st, w, K.vec(0), K.n, 1, K.ab + 1, K.n,
-                       1.0, 0);
-  else
-    hipLaunchKernelGGL(k_combine<T>, dim3(grid_for(K.n)), dim3(256), 0, K.st, w, K.vec(last - 2), K.n, 2,
-                       K.ab + 2 * last - 2, K.n, 1.0, 0);
-  K.normalize_dev(w, K.ab + 2 * last);
-  K.dots_dev(last, K.V, w, true, K.dots);
-  hipLaunchKernelGGL(k_gs_coef, dim3((last + 63) / 64), dim3(64), 0, K.st, K.coef, K.dots, last);
-  hipLaunchKernelGGL(k_combine<T>, dim3(grid_for(K.n)), dim3(256), 0, K.st, w, K.V, K.n, last, K.coef, K.n, 1.0, 0);
-  K.normalize_dev(w, K.tmp + 9);
-  HIPCHK(hipGetLastError());
-  int rc = std::is_same<T, double>::value ? apply_f64(K.p, precision, (const double*)w, (double*)K.vec(last + 1), K.st)
-                                          : apply_prec(K.p, precision, w, K.vec(last + 1), K.st);
-  if (rc) return rc;
-  K.dots_dev(1, w, K.vec(last + 1), false, K.ab + 2 * last + 1);
-  HIPCHK(hipGetLastError());
-  return TJSQ_OK;
-}
-
-// The same iteration for a mixed-precision solve on an FP32-stored basis:
-// the new direction is formed, normalised and reorthogonalised in an FP64
-// working vector W (FP64 coefficients, FP64 subtraction of the FP32 basis
-// vectors), then stored to the basis in FP32, which the FP32 / BF16x3 apply
-// reads directly; alpha = <v, H v> of the stored vector, FP64 accumulation.
-int iteration_launches_fp32_basis(Krylov<float>& K, int precision, int last) {
-  double* W = K.W;
   const int64_t n = K.n;
-  const float* y = K.vec(last);   // H v_{last-1}
+  double* W = K.W;
+  int* flags = K.flags();
+  const dim3 g(RED_BLOCKS), b(RED_THREADS);
+  // partial sums: the Gram-Schmidt dots at the start of K.partial, the
+  // scalar ones (|W|^2, v . v, alpha) in three RED_BLOCKS regions after them
+  double* pw = K.partial + (int64_t)RED_BLOCKS * RED_MAXV;
+  double* pv = pw + RED_BLOCKS;
+  double* pa = pv + RED_BLOCKS;
   if (last < 2)
-    hipLaunchKernelGGL(k_combine_from<float>, dim3(grid_for(n)), dim3(256), 0, K.st, W, y, K.vec(0), n, 1, K.ab + 1, n);
+    hipLaunchKernelGGL(k_lz_start<T>, g, b, 0, K.st, W, K.vec(last), K.vec(0), n, 1, K.ab + 1, n, pw, flags);
   else
-    hipLaunchKernelGGL(k_combine_from<float>, dim3(grid_for(n)), dim3(256), 0, K.st, W, y, K.vec(last - 2), n, 2,
-                       K.ab + 2 * last - 2, n);
-  auto normalize = [&](double* nrm) {
-    hipLaunchKernelGGL(k_multidot<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, K.st, W, n, 1, W, n, K.partial, 0);
-    hipLaunchKernelGGL(k_finish_dots, dim3(1), dim3(RED_THREADS), 0, K.st, K.partial, 1, RED_BLOCKS, 0, K.tmp + 8);
-    hipLaunchKernelGGL(k_sqrt_to, dim3(1), dim3(64), 0, K.st, nrm, K.tmp + 8);
-    hipLaunchKernelGGL(k_scale_inv_dev<double>, dim3(grid_for(n)), dim3(256), 0, K.st, W, n, nrm);
-  };
-  normalize(K.ab + 2 * last);
-  hipLaunchKernelGGL((k_multidot<float, double>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, K.st, K.V, n, last, W, n,
-                     K.partial, 1);
-  hipLaunchKernelGGL(k_finish_dots, dim3(2 * last), dim3(RED_THREADS), 0, K.st, K.partial, last, RED_BLOCKS, 1, K.dots);
-  hipLaunchKernelGGL(k_gs_coef, dim3((last + 63) / 64), dim3(64), 0, K.st, K.coef, K.dots, last);
-  hipLaunchKernelGGL((k_combine<double, float>), dim3(grid_for(n)), dim3(256), 0, K.st, W, K.V, n, last, K.coef, n, 1.0,
-                     0);
-  normalize(K.tmp + 9);
-  hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(n)), dim3(256), 0, K.st, W, K.vec(last), n);
+    hipLaunchKernelGGL(k_lz_start<T>, g, b, 0, K.st, W, K.vec(last), K.vec(last - 2), n, 2, K.ab + 2 * last - 2, n,
+                       pw, flags);
+  hipLaunchKernelGGL(k_lz_dots<T>, dim3(LZ_DOT_BLOCKS, (last + LZ_DOT_GROUP - 1) / LZ_DOT_GROUP), b, 0, K.st, K.V, n,
+                     last, W, n, K.partial, flags);
+  hipLaunchKernelGGL(k_lz_coefs, dim3(last), b, 0, K.st, K.partial, last, LZ_DOT_BLOCKS, pw, RED_BLOCKS,
+                     K.ab + 2 * last, K.vv(), K.coef, K.tmp, flags);
+  hipLaunchKernelGGL(k_lz_subtract<T>, g, b, 0, K.st, W, K.V, n, last, K.coef, K.ab + 2 * last, n, pw, flags);
+  hipLaunchKernelGGL(k_lz_store<T>, g, b, 0, K.st, W, K.vec(last), n, pw, RED_BLOCKS, pv, flags);
   HIPCHK(hipGetLastError());
-  int rc = apply_prec(K.p, precision, K.vec(last), K.vec(last + 1), K.st);
+  K.p->gate = flags;
+  int rc = std::is_same<T, double>::value ? apply_f64(K.p, precision, (const double*)K.vec(last),
+                                                      (double*)K.vec(last + 1), K.st)
+                                          : apply_prec(K.p, precision, K.vec(last), K.vec(last + 1), K.st);
+  K.p->gate = nullptr;
   if (rc) return rc;
-  K.dots_dev(1, K.vec(last), K.vec(last + 1), false, K.ab + 2 * last + 1);
+  hipLaunchKernelGGL(k_lz_dot2<T>, g, b, 0, K.st, K.vec(last), K.vec(last + 1), n, pa, flags);
+  hipLaunchKernelGGL(k_lz_check, dim3(1), dim3(LZ_CHECK_THREADS), 0, K.st, pa, RED_BLOCKS, pv, K.vv() + last, K.ab,
+                     last + 1, K.tmp, flags);
   HIPCHK(hipGetLastError());
   return TJSQ_OK;
 }
@@ -2111,6 +2448,13 @@ int iteration(Krylov<T>& K, int precision, int last) {
   return TJSQ_OK;
 }
 
+// One Lanczos pass from v0 (already in slot 0, normalised).  Returns the
+// number of vectors, the Ritz value and the tridiagonal data.  Iterations
+// are issued LZ_CHUNK at a time and the host reads the pass's flags once per
+// chunk, while the next chunk is already queued; the iterations queued after
+// the pass has ended return at once.
+constexpr int LZ_CHUNK = 8;
+
 template <typename T>
 int lanczos_pass(Krylov<T>& K, int precision, double tol, int max_iter, int& last_out, double& ritz_out,
                  std::vector<double>& alpha, std::vector<double>& beta, int& breakdown, int& applies,
@@ -2118,43 +2462,71 @@ int lanczos_pass(Krylov<T>& K, int precision, double tol, int max_iter, int& las
   alpha.assign(max_iter + 1, 0.0);
   beta.assign(max_iter + 1, 0.0);
   breakdown = 0;
-  int last = 1;  // number of basis vectors (1-based as in eigen.f90)
+  tjsq_plan* p = K.p;
   if (precision == TJSQ_FP64) {
-    int rc0 = ensure_fp64_scratch(K.p);   // no allocation may happen inside a capture
+    int rc0 = ensure_fp64_scratch(p);   // no allocation may happen inside a capture
     if (rc0) return rc0;
   }
+  if (!p->lz_host) {
+    if (hipHostMalloc((void**)&p->lz_host, 2 * 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+      return fail(TJSQ_ERR_NOMEM, "pinned flag buffer allocation failed");
+    for (int i = 0; i < 2; ++i) HIPCHK(hipEventCreateWithFlags(&p->lz_ev[i], hipEventDisableTiming));
+  }
+  int* flags = K.flags();
+  const dim3 g(RED_BLOCKS), b(RED_THREADS);
+  hipLaunchKernelGGL(k_lz_init, dim3(1), dim3(64), 0, K.st, K.tmp, flags, tol, rtol, max_iter);
   int rc = std::is_same<T, double>::value
-               ? apply_f64(K.p, precision, (const double*)K.vec(0), (double*)K.vec(1), K.st)
-               : apply_prec(K.p, precision, K.vec(0), K.vec(1), K.st);
+               ? apply_f64(p, precision, (const double*)K.vec(0), (double*)K.vec(1), K.st)
+               : apply_prec(p, precision, K.vec(0), K.vec(1), K.st);
   if (rc) return rc;
   ++applies;
-  K.dots_dev(1, K.vec(0), K.vec(1), false, K.ab + 1);
-  HIPCHK(hipMemcpyAsync(&alpha[1], K.ab + 1, sizeof(double), hipMemcpyDeviceToHost, K.st));
-  HIPCHK(hipStreamSynchronize(K.st));
-  double ritz = alpha[1];
-  double ba[2];
-  while (last < max_iter) {
-    if ((rc = iteration(K, precision, last))) return rc;
-    HIPCHK(hipMemcpyAsync(ba, K.ab + 2 * last, 2 * sizeof(double), hipMemcpyDeviceToHost, K.st));
-    HIPCHK(hipStreamSynchronize(K.st));
-    beta[last] = ba[0];
-    if (std::fabs(ba[0]) <= tol) {  // eigen.f90:74-77
-      breakdown = 1;
-      break;
+  hipLaunchKernelGGL(k_lz_dot2<T>, g, b, 0, K.st, K.vec(0), K.vec(1), K.n, K.partial, flags);
+  hipLaunchKernelGGL(k_lz_sum, dim3(1), b, 0, K.st, K.partial, RED_BLOCKS, K.ab + 1, flags);
+  hipLaunchKernelGGL(k_lz_dot2<T>, g, b, 0, K.st, K.vec(0), K.vec(0), K.n, K.partial, flags);
+  hipLaunchKernelGGL(k_lz_sum, dim3(1), b, 0, K.st, K.partial, RED_BLOCKS, K.vv(), flags);
+  hipLaunchKernelGGL(k_lz_ritz0, dim3(1), dim3(64), 0, K.st, K.ab, K.tmp);
+  HIPCHK(hipGetLastError());
+  int next = 1;        // basis size of the next iteration to issue
+  auto issue = [&](int slot) -> int {
+    for (int c = 0; c < LZ_CHUNK && next < max_iter; ++c, ++next) {
+      int r = iteration(K, precision, next);
+      if (r) return r;
     }
-    ++last;
-    ++applies;
-    alpha[last] = ba[1];
+    HIPCHK(hipMemcpyAsync(p->lz_host + 4 * slot, flags, 3 * sizeof(int), hipMemcpyDeviceToHost, K.st));
+    HIPCHK(hipEventRecord(p->lz_ev[slot], K.st));
+    return TJSQ_OK;
+  };
+  int slot = 0;
+  if ((rc = issue(0))) return rc;
+  for (;;) {
+    const bool more = next < max_iter;
+    if (more && (rc = issue(1 - slot))) return rc;
+    HIPCHK(hipEventSynchronize(p->lz_ev[slot]));
+    if (p->lz_host[4 * slot + LZ_DONE] || !more) break;
+    slot = 1 - slot;
+  }
+  // the pass's tridiagonal, its size and flags (the queued iterations after
+  // the end changed nothing)
+  std::vector<double> ab(2 * max_iter + 2, 0.0);
+  double tmp01[2];
+  HIPCHK(hipMemcpyAsync(ab.data(), K.ab, sizeof(double) * ab.size(), hipMemcpyDeviceToHost, K.st));
+  HIPCHK(hipMemcpyAsync(tmp01, K.tmp, sizeof(tmp01), hipMemcpyDeviceToHost, K.st));
+  HIPCHK(hipMemcpyAsync(p->lz_host, flags, 3 * sizeof(int), hipMemcpyDeviceToHost, K.st));
+  HIPCHK(hipStreamSynchronize(K.st));
+  const int last = p->lz_host[LZ_LAST];
+  breakdown = p->lz_host[LZ_BRK];
+  if (last < 1 || last > max_iter) return fail(TJSQ_ERR_HIP, "Lanczos pass state out of range");
+  for (int i = 1; i <= last; ++i) alpha[i] = ab[2 * i - 1];
+  for (int i = 1; i < last + (breakdown ? 1 : 0) && i <= max_iter; ++i) beta[i] = ab[2 * i];
+  applies += last - 1;
+  if (theta2) {
     std::vector<double> dia(alpha.begin() + 1, alpha.begin() + last + 1);
     std::vector<double> off(beta.begin() + 1, beta.begin() + last + 1);
     tridiag_eig(last, dia, off, nullptr);
-    const bool done = std::fabs(dia[0] - ritz) <= std::max(tol, rtol * std::fabs(dia[0])) || last == max_iter;
-    ritz = dia[0];
-    if (theta2) *theta2 = last > 1 ? dia[1] : 0.0;
-    if (done) break;
+    *theta2 = last > 1 ? dia[1] : 0.0;
   }
   last_out = last;
-  ritz_out = ritz;
+  ritz_out = tmp01[0];
   return TJSQ_OK;
 }
 
@@ -2250,6 +2622,7 @@ extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_
   if (opts->precision == TJSQ_FP64) {
     Krylov<double> K;
     if ((rc = ensure_krylov(p, opts->max_iter, K, st))) return rc;
+    K.W = p->x64b;
     HIPCHK(hipMemcpyAsync(K.vec(0), psi, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     double nrm;
     if ((rc = K.norm(K.vec(0), nrm))) return rc;
@@ -2312,6 +2685,7 @@ extern "C" int tjsq_lanczos_solve(tjsq_plan* p, double* psi, const tjsq_lanczos_
     // refinement restart: FP64 Lanczos from the current psi
     Krylov<double> K;
     if ((rc = ensure_krylov(p, opts->max_iter, K, st))) return rc;
+    K.W = p->x64b;
     HIPCHK(hipMemcpyAsync(K.vec(0), psi, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     double nrm;
     if ((rc = K.norm(K.vec(0), nrm))) return rc;
