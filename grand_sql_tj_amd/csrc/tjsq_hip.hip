@@ -1323,6 +1323,31 @@ __global__ void k_lz_ritz0(const double* __restrict__ ab, double* __restrict__ t
   }
 }
 
+// Pinned host slots for the flags a pass copies back (8 ints each, two
+// chunks): carved from slabs that are allocated once and kept until the
+// process ends, since pinning and unpinning host memory per plan would
+// synchronise the device under every solve.
+std::mutex g_lz_mu;
+std::vector<int*> g_lz_free;
+
+int* lz_slot_take() {
+  std::lock_guard<std::mutex> lk(g_lz_mu);
+  if (g_lz_free.empty()) {
+    constexpr int SLOTS = 256;
+    int* slab = nullptr;
+    if (hipHostMalloc((void**)&slab, sizeof(int) * 8 * SLOTS, hipHostMallocDefault) != hipSuccess) return nullptr;
+    for (int i = SLOTS - 1; i >= 0; --i) g_lz_free.push_back(slab + 8 * i);
+  }
+  int* s = g_lz_free.back();
+  g_lz_free.pop_back();
+  return s;
+}
+
+void lz_slot_give(int* s) {
+  std::lock_guard<std::mutex> lk(g_lz_mu);
+  g_lz_free.push_back(s);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -2122,7 +2147,7 @@ int tjsq_plan_destroy(tjsq_plan* p) {
   p->iter_graphs.clear();
   for (auto& e : p->lz_ev)
     if (e) (void)hipEventDestroy(e);
-  if (p->lz_host) (void)hipHostFree(p->lz_host);
+  if (p->lz_host) lz_slot_give(p->lz_host);
   (void)hipFree(p->arena);
   (void)hipFree(p->arena32);
   (void)hipFree(p->arenab);
@@ -2468,8 +2493,7 @@ int lanczos_pass(Krylov<T>& K, int precision, double tol, int max_iter, int& las
     if (rc0) return rc0;
   }
   if (!p->lz_host) {
-    if (hipHostMalloc((void**)&p->lz_host, 2 * 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
-      return fail(TJSQ_ERR_NOMEM, "pinned flag buffer allocation failed");
+    if (!(p->lz_host = lz_slot_take())) return fail(TJSQ_ERR_NOMEM, "pinned flag buffer allocation failed");
     for (int i = 0; i < 2; ++i) HIPCHK(hipEventCreateWithFlags(&p->lz_ev[i], hipEventDisableTiming));
   }
   int* flags = K.flags();
