@@ -2895,8 +2895,11 @@ void solver_release(SolverState* S) {
 // cut into at most `groups` contiguous classes (each padded to its largest
 // sector, rounded up to 16) minimising the largest class cost; the classes
 // run concurrently (the largest on the caller's stream, the others on solver
-// streams).  At the m = 8000 bond (91 sectors up to 360) three classes took
-// 13.8 ms against 18.9 ms for classes of multiples of 64.
+// streams).  At a synthetic m = 8000 bond (91 sectors up to 360) three
+// classes took 13.8 ms against 18.9 ms for classes of multiples of 64; at
+// the swept m = 8000 bond (81 sectors, 37 of them 256-797) four classes take
+// 30.8 ms against 33.2 ms for three, and five or six, more classes than a
+// process has hardware queues, 38.6-41.3 ms.
 std::vector<int> eig_classes(const std::vector<int>& dims, int groups) {
   const int n = (int)dims.size();
   std::vector<int> idx(n);
@@ -3310,16 +3313,16 @@ int tjsq_truncation_operator(int nsec, const int* den_sub, const int64_t* den_of
       return fail(TJSQ_ERR_ARG, "tjsq_truncation_operator: density-matrix sector is not square");
     dim += den_sub[5 * k + 4];
   }
-  // sectors grouped into padded size classes, one strided-batched DSYEV-like
-  // call (ascending eigenvalues, eigenvectors in place; den_mat.f90:192) per
-  // class, the classes run concurrently
+  // sectors grouped into at most four padded size classes, one
+  // strided-batched DSYEV-like call (ascending eigenvalues, eigenvectors in
+  // place; den_mat.f90:192) per class, the classes run concurrently
   std::vector<int> dims, sec_of;
   for (int k = 0; k < nsec; ++k)
     if (den_sub[5 * k + 4] > 0) {
       dims.push_back(den_sub[5 * k + 4]);
       sec_of.push_back(k);
     }
-  const std::vector<int> ncls = eig_classes(dims, 3);
+  const std::vector<int> ncls = eig_classes(dims, 4);
   std::vector<int> cls(nsec, 0);            // padded size of each sector's class
   std::vector<std::pair<int, int>> order;   // (class, sector)
   for (size_t q = 0; q < dims.size(); ++q) {

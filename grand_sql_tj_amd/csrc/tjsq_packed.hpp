@@ -84,7 +84,9 @@ struct PackedPlan {
 
 struct PackShape {
   int k_split = 256;       // K pieces of stage-2 segments (a multiple of 32)
-  int item_chunks = 32;    // K chunks of 32 per stage-2 work item
+  // K chunks of 32 per stage-2 work item: at the m = 8000 bench bond, 16 /
+  // 32 / 64 / 128 gave stage 2 + partial sums of 1.07 / 0.93 / 0.90 / 0.91 ms
+  int item_chunks = 64;
   // stage-1 tiles sorted by their x block (sys sector outer, so the
   // operators of one sys sector are read while the sector's x blocks are
   // swept) and then by operator copy; 0: in compile order
