@@ -543,20 +543,28 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc_n(const void* base, 
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)nbytes, 0x00020000);
 }
 
-// one packed copy per workgroup (grid-stride over jobs); a thread packs one
-// group of eight k of one line (device jobs hold absolute destinations).
-// Consecutive threads take consecutive source elements: along a line where
-// k runs down the source rows (trans = 0), across lines where it runs along
-// them (trans = 1).
+// A packing job is cut into pieces of PACK_PIECE (line, group) items, one
+// piece per workgroup (grid-stride over the pieces), so a large block is
+// spread over many CUs instead of serialising one workgroup.
+constexpr int PACK_PIECE = 256;
+struct PackPiece { int job, e0; };
+
+// a thread packs one group of eight k of one line (device jobs hold
+// absolute destinations).  Consecutive threads take consecutive source
+// elements: along a line where k runs down the source rows (trans = 0),
+// across lines where it runs along them (trans = 1).
 template <typename S>
-__global__ __launch_bounds__(256) void k_pack(const tjsq::PackJob* __restrict__ jobs, int njobs, const S* __restrict__ src,
-                                              const int* __restrict__ gate) {
+__global__ __launch_bounds__(PACK_PIECE) void k_pack(const tjsq::PackJob* __restrict__ jobs,
+                                                     const PackPiece* __restrict__ pieces, int npieces,
+                                                     const S* __restrict__ src, const int* __restrict__ gate) {
   if (gate && *gate) return;
-  for (int jb = blockIdx.x; jb < njobs; jb += gridDim.x) {
-    const tjsq::PackJob J = jobs[jb];
+  for (int pc = blockIdx.x; pc < npieces; pc += gridDim.x) {
+    const PackPiece P = pieces[pc];
+    const tjsq::PackJob J = jobs[P.job];
     const int groups = J.kpad / 8;
     const int n = J.lines * groups;
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const int e = P.e0 + threadIdx.x;
+    if (e < n) {
       const int l = J.trans ? e % J.lines : e / groups, g = J.trans ? e / J.lines : e % groups;
       unsigned short h[8], w[8];
 #pragma unroll
@@ -586,6 +594,15 @@ __global__ __launch_bounds__(256) void k_pack(const tjsq::PackJob* __restrict__ 
       d[1] = lo;
     }
   }
+}
+
+std::vector<PackPiece> pack_pieces(const std::vector<tjsq::PackJob>& jobs) {
+  std::vector<PackPiece> out;
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    const int n = jobs[j].lines * (jobs[j].kpad / 8);
+    for (int e0 = 0; e0 < n; e0 += PACK_PIECE) out.push_back({(int)j, e0});
+  }
+  return out;
 }
 
 struct PSide {
@@ -1399,6 +1416,8 @@ struct tjsq_plan {
   uint8_t* pk_x = nullptr;       // packed copies of an apply's x
   uint8_t* pk_t = nullptr;       // stage-1 products
   tjsq::PackJob* pk_xjobs = nullptr;
+  PackPiece* pk_xpieces = nullptr;   // the x jobs cut into pieces
+  int pk_nxpieces = 0;
   tjsq::PSeg* pk_seg1 = nullptr;
   tjsq::PSeg* pk_seg2 = nullptr;
   tjsq::PTile* pk_tile1 = nullptr;
@@ -1570,11 +1589,12 @@ PSpaces packed_spaces(const float* x, float* y) { return PSpaces{x, y}; }
 int packed_apply(const tjsq_plan* p, const float* x, float* y, hipStream_t st) {
   const tjsq::PackedPlan& K = p->pk;
   const PSpaces S = packed_spaces(x, y);
-  const int nj = (int)K.x_jobs.size();
+  const int np = p->pk_nxpieces;
   int rc = TJSQ_OK;
-  if (nj)
+  if (np)
     rc = timed_launch(p, 1, st, [&] {
-      hipLaunchKernelGGL(k_pack<float>, dim3(std::min(nj, 8192)), dim3(256), 0, st, p->pk_xjobs, nj, x, p->gate);
+      hipLaunchKernelGGL(k_pack<float>, dim3(std::min(np, 65535)), dim3(PACK_PIECE), 0, st, p->pk_xjobs,
+                         p->pk_xpieces, np, x, p->gate);
     });
   const int n1 = (int)K.tile1.size(), n2 = (int)K.tile2.size(), nr = (int)K.rtile.size();
   if (!rc && n1)
@@ -1651,8 +1671,11 @@ int pk_upload(tjsq_plan* p, const tjsq::PackedPlan& D, hipStream_t st) {
     return vec.empty() ||
            hipMemcpyAsync(*dptr, vec.data(), sizeof(E) * vec.size(), hipMemcpyHostToDevice, st) == hipSuccess;
   };
+  const std::vector<PackPiece> xp = pack_pieces(D.x_jobs);
+  p->pk_nxpieces = (int)xp.size();
   if (hipMalloc(&p->pk_part, sizeof(float) * 4096 * std::max<int64_t>(1, D.nslots)) != hipSuccess ||
-      !put(&p->pk_xjobs, D.x_jobs) || !put(&p->pk_seg1, D.seg1) || !put(&p->pk_seg2, D.seg2) ||
+      !put(&p->pk_xjobs, D.x_jobs) || !put(&p->pk_xpieces, xp) || !put(&p->pk_seg1, D.seg1) ||
+      !put(&p->pk_seg2, D.seg2) ||
       !put(&p->pk_tile1, D.tile1) || !put(&p->pk_tile2, D.tile2) || !put(&p->pk_rtile, D.rtile))
     return fail(TJSQ_ERR_NOMEM, "packed BF16x3 table allocation failed");
   HIPCHK(hipStreamSynchronize(st));   // the host vectors may go after this
@@ -1665,15 +1688,25 @@ int pack_ops(tjsq_plan* p, const std::vector<tjsq::PackJob>& jobs, hipStream_t s
   if (jobs.empty()) return TJSQ_OK;
   std::vector<tjsq::PackJob> oj = jobs;
   for (tjsq::PackJob& j : oj) j.dst_off = (int64_t)(uintptr_t)(p->pk_ops + j.dst_off);
+  const std::vector<PackPiece> op = pack_pieces(oj);
   tjsq::PackJob* ojobs = nullptr;
-  if (hipMalloc(&ojobs, sizeof(tjsq::PackJob) * oj.size()) != hipSuccess)
+  PackPiece* opieces = nullptr;
+  if (hipMalloc(&ojobs, sizeof(tjsq::PackJob) * oj.size()) != hipSuccess ||
+      hipMalloc(&opieces, sizeof(PackPiece) * std::max<size_t>(1, op.size())) != hipSuccess) {
+    (void)hipFree(ojobs);
     return fail(TJSQ_ERR_NOMEM, "packed BF16x3 allocation failed");
+  }
   HIPCHK(hipMemcpyAsync(ojobs, oj.data(), sizeof(tjsq::PackJob) * oj.size(), hipMemcpyHostToDevice, st));
-  const int nj = (int)oj.size();
-  hipLaunchKernelGGL(k_pack<double>, dim3(std::min(nj, 8192)), dim3(256), 0, st, ojobs, nj, p->arena, nullptr);
+  if (!op.empty())
+    HIPCHK(hipMemcpyAsync(opieces, op.data(), sizeof(PackPiece) * op.size(), hipMemcpyHostToDevice, st));
+  const int np = (int)op.size();
+  if (np)
+    hipLaunchKernelGGL(k_pack<double>, dim3(std::min(np, 65535)), dim3(PACK_PIECE), 0, st, ojobs, opieces, np,
+                       p->arena, nullptr);
   const hipError_t e = hipGetLastError();
   const hipError_t e2 = hipStreamSynchronize(st);
   (void)hipFree(ojobs);
+  (void)hipFree(opieces);
   if (e != hipSuccess || e2 != hipSuccess) return fail(TJSQ_ERR_HIP, "packing the BF16x3 operators failed");
   return TJSQ_OK;
 }
@@ -2171,6 +2204,7 @@ int tjsq_plan_destroy(tjsq_plan* p) {
   (void)hipFree(p->pk_x);
   (void)hipFree(p->pk_t);
   (void)hipFree(p->pk_xjobs);
+  (void)hipFree(p->pk_xpieces);
   (void)hipFree(p->pk_seg1);
   (void)hipFree(p->pk_seg2);
   (void)hipFree(p->pk_tile1);
