@@ -808,19 +808,42 @@ __global__ __launch_bounds__(64, 2) void k_packed(const tjsq::PTile* __restrict_
   packed_tile<STAGE>(tl, segs, S, partial);
 }
 
-// ordered sum of the partial slots of split packed tiles
+// ordered sum of the partial slots of split packed tiles: thread t sums
+// rows 16 (t % 4) .. + 15 of column t / 4 of the 64 x 64 slots, four slots'
+// loads in flight at a time, each element's adds in slot order
 __global__ __launch_bounds__(256) void k_preduce(const tjsq::PReduce* __restrict__ rt, const float* __restrict__ partial,
                                                  float* __restrict__ y, int n, const int* __restrict__ gate) {
   const int t = blockIdx.x;
   if (t >= n || (gate && *gate)) return;
   const tjsq::PReduce R = rt[t];
-  for (int e = threadIdx.x; e < R.mt * R.nt; e += blockDim.x) {
-    const int row = e % R.mt, col = e / R.mt;
-    const float* p = partial + (int64_t)R.s0 * 4096 + row + col * 64;
-    float acc = 0.f;
-    for (int s = 0; s < R.s1 - R.s0; ++s) acc += p[(int64_t)s * 4096];
-    y[R.out_off + (R.m0 + row) + (int64_t)(R.n0 + col) * R.out_ld] = acc;
+  const int col = threadIdx.x >> 2, r0 = 16 * (threadIdx.x & 3);
+  if (col >= R.nt || r0 >= R.mt) return;
+  const f4* p = (const f4*)(partial + (int64_t)R.s0 * 4096 + r0 + col * 64);
+  const int ns = R.s1 - R.s0;
+  f4 acc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) acc[q] = f4{0.f, 0.f, 0.f, 0.f};
+  int s = 0;
+  for (; s + 4 <= ns; s += 4) {
+    f4 v[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[u][q] = p[(int64_t)(s + u) * 1024 + q];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] += v[u][q];
   }
+  for (; s < ns; ++s)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] += p[(int64_t)s * 1024 + q];
+  float* d = y + R.out_off + (R.m0 + r0) + (int64_t)(R.n0 + col) * R.out_ld;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (r0 + 4 * q + e < R.mt) d[4 * q + e] = acc[q][e];
 }
 
 // ---------------------------------------------------------------------------
