@@ -149,6 +149,12 @@ def side_pair_vectors(side, site: OpSet, bases, truns, length, wsec, psi):
         pairs = pairs + [(i[0], L) for i in singles]
         singles = singles + [(L,)]
         cur, cur_data = OpSet(L, hdr, sub, off, np.zeros(n)), out
+        # the operator set grows every step (L singles, L (L - 1) / 2 pairs):
+        # return the previous step's buffers and the program's scratch, so
+        # that cached blocks of the smaller earlier sizes do not pile up
+        # (a 70-site side at m = 2000 otherwise ran out of device memory)
+        del pg, t_data
+        torch.cuda.empty_cache()
     mp = capi.Program.measure(side, cur, None, _trivial_basis(bases[length]), wsec)
     D = psi.numel()
     y = torch.zeros(max(1, mp.sizes["ndata"]), dtype=torch.float64, device=dev)
