@@ -185,6 +185,7 @@ int compile_packed(const HostPlan& P, const PackShape& shape, PackedPlan& out, s
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return P.contrib[a].out_block < P.contrib[b].out_block; });
   std::vector<std::vector<PSeg>> per_block(P.out_blocks.size());
+  std::vector<int> tgroup;   // column strip of each stage-2 tile
   for (int q : order) {
     const Contrib& c = P.contrib[q];
     if (c.kind == C_BOTH && lead[q] >= 0 && lead[q] != q) continue;   // inside its group's product
@@ -266,12 +267,14 @@ int compile_packed(const HostPlan& P, const PackShape& shape, PackedPlan& out, s
         if (items.size() == 1) {
           tl.seg_begin = items[0].first; tl.seg_end = items[0].second; tl.slot = -1;
           out.tile2.push_back(tl);
+          tgroup.push_back(ob.group);
           continue;
         }
         const int s0 = (int)out.nslots;
         for (auto& it : items) {
           tl.seg_begin = it.first; tl.seg_end = it.second; tl.slot = (int)out.nslots++;
           out.tile2.push_back(tl);
+          tgroup.push_back(ob.group);
         }
         PReduce r{};
         r.out_off = ob.off; r.out_ld = ob.ld; r.m0 = m0; r.n0 = n0;
@@ -279,6 +282,48 @@ int compile_packed(const HostPlan& P, const PackShape& shape, PackedPlan& out, s
         r.s0 = s0; r.s1 = (int)out.nslots;
         out.rtile.push_back(r);
       }
+  }
+  if (shape.xcds2 > 1 && !out.tile2.empty()) {
+    // strips onto the least loaded XCD, largest first; within an XCD the
+    // strips run in that order, their tiles in table order.  Bins are padded
+    // with empty tiles (no blocks, no segments: nothing read or written).
+    const int NX = shape.xcds2;
+    const size_t n = out.tile2.size();
+    std::vector<int64_t> cost(n, 0);
+    std::map<int, int64_t> gcost;
+    for (size_t t = 0; t < n; ++t) {
+      const PTile& tl = out.tile2[t];
+      for (int q = tl.seg_begin; q < tl.seg_end; ++q) cost[t] += out.seg2[q].K > 0 ? out.seg2[q].K : 1;
+      cost[t] = std::max<int64_t>(1, cost[t]) * tl.mb * tl.nb;
+      gcost[tgroup[t]] += cost[t];
+    }
+    std::vector<std::pair<int64_t, int>> order;
+    for (auto& kv : gcost) order.push_back({kv.second, kv.first});
+    std::sort(order.begin(), order.end(), [](const std::pair<int64_t, int>& a, const std::pair<int64_t, int>& b) {
+      return a.first != b.first ? a.first > b.first : a.second < b.second;
+    });
+    std::map<int, int> gbin, grank;
+    std::vector<int64_t> load(NX, 0);
+    for (size_t r = 0; r < order.size(); ++r) {
+      const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      gbin[order[r].second] = b;
+      grank[order[r].second] = (int)r;
+      load[b] += order[r].first;
+    }
+    std::vector<int> idx(n);
+    for (size_t t = 0; t < n; ++t) idx[t] = (int)t;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return grank[tgroup[a]] < grank[tgroup[b]]; });
+    std::vector<std::vector<int>> bins(NX);
+    for (int t : idx) bins[gbin[tgroup[t]]].push_back(t);
+    size_t maxlen = 0;
+    for (auto& b : bins) maxlen = std::max(maxlen, b.size());
+    PTile null{};
+    null.out_kind = PK_OUT_Y;
+    null.slot = -1;
+    std::vector<PTile> sorted(maxlen * NX, null);
+    for (int x = 0; x < NX; ++x)
+      for (size_t q = 0; q < bins[x].size(); ++q) sorted[q * NX + x] = out.tile2[bins[x][q]];
+    out.tile2.swap(sorted);
   }
   if (ops.bad || xs.bad) {
     msg = "packed layout: one operand offset referenced with two shapes";

@@ -91,6 +91,12 @@ struct PackShape {
   // operators of one sys sector are read while the sector's x blocks are
   // swept) and then by operator copy; 0: in compile order
   int order1 = 1;
+  // stage-2 tiles of one output column strip (a wavefunction sector's env
+  // sub-block: the blocks that share the right operators) kept on one XCD,
+  // strips spread over `xcds2` XCDs by work (workgroup g runs on XCD
+  // g mod xcds2); 1: table order.  At the m = 8000 bench bond 8 XCDs took
+  // stage 2 from 0.834 to 0.772 ms.
+  int xcds2 = 8;
 };
 
 // Builds the packed tables from P.contrib / P.out_blocks (filled by

@@ -49,9 +49,26 @@ def test_packed_layout_geometry():
         g = segs[(segs["spaces"] & 0xff) != 3]
         assert (g["K"] % 16 == 0).all() and (g["K"] > 0).all()
         assert (g["l_off"] % 64 == 0).all() and (g["r_off"] % 64 == 0).all()
-    for t in (pk["tile1"], pk["tile2"]):
+    # stage 2 runs column strips on one XCD each (workgroup g on XCD g mod 8):
+    # its table is 8 interleaved bins, padded with empty tiles that read and
+    # write nothing
+    t2 = pk["tile2"]
+    pad = t2["mb"] == 0
+    assert t2.size % 8 == 0
+    assert (t2["nb"][pad] == 0).all() and (t2["seg_begin"][pad] == t2["seg_end"][pad]).all()
+    assert (t2["slot"][pad] == -1).all()
+    for t in (pk["tile1"], t2[~pad]):
         assert ((t["mb"] >= 1) & (t["mb"] <= 4) & (t["nb"] >= 1) & (t["nb"] <= 4)).all()
         assert (t["m0"] % 64 == 0).all() and (t["n0"] % 64 == 0).all()
+    # every output tile of every block is there once (or split into partial
+    # slots that one reduction sums)
+    real = t2[~pad]
+    whole = real[real["slot"] < 0]
+    keys = set(zip(whole["out_off"].tolist(), whole["m0"].tolist(), whole["n0"].tolist()))
+    assert len(keys) == whole.size
+    red = set(zip(pk["rtile"]["out_off"].tolist(), pk["rtile"]["m0"].tolist(), pk["rtile"]["n0"].tolist()))
+    assert not keys & red and len(red) == pk["rtile"].size
+    assert sorted(real["slot"][real["slot"] >= 0].tolist()) == list(range(pk["nslots"]))
     # stage-1 tiles in locality order: x block of the first segment, then
     # the operator copy
     src = {int(j["dst_off"]): int(j["src_off"]) for j in pk["x_jobs"]}
