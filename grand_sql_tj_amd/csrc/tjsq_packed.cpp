@@ -59,6 +59,56 @@ Line right_operand(Copies& cp, int64_t off, int ld, int t, int rows_op, int cols
   return l;
 }
 
+// Tiles of one locality group on one XCD: groups onto the least loaded
+// of NX bins, largest first; within a bin the groups run in that order,
+// their tiles in table order.  Workgroup g runs on XCD g mod NX, so bin x's
+// q-th tile takes index q NX + x; bins are padded with empty tiles (no
+// blocks, no segments: nothing read or written).
+void xcd_bins(std::vector<PTile>& tiles, const std::vector<int>& group, const std::vector<int64_t>& cost, int NX,
+              int out_kind) {
+  const size_t n = tiles.size();
+  std::map<int, int64_t> gcost;
+  for (size_t t = 0; t < n; ++t) gcost[group[t]] += cost[t];
+  std::vector<std::pair<int64_t, int>> order;
+  for (auto& kv : gcost) order.push_back({kv.second, kv.first});
+  std::sort(order.begin(), order.end(), [](const std::pair<int64_t, int>& a, const std::pair<int64_t, int>& b) {
+    return a.first != b.first ? a.first > b.first : a.second < b.second;
+  });
+  std::map<int, int> gbin, grank;
+  std::vector<int64_t> load(NX, 0);
+  for (size_t r = 0; r < order.size(); ++r) {
+    const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    gbin[order[r].second] = b;
+    grank[order[r].second] = (int)r;
+    load[b] += order[r].first;
+  }
+  std::vector<int> idx(n);
+  for (size_t t = 0; t < n; ++t) idx[t] = (int)t;
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return grank[group[a]] < grank[group[b]]; });
+  std::vector<std::vector<int>> bins(NX);
+  for (int t : idx) bins[gbin[group[t]]].push_back(t);
+  size_t maxlen = 0;
+  for (auto& b : bins) maxlen = std::max(maxlen, b.size());
+  PTile null{};
+  null.out_kind = out_kind;
+  null.slot = -1;
+  std::vector<PTile> sorted(maxlen * NX, null);
+  for (int x = 0; x < NX; ++x)
+    for (size_t q = 0; q < bins[x].size(); ++q) sorted[q * NX + x] = tiles[bins[x][q]];
+  tiles.swap(sorted);
+}
+
+// work of a tile: K summed over its segments (an axpy counts 1) times its blocks
+std::vector<int64_t> tile_costs(const std::vector<PTile>& tiles, const std::vector<PSeg>& segs) {
+  std::vector<int64_t> cost(tiles.size(), 0);
+  for (size_t t = 0; t < tiles.size(); ++t) {
+    const PTile& tl = tiles[t];
+    for (int q = tl.seg_begin; q < tl.seg_end; ++q) cost[t] += segs[q].K > 0 ? segs[q].K : 1;
+    cost[t] = std::max<int64_t>(1, cost[t]) * tl.mb * tl.nb;
+  }
+  return cost;
+}
+
 }  // namespace
 
 int compile_packed(const HostPlan& P, const PackShape& shape, PackedPlan& out, std::string& msg) {
@@ -177,6 +227,15 @@ int compile_packed(const HostPlan& P, const PackShape& shape, PackedPlan& out, s
     std::vector<PTile> sorted(n);
     for (size_t i = 0; i < n; ++i) sorted[i] = out.tile1[order[i]];
     out.tile1.swap(sorted);
+    if (shape.xcds1 > 1) {
+      // locality group: the operator copy (right operand) of the tile, so
+      // that one XCD reads each operator block for every x block it meets
+      std::map<int64_t, int> gid;
+      std::vector<int> g1(n);
+      for (size_t t = 0; t < n; ++t)
+        g1[t] = gid.emplace(out.seg1[out.tile1[t].seg_begin].r_off, (int)gid.size()).first->second;
+      xcd_bins(out.tile1, g1, tile_costs(out.tile1, out.seg1), shape.xcds1, PK_OUT_T);
+    }
   }
 
   // ---- stage 2: per output block, axpy segments first, then GEMM pieces --
@@ -283,48 +342,8 @@ int compile_packed(const HostPlan& P, const PackShape& shape, PackedPlan& out, s
         out.rtile.push_back(r);
       }
   }
-  if (shape.xcds2 > 1 && !out.tile2.empty()) {
-    // strips onto the least loaded XCD, largest first; within an XCD the
-    // strips run in that order, their tiles in table order.  Bins are padded
-    // with empty tiles (no blocks, no segments: nothing read or written).
-    const int NX = shape.xcds2;
-    const size_t n = out.tile2.size();
-    std::vector<int64_t> cost(n, 0);
-    std::map<int, int64_t> gcost;
-    for (size_t t = 0; t < n; ++t) {
-      const PTile& tl = out.tile2[t];
-      for (int q = tl.seg_begin; q < tl.seg_end; ++q) cost[t] += out.seg2[q].K > 0 ? out.seg2[q].K : 1;
-      cost[t] = std::max<int64_t>(1, cost[t]) * tl.mb * tl.nb;
-      gcost[tgroup[t]] += cost[t];
-    }
-    std::vector<std::pair<int64_t, int>> order;
-    for (auto& kv : gcost) order.push_back({kv.second, kv.first});
-    std::sort(order.begin(), order.end(), [](const std::pair<int64_t, int>& a, const std::pair<int64_t, int>& b) {
-      return a.first != b.first ? a.first > b.first : a.second < b.second;
-    });
-    std::map<int, int> gbin, grank;
-    std::vector<int64_t> load(NX, 0);
-    for (size_t r = 0; r < order.size(); ++r) {
-      const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      gbin[order[r].second] = b;
-      grank[order[r].second] = (int)r;
-      load[b] += order[r].first;
-    }
-    std::vector<int> idx(n);
-    for (size_t t = 0; t < n; ++t) idx[t] = (int)t;
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return grank[tgroup[a]] < grank[tgroup[b]]; });
-    std::vector<std::vector<int>> bins(NX);
-    for (int t : idx) bins[gbin[tgroup[t]]].push_back(t);
-    size_t maxlen = 0;
-    for (auto& b : bins) maxlen = std::max(maxlen, b.size());
-    PTile null{};
-    null.out_kind = PK_OUT_Y;
-    null.slot = -1;
-    std::vector<PTile> sorted(maxlen * NX, null);
-    for (int x = 0; x < NX; ++x)
-      for (size_t q = 0; q < bins[x].size(); ++q) sorted[q * NX + x] = out.tile2[bins[x][q]];
-    out.tile2.swap(sorted);
-  }
+  if (shape.xcds2 > 1 && !out.tile2.empty())
+    xcd_bins(out.tile2, tgroup, tile_costs(out.tile2, out.seg2), shape.xcds2, PK_OUT_Y);
   if (ops.bad || xs.bad) {
     msg = "packed layout: one operand offset referenced with two shapes";
     return TJSQ_ERR_ARG;

@@ -91,6 +91,12 @@ struct PackShape {
   // operators of one sys sector are read while the sector's x blocks are
   // swept) and then by operator copy; 0: in compile order
   int order1 = 1;
+  // with order1: the tiles of one operator copy (the right operand) kept on
+  // one XCD, spread over `xcds1` XCDs by work as for stage 2 below.  At the
+  // m = 8000 bench bond 8 XCDs took stage 1 from 1.093 to 1.053 ms; grouping
+  // by x block instead left it at 1.10 ms, by wave sector or by system
+  // sector (fewer, larger groups) made it 1.19 ms.
+  int xcds1 = 8;
   // stage-2 tiles of one output column strip (a wavefunction sector's env
   // sub-block: the blocks that share the right operators) kept on one XCD,
   // strips spread over `xcds2` XCDs by work (workgroup g runs on XCD

@@ -97,6 +97,9 @@ def evaluate(hp, sb, x):
     spaces = {PK_OPS: pack(pk["ops_jobs"], arena, pk["ops_bytes"]), PK_X: pack(pk["x_jobs"], xf, pk["x_bytes"]),
               PK_T: np.zeros(pk["t_bytes"] // 2, np.uint16)}
     for tl in pk["tile1"]:
+        if tl["mb"] == 0:          # an XCD bin's padding: reads and writes nothing
+            assert tl["nb"] == 0 and tl["seg_begin"] == tl["seg_end"]
+            continue
         D = _tile(pk, spaces, tl, pk["seg1"], xf)
         u = np.arange(int(tl["m0"]), int(tl["m0"]) + D.shape[0])[:, None]
         v = np.arange(int(tl["n0"]), int(tl["n0"]) + D.shape[1])[None, :]
@@ -109,6 +112,9 @@ def evaluate(hp, sb, x):
     written = np.zeros(hp.dim, bool)
     part = np.zeros((max(1, pk["nslots"]), 64, 64))
     for tl in pk["tile2"]:
+        if tl["mb"] == 0:
+            assert tl["nb"] == 0 and tl["seg_begin"] == tl["seg_end"] and tl["slot"] == -1
+            continue
         A = _tile(pk, spaces, tl, pk["seg2"], xf)
         if tl["slot"] >= 0:
             part[tl["slot"], :A.shape[0], :A.shape[1]] = A

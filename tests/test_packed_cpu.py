@@ -57,7 +57,11 @@ def test_packed_layout_geometry():
     assert t2.size % 8 == 0
     assert (t2["nb"][pad] == 0).all() and (t2["seg_begin"][pad] == t2["seg_end"][pad]).all()
     assert (t2["slot"][pad] == -1).all()
-    for t in (pk["tile1"], t2[~pad]):
+    t1 = pk["tile1"]
+    pad1 = t1["mb"] == 0
+    assert t1.size % 8 == 0
+    assert (t1["nb"][pad1] == 0).all() and (t1["seg_begin"][pad1] == t1["seg_end"][pad1]).all()
+    for t in (t1[~pad1], t2[~pad]):
         assert ((t["mb"] >= 1) & (t["mb"] <= 4) & (t["nb"] >= 1) & (t["nb"] <= 4)).all()
         assert (t["m0"] % 64 == 0).all() and (t["n0"] % 64 == 0).all()
     # every output tile of every block is there once (or split into partial
@@ -69,9 +73,22 @@ def test_packed_layout_geometry():
     red = set(zip(pk["rtile"]["out_off"].tolist(), pk["rtile"]["m0"].tolist(), pk["rtile"]["n0"].tolist()))
     assert not keys & red and len(red) == pk["rtile"].size
     assert sorted(real["slot"][real["slot"] >= 0].tolist()) == list(range(pk["nslots"]))
-    # stage-1 tiles in locality order: x block of the first segment, then
-    # the operator copy
+    # stage 1: the tiles of one operator copy (right operand) run on one XCD
+    # (bin g mod 8), one after the other, in x-block order
     src = {int(j["dst_off"]): int(j["src_off"]) for j in pk["x_jobs"]}
-    first = pk["seg1"][pk["tile1"]["seg_begin"]]
-    key = [(src[int(s["l_off"])], int(s["r_off"])) for s in first]
-    assert key == sorted(key)
+    home = {}
+    for x in range(8):
+        b = t1[x::8]
+        b = b[b["mb"] > 0]
+        first = pk["seg1"][b["seg_begin"]]
+        ops = first["r_off"].tolist()
+        seen = []
+        for r in ops:
+            assert home.setdefault(r, x) == x
+            if not seen or seen[-1] != r:
+                assert r not in seen, "an operator copy's tiles are not contiguous in its bin"
+                seen.append(r)
+        for r in seen:
+            sel = first["r_off"] == r
+            key = [(src[int(s["l_off"])], int(tl["n0"]), int(tl["m0"])) for s, tl in zip(first[sel], b[sel])]
+            assert key == sorted(key)
