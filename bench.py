@@ -635,7 +635,7 @@ def main():
     ap.add_argument("--batch-precision", default="fp32", choices=["fp32", "bf16x3"])
     ap.add_argument("--serve", type=int, default=0,
                     help="instead of the solve benchmark: N solve requests with Poisson arrivals")
-    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=2, help="--serve: solve slots (HIP streams)")
     ap.add_argument("--load", type=float, default=0.8,
                     help="arrival rate as a fraction of the one-at-a-time service rate")
     ap.add_argument("--sweep", action="store_true",

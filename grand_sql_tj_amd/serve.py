@@ -93,7 +93,11 @@ def footprint(model, sb, precision: str, max_iter: int) -> int:
 class Server:
     """Continuous-admission solve server (one HIP stream per slot)."""
 
-    def __init__(self, slots: int = 4, hbm_budget: int | None = None):
+    # two slots: on the serving scan (m 500-6000) one request already fills
+    # the GPU, so 1, 2 and 4 slots serve at the same rate; two gave the
+    # lowest p50 / p99 at 0.8 load (0.97 / 2.27 s against 1.12 / 2.36 s for
+    # one and 1.42 / 2.71 s for four)
+    def __init__(self, slots: int = 2, hbm_budget: int | None = None):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("the solve server needs a GPU")
