@@ -1025,8 +1025,9 @@ __global__ void k_gs_coef(double* coef, const double* dots, int nv) {
 constexpr int LZ_DONE = 0, LZ_LAST = 1, LZ_BRK = 2;
 // slices of the Gram-Schmidt dot kernel and vectors per block
 constexpr int LZ_DOT_BLOCKS = 1024, LZ_DOT_GROUP = 16;
-// threads (bisection points per round) of the convergence check
-constexpr int LZ_CHECK_THREADS = 256;
+// threads (bisection points per round) of the convergence check, and the
+// largest tridiagonal it holds (the basis limit, 250 at most)
+constexpr int LZ_CHECK_THREADS = 256, LZ_MAXM = 256;
 
 // sum of this block's values of v (one per thread) in a fixed order, given
 // to every thread (red: RED_THREADS / 64 + 1 doubles)
@@ -1236,16 +1237,16 @@ __global__ __launch_bounds__(RED_THREADS) void k_lz_dot2(const T* __restrict__ a
 }
 
 // Sturm count: eigenvalues below x of the m x m tridiagonal with diagonal
-// ab[2i - 1] and off-diagonal ab[2i] (i = 1..m), LDL^T pivots guarded by
-// pivmin (the recurrence of LAPACK's DLAEBZ)
-__device__ int lz_sturm(const double* __restrict__ ab, int m, double x, double pivmin) {
+// d[i] and squared off-diagonal e2[i] (between i and i + 1), i = 1..m, held
+// in LDS; LDL^T pivots guarded by pivmin (the recurrence of LAPACK's DLAEBZ)
+__device__ int lz_sturm(const double* __restrict__ d, const double* __restrict__ e2, int m, double x,
+                        double pivmin) {
   int c = 0;
-  double q = ab[1] - x;
+  double q = d[1] - x;
   if (fabs(q) < pivmin) q = -pivmin;
   if (q < 0.0) ++c;
   for (int i = 2; i <= m; ++i) {
-    const double e = ab[2 * i - 2];
-    q = (ab[2 * i - 1] - x) - e * e / q;
+    q = (d[i] - x) - e2[i - 1] / q;
     if (fabs(q) < pivmin) q = -pivmin;
     if (q < 0.0) ++c;
   }
@@ -1267,8 +1268,9 @@ __global__ __launch_bounds__(LZ_CHECK_THREADS) void k_lz_check(const double* __r
                                                                int* __restrict__ flags) {
   constexpr int NT = LZ_CHECK_THREADS, NW = NT / 64;
   __shared__ double sh[4];
-  __shared__ double red[2][NW];
+  __shared__ double red[3][NW];
   __shared__ int first_w[NW];
+  __shared__ double d[LZ_MAXM + 1], e2[LZ_MAXM + 1];
   if (flags[LZ_DONE]) return;
   const double tol = tmp[5], rtol = tmp[6];
   const int max_iter = (int)tmp[7];
@@ -1295,18 +1297,42 @@ __global__ __launch_bounds__(LZ_CHECK_THREADS) void k_lz_check(const double* __r
     }
     vv_out[0] = v;
     ab[2 * m - 1] = a;
-    // Gershgorin interval and the pivot guard
-    double gl = 1e300, gu = -1e300, emax = 1.0;
-    for (int i = 1; i <= m; ++i) {
-      const double el = i > 1 ? fabs(ab[2 * i - 2]) : 0.0, er = i < m ? fabs(ab[2 * i]) : 0.0;
-      const double d = i == m ? a : ab[2 * i - 1];
-      gl = fmin(gl, d - el - er);
-      gu = fmax(gu, d + el + er);
-      if (i < m) emax = fmax(emax, ab[2 * i] * ab[2 * i]);
+    sh[3] = a;
+  }
+  __syncthreads();
+  // the tridiagonal into LDS, and the Gershgorin interval and the pivot
+  // guard from it (min / max reductions: any order gives the same values)
+  double glo = 1e300, ghi = -1e300, emx = 1.0;
+  for (int i = tid + 1; i <= m; i += NT) {
+    const double di = i == m ? sh[3] : ab[2 * i - 1];
+    const double er = i < m ? ab[2 * i] : 0.0, el = i > 1 ? ab[2 * i - 2] : 0.0;
+    d[i] = di;
+    e2[i] = er * er;
+    glo = fmin(glo, di - fabs(el) - fabs(er));
+    ghi = fmax(ghi, di + fabs(el) + fabs(er));
+    emx = fmax(emx, er * er);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    glo = fmin(glo, __shfl_down(glo, o, 64));
+    ghi = fmax(ghi, __shfl_down(ghi, o, 64));
+    emx = fmax(emx, __shfl_down(emx, o, 64));
+  }
+  if (lane == 0) {
+    red[0][wid] = glo;
+    red[1][wid] = ghi;
+    red[2][wid] = emx;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double g0 = red[0][0], g1 = red[1][0], em = red[2][0];
+    for (int w = 1; w < NW; ++w) {
+      g0 = fmin(g0, red[0][w]);
+      g1 = fmax(g1, red[1][w]);
+      em = fmax(em, red[2][w]);
     }
-    sh[0] = gl;
-    sh[1] = gu;
-    sh[2] = 2.2250738585072014e-308 * emax;   // pivmin
+    sh[0] = g0;
+    sh[1] = g1;
+    sh[2] = 2.2250738585072014e-308 * em;   // pivmin
   }
   __syncthreads();
   const double gl = sh[0], gu = sh[1], pivmin = sh[2];
@@ -1314,15 +1340,15 @@ __global__ __launch_bounds__(LZ_CHECK_THREADS) void k_lz_check(const double* __r
   const double tiny = 4.0 * 2.220446049250313e-16 * fmax(fabs(gl), fabs(gu)) + pivmin;
   double hi = fmin(gu, prev + tiny), lo = fmax(gl, prev - fmax(4.0 * moved, 1e-9 * (1.0 + fabs(prev))));
   // accept the bracket only if it holds exactly the lowest eigenvalue
-  if (lz_sturm(ab, m, lo, pivmin) != 0) lo = gl - tiny;
-  if (lz_sturm(ab, m, hi, pivmin) < 1) hi = gu + tiny;
+  if (lz_sturm(d, e2, m, lo, pivmin) != 0) lo = gl - tiny;
+  if (lz_sturm(d, e2, m, hi, pivmin) < 1) hi = gu + tiny;
   for (int round = 0; round < 64; ++round) {
     const double width = hi - lo;
     if (width <= 2.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)) + pivmin) break;
     // NT points split [lo, hi] into NT + 1 parts; the first point with the
     // eigenvalue below it bounds the new interval
     const double x = lo + width * (double)(tid + 1) / (double)(NT + 1);
-    const unsigned long long mask = __ballot(lz_sturm(ab, m, x, pivmin) >= 1);
+    const unsigned long long mask = __ballot(lz_sturm(d, e2, m, x, pivmin) >= 1);
     if (lane == 0) first_w[wid] = mask ? __ffsll((long long)mask) - 1 + 64 * wid : NT;
     __syncthreads();
     int first = NT;
