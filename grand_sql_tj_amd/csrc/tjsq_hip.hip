@@ -627,6 +627,30 @@ __device__ __forceinline__ void ld_op(const PSide& s, int b, uint32_t ko, u4& hi
   lo = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(s.r, (int)(s.v[b] + ko + 16u), 0, 0));
 }
 
+// stage 1's output: lane (li, lq) holds D[u][v] = T[v][u] for u = m0 + 16 bi
+// + 4 lq + r, v = n0 + 16 bj + li: four consecutive k of line v of T, in
+// group m0 / 8 + 2 bi + lq / 2, half lq % 2 of its eight, written as the
+// next stage's packed operand (bf16 high and low halves)
+__device__ __forceinline__ void store_t_lines(const tjsq::PTile& tl, const f4 (&acc)[4][4], int mb, int nb, int li,
+                                              int lq) {
+  uint8_t* tb = (uint8_t*)tl.out_off + (int64_t)(tl.n0 + li) * tl.out_ld + (tl.m0 / 8 + lq / 2) * 32 + (lq % 2) * 8;
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) {
+      if (bi >= mb || bj >= nb) continue;
+      unsigned short h[4], w[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        h[r] = bf16_rne(acc[bi][bj][r]);
+        w[r] = bf16_rne(acc[bi][bj][r] - bf16_to_f(h[r]));
+      }
+      uint8_t* d = tb + (int64_t)(16 * bj) * tl.out_ld + 64 * bi;
+      *(uint2*)d = uint2{h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16)};
+      *(uint2*)(d + 16) = uint2{w[0] | ((uint32_t)w[1] << 16), w[2] | ((uint32_t)w[3] << 16)};
+    }
+}
+
 // one output tile of a packed stage (64 lanes)
 template <int STAGE>
 __device__ __forceinline__ void packed_tile(const tjsq::PTile& tl, const tjsq::PSeg* __restrict__ segs, const PSpaces& S,
@@ -751,25 +775,7 @@ __device__ __forceinline__ void packed_tile(const tjsq::PTile& tl, const tjsq::P
       for (int b = 0; b < 4; ++b) acc[a][b] *= rs;
   }
   if (tl.out_kind == tjsq::PK_OUT_T) {
-    // lane (li, lq) holds D[u][v] = T[v][u] for u = m0 + 16 bi + 4 lq + r,
-    // v = n0 + 16 bj + li: four consecutive k of line v of T, in group
-    // m0 / 8 + 2 bi + lq / 2, half lq % 2 of its eight
-    uint8_t* tb = (uint8_t*)tl.out_off + (int64_t)col0 * tl.out_ld + (tl.m0 / 8 + lq / 2) * 32 + (lq % 2) * 8;
-#pragma unroll
-    for (int bi = 0; bi < 4; ++bi)
-#pragma unroll
-      for (int bj = 0; bj < 4; ++bj) {
-        if (bi >= mb || bj >= nb) continue;
-        unsigned short h[4], w[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          h[r] = bf16_rne(acc[bi][bj][r]);
-          w[r] = bf16_rne(acc[bi][bj][r] - bf16_to_f(h[r]));
-        }
-        uint8_t* d = tb + (int64_t)(16 * bj) * tl.out_ld + 64 * bi;
-        *(uint2*)d = uint2{h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16)};
-        *(uint2*)(d + 16) = uint2{w[0] | ((uint32_t)w[1] << 16), w[2] | ((uint32_t)w[3] << 16)};
-      }
+    store_t_lines(tl, acc, mb, nb, li, lq);
     return;
   }
   if (tl.slot >= 0) {   // partial tile: 64 x 64, column-major, tile coordinates
@@ -806,6 +812,139 @@ __global__ __launch_bounds__(64, 2) void k_packed(const tjsq::PTile* __restrict_
   if (bid >= ntiles || (gate && *gate)) return;
   const tjsq::PTile tl = tiles[bid];
   packed_tile<STAGE>(tl, segs, S, partial);
+}
+
+// Stage 1 by quads.  The four waves of a workgroup form four tiles of T
+// that read the same x lines, segment by segment (the same x blocks, rows
+// m0.., the same K), each with its own operator copy and coefficients.
+// Wave w loads block w of each shared x chunk and stores it to LDS; after
+// one barrier per chunk every wave reads the chunk's x blocks from LDS, so
+// the quad fetches each x byte from L2 once instead of four times.  The
+// chunk sequence is the lead tile's (tile 4 q), the same for all four, so
+// every wave reaches every barrier; a padding wave (mb = 0) loads its share,
+// synchronises and forms nothing.  LDS holds two chunks (written while the
+// other is read), lines padded to 144 bytes so that the 16-byte reads and
+// writes of a wave meet no bank twice.
+constexpr int QL_LINE = 144;
+constexpr int QL_BUF = 4 * 16 * QL_LINE;
+
+__global__ __launch_bounds__(256, 2) void k_packed_quad(const tjsq::PTile* __restrict__ tiles,
+                                                       const tjsq::PSeg* __restrict__ segs, int nquads,
+                                                       const int* __restrict__ gate) {
+  __shared__ __align__(16) uint8_t lds[2 * QL_BUF];
+  const int q = blockIdx.x;
+  if (q >= nquads || (gate && *gate)) return;   // the whole workgroup or none of it
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lq = lane >> 4;
+  const tjsq::PTile lead = tiles[4 * q];
+  const tjsq::PTile tl = tiles[4 * q + w];
+  const bool mine = tl.mb > 0;
+  const int mb = lead.mb, nb = mine ? tl.nb : 0;
+  const int ns = lead.seg_end - lead.seg_begin;
+  f4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  // the current segment: its x side (block w only) and this wave's operator side
+  int j = 0, k0 = 0;
+  tjsq::PSeg ls = segs[lead.seg_begin];
+  tjsq::PSeg rs = mine ? segs[tl.seg_begin] : ls;
+  // this wave's share of the x chunk: line lead.m0 + 16 w + li of the
+  // current segment's x copy (none past the tile's blocks)
+  __amdgpu_buffer_rsrc_t xr = make_rsrc_n((const void*)ls.l_off, ls.l_lim);
+  uint32_t xv = w < mb ? (uint32_t)(lead.m0 + 16 * w + li) * (uint32_t)ls.l_stride + (uint32_t)lq * 32u : OOB;
+  PSide R = packed_side(rs.r_off, rs.r_lim, rs.r_stride, tl.n0, nb, li, lq);
+  // two register sets, A and B: the next chunk loads into one while the
+  // current one is multiplied from the other
+  u4 xhA = u4{0u, 0u, 0u, 0u}, xlA = xhA, xhB = xhA, xlB = xhA;
+  u4 rhA[4], rlA[4], rhB[4], rlB[4];
+  auto issue = [&](uint32_t ko, u4& xh, u4& xl, u4 (&rh)[4], u4 (&rl)[4]) {
+    if (w < mb) {
+      xh = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(xr, (int)(xv + ko), 0, 0));
+      xl = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(xr, (int)(xv + ko + 16u), 0, 0));
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) ld_op(R, b, ko, rh[b], rl[b]);
+  };
+  int nc = 0;
+  for (int t = 0; t < ns; ++t) nc += (segs[lead.seg_begin + t].K + 31) / 32;
+  double scale = 1.0;
+  // chunk c: its x share (xh, xl) to LDS, the next chunk's loads into the
+  // other set, one barrier, then the chunk's products
+  auto step = [&](int c, const u4& xh, const u4& xl, const u4 (&rh)[4], const u4 (&rl)[4], u4& nxh, u4& nxl,
+                  u4 (&nrh)[4], u4 (&nrl)[4]) {
+    const bool first = k0 == 0, half = ls.K - k0 == 16;
+    const double cseg = rs.c;
+    uint8_t* buf = lds + (c & 1) * QL_BUF;
+    if (w < mb) {
+      u4* d = (u4*)(buf + (w * 16 + li) * QL_LINE + lq * 32);
+      d[0] = xh;
+      d[1] = xl;
+    }
+    if (c + 1 < nc) {   // the next chunk: past a segment's end, the next segment's sides
+      k0 += 32;
+      if (k0 >= ls.K) {
+        ++j;
+        k0 = 0;
+        ls = segs[lead.seg_begin + j];
+        rs = mine ? segs[tl.seg_begin + j] : ls;
+        xr = make_rsrc_n((const void*)ls.l_off, ls.l_lim);
+        xv = w < mb ? (uint32_t)(lead.m0 + 16 * w + li) * (uint32_t)ls.l_stride + (uint32_t)lq * 32u : OOB;
+        R = packed_side(rs.r_off, rs.r_lim, rs.r_stride, tl.n0, nb, li, lq);
+      }
+      issue((uint32_t)k0 * 4u, nxh, nxl, nrh, nrl);
+    }
+    __syncthreads();
+    u4 ah[4], al[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      ah[b] = u4{0u, 0u, 0u, 0u};
+      al[b] = u4{0u, 0u, 0u, 0u};
+      if (b < mb && !(half && lq >= 2)) {   // a 16-long chunk: lanes supplying k 16..31 add nothing
+        const u4* s4 = (const u4*)(buf + (b * 16 + li) * QL_LINE + lq * 32);
+        ah[b] = s4[0];
+        al[b] = s4[1];
+      }
+    }
+    if (!mine) return;
+    if (first) {
+      // the running sum is kept as scale * acc: acc *= scale / c, scale = c
+      const float r = (float)(scale / cseg);
+      scale = cseg;
+      if (r != 1.0f) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a][b] *= r;
+      }
+    }
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi) {
+      if (bi >= mb) continue;
+      const bf8 xh8 = __builtin_bit_cast(bf8, ah[bi]), xl8 = __builtin_bit_cast(bf8, al[bi]);
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj) {
+        if (bj >= nb) continue;
+        const bf8 yh = __builtin_bit_cast(bf8, rh[bj]), yl = __builtin_bit_cast(bf8, rl[bj]);
+        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl8, yh, acc[bi][bj], 0, 0, 0);
+        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh8, yl, acc[bi][bj], 0, 0, 0);
+        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh8, yh, acc[bi][bj], 0, 0, 0);
+      }
+    }
+  };
+  if (nc > 0) issue(0u, xhA, xlA, rhA, rlA);
+  for (int c = 0; c < nc; c += 2) {
+    step(c, xhA, xlA, rhA, rlA, xhB, xlB, rhB, rlB);
+    if (c + 1 >= nc) break;
+    step(c + 1, xhB, xlB, rhB, rlB, xhA, xlA, rhA, rlA);
+  }
+  if (!mine) return;
+  const float r = (float)scale;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] *= r;
+  store_t_lines(tl, acc, mb, nb, li, lq);
 }
 
 // ordered sum of the partial slots of split packed tiles: thread t sums
@@ -1648,7 +1787,11 @@ int packed_apply(const tjsq_plan* p, const float* x, float* y, hipStream_t st) {
   const int n1 = (int)K.tile1.size(), n2 = (int)K.tile2.size(), nr = (int)K.rtile.size();
   if (!rc && n1)
     rc = timed_launch(p, 0, st, [&] {
-      hipLaunchKernelGGL(k_packed<1>, dim3(n1), dim3(64), 0, st, p->pk_tile1, p->pk_seg1, S, p->pk_part, n1, p->gate);
+      if (K.quad1)
+        hipLaunchKernelGGL(k_packed_quad, dim3(n1 / 4), dim3(256), 0, st, p->pk_tile1, p->pk_seg1, n1 / 4, p->gate);
+      else
+        hipLaunchKernelGGL(k_packed<1>, dim3(n1), dim3(64), 0, st, p->pk_tile1, p->pk_seg1, S, p->pk_part, n1,
+                           p->gate);
     });
   if (!rc && n2)
     rc = timed_launch(p, 2, st, [&] {
@@ -1710,6 +1853,7 @@ void pk_append_device(const tjsq::PackedPlan& K, uint8_t* ops, uint8_t* x, uint8
   }
   D.nslots += K.nslots;
   D.flops_exec += K.flops_exec;
+  D.quad1 = K.quad1;   // every plan's tables are compiled with the same shape
 }
 
 // device tables of the packed form D (and its partial slots) into p

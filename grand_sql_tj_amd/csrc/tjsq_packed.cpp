@@ -65,8 +65,9 @@ Line right_operand(Copies& cp, int64_t off, int ld, int t, int rows_op, int cols
 // q-th tile takes index q NX + x; bins are padded with empty tiles (no
 // blocks, no segments: nothing read or written).
 void xcd_bins(std::vector<PTile>& tiles, const std::vector<int>& group, const std::vector<int64_t>& cost, int NX,
-              int out_kind) {
-  const size_t n = tiles.size();
+              int out_kind, int unit = 1) {
+  // items of `unit` consecutive tiles (a workgroup's), group and cost per item
+  const size_t n = tiles.size() / unit;
   std::map<int, int64_t> gcost;
   for (size_t t = 0; t < n; ++t) gcost[group[t]] += cost[t];
   std::vector<std::pair<int64_t, int>> order;
@@ -92,9 +93,10 @@ void xcd_bins(std::vector<PTile>& tiles, const std::vector<int>& group, const st
   PTile null{};
   null.out_kind = out_kind;
   null.slot = -1;
-  std::vector<PTile> sorted(maxlen * NX, null);
+  std::vector<PTile> sorted(maxlen * NX * unit, null);
   for (int x = 0; x < NX; ++x)
-    for (size_t q = 0; q < bins[x].size(); ++q) sorted[q * NX + x] = tiles[bins[x][q]];
+    for (size_t q = 0; q < bins[x].size(); ++q)
+      for (int u = 0; u < unit; ++u) sorted[(q * NX + x) * unit + u] = tiles[bins[x][q] * unit + u];
   tiles.swap(sorted);
 }
 
@@ -227,14 +229,47 @@ int compile_packed(const HostPlan& P, const PackShape& shape, PackedPlan& out, s
     std::vector<PTile> sorted(n);
     for (size_t i = 0; i < n; ++i) sorted[i] = out.tile1[order[i]];
     out.tile1.swap(sorted);
+    const int unit = shape.quad1 ? 4 : 1;
+    if (shape.quad1) {
+      // quads: tiles with the same x rows (m0, mb) and the same x segments
+      // (copy offset and K, segment by segment), in fours, in table order
+      std::map<std::vector<int64_t>, int> gid;
+      std::vector<std::vector<int>> groups;
+      for (size_t t = 0; t < n; ++t) {
+        const PTile& tl = out.tile1[t];
+        std::vector<int64_t> key{tl.m0, tl.mb};
+        for (int q = tl.seg_begin; q < tl.seg_end; ++q) {
+          key.push_back(out.seg1[q].l_off);
+          key.push_back(out.seg1[q].K);
+        }
+        auto it = gid.emplace(key, (int)groups.size());
+        if (it.second) groups.emplace_back();
+        groups[it.first->second].push_back((int)t);
+      }
+      PTile null{};
+      null.out_kind = PK_OUT_T;
+      null.slot = -1;
+      std::vector<PTile> quads;
+      for (const std::vector<int>& g : groups)
+        for (size_t i = 0; i < g.size(); i += 4)
+          for (size_t u = 0; u < 4; ++u) quads.push_back(i + u < g.size() ? out.tile1[g[i + u]] : null);
+      out.tile1.swap(quads);
+      out.quad1 = true;
+    }
     if (shape.xcds1 > 1) {
-      // locality group: the operator copy (right operand) of the tile, so
-      // that one XCD reads each operator block for every x block it meets
+      // locality group: the operator copy (right operand) of the tile (of a
+      // quad's lead tile), so that one XCD reads each operator block for
+      // every x block it meets
+      const size_t nu = out.tile1.size() / unit;
       std::map<int64_t, int> gid;
-      std::vector<int> g1(n);
-      for (size_t t = 0; t < n; ++t)
-        g1[t] = gid.emplace(out.seg1[out.tile1[t].seg_begin].r_off, (int)gid.size()).first->second;
-      xcd_bins(out.tile1, g1, tile_costs(out.tile1, out.seg1), shape.xcds1, PK_OUT_T);
+      std::vector<int> g1(nu);
+      std::vector<int64_t> c1(nu, 0);
+      const std::vector<int64_t> ct = tile_costs(out.tile1, out.seg1);
+      for (size_t t = 0; t < nu; ++t) {
+        g1[t] = gid.emplace(out.seg1[out.tile1[t * unit].seg_begin].r_off, (int)gid.size()).first->second;
+        for (int u = 0; u < unit; ++u) c1[t] += out.tile1[t * unit + u].mb > 0 ? ct[t * unit + u] : 0;
+      }
+      xcd_bins(out.tile1, g1, c1, shape.xcds1, PK_OUT_T, unit);
     }
   }
 

@@ -80,6 +80,9 @@ struct PackedPlan {
   std::vector<PReduce> rtile;
   int64_t nslots = 0;
   double flops_exec = 0;   // MFMA flops of the padded tiles (one product per algorithmic multiply-add)
+  // stage-1 tiles in quads: tiles 4 q .. 4 q + 3 read the same x lines
+  // (tile 4 q leads; padding tiles have no blocks) and run as one workgroup
+  bool quad1 = false;
 };
 
 struct PackShape {
@@ -97,6 +100,11 @@ struct PackShape {
   // by x block instead left it at 1.10 ms, by wave sector or by system
   // sector (fewer, larger groups) made it 1.19 ms.
   int xcds1 = 8;
+  // stage-1 tiles that read the same x lines (the same rows of the same x
+  // blocks, segment by segment) grouped in fours, one workgroup each, which
+  // shares each x chunk through LDS (with xcds1: the quads of one operator
+  // copy of their lead tile kept on one XCD)
+  int quad1 = 1;
   // stage-2 tiles of one output column strip (a wavefunction sector's env
   // sub-block: the blocks that share the right operators) kept on one XCD,
   // strips spread over `xcds2` XCDs by work (workgroup g runs on XCD

@@ -59,7 +59,7 @@ def test_packed_layout_geometry():
     assert (t2["slot"][pad] == -1).all()
     t1 = pk["tile1"]
     pad1 = t1["mb"] == 0
-    assert t1.size % 8 == 0
+    assert t1.size % 32 == 0      # quads of 4 tiles in 8 XCD bins
     assert (t1["nb"][pad1] == 0).all() and (t1["seg_begin"][pad1] == t1["seg_end"][pad1]).all()
     for t in (t1[~pad1], t2[~pad]):
         assert ((t["mb"] >= 1) & (t["mb"] <= 4) & (t["nb"] >= 1) & (t["nb"] <= 4)).all()
@@ -73,22 +73,24 @@ def test_packed_layout_geometry():
     red = set(zip(pk["rtile"]["out_off"].tolist(), pk["rtile"]["m0"].tolist(), pk["rtile"]["n0"].tolist()))
     assert not keys & red and len(red) == pk["rtile"].size
     assert sorted(real["slot"][real["slot"] >= 0].tolist()) == list(range(pk["nslots"]))
-    # stage 1: the tiles of one operator copy (right operand) run on one XCD
-    # (bin g mod 8), one after the other, in x-block order
-    src = {int(j["dst_off"]): int(j["src_off"]) for j in pk["x_jobs"]}
+    # stage 1 runs in quads (tiles 4q..4q+3, one workgroup) that share the
+    # x lines: every tile of a quad reads the lead tile's x rows and x
+    # segments; the quads of one operator copy of their lead (bin q mod 8)
+    # run on one XCD
+    seg1 = pk["seg1"]
     home = {}
-    for x in range(8):
-        b = t1[x::8]
-        b = b[b["mb"] > 0]
-        first = pk["seg1"][b["seg_begin"]]
-        ops = first["r_off"].tolist()
-        seen = []
-        for r in ops:
-            assert home.setdefault(r, x) == x
-            if not seen or seen[-1] != r:
-                assert r not in seen, "an operator copy's tiles are not contiguous in its bin"
-                seen.append(r)
-        for r in seen:
-            sel = first["r_off"] == r
-            key = [(src[int(s["l_off"])], int(tl["n0"]), int(tl["m0"])) for s, tl in zip(first[sel], b[sel])]
-            assert key == sorted(key)
+    for q in range(t1.size // 4):
+        lead = t1[4 * q]
+        quad = t1[4 * q:4 * q + 4]
+        if lead["mb"] == 0:
+            assert (quad["mb"] == 0).all()      # a whole padding quad of an XCD bin
+            continue
+        ls = seg1[lead["seg_begin"]:lead["seg_end"]]
+        for tl in quad[quad["mb"] > 0]:
+            assert tl["m0"] == lead["m0"] and tl["mb"] == lead["mb"]
+            s_ = seg1[tl["seg_begin"]:tl["seg_end"]]
+            assert len(s_) == len(ls)
+            for f in ("l_off", "l_stride", "l_lim", "K"):
+                assert (s_[f] == ls[f]).all()
+        r = int(ls["r_off"][0])
+        assert home.setdefault(r, q % 8) == q % 8
