@@ -66,26 +66,36 @@ Line right_operand(Copies& cp, int64_t off, int ld, int t, int rows_op, int cols
 // blocks, no segments: nothing read or written).
 void xcd_bins(std::vector<PTile>& tiles, const std::vector<int>& group, const std::vector<int64_t>& cost, int NX,
               int out_kind, int unit = 1) {
-  // items of `unit` consecutive tiles (a workgroup's), group and cost per item
+  // items of `unit` consecutive tiles (a workgroup's), group (0 .. G - 1)
+  // and cost per item
   const size_t n = tiles.size() / unit;
-  std::map<int, int64_t> gcost;
-  for (size_t t = 0; t < n; ++t) gcost[group[t]] += cost[t];
-  std::vector<std::pair<int64_t, int>> order;
-  for (auto& kv : gcost) order.push_back({kv.second, kv.first});
-  std::sort(order.begin(), order.end(), [](const std::pair<int64_t, int>& a, const std::pair<int64_t, int>& b) {
-    return a.first != b.first ? a.first > b.first : a.second < b.second;
-  });
-  std::map<int, int> gbin, grank;
+  int G = 0;
+  for (size_t t = 0; t < n; ++t) G = std::max(G, group[t] + 1);
+  std::vector<int64_t> gcost(G, 0);
+  std::vector<char> used(G, 0);
+  for (size_t t = 0; t < n; ++t) {
+    gcost[group[t]] += cost[t];
+    used[group[t]] = 1;
+  }
+  std::vector<int> order;
+  for (int g = 0; g < G; ++g)
+    if (used[g]) order.push_back(g);
+  std::sort(order.begin(), order.end(),
+            [&](int a, int b) { return gcost[a] != gcost[b] ? gcost[a] > gcost[b] : a < b; });
+  std::vector<int> gbin(G, 0), grank(G, 0);
   std::vector<int64_t> load(NX, 0);
   for (size_t r = 0; r < order.size(); ++r) {
     const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    gbin[order[r].second] = b;
-    grank[order[r].second] = (int)r;
-    load[b] += order[r].first;
+    gbin[order[r]] = b;
+    grank[order[r]] = (int)r;
+    load[b] += gcost[order[r]];
   }
+  // items by group rank (a counting sort: stable, table order within a group)
+  std::vector<int> start(order.size() + 1, 0);
+  for (size_t t = 0; t < n; ++t) ++start[grank[group[t]] + 1];
+  for (size_t r = 0; r < order.size(); ++r) start[r + 1] += start[r];
   std::vector<int> idx(n);
-  for (size_t t = 0; t < n; ++t) idx[t] = (int)t;
-  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return grank[group[a]] < grank[group[b]]; });
+  for (size_t t = 0; t < n; ++t) idx[start[grank[group[t]]]++] = (int)t;
   std::vector<std::vector<int>> bins(NX);
   for (int t : idx) bins[gbin[group[t]]].push_back(t);
   size_t maxlen = 0;
