@@ -45,7 +45,7 @@ KERNEL_SYMBOL = {"stage1_64": "k_tiles<{mode}, 64, 1>", "stage1_32": "k_tiles<{m
 # the packed BF16x3 apply: its kernels in the same profiling slots (slot 1 =
 # the per-apply packing of x, slot 3 unused; stage 1 runs in quads of tiles
 # that share their x lines)
-PACKED_SYMBOL = {"stage1_64": "k_packed_quad", "stage1_32": "k_pack<float>", "stage2_64": "k_packed<2>",
+PACKED_SYMBOL = {"stage1_64": "k_packed_group<4>", "stage1_32": "k_pack<float>", "stage2_64": "k_packed<2>",
                  "stage2_32": "-", "reduce": "k_preduce"}
 MODE_ID = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "heff_apply_m8000_traffic.json")

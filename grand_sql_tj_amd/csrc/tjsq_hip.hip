@@ -814,82 +814,95 @@ __global__ __launch_bounds__(64, 2) void k_packed(const tjsq::PTile* __restrict_
   packed_tile<STAGE>(tl, segs, S, partial);
 }
 
-// Stage 1 by quads.  The four waves of a workgroup form four tiles of T
-// that read the same x lines, segment by segment (the same x blocks, rows
-// m0.., the same K), each with its own operator copy and coefficients.
-// Wave w loads block w of each shared x chunk and stores it to LDS; after
-// one barrier per chunk every wave reads the chunk's x blocks from LDS, so
-// the quad fetches each x byte from L2 once instead of four times.  The
-// chunk sequence is the lead tile's (tile 4 q), the same for all four, so
-// every wave reaches every barrier; a padding wave (mb = 0) loads its share,
-// synchronises and forms nothing.  LDS holds two chunks (written while the
-// other is read), lines padded to 144 bytes so that the 16-byte reads and
-// writes of a wave meet no bank twice.
+// Stage 1 by groups of G tiles.  The G waves of a workgroup form G tiles of
+// T that read the same x lines, segment by segment (the same rows of the
+// same x blocks, the same K), each with its own operator copy and
+// coefficients.  Wave w loads the x blocks b with b % G == w of each chunk
+// and stores them to LDS; after one barrier per chunk every wave reads the
+// chunk's x blocks from LDS, so the group fetches each x byte from L2 once
+// instead of G times.  The chunk sequence is the lead tile's (tile G q), the
+// same for all, so every wave reaches every barrier; a padding wave (mb = 0)
+// loads its share, synchronises and forms nothing.  LDS holds two chunks
+// (written while the other is read), lines padded to 144 bytes so that the
+// 16-byte reads and writes of a wave meet no bank twice.
 constexpr int QL_LINE = 144;
 constexpr int QL_BUF = 4 * 16 * QL_LINE;
 
-__global__ __launch_bounds__(256, 2) void k_packed_quad(const tjsq::PTile* __restrict__ tiles,
-                                                       const tjsq::PSeg* __restrict__ segs, int nquads,
-                                                       const int* __restrict__ gate) {
+template <int G>
+__global__ __launch_bounds__(64 * G, 2) void k_packed_group(const tjsq::PTile* __restrict__ tiles,
+                                                               const tjsq::PSeg* __restrict__ segs, int ngroups,
+                                                               const int* __restrict__ gate) {
   __shared__ __align__(16) uint8_t lds[2 * QL_BUF];
   const int q = blockIdx.x;
-  if (q >= nquads || (gate && *gate)) return;   // the whole workgroup or none of it
+  if (q >= ngroups || (gate && *gate)) return;   // the whole workgroup or none of it
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lq = lane >> 4;
-  const tjsq::PTile lead = tiles[4 * q];
-  const tjsq::PTile tl = tiles[4 * q + w];
+  const tjsq::PTile lead = tiles[G * q];
+  const tjsq::PTile tl = tiles[G * q + w];
   const bool mine = tl.mb > 0;
   const int mb = lead.mb, nb = mine ? tl.nb : 0;
-  const int ns = lead.seg_end - lead.seg_begin;
   f4 acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-  // the current segment: its x side (block w only) and this wave's operator side
+  const int g0 = lead.seg_begin, gm = tl.seg_begin;
+  const int ns = lead.seg_end - g0;
+  // the current segment: its x side (this wave's blocks) and its operator side
   int j = 0, k0 = 0;
-  tjsq::PSeg ls = segs[lead.seg_begin];
-  tjsq::PSeg rs = mine ? segs[tl.seg_begin] : ls;
-  // this wave's share of the x chunk: line lead.m0 + 16 w + li of the
-  // current segment's x copy (none past the tile's blocks)
+  tjsq::PSeg ls{};   // (no segment: nothing is loaded)
+  if (ns > 0) ls = segs[g0];
+  tjsq::PSeg rs = mine && ns > 0 ? segs[gm] : ls;
   __amdgpu_buffer_rsrc_t xr = make_rsrc_n((const void*)ls.l_off, ls.l_lim);
-  uint32_t xv = w < mb ? (uint32_t)(lead.m0 + 16 * w + li) * (uint32_t)ls.l_stride + (uint32_t)lq * 32u : OOB;
+  uint32_t xv0 = (uint32_t)(lead.m0 + li) * (uint32_t)ls.l_stride + (uint32_t)lq * 32u;
   PSide R = packed_side(rs.r_off, rs.r_lim, rs.r_stride, tl.n0, nb, li, lq);
+  constexpr int NS = 4 / G;   // x blocks per wave: w, w + G, ...
   // two register sets, A and B: the next chunk loads into one while the
   // current one is multiplied from the other
-  u4 xhA = u4{0u, 0u, 0u, 0u}, xlA = xhA, xhB = xhA, xlB = xhA;
+  u4 xhA[NS], xlA[NS], xhB[NS], xlB[NS];
+#pragma unroll
+  for (int t = 0; t < NS; ++t) xhA[t] = xlA[t] = xhB[t] = xlB[t] = u4{0u, 0u, 0u, 0u};
   u4 rhA[4], rlA[4], rhB[4], rlB[4];
-  auto issue = [&](uint32_t ko, u4& xh, u4& xl, u4 (&rh)[4], u4 (&rl)[4]) {
-    if (w < mb) {
-      xh = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(xr, (int)(xv + ko), 0, 0));
-      xl = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(xr, (int)(xv + ko + 16u), 0, 0));
+  auto issue = [&](uint32_t ko, u4 (&xh)[NS], u4 (&xl)[NS], u4 (&rh)[4], u4 (&rl)[4]) {
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+      const int b = w + G * t;
+      if (b < mb) {
+        const uint32_t v = xv0 + (uint32_t)(16 * b) * (uint32_t)ls.l_stride + ko;
+        xh[t] = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(xr, (int)v, 0, 0));
+        xl[t] = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(xr, (int)(v + 16u), 0, 0));
+      }
     }
 #pragma unroll
     for (int b = 0; b < 4; ++b) ld_op(R, b, ko, rh[b], rl[b]);
   };
   int nc = 0;
-  for (int t = 0; t < ns; ++t) nc += (segs[lead.seg_begin + t].K + 31) / 32;
+  for (int t = 0; t < ns; ++t) nc += (segs[g0 + t].K + 31) / 32;
   double scale = 1.0;
-  // chunk c: its x share (xh, xl) to LDS, the next chunk's loads into the
+  // chunk c: this wave's x blocks to LDS, the next chunk's loads into the
   // other set, one barrier, then the chunk's products
-  auto step = [&](int c, const u4& xh, const u4& xl, const u4 (&rh)[4], const u4 (&rl)[4], u4& nxh, u4& nxl,
-                  u4 (&nrh)[4], u4 (&nrl)[4]) {
+  auto step = [&](int c, const u4 (&xh)[NS], const u4 (&xl)[NS], const u4 (&rh)[4], const u4 (&rl)[4],
+                  u4 (&nxh)[NS], u4 (&nxl)[NS], u4 (&nrh)[4], u4 (&nrl)[4]) {
     const bool first = k0 == 0, half = ls.K - k0 == 16;
     const double cseg = rs.c;
     uint8_t* buf = lds + (c & 1) * QL_BUF;
-    if (w < mb) {
-      u4* d = (u4*)(buf + (w * 16 + li) * QL_LINE + lq * 32);
-      d[0] = xh;
-      d[1] = xl;
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+      const int b = w + G * t;
+      if (b < mb) {
+        u4* d = (u4*)(buf + (b * 16 + li) * QL_LINE + lq * 32);
+        d[0] = xh[t];
+        d[1] = xl[t];
+      }
     }
     if (c + 1 < nc) {   // the next chunk: past a segment's end, the next segment's sides
       k0 += 32;
       if (k0 >= ls.K) {
         ++j;
         k0 = 0;
-        ls = segs[lead.seg_begin + j];
-        rs = mine ? segs[tl.seg_begin + j] : ls;
+        ls = segs[g0 + j];
+        rs = mine ? segs[gm + j] : ls;
         xr = make_rsrc_n((const void*)ls.l_off, ls.l_lim);
-        xv = w < mb ? (uint32_t)(lead.m0 + 16 * w + li) * (uint32_t)ls.l_stride + (uint32_t)lq * 32u : OOB;
+        xv0 = (uint32_t)(lead.m0 + li) * (uint32_t)ls.l_stride + (uint32_t)lq * 32u;
         R = packed_side(rs.r_off, rs.r_lim, rs.r_stride, tl.n0, nb, li, lq);
       }
       issue((uint32_t)k0 * 4u, nxh, nxl, nrh, nrl);
@@ -939,11 +952,13 @@ __global__ __launch_bounds__(256, 2) void k_packed_quad(const tjsq::PTile* __res
     step(c + 1, xhB, xlB, rhB, rlB, xhA, xlA, rhA, rlA);
   }
   if (!mine) return;
-  const float r = (float)scale;
+  if (nc > 0) {
+    const float r = (float)scale;
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] *= r;
+      for (int b = 0; b < 4; ++b) acc[a][b] *= r;
+  }
   store_t_lines(tl, acc, mb, nb, li, lq);
 }
 
@@ -1788,7 +1803,8 @@ int packed_apply(const tjsq_plan* p, const float* x, float* y, hipStream_t st) {
   if (!rc && n1)
     rc = timed_launch(p, 0, st, [&] {
       if (K.quad1)
-        hipLaunchKernelGGL(k_packed_quad, dim3(n1 / 4), dim3(256), 0, st, p->pk_tile1, p->pk_seg1, n1 / 4, p->gate);
+        hipLaunchKernelGGL(k_packed_group<4>, dim3(n1 / 4), dim3(256), 0, st, p->pk_tile1, p->pk_seg1, n1 / 4,
+                           p->gate);
       else
         hipLaunchKernelGGL(k_packed<1>, dim3(n1), dim3(64), 0, st, p->pk_tile1, p->pk_seg1, S, p->pk_part, n1,
                            p->gate);
