@@ -387,8 +387,23 @@ int compile_packed(const HostPlan& P, const PackShape& shape, PackedPlan& out, s
         out.rtile.push_back(r);
       }
   }
-  if (shape.xcds2 > 1 && !out.tile2.empty())
-    xcd_bins(out.tile2, tgroup, tile_costs(out.tile2, out.seg2), shape.xcds2, PK_OUT_Y);
+  if (shape.xcds2 > 1 && !out.tile2.empty()) {
+    // locality group: the left operand of the tile's first GEMM segment (a
+    // T block, or an operator or x block), which the tiles of several
+    // output blocks read; a tile of axpy segments only, its column strip
+    std::map<int64_t, int> gid;
+    std::vector<int> g2(out.tile2.size());
+    for (size_t t = 0; t < out.tile2.size(); ++t) {
+      int64_t k = -1 - (int64_t)tgroup[t];
+      for (int q = out.tile2[t].seg_begin; q < out.tile2[t].seg_end; ++q)
+        if (out.seg2[q].l_space() != PK_XF) {
+          k = out.seg2[q].l_off * 4 + out.seg2[q].l_space();
+          break;
+        }
+      g2[t] = gid.emplace(k, (int)gid.size()).first->second;
+    }
+    xcd_bins(out.tile2, g2, tile_costs(out.tile2, out.seg2), shape.xcds2, PK_OUT_Y);
+  }
   if (ops.bad || xs.bad) {
     msg = "packed layout: one operand offset referenced with two shapes";
     return TJSQ_ERR_ARG;

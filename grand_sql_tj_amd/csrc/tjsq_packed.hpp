@@ -105,11 +105,11 @@ struct PackShape {
   // shares each x chunk through LDS (with xcds1: the quads of one operator
   // copy of their lead tile kept on one XCD)
   int quad1 = 1;
-  // stage-2 tiles of one output column strip (a wavefunction sector's env
-  // sub-block: the blocks that share the right operators) kept on one XCD,
-  // strips spread over `xcds2` XCDs by work (workgroup g runs on XCD
-  // g mod xcds2); 1: table order.  At the m = 8000 bench bond 8 XCDs took
-  // stage 2 from 0.834 to 0.772 ms.
+  // stage-2 tiles that read the same left operand first (a T block, or an
+  // operator or x block) kept on one XCD, groups spread over `xcds2` XCDs by
+  // work (workgroup g runs on XCD g mod xcds2); 1: table order.  At the
+  // m = 8000 bench bond 8 XCDs took stage 2 from 0.834 to 0.72 ms (grouping
+  // by output column strip, the blocks that share right operators: 0.772 ms).
   int xcds2 = 8;
 };
 
