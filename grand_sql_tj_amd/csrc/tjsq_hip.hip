@@ -1615,6 +1615,15 @@ struct tjsq_plan {
   tjsq::PackedPlan pk;           // device form of the tables (absolute addresses)
   tjsq::PackedPlan pk_host;      // host form (offsets into the plan's spaces)
   std::vector<void*> pk_owned;   // further buffers the tables address (batch copies)
+  // device tables of the set-up kernels (operator packing, hmerge), kept
+  // with the plan: freeing them would synchronise the whole device, which
+  // stalls plan creation behind other streams' work (a serving mix)
+  tjsq::PackJob* pk_ojobs = nullptr;
+  PackPiece* pk_opieces = nullptr;
+  int pk_nop = 0;
+  MergeDst* mg_dsts = nullptr;
+  tjsq::MergeTerm* mg_terms = nullptr;
+  int mg_ndst = 0;
   uint8_t* pk_ops = nullptr;     // packed operator copies
   uint8_t* pk_x = nullptr;       // packed copies of an apply's x
   uint8_t* pk_t = nullptr;       // stage-1 products
@@ -1897,25 +1906,26 @@ int pack_ops(tjsq_plan* p, const std::vector<tjsq::PackJob>& jobs, hipStream_t s
   if (jobs.empty()) return TJSQ_OK;
   std::vector<tjsq::PackJob> oj = jobs;
   for (tjsq::PackJob& j : oj) j.dst_off = (int64_t)(uintptr_t)(p->pk_ops + j.dst_off);
-  const std::vector<PackPiece> op = pack_pieces(oj);
-  tjsq::PackJob* ojobs = nullptr;
-  PackPiece* opieces = nullptr;
-  if (hipMalloc(&ojobs, sizeof(tjsq::PackJob) * oj.size()) != hipSuccess ||
-      hipMalloc(&opieces, sizeof(PackPiece) * std::max<size_t>(1, op.size())) != hipSuccess) {
-    (void)hipFree(ojobs);
-    return fail(TJSQ_ERR_NOMEM, "packed BF16x3 allocation failed");
+  if (!p->pk_ojobs) {   // the tables once per plan (the same for every refresh)
+    const std::vector<PackPiece> op = pack_pieces(oj);
+    if (hipMalloc(&p->pk_ojobs, sizeof(tjsq::PackJob) * oj.size()) != hipSuccess ||
+        hipMalloc(&p->pk_opieces, sizeof(PackPiece) * std::max<size_t>(1, op.size())) != hipSuccess) {
+      (void)hipFree(p->pk_ojobs);
+      p->pk_ojobs = nullptr;
+      p->pk_opieces = nullptr;
+      return fail(TJSQ_ERR_NOMEM, "packed BF16x3 allocation failed");
+    }
+    HIPCHK(hipMemcpyAsync(p->pk_ojobs, oj.data(), sizeof(tjsq::PackJob) * oj.size(), hipMemcpyHostToDevice, st));
+    if (!op.empty())
+      HIPCHK(hipMemcpyAsync(p->pk_opieces, op.data(), sizeof(PackPiece) * op.size(), hipMemcpyHostToDevice, st));
+    p->pk_nop = (int)op.size();
   }
-  HIPCHK(hipMemcpyAsync(ojobs, oj.data(), sizeof(tjsq::PackJob) * oj.size(), hipMemcpyHostToDevice, st));
-  if (!op.empty())
-    HIPCHK(hipMemcpyAsync(opieces, op.data(), sizeof(PackPiece) * op.size(), hipMemcpyHostToDevice, st));
-  const int np = (int)op.size();
+  const int np = p->pk_nop;
   if (np)
-    hipLaunchKernelGGL(k_pack<double>, dim3(std::min(np, 65535)), dim3(PACK_PIECE), 0, st, ojobs, opieces, np,
-                       p->arena, nullptr);
+    hipLaunchKernelGGL(k_pack<double>, dim3(std::min(np, 65535)), dim3(PACK_PIECE), 0, st, p->pk_ojobs, p->pk_opieces,
+                       np, p->arena, nullptr);
   const hipError_t e = hipGetLastError();
   const hipError_t e2 = hipStreamSynchronize(st);
-  (void)hipFree(ojobs);
-  (void)hipFree(opieces);
   if (e != hipSuccess || e2 != hipSuccess) return fail(TJSQ_ERR_HIP, "packing the BF16x3 operators failed");
   return TJSQ_OK;
 }
@@ -1968,23 +1978,25 @@ int run_merge(tjsq_plan* p, hipStream_t st) {
     t = u;
   }
   if (!dsts.empty()) {
-    MergeDst* dd = nullptr;
-    tjsq::MergeTerm* dt = nullptr;
-    if (hipMalloc(&dd, sizeof(MergeDst) * dsts.size()) != hipSuccess ||
-        hipMalloc(&dt, sizeof(tjsq::MergeTerm) * terms.size()) != hipSuccess) {
-      (void)hipFree(dd);
-      return fail(TJSQ_ERR_NOMEM, "merge table allocation failed");
+    if (!p->mg_dsts) {   // the tables once per plan (the same for every refresh)
+      if (hipMalloc(&p->mg_dsts, sizeof(MergeDst) * dsts.size()) != hipSuccess ||
+          hipMalloc(&p->mg_terms, sizeof(tjsq::MergeTerm) * terms.size()) != hipSuccess) {
+        (void)hipFree(p->mg_dsts);
+        p->mg_dsts = nullptr;
+        p->mg_terms = nullptr;
+        return fail(TJSQ_ERR_NOMEM, "merge table allocation failed");
+      }
+      HIPCHK(hipMemcpyAsync(p->mg_dsts, dsts.data(), sizeof(MergeDst) * dsts.size(), hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(p->mg_terms, terms.data(), sizeof(tjsq::MergeTerm) * terms.size(), hipMemcpyHostToDevice,
+                            st));
+      p->mg_ndst = (int)dsts.size();
     }
-    HIPCHK(hipMemcpyAsync(dd, dsts.data(), sizeof(MergeDst) * dsts.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dt, terms.data(), sizeof(tjsq::MergeTerm) * terms.size(), hipMemcpyHostToDevice, st));
     int64_t maxn = 1;
     for (auto& d : dsts) maxn = std::max(maxn, d.n);
     dim3 grid((unsigned)std::min<int64_t>((maxn + 255) / 256, 64), (unsigned)std::min<size_t>(dsts.size(), 32768));
-    hipLaunchKernelGGL(k_merge, grid, dim3(256), 0, st, p->arena, dd, dt, (int)dsts.size());
+    hipLaunchKernelGGL(k_merge, grid, dim3(256), 0, st, p->arena, p->mg_dsts, p->mg_terms, p->mg_ndst);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
-    (void)hipFree(dd);
-    (void)hipFree(dt);
   }
   return TJSQ_OK;
 }
@@ -2421,6 +2433,10 @@ int tjsq_plan_destroy(tjsq_plan* p) {
   (void)hipFree(p->pk_rtile);
   (void)hipFree(p->pk_part);
   for (void* b : p->pk_owned) (void)hipFree(b);
+  (void)hipFree(p->pk_ojobs);
+  (void)hipFree(p->pk_opieces);
+  (void)hipFree(p->mg_dsts);
+  (void)hipFree(p->mg_terms);
   delete p;
   return TJSQ_OK;
 }
