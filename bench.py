@@ -435,9 +435,11 @@ def serve_bench(args):
             r.wait(600)
             print(f"[bench] done {r.id} slot={r.slot} latency={r.latency * 1e3:.0f} ms", file=sys.stderr, flush=True)
         t_end = max(r.t_done for r in results)
-        busy = srv.busy_time
-        peak = srv.peak_in_use
-        budget = srv.budget
+    # (read after close(): a request's result is published before its plan
+    # is freed, and its busy interval ends with the free)
+    busy = min(srv.busy_time, t_end - t_begin)
+    peak = srv.peak_in_use
+    budget = srv.budget
     lat = np.array([r.latency for r in results])
     wait = np.array([r.t_admit - r.t_arrival for r in results])
     t_plan = np.array([r.t_plan - r.t_start for r in results])

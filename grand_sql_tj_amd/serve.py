@@ -182,14 +182,21 @@ class Server:
                     res.psi = psi.cpu().numpy()
                     res.e0 = info["e0"]
                     res.info = info
+                    # the result is complete: publish it before the plan is
+                    # freed (hipFree waits for the other slots' queued work);
+                    # the request's HBM is returned to the budget after the free
+                    res.t_done = time.perf_counter()
+                    res._ev.set()
                     del plan, psi
                     stream.synchronize()
             except Exception as e:  # reported through the result
-                res.error = repr(e)
+                if not res._ev.is_set():
+                    res.error = repr(e)
             finally:
                 self._release(req)
-                res.t_done = time.perf_counter()
-                res._ev.set()
+                if not res._ev.is_set():
+                    res.t_done = time.perf_counter()
+                    res._ev.set()
 
     def close(self):
         with self.cv:
